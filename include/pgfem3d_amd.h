@@ -1,0 +1,202 @@
+/*
+ * pgfem3d_amd.h -- C ABI of the MI355X (gfx950) finite-strain element hot path.
+ *
+ * The library replaces, for one GPU, the per-Newton-iteration element work of PGFem3D:
+ *   deformation gradient per quadrature point (QP) -> constitutive update
+ *   (neo-Hookean, J2, crystal plasticity) -> element residual / tangent -> CSR assembly.
+ *
+ * All citations are file:line into the PGFem3D sources (C-SWARM/pgfem-3d).
+ *
+ * Conventions (kept from the reference):
+ *   - DOF ids are 1-based for free DOFs, 0 for fixed DOFs and -k for the k-th prescribed
+ *     value (Node::id_map, include/node.h:7-35; generate_local_dof_ids,
+ *     src/generate_dof_ids.cc:69-171).
+ *   - The CSR pattern is the one Psparse_ApAi builds for one rank: int row pointers Ap
+ *     [ndof+1], int (Ai_t) column ids Ai, 0-based, sorted and unique per row
+ *     (src/Psparse_ApAi.cc:238-670, include/datatype.hpp:4).
+ *   - Element DOFs are node-major (a*3+b); 3x3 tensors are row-major; 4th-order tensors
+ *     use idx_4 = ((i*3+j)*3+k)*3+l (include/index_macros.h:15-30).
+ *   - Every function returns an int status: 0 = success, > 0 = number of quadrature
+ *     points / elements that failed (the reference's INFO/err count: J <= 0, singular
+ *     inverse, non-normal tangent diagonal, local Newton failure), < 0 = API error
+ *     (PGF_E*).
+ *
+ * Threading: a handle is owned by one host thread; calls on one handle are serialised on
+ * the handle's stream.  Vector arguments of pgf_residual / pgf_tangent are DEVICE
+ * pointers (HBM-resident), everything in the set-up functions is HOST memory.
+ */
+#ifndef PGFEM3D_AMD_H
+#define PGFEM3D_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status codes */
+#define PGF_OK 0
+#define PGF_EINVAL (-1)   /* bad argument / inconsistent sizes */
+#define PGF_ENOMEM (-2)   /* device or host allocation failed */
+#define PGF_EHIP (-3)     /* a HIP runtime call failed */
+#define PGF_ENOTSUP (-4)  /* element type / model combination not supported */
+#define PGF_ENODEV (-5)   /* no usable gfx950 device */
+
+/* ------------------------------------------------------------- element types */
+/* Identified by nodes per element, as FEMLIB does (FEMLIB/src/femlib.cc:33-37,75-83). */
+#define PGF_TET4 4    /* linear tetrahedron, 1 QP */
+#define PGF_TET10 10  /* quadratic tetrahedron, 4 QPs (order upgraded, femlib.cc:1010-1016) */
+#define PGF_HEX8 8    /* trilinear hexahedron, 2x2x2 Gauss */
+
+/* ---------------------------------------------------------------- model types */
+/* Values of the reference's enum model_type (include/constitutive_model.h:41-54). */
+#define PGF_MODEL_HYPER_ELASTICITY 0
+#define PGF_MODEL_CRYSTAL_PLASTICITY 1
+#define PGF_MODEL_J2_PLASTICITY 4
+
+/* Material record.  Elastic data follow HOMMAT (include/hommat.h:13-28) as filled for the
+ * CM analysis (src/homogen.cc:172-191): m10 = Ey, m01 = Ez, E = Ex, G = Gyz, nu = nyz,
+ * and kappa = E/(3(1-2nu)) (src/constitutive_model.cc:1006). */
+typedef struct pgf_material {
+  int32_t model;       /* PGF_MODEL_* */
+  int32_t devPotFlag;  /* 1 = Mooney-Rivlin (src/new_potentials.cc:106-127) */
+  int32_t volPotFlag;  /* 2 = Doll-Schweizerhof 7, 3 = DS 8, 99 = quadratic
+                          (src/new_potentials.cc:184-210) */
+  int32_t pad0;
+  double E, nu, G, m10, m01;
+  /* J2 plasticity, linear isotropic hardening (CM_J2P_PARAM PARAM_hp / PARAM_k0,
+   * src/cm_j2_plasticity.cc:86-103): k0 = initial uniaxial yield stress, hp = linear
+   * hardening modulus. */
+  double k0, hp;
+  /* Crystal plasticity (CP_PARAM, src/plasticity_model.cc:89-101,960-967). */
+  double gamma_dot_0, m_rate, G0, g0, gs_0, gamma_dot_s, w_sat;
+  double C11, C12, C44;           /* cubic elasticity in the crystal frame */
+  double tol_M, tol_hardening;    /* local Newton tolerances */
+  int32_t max_itr_M, max_itr_stag;
+} pgf_material;
+
+/* Mesh record (host memory).  Element connectivity is CSR-like so that several element
+ * types can share one mesh (Element::toe + Element::nod, include/element.h:16-60). */
+typedef struct pgf_mesh {
+  int64_t n_nodes;
+  const double *X;           /* [n_nodes*3] reference coordinates, node-major */
+  int64_t n_elem;
+  const int64_t *elem_ptr;   /* [n_elem+1] offsets into elem_conn */
+  const int64_t *elem_conn;  /* node ids, 0-based; nodes per element = 4, 10 or 8 */
+  const int32_t *elem_mat;   /* [n_elem] material index, NULL = all 0 */
+  int64_t n_coel;            /* cohesive elements (COEL): they enter the DOF / CSR graph */
+  const int64_t *coel_ptr;   /* [n_coel+1] or NULL */
+  const int64_t *coel_conn;  /* node ids or NULL */
+} pgf_mesh;
+
+/* ==================================================================== set-up (host) */
+
+/* Number the DOFs by element traversal.  On input id[n_nodes*3] holds support codes
+ * (0 free, 1 fixed, -k prescribed by value k, as read by read_Dirichlet_BCs,
+ * src/in.cc:29-110); on output it holds DOF ids (>0 free, 0 fixed, <0 prescribed).
+ * Replaces generate_local_dof_ids / generate_local_dof_ids_on_elem
+ * (src/generate_dof_ids.cc:69-171,306-412); on one rank the global ids equal the local
+ * ones (generate_global_dof_ids, :173-219,470-575).  *ndof = number of free DOFs. */
+int pgf_generate_dof_ids(const pgf_mesh *mesh, int64_t *id, int64_t *ndof);
+
+/* Build the CSR pattern of the free DOFs.  Replaces Psparse_ApAi on one rank
+ * (src/Psparse_ApAi.cc:238-670; countDof/mapDof :160-213).  Ap has ndof+1 entries.  With
+ * Ai == NULL only Ap and *nnz are computed; otherwise Ai must hold Ap[ndof] entries. */
+int pgf_psparse_apai(const pgf_mesh *mesh, const int64_t *id, int64_t ndof, int32_t *Ap,
+                     int32_t *Ai, int64_t *nnz);
+
+/* Synthetic structured meshes of the unit cube (n^3 cells).  etype = PGF_TET4 (Kuhn split
+ * of each cell into 6 tets), PGF_TET10 (same, plus mid-edge nodes) or PGF_HEX8.  Interior
+ * vertices are moved by U(-jitter*h, +jitter*h) per coordinate from std::mt19937_64(seed).
+ * First call with X == NULL returns the sizes. */
+int pgf_mesh_box(int32_t n, int32_t etype, double jitter, uint64_t seed, int64_t *n_nodes,
+                 int64_t *n_elem, int64_t *conn_len, double *X, int64_t *elem_ptr,
+                 int64_t *elem_conn);
+
+/* ================================================================= device handle */
+typedef struct pgf_handle_s *pgf_handle;
+
+/* Create the device-side model: copies mesh, DOF ids, CSR pattern and materials to HBM,
+ * colours the elements (no two elements of one colour share a node) and builds the
+ * element -> CSR scatter map.  The pattern must be the one pgf_psparse_apai returns.
+ * Replaces the per-element set-up of stiffmat_fd_MP / fd_residuals_MP
+ * (src/stiffmat_fd.cc:831-993, src/fd_residuals.cc:468-636) and the per-QP
+ * Constitutive_model objects (include/constitutive_model.h:766-844). */
+int pgf_create(const pgf_mesh *mesh, const int64_t *id, int64_t ndof, const int32_t *Ap,
+               const int32_t *Ai, int32_t n_mat, const pgf_material *mats, pgf_handle *out);
+int pgf_destroy(pgf_handle h);
+
+/* Use a caller-provided hipStream_t (passed as void*); NULL = the handle's own stream. */
+int pgf_set_stream(pgf_handle h, void *stream);
+int pgf_synchronize(pgf_handle h);
+
+/* Sizes: n_qp = total quadrature points, nnz = CSR entries, ndof, n_colors. */
+int pgf_get_sizes(pgf_handle h, int64_t *n_qp, int64_t *nnz, int64_t *ndof,
+                  int32_t *n_colors);
+
+/* ============================================================ per Newton iteration */
+/* Nodal unknowns follow def_elem_total (src/utils.cc:1391-1412): a free DOF id>0 takes
+ * r[id-1] + d_r[id-1]; a prescribed DOF id<0 takes defl[-id-1] + defl_d[-id-1]; a fixed DOF
+ * is 0.  d_r, defl, defl_d may be NULL (= zeros).  All vectors are device pointers. */
+typedef struct pgf_nodal_input {
+  const double *r;       /* [ndof] */
+  const double *d_r;     /* [ndof] or NULL */
+  const double *defl;    /* [n_prescribed] or NULL */
+  const double *defl_d;  /* [n_prescribed] or NULL */
+} pgf_nodal_input;
+
+/* Internal force vector f[ndof] (zeroed first), with the QP integration algorithm run
+ * when run_integration != 0 (Solver::run_integration_algorithm).  Replaces fd_residuals_MP
+ * (src/fd_residuals.cc:468-636) -> residuals_el_constitutive_model_1f
+ * (src/constitutive_model.cc:3322-3539) -> compute_residual_vector (:1628-1662) ->
+ * fd_res_assemble (src/fd_residuals.cc:58-73). */
+int pgf_residual(pgf_handle h, const pgf_nodal_input *in, double dt, int run_integration,
+                 double *f);
+
+/* Tangent values vals[nnz] (zeroed first, as SparseSystem::zero) in the Psparse_ApAi
+ * pattern, from the state left by the last integration.  Replaces stiffmat_fd_MP
+ * (src/stiffmat_fd.cc:831-993) -> stiffness_el_constitutive_model_1f
+ * (src/constitutive_model.cc:2474-2692) -> compute_stiffness_matrix (:1520-1616) ->
+ * PLoc_Sparse (src/PLoc_Sparse.cc:15-175). */
+int pgf_tangent(pgf_handle h, const pgf_nodal_input *in, double dt, double *vals);
+
+/* Both of the above at one displacement in one pass over the elements (integration
+ * run, then residual and tangent from the updated state).  Equivalent to pgf_residual
+ * (run_integration=1) followed by pgf_tangent at the same input. */
+int pgf_residual_tangent(pgf_handle h, const pgf_nodal_input *in, double dt, double *f,
+                         double *vals);
+
+/* ======================================================================= state */
+/* After convergence: n+1 -> n, n -> n-1 for every QP (constitutive_model_update_time_steps,
+ * src/constitutive_model.cc:1442-1497; CM_J2P_PARAM::update_state_vars,
+ * src/cm_j2_plasticity.cc:463-496; CP_PARAM::update_state_vars,
+ * src/plasticity_model.cc:174-192). */
+int pgf_commit_step(pgf_handle h);
+/* Subdivision reset: n+1 <- n (reset_state_vars, src/cm_j2_plasticity.cc:498-520;
+ * constitutive_model_reset_state_using_temporal, src/constitutive_model.cc:1421-1440). */
+int pgf_reset_step(pgf_handle h);
+
+/* State fields, stored structure-of-arrays [component][n_qp] in QP order (element order,
+ * then QP order inside the element). */
+#define PGF_STATE_F_NP1 0     /* 9 comps: F at n+1 */
+#define PGF_STATE_F_N 1       /* 9 comps */
+#define PGF_STATE_F_NM1 2     /* 9 comps */
+#define PGF_STATE_PLASTIC_NP1 3  /* J2: 6 comps Cp^-1 (xx,yy,zz,xy,yz,xz); CP: 9 comps pF */
+#define PGF_STATE_PLASTIC_N 4
+#define PGF_STATE_PLASTIC_NM1 5
+#define PGF_STATE_SCALAR_NP1 6   /* J2: ep, gam; CP: g (slip resistance), L */
+#define PGF_STATE_SCALAR_N 7
+#define PGF_STATE_SCALAR_NM1 8
+#define PGF_STATE_CAUCHY 9       /* 6 comps: Cauchy stress of the last integration */
+#define PGF_STATE_ITERS 10       /* 1 comp: local Newton iterations of the last integration */
+/* Number of components of a field (0 if the field does not exist for the mesh). */
+int pgf_state_ncomp(pgf_handle h, int32_t field, int32_t *ncomp);
+/* Copy a field to / from host memory laid out [comp][n_qp]. */
+int pgf_get_state(pgf_handle h, int32_t field, double *host_out);
+int pgf_set_state(pgf_handle h, int32_t field, const double *host_in);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PGFEM3D_AMD_H */

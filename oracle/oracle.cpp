@@ -1,0 +1,788 @@
+// CPU oracle of the PGFem3D finite-strain element hot path -- TEST INFRASTRUCTURE ONLY.
+//
+// Plain FP64 restatement of the reference (C-SWARM/pgfem-3d) for one MPI rank, total
+// Lagrangian CM path, no thermal coupling.  Only tests/, __graft_entry__.smoke() and
+// bench.py's cpu_baseline leg use it, as the checker; the product never does.
+//
+// Pinning (tests that would fail on a plausible mistake here):
+//   orc_generate_local_dof_ids, orc_psparse_apai
+//       brute force on hand-enumerated one-cell meshes (every DOF of a cell couples to every
+//       other), pattern symmetry, diagonal present          -> tests/test_oracle_pattern.py
+//   orc_basis   partition of unity, sum dN = 0, element volume = sum detJxW, exact
+//               integration of linear fields, affine u gives F exactly -> test_oracle_kinematics.py
+//   orc_he_update  S(I) = 0, closed-form uniaxial neo-Hookean S, L = central FD of S wrt
+//               E (1e-7), frame invariance S(QF) = S(F)       -> test_oracle_materials.py
+//   orc_j2_update  elastic range == orc_he_update, consistency ||s|| = sqrt(2/3)(k0+hp ep)
+//               after return, small-strain uniaxial closed form (E_t = E hp/(E+hp)),
+//               L = central FD of the return map (1e-6)        -> test_oracle_materials.py
+//   orc_assemble   K = central FD of f (1e-6); rigid rotation gives f = 0; patch test
+//               (affine u: f = 0 at interior DOFs)             -> test_oracle_element.py
+// The J2 return map itself lives in the un-vendored GCM library in the reference
+// (src/cm_j2_plasticity.cc:197-205); it is restated here from Simo & Hughes,
+// Computational Inelasticity (1998), boxes 9.1-9.2, and pinned only by the checks above.
+#include "oracle.h"
+
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+namespace {
+
+inline int idx4(int i, int j, int k, int l) { return ((i * 3 + j) * 3 + k) * 3 + l; }
+
+// det3x3, src/utils.cc:478-483
+double det3x3(const double *m) {
+  return ((m[0] * (m[4] * m[8] - m[5] * m[7])) + m[1] * (m[5] * m[6] - m[3] * m[8]) +
+          m[2] * (m[3] * m[7] - m[4] * m[6]));
+}
+
+// inv3x3, src/utils.cc:542-562 (singular if |det| < 1e-15)
+int inv3x3(const double *m, double *mi) {
+  const double A = det3x3(m);
+  if (!(std::fabs(A) >= 1.0e-15)) return 1;
+  mi[0] = 1 / A * (m[4] * m[8] - m[5] * m[7]);
+  mi[1] = -1 / A * (m[1] * m[8] - m[2] * m[7]);
+  mi[2] = 1 / A * (m[1] * m[5] - m[2] * m[4]);
+  mi[3] = -1 / A * (m[3] * m[8] - m[5] * m[6]);
+  mi[4] = 1 / A * (m[0] * m[8] - m[2] * m[6]);
+  mi[5] = -1 / A * (m[0] * m[5] - m[2] * m[3]);
+  mi[6] = 1 / A * (m[3] * m[7] - m[4] * m[6]);
+  mi[7] = -1 / A * (m[0] * m[7] - m[1] * m[6]);
+  mi[8] = 1 / A * (m[0] * m[4] - m[1] * m[3]);
+  return 0;
+}
+
+void matmul(const double *A, const double *B, double *C) {  // C = A B
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double s = 0.0;
+      for (int k = 0; k < 3; ++k) s += A[i * 3 + k] * B[k * 3 + j];
+      C[i * 3 + j] = s;
+    }
+}
+void matmul_tn(const double *A, const double *B, double *C) {  // C = A^T B
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double s = 0.0;
+      for (int k = 0; k < 3; ++k) s += A[k * 3 + i] * B[k * 3 + j];
+      C[i * 3 + j] = s;
+    }
+}
+void matmul_nt(const double *A, const double *B, double *C) {  // C = A B^T
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double s = 0.0;
+      for (int k = 0; k < 3; ++k) s += A[i * 3 + k] * B[j * 3 + k];
+      C[i * 3 + j] = s;
+    }
+}
+
+// ------------------------------------------------------------------ FEMLIB
+// Quadrature (FEMLIB/src/femlib.cc:544-588): truncated constants kept verbatim.
+const double kOneOverSqrt3 = 0.57735026918962584;
+const double kTet4A = 0.58541020, kTet4B = 0.13819660;
+
+int nint_of(int nne) {
+  switch (nne) {
+    case 4: return 1;   // TetIntegrationPoints1 (order 0)
+    case 10: return 4;  // order upgraded to 1 -> TetIntegrationPoints4 (femlib.cc:1010-1016)
+    case 8: return 8;   // gpno = order+1 = 2 -> 2x2x2 Gauss (femlib.cc:1067-1076)
+  }
+  return 0;
+}
+
+// integration point ip -> (ksi, eta, zet, weight); itg_ids as in femlib.cc:624-700
+void int_point(int nne, int ip, double *k, double *e, double *z, double *w) {
+  if (nne == 4) {
+    *k = *e = *z = 1.0 / 4.0;
+    *w = 1.0 / 6.0;
+  } else if (nne == 10) {
+    const double gk[4] = {kTet4A, kTet4B, kTet4B, kTet4B};
+    const double ge[4] = {kTet4B, kTet4A, kTet4B, kTet4B};
+    const double gz[4] = {kTet4B, kTet4B, kTet4A, kTet4B};
+    *k = gk[ip];
+    *e = ge[ip];
+    *z = gz[ip];
+    *w = (1.0 / 4.0) / 6.0;
+  } else {  // hex: itg_ids(cnt) = (ia, ib, ic) in a triple loop, weights 1*1*1
+    const double g[2] = {-kOneOverSqrt3, kOneOverSqrt3};
+    *k = g[ip / 4];
+    *e = g[(ip / 2) % 2];
+    *z = g[ip % 2];
+    *w = 1.0 * 1.0 * 1.0;
+  }
+}
+
+// PGFem3D_N, femlib.cc:124-203
+void shape_N(int nne, double ksi, double eta, double zet, double *N) {
+  if (nne == 4) {
+    N[0] = (1.0 - ksi - eta - zet);
+    N[1] = ksi;
+    N[2] = eta;
+    N[3] = zet;
+  } else if (nne == 10) {
+    const double l = 1.0 - ksi - eta - zet;
+    N[0] = (2.0 * l - 1.0) * l;
+    N[1] = (2.0 * ksi - 1.0) * ksi;
+    N[2] = (2.0 * eta - 1.0) * eta;
+    N[3] = (2.0 * zet - 1.0) * zet;
+    N[4] = 4.0 * l * ksi;
+    N[5] = 4.0 * ksi * eta;
+    N[6] = 4.0 * l * eta;
+    N[7] = 4.0 * l * zet;
+    N[8] = 4.0 * ksi * zet;
+    N[9] = 4.0 * eta * zet;
+  } else {
+    const double o = 0.125;
+    N[0] = o * (1.0 - ksi) * (1.0 - eta) * (1.0 - zet);
+    N[1] = o * (1.0 + ksi) * (1.0 - eta) * (1.0 - zet);
+    N[2] = o * (1.0 + ksi) * (1.0 + eta) * (1.0 - zet);
+    N[3] = o * (1.0 - ksi) * (1.0 + eta) * (1.0 - zet);
+    N[4] = o * (1.0 - ksi) * (1.0 - eta) * (1.0 + zet);
+    N[5] = o * (1.0 + ksi) * (1.0 - eta) * (1.0 + zet);
+    N[6] = o * (1.0 + ksi) * (1.0 + eta) * (1.0 + zet);
+    N[7] = o * (1.0 - ksi) * (1.0 + eta) * (1.0 + zet);
+  }
+}
+
+// compute_dN_dkez, femlib.cc:214-340; dN[a*3+c]
+void shape_dN(int nne, double ksi, double eta, double zet, double *dN) {
+  if (nne == 4) {
+    const double t[12] = {-1, -1, -1, 1, 0, 0, 0, 1, 0, 0, 0, 1};
+    for (int i = 0; i < 12; ++i) dN[i] = t[i];
+  } else if (nne == 10) {
+    dN[0] = 4.0 * (ksi + eta + zet) - 3.0;
+    dN[1] = 4.0 * (eta + ksi + zet) - 3.0;
+    dN[2] = 4.0 * (zet + ksi + eta) - 3.0;
+    dN[3] = 4.0 * ksi - 1.0; dN[4] = 0.0; dN[5] = 0.0;
+    dN[6] = 0.0; dN[7] = 4.0 * eta - 1.0; dN[8] = 0.0;
+    dN[9] = 0.0; dN[10] = 0.0; dN[11] = 4.0 * zet - 1.0;
+    dN[12] = 4.0 * (-2.0 * ksi - eta - zet + 1.0); dN[13] = -4.0 * ksi; dN[14] = -4.0 * ksi;
+    dN[15] = 4.0 * eta; dN[16] = 4.0 * ksi; dN[17] = 0.0;
+    dN[18] = -4.0 * eta; dN[19] = 4.0 * (-ksi - 2.0 * eta - zet + 1.0); dN[20] = -4.0 * eta;
+    dN[21] = -4.0 * zet; dN[22] = -4.0 * zet; dN[23] = 4.0 * (-ksi - eta - 2.0 * zet + 1.0);
+    dN[24] = 4.0 * zet; dN[25] = 0.0; dN[26] = 4.0 * ksi;
+    dN[27] = 0.0; dN[28] = 4.0 * zet; dN[29] = 4.0 * eta;
+  } else {
+    const double o = 0.125;
+    const double s[8][3] = {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1},
+                            {-1, -1, 1},  {1, -1, 1},  {1, 1, 1},  {-1, 1, 1}};
+    for (int a = 0; a < 8; ++a) {
+      dN[a * 3 + 0] = s[a][0] * o * (1.0 + s[a][1] * eta) * (1.0 + s[a][2] * zet);
+      dN[a * 3 + 1] = s[a][1] * o * (1.0 + s[a][0] * ksi) * (1.0 + s[a][2] * zet);
+      dN[a * 3 + 2] = s[a][2] * o * (1.0 + s[a][0] * ksi) * (1.0 + s[a][1] * eta);
+    }
+  }
+}
+
+// PGFem3D_dNdX, femlib.cc:403-467 (+ compute_dX_dkez :350-364 and det :373-392)
+double dNdX_of(int nne, const double *dNk, const double *X, double *dNdX) {
+  double dX[9] = {0};
+  for (int ib = 0; ib < 3; ++ib)
+    for (int ic = 0; ic < 3; ++ic)
+      for (int ia = 0; ia < nne; ++ia) dX[ib * 3 + ic] += dNk[ia * 3 + ic] * X[ia * 3 + ib];
+  const double J = dX[0] * (dX[4] * dX[8] - dX[5] * dX[7]) + dX[1] * (dX[5] * dX[6] - dX[3] * dX[8]) +
+                   dX[2] * (dX[3] * dX[7] - dX[4] * dX[6]);
+  if (J <= 0.0) return J;
+  double I[9];
+  I[0] = dX[4] * dX[8] - dX[5] * dX[7];
+  I[1] = dX[2] * dX[7] - dX[1] * dX[8];
+  I[2] = dX[1] * dX[5] - dX[2] * dX[4];
+  I[3] = dX[5] * dX[6] - dX[3] * dX[8];
+  I[4] = dX[0] * dX[8] - dX[2] * dX[6];
+  I[5] = dX[2] * dX[3] - dX[0] * dX[5];
+  I[6] = dX[3] * dX[7] - dX[4] * dX[6];
+  I[7] = dX[1] * dX[6] - dX[0] * dX[7];
+  I[8] = dX[0] * dX[4] - dX[1] * dX[3];
+  for (int ia = 0; ia < nne; ++ia)
+    for (int ib = 0; ib < 3; ++ib) {
+      double s = 0.0;
+      for (int ic = 0; ic < 3; ++ic) s += 1.0 / J * dNk[ia * 3 + ic] * I[ic * 3 + ib];
+      dNdX[ia * 3 + ib] = s;
+    }
+  return J;
+}
+
+// ------------------------------------------------------------------ hyperelasticity
+// kappa on the CM path: E/(3(1-2nu)) (src/constitutive_model.cc:1006)
+double kappa_of(const orc_material &m) { return m.E / (3.0 * (1.0 - 2.0 * m.nu)); }
+
+// dU/dJ and d2U/dJ2 (src/new_potentials.cc:575-616), flag selection :184-210
+void vol_pot(int flag, double J, double *dU, double *d2U, double *U) {
+  switch (flag) {
+    case 3:  // Doll-Schweizerhof 8
+      *U = 0.5 * (J - 1.0) * std::log(J);
+      *dU = (-1 + J) / (2. * J) + std::log(J) * .5;
+      *d2U = 1. / J - (-1 + J) / (2. * J * J);
+      break;
+    case 99:  // quadratic
+      *U = 0.5 * (J - 1.0) * (J - 1.0);
+      *dU = J - 1;
+      *d2U = 1.0;
+      break;
+    default:  // 2: Doll-Schweizerhof 7
+      *U = 0.5 * (std::exp(J - 1.0) - std::log(J) - 1.0);
+      *dU = (std::exp(-1 + J) - 1. / J) * .5;
+      *d2U = (std::exp(-1 + J) + 1. / (J * J)) * .5;
+  }
+}
+
+// devStress_Mooney_Rivlin, src/new_potentials.cc:329-358
+void dev_stress_mr(const double *C, double m10, double m01, double *S) {
+  double invC[9];
+  inv3x3(C, invC);
+  const double id[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+  const double detC = det3x3(C);
+  const double trC = C[0] + C[4] + C[8];
+  double CC = 0.0;
+  for (int i = 0; i < 9; ++i) CC += C[i] * C[i];
+  for (int i = 0; i < 9; ++i)
+    S[i] = ((2 * m10 * std::pow(detC, -1. / 3.) * (id[i] - 1. / 3. * trC * invC[i])) +
+            (2 * m01 * std::pow(detC, -2. / 3.) *
+             (trC * id[i] - C[i] + 1. / 3. * (CC - trC * trC) * invC[i])));
+}
+
+// matStiff_Mooney_Rivlin, src/new_potentials.cc:376-432
+void mat_stiff_mr(const double *C, double m10, double m01, double *L) {
+  double invC[9];
+  inv3x3(C, invC);
+  const double id[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+  const double d1 = std::pow(det3x3(C), -1. / 3.);
+  const double d2 = d1 * d1;
+  const double trC = C[0] + C[4] + C[8];
+  double CC = 0.0;
+  for (int i = 0; i < 9; ++i) CC += C[i] * C[i];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      for (int k = 0; k < 3; ++k)
+        for (int l = 0; l < 3; ++l) {
+          const int ij = i * 3 + j, kl = k * 3 + l, ik = i * 3 + k, lj = l * 3 + j;
+          L[idx4(i, j, k, l)] =
+              (4 * m10 * d1 *
+               ((1. / 9. * trC * invC[ij] - 1. / 3. * id[ij]) * invC[kl] +
+                1. / 3. * (trC * invC[ik] * invC[lj] - id[kl] * invC[ij]))) +
+              (4 * m01 * d2 *
+               (id[ij] * id[kl] - id[ik] * id[lj] + 2. / 3. * ((C[ij] - trC * id[ij]) * invC[kl]) -
+                ((CC - trC * trC) * (2. / 9. * invC[ij] * invC[kl] + 1. / 3. * invC[ik] * invC[lj])) +
+                2. / 3. * ((C[kl] - trC * id[kl]) * invC[ij])));
+        }
+}
+
+// Total S and L of the hyperelastic model (HE_PARAM::update_elasticity,
+// src/plasticity_model_none.cc:329-362, through the in-tree mirror get_material_stress /
+// get_material_stiffness, src/displacement_based_element.cc:1368-1445, damage off).
+int he_update(const orc_material &m, const double *F, double *S, double *L) {
+  double C[9], CI[9];
+  matmul_tn(F, F, C);
+  if (inv3x3(C, CI)) return 1;
+  const double J = det3x3(F);
+  if (!(J > 0.0)) return 1;
+  const double kappa = kappa_of(m);
+  double dU, d2U, U;
+  vol_pot(m.volPotFlag, J, &dU, &d2U, &U);
+  dev_stress_mr(C, m.m10, m.m01, S);
+  for (int i = 0; i < 9; ++i) S[i] += kappa * J * dU * CI[i];
+  if (L) {
+    mat_stiff_mr(C, m.m10, m.m01, L);
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j)
+        for (int k = 0; k < 3; ++k)
+          for (int l = 0; l < 3; ++l) {
+            const int q = idx4(i, j, k, l);
+            L[q] = L[q] + kappa * (J * dU + J * J * d2U) * CI[i * 3 + j] * CI[k * 3 + l] -
+                   2.0 * kappa * J * dU * CI[i * 3 + k] * CI[l * 3 + j];
+          }
+  }
+  return 0;
+}
+
+// ------------------------------------------------------------------ J2 plasticity
+// Simo & Hughes (1998) box 9.1 (return map) and box 9.2 (algorithmic tangent), total
+// Lagrangian: internal variables Cp^-1 (reference configuration) and ep; S = F^-1 tau F^-T,
+// L = F^-1 F^-1 F^-1 F^-1 : c.  The tangent's n (x) dev(n^2) term is kept unsymmetrised,
+// which makes L the exact derivative of the return map.
+void sym6_to_9(const double *a, double *A) {
+  A[0] = a[0]; A[4] = a[1]; A[8] = a[2];
+  A[1] = A[3] = a[3];
+  A[5] = A[7] = a[4];
+  A[2] = A[6] = a[5];
+}
+void sym9_to_6(const double *A, double *a) {
+  a[0] = A[0]; a[1] = A[4]; a[2] = A[8];
+  a[3] = 0.5 * (A[1] + A[3]);
+  a[4] = 0.5 * (A[5] + A[7]);
+  a[5] = 0.5 * (A[2] + A[6]);
+}
+
+int j2_update(const orc_material &m, const double *F, const double *Cpinv_n, double ep_n,
+              double *Cpinv_np1, double *ep_np1, double *gam, double *S, double *L) {
+  const double J = det3x3(F);
+  if (!(J > 0.0)) return 1;
+  double Fi[9];
+  if (inv3x3(F, Fi)) return 1;
+  const double G = m.G, H = m.hp, kappa = kappa_of(m);
+  double Cn[9], T[9], b[9];
+  sym6_to_9(Cpinv_n, Cn);
+  matmul(F, Cn, T);
+  matmul_nt(T, F, b);
+  const double Jm23 = std::pow(J, -2.0 / 3.0);
+  for (int i = 0; i < 9; ++i) b[i] *= Jm23;  // b_bar trial
+  const double Ibar = (b[0] + b[4] + b[8]) / 3.0;
+  double s_tr[9];
+  for (int i = 0; i < 9; ++i) s_tr[i] = G * b[i];
+  s_tr[0] -= G * Ibar;
+  s_tr[4] -= G * Ibar;
+  s_tr[8] -= G * Ibar;
+  double nrm = 0.0;
+  for (int i = 0; i < 9; ++i) nrm += s_tr[i] * s_tr[i];
+  nrm = std::sqrt(nrm);
+  const double f_tr = nrm - std::sqrt(2.0 / 3.0) * (m.k0 + H * ep_n);
+  const double mubar = G * Ibar;
+  double dg = 0.0, s[9], n[9] = {0};
+  std::memcpy(s, s_tr, sizeof(s));
+  if (f_tr > 0.0) {
+    dg = f_tr / (2.0 * mubar + 2.0 / 3.0 * H);
+    for (int i = 0; i < 9; ++i) {
+      n[i] = s_tr[i] / nrm;
+      s[i] = s_tr[i] - 2.0 * mubar * dg * n[i];
+    }
+  }
+  if (Cpinv_np1) {
+    if (f_tr > 0.0) {
+      double be[9];  // b_bar_e(n+1) = s/G + Ibar 1
+      for (int i = 0; i < 9; ++i) be[i] = s[i] / G;
+      be[0] += Ibar; be[4] += Ibar; be[8] += Ibar;
+      double T2[9], C2[9];
+      matmul(Fi, be, T2);
+      matmul_nt(T2, Fi, C2);
+      const double Jp23 = std::pow(J, 2.0 / 3.0);
+      for (int i = 0; i < 9; ++i) C2[i] *= Jp23;
+      sym9_to_6(C2, Cpinv_np1);
+    } else {
+      for (int i = 0; i < 6; ++i) Cpinv_np1[i] = Cpinv_n[i];
+    }
+  }
+  if (ep_np1) *ep_np1 = ep_n + std::sqrt(2.0 / 3.0) * dg;
+  if (gam) *gam = dg;
+  double dU, d2U, U;
+  vol_pot(m.volPotFlag, J, &dU, &d2U, &U);
+  double tau[9];
+  for (int i = 0; i < 9; ++i) tau[i] = s[i];
+  const double Jp = J * kappa * dU;
+  tau[0] += Jp; tau[4] += Jp; tau[8] += Jp;
+  if (S) {
+    double T3[9];
+    matmul(Fi, tau, T3);
+    matmul_nt(T3, Fi, S);
+  }
+  if (L) {
+    const double id[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    const double cv1 = kappa * (J * J * d2U + J * dU), cv2 = -2.0 * kappa * J * dU;
+    double beta1 = 0, beta3 = 0, beta4 = 0, dn2[9] = {0};
+    if (f_tr > 0.0) {
+      const double beta0 = 1.0 + H / (3.0 * mubar);
+      beta1 = 2.0 * mubar * dg / nrm;
+      const double beta2 = (1.0 - 1.0 / beta0) * (2.0 / 3.0) * (nrm / mubar) * dg;
+      beta3 = 1.0 / beta0 - beta1 + beta2;
+      beta4 = (1.0 / beta0 - beta1) * nrm / mubar;
+      double n2[9];
+      matmul(n, n, n2);
+      const double t3 = (n2[0] + n2[4] + n2[8]) / 3.0;
+      for (int i = 0; i < 9; ++i) dn2[i] = n2[i] - t3 * id[i];
+    }
+    double c[81];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j)
+        for (int k = 0; k < 3; ++k)
+          for (int l = 0; l < 3; ++l) {
+            const double Isym = 0.5 * (id[i * 3 + k] * id[j * 3 + l] + id[i * 3 + l] * id[j * 3 + k]);
+            const double oo = id[i * 3 + j] * id[k * 3 + l];
+            const double cbar = 2.0 * mubar * (Isym - oo / 3.0) -
+                                2.0 / 3.0 * (s_tr[i * 3 + j] * id[k * 3 + l] + id[i * 3 + j] * s_tr[k * 3 + l]);
+            double v = cv1 * oo + cv2 * Isym + (1.0 - beta1) * cbar;
+            if (f_tr > 0.0)
+              v += -2.0 * mubar * beta3 * n[i * 3 + j] * n[k * 3 + l] -
+                   2.0 * mubar * beta4 * n[i * 3 + j] * dn2[k * 3 + l];
+            c[idx4(i, j, k, l)] = v;
+          }
+    // pull back: L_IJKL = Fi_Ii Fi_Jj Fi_Kk Fi_Ll c_ijkl, one index at a time
+    double t1[81], t2[81];
+    for (int I = 0; I < 3; ++I)
+      for (int j = 0; j < 3; ++j)
+        for (int k = 0; k < 3; ++k)
+          for (int l = 0; l < 3; ++l) {
+            double a = 0.0;
+            for (int i = 0; i < 3; ++i) a += Fi[I * 3 + i] * c[idx4(i, j, k, l)];
+            t1[idx4(I, j, k, l)] = a;
+          }
+    for (int I = 0; I < 3; ++I)
+      for (int Jx = 0; Jx < 3; ++Jx)
+        for (int k = 0; k < 3; ++k)
+          for (int l = 0; l < 3; ++l) {
+            double a = 0.0;
+            for (int j = 0; j < 3; ++j) a += Fi[Jx * 3 + j] * t1[idx4(I, j, k, l)];
+            t2[idx4(I, Jx, k, l)] = a;
+          }
+    for (int I = 0; I < 3; ++I)
+      for (int Jx = 0; Jx < 3; ++Jx)
+        for (int K = 0; K < 3; ++K)
+          for (int l = 0; l < 3; ++l) {
+            double a = 0.0;
+            for (int k = 0; k < 3; ++k) a += Fi[K * 3 + k] * t2[idx4(I, Jx, k, l)];
+            t1[idx4(I, Jx, K, l)] = a;
+          }
+    for (int I = 0; I < 3; ++I)
+      for (int Jx = 0; Jx < 3; ++Jx)
+        for (int K = 0; K < 3; ++K)
+          for (int Lx = 0; Lx < 3; ++Lx) {
+            double a = 0.0;
+            for (int l = 0; l < 3; ++l) a += Fi[Lx * 3 + l] * t1[idx4(I, Jx, K, l)];
+            L[idx4(I, Jx, K, Lx)] = a;
+          }
+  }
+  return 0;
+}
+
+// ------------------------------------------------------------------ element contractions
+// compute_residual_vector, src/constitutive_model.cc:1628-1662
+void compute_residual_vector(double *f, int nne, const double *ST, double detJxW,
+                             const double *Fr, const double *eFnMT, const double *eFnM,
+                             const double *S, double one_over_Jn) {
+  for (int a = 0; a < nne; a++)
+    for (int b = 0; b < 3; b++) {
+      const double *STab = ST + a * 27 + b * 9;
+      double AA[9], sAA[9], T[9], V[9];
+      matmul_tn(Fr, STab, AA);  // AA(i,j) = Fr(k,i) ST_ab(k,j)
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) sAA[i * 3 + j] = 0.5 * (AA[i * 3 + j] + AA[j * 3 + i]);
+      matmul(eFnMT, sAA, T);
+      matmul(T, eFnM, V);
+      double v = 0.0;
+      for (int i = 0; i < 9; ++i) v += V[i] * S[i];
+      f[a * 3 + b] += one_over_Jn * detJxW * v;
+    }
+}
+
+// compute_stiffness_matrix, src/constitutive_model.cc:1520-1616
+void compute_stiffness_matrix(double *lk, int nne, const double *ST, double detJxW,
+                              const double *Fr, const double *eFnMT, const double *eFn,
+                              const double *M, const double *FrTFr, const double *eFnM,
+                              const double *S, const double *L, const double *dMdu_all,
+                              double one_over_Jn) {
+  const int ndofe = nne * 3;
+  double MI[9];
+  inv3x3(M, MI);
+  for (int a = 0; a < nne; a++)
+    for (int b = 0; b < 3; b++) {
+      const double *STab = ST + a * 27 + b * 9;
+      double AA[9], sAA[9], T[9], A_eFn[9], A_eFnM[9];
+      matmul_tn(Fr, STab, AA);
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) sAA[i * 3 + j] = 0.5 * (AA[i * 3 + j] + AA[j * 3 + i]);
+      matmul(eFnMT, sAA, T);
+      matmul(T, eFn, A_eFn);      // MTeFnT_sAA_eFn
+      matmul(A_eFn, M, A_eFnM);   // MTeFnT_sAA_eFnM
+      for (int w = 0; w < nne; w++)
+        for (int g = 0; g < 3; g++) {
+          const double *STwg = ST + w * 27 + g * 9;
+          const double *dMdu = dMdu_all + w * 27 + g * 9;
+          double BB[9], sBB[9], CC[9], sCC[9], T1[9], T2[9], X1[9], dCdu[9];
+          matmul_tn(Fr, STwg, BB);
+          matmul_tn(STab, STwg, CC);
+          for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+              sBB[i * 3 + j] = 0.5 * (BB[i * 3 + j] + BB[j * 3 + i]);
+              sCC[i * 3 + j] = 0.5 * (CC[i * 3 + j] + CC[j * 3 + i]);
+            }
+          // dCdu = sym(M^T eFn^T Fr^T Fr eFn dMdu) + M^T eFn^T sBB eFn M
+          matmul(eFnMT, FrTFr, T1);
+          matmul(T1, eFn, T2);
+          matmul(T2, dMdu, X1);
+          double Y1[9], Y2[9];
+          matmul(eFnMT, sBB, Y1);
+          matmul(Y1, eFnM, Y2);
+          for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+              dCdu[i * 3 + j] = 0.5 * (X1[i * 3 + j] + X1[j * 3 + i]) + Y2[i * 3 + j];
+          double L_dCdu[9];
+          for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+              double s = 0.0;
+              for (int k = 0; k < 3; ++k)
+                for (int l = 0; l < 3; ++l) s += L[idx4(i, j, k, l)] * dCdu[k * 3 + l];
+              L_dCdu[i * 3 + j] = s;
+            }
+          double t_LdC = 0.0, t_CS = 0.0, t_dM = 0.0, t_S = 0.0, DpJ = 0.0;
+          double Z1[9], Z2[9], W1[9];
+          matmul(eFnMT, sCC, Z1);
+          matmul(Z1, eFnM, Z2);
+          matmul(A_eFn, dMdu, W1);
+          for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+              t_LdC += A_eFnM[i * 3 + j] * L_dCdu[i * 3 + j];
+              t_CS += Z2[i * 3 + j] * S[i * 3 + j];
+              t_dM += 0.5 * (W1[i * 3 + j] + W1[j * 3 + i]) * S[i * 3 + j];
+              t_S += A_eFnM[i * 3 + j] * S[i * 3 + j];
+              DpJ += -MI[j * 3 + i] * dMdu[i * 3 + j];
+            }
+          lk[(a * 3 + b) * ndofe + (w * 3 + g)] +=
+              one_over_Jn * detJxW * (t_LdC + 2.0 * t_dM + t_CS + DpJ * t_S);
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int orc_nint(int nne) { return nint_of(nne); }
+
+int orc_basis(int nne, const double *X, int ip, double *N, double *dNdX, double *detJxW) {
+  double k, e, z, w, dNk[30];
+  int_point(nne, ip, &k, &e, &z, &w);
+  shape_N(nne, k, e, z, N);
+  shape_dN(nne, k, e, z, dNk);
+  const double J = dNdX_of(nne, dNk, X, dNdX);
+  *detJxW = J * w;
+  return J > 0.0 ? 0 : 1;
+}
+
+int orc_he_update(const orc_material *mat, const double *F, double *S, double *L) {
+  return he_update(*mat, F, S, L);
+}
+
+int orc_j2_update(const orc_material *mat, const double *F, const double *Cpinv_n, double ep_n,
+                  double *Cpinv_np1, double *ep_np1, double *gam, double *S, double *L) {
+  return j2_update(*mat, F, Cpinv_n, ep_n, Cpinv_np1, ep_np1, gam, S, L);
+}
+
+// generate_local_dof_ids (+ _on_elem), src/generate_dof_ids.cc:69-171,306-412, without
+// periodic nodes, element DOFs or bounding elements.
+int orc_generate_local_dof_ids(int64_t n_nodes, int64_t n_elem, const int64_t *elem_ptr,
+                               const int64_t *elem_conn, int64_t n_coel, const int64_t *coel_ptr,
+                               const int64_t *coel_conn, int64_t *id, int64_t *ndof_out) {
+  const int64_t ndofn = 3;
+  std::vector<long> visited_node_dof(ndofn * n_nodes, 0);
+  long ndof = 1;
+  auto on_elem = [&](const int64_t *nod, int64_t nne) {
+    long id_adder = 0;
+    for (int64_t i = 0; i < nne; i++) {
+      const int64_t node_id = nod[i];
+      for (int64_t j = 0; j < ndofn; j++) {
+        const int64_t dof_idx = node_id * ndofn + j;
+        if (!visited_node_dof[dof_idx]) {
+          if (id[dof_idx] < 0) {
+            visited_node_dof[dof_idx] = 1;
+          } else if (id[dof_idx] == 1) {
+            id[dof_idx] = 0;
+            visited_node_dof[dof_idx] = 1;
+          } else if (id[dof_idx] >= 0) {
+            id[dof_idx] = ndof + id_adder;
+            id_adder++;
+            visited_node_dof[dof_idx] = 1;
+          }
+        }
+      }
+    }
+    return id_adder;
+  };
+  for (int64_t i = 0; i < n_elem; i++) ndof += on_elem(elem_conn + elem_ptr[i], elem_ptr[i + 1] - elem_ptr[i]);
+  for (int64_t i = 0; i < n_coel; i++) ndof += on_elem(coel_conn + coel_ptr[i], coel_ptr[i + 1] - coel_ptr[i]);
+  for (int64_t i = 0; i < ndofn * n_nodes; i++)
+    if (!visited_node_dof[i]) return 1;  // hanging node
+  *ndof_out = ndof - 1;
+  return 0;
+}
+
+static int compare_long(const void *a, const void *b) {
+  const long x = *(const long *)a, y = *(const long *)b;
+  return (x > y) - (x < y);
+}
+
+// Psparse_ApAi for one rank, src/Psparse_ApAi.cc:238-338,600-658 with countDof/mapDof
+// (:160-213); on one rank the local and global ids coincide (LG = identity).
+int orc_psparse_apai(int64_t n_nodes, int64_t n_elem, const int64_t *elem_ptr,
+                     const int64_t *elem_conn, int64_t n_coel, const int64_t *coel_ptr,
+                     const int64_t *coel_conn, const int64_t *id, int64_t ndofd, int32_t *Ap,
+                     int32_t *Ai, int64_t Ai_cap, int64_t *nnz) {
+  (void)n_nodes;
+  std::vector<long> ncols(ndofd, 0);
+  auto for_each_dof = [&](const int64_t *nod, int64_t nne, std::vector<long> &lids) {
+    lids.clear();
+    for (int64_t a = 0; a < nne; ++a)
+      for (int j = 0; j < 3; ++j) {
+        const long lid = id[nod[a] * 3 + j];
+        if (lid > 0) lids.push_back(lid - 1);
+      }
+  };
+  std::vector<long> lids;
+  // 1. countDof
+  for (int64_t e = 0; e < n_elem; ++e) {
+    for_each_dof(elem_conn + elem_ptr[e], elem_ptr[e + 1] - elem_ptr[e], lids);
+    for (long l : lids) ncols[l] += lids.size();
+  }
+  for (int64_t c = 0; c < n_coel; ++c) {
+    for_each_dof(coel_conn + coel_ptr[c], coel_ptr[c + 1] - coel_ptr[c], lids);
+    for (long l : lids) ncols[l] += lids.size();
+  }
+  std::vector<std::vector<long>> AA(ndofd);
+  for (int64_t i = 0; i < ndofd; i++) AA[i].resize(ncols[i]);
+  std::vector<long> ap1(ndofd, 0);
+  // 2. mapDof (gids = lids on one rank)
+  auto map_dof = [&](const int64_t *nod, int64_t nne) {
+    for_each_dof(nod, nne, lids);
+    for (long l : lids)
+      for (long g : lids) AA[l][ap1[l]++] = g;
+  };
+  for (int64_t e = 0; e < n_elem; ++e) map_dof(elem_conn + elem_ptr[e], elem_ptr[e + 1] - elem_ptr[e]);
+  for (int64_t c = 0; c < n_coel; ++c) map_dof(coel_conn + coel_ptr[c], coel_ptr[c + 1] - coel_ptr[c]);
+  // 3. sort each row, count unique
+  for (int64_t k = 0; k < ndofd; k++) qsort(AA[k].data(), ap1[k], sizeof(long), compare_long);
+  std::vector<long> ap(ndofd, 0);
+  for (int64_t k = 0; k < ndofd; k++) {
+    for (long j = 0; j < ap1[k] - 1; j++)
+      if (AA[k][j] < AA[k][j + 1]) ap[k]++;
+    ap[k]++;
+  }
+  // 4. Ap as running sum, Ai = unique sorted ids (:639-658)
+  int64_t total = 0;
+  for (int64_t k = 0; k < ndofd; ++k) total += ap[k];
+  *nnz = total;
+  Ap[0] = 0;
+  for (int64_t k = 0; k < ndofd; ++k) Ap[k + 1] = Ap[k] + int32_t(ap[k]);
+  if (!Ai) return 0;
+  if (Ai_cap < total) return 1;
+  for (int64_t k = 0; k < ndofd; k++) {
+    int64_t p = Ap[k];
+    for (long j = 0; j < ap1[k] - 1; j++)
+      if (AA[k][j] < AA[k][j + 1]) Ai[p++] = int32_t(AA[k][j]);
+    Ai[p++] = int32_t(AA[k][ap1[k] - 1]);
+  }
+  return 0;
+}
+
+// One rank of fd_residuals_MP (src/fd_residuals.cc:468-636) and stiffmat_fd_MP
+// (src/stiffmat_fd.cc:831-993) on the CM path, total Lagrangian: elements in mesh order;
+// per element def_elem_total (src/utils.cc:1391-1412), then
+// residuals_el_constitutive_model_1f (src/constitutive_model.cc:3322-3539) assembled by
+// fd_res_assemble (src/fd_residuals.cc:58-73) and stiffness_el_constitutive_model_1f
+// (:2474-2692) assembled by PLoc_Sparse (src/PLoc_Sparse.cc:15-131) into the CSR values.
+int orc_assemble(int64_t n_nodes, const double *X, int64_t n_elem, const int64_t *elem_ptr,
+                 const int64_t *elem_conn, const int32_t *elem_mat, const orc_material *mats,
+                 const int64_t *id, int64_t ndof, const int32_t *Ap, const int32_t *Ai,
+                 const double *r, const double *d_r, const double *defl, const double *defl_d,
+                 double dt, int do_residual, int run_integration, int do_tangent,
+                 double *F_np1, const double *plastic_n, double *plastic_np1,
+                 const double *scalar_n, double *scalar_np1, double *f, double *vals) {
+  (void)n_nodes;
+  (void)dt;
+  int64_t nqp_total = 0;
+  for (int64_t e = 0; e < n_elem; ++e) nqp_total += nint_of(int(elem_ptr[e + 1] - elem_ptr[e]));
+  const int64_t NQ = nqp_total;
+  if (do_residual && f) std::memset(f, 0, sizeof(double) * ndof);
+  if (do_tangent && vals) std::memset(vals, 0, sizeof(double) * Ap[ndof]);
+  int err = 0;
+  int64_t qp0 = 0;
+  for (int64_t e = 0; e < n_elem; ++e) {
+    const int nne = int(elem_ptr[e + 1] - elem_ptr[e]);
+    const int nint = nint_of(nne);
+    const int ndofe = nne * 3;
+    const int64_t *nod = elem_conn + elem_ptr[e];
+    const orc_material &mat = mats[elem_mat ? elem_mat[e] : 0];
+    double x[30], u[30];
+    int64_t cn[30];
+    for (int a = 0; a < nne; ++a)
+      for (int b = 0; b < 3; ++b) {
+        x[a * 3 + b] = X[nod[a] * 3 + b];
+        const int64_t dof = id[nod[a] * 3 + b];
+        cn[a * 3 + b] = dof;
+        const int64_t aid = std::llabs(dof) - 1;
+        if (dof == 0) u[a * 3 + b] = 0.0;
+        else if (dof > 0) u[a * 3 + b] = r[aid] + (d_r ? d_r[aid] : 0.0);
+        else u[a * 3 + b] = (defl ? defl[aid] : 0.0) + (defl_d ? defl_d[aid] : 0.0);
+      }
+    double fe[30] = {0};
+    std::vector<double> lk(size_t(ndofe) * ndofe, 0.0);
+    const double I9[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    double dMdu_all[270];
+    std::memset(dMdu_all, 0, sizeof(dMdu_all));
+    for (int pass = 0; pass < 2; ++pass) {
+      if (pass == 0 && !do_residual) continue;
+      if (pass == 1 && !do_tangent) continue;
+      for (int ip = 0; ip < nint; ++ip) {
+        const int64_t q = qp0 + ip;
+        double N[10], dNdX[30], detJxW;
+        if (orc_basis(nne, x, ip, N, dNdX, &detJxW)) { ++err; continue; }
+        double ST[270];
+        for (int a = 0; a < nne; ++a)  // compute_shape_tensor, femlib.cc:47-69
+          for (int b = 0; b < 3; ++b)
+            for (int i = 0; i < 3; ++i)
+              for (int J = 0; J < 3; ++J) ST[a * 27 + b * 9 + i * 3 + J] = (b == i) ? dNdX[a * 3 + J] : 0.0;
+        double Fr[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};  // compute_deformation_gradient :1289-1324
+        for (int a = 0; a < nne; a++)
+          for (int b = 0; b < 3; ++b)
+            for (int c = 0; c < 3; ++c) Fr[b * 3 + c] += u[a * 3 + b] * dNdX[a * 3 + c];
+        double Fs[9];
+        if (pass == 0) {
+          if (!(det3x3(Fr) > 0.0)) { ++err; continue; }  // getJacobian
+          if (run_integration)
+            for (int i = 0; i < 9; ++i) F_np1[i * NQ + q] = Fr[i];
+        }
+        for (int i = 0; i < 9; ++i) Fs[i] = F_np1[i * NQ + q];
+        double S[9], L[81];
+        double *Lp = (pass == 1) ? L : nullptr;
+        int merr = 0;
+        if (mat.model == 4) {  // J2: integration stores the n+1 state, S/L from the state
+          double cn6[6], c6[6], ep_n = scalar_n[0 * NQ + q], ep1, gm;
+          for (int i = 0; i < 6; ++i) cn6[i] = plastic_n[i * NQ + q];
+          merr = j2_update(mat, Fs, cn6, ep_n, c6, &ep1, &gm, S, Lp);
+          if (pass == 0 && run_integration && !merr) {
+            for (int i = 0; i < 6; ++i) plastic_np1[i * NQ + q] = c6[i];
+            scalar_np1[0 * NQ + q] = ep1;
+            scalar_np1[1 * NQ + q] = gm;
+          }
+        } else {
+          merr = he_update(mat, Fs, S, Lp);
+        }
+        if (merr) { ++err; continue; }
+        // TL: eFn = I, pFnp1 = I -> M = I, pJ = 1 (HE_PARAM / CM_J2P_PARAM::get_pF)
+        if (pass == 0) {
+          compute_residual_vector(fe, nne, ST, detJxW, Fr, I9, I9, S, 1.0);
+        } else {
+          double FrTFr[9];
+          matmul_tn(Fr, Fr, FrTFr);
+          compute_stiffness_matrix(lk.data(), nne, ST, detJxW, Fr, I9, I9, I9, FrTFr, I9, S, L,
+                                   dMdu_all, 1.0);
+        }
+      }
+    }
+    if (do_tangent)  // non-normal diagonal check, src/constitutive_model.cc:2682-2687
+      for (int a = 0; a < ndofe; ++a)
+        if (!std::isnormal(lk[a * ndofe + a])) ++err;
+    if (do_residual && f)
+      for (int k = 0; k < ndofe; ++k) {
+        const int64_t II = cn[k] - 1;
+        if (II < 0) continue;
+        f[II] += fe[k];
+      }
+    if (do_tangent && vals)
+      for (int i = 0; i < ndofe; ++i) {
+        const int64_t row = cn[i] - 1;
+        if (row < 0) continue;
+        for (int j = 0; j < ndofe; ++j) {
+          const int64_t col = cn[j] - 1;
+          if (col < 0) continue;
+          const int32_t *b = Ai + Ap[row], *en = Ai + Ap[row + 1];
+          const int32_t *p = b;
+          while (p < en && *p != col) ++p;  // SparseSystem::add by (row, col)
+          if (p == en) return -1;
+          vals[p - Ai] += lk[i * ndofe + j];
+        }
+      }
+    qp0 += nint;
+  }
+  return err;
+}
+
+}  // extern "C"

@@ -1,0 +1,51 @@
+// CPU oracle of the PGFem3D finite-strain element hot path -- TEST INFRASTRUCTURE ONLY.
+//
+// A plain FP64 restatement of the reference algorithm, used by tests/, by
+// __graft_entry__.smoke() and by bench.py's CPU-baseline leg as the checker.  The product
+// library (pgfem-3d_amd/) never links, loads or calls it.  Every function cites the
+// reference file:line it follows.  Each function's pinning check lives in tests/ (see the
+// header of oracle.cpp for the list).
+#pragma once
+#include <cstdint>
+
+extern "C" {
+
+struct orc_material {  // same field order as pgf_material (include/pgfem3d_amd.h)
+  int32_t model, devPotFlag, volPotFlag, pad0;
+  double E, nu, G, m10, m01;
+  double k0, hp;
+  double gamma_dot_0, m_rate, G0, g0, gs_0, gamma_dot_s, w_sat;
+  double C11, C12, C44;
+  double tol_M, tol_hardening;
+  int32_t max_itr_M, max_itr_stag;
+};
+
+// --- DOF numbering / sparsity (src/generate_dof_ids.cc, src/Psparse_ApAi.cc)
+int orc_generate_local_dof_ids(int64_t n_nodes, int64_t n_elem, const int64_t *elem_ptr,
+                               const int64_t *elem_conn, int64_t n_coel, const int64_t *coel_ptr,
+                               const int64_t *coel_conn, int64_t *id, int64_t *ndof);
+int orc_psparse_apai(int64_t n_nodes, int64_t n_elem, const int64_t *elem_ptr,
+                     const int64_t *elem_conn, int64_t n_coel, const int64_t *coel_ptr,
+                     const int64_t *coel_conn, const int64_t *id, int64_t ndof, int32_t *Ap,
+                     int32_t *Ai, int64_t Ai_cap, int64_t *nnz);
+
+// --- FEMLIB (FEMLIB/src/femlib.cc)
+int orc_nint(int nne);
+int orc_basis(int nne, const double *X, int ip, double *N, double *dNdX, double *detJxW);
+
+// --- constitutive updates at one material point (row-major 3x3, idx_4 4th order)
+int orc_he_update(const orc_material *mat, const double *F, double *S, double *L);
+// J2: state in: Cpinv_n[6] (xx,yy,zz,xy,yz,xz), ep_n.  out: Cpinv_np1[6], ep_np1, gam, S, L
+int orc_j2_update(const orc_material *mat, const double *F, const double *Cpinv_n, double ep_n,
+                  double *Cpinv_np1, double *ep_np1, double *gam, double *S, double *L);
+
+// --- whole mesh: fd_residuals_MP / stiffmat_fd_MP restated for one rank (TL, CM path)
+// state arrays are [comp][n_qp]; plastic = Cpinv (6 comps) for J2, scalar = (ep, gam).
+int orc_assemble(int64_t n_nodes, const double *X, int64_t n_elem, const int64_t *elem_ptr,
+                 const int64_t *elem_conn, const int32_t *elem_mat, const orc_material *mats,
+                 const int64_t *id, int64_t ndof, const int32_t *Ap, const int32_t *Ai,
+                 const double *r, const double *d_r, const double *defl, const double *defl_d,
+                 double dt, int do_residual, int run_integration, int do_tangent,
+                 double *F_np1, const double *plastic_n, double *plastic_np1,
+                 const double *scalar_n, double *scalar_np1, double *f, double *vals);
+}

@@ -1,0 +1,165 @@
+"""ctypes loader of the CPU oracle (oracle/_build/liboracle.so) -- TEST INFRASTRUCTURE ONLY.
+
+Used by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg as the checker.  The
+product (pgfem-3d_amd/) never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+
+
+class OrcMaterial(C.Structure):
+    _fields_ = [
+        ("model", C.c_int32), ("devPotFlag", C.c_int32), ("volPotFlag", C.c_int32), ("pad0", C.c_int32),
+        ("E", C.c_double), ("nu", C.c_double), ("G", C.c_double), ("m10", C.c_double), ("m01", C.c_double),
+        ("k0", C.c_double), ("hp", C.c_double),
+        ("gamma_dot_0", C.c_double), ("m_rate", C.c_double), ("G0", C.c_double), ("g0", C.c_double),
+        ("gs_0", C.c_double), ("gamma_dot_s", C.c_double), ("w_sat", C.c_double),
+        ("C11", C.c_double), ("C12", C.c_double), ("C44", C.c_double),
+        ("tol_M", C.c_double), ("tol_hardening", C.c_double),
+        ("max_itr_M", C.c_int32), ("max_itr_stag", C.c_int32),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} missing: run `make -C oracle`")
+        _lib = C.CDLL(LIB_PATH)
+        for fn in ("orc_generate_local_dof_ids", "orc_psparse_apai", "orc_basis", "orc_he_update",
+                   "orc_j2_update", "orc_assemble", "orc_nint"):
+            getattr(_lib, fn).restype = C.c_int
+        _lib.orc_j2_update.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_void_p, C.c_void_p,
+                                       C.c_void_p, C.c_void_p, C.c_void_p]
+        _lib.orc_assemble.argtypes = [C.c_int64, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
+                                      C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
+                                      C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_int,
+                                      C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                      C.c_void_p]
+        _lib.orc_psparse_apai.argtypes = [C.c_int64, C.c_int64, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p,
+                                          C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_int64,
+                                          C.c_void_p]
+        _lib.orc_generate_local_dof_ids.argtypes = [C.c_int64, C.c_int64, C.c_void_p, C.c_void_p, C.c_int64,
+                                                    C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        _lib.orc_basis.argtypes = [C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+        _lib.orc_he_update.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    return _lib
+
+
+def _p(a):
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data
+
+
+def material(m) -> OrcMaterial:
+    """Copy a pgf_material-like ctypes struct (same field order) into an OrcMaterial."""
+    o = OrcMaterial()
+    for name, _ in OrcMaterial._fields_:
+        setattr(o, name, getattr(m, name))
+    return o
+
+
+def generate_dof_ids(X, elem_ptr, elem_conn, supports, coel_ptr=None, coel_conn=None):
+    ids = np.ascontiguousarray(supports, dtype=np.int64).copy()
+    nd = C.c_int64()
+    nco = 0 if coel_ptr is None else coel_ptr.shape[0] - 1
+    rc = lib().orc_generate_local_dof_ids(X.shape[0], elem_ptr.shape[0] - 1, _p(elem_ptr), _p(elem_conn), nco,
+                                          _p(coel_ptr), _p(coel_conn), _p(ids), C.byref(nd))
+    if rc:
+        raise RuntimeError("orc_generate_local_dof_ids failed")
+    return ids, nd.value
+
+
+def psparse_apai(X, elem_ptr, elem_conn, ids, ndof, coel_ptr=None, coel_conn=None):
+    L = lib()
+    nco = 0 if coel_ptr is None else coel_ptr.shape[0] - 1
+    Ap = np.zeros(ndof + 1, dtype=np.int32)
+    nnz = C.c_int64()
+    L.orc_psparse_apai(X.shape[0], elem_ptr.shape[0] - 1, _p(elem_ptr), _p(elem_conn), nco, _p(coel_ptr),
+                       _p(coel_conn), _p(ids), ndof, _p(Ap), None, 0, C.byref(nnz))
+    Ai = np.zeros(nnz.value, dtype=np.int32)
+    rc = L.orc_psparse_apai(X.shape[0], elem_ptr.shape[0] - 1, _p(elem_ptr), _p(elem_conn), nco, _p(coel_ptr),
+                            _p(coel_conn), _p(ids), ndof, _p(Ap), _p(Ai), nnz.value, C.byref(nnz))
+    if rc:
+        raise RuntimeError("orc_psparse_apai failed")
+    return Ap, Ai
+
+
+def basis(nne, X, ip):
+    N = np.zeros(nne)
+    dNdX = np.zeros((nne, 3))
+    w = C.c_double()
+    rc = lib().orc_basis(nne, _p(np.ascontiguousarray(X, dtype=np.float64)), ip, _p(N), _p(dNdX), C.byref(w))
+    return N, dNdX, w.value, rc
+
+
+def nint(nne):
+    return lib().orc_nint(nne)
+
+
+def he_update(mat, F, want_L=True):
+    S = np.zeros(9)
+    L = np.zeros(81) if want_L else None
+    F = np.ascontiguousarray(F, dtype=np.float64).reshape(9)
+    om = material(mat)
+    rc = lib().orc_he_update(C.byref(om), _p(F), _p(S), _p(L))
+    return S.reshape(3, 3), (L.reshape(3, 3, 3, 3) if want_L else None), rc
+
+
+def j2_update(mat, F, Cpinv_n, ep_n, want_L=True):
+    S = np.zeros(9)
+    L = np.zeros(81) if want_L else None
+    c6 = np.zeros(6)
+    ep1, gam = C.c_double(), C.c_double()
+    F = np.ascontiguousarray(F, dtype=np.float64).reshape(9)
+    cn = np.ascontiguousarray(Cpinv_n, dtype=np.float64)
+    om = material(mat)
+    rc = lib().orc_j2_update(C.byref(om), _p(F), _p(cn), float(ep_n), _p(c6), C.byref(ep1), C.byref(gam), _p(S),
+                             _p(L))
+    return dict(S=S.reshape(3, 3), L=(L.reshape(3, 3, 3, 3) if want_L else None), Cpinv=c6, ep=ep1.value,
+                gam=gam.value, rc=rc)
+
+
+class OracleState:
+    """SoA state arrays [comp][n_qp] in mesh QP order, as the oracle's assembly expects."""
+
+    def __init__(self, n_qp, has_j2):
+        self.n_qp = n_qp
+        eye = np.array([1.0, 0, 0, 0, 1, 0, 0, 0, 1])
+        self.F_np1 = np.ascontiguousarray(np.repeat(eye[:, None], n_qp, axis=1))
+        c6 = np.array([1.0, 1, 1, 0, 0, 0])
+        self.P_n = np.ascontiguousarray(np.repeat(c6[:, None], n_qp, axis=1))
+        self.P_np1 = self.P_n.copy()
+        self.S_n = np.zeros((2, n_qp))
+        self.S_np1 = np.zeros((2, n_qp))
+
+    def commit(self):
+        self.P_n[:] = self.P_np1
+        self.S_n[:] = self.S_np1
+
+
+def assemble(mesh, mats, ids, ndof, Ap, Ai, state: OracleState, r, d_r=None, defl=None, defl_d=None, dt=1.0,
+             residual=True, integrate=True, tangent=True):
+    """fd_residuals_MP + stiffmat_fd_MP restated on the CPU. Returns (f, vals, err)."""
+    L = lib()
+    om = (OrcMaterial * len(mats))(*[material(m) for m in mats])
+    f = np.zeros(ndof)
+    vals = np.zeros(Ap[-1])
+    em = mesh.elem_mat if mesh.elem_mat is not None else None
+    err = L.orc_assemble(mesh.n_nodes, _p(mesh.X), mesh.n_elem, _p(mesh.elem_ptr), _p(mesh.elem_conn), _p(em), om,
+                         _p(ids), ndof, _p(Ap), _p(Ai), _p(r), _p(d_r), _p(defl), _p(defl_d), dt, int(residual),
+                         int(integrate), int(tangent), _p(state.F_np1), _p(state.P_n), _p(state.P_np1),
+                         _p(state.S_n), _p(state.S_np1), _p(f), _p(vals))
+    return f, vals, err

@@ -1,0 +1,453 @@
+// C ABI of the element hot path: device handle, schedule of coloured launches, state.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "pgf_device.h"
+#include "pgf_internal.h"
+#include "pgfem3d_amd.h"
+
+namespace pgf {
+namespace {
+
+constexpr int nqp_of(int nne) { return nne == 4 ? 1 : (nne == 10 ? 4 : 8); }
+
+// Elements of one (element type, material model) pair, in device order (sorted by colour,
+// then by mesh order).  Their QPs occupy [qp_base, qp_base + ne*nqp) of the state arrays.
+struct Block {
+  int nne = 0, nqp = 0, model = 0;
+  int64_t ne = 0, qp_base = 0;
+  std::vector<int64_t> elems;      // mesh element ids in device order
+  std::vector<int64_t> color_ptr;  // [n_colors+1] segments of elems
+  int32_t *d_conn = nullptr;
+  int16_t *d_off = nullptr;
+  int32_t *d_mat = nullptr;
+};
+
+template <class T>
+int dalloc(T **p, size_t n) {
+  *p = nullptr;
+  if (n == 0) return PGF_OK;
+  return hipMalloc(reinterpret_cast<void **>(p), n * sizeof(T)) == hipSuccess ? PGF_OK : PGF_ENOMEM;
+}
+template <class T>
+int upload(T **p, const T *h, size_t n) {
+  int rc = dalloc(p, n);
+  if (rc) return rc;
+  if (n && hipMemcpy(*p, h, n * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) return PGF_EHIP;
+  return PGF_OK;
+}
+
+}  // namespace
+}  // namespace pgf
+
+struct pgf_handle_s {
+  int device = 0;
+  hipStream_t own_stream = nullptr, stream = nullptr;
+  int64_t n_nodes = 0, ndof = 0, nnz = 0, n_qp = 0;
+  int32_t n_colors = 0;
+  bool has_j2 = false;
+  double *d_X = nullptr;
+  int32_t *d_dofid = nullptr, *d_row_start = nullptr, *d_rowlen = nullptr;
+  pgf::DevMaterial *d_mats = nullptr;
+  std::vector<pgf::Block> blocks;
+  std::vector<int64_t> qp_mesh;  // device QP -> mesh QP
+  // state, SoA [comp][n_qp]; index 0 = n+1, 1 = n, 2 = n-1
+  double *d_F[3] = {nullptr, nullptr, nullptr};
+  double *d_P[3] = {nullptr, nullptr, nullptr};   // J2: Cp^-1 (6)
+  double *d_Sc[3] = {nullptr, nullptr, nullptr};  // J2: ep, gam
+  int32_t *d_err = nullptr;
+};
+
+namespace {
+
+using pgf::Block;
+
+int set_identity_state(pgf_handle h) {
+  const int64_t NQ = h->n_qp;
+  for (int t = 0; t < 3; ++t) {
+    for (int i = 0; i < 9; ++i) pgf::launch_fill(h->d_F[t] + i * NQ, NQ, (i % 4 == 0) ? 1.0 : 0.0, h->stream);
+    if (h->has_j2) {
+      for (int I = 0; I < 6; ++I) pgf::launch_fill(h->d_P[t] + I * NQ, NQ, I < 3 ? 1.0 : 0.0, h->stream);
+      pgf::launch_fill(h->d_Sc[t], 2 * NQ, 0.0, h->stream);
+    }
+  }
+  return hipStreamSynchronize(h->stream) == hipSuccess ? PGF_OK : PGF_EHIP;
+}
+
+pgf::ElemArgs base_args(pgf_handle h, const pgf_nodal_input *in) {
+  pgf::ElemArgs a{};
+  a.X = h->d_X;
+  a.dofid = h->d_dofid;
+  a.row_start = h->d_row_start;
+  a.rowlen = h->d_rowlen;
+  a.mats = h->d_mats;
+  a.n_qp = h->n_qp;
+  if (in) {
+    a.r = in->r;
+    a.d_r = in->d_r;
+    a.defl = in->defl;
+    a.defl_d = in->defl_d;
+  }
+  a.F_np1 = h->d_F[0];
+  a.P_n = h->d_P[1];
+  a.P_np1 = h->d_P[0];
+  a.Sc_n = h->d_Sc[1];
+  a.Sc_np1 = h->d_Sc[0];
+  a.err = h->d_err;
+  return a;
+}
+
+// One pass over all elements in the coloured schedule.
+int run_pass(pgf_handle h, const pgf_nodal_input *in, int mode, double *f, double *vals) {
+  if (!in || !in->r) return PGF_EINVAL;
+  if (hipMemsetAsync(h->d_err, 0, sizeof(int32_t), h->stream) != hipSuccess) return PGF_EHIP;
+  pgf::ElemArgs a = base_args(h, in);
+  a.f = f;
+  a.vals = vals;
+  for (int32_t c = 0; c < h->n_colors; ++c)
+    for (auto &b : h->blocks) {
+      const int64_t e0 = b.color_ptr[c], e1 = b.color_ptr[c + 1];
+      if (e1 <= e0) continue;
+      a.conn = b.d_conn;
+      a.off = b.d_off;
+      a.mat = b.d_mat;
+      a.qp_base = b.qp_base;
+      pgf::launch_elem(b.nne, mode, a, e0, e1, h->stream);
+    }
+  if (hipGetLastError() != hipSuccess) return PGF_EHIP;
+  int32_t err = 0;
+  if (hipMemcpyAsync(&err, h->d_err, sizeof(int32_t), hipMemcpyDeviceToHost, h->stream) != hipSuccess)
+    return PGF_EHIP;
+  if (hipStreamSynchronize(h->stream) != hipSuccess) return PGF_EHIP;
+  return err;
+}
+
+int field_buffer(pgf_handle h, int32_t field, double **buf, int32_t *ncomp) {
+  *buf = nullptr;
+  *ncomp = 0;
+  switch (field) {
+    case PGF_STATE_F_NP1: case PGF_STATE_F_N: case PGF_STATE_F_NM1:
+      *buf = h->d_F[field - PGF_STATE_F_NP1];
+      *ncomp = 9;
+      return PGF_OK;
+    case PGF_STATE_PLASTIC_NP1: case PGF_STATE_PLASTIC_N: case PGF_STATE_PLASTIC_NM1:
+      if (!h->has_j2) return PGF_OK;
+      *buf = h->d_P[field - PGF_STATE_PLASTIC_NP1];
+      *ncomp = 6;
+      return PGF_OK;
+    case PGF_STATE_SCALAR_NP1: case PGF_STATE_SCALAR_N: case PGF_STATE_SCALAR_NM1:
+      if (!h->has_j2) return PGF_OK;
+      *buf = h->d_Sc[field - PGF_STATE_SCALAR_NP1];
+      *ncomp = 2;
+      return PGF_OK;
+  }
+  return PGF_EINVAL;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pgf_create(const pgf_mesh *mesh, const int64_t *id, int64_t ndof, const int32_t *Ap,
+               const int32_t *Ai, int32_t n_mat, const pgf_material *mats, pgf_handle *out) {
+  if (!mesh || !id || !Ap || !Ai || n_mat < 1 || !mats || !out) return PGF_EINVAL;
+  *out = nullptr;
+  const pgf_mesh &m = *mesh;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return PGF_ENODEV;
+  // --- validate
+  for (int64_t e = 0; e < m.n_elem; ++e) {
+    const int64_t nne = m.elem_ptr[e + 1] - m.elem_ptr[e];
+    if (!(nne == 4 || nne == 10 || nne == 8)) return PGF_ENOTSUP;
+    const int32_t mi = m.elem_mat ? m.elem_mat[e] : 0;
+    if (mi < 0 || mi >= n_mat) return PGF_EINVAL;
+    for (int64_t k = m.elem_ptr[e]; k < m.elem_ptr[e + 1]; ++k)
+      if (m.elem_conn[k] < 0 || m.elem_conn[k] >= m.n_nodes) return PGF_EINVAL;
+  }
+  for (int32_t i = 0; i < n_mat; ++i) {
+    const int md = mats[i].model;
+    if (md == PGF_MODEL_HYPER_ELASTICITY) {
+      if (mats[i].m01 != 0.0 || mats[i].devPotFlag != 1) return PGF_ENOTSUP;  // neo-Hookean only
+    } else if (md != PGF_MODEL_J2_PLASTICITY) {
+      return PGF_ENOTSUP;
+    }
+  }
+  // free DOFs of a node must be consecutive ids (no periodic nodes)
+  std::vector<int32_t> dofid(size_t(m.n_nodes) * 3), row_start(m.n_nodes, -1), rowlen(m.n_nodes, 0);
+  for (int64_t n = 0; n < m.n_nodes; ++n) {
+    int64_t first = -1, prev = -1;
+    for (int j = 0; j < 3; ++j) {
+      const int64_t d = id[n * 3 + j];
+      if (d > ndof || d < INT32_MIN) return PGF_EINVAL;
+      dofid[n * 3 + j] = int32_t(d);
+      if (d > 0) {
+        if (prev > 0 && d != prev + 1) return PGF_ENOTSUP;
+        if (first < 0) first = d;
+        prev = d;
+      }
+    }
+    if (first > 0) {
+      row_start[n] = Ap[first - 1];
+      rowlen[n] = Ap[first] - Ap[first - 1];
+      for (int64_t d = first; d <= prev; ++d)
+        if (Ap[d] - Ap[d - 1] != rowlen[n]) return PGF_EINVAL;  // rows of a node share a pattern
+    }
+  }
+  // --- colouring and blocks
+  std::vector<int32_t> color;
+  const int ncol = pgf::color_elements(m, color);
+  if (ncol < 0) return PGF_ENOTSUP;
+  auto *h = new pgf_handle_s();
+  h->n_nodes = m.n_nodes;
+  h->ndof = ndof;
+  h->nnz = Ap[ndof];
+  h->n_colors = ncol;
+  hipGetDevice(&h->device);
+  if (hipStreamCreate(&h->own_stream) != hipSuccess) {
+    delete h;
+    return PGF_EHIP;
+  }
+  h->stream = h->own_stream;
+  auto fail = [&](int rc) {
+    pgf_destroy(h);
+    return rc;
+  };
+  std::vector<std::pair<int, int>> keys;  // (nne, model)
+  std::vector<int> elem_block(m.n_elem);
+  for (int64_t e = 0; e < m.n_elem; ++e) {
+    const int nne = int(m.elem_ptr[e + 1] - m.elem_ptr[e]);
+    const int md = mats[m.elem_mat ? m.elem_mat[e] : 0].model;
+    const auto key = std::make_pair(nne, md);
+    auto it = std::find(keys.begin(), keys.end(), key);
+    if (it == keys.end()) {
+      keys.push_back(key);
+      it = keys.end() - 1;
+    }
+    elem_block[e] = int(it - keys.begin());
+    if (md == PGF_MODEL_J2_PLASTICITY) h->has_j2 = true;
+  }
+  h->blocks.resize(keys.size());
+  int64_t qp = 0;
+  for (size_t bi = 0; bi < keys.size(); ++bi) {
+    Block &b = h->blocks[bi];
+    b.nne = keys[bi].first;
+    b.model = keys[bi].second;
+    b.nqp = pgf::nqp_of(b.nne);
+    for (int64_t e = 0; e < m.n_elem; ++e)
+      if (elem_block[e] == int(bi)) b.elems.push_back(e);
+    std::stable_sort(b.elems.begin(), b.elems.end(), [&](int64_t x, int64_t y) { return color[x] < color[y]; });
+    b.ne = int64_t(b.elems.size());
+    b.color_ptr.assign(ncol + 1, 0);
+    for (int64_t e : b.elems) b.color_ptr[color[e] + 1]++;
+    for (int c = 0; c < ncol; ++c) b.color_ptr[c + 1] += b.color_ptr[c];
+    b.qp_base = qp;
+    qp += b.ne * b.nqp;
+  }
+  h->n_qp = qp;
+  // mesh QP numbering: elements in mesh order, QPs in element order
+  std::vector<int64_t> qp_of_elem(m.n_elem + 1, 0);
+  for (int64_t e = 0; e < m.n_elem; ++e)
+    qp_of_elem[e + 1] = qp_of_elem[e] + pgf::nqp_of(int(m.elem_ptr[e + 1] - m.elem_ptr[e]));
+  h->qp_mesh.resize(h->n_qp);
+  for (auto &b : h->blocks)
+    for (int64_t k = 0; k < b.ne; ++k)
+      for (int ip = 0; ip < b.nqp; ++ip) h->qp_mesh[b.qp_base + k * b.nqp + ip] = qp_of_elem[b.elems[k]] + ip;
+  // --- scatter offsets in device order, with first-touch marks following the schedule
+  std::vector<int64_t> pair_ptr;
+  std::vector<int16_t> off;
+  if (pgf::build_scatter_offsets(m, id, ndof, Ap, Ai, pair_ptr, off) != PGF_OK) return fail(PGF_EINVAL);
+  std::vector<uint8_t> touched(size_t(h->nnz), 0);
+  std::vector<std::vector<int16_t>> boff(h->blocks.size());
+  for (size_t bi = 0; bi < h->blocks.size(); ++bi) boff[bi].resize(h->blocks[bi].ne * h->blocks[bi].nne * h->blocks[bi].nne);
+  for (int32_t c = 0; c < ncol; ++c)
+    for (size_t bi = 0; bi < h->blocks.size(); ++bi) {
+      Block &b = h->blocks[bi];
+      const int nne = b.nne;
+      for (int64_t k = b.color_ptr[c]; k < b.color_ptr[c + 1]; ++k) {
+        const int64_t e = b.elems[k];
+        const int64_t *conn = m.elem_conn + m.elem_ptr[e];
+        for (int a = 0; a < nne; ++a)
+          for (int w = 0; w < nne; ++w) {
+            int16_t o = off[pair_ptr[e] + a * nne + w];
+            if (o >= 0) {
+              const int64_t pos = int64_t(row_start[conn[a]]) + o;
+              if (!touched[pos]) {
+                touched[pos] = 1;
+                o = int16_t(-int(o) - 2);
+              }
+            }
+            boff[bi][(k * nne + a) * nne + w] = o;
+          }
+      }
+    }
+  // --- upload
+  int rc = PGF_OK;
+  std::vector<pgf::DevMaterial> dm(n_mat);
+  for (int32_t i = 0; i < n_mat; ++i) {
+    dm[i].model = mats[i].model;
+    dm[i].volPotFlag = mats[i].volPotFlag;
+    dm[i].G = (mats[i].model == PGF_MODEL_HYPER_ELASTICITY) ? 2.0 * mats[i].m10 : mats[i].G;
+    dm[i].kappa = mats[i].E / (3.0 * (1.0 - 2.0 * mats[i].nu));
+    dm[i].k0 = mats[i].k0;
+    dm[i].hp = mats[i].hp;
+  }
+  if ((rc = pgf::upload(&h->d_mats, dm.data(), dm.size()))) return fail(rc);
+  if ((rc = pgf::upload(&h->d_X, m.X, size_t(m.n_nodes) * 3))) return fail(rc);
+  if ((rc = pgf::upload(&h->d_dofid, dofid.data(), dofid.size()))) return fail(rc);
+  if ((rc = pgf::upload(&h->d_row_start, row_start.data(), row_start.size()))) return fail(rc);
+  if ((rc = pgf::upload(&h->d_rowlen, rowlen.data(), rowlen.size()))) return fail(rc);
+  for (size_t bi = 0; bi < h->blocks.size(); ++bi) {
+    Block &b = h->blocks[bi];
+    std::vector<int32_t> conn(b.ne * b.nne), mat(b.ne);
+    for (int64_t k = 0; k < b.ne; ++k) {
+      const int64_t e = b.elems[k];
+      for (int a = 0; a < b.nne; ++a) conn[k * b.nne + a] = int32_t(m.elem_conn[m.elem_ptr[e] + a]);
+      mat[k] = m.elem_mat ? m.elem_mat[e] : 0;
+    }
+    if ((rc = pgf::upload(&b.d_conn, conn.data(), conn.size()))) return fail(rc);
+    if ((rc = pgf::upload(&b.d_mat, mat.data(), mat.size()))) return fail(rc);
+    if ((rc = pgf::upload(&b.d_off, boff[bi].data(), boff[bi].size()))) return fail(rc);
+  }
+  for (int t = 0; t < 3; ++t) {
+    if ((rc = pgf::dalloc(&h->d_F[t], size_t(h->n_qp) * 9))) return fail(rc);
+    if (h->has_j2) {
+      if ((rc = pgf::dalloc(&h->d_P[t], size_t(h->n_qp) * 6))) return fail(rc);
+      if ((rc = pgf::dalloc(&h->d_Sc[t], size_t(h->n_qp) * 2))) return fail(rc);
+    }
+  }
+  if ((rc = pgf::dalloc(&h->d_err, 1))) return fail(rc);
+  if ((rc = set_identity_state(h))) return fail(rc);
+  *out = h;
+  return PGF_OK;
+}
+
+int pgf_destroy(pgf_handle h) {
+  if (!h) return PGF_OK;
+  if (h->stream) hipStreamSynchronize(h->stream);
+  hipFree(h->d_X);
+  hipFree(h->d_dofid);
+  hipFree(h->d_row_start);
+  hipFree(h->d_rowlen);
+  hipFree(h->d_mats);
+  for (auto &b : h->blocks) {
+    hipFree(b.d_conn);
+    hipFree(b.d_off);
+    hipFree(b.d_mat);
+  }
+  for (int t = 0; t < 3; ++t) {
+    hipFree(h->d_F[t]);
+    hipFree(h->d_P[t]);
+    hipFree(h->d_Sc[t]);
+  }
+  hipFree(h->d_err);
+  if (h->own_stream) hipStreamDestroy(h->own_stream);
+  delete h;
+  return PGF_OK;
+}
+
+int pgf_set_stream(pgf_handle h, void *stream) {
+  if (!h) return PGF_EINVAL;
+  h->stream = stream ? reinterpret_cast<hipStream_t>(stream) : h->own_stream;
+  return PGF_OK;
+}
+
+int pgf_synchronize(pgf_handle h) {
+  if (!h) return PGF_EINVAL;
+  return hipStreamSynchronize(h->stream) == hipSuccess ? PGF_OK : PGF_EHIP;
+}
+
+int pgf_get_sizes(pgf_handle h, int64_t *n_qp, int64_t *nnz, int64_t *ndof, int32_t *n_colors) {
+  if (!h) return PGF_EINVAL;
+  if (n_qp) *n_qp = h->n_qp;
+  if (nnz) *nnz = h->nnz;
+  if (ndof) *ndof = h->ndof;
+  if (n_colors) *n_colors = h->n_colors;
+  return PGF_OK;
+}
+
+int pgf_residual(pgf_handle h, const pgf_nodal_input *in, double dt, int run_integration, double *f) {
+  if (!h || !f) return PGF_EINVAL;
+  (void)dt;
+  return run_pass(h, in, run_integration ? (pgf::MODE_INTEGRATE | pgf::MODE_RESIDUAL) : pgf::MODE_RESIDUAL,
+                  f, nullptr);
+}
+
+int pgf_tangent(pgf_handle h, const pgf_nodal_input *in, double dt, double *vals) {
+  if (!h || !vals) return PGF_EINVAL;
+  (void)dt;
+  return run_pass(h, in, pgf::MODE_TANGENT, nullptr, vals);
+}
+
+int pgf_residual_tangent(pgf_handle h, const pgf_nodal_input *in, double dt, double *f, double *vals) {
+  if (!h || !f || !vals) return PGF_EINVAL;
+  (void)dt;
+  return run_pass(h, in, pgf::MODE_INTEGRATE | pgf::MODE_RESIDUAL | pgf::MODE_TANGENT, f, vals);
+}
+
+int pgf_commit_step(pgf_handle h) {
+  if (!h) return PGF_EINVAL;
+  const int64_t NQ = h->n_qp;
+  pgf::launch_copy(h->d_F[2], h->d_F[1], 9 * NQ, h->stream);
+  pgf::launch_copy(h->d_F[1], h->d_F[0], 9 * NQ, h->stream);
+  if (h->has_j2) {
+    pgf::launch_copy(h->d_P[2], h->d_P[1], 6 * NQ, h->stream);
+    pgf::launch_copy(h->d_P[1], h->d_P[0], 6 * NQ, h->stream);
+    pgf::launch_copy(h->d_Sc[2], h->d_Sc[1], 2 * NQ, h->stream);
+    pgf::launch_copy(h->d_Sc[1], h->d_Sc[0], 2 * NQ, h->stream);
+  }
+  return hipStreamSynchronize(h->stream) == hipSuccess ? PGF_OK : PGF_EHIP;
+}
+
+int pgf_reset_step(pgf_handle h) {
+  if (!h) return PGF_EINVAL;
+  const int64_t NQ = h->n_qp;
+  pgf::launch_copy(h->d_F[0], h->d_F[1], 9 * NQ, h->stream);
+  if (h->has_j2) {
+    pgf::launch_copy(h->d_P[0], h->d_P[1], 6 * NQ, h->stream);
+    pgf::launch_copy(h->d_Sc[0], h->d_Sc[1], 2 * NQ, h->stream);
+  }
+  return hipStreamSynchronize(h->stream) == hipSuccess ? PGF_OK : PGF_EHIP;
+}
+
+int pgf_state_ncomp(pgf_handle h, int32_t field, int32_t *ncomp) {
+  if (!h || !ncomp) return PGF_EINVAL;
+  double *buf;
+  return field_buffer(h, field, &buf, ncomp);
+}
+
+int pgf_get_state(pgf_handle h, int32_t field, double *host_out) {
+  if (!h || !host_out) return PGF_EINVAL;
+  double *buf;
+  int32_t nc;
+  int rc = field_buffer(h, field, &buf, &nc);
+  if (rc || nc == 0) return rc ? rc : PGF_EINVAL;
+  const int64_t NQ = h->n_qp;
+  std::vector<double> tmp(size_t(nc) * NQ);
+  if (hipStreamSynchronize(h->stream) != hipSuccess) return PGF_EHIP;
+  if (hipMemcpy(tmp.data(), buf, tmp.size() * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) return PGF_EHIP;
+  for (int32_t c = 0; c < nc; ++c)
+    for (int64_t q = 0; q < NQ; ++q) host_out[c * NQ + h->qp_mesh[q]] = tmp[c * NQ + q];
+  return PGF_OK;
+}
+
+int pgf_set_state(pgf_handle h, int32_t field, const double *host_in) {
+  if (!h || !host_in) return PGF_EINVAL;
+  double *buf;
+  int32_t nc;
+  int rc = field_buffer(h, field, &buf, &nc);
+  if (rc || nc == 0) return rc ? rc : PGF_EINVAL;
+  const int64_t NQ = h->n_qp;
+  std::vector<double> tmp(size_t(nc) * NQ);
+  for (int32_t c = 0; c < nc; ++c)
+    for (int64_t q = 0; q < NQ; ++q) tmp[c * NQ + q] = host_in[c * NQ + h->qp_mesh[q]];
+  if (hipStreamSynchronize(h->stream) != hipSuccess) return PGF_EHIP;
+  if (hipMemcpy(buf, tmp.data(), tmp.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) return PGF_EHIP;
+  return PGF_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,560 @@
+// Element hot-path kernels for gfx950 (FP64 throughout).
+//
+// One launch processes the elements of one (element type, colour) segment.  A 256-thread
+// workgroup holds EPB elements; each element owns TPE consecutive lanes:
+//   phase 0  lanes a < nne        gather node coordinates, DOF ids and displacements
+//   phase 1  lanes q < nqp        shape gradients, F, constitutive update, tau, c  -> LDS
+//   phase 2  lanes a*3+b < ndofe  one element row each: residual entry f_ab and the
+//                                 tangent row K_ab,wg summed over the QPs, then scattered
+//                                 into the CSR values by node-pair blocks.
+// Elements of one colour share no node, so the scatter is a plain read-modify-write (no
+// atomics); the first element (in launch order) to touch a node-pair block stores instead
+// of adding, so neither f nor the CSR values need zeroing and each entry's sum runs in the
+// same fixed order on every call.
+//
+// The contraction is the reference's compute_residual_vector / compute_stiffness_matrix
+// (src/constitutive_model.cc:1520-1662) for TL with M = I, written in the current
+// configuration: with Fs the stored F(n+1), tau = Fs S Fs^T, h_a = Fs^-T dN_a/dX and
+// Gr = Fr Fs^-1 (Gr = I when the integration ran in the same pass):
+//   f_ab      = w detJ (Gr tau h_a)_b
+//   K_ab,wg   = w detJ [ (Gr R_ab h_w)_g + delta_bg h_a.tau.h_w ],  R_ab = mat(b_ab^T c)
+// where c is the spatial (Kirchhoff) tangent in Voigt form and b_ab the engineering-strain
+// vector of sym(Gr^T e_b (x) h_a).  This equals B_ab:L:B_wg + delta_bg dN_a.S.dN_w with
+// L = Fs^-1 Fs^-1 Fs^-1 Fs^-1 : c.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "pgf_device.h"
+
+namespace pgf {
+namespace {
+
+constexpr int nqp_of(int nne) { return nne == 4 ? 1 : (nne == 10 ? 4 : 8); }
+
+// FEMLIB quadrature (FEMLIB/src/femlib.cc:544-588), truncated constants kept.
+__device__ __forceinline__ void int_point(int nne, int ip, double &k, double &e, double &z,
+                                          double &w) {
+  if (nne == 4) {
+    k = e = z = 1.0 / 4.0;
+    w = 1.0 / 6.0;
+  } else if (nne == 10) {
+    const double A = 0.58541020, B = 0.13819660;
+    k = (ip == 0) ? A : B;
+    e = (ip == 1) ? A : B;
+    z = (ip == 2) ? A : B;
+    w = (1.0 / 4.0) / 6.0;
+  } else {
+    const double g = 0.57735026918962584;
+    k = (ip / 4) ? g : -g;
+    e = ((ip / 2) % 2) ? g : -g;
+    z = (ip % 2) ? g : -g;
+    w = 1.0;
+  }
+}
+
+// dN/d(ksi,eta,zet), FEMLIB/src/femlib.cc:214-340; dN[a*3+c]
+template <int NNE>
+__device__ __forceinline__ void shape_dN(double k, double e, double z, double *dN) {
+  if constexpr (NNE == 4) {
+    dN[0] = -1.0; dN[1] = -1.0; dN[2] = -1.0;
+    dN[3] = 1.0;  dN[4] = 0.0;  dN[5] = 0.0;
+    dN[6] = 0.0;  dN[7] = 1.0;  dN[8] = 0.0;
+    dN[9] = 0.0;  dN[10] = 0.0; dN[11] = 1.0;
+  } else if constexpr (NNE == 10) {
+    dN[0] = 4.0 * (k + e + z) - 3.0;
+    dN[1] = 4.0 * (e + k + z) - 3.0;
+    dN[2] = 4.0 * (z + k + e) - 3.0;
+    dN[3] = 4.0 * k - 1.0; dN[4] = 0.0; dN[5] = 0.0;
+    dN[6] = 0.0; dN[7] = 4.0 * e - 1.0; dN[8] = 0.0;
+    dN[9] = 0.0; dN[10] = 0.0; dN[11] = 4.0 * z - 1.0;
+    dN[12] = 4.0 * (-2.0 * k - e - z + 1.0); dN[13] = -4.0 * k; dN[14] = -4.0 * k;
+    dN[15] = 4.0 * e; dN[16] = 4.0 * k; dN[17] = 0.0;
+    dN[18] = -4.0 * e; dN[19] = 4.0 * (-k - 2.0 * e - z + 1.0); dN[20] = -4.0 * e;
+    dN[21] = -4.0 * z; dN[22] = -4.0 * z; dN[23] = 4.0 * (-k - e - 2.0 * z + 1.0);
+    dN[24] = 4.0 * z; dN[25] = 0.0; dN[26] = 4.0 * k;
+    dN[27] = 0.0; dN[28] = 4.0 * z; dN[29] = 4.0 * e;
+  } else {
+    const double o = 0.125;
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+      const double sk = ((a + 1) & 2) ? 1.0 : -1.0;  // nodes 1,2,5,6: +ksi
+      const double se = (a & 2) ? 1.0 : -1.0;        // nodes 2,3,6,7: +eta
+      const double sz = (a & 4) ? 1.0 : -1.0;        // nodes 4..7: +zet
+      dN[a * 3 + 0] = sk * o * (1.0 + se * e) * (1.0 + sz * z);
+      dN[a * 3 + 1] = se * o * (1.0 + sk * k) * (1.0 + sz * z);
+      dN[a * 3 + 2] = sz * o * (1.0 + sk * k) * (1.0 + se * e);
+    }
+  }
+}
+
+__device__ __forceinline__ double det3(const double *m) {
+  return m[0] * (m[4] * m[8] - m[5] * m[7]) + m[1] * (m[5] * m[6] - m[3] * m[8]) +
+         m[2] * (m[3] * m[7] - m[4] * m[6]);
+}
+
+// inv3x3 with the reference's singularity threshold (src/utils.cc:542-562)
+__device__ __forceinline__ bool inv3(const double *m, double *mi) {
+  const double A = det3(m);
+  if (!(fabs(A) >= 1.0e-15)) return false;
+  const double r = 1.0 / A;
+  mi[0] = r * (m[4] * m[8] - m[5] * m[7]);
+  mi[1] = -r * (m[1] * m[8] - m[2] * m[7]);
+  mi[2] = r * (m[1] * m[5] - m[2] * m[4]);
+  mi[3] = -r * (m[3] * m[8] - m[5] * m[6]);
+  mi[4] = r * (m[0] * m[8] - m[2] * m[6]);
+  mi[5] = -r * (m[0] * m[5] - m[2] * m[3]);
+  mi[6] = r * (m[3] * m[7] - m[4] * m[6]);
+  mi[7] = -r * (m[0] * m[7] - m[1] * m[6]);
+  mi[8] = r * (m[0] * m[4] - m[1] * m[3]);
+  return true;
+}
+
+// volumetric potential derivatives (src/new_potentials.cc:575-616)
+__device__ __forceinline__ void vol_pot(int flag, double J, double &dU, double &d2U) {
+  if (flag == 3) {
+    dU = (-1 + J) / (2. * J) + log(J) * .5;
+    d2U = 1. / J - (-1 + J) / (2. * J * J);
+  } else if (flag == 99) {
+    dU = J - 1;
+    d2U = 1.0;
+  } else {
+    const double ex = exp(J - 1.0);
+    dU = (ex - 1. / J) * .5;
+    d2U = (ex + 1. / (J * J)) * .5;
+  }
+}
+
+// symmetric 3x3 in Voigt (xx,yy,zz,xy,yz,xz) times vector
+__device__ __forceinline__ void symv(const double *s, const double *x, double *y) {
+  y[0] = s[0] * x[0] + s[3] * x[1] + s[5] * x[2];
+  y[1] = s[3] * x[0] + s[1] * x[1] + s[4] * x[2];
+  y[2] = s[5] * x[0] + s[4] * x[1] + s[2] * x[2];
+}
+
+// Neo-Hookean / J2 material point in the current configuration.
+// In: Fs, its inverse and J; J2 state (Cp^-1 at n, ep_n).  Out: tau (Voigt), c (Voigt 6x6,
+// row I = stress component, column J = engineering strain component) and, when
+// integrating, the n+1 state.  HE is the J2 elastic branch with Cp^-1 = I.
+template <bool WITH_C>
+__device__ __forceinline__ void j2_point(const DevMaterial &m, bool plastic, const double *Fs,
+                                         const double *Fi, double J, const double *Cn,
+                                         double ep_n, double *Cnp1, double *ep1, double *gam,
+                                         double *tau, double *c) {
+  // b_bar trial = J^-2/3 Fs Cn Fs^T (Voigt)
+  double T[9];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const double f0 = Fs[i * 3 + 0], f1 = Fs[i * 3 + 1], f2 = Fs[i * 3 + 2];
+    T[i * 3 + 0] = f0 * Cn[0] + f1 * Cn[3] + f2 * Cn[5];
+    T[i * 3 + 1] = f0 * Cn[3] + f1 * Cn[1] + f2 * Cn[4];
+    T[i * 3 + 2] = f0 * Cn[5] + f1 * Cn[4] + f2 * Cn[2];
+  }
+  const double cj = cbrt(J);
+  const double Jm23 = 1.0 / (cj * cj);
+  double b[6];
+#pragma unroll
+  for (int I = 0; I < 6; ++I) {
+    const int i = voigt_i(I), j = voigt_j(I);
+    b[I] = Jm23 * (T[i * 3 + 0] * Fs[j * 3 + 0] + T[i * 3 + 1] * Fs[j * 3 + 1] + T[i * 3 + 2] * Fs[j * 3 + 2]);
+  }
+  const double G = m.G;
+  const double Ibar = (b[0] + b[1] + b[2]) * (1.0 / 3.0);
+  double s_tr[6];
+#pragma unroll
+  for (int I = 0; I < 6; ++I) s_tr[I] = G * b[I] - (I < 3 ? G * Ibar : 0.0);
+  const double nrm = sqrt(s_tr[0] * s_tr[0] + s_tr[1] * s_tr[1] + s_tr[2] * s_tr[2] +
+                          2.0 * (s_tr[3] * s_tr[3] + s_tr[4] * s_tr[4] + s_tr[5] * s_tr[5]));
+  const double mubar = G * Ibar;
+  double f_tr = -1.0;
+  if (plastic) f_tr = nrm - sqrt(2.0 / 3.0) * (m.k0 + m.hp * ep_n);
+  const bool yield = f_tr > 0.0;
+  double dg = 0.0, n[6] = {0, 0, 0, 0, 0, 0}, s[6];
+  if (yield) {
+    dg = f_tr / (2.0 * mubar + 2.0 / 3.0 * m.hp);
+    const double rn = 1.0 / nrm;
+#pragma unroll
+    for (int I = 0; I < 6; ++I) {
+      n[I] = s_tr[I] * rn;
+      s[I] = s_tr[I] - 2.0 * mubar * dg * n[I];
+    }
+  } else {
+#pragma unroll
+    for (int I = 0; I < 6; ++I) s[I] = s_tr[I];
+  }
+  if (Cnp1) {
+    if (yield) {
+      // Cp^-1(n+1) = J^2/3 Fi (s/G + Ibar 1) Fi^T
+      double be[6];
+#pragma unroll
+      for (int I = 0; I < 6; ++I) be[I] = s[I] / G + (I < 3 ? Ibar : 0.0);
+      double U[9];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        const double f0 = Fi[i * 3 + 0], f1 = Fi[i * 3 + 1], f2 = Fi[i * 3 + 2];
+        U[i * 3 + 0] = f0 * be[0] + f1 * be[3] + f2 * be[5];
+        U[i * 3 + 1] = f0 * be[3] + f1 * be[1] + f2 * be[4];
+        U[i * 3 + 2] = f0 * be[5] + f1 * be[4] + f2 * be[2];
+      }
+      const double Jp23 = cj * cj;
+#pragma unroll
+      for (int I = 0; I < 6; ++I) {
+        const int i = voigt_i(I), j = voigt_j(I);
+        Cnp1[I] = Jp23 * (U[i * 3 + 0] * Fi[j * 3 + 0] + U[i * 3 + 1] * Fi[j * 3 + 1] + U[i * 3 + 2] * Fi[j * 3 + 2]);
+      }
+    } else {
+#pragma unroll
+      for (int I = 0; I < 6; ++I) Cnp1[I] = Cn[I];
+    }
+    *ep1 = ep_n + sqrt(2.0 / 3.0) * dg;
+    *gam = dg;
+  }
+  double dU, d2U;
+  vol_pot(m.volPotFlag, J, dU, d2U);
+  const double Jp = J * m.kappa * dU;
+#pragma unroll
+  for (int I = 0; I < 6; ++I) tau[I] = s[I] + (I < 3 ? Jp : 0.0);
+  if constexpr (WITH_C) {
+    const double cv1 = m.kappa * (J * J * d2U + J * dU), cv2 = -2.0 * m.kappa * J * dU;
+    double beta1 = 0.0, beta3 = 0.0, beta4 = 0.0, dn2[6] = {0, 0, 0, 0, 0, 0};
+    if (yield) {
+      const double beta0 = 1.0 + m.hp / (3.0 * mubar);
+      beta1 = 2.0 * mubar * dg / nrm;
+      const double beta2 = (1.0 - 1.0 / beta0) * (2.0 / 3.0) * (nrm / mubar) * dg;
+      beta3 = 1.0 / beta0 - beta1 + beta2;
+      beta4 = (1.0 / beta0 - beta1) * nrm / mubar;
+      // n^2 (symmetric), then its deviator
+      double n2[6];
+      n2[0] = n[0] * n[0] + n[3] * n[3] + n[5] * n[5];
+      n2[1] = n[3] * n[3] + n[1] * n[1] + n[4] * n[4];
+      n2[2] = n[5] * n[5] + n[4] * n[4] + n[2] * n[2];
+      n2[3] = n[0] * n[3] + n[3] * n[1] + n[5] * n[4];
+      n2[4] = n[3] * n[5] + n[1] * n[4] + n[4] * n[2];
+      n2[5] = n[0] * n[5] + n[3] * n[4] + n[5] * n[2];
+      const double t3 = (n2[0] + n2[1] + n2[2]) * (1.0 / 3.0);
+#pragma unroll
+      for (int I = 0; I < 6; ++I) dn2[I] = n2[I] - (I < 3 ? t3 : 0.0);
+    }
+    const double a1 = 1.0 - beta1, tm = 2.0 * mubar;
+#pragma unroll
+    for (int I = 0; I < 6; ++I)
+#pragma unroll
+      for (int K = 0; K < 6; ++K) {
+        const double oo = (I < 3 && K < 3) ? 1.0 : 0.0;
+        const double Is = (I == K) ? (I < 3 ? 1.0 : 0.5) : 0.0;
+        const double cbar = tm * (Is - oo * (1.0 / 3.0)) -
+                            (2.0 / 3.0) * (s_tr[I] * (K < 3 ? 1.0 : 0.0) + (I < 3 ? 1.0 : 0.0) * s_tr[K]);
+        c[I * 6 + K] = cv1 * oo + cv2 * Is + a1 * cbar - tm * beta3 * n[I] * n[K] - tm * beta4 * n[I] * dn2[K];
+      }
+  }
+}
+
+template <int NNE, int MODE>
+__global__ __launch_bounds__(256) void elem_kernel(ElemArgs a, int64_t e_begin, int64_t e_end) {
+  constexpr int NQP = nqp_of(NNE);
+  constexpr int NDE = NNE * 3;
+  constexpr int TPE = (NNE == 4) ? 16 : 32;
+  constexpr int EPB = 256 / TPE;
+  constexpr bool INTEG = (MODE & MODE_INTEGRATE) != 0;
+  constexpr bool RES = (MODE & MODE_RESIDUAL) != 0;
+  constexpr bool TAN = (MODE & MODE_TANGENT) != 0;
+  constexpr bool GEN = !INTEG;  // Fr may differ from the stored F(n+1)
+
+  __shared__ double s_u[EPB][NDE];
+  __shared__ int32_t s_dof[EPB][NDE];
+  __shared__ int32_t s_node[EPB][NNE];
+  __shared__ double s_x[EPB][NDE];
+  __shared__ double s_h[EPB][NQP][NDE];
+  __shared__ double s_tau[EPB][NQP][6];
+  __shared__ double s_c[EPB][NQP][TAN ? 36 : 1];
+  __shared__ double s_g[EPB][NQP][GEN ? 9 : 1];
+  __shared__ double s_w[EPB][NQP];
+
+  const int slot = threadIdx.x / TPE;
+  const int lane = threadIdx.x % TPE;
+  const int64_t e = e_begin + int64_t(blockIdx.x) * EPB + slot;
+  const bool active = e < e_end;
+
+  // ---- phase 0: gather
+  if (active && lane < NNE) {
+    const int32_t node = a.conn[e * NNE + lane];
+    s_node[slot][lane] = node;
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      s_x[slot][lane * 3 + b] = a.X[int64_t(node) * 3 + b];
+      const int32_t d = a.dofid[int64_t(node) * 3 + b];
+      s_dof[slot][lane * 3 + b] = d;
+      double u = 0.0;  // def_elem_total, src/utils.cc:1391-1412
+      if (d > 0) {
+        u = a.r[d - 1] + (a.d_r ? a.d_r[d - 1] : 0.0);
+      } else if (d < 0) {
+        const int k = -d - 1;
+        u = (a.defl ? a.defl[k] : 0.0) + (a.defl_d ? a.defl_d[k] : 0.0);
+      }
+      s_u[slot][lane * 3 + b] = u;
+    }
+  }
+  __syncthreads();
+
+  // ---- phase 1: quadrature points
+  if (active && lane < NQP) {
+    const int ip = lane;
+    double k, ee, z, wt;
+    int_point(NNE, ip, k, ee, z, wt);
+    double dN[NDE];
+    shape_dN<NNE>(k, ee, z, dN);
+    double dX[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) dX[i] = 0.0;
+#pragma unroll
+    for (int n = 0; n < NNE; ++n)
+#pragma unroll
+      for (int ib = 0; ib < 3; ++ib)
+#pragma unroll
+        for (int ic = 0; ic < 3; ++ic) dX[ib * 3 + ic] += dN[n * 3 + ic] * s_x[slot][n * 3 + ib];
+    const double J0 = det3(dX);
+    bool ok = J0 > 0.0;
+    double cof[9];
+    cof[0] = dX[4] * dX[8] - dX[5] * dX[7];
+    cof[1] = dX[2] * dX[7] - dX[1] * dX[8];
+    cof[2] = dX[1] * dX[5] - dX[2] * dX[4];
+    cof[3] = dX[5] * dX[6] - dX[3] * dX[8];
+    cof[4] = dX[0] * dX[8] - dX[2] * dX[6];
+    cof[5] = dX[2] * dX[3] - dX[0] * dX[5];
+    cof[6] = dX[3] * dX[7] - dX[4] * dX[6];
+    cof[7] = dX[1] * dX[6] - dX[0] * dX[7];
+    cof[8] = dX[0] * dX[4] - dX[1] * dX[3];
+    const double rJ0 = 1.0 / J0;
+    double Fr[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+#pragma unroll
+    for (int n = 0; n < NNE; ++n) {
+      double g[3];
+#pragma unroll
+      for (int ib = 0; ib < 3; ++ib)
+        g[ib] = rJ0 * (dN[n * 3 + 0] * cof[0 * 3 + ib] + dN[n * 3 + 1] * cof[1 * 3 + ib] +
+                       dN[n * 3 + 2] * cof[2 * 3 + ib]);
+#pragma unroll
+      for (int ib = 0; ib < 3; ++ib) {
+        dN[n * 3 + ib] = g[ib];  // now dN/dX
+        const double uu = s_u[slot][n * 3 + ib];
+#pragma unroll
+        for (int ic = 0; ic < 3; ++ic) Fr[ib * 3 + ic] += uu * g[ic];
+      }
+    }
+    const int64_t q = a.qp_base + e * NQP + ip;  // e is the block-local element index
+    const int64_t NQ = a.n_qp;
+    double Fs[9];
+    if (INTEG) {
+#pragma unroll
+      for (int i = 0; i < 9; ++i) {
+        Fs[i] = Fr[i];
+        a.F_np1[i * NQ + q] = Fr[i];
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < 9; ++i) Fs[i] = a.F_np1[i * NQ + q];
+    }
+    const double J = det3(Fs);
+    double Fi[9];
+    ok = ok && (J > 0.0) && inv3(Fs, Fi);
+    if (!ok) {
+      atomicAdd(a.err, 1);
+#pragma unroll
+      for (int i = 0; i < 9; ++i) Fi[i] = (i % 4 == 0) ? 1.0 : 0.0;
+    }
+    const DevMaterial m = a.mats[a.mat[e]];
+    const bool is_j2 = (m.model == PGF_MODEL_J2_PLASTICITY);
+    double Cn[6] = {1, 1, 1, 0, 0, 0}, ep_n = 0.0;
+    if (is_j2) {
+#pragma unroll
+      for (int I = 0; I < 6; ++I) Cn[I] = a.P_n[I * NQ + q];
+      ep_n = a.Sc_n[q];
+    }
+    double tau[6], c[36], Cnp1[6], ep1, gm;
+    j2_point<TAN>(m, is_j2, Fs, Fi, ok ? J : 1.0, Cn, ep_n, (INTEG && is_j2) ? Cnp1 : nullptr, &ep1,
+                  &gm, tau, c);
+    if (INTEG && is_j2) {
+#pragma unroll
+      for (int I = 0; I < 6; ++I) a.P_np1[I * NQ + q] = Cnp1[I];
+      a.Sc_np1[q] = ep1;
+      a.Sc_np1[NQ + q] = gm;
+    }
+    const double wq = ok ? J0 * wt : 0.0;
+    s_w[slot][ip] = wq;
+#pragma unroll
+    for (int I = 0; I < 6; ++I) s_tau[slot][ip][I] = tau[I];
+    if constexpr (TAN) {
+#pragma unroll
+      for (int I = 0; I < 36; ++I) s_c[slot][ip][I] = c[I];
+    }
+    if constexpr (GEN) {  // Gr = Fr Fs^-1
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+          s_g[slot][ip][i * 3 + j] = Fr[i * 3 + 0] * Fi[0 * 3 + j] + Fr[i * 3 + 1] * Fi[1 * 3 + j] + Fr[i * 3 + 2] * Fi[2 * 3 + j];
+    }
+    // h_a = Fs^-T dN_a/dX
+#pragma unroll
+    for (int n = 0; n < NNE; ++n)
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+        s_h[slot][ip][n * 3 + i] = Fi[0 * 3 + i] * dN[n * 3 + 0] + Fi[1 * 3 + i] * dN[n * 3 + 1] + Fi[2 * 3 + i] * dN[n * 3 + 2];
+  }
+  __syncthreads();
+
+  // ---- phase 2: element rows
+  if (!(active && lane < NDE)) return;
+  const int ra = lane / 3, rb = lane % 3;
+  double K[TAN ? NDE : 1];
+  if constexpr (TAN) {
+#pragma unroll
+    for (int i = 0; i < NDE; ++i) K[i] = 0.0;
+  }
+  double fe = 0.0;
+#pragma unroll
+  for (int ip = 0; ip < NQP; ++ip) {
+    const double wq = s_w[slot][ip];
+    const double *h = s_h[slot][ip];
+    const double *tau = s_tau[slot][ip];
+    const double ha[3] = {h[ra * 3 + 0], h[ra * 3 + 1], h[ra * 3 + 2]};
+    double t[3];
+    symv(tau, ha, t);  // tau h_a
+    double g[3] = {rb == 0 ? 1.0 : 0.0, rb == 1 ? 1.0 : 0.0, rb == 2 ? 1.0 : 0.0};
+    if constexpr (GEN) {
+      g[0] = s_g[slot][ip][rb * 3 + 0];
+      g[1] = s_g[slot][ip][rb * 3 + 1];
+      g[2] = s_g[slot][ip][rb * 3 + 2];
+    }
+    if constexpr (RES) fe += wq * (g[0] * t[0] + g[1] * t[1] + g[2] * t[2]);
+    if constexpr (TAN) {
+      const double *c = s_c[slot][ip];
+      // engineering strain vector of sym(g (x) h_a)
+      const double bv[6] = {g[0] * ha[0], g[1] * ha[1], g[2] * ha[2], g[0] * ha[1] + g[1] * ha[0],
+                            g[1] * ha[2] + g[2] * ha[1], g[0] * ha[2] + g[2] * ha[0]};
+      double r[6];
+#pragma unroll
+      for (int J = 0; J < 6; ++J) {
+        double acc = 0.0;
+#pragma unroll
+        for (int I = 0; I < 6; ++I) acc += bv[I] * c[I * 6 + J];
+        r[J] = wq * acc;
+      }
+      // M = Gr R with R = mat(r) (symmetric)
+      double M[9];
+      if constexpr (GEN) {
+        const double *Gm = s_g[slot][ip];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          M[i * 3 + 0] = Gm[i * 3 + 0] * r[0] + Gm[i * 3 + 1] * r[3] + Gm[i * 3 + 2] * r[5];
+          M[i * 3 + 1] = Gm[i * 3 + 0] * r[3] + Gm[i * 3 + 1] * r[1] + Gm[i * 3 + 2] * r[4];
+          M[i * 3 + 2] = Gm[i * 3 + 0] * r[5] + Gm[i * 3 + 1] * r[4] + Gm[i * 3 + 2] * r[2];
+        }
+      } else {
+        M[0] = r[0]; M[1] = r[3]; M[2] = r[5];
+        M[3] = r[3]; M[4] = r[1]; M[5] = r[4];
+        M[6] = r[5]; M[7] = r[4]; M[8] = r[2];
+      }
+      const double wt0 = wq * t[0], wt1 = wq * t[1], wt2 = wq * t[2];
+#pragma unroll
+      for (int w = 0; w < NNE; ++w) {
+        const double h0 = h[w * 3 + 0], h1 = h[w * 3 + 1], h2 = h[w * 3 + 2];
+        K[w * 3 + 0] += M[0] * h0 + M[1] * h1 + M[2] * h2;
+        K[w * 3 + 1] += M[3] * h0 + M[4] * h1 + M[5] * h2;
+        K[w * 3 + 2] += M[6] * h0 + M[7] * h1 + M[8] * h2;
+        K[w * 3 + rb] += wt0 * h0 + wt1 * h1 + wt2 * h2;
+      }
+    }
+  }
+  if constexpr (TAN) {
+    if (!isnormal(K[lane])) atomicAdd(a.err, 1);  // src/constitutive_model.cc:2682-2687
+  }
+  const int32_t dof = s_dof[slot][lane];
+  if (dof <= 0) return;
+  const int16_t *off = a.off + (e * NNE + ra) * NNE;
+  if constexpr (RES) {
+    const bool first = off[ra] <= -2;
+    if (first) a.f[dof - 1] = fe;
+    else a.f[dof - 1] += fe;
+  }
+  if constexpr (TAN) {
+    const int32_t node = s_node[slot][ra];
+    int bp = 0;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) bp += (j < rb && s_dof[slot][ra * 3 + j] > 0) ? 1 : 0;
+    const int64_t rowpos = int64_t(a.row_start[node]) + int64_t(bp) * a.rowlen[node];
+#pragma unroll
+    for (int w = 0; w < NNE; ++w) {
+      const int o = off[w];
+      if (o == -1) continue;
+      const bool first = o <= -2;
+      double *dst = a.vals + rowpos + (first ? (-o - 2) : o);
+      int gp = 0;
+#pragma unroll
+      for (int gg = 0; gg < 3; ++gg) {
+        if (s_dof[slot][w * 3 + gg] > 0) {
+          if (first) dst[gp] = K[w * 3 + gg];
+          else dst[gp] += K[w * 3 + gg];
+          ++gp;
+        }
+      }
+    }
+  }
+}
+
+__global__ void fill_kernel(double *p, int64_t n, double v) {
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+    p[i] = v;
+}
+__global__ void copy_kernel(double *dst, const double *src, int64_t n) {
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+    dst[i] = src[i];
+}
+
+template <int NNE>
+void launch_nne(int mode, const ElemArgs &a, int64_t e_begin, int64_t e_end, hipStream_t s) {
+  constexpr int TPE = (NNE == 4) ? 16 : 32;
+  constexpr int EPB = 256 / TPE;
+  const int64_t n = e_end - e_begin;
+  if (n <= 0) return;
+  const unsigned grid = unsigned((n + EPB - 1) / EPB);
+  switch (mode) {
+    case MODE_INTEGRATE | MODE_RESIDUAL:
+      elem_kernel<NNE, MODE_INTEGRATE | MODE_RESIDUAL><<<grid, 256, 0, s>>>(a, e_begin, e_end);
+      break;
+    case MODE_RESIDUAL:
+      elem_kernel<NNE, MODE_RESIDUAL><<<grid, 256, 0, s>>>(a, e_begin, e_end);
+      break;
+    case MODE_TANGENT:
+      elem_kernel<NNE, MODE_TANGENT><<<grid, 256, 0, s>>>(a, e_begin, e_end);
+      break;
+    case MODE_INTEGRATE | MODE_RESIDUAL | MODE_TANGENT:
+      elem_kernel<NNE, MODE_INTEGRATE | MODE_RESIDUAL | MODE_TANGENT><<<grid, 256, 0, s>>>(a, e_begin, e_end);
+      break;
+    default:
+      break;
+  }
+}
+
+}  // namespace
+
+void launch_elem(int nne, int mode, const ElemArgs &a, int64_t e_begin, int64_t e_end,
+                 hipStream_t s) {
+  if (nne == 4) launch_nne<4>(mode, a, e_begin, e_end, s);
+  else if (nne == 10) launch_nne<10>(mode, a, e_begin, e_end, s);
+  else if (nne == 8) launch_nne<8>(mode, a, e_begin, e_end, s);
+}
+
+void launch_fill(double *p, int64_t n, double v, hipStream_t s) {
+  if (n <= 0) return;
+  const int64_t blocks = std::min<int64_t>((n + 255) / 256, 8192);
+  fill_kernel<<<unsigned(blocks), 256, 0, s>>>(p, n, v);
+}
+
+void launch_copy(double *dst, const double *src, int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  const int64_t blocks = std::min<int64_t>((n + 255) / 256, 8192);
+  copy_kernel<<<unsigned(blocks), 256, 0, s>>>(dst, src, n);
+}
+
+}  // namespace pgf

@@ -1,0 +1,306 @@
+"""ctypes binding of libpgfem3d_amd.so (the C ABI declared in include/pgfem3d_amd.h).
+
+The library is the product: HIP kernels for gfx950 behind a C ABI.  This module only moves
+arrays across the boundary; device vectors are torch tensors on the GPU whose data pointers
+are handed to the library.  There is no fallback: if the library is missing or no GPU is
+present, the calls raise.
+
+Naming follows the reference (C-SWARM/pgfem-3d): ``generate_dof_ids`` mirrors
+generate_local_dof_ids (src/generate_dof_ids.cc:69-171), ``psparse_apai`` mirrors
+Psparse_ApAi (src/Psparse_ApAi.cc:238-670), ``Handle.residual`` mirrors fd_residuals_MP
+(src/fd_residuals.cc:468-636) and ``Handle.tangent`` mirrors stiffmat_fd_MP
+(src/stiffmat_fd.cc:831-993).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "libpgfem3d_amd.so")
+
+PGF_TET4, PGF_TET10, PGF_HEX8 = 4, 10, 8
+MODEL_HYPER_ELASTICITY, MODEL_CRYSTAL_PLASTICITY, MODEL_J2_PLASTICITY = 0, 1, 4
+STATE_F_NP1, STATE_F_N, STATE_F_NM1 = 0, 1, 2
+STATE_PLASTIC_NP1, STATE_PLASTIC_N, STATE_PLASTIC_NM1 = 3, 4, 5
+STATE_SCALAR_NP1, STATE_SCALAR_N, STATE_SCALAR_NM1 = 6, 7, 8
+
+ERRORS = {-1: "PGF_EINVAL", -2: "PGF_ENOMEM", -3: "PGF_EHIP", -4: "PGF_ENOTSUP", -5: "PGF_ENODEV"}
+
+
+class PGFError(RuntimeError):
+    pass
+
+
+class Material(C.Structure):
+    """pgf_material (include/pgfem3d_amd.h)."""
+
+    _fields_ = [
+        ("model", C.c_int32), ("devPotFlag", C.c_int32), ("volPotFlag", C.c_int32), ("pad0", C.c_int32),
+        ("E", C.c_double), ("nu", C.c_double), ("G", C.c_double), ("m10", C.c_double), ("m01", C.c_double),
+        ("k0", C.c_double), ("hp", C.c_double),
+        ("gamma_dot_0", C.c_double), ("m_rate", C.c_double), ("G0", C.c_double), ("g0", C.c_double),
+        ("gs_0", C.c_double), ("gamma_dot_s", C.c_double), ("w_sat", C.c_double),
+        ("C11", C.c_double), ("C12", C.c_double), ("C44", C.c_double),
+        ("tol_M", C.c_double), ("tol_hardening", C.c_double),
+        ("max_itr_M", C.c_int32), ("max_itr_stag", C.c_int32),
+    ]
+
+
+class Mesh(C.Structure):
+    _fields_ = [
+        ("n_nodes", C.c_int64), ("X", C.c_void_p), ("n_elem", C.c_int64), ("elem_ptr", C.c_void_p),
+        ("elem_conn", C.c_void_p), ("elem_mat", C.c_void_p), ("n_coel", C.c_int64),
+        ("coel_ptr", C.c_void_p), ("coel_conn", C.c_void_p),
+    ]
+
+
+class NodalInput(C.Structure):
+    _fields_ = [("r", C.c_void_p), ("d_r", C.c_void_p), ("defl", C.c_void_p), ("defl_d", C.c_void_p)]
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise PGFError(f"{LIB_PATH} is not built (run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        P, I64, I32, D = C.c_void_p, C.c_int64, C.c_int32, C.c_double
+        sig = {
+            "pgf_generate_dof_ids": [P, P, P],
+            "pgf_psparse_apai": [P, P, I64, P, P, P],
+            "pgf_mesh_box": [I32, I32, D, C.c_uint64, P, P, P, P, P, P],
+            "pgf_create": [P, P, I64, P, P, I32, P, P],
+            "pgf_destroy": [P],
+            "pgf_set_stream": [P, P],
+            "pgf_synchronize": [P],
+            "pgf_get_sizes": [P, P, P, P, P],
+            "pgf_residual": [P, P, D, C.c_int, P],
+            "pgf_tangent": [P, P, D, P],
+            "pgf_residual_tangent": [P, P, D, P, P],
+            "pgf_commit_step": [P],
+            "pgf_reset_step": [P],
+            "pgf_state_ncomp": [P, I32, P],
+            "pgf_get_state": [P, I32, P],
+            "pgf_set_state": [P, I32, P],
+        }
+        for name, args in sig.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    """Names the C ABI exports (all functions declared in include/pgfem3d_amd.h)."""
+    return [
+        "pgf_generate_dof_ids", "pgf_psparse_apai", "pgf_mesh_box", "pgf_create", "pgf_destroy",
+        "pgf_set_stream", "pgf_synchronize", "pgf_get_sizes", "pgf_residual", "pgf_tangent",
+        "pgf_residual_tangent", "pgf_commit_step", "pgf_reset_step", "pgf_state_ncomp",
+        "pgf_get_state", "pgf_set_state",
+    ]
+
+
+def _check(rc: int, what: str) -> int:
+    if rc < 0:
+        raise PGFError(f"{what} failed: {ERRORS.get(rc, rc)}")
+    return rc
+
+
+def _ptr(a) -> int:
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        assert a.flags["C_CONTIGUOUS"]
+        return a.ctypes.data
+    return a.data_ptr()  # torch tensor
+
+
+# ----------------------------------------------------------------------------- meshes
+@dataclass
+class MeshArrays:
+    X: np.ndarray          # [n_nodes, 3]
+    elem_ptr: np.ndarray   # [n_elem+1] int64
+    elem_conn: np.ndarray  # int64
+    elem_mat: np.ndarray | None = None
+    coel_ptr: np.ndarray | None = None
+    coel_conn: np.ndarray | None = None
+
+    @property
+    def n_nodes(self) -> int:
+        return self.X.shape[0]
+
+    @property
+    def n_elem(self) -> int:
+        return self.elem_ptr.shape[0] - 1
+
+    def struct(self) -> Mesh:
+        m = Mesh()
+        m.n_nodes = self.n_nodes
+        m.X = _ptr(self.X)
+        m.n_elem = self.n_elem
+        m.elem_ptr = _ptr(self.elem_ptr)
+        m.elem_conn = _ptr(self.elem_conn)
+        m.elem_mat = _ptr(self.elem_mat) if self.elem_mat is not None else None
+        if self.coel_ptr is not None:
+            m.n_coel = self.coel_ptr.shape[0] - 1
+            m.coel_ptr = _ptr(self.coel_ptr)
+            m.coel_conn = _ptr(self.coel_conn)
+        else:
+            m.n_coel = 0
+        return m
+
+
+def mesh_box(n: int, etype: int, jitter: float = 0.1, seed: int = 0) -> MeshArrays:
+    """Unit cube of n^3 cells (Kuhn 6-tet split for TET4/TET10, or HEX8)."""
+    L = lib()
+    nn, ne, cl = C.c_int64(), C.c_int64(), C.c_int64()
+    _check(L.pgf_mesh_box(n, etype, jitter, seed, C.byref(nn), C.byref(ne), C.byref(cl), None, None, None),
+           "pgf_mesh_box")
+    X = np.zeros((nn.value, 3))
+    ep = np.zeros(ne.value + 1, dtype=np.int64)
+    ec = np.zeros(cl.value, dtype=np.int64)
+    _check(L.pgf_mesh_box(n, etype, jitter, seed, C.byref(nn), C.byref(ne), C.byref(cl), _ptr(X), _ptr(ep),
+                          _ptr(ec)), "pgf_mesh_box")
+    return MeshArrays(X, ep, ec)
+
+
+def generate_dof_ids(mesh: MeshArrays, supports: np.ndarray):
+    """Support codes [n_nodes,3] (0 free, 1 fixed, -k prescribed) -> (DOF ids, ndof)."""
+    ids = np.ascontiguousarray(supports, dtype=np.int64).copy()
+    nd = C.c_int64()
+    m = mesh.struct()
+    _check(lib().pgf_generate_dof_ids(C.byref(m), _ptr(ids), C.byref(nd)), "pgf_generate_dof_ids")
+    return ids, nd.value
+
+
+def psparse_apai(mesh: MeshArrays, ids: np.ndarray, ndof: int):
+    Ap = np.zeros(ndof + 1, dtype=np.int32)
+    nnz = C.c_int64()
+    m = mesh.struct()
+    _check(lib().pgf_psparse_apai(C.byref(m), _ptr(ids), ndof, _ptr(Ap), None, C.byref(nnz)), "pgf_psparse_apai")
+    Ai = np.zeros(nnz.value, dtype=np.int32)
+    _check(lib().pgf_psparse_apai(C.byref(m), _ptr(ids), ndof, _ptr(Ap), _ptr(Ai), C.byref(nnz)),
+           "pgf_psparse_apai")
+    return Ap, Ai
+
+
+def box_supports(mesh: MeshArrays, tol: float = 1e-12) -> np.ndarray:
+    """Bottom face z=0 fixed in x, y, z; top face z=1 has u_z prescribed by value 1 (code -1)."""
+    sup = np.zeros((mesh.n_nodes, 3), dtype=np.int64)
+    z = mesh.X[:, 2]
+    sup[z < tol] = 1
+    sup[z > 1.0 - tol, 2] = -1
+    return sup
+
+
+# ------------------------------------------------------------------------- materials
+def neo_hookean(E: float, nu: float, vol_pot: int = 2) -> Material:
+    """Compressible neo-Hookean: Mooney-Rivlin with m01 = 0, m10 = G/2 (src/homogen.cc:184-186)."""
+    G = E / (2.0 * (1.0 + nu))
+    return Material(model=MODEL_HYPER_ELASTICITY, devPotFlag=1, volPotFlag=vol_pot, E=E, nu=nu, G=G,
+                    m10=0.5 * G, m01=0.0)
+
+
+def j2_linear(E: float, nu: float, sigma_y: float, H: float, vol_pot: int = 2) -> Material:
+    G = E / (2.0 * (1.0 + nu))
+    return Material(model=MODEL_J2_PLASTICITY, devPotFlag=1, volPotFlag=vol_pot, E=E, nu=nu, G=G,
+                    m10=0.5 * G, m01=0.0, k0=sigma_y, hp=H)
+
+
+# ---------------------------------------------------------------------------- handle
+class Handle:
+    """Device-side element hot path for one mesh (pgf_create ... pgf_destroy)."""
+
+    def __init__(self, mesh: MeshArrays, ids: np.ndarray, ndof: int, Ap: np.ndarray, Ai: np.ndarray,
+                 materials):
+        self._lib = lib()
+        mats = (Material * len(materials))(*materials)
+        self._mats = mats
+        h = C.c_void_p()
+        m = mesh.struct()
+        _check(self._lib.pgf_create(C.byref(m), _ptr(ids), ndof, _ptr(Ap), _ptr(Ai), len(materials), mats,
+                                    C.byref(h)), "pgf_create")
+        self.h = h
+        nq, nz, nd, nc = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int32()
+        _check(self._lib.pgf_get_sizes(h, C.byref(nq), C.byref(nz), C.byref(nd), C.byref(nc)), "pgf_get_sizes")
+        self.n_qp, self.nnz, self.ndof, self.n_colors = nq.value, nz.value, nd.value, nc.value
+        self.n_prescribed = int(max(0, -int(np.min(ids)))) if ids.size else 0
+
+    def close(self):
+        if self.h:
+            self._lib.pgf_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def _check_vec(t, n, what, exact=True):
+        if t is None:
+            return
+        import torch
+
+        if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()):
+            raise PGFError(f"{what} must be a contiguous float64 GPU tensor")
+        if (t.numel() != n) if exact else (t.numel() < n):
+            raise PGFError(f"{what} has {t.numel()} entries, expected {n}")
+
+    def _input(self, r, d_r=None, defl=None, defl_d=None) -> NodalInput:
+        if r is None:
+            raise PGFError("r is required")
+        self._check_vec(r, self.ndof, "r")
+        self._check_vec(d_r, self.ndof, "d_r")
+        self._check_vec(defl, self.n_prescribed, "defl", exact=False)
+        self._check_vec(defl_d, self.n_prescribed, "defl_d", exact=False)
+        if self.n_prescribed and defl is None and defl_d is None:
+            pass  # prescribed values default to zero
+        return NodalInput(_ptr(r), _ptr(d_r), _ptr(defl), _ptr(defl_d))
+
+    def set_stream(self, stream_ptr: int | None):
+        _check(self._lib.pgf_set_stream(self.h, stream_ptr), "pgf_set_stream")
+
+    def residual(self, f, r, d_r=None, defl=None, defl_d=None, dt=1.0, run_integration=True) -> int:
+        inp = self._input(r, d_r, defl, defl_d)
+        self._check_vec(f, self.ndof, "f")
+        return _check(self._lib.pgf_residual(self.h, C.byref(inp), dt, int(run_integration), _ptr(f)),
+                      "pgf_residual")
+
+    def tangent(self, vals, r, d_r=None, defl=None, defl_d=None, dt=1.0) -> int:
+        inp = self._input(r, d_r, defl, defl_d)
+        self._check_vec(vals, self.nnz, "vals")
+        return _check(self._lib.pgf_tangent(self.h, C.byref(inp), dt, _ptr(vals)), "pgf_tangent")
+
+    def residual_tangent(self, f, vals, r, d_r=None, defl=None, defl_d=None, dt=1.0) -> int:
+        inp = self._input(r, d_r, defl, defl_d)
+        self._check_vec(f, self.ndof, "f")
+        self._check_vec(vals, self.nnz, "vals")
+        return _check(self._lib.pgf_residual_tangent(self.h, C.byref(inp), dt, _ptr(f), _ptr(vals)),
+                      "pgf_residual_tangent")
+
+    def commit_step(self):
+        _check(self._lib.pgf_commit_step(self.h), "pgf_commit_step")
+
+    def reset_step(self):
+        _check(self._lib.pgf_reset_step(self.h), "pgf_reset_step")
+
+    def get_state(self, field: int) -> np.ndarray:
+        nc = C.c_int32()
+        _check(self._lib.pgf_state_ncomp(self.h, field, C.byref(nc)), "pgf_state_ncomp")
+        out = np.zeros((nc.value, self.n_qp))
+        _check(self._lib.pgf_get_state(self.h, field, _ptr(out)), "pgf_get_state")
+        return out
+
+    def set_state(self, field: int, values: np.ndarray):
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        _check(self._lib.pgf_set_state(self.h, field, _ptr(v)), "pgf_set_state")

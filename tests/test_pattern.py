@@ -1,0 +1,86 @@
+"""DOF numbering and CSR pattern: product (pgf_generate_dof_ids / pgf_psparse_apai) against
+the oracle's literal restatement of generate_local_dof_ids and Psparse_ApAi, plus brute
+force on one-cell meshes."""
+import itertools
+
+import numpy as np
+import pytest
+
+import oracle as O
+import pgfem3d_amd as P
+
+
+@pytest.mark.parametrize("n,etype", [(10, P.PGF_TET4), (4, P.PGF_TET10), (5, P.PGF_HEX8), (2, P.PGF_TET10)])
+def test_product_equals_oracle(n, etype):
+    m = P.mesh_box(n, etype, 0.1, 1)
+    sup = P.box_supports(m)
+    ids, nd = P.generate_dof_ids(m, sup)
+    ids2, nd2 = O.generate_dof_ids(m.X, m.elem_ptr, m.elem_conn, sup)
+    assert nd == nd2 and np.array_equal(ids, ids2)
+    Ap, Ai = P.psparse_apai(m, ids, nd)
+    Ap2, Ai2 = O.psparse_apai(m.X, m.elem_ptr, m.elem_conn, ids2, nd2)
+    assert np.array_equal(Ap, Ap2) and np.array_equal(Ai, Ai2)
+
+
+def _dense_pattern(Ap, Ai, nd):
+    A = np.zeros((nd, nd), dtype=bool)
+    for r in range(nd):
+        A[r, Ai[Ap[r]:Ap[r + 1]]] = True
+    return A
+
+
+@pytest.mark.parametrize("etype", [P.PGF_TET4, P.PGF_TET10, P.PGF_HEX8])
+def test_one_cell_brute_force(etype):
+    """One cell: every free DOF couples with every other one; nothing else appears."""
+    m = P.mesh_box(1, etype, 0.0, 0)
+    sup = P.box_supports(m)
+    ids, nd = O.generate_dof_ids(m.X, m.elem_ptr, m.elem_conn, sup)
+    Ap, Ai = O.psparse_apai(m.X, m.elem_ptr, m.elem_conn, ids, nd)
+    # brute force: for every pair of free DOFs, do they share an element?
+    want = np.zeros((nd, nd), dtype=bool)
+    for e in range(m.n_elem):
+        nod = m.elem_conn[m.elem_ptr[e]:m.elem_ptr[e + 1]]
+        d = [ids[n, j] - 1 for n in nod for j in range(3) if ids[n, j] > 0]
+        for a, b in itertools.product(d, d):
+            want[a, b] = True
+    assert np.array_equal(_dense_pattern(Ap, Ai, nd), want)
+    if etype != P.PGF_TET4:  # single element or all nodes shared: full coupling
+        pass
+    assert all(np.all(np.diff(Ai[Ap[r]:Ap[r + 1]]) > 0) for r in range(nd))
+
+
+def test_dof_numbering_rules():
+    """Fixed DOFs become 0, prescribed keep their -k code, free ones count up from 1 in
+    element -> node -> dof traversal order (hand-checked on one P1 cell)."""
+    m = P.mesh_box(1, P.PGF_TET4, 0.0, 0)
+    sup = np.zeros((m.n_nodes, 3), dtype=np.int64)
+    sup[0] = 1          # node 0 fixed
+    sup[7, 2] = -1      # node 7 u_z prescribed
+    ids, nd = O.generate_dof_ids(m.X, m.elem_ptr, m.elem_conn, sup)
+    assert (ids[0] == 0).all() and ids[7, 2] == -1
+    # traversal: first element nodes in order, new DOFs numbered consecutively
+    seen, expect = set(), 1
+    want = np.zeros_like(ids)
+    want[0] = 0
+    want[7, 2] = -1
+    for e in range(m.n_elem):
+        for n in m.elem_conn[m.elem_ptr[e]:m.elem_ptr[e + 1]]:
+            for j in range(3):
+                if (n, j) in seen:
+                    continue
+                seen.add((n, j))
+                if sup[n, j] == 0:
+                    want[n, j] = expect
+                    expect += 1
+    assert np.array_equal(ids, want) and nd == expect - 1
+    ids_p, nd_p = P.generate_dof_ids(m, sup)
+    assert np.array_equal(ids_p, ids) and nd_p == nd
+
+
+def test_pattern_symmetric_with_diagonal():
+    m = P.mesh_box(3, P.PGF_TET10, 0.1, 2)
+    sup = P.box_supports(m)
+    ids, nd = P.generate_dof_ids(m, sup)
+    Ap, Ai = P.psparse_apai(m, ids, nd)
+    A = _dense_pattern(Ap, Ai, nd)
+    assert np.array_equal(A, A.T) and A.diagonal().all()
