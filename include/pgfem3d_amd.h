@@ -131,6 +131,13 @@ int pgf_destroy(pgf_handle h);
 int pgf_set_stream(pgf_handle h, void *stream);
 int pgf_synchronize(pgf_handle h);
 
+/* Pass timing (no reference counterpart; a measurement aid): when enabled, every pass of
+ * pgf_residual / pgf_tangent / pgf_residual_tangent records HIP events on the handle's
+ * stream around its element launches; pgf_last_pass_ms returns the elapsed GPU time of the
+ * last pass and the number of element-kernel launches it made. */
+int pgf_set_profiling(pgf_handle h, int on);
+int pgf_last_pass_ms(pgf_handle h, double *ms, int32_t *n_launches);
+
 /* Sizes: n_qp = total quadrature points, nnz = CSR entries, ndof, n_colors. */
 int pgf_get_sizes(pgf_handle h, int64_t *n_qp, int64_t *nnz, int64_t *ndof,
                   int32_t *n_colors);

@@ -61,6 +61,10 @@ struct pgf_handle_s {
   double *d_P[3] = {nullptr, nullptr, nullptr};   // J2: Cp^-1 (6)
   double *d_Sc[3] = {nullptr, nullptr, nullptr};  // J2: ep, gam
   int32_t *d_err = nullptr;
+  bool profiling = false;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double last_ms = 0.0;
+  int32_t last_launches = 0;
 };
 
 namespace {
@@ -109,6 +113,8 @@ int run_pass(pgf_handle h, const pgf_nodal_input *in, int mode, double *f, doubl
   pgf::ElemArgs a = base_args(h, in);
   a.f = f;
   a.vals = vals;
+  if (h->profiling) hipEventRecord(h->ev0, h->stream);
+  int32_t launches = 0;
   for (int32_t c = 0; c < h->n_colors; ++c)
     for (auto &b : h->blocks) {
       const int64_t e0 = b.color_ptr[c], e1 = b.color_ptr[c + 1];
@@ -118,12 +124,20 @@ int run_pass(pgf_handle h, const pgf_nodal_input *in, int mode, double *f, doubl
       a.mat = b.d_mat;
       a.qp_base = b.qp_base;
       pgf::launch_elem(b.nne, mode, a, e0, e1, h->stream);
+      ++launches;
     }
+  if (h->profiling) hipEventRecord(h->ev1, h->stream);
+  h->last_launches = launches;
   if (hipGetLastError() != hipSuccess) return PGF_EHIP;
   int32_t err = 0;
   if (hipMemcpyAsync(&err, h->d_err, sizeof(int32_t), hipMemcpyDeviceToHost, h->stream) != hipSuccess)
     return PGF_EHIP;
   if (hipStreamSynchronize(h->stream) != hipSuccess) return PGF_EHIP;
+  if (h->profiling) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, h->ev0, h->ev1) != hipSuccess) return PGF_EHIP;
+    h->last_ms = ms;
+  }
   return err;
 }
 
@@ -345,8 +359,26 @@ int pgf_destroy(pgf_handle h) {
     hipFree(h->d_Sc[t]);
   }
   hipFree(h->d_err);
+  if (h->ev0) hipEventDestroy(h->ev0);
+  if (h->ev1) hipEventDestroy(h->ev1);
   if (h->own_stream) hipStreamDestroy(h->own_stream);
   delete h;
+  return PGF_OK;
+}
+
+int pgf_set_profiling(pgf_handle h, int on) {
+  if (!h) return PGF_EINVAL;
+  if (on && !h->ev0) {
+    if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess) return PGF_EHIP;
+  }
+  h->profiling = on != 0;
+  return PGF_OK;
+}
+
+int pgf_last_pass_ms(pgf_handle h, double *ms, int32_t *n_launches) {
+  if (!h || !ms) return PGF_EINVAL;
+  *ms = h->last_ms;
+  if (n_launches) *n_launches = h->last_launches;
   return PGF_OK;
 }
 

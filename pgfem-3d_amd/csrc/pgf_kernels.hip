@@ -301,17 +301,27 @@ __global__ __launch_bounds__(256) void elem_kernel(ElemArgs a, int64_t e_begin, 
     const int ip = lane;
     double k, ee, z, wt;
     int_point(NNE, ip, k, ee, z, wt);
-    double dN[NDE];
-    shape_dN<NNE>(k, ee, z, dN);
+    double *hq = s_h[slot][ip];  // dN/dX first, h_a = Fs^-T dN_a/dX at the end
+    {
+      double dN[NDE];
+      shape_dN<NNE>(k, ee, z, dN);
+#pragma unroll
+      for (int i = 0; i < NDE; ++i) hq[i] = dN[i];
+    }
     double dX[9];
 #pragma unroll
     for (int i = 0; i < 9; ++i) dX[i] = 0.0;
 #pragma unroll
-    for (int n = 0; n < NNE; ++n)
+    for (int n = 0; n < NNE; ++n) {
+      const double d0 = hq[n * 3 + 0], d1 = hq[n * 3 + 1], d2 = hq[n * 3 + 2];
 #pragma unroll
-      for (int ib = 0; ib < 3; ++ib)
-#pragma unroll
-        for (int ic = 0; ic < 3; ++ic) dX[ib * 3 + ic] += dN[n * 3 + ic] * s_x[slot][n * 3 + ib];
+      for (int ib = 0; ib < 3; ++ib) {
+        const double xb = s_x[slot][n * 3 + ib];
+        dX[ib * 3 + 0] += d0 * xb;
+        dX[ib * 3 + 1] += d1 * xb;
+        dX[ib * 3 + 2] += d2 * xb;
+      }
+    }
     const double J0 = det3(dX);
     bool ok = J0 > 0.0;
     double cof[9];
@@ -328,14 +338,13 @@ __global__ __launch_bounds__(256) void elem_kernel(ElemArgs a, int64_t e_begin, 
     double Fr[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
 #pragma unroll
     for (int n = 0; n < NNE; ++n) {
+      const double d0 = hq[n * 3 + 0], d1 = hq[n * 3 + 1], d2 = hq[n * 3 + 2];
       double g[3];
 #pragma unroll
-      for (int ib = 0; ib < 3; ++ib)
-        g[ib] = rJ0 * (dN[n * 3 + 0] * cof[0 * 3 + ib] + dN[n * 3 + 1] * cof[1 * 3 + ib] +
-                       dN[n * 3 + 2] * cof[2 * 3 + ib]);
+      for (int ib = 0; ib < 3; ++ib) g[ib] = rJ0 * (d0 * cof[0 * 3 + ib] + d1 * cof[1 * 3 + ib] + d2 * cof[2 * 3 + ib]);
 #pragma unroll
       for (int ib = 0; ib < 3; ++ib) {
-        dN[n * 3 + ib] = g[ib];  // now dN/dX
+        hq[n * 3 + ib] = g[ib];  // dN/dX
         const double uu = s_u[slot][n * 3 + ib];
 #pragma unroll
         for (int ic = 0; ic < 3; ++ic) Fr[ib * 3 + ic] += uu * g[ic];
@@ -370,23 +379,16 @@ __global__ __launch_bounds__(256) void elem_kernel(ElemArgs a, int64_t e_begin, 
       for (int I = 0; I < 6; ++I) Cn[I] = a.P_n[I * NQ + q];
       ep_n = a.Sc_n[q];
     }
-    double tau[6], c[36], Cnp1[6], ep1, gm;
+    double Cnp1[6], ep1, gm;
     j2_point<TAN>(m, is_j2, Fs, Fi, ok ? J : 1.0, Cn, ep_n, (INTEG && is_j2) ? Cnp1 : nullptr, &ep1,
-                  &gm, tau, c);
+                  &gm, s_tau[slot][ip], TAN ? s_c[slot][ip] : nullptr);
     if (INTEG && is_j2) {
 #pragma unroll
       for (int I = 0; I < 6; ++I) a.P_np1[I * NQ + q] = Cnp1[I];
       a.Sc_np1[q] = ep1;
       a.Sc_np1[NQ + q] = gm;
     }
-    const double wq = ok ? J0 * wt : 0.0;
-    s_w[slot][ip] = wq;
-#pragma unroll
-    for (int I = 0; I < 6; ++I) s_tau[slot][ip][I] = tau[I];
-    if constexpr (TAN) {
-#pragma unroll
-      for (int I = 0; I < 36; ++I) s_c[slot][ip][I] = c[I];
-    }
+    s_w[slot][ip] = ok ? J0 * wt : 0.0;
     if constexpr (GEN) {  // Gr = Fr Fs^-1
 #pragma unroll
       for (int i = 0; i < 3; ++i)
@@ -394,18 +396,19 @@ __global__ __launch_bounds__(256) void elem_kernel(ElemArgs a, int64_t e_begin, 
         for (int j = 0; j < 3; ++j)
           s_g[slot][ip][i * 3 + j] = Fr[i * 3 + 0] * Fi[0 * 3 + j] + Fr[i * 3 + 1] * Fi[1 * 3 + j] + Fr[i * 3 + 2] * Fi[2 * 3 + j];
     }
-    // h_a = Fs^-T dN_a/dX
 #pragma unroll
-    for (int n = 0; n < NNE; ++n)
+    for (int n = 0; n < NNE; ++n) {
+      const double d0 = hq[n * 3 + 0], d1 = hq[n * 3 + 1], d2 = hq[n * 3 + 2];
 #pragma unroll
-      for (int i = 0; i < 3; ++i)
-        s_h[slot][ip][n * 3 + i] = Fi[0 * 3 + i] * dN[n * 3 + 0] + Fi[1 * 3 + i] * dN[n * 3 + 1] + Fi[2 * 3 + i] * dN[n * 3 + 2];
+      for (int i = 0; i < 3; ++i) hq[n * 3 + i] = Fi[0 * 3 + i] * d0 + Fi[1 * 3 + i] * d1 + Fi[2 * 3 + i] * d2;
+    }
   }
   __syncthreads();
 
   // ---- phase 2: element rows
   if (!(active && lane < NDE)) return;
   const int ra = lane / 3, rb = lane % 3;
+  const double eb0 = rb == 0 ? 1.0 : 0.0, eb1 = rb == 1 ? 1.0 : 0.0, eb2 = rb == 2 ? 1.0 : 0.0;
   double K[TAN ? NDE : 1];
   if constexpr (TAN) {
 #pragma unroll
@@ -420,7 +423,7 @@ __global__ __launch_bounds__(256) void elem_kernel(ElemArgs a, int64_t e_begin, 
     const double ha[3] = {h[ra * 3 + 0], h[ra * 3 + 1], h[ra * 3 + 2]};
     double t[3];
     symv(tau, ha, t);  // tau h_a
-    double g[3] = {rb == 0 ? 1.0 : 0.0, rb == 1 ? 1.0 : 0.0, rb == 2 ? 1.0 : 0.0};
+    double g[3] = {eb0, eb1, eb2};
     if constexpr (GEN) {
       g[0] = s_g[slot][ip][rb * 3 + 0];
       g[1] = s_g[slot][ip][rb * 3 + 1];
@@ -459,15 +462,18 @@ __global__ __launch_bounds__(256) void elem_kernel(ElemArgs a, int64_t e_begin, 
 #pragma unroll
       for (int w = 0; w < NNE; ++w) {
         const double h0 = h[w * 3 + 0], h1 = h[w * 3 + 1], h2 = h[w * 3 + 2];
-        K[w * 3 + 0] += M[0] * h0 + M[1] * h1 + M[2] * h2;
-        K[w * 3 + 1] += M[3] * h0 + M[4] * h1 + M[5] * h2;
-        K[w * 3 + 2] += M[6] * h0 + M[7] * h1 + M[8] * h2;
-        K[w * 3 + rb] += wt0 * h0 + wt1 * h1 + wt2 * h2;
+        const double geo = wt0 * h0 + wt1 * h1 + wt2 * h2;  // delta_bg h_a.tau.h_w
+        K[w * 3 + 0] += M[0] * h0 + M[1] * h1 + M[2] * h2 + eb0 * geo;
+        K[w * 3 + 1] += M[3] * h0 + M[4] * h1 + M[5] * h2 + eb1 * geo;
+        K[w * 3 + 2] += M[6] * h0 + M[7] * h1 + M[8] * h2 + eb2 * geo;
       }
     }
   }
-  if constexpr (TAN) {
-    if (!isnormal(K[lane])) atomicAdd(a.err, 1);  // src/constitutive_model.cc:2682-2687
+  if constexpr (TAN) {  // non-normal diagonal, src/constitutive_model.cc:2682-2687
+    double diag = 0.0;
+#pragma unroll
+    for (int i = 0; i < NDE; ++i) diag = (i == lane) ? K[i] : diag;
+    if (!isnormal(diag)) atomicAdd(a.err, 1);
   }
   const int32_t dof = s_dof[slot][lane];
   if (dof <= 0) return;
@@ -482,22 +488,36 @@ __global__ __launch_bounds__(256) void elem_kernel(ElemArgs a, int64_t e_begin, 
     int bp = 0;
 #pragma unroll
     for (int j = 0; j < 3; ++j) bp += (j < rb && s_dof[slot][ra * 3 + j] > 0) ? 1 : 0;
-    const int64_t rowpos = int64_t(a.row_start[node]) + int64_t(bp) * a.rowlen[node];
+    double *row = a.vals + (int64_t(a.row_start[node]) + int64_t(bp) * a.rowlen[node]);
+    // Read-modify-write in two halves: every load of a half is issued before its stores
+    // (the addresses of one row are distinct), so the loads overlap in flight.
+    constexpr int H1 = (NNE + 1) / 2;
 #pragma unroll
-    for (int w = 0; w < NNE; ++w) {
-      const int o = off[w];
-      if (o == -1) continue;
-      const bool first = o <= -2;
-      double *dst = a.vals + rowpos + (first ? (-o - 2) : o);
-      int gp = 0;
+    for (int half = 0; half < 2; ++half) {
+      const int w0 = half ? H1 : 0, w1 = half ? NNE : H1;
+      int pos[3 * H1];
+      double old[3 * H1];
 #pragma unroll
-      for (int gg = 0; gg < 3; ++gg) {
-        if (s_dof[slot][w * 3 + gg] > 0) {
-          if (first) dst[gp] = K[w * 3 + gg];
-          else dst[gp] += K[w * 3 + gg];
-          ++gp;
+      for (int w = w0; w < w1; ++w) {
+        const int o = off[w];
+        const bool first = o <= -2;
+        const int base = first ? (-o - 2) : o;
+        int gp = 0;
+#pragma unroll
+        for (int gg = 0; gg < 3; ++gg) {
+          const bool fr = (o != -1) && s_dof[slot][w * 3 + gg] > 0;
+          pos[(w - w0) * 3 + gg] = fr ? base + gp : -1;
+          gp += fr ? 1 : 0;
+          old[(w - w0) * 3 + gg] = (fr && !first) ? row[base + gp - 1] : 0.0;
         }
       }
+#pragma unroll
+      for (int w = w0; w < w1; ++w)
+#pragma unroll
+        for (int gg = 0; gg < 3; ++gg) {
+          const int p = pos[(w - w0) * 3 + gg];
+          if (p >= 0) row[p] = old[(w - w0) * 3 + gg] + K[w * 3 + gg];
+        }
     }
   }
 }

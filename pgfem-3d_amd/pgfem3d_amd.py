@@ -79,6 +79,8 @@ def lib() -> C.CDLL:
             "pgf_create": [P, P, I64, P, P, I32, P, P],
             "pgf_destroy": [P],
             "pgf_set_stream": [P, P],
+            "pgf_set_profiling": [P, C.c_int],
+            "pgf_last_pass_ms": [P, P, P],
             "pgf_synchronize": [P],
             "pgf_get_sizes": [P, P, P, P, P],
             "pgf_residual": [P, P, D, C.c_int, P],
@@ -102,7 +104,7 @@ def exported_symbols():
     """Names the C ABI exports (all functions declared in include/pgfem3d_amd.h)."""
     return [
         "pgf_generate_dof_ids", "pgf_psparse_apai", "pgf_mesh_box", "pgf_create", "pgf_destroy",
-        "pgf_set_stream", "pgf_synchronize", "pgf_get_sizes", "pgf_residual", "pgf_tangent",
+        "pgf_set_stream", "pgf_set_profiling", "pgf_last_pass_ms", "pgf_synchronize", "pgf_get_sizes", "pgf_residual", "pgf_tangent",
         "pgf_residual_tangent", "pgf_commit_step", "pgf_reset_step", "pgf_state_ncomp",
         "pgf_get_state", "pgf_set_state",
     ]
@@ -266,6 +268,14 @@ class Handle:
         if self.n_prescribed and defl is None and defl_d is None:
             pass  # prescribed values default to zero
         return NodalInput(_ptr(r), _ptr(d_r), _ptr(defl), _ptr(defl_d))
+
+    def set_profiling(self, on: bool):
+        _check(self._lib.pgf_set_profiling(self.h, int(on)), "pgf_set_profiling")
+
+    def last_pass_ms(self):
+        ms, nl = C.c_double(), C.c_int32()
+        _check(self._lib.pgf_last_pass_ms(self.h, C.byref(ms), C.byref(nl)), "pgf_last_pass_ms")
+        return ms.value, nl.value
 
     def set_stream(self, stream_ptr: int | None):
         _check(self._lib.pgf_set_stream(self.h, stream_ptr), "pgf_set_stream")
