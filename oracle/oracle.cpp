@@ -536,6 +536,13 @@ void compute_stiffness_matrix(double *lk, int nne, const double *ST, double detJ
 
 extern "C" {
 
+void orc_stiffness_contract(double *lk, int nne, const double *ST, double detJxW, const double *Fr,
+                            const double *eFnMT, const double *eFn, const double *M, const double *FrTFr,
+                            const double *eFnM, const double *S, const double *L, const double *dMdu_all,
+                            double one_over_Jn) {
+  compute_stiffness_matrix(lk, nne, ST, detJxW, Fr, eFnMT, eFn, M, FrTFr, eFnM, S, L, dMdu_all, one_over_Jn);
+}
+
 int orc_nint(int nne) { return nint_of(nne); }
 
 int orc_basis(int nne, const double *X, int ip, double *N, double *dNdX, double *detJxW) {

@@ -39,6 +39,21 @@ int orc_he_update(const orc_material *mat, const double *F, double *S, double *L
 int orc_j2_update(const orc_material *mat, const double *F, const double *Cpinv_n, double ep_n,
                   double *Cpinv_np1, double *ep_np1, double *gam, double *S, double *L);
 
+// --- element contraction of the reference (compute_stiffness_matrix,
+// src/constitutive_model.cc:1520-1616)
+void orc_stiffness_contract(double *lk, int nne, const double *ST, double detJxW, const double *Fr,
+                            const double *eFnMT, const double *eFn, const double *M, const double *FrTFr,
+                            const double *eFnM, const double *S, const double *L, const double *dMdu_all,
+                            double one_over_Jn);
+
+// --- crystal plasticity (oracle_cp.cpp)
+void orc_cp_slip_systems(double *P0);
+void orc_cp_grain(const orc_material *mat, const double *R, double *P_out, double *L_out);
+int orc_cp_update(const orc_material *mat, const double *P_grain, const double *L_grain, const double *F,
+                  const double *pFn, double g_n, double lam_n, double dt, const double *x0, int solve,
+                  double *pFnp1, double *g_np1, double *lam_np1, double *S, double *tau, double *gdot,
+                  double *dMdF, int *iters);
+
 // --- whole mesh: fd_residuals_MP / stiffmat_fd_MP restated for one rank (TL, CM path)
 // state arrays are [comp][n_qp]; plastic = Cpinv (6 comps) for J2, scalar = (ep, gam).
 int orc_assemble(int64_t n_nodes, const double *X, int64_t n_elem, const int64_t *elem_ptr,

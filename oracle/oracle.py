@@ -163,3 +163,99 @@ def assemble(mesh, mats, ids, ndof, Ap, Ai, state: OracleState, r, d_r=None, def
                          int(integrate), int(tangent), _p(state.F_np1), _p(state.P_n), _p(state.P_np1),
                          _p(state.S_n), _p(state.S_np1), _p(f), _p(vals))
     return f, vals, err
+
+
+# ----------------------------------------------------------------------------- crystal plasticity
+def _cp_lib():
+    L = lib()
+    if not getattr(L, "_cp_ready", False):
+        L.orc_cp_slip_systems.argtypes = [C.c_void_p]
+        L.orc_cp_slip_systems.restype = None
+        L.orc_cp_grain.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_cp_grain.restype = None
+        L.orc_cp_update.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
+                                    C.c_double, C.c_double, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
+                                    C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_cp_update.restype = C.c_int
+        L.orc_assemble_cp.argtypes = [C.c_int64, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
+                                      C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
+                                      C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                      C.c_double, C.c_int, C.c_int, C.c_int] + [C.c_void_p] * 14
+        L.orc_assemble_cp.restype = C.c_int
+        L._cp_ready = True
+    return L
+
+
+def cp_slip_systems():
+    P0 = np.zeros((12, 3, 3))
+    _cp_lib().orc_cp_slip_systems(_p(P0))
+    return P0
+
+
+def cp_grain(mat, R):
+    P = np.zeros((12, 3, 3))
+    Lg = np.zeros((3, 3, 3, 3))
+    om = material(mat)
+    _cp_lib().orc_cp_grain(C.byref(om), _p(np.ascontiguousarray(R, dtype=np.float64)), _p(P), _p(Lg))
+    return P, Lg
+
+
+def cp_update(mat, P, Lg, F, pFn, g_n, lam_n, dt, x0=None, solve=True, tangent=True):
+    om = material(mat)
+    pF1 = np.zeros(9)
+    S = np.zeros(9)
+    tau = np.zeros(12)
+    gdot = np.zeros(12)
+    dMdF = np.zeros(81) if tangent else None
+    g1, l1, its = C.c_double(), C.c_double(), C.c_int()
+    x0a = None if x0 is None else np.ascontiguousarray(x0, dtype=np.float64)
+    rc = _cp_lib().orc_cp_update(C.byref(om), _p(np.ascontiguousarray(P)), _p(np.ascontiguousarray(Lg)),
+                                 _p(np.ascontiguousarray(F, dtype=np.float64).reshape(9)),
+                                 _p(np.ascontiguousarray(pFn, dtype=np.float64).reshape(9)), g_n, lam_n, dt, _p(x0a),
+                                 int(solve), _p(pF1), C.byref(g1), C.byref(l1), _p(S), _p(tau), _p(gdot), _p(dMdF),
+                                 C.byref(its))
+    return dict(rc=rc, pF=pF1.reshape(3, 3), g=g1.value, lam=l1.value, S=S.reshape(3, 3), tau=tau, gdot=gdot,
+                dMdF=None if dMdF is None else dMdF.reshape(9, 9), iters=its.value)
+
+
+class CPState:
+    """Crystal-plasticity state [comp][n_qp] in mesh QP order (oracle layout)."""
+
+    def __init__(self, n_qp, g0):
+        eye = np.array([1.0, 0, 0, 0, 1, 0, 0, 0, 1])
+        self.n_qp = n_qp
+        self.F_np1 = np.ascontiguousarray(np.repeat(eye[:, None], n_qp, axis=1))
+        self.pF_n = self.F_np1.copy()
+        self.pF_np1 = self.F_np1.copy()
+        self.g_n = np.full(n_qp, float(g0))
+        self.g_np1 = self.g_n.copy()
+        self.lam_n = np.zeros(n_qp)
+        self.lam_np1 = np.zeros(n_qp)
+        self.tau = np.zeros((12, n_qp))
+        self.gdot = np.zeros((12, n_qp))
+        self.gdot_n = np.zeros((12, n_qp))
+        self.iters = np.zeros(n_qp, dtype=np.int32)
+
+    def commit(self):
+        self.pF_n[:] = self.pF_np1
+        self.g_n[:] = self.g_np1
+        self.lam_n[:] = self.lam_np1
+        self.gdot_n[:] = self.gdot
+
+
+def assemble_cp(mesh, mats, elem_grain, grain_P, grain_L, ids, ndof, Ap, Ai, st: CPState, r, d_r=None, defl=None,
+                defl_d=None, dt=1.0, residual=True, integrate=True, tangent=True, warm=None):
+    L = _cp_lib()
+    om = (OrcMaterial * len(mats))(*[material(m) for m in mats])
+    f = np.zeros(ndof)
+    vals = np.zeros(Ap[-1])
+    em = mesh.elem_mat if mesh.elem_mat is not None else None
+    eg = np.ascontiguousarray(elem_grain, dtype=np.int32)
+    gP = np.ascontiguousarray(grain_P, dtype=np.float64)
+    gL = np.ascontiguousarray(grain_L, dtype=np.float64)
+    err = L.orc_assemble_cp(mesh.n_nodes, _p(mesh.X), mesh.n_elem, _p(mesh.elem_ptr), _p(mesh.elem_conn), _p(em), om,
+                            _p(eg), _p(gP), _p(gL), _p(ids), ndof, _p(Ap), _p(Ai), _p(r), _p(d_r), _p(defl),
+                            _p(defl_d), dt, int(residual), int(integrate), int(tangent), _p(st.F_np1), _p(st.pF_n),
+                            _p(st.pF_np1), _p(st.g_n), _p(st.g_np1), _p(st.lam_n), _p(st.lam_np1), _p(st.tau),
+                            _p(st.gdot), _p(st.iters), _p(st.gdot_n), _p(warm), _p(f), _p(vals))
+    return f, vals, err

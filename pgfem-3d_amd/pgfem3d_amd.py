@@ -314,3 +314,47 @@ class Handle:
     def set_state(self, field: int, values: np.ndarray):
         v = np.ascontiguousarray(values, dtype=np.float64)
         _check(self._lib.pgf_set_state(self.h, field, _ptr(v)), "pgf_set_state")
+
+
+# ------------------------------------------------------------------ crystal plasticity inputs
+def fcc_crystal(C11=168.4e3, C12=121.4e3, C44=75.4e3, gamma_dot_0=1e-3, m=0.05, g0=16.0, gs=148.0, h0=180.0,
+                w=0.0, gamma_dot_s=1.0, tol=1e-12, max_itr=60) -> Material:
+    """FCC crystal plasticity (CP_PARAM, src/plasticity_model.cc:89-101): power-law flow and
+    saturation hardening with the parameter names of the reference (G0 = h0, gs_0 = gs)."""
+    return Material(model=MODEL_CRYSTAL_PLASTICITY, gamma_dot_0=gamma_dot_0, m_rate=m, G0=h0, g0=g0, gs_0=gs,
+                    gamma_dot_s=gamma_dot_s, w_sat=w, C11=C11, C12=C12, C44=C44, tol_M=tol, max_itr_M=max_itr)
+
+
+def random_quaternions(n: int, seed: int) -> np.ndarray:
+    """n uniformly distributed rotations of SO(3) (Shoemake's method), as (w, x, y, z)."""
+    u = np.random.default_rng(seed).random((n, 3))
+    a, b, c = np.sqrt(1 - u[:, 0]), np.sqrt(u[:, 0]), 2 * np.pi * u[:, 1:]
+    q = np.stack([b * np.cos(c[:, 1]), a * np.sin(c[:, 0]), a * np.cos(c[:, 0]), b * np.sin(c[:, 1])], axis=1)
+    return q
+
+
+def quat_to_rot(q: np.ndarray) -> np.ndarray:
+    """Rotation matrices (crystal -> sample) of unit quaternions (w, x, y, z)."""
+    w, x, y, z = q[..., 0], q[..., 1], q[..., 2], q[..., 3]
+    R = np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y),
+                  2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x),
+                  2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)], axis=-1)
+    return R.reshape(q.shape[:-1] + (3, 3))
+
+
+def voronoi_grains(mesh: MeshArrays, n_grains: int, seed: int):
+    """Element -> grain id by the nearest of n_grains seeded uniform points in the unit cube
+    (element centroid), and one uniform random orientation quaternion per grain."""
+    rng = np.random.default_rng(seed)
+    pts = rng.random((n_grains, 3))
+    cen = np.zeros((mesh.n_elem, 3))
+    nne = np.diff(mesh.elem_ptr)
+    for k in np.unique(nne):
+        sel = np.where(nne == k)[0]
+        idx = mesh.elem_ptr[sel][:, None] + np.arange(k)[None, :]
+        cen[sel] = mesh.X[mesh.elem_conn[idx]][:, :4 if k == 10 else k].mean(axis=1)
+    grain = np.empty(mesh.n_elem, dtype=np.int32)
+    for s in range(0, mesh.n_elem, 65536):
+        d = ((cen[s:s + 65536, None, :] - pts[None, :, :]) ** 2).sum(-1)
+        grain[s:s + 65536] = d.argmin(axis=1)
+    return grain, random_quaternions(n_grains, seed + 1)

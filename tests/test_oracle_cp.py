@@ -1,0 +1,198 @@
+"""Pins of the oracle's crystal-plasticity return map (CPU only)."""
+import numpy as np
+import pytest
+from scipy.optimize import fsolve
+
+import oracle as O
+import pgfem3d_amd as P
+
+MAT = P.fcc_crystal()
+
+
+def _rot(seed):
+    return P.quat_to_rot(P.random_quaternions(1, seed)[0])
+
+
+def _march(Pg, Lg, F_of_step, nsteps, mat=MAT, dt=1.0):
+    """Single material point driven through F(step) with the plain start (previous rates)."""
+    pFn, g_n, lam_n, gd_n = np.eye(3), mat.g0, 0.0, np.zeros(12)
+    out = []
+    for s in range(1, nsteps + 1):
+        F = F_of_step(s)
+        x0 = None
+        if np.any(gd_n):
+            M0 = np.eye(3) - dt * np.einsum("a,aij->ij", gd_n, Pg)
+            x0 = np.r_[M0.reshape(9), lam_n, g_n]
+        r = O.cp_update(mat, Pg, Lg, F, pFn, g_n, lam_n, dt, x0=x0)
+        assert r["rc"] == 0, (s, r["iters"])
+        out.append((F, pFn.copy(), g_n, lam_n, r))
+        pFn, g_n, lam_n, gd_n = r["pF"], r["g"], r["lam"], r["gdot"]
+    return out
+
+
+def _uniax(s, de=2.5e-4):
+    e = de * s
+    return np.diag([1 + e, 1 - 0.45 * e, 1 - 0.45 * e])
+
+
+@pytest.mark.parametrize("seed", [5, 9])
+def test_isochoric_and_hardening_relation(seed):
+    Pg, Lg = O.cp_grain(MAT, _rot(seed))
+    hist = _march(Pg, Lg, _uniax, 16)
+    for F, pFn, g_n, lam_n, r in hist:
+        assert abs(np.linalg.det(r["pF"]) - 1.0) < 1e-12
+        gdot = np.abs(r["gdot"]).sum()
+        gs, g0, h0 = MAT.gs_0, MAT.g0, MAT.G0
+        g_closed = ((gs - g0) * g_n + h0 * gs * gdot) / (gs - g0 + h0 * gdot)  # int_HA, dt = 1
+        assert abs(r["g"] - g_closed) < 1e-9 * g_closed
+    assert hist[-1][4]["g"] > MAT.g0  # it hardened
+    # steady plastic steps converge in a handful of iterations from the previous rates
+    assert max(h[4]["iters"] for h in hist[-6:]) <= 12
+
+
+def test_flow_rule_at_convergence():
+    """gdot_a = gdot0 (tau_a/g)|tau_a/g|^(1/m-1) with tau_a = (C_e S):P_a recomputed here from
+    the converged pF (numpy restatement, independent of the C++ evaluation)."""
+    Pg, Lg = O.cp_grain(MAT, _rot(7))
+    F, pFn, g_n, lam_n, r = _march(Pg, Lg, _uniax, 10)[-1]
+    eF = F @ np.linalg.inv(r["pF"])
+    Ce = eF.T @ eF
+    S = np.einsum("ijkl,kl->ij", Lg, 0.5 * (Ce - np.eye(3)))
+    tau = np.einsum("ij,aij->a", Ce @ S, Pg)
+    np.testing.assert_allclose(tau, r["tau"], rtol=1e-10, atol=1e-10)
+    np.testing.assert_allclose(r["S"], S, rtol=1e-10, atol=1e-9)
+    gd = MAT.gamma_dot_0 * (tau / r["g"]) * np.abs(tau / r["g"]) ** (1 / MAT.m_rate - 1)
+    np.testing.assert_allclose(r["gdot"], gd, rtol=1e-9, atol=1e-20)
+    # backward Euler of the plastic flow with the isochoric constraint
+    M = pFn @ np.linalg.inv(r["pF"])
+    pJ = np.linalg.det(np.linalg.inv(M) @ pFn)
+    RM = M - np.eye(3) + np.einsum("a,aij->ij", r["gdot"], Pg) - r["lam"] * pJ * np.linalg.inv(M).T
+    assert np.abs(RM).max() < 1e-12
+
+
+def test_scipy_solves_same_system():
+    """fsolve on a numpy restatement of the three equations lands on the oracle's root."""
+    Pg, Lg = O.cp_grain(MAT, _rot(11))
+    F, pFn, g_n, lam_n, r = _march(Pg, Lg, _uniax, 8)[-1]
+    A = F @ np.linalg.inv(pFn)
+
+    def res(x):
+        M, lam, g = x[:9].reshape(3, 3), x[9], x[10]
+        eF = A @ M
+        Ce = eF.T @ eF
+        S = np.einsum("ijkl,kl->ij", Lg, 0.5 * (Ce - np.eye(3)))
+        tau = np.einsum("ij,aij->a", Ce @ S, Pg)
+        gd = MAT.gamma_dot_0 * (tau / g) * np.abs(tau / g) ** (1 / MAT.m_rate - 1)
+        pJ = np.linalg.det(pFn) / np.linalg.det(M)
+        RM = M - np.eye(3) + np.einsum("a,aij->ij", gd, Pg) - lam * pJ * np.linalg.inv(M).T
+        gsum = np.abs(gd).sum()
+        gs, g0, h0 = MAT.gs_0, MAT.g0, MAT.G0
+        Rg = g * (gs - g0 + h0 * gsum) - (gs - g0) * g_n - h0 * gs * gsum
+        return np.r_[RM.reshape(9), pJ - 1, Rg / (gs * g_n)]
+
+    M_or = pFn @ np.linalg.inv(r["pF"])
+    x_or = np.r_[M_or.reshape(9), r["lam"], r["g"]]
+    x0 = x_or * (1 + 1e-6 * np.random.default_rng(0).standard_normal(11))
+    x_sp = fsolve(res, x0, xtol=1e-14)
+    assert np.abs(res(x_or)).max() < 1e-11
+    np.testing.assert_allclose(x_sp[:9], x_or[:9], atol=1e-11)
+    np.testing.assert_allclose(x_sp[10], x_or[10], rtol=1e-10)
+
+
+def test_dMdF_matches_fd():
+    Pg, Lg = O.cp_grain(MAT, _rot(3))
+    F, pFn, g_n, lam_n, r = _march(Pg, Lg, _uniax, 10)[-1]
+    assert r["gdot"].any()
+    x_conv = np.r_[(pFn @ np.linalg.inv(r["pF"])).reshape(9), r["lam"], r["g"]]
+
+    def M_of(Fp):
+        rr = O.cp_update(MAT, Pg, Lg, Fp, pFn, g_n, lam_n, 1.0, x0=x_conv, tangent=False)
+        assert rr["rc"] == 0
+        return (pFn @ np.linalg.inv(rr["pF"])).reshape(9)
+
+    h = 1e-7
+    fd = np.zeros((9, 9))
+    for k in range(9):
+        dF = np.zeros(9)
+        dF[k] = h
+        fd[:, k] = (M_of(F + dF.reshape(3, 3)) - M_of(F - dF.reshape(3, 3))) / (2 * h)
+    assert np.abs(fd - r["dMdF"]).max() / np.abs(fd).max() < 1e-6
+
+
+def test_slip_off_is_st_venant_kirchhoff():
+    """gdot0 -> 0: no plastic flow; S = L : (F^T F - I)/2 with the rotated cubic L."""
+    mat = P.fcc_crystal(gamma_dot_0=1e-100)
+    R = _rot(2)
+    Pg, Lg = O.cp_grain(mat, R)
+    F = np.eye(3) + 0.01 * np.random.default_rng(1).standard_normal((3, 3))
+    r = O.cp_update(mat, Pg, Lg, F, np.eye(3), mat.g0, 0.0, 1.0)
+    assert r["rc"] == 0
+    np.testing.assert_allclose(r["pF"], np.eye(3), atol=1e-14)
+    S = np.einsum("ijkl,kl->ij", Lg, 0.5 * (F.T @ F - np.eye(3)))
+    np.testing.assert_allclose(r["S"], S, rtol=1e-12, atol=1e-9)
+    # rotated cubic stiffness: crystal-frame components come back under R^T
+    L0 = np.einsum("ai,bj,ck,dl,abcd->ijkl", R, R, R, R, Lg)
+    assert abs(L0[0, 0, 0, 0] - mat.C11) < 1e-8 and abs(L0[0, 0, 1, 1] - mat.C12) < 1e-8
+    assert abs(L0[0, 1, 0, 1] - mat.C44) < 1e-8 and abs(L0[0, 0, 0, 1]) < 1e-8
+
+
+def test_fcc_slip_systems():
+    P0 = O.cp_slip_systems()
+    for a in range(12):
+        # unit slip direction and normal, orthogonal: P = s (x) m has |P| = 1 and tr P = 0
+        assert abs(np.linalg.norm(P0[a]) - 1) < 1e-15 and abs(np.trace(P0[a])) < 1e-15
+        n = P0[a].sum(axis=0) / np.abs(P0[a]).sum()  # rank one
+        assert np.linalg.matrix_rank(P0[a], tol=1e-12) == 1
+    # 12 distinct systems spanning the 8-dim traceless space
+    assert np.linalg.matrix_rank(P0.reshape(12, 9), tol=1e-10) == 8
+
+
+def _cp_problem(n, etype, seed=1, n_grains=3):
+    from problems import BoxProblem
+
+    prob = BoxProblem(n, etype, MAT, seed=seed)
+    grain, q = P.voronoi_grains(prob.mesh, n_grains, seed)
+    R = P.quat_to_rot(q)
+    tabs = [O.cp_grain(MAT, R[g]) for g in range(n_grains)]
+    prob.grain = grain
+    prob.gP = np.ascontiguousarray(np.stack([t[0] for t in tabs]).reshape(n_grains, 108))
+    prob.gL = np.ascontiguousarray(np.stack([t[1] for t in tabs]).reshape(n_grains, 81))
+    return prob
+
+
+def _cp_assemble(prob, st, r, defl, **kw):
+    return O.assemble_cp(prob.mesh, prob.mats, prob.grain, prob.gP, prob.gL, prob.ids, prob.ndof, prob.Ap, prob.Ai,
+                         st, r, defl=defl, **kw)
+
+
+@pytest.mark.parametrize("etype", [4, 10])
+def test_cp_element_tangent_is_fd_of_residual(etype):
+    prob = _cp_problem(1, etype)
+    st = O.CPState(prob.nqp, MAT.g0)
+    for s in range(1, 11):
+        r, defl = prob.displacement(1.5e-4 * s, 2e-5, seed=s)
+        f, vals, err = _cp_assemble(prob, st, r, defl)
+        assert err == 0, s
+        st.commit()
+    r, defl = prob.displacement(1.5e-4 * 11, 2e-5, seed=11)
+    n_state = {k: getattr(st, k).copy() for k in ("pF_n", "g_n", "lam_n", "gdot_n")}
+    f0, vals, err = _cp_assemble(prob, st, r, defl)
+    assert err == 0 and np.abs(st.gdot).max() > 1e-5  # plastic
+    K = np.zeros((prob.ndof, prob.ndof))
+    for i in range(prob.ndof):
+        K[i, prob.Ai[prob.Ap[i]:prob.Ap[i + 1]]] = vals[prob.Ap[i]:prob.Ap[i + 1]]
+    h = 1e-8
+    Kfd = np.zeros_like(K)
+    for j in range(prob.ndof):
+        col = []
+        for sgn in (1, -1):
+            st2 = O.CPState(prob.nqp, MAT.g0)
+            for k, v in n_state.items():
+                getattr(st2, k)[:] = v
+            rp = r.copy()
+            rp[j] += sgn * h
+            fp, _, e2 = _cp_assemble(prob, st2, rp, defl, tangent=False)
+            assert e2 == 0
+            col.append(fp)
+        Kfd[:, j] = (col[0] - col[1]) / (2 * h)
+    assert np.abs(K - Kfd).max() / np.abs(K).max() < 1e-6
