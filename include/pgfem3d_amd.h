@@ -142,6 +142,31 @@ int pgf_last_pass_ms(pgf_handle h, double *ms, int32_t *n_launches);
 int pgf_get_sizes(pgf_handle h, int64_t *n_qp, int64_t *nnz, int64_t *ndof,
                   int32_t *n_colors);
 
+/* Crystal orientations: n_grains rotation matrices R (crystal -> sample frame, row-major,
+ * [n_grains*9], host) and the grain of every element ([n_elem], host).  Replaces
+ * plasticity_model_set_orientations / plasticity_model_construct_rotation
+ * (src/plasticity_model.cc:1226-1258,1455), which store R per QP (TENSOR_R); here R is stored
+ * per grain and slip systems and stiffness are rotated once per grain.  Without this call
+ * every element uses the crystal frame. */
+int pgf_set_orientations(pgf_handle h, int32_t n_grains, const double *R, const int32_t *elem_grain);
+
+/* Time step size dt = t(n+1) - t(n) used by rate-dependent models (CM_Ctx::dt). */
+int pgf_set_time_step(pgf_handle h, double dt);
+
+/* Crystal plasticity: start the next integrations' local Newton from a caller-provided point
+ * (device pointer, [11][n_qp] in DEVICE QP order: M = pF(n) pF(n+1)^-1 (9), lambda, g), e.g. a
+ * learned prediction; NULL restores the plain start (slip rates of the last converged step).
+ * No counterpart in the reference; the converged result does not depend on the start. */
+int pgf_set_warm_start(pgf_handle h, const double *warm);
+
+/* Device QP order: mesh_qp[d] = mesh QP index (element order, then QP order) of device QP d
+ * (host array [n_qp]).  State arrays on the device are stored in device QP order. */
+int pgf_get_qp_order(pgf_handle h, int64_t *mesh_qp);
+
+/* Device pointer of a state field ([ncomp][n_qp], device QP order), for callers that read
+ * the state on the GPU (e.g. to build warm starts); NULL if the field does not exist. */
+int pgf_state_device_ptr(pgf_handle h, int32_t field, double **ptr);
+
 /* ============================================================ per Newton iteration */
 /* Nodal unknowns follow def_elem_total (src/utils.cc:1391-1412): a free DOF id>0 takes
  * r[id-1] + d_r[id-1]; a prescribed DOF id<0 takes defl[-id-1] + defl_d[-id-1]; a fixed DOF
@@ -192,11 +217,14 @@ int pgf_reset_step(pgf_handle h);
 #define PGF_STATE_PLASTIC_NP1 3  /* J2: 6 comps Cp^-1 (xx,yy,zz,xy,yz,xz); CP: 9 comps pF */
 #define PGF_STATE_PLASTIC_N 4
 #define PGF_STATE_PLASTIC_NM1 5
-#define PGF_STATE_SCALAR_NP1 6   /* J2: ep, gam; CP: g (slip resistance), L */
+#define PGF_STATE_SCALAR_NP1 6   /* J2: ep, gam; CP: g (slip resistance), lambda */
 #define PGF_STATE_SCALAR_N 7
 #define PGF_STATE_SCALAR_NM1 8
-#define PGF_STATE_CAUCHY 9       /* 6 comps: Cauchy stress of the last integration */
-#define PGF_STATE_ITERS 10       /* 1 comp: local Newton iterations of the last integration */
+#define PGF_STATE_TAU 9          /* CP: 12 resolved shear stresses at n+1 (TENSOR_tau) */
+#define PGF_STATE_TAU_N 10       /* CP: at n (TENSOR_tau_n) */
+#define PGF_STATE_GDOT 11        /* CP: 12 slip rates at n+1 (TENSOR_gamma_dot) */
+#define PGF_STATE_GDOT_N 12      /* CP: at n (TENSOR_gamma_dot_n) */
+#define PGF_STATE_ITERS 13       /* CP: local Newton iterations of the last integration */
 /* Number of components of a field (0 if the field does not exist for the mesh). */
 int pgf_state_ncomp(pgf_handle h, int32_t field, int32_t *ncomp);
 /* Copy a field to / from host memory laid out [comp][n_qp]. */

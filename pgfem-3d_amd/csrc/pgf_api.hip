@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <numeric>
@@ -26,6 +27,8 @@ struct Block {
   int32_t *d_conn = nullptr;
   int16_t *d_off = nullptr;
   int32_t *d_mat = nullptr;
+  int32_t *d_egrain = nullptr;  // crystal plasticity: grain per element / per QP (device order)
+  int32_t *d_qgrain = nullptr;
 };
 
 template <class T>
@@ -43,6 +46,44 @@ int upload(T **p, const T *h, size_t n) {
 }
 
 }  // namespace
+
+// Host tables of one grain: the 12 FCC {111}<110> systems (unit slip direction s and plane
+// normal m, Schmid tensor s (x) m) and the cubic stiffness, both rotated by R (crystal ->
+// sample frame).
+void make_grain(const double *R, double C11, double C12, double C44, DevGrain &g) {
+  const double n[4][3] = {{1, 1, 1}, {-1, 1, 1}, {1, -1, 1}, {1, 1, -1}};
+  const double sd[12][3] = {{0, 1, -1}, {-1, 0, 1}, {1, -1, 0}, {0, 1, -1}, {1, 0, 1}, {1, 1, 0},
+                            {0, 1, 1},  {1, 0, -1}, {1, 1, 0},  {0, 1, 1},  {1, 0, 1}, {1, -1, 0}};
+  const double rn = 1.0 / std::sqrt(3.0), rs = 1.0 / std::sqrt(2.0);
+  for (int a = 0; a < 12; ++a) {
+    double s[3], m[3];
+    for (int i = 0; i < 3; ++i) {
+      s[i] = R[i * 3] * sd[a][0] * rs + R[i * 3 + 1] * sd[a][1] * rs + R[i * 3 + 2] * sd[a][2] * rs;
+      m[i] = R[i * 3] * n[a / 3][0] * rn + R[i * 3 + 1] * n[a / 3][1] * rn + R[i * 3 + 2] * n[a / 3][2] * rn;
+    }
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) g.P[a][i * 3 + j] = s[i] * m[j];
+  }
+  auto L0 = [&](int i, int j, int k, int l) {
+    double v = 0.0;
+    if (i == j && k == l) v += (i == k) ? C11 : C12;
+    if (i != j && ((i == k && j == l) || (i == l && j == k))) v += C44;
+    return v;
+  };
+  static const int vi[6] = {0, 1, 2, 0, 1, 0}, vj[6] = {0, 1, 2, 1, 2, 2};
+  for (int I = 0; I < 6; ++I)
+    for (int J = 0; J < 6; ++J) {
+      const int i = vi[I], j = vj[I], k = vi[J], l = vj[J];
+      double v = 0.0;
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b)
+          for (int c = 0; c < 3; ++c)
+            for (int d = 0; d < 3; ++d)
+              v += R[i * 3 + a] * R[j * 3 + b] * R[k * 3 + c] * R[l * 3 + d] * L0(a, b, c, d);
+      g.Lv[I * 6 + J] = v * (J < 3 ? 1.0 : 2.0);  // tensor-component E: shear pairs count twice
+    }
+}
+
 }  // namespace pgf
 
 struct pgf_handle_s {
@@ -61,6 +102,16 @@ struct pgf_handle_s {
   double *d_P[3] = {nullptr, nullptr, nullptr};   // J2: Cp^-1 (6)
   double *d_Sc[3] = {nullptr, nullptr, nullptr};  // J2: ep, gam
   int32_t *d_err = nullptr;
+  // crystal plasticity
+  bool has_cp = false;
+  pgf::CPMaterial *d_cpmat = nullptr;
+  pgf::DevGrain *d_grains = nullptr;
+  double *d_tau[2] = {nullptr, nullptr}, *d_gdot[2] = {nullptr, nullptr};  // n+1, n
+  int32_t *d_iters = nullptr;
+  double *d_qpbuf = nullptr;
+  const double *warm = nullptr;
+  double cp_g0 = 0.0, dt = 1.0;
+  std::vector<double> cp_mat_host;  // C11, C12, C44
   bool profiling = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_ms = 0.0;
@@ -79,8 +130,58 @@ int set_identity_state(pgf_handle h) {
       for (int I = 0; I < 6; ++I) pgf::launch_fill(h->d_P[t] + I * NQ, NQ, I < 3 ? 1.0 : 0.0, h->stream);
       pgf::launch_fill(h->d_Sc[t], 2 * NQ, 0.0, h->stream);
     }
+    if (h->has_cp) {  // pF = I, g = g0, lambda = 0
+      for (int i = 0; i < 9; ++i) pgf::launch_fill(h->d_P[t] + i * NQ, NQ, (i % 4 == 0) ? 1.0 : 0.0, h->stream);
+      pgf::launch_fill(h->d_Sc[t], NQ, h->cp_g0, h->stream);
+      pgf::launch_fill(h->d_Sc[t] + NQ, NQ, 0.0, h->stream);
+    }
   }
+  if (h->has_cp)
+    for (int t = 0; t < 2; ++t) {
+      pgf::launch_fill(h->d_tau[t], 12 * NQ, 0.0, h->stream);
+      pgf::launch_fill(h->d_gdot[t], 12 * NQ, 0.0, h->stream);
+    }
   return hipStreamSynchronize(h->stream) == hipSuccess ? PGF_OK : PGF_EHIP;
+}
+
+// Crystal-plasticity pre-pass of one block: F(n+1), the return map, and the per-QP data of
+// the element kernel.
+void cp_prepass(pgf_handle h, const Block &b, const pgf_nodal_input *in, int mode) {
+  pgf::CPArgs c{};
+  c.X = h->d_X;
+  c.dofid = h->d_dofid;
+  c.conn = b.d_conn;
+  c.qgrain = b.d_qgrain;
+  c.grains = h->d_grains;
+  c.cpmats = h->d_cpmat;
+  c.qp_base = b.qp_base;
+  c.n_qp = h->n_qp;
+  c.ne = b.ne;
+  c.nq = b.ne * b.nqp;
+  c.r = in->r;
+  c.d_r = in->d_r;
+  c.defl = in->defl;
+  c.defl_d = in->defl_d;
+  c.dt = h->dt;
+  const int64_t NQ = h->n_qp;
+  c.F_np1 = h->d_F[0];
+  c.pF_n = h->d_P[1];
+  c.pF_np1 = h->d_P[0];
+  c.g_n = h->d_Sc[1];
+  c.lam_n = h->d_Sc[1] + NQ;
+  c.Sc_np1 = h->d_Sc[0];
+  c.gdot_n = h->d_gdot[1];
+  c.tau = h->d_tau[0];
+  c.gdot = h->d_gdot[0];
+  c.iters = h->d_iters;
+  c.warm = h->warm;
+  c.qpbuf = h->d_qpbuf + b.qp_base * pgf::kCPBuf;
+  c.err = h->d_err;
+  if (mode & pgf::MODE_INTEGRATE) {
+    pgf::launch_cp_kinematics(b.nne, c, h->stream);
+    pgf::launch_cp_newton(b.nne, c, h->stream);
+  }
+  pgf::launch_cp_tangent(b.nne, c, (mode & pgf::MODE_TANGENT) != 0, h->stream);
 }
 
 pgf::ElemArgs base_args(pgf_handle h, const pgf_nodal_input *in) {
@@ -115,15 +216,39 @@ int run_pass(pgf_handle h, const pgf_nodal_input *in, int mode, double *f, doubl
   a.vals = vals;
   if (h->profiling) hipEventRecord(h->ev0, h->stream);
   int32_t launches = 0;
+  for (auto &b : h->blocks)
+    if (b.model == PGF_MODEL_CRYSTAL_PLASTICITY) cp_prepass(h, b, in, mode);
   for (int32_t c = 0; c < h->n_colors; ++c)
     for (auto &b : h->blocks) {
       const int64_t e0 = b.color_ptr[c], e1 = b.color_ptr[c + 1];
       if (e1 <= e0) continue;
-      a.conn = b.d_conn;
-      a.off = b.d_off;
-      a.mat = b.d_mat;
-      a.qp_base = b.qp_base;
-      pgf::launch_elem(b.nne, mode, a, e0, e1, h->stream);
+      if (b.model == PGF_MODEL_CRYSTAL_PLASTICITY) {
+        pgf::CPElemArgs ce{};
+        ce.X = h->d_X;
+        ce.dofid = h->d_dofid;
+        ce.row_start = h->d_row_start;
+        ce.rowlen = h->d_rowlen;
+        ce.conn = b.d_conn;
+        ce.grain = b.d_egrain;
+        ce.off = b.d_off;
+        ce.grains = h->d_grains;
+        ce.qpbuf = h->d_qpbuf + b.qp_base * pgf::kCPBuf;
+        ce.qp_base = b.qp_base;
+        ce.r = in->r;
+        ce.d_r = in->d_r;
+        ce.defl = in->defl;
+        ce.defl_d = in->defl_d;
+        ce.f = f;
+        ce.vals = vals;
+        ce.err = h->d_err;
+        pgf::launch_cp_elem(b.nne, mode, ce, e0, e1, h->stream);
+      } else {
+        a.conn = b.d_conn;
+        a.off = b.d_off;
+        a.mat = b.d_mat;
+        a.qp_base = b.qp_base;
+        pgf::launch_elem(b.nne, mode, a, e0, e1, h->stream);
+      }
       ++launches;
     }
   if (h->profiling) hipEventRecord(h->ev1, h->stream);
@@ -150,14 +275,24 @@ int field_buffer(pgf_handle h, int32_t field, double **buf, int32_t *ncomp) {
       *ncomp = 9;
       return PGF_OK;
     case PGF_STATE_PLASTIC_NP1: case PGF_STATE_PLASTIC_N: case PGF_STATE_PLASTIC_NM1:
-      if (!h->has_j2) return PGF_OK;
+      if (!h->has_j2 && !h->has_cp) return PGF_OK;
       *buf = h->d_P[field - PGF_STATE_PLASTIC_NP1];
-      *ncomp = 6;
+      *ncomp = h->has_cp ? 9 : 6;
       return PGF_OK;
     case PGF_STATE_SCALAR_NP1: case PGF_STATE_SCALAR_N: case PGF_STATE_SCALAR_NM1:
-      if (!h->has_j2) return PGF_OK;
+      if (!h->has_j2 && !h->has_cp) return PGF_OK;
       *buf = h->d_Sc[field - PGF_STATE_SCALAR_NP1];
       *ncomp = 2;
+      return PGF_OK;
+    case PGF_STATE_TAU: case PGF_STATE_TAU_N:
+      if (!h->has_cp) return PGF_OK;
+      *buf = h->d_tau[field - PGF_STATE_TAU];
+      *ncomp = 12;
+      return PGF_OK;
+    case PGF_STATE_GDOT: case PGF_STATE_GDOT_N:
+      if (!h->has_cp) return PGF_OK;
+      *buf = h->d_gdot[field - PGF_STATE_GDOT];
+      *ncomp = 12;
       return PGF_OK;
   }
   return PGF_EINVAL;
@@ -183,14 +318,23 @@ int pgf_create(const pgf_mesh *mesh, const int64_t *id, int64_t ndof, const int3
     for (int64_t k = m.elem_ptr[e]; k < m.elem_ptr[e + 1]; ++k)
       if (m.elem_conn[k] < 0 || m.elem_conn[k] >= m.n_nodes) return PGF_EINVAL;
   }
+  int n_cp = 0, n_j2 = 0, cp_index = -1;
   for (int32_t i = 0; i < n_mat; ++i) {
     const int md = mats[i].model;
     if (md == PGF_MODEL_HYPER_ELASTICITY) {
       if (mats[i].m01 != 0.0 || mats[i].devPotFlag != 1) return PGF_ENOTSUP;  // neo-Hookean only
-    } else if (md != PGF_MODEL_J2_PLASTICITY) {
+    } else if (md == PGF_MODEL_J2_PLASTICITY) {
+      ++n_j2;
+    } else if (md == PGF_MODEL_CRYSTAL_PLASTICITY) {
+      ++n_cp;
+      cp_index = i;
+      if (!(mats[i].m_rate > 0.0) || !(mats[i].g0 > 0.0)) return PGF_EINVAL;
+    } else {
       return PGF_ENOTSUP;
     }
   }
+  // one crystal-plasticity material per handle, not mixed with J2 (they share state slots)
+  if (n_cp > 1 || (n_cp && n_j2)) return PGF_ENOTSUP;
   // free DOFs of a node must be consecutive ids (no periodic nodes)
   std::vector<int32_t> dofid(size_t(m.n_nodes) * 3), row_start(m.n_nodes, -1), rowlen(m.n_nodes, 0);
   for (int64_t n = 0; n < m.n_nodes; ++n) {
@@ -244,6 +388,7 @@ int pgf_create(const pgf_mesh *mesh, const int64_t *id, int64_t ndof, const int3
     }
     elem_block[e] = int(it - keys.begin());
     if (md == PGF_MODEL_J2_PLASTICITY) h->has_j2 = true;
+    if (md == PGF_MODEL_CRYSTAL_PLASTICITY) h->has_cp = true;
   }
   h->blocks.resize(keys.size());
   int64_t qp = 0;
@@ -329,10 +474,43 @@ int pgf_create(const pgf_mesh *mesh, const int64_t *id, int64_t ndof, const int3
   }
   for (int t = 0; t < 3; ++t) {
     if ((rc = pgf::dalloc(&h->d_F[t], size_t(h->n_qp) * 9))) return fail(rc);
-    if (h->has_j2) {
-      if ((rc = pgf::dalloc(&h->d_P[t], size_t(h->n_qp) * 6))) return fail(rc);
+    if (h->has_j2 || h->has_cp) {
+      if ((rc = pgf::dalloc(&h->d_P[t], size_t(h->n_qp) * (h->has_cp ? 9 : 6)))) return fail(rc);
       if ((rc = pgf::dalloc(&h->d_Sc[t], size_t(h->n_qp) * 2))) return fail(rc);
     }
+  }
+  if (h->has_cp) {
+    const pgf_material &cpm = mats[cp_index];
+    pgf::CPMaterial c{};
+    c.gd0 = cpm.gamma_dot_0;
+    c.m = cpm.m_rate;
+    c.h0 = cpm.G0;
+    c.g0 = cpm.g0;
+    c.gs0 = cpm.gs_0;
+    c.gds = cpm.gamma_dot_s;
+    c.w = cpm.w_sat;
+    c.tol = cpm.tol_M > 0.0 ? cpm.tol_M : 1.0e-10;
+    c.maxit = cpm.max_itr_M > 0 ? cpm.max_itr_M : 50;
+    h->cp_g0 = cpm.g0;
+    h->cp_mat_host = {cpm.C11, cpm.C12, cpm.C44};
+    if ((rc = pgf::upload(&h->d_cpmat, &c, 1))) return fail(rc);
+    for (int t = 0; t < 2; ++t) {
+      if ((rc = pgf::dalloc(&h->d_tau[t], size_t(h->n_qp) * 12))) return fail(rc);
+      if ((rc = pgf::dalloc(&h->d_gdot[t], size_t(h->n_qp) * 12))) return fail(rc);
+    }
+    if ((rc = pgf::dalloc(&h->d_iters, size_t(h->n_qp)))) return fail(rc);
+    if ((rc = pgf::dalloc(&h->d_qpbuf, size_t(h->n_qp) * pgf::kCPBuf))) return fail(rc);
+    if (hipMemset(h->d_iters, 0, sizeof(int32_t) * h->n_qp) != hipSuccess) return fail(PGF_EHIP);
+    for (auto &b : h->blocks) {
+      if (b.model != PGF_MODEL_CRYSTAL_PLASTICITY) continue;
+      if ((rc = pgf::dalloc(&b.d_egrain, size_t(b.ne)))) return fail(rc);
+      if ((rc = pgf::dalloc(&b.d_qgrain, size_t(b.ne * b.nqp)))) return fail(rc);
+      if (hipMemset(b.d_egrain, 0, sizeof(int32_t) * b.ne) != hipSuccess ||
+          hipMemset(b.d_qgrain, 0, sizeof(int32_t) * b.ne * b.nqp) != hipSuccess)
+        return fail(PGF_EHIP);
+    }
+    const double I9[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    if ((rc = pgf_set_orientations(h, 1, I9, nullptr))) return fail(rc);
   }
   if ((rc = pgf::dalloc(&h->d_err, 1))) return fail(rc);
   if ((rc = set_identity_state(h))) return fail(rc);
@@ -352,7 +530,17 @@ int pgf_destroy(pgf_handle h) {
     hipFree(b.d_conn);
     hipFree(b.d_off);
     hipFree(b.d_mat);
+    hipFree(b.d_egrain);
+    hipFree(b.d_qgrain);
   }
+  hipFree(h->d_cpmat);
+  hipFree(h->d_grains);
+  for (int t = 0; t < 2; ++t) {
+    hipFree(h->d_tau[t]);
+    hipFree(h->d_gdot[t]);
+  }
+  hipFree(h->d_iters);
+  hipFree(h->d_qpbuf);
   for (int t = 0; t < 3; ++t) {
     hipFree(h->d_F[t]);
     hipFree(h->d_P[t]);
@@ -404,20 +592,20 @@ int pgf_get_sizes(pgf_handle h, int64_t *n_qp, int64_t *nnz, int64_t *ndof, int3
 
 int pgf_residual(pgf_handle h, const pgf_nodal_input *in, double dt, int run_integration, double *f) {
   if (!h || !f) return PGF_EINVAL;
-  (void)dt;
+  if (dt > 0.0) h->dt = dt;
   return run_pass(h, in, run_integration ? (pgf::MODE_INTEGRATE | pgf::MODE_RESIDUAL) : pgf::MODE_RESIDUAL,
                   f, nullptr);
 }
 
 int pgf_tangent(pgf_handle h, const pgf_nodal_input *in, double dt, double *vals) {
   if (!h || !vals) return PGF_EINVAL;
-  (void)dt;
+  if (dt > 0.0) h->dt = dt;
   return run_pass(h, in, pgf::MODE_TANGENT, nullptr, vals);
 }
 
 int pgf_residual_tangent(pgf_handle h, const pgf_nodal_input *in, double dt, double *f, double *vals) {
   if (!h || !f || !vals) return PGF_EINVAL;
-  (void)dt;
+  if (dt > 0.0) h->dt = dt;
   return run_pass(h, in, pgf::MODE_INTEGRATE | pgf::MODE_RESIDUAL | pgf::MODE_TANGENT, f, vals);
 }
 
@@ -426,11 +614,16 @@ int pgf_commit_step(pgf_handle h) {
   const int64_t NQ = h->n_qp;
   pgf::launch_copy(h->d_F[2], h->d_F[1], 9 * NQ, h->stream);
   pgf::launch_copy(h->d_F[1], h->d_F[0], 9 * NQ, h->stream);
-  if (h->has_j2) {
-    pgf::launch_copy(h->d_P[2], h->d_P[1], 6 * NQ, h->stream);
-    pgf::launch_copy(h->d_P[1], h->d_P[0], 6 * NQ, h->stream);
+  if (h->has_j2 || h->has_cp) {
+    const int np = h->has_cp ? 9 : 6;
+    pgf::launch_copy(h->d_P[2], h->d_P[1], np * NQ, h->stream);
+    pgf::launch_copy(h->d_P[1], h->d_P[0], np * NQ, h->stream);
     pgf::launch_copy(h->d_Sc[2], h->d_Sc[1], 2 * NQ, h->stream);
     pgf::launch_copy(h->d_Sc[1], h->d_Sc[0], 2 * NQ, h->stream);
+  }
+  if (h->has_cp) {  // tau_n, gamma_dot_n (src/plasticity_model.cc:174-192)
+    pgf::launch_copy(h->d_tau[1], h->d_tau[0], 12 * NQ, h->stream);
+    pgf::launch_copy(h->d_gdot[1], h->d_gdot[0], 12 * NQ, h->stream);
   }
   return hipStreamSynchronize(h->stream) == hipSuccess ? PGF_OK : PGF_EHIP;
 }
@@ -439,21 +632,38 @@ int pgf_reset_step(pgf_handle h) {
   if (!h) return PGF_EINVAL;
   const int64_t NQ = h->n_qp;
   pgf::launch_copy(h->d_F[0], h->d_F[1], 9 * NQ, h->stream);
-  if (h->has_j2) {
-    pgf::launch_copy(h->d_P[0], h->d_P[1], 6 * NQ, h->stream);
+  if (h->has_j2 || h->has_cp) {
+    pgf::launch_copy(h->d_P[0], h->d_P[1], (h->has_cp ? 9 : 6) * NQ, h->stream);
     pgf::launch_copy(h->d_Sc[0], h->d_Sc[1], 2 * NQ, h->stream);
+  }
+  if (h->has_cp) {
+    pgf::launch_copy(h->d_tau[0], h->d_tau[1], 12 * NQ, h->stream);
+    pgf::launch_copy(h->d_gdot[0], h->d_gdot[1], 12 * NQ, h->stream);
   }
   return hipStreamSynchronize(h->stream) == hipSuccess ? PGF_OK : PGF_EHIP;
 }
 
 int pgf_state_ncomp(pgf_handle h, int32_t field, int32_t *ncomp) {
   if (!h || !ncomp) return PGF_EINVAL;
+  if (field == PGF_STATE_ITERS) {
+    *ncomp = h->has_cp ? 1 : 0;
+    return PGF_OK;
+  }
   double *buf;
   return field_buffer(h, field, &buf, ncomp);
 }
 
 int pgf_get_state(pgf_handle h, int32_t field, double *host_out) {
   if (!h || !host_out) return PGF_EINVAL;
+  if (field == PGF_STATE_ITERS) {
+    if (!h->has_cp) return PGF_EINVAL;
+    std::vector<int32_t> it(h->n_qp);
+    if (hipStreamSynchronize(h->stream) != hipSuccess) return PGF_EHIP;
+    if (hipMemcpy(it.data(), h->d_iters, sizeof(int32_t) * h->n_qp, hipMemcpyDeviceToHost) != hipSuccess)
+      return PGF_EHIP;
+    for (int64_t q = 0; q < h->n_qp; ++q) host_out[h->qp_mesh[q]] = double(it[q]);
+    return PGF_OK;
+  }
   double *buf;
   int32_t nc;
   int rc = field_buffer(h, field, &buf, &nc);
@@ -479,6 +689,57 @@ int pgf_set_state(pgf_handle h, int32_t field, const double *host_in) {
     for (int64_t q = 0; q < NQ; ++q) tmp[c * NQ + q] = host_in[c * NQ + h->qp_mesh[q]];
   if (hipStreamSynchronize(h->stream) != hipSuccess) return PGF_EHIP;
   if (hipMemcpy(buf, tmp.data(), tmp.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) return PGF_EHIP;
+  return PGF_OK;
+}
+
+int pgf_set_time_step(pgf_handle h, double dt) {
+  if (!h || !(dt > 0.0)) return PGF_EINVAL;
+  h->dt = dt;
+  return PGF_OK;
+}
+
+int pgf_set_warm_start(pgf_handle h, const double *warm) {
+  if (!h) return PGF_EINVAL;
+  if (warm && !h->has_cp) return PGF_ENOTSUP;
+  h->warm = warm;
+  return PGF_OK;
+}
+
+int pgf_get_qp_order(pgf_handle h, int64_t *mesh_qp) {
+  if (!h || !mesh_qp) return PGF_EINVAL;
+  std::copy(h->qp_mesh.begin(), h->qp_mesh.end(), mesh_qp);
+  return PGF_OK;
+}
+
+int pgf_state_device_ptr(pgf_handle h, int32_t field, double **ptr) {
+  if (!h || !ptr) return PGF_EINVAL;
+  int32_t nc = 0;
+  return field_buffer(h, field, ptr, &nc);
+}
+
+int pgf_set_orientations(pgf_handle h, int32_t n_grains, const double *R, const int32_t *elem_grain) {
+  if (!h || n_grains < 1 || !R) return PGF_EINVAL;
+  if (!h->has_cp) return PGF_ENOTSUP;
+  std::vector<pgf::DevGrain> g(n_grains);
+  for (int32_t k = 0; k < n_grains; ++k)
+    pgf::make_grain(R + 9 * k, h->cp_mat_host[0], h->cp_mat_host[1], h->cp_mat_host[2], g[k]);
+  hipFree(h->d_grains);
+  int rc = pgf::upload(&h->d_grains, g.data(), g.size());
+  if (rc) return rc;
+  if (!elem_grain) return PGF_OK;
+  for (auto &b : h->blocks) {
+    if (b.model != PGF_MODEL_CRYSTAL_PLASTICITY) continue;
+    std::vector<int32_t> eg(b.ne), qg(b.ne * b.nqp);
+    for (int64_t k = 0; k < b.ne; ++k) {
+      const int32_t gk = elem_grain[b.elems[k]];
+      if (gk < 0 || gk >= n_grains) return PGF_EINVAL;
+      eg[k] = gk;
+      for (int ip = 0; ip < b.nqp; ++ip) qg[k * b.nqp + ip] = gk;
+    }
+    if (hipMemcpy(b.d_egrain, eg.data(), eg.size() * sizeof(int32_t), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(b.d_qgrain, qg.data(), qg.size() * sizeof(int32_t), hipMemcpyHostToDevice) != hipSuccess)
+      return PGF_EHIP;
+  }
   return PGF_OK;
 }
 

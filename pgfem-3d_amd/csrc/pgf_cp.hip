@@ -1,0 +1,962 @@
+// Crystal-plasticity kernels for gfx950 (FP64).
+//
+// Constitutive update per QP (the role of CP_PARAM::integration_algorithm,
+// src/plasticity_model.cc:1148-1224, whose return map lives in the un-vendored GCM library):
+//   unknowns x = (M, lambda, g), M = pF(n) pF(n+1)^-1
+//   R_M      = M - I + dt sum_a gdot_a P_a - dt lambda pJ M^-T        (src/integration.cc:54-100)
+//   R_lambda = pJ - 1,  pJ = det(pF(n)) / det(M)                      (:88-95)
+//   R_g      = g (gs - g0 + dt h0 gdot) - (gs - g0) g_n - dt h0 gs gdot (:395-403)
+//   gdot_a   = gdot0 (tau_a/g)|tau_a/g|^(1/m-1), tau_a = (C_e S):P_a,
+//   S = L : (C_e - I)/2, C_e = eF^T eF, eF = F pF(n)^-1 M           (src/TA_GA.cc:8-41)
+// Newton: stage 1 freezes g until the (M, lambda) update is below 1e-6, then g jumps to the
+// closed-form hardening value and stage 2 solves all three; backtracking on half the squared
+// residual norm; converged when the full Newton step is below tol.  The 11x11 system is
+// solved through the Schur complement of its (M, M) block: with the bordered structure
+//   [A b c; d^T 0 0; e^T 0 f]
+// one 9x9 LU (register-resident, pivoting by predicated row swaps so nothing is indexed at
+// run time) serves the two right-hand sides of a Newton step and the nine of the tangent.
+//
+// Kernels:
+//   cp_kinematics  F(n+1) at every QP of a block (gather of the element nodes)
+//   cp_newton      persistent: each wave owns a QP range; a lane that converges takes the
+//                  next QP of its wave at once, so uneven iteration counts do not idle lanes
+//   cp_tangent     at the converged state: S, tau, gdot and dMe/dF (Me = pF(n+1)^-1), for the
+//                  element kernel (qpbuf)
+//   cp_elem        coloured element rows: the reference contraction (compute_stiffness_matrix,
+//                  src/constitutive_model.cc:1520-1616) regrouped per row (a,b) as
+//                  K_ab,wg = pJ w [ z_ab : (e_g (x) dN_w) + (Fr W_ab dN_w)_g + d_bg dN_a.Sh.dN_w ]
+//                  with Z_ab = Cr Me Y + 2 sAA Me S - (B:S) Me^-T, z_ab = Z_ab : dMe/dF,
+//                  B = Me^T sAA Me, Y = L:B, W = Me Y Me^T, Sh = Me S Me^T, Cr = Fr^T Fr.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "pgf_device.h"
+#include "pgf_math.h"
+
+namespace pgf {
+namespace {
+
+using namespace dev;
+
+constexpr int kWave = 64;
+
+// Voigt helpers: index I -> (i,j) in xx,yy,zz,xy,yz,xz
+__device__ __forceinline__ int vi(int I) { return I < 3 ? I : (I == 3 ? 0 : (I == 4 ? 1 : 0)); }
+__device__ __forceinline__ int vj(int I) { return I < 3 ? I : (I == 3 ? 1 : 2); }
+
+__device__ __forceinline__ void mm3(const double *A, const double *B, double *C) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) C[i * 3 + j] = A[i * 3] * B[j] + A[i * 3 + 1] * B[3 + j] + A[i * 3 + 2] * B[6 + j];
+}
+__device__ __forceinline__ void mtm3(const double *A, const double *B, double *C) {  // A^T B
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) C[i * 3 + j] = A[i] * B[j] + A[3 + i] * B[3 + j] + A[6 + i] * B[6 + j];
+}
+// S (Voigt tensor comps) = Lv : E, E symmetric 3x3 given as full 3x3
+__device__ __forceinline__ void lv_dot(const double *Lv, const double *E9, double *S6) {
+  const double e[6] = {E9[0], E9[4], E9[8], 0.5 * (E9[1] + E9[3]), 0.5 * (E9[5] + E9[7]), 0.5 * (E9[2] + E9[6])};
+#pragma unroll
+  for (int I = 0; I < 6; ++I) {
+    double s = 0.0;
+#pragma unroll
+    for (int J = 0; J < 6; ++J) s += Lv[I * 6 + J] * e[J];
+    S6[I] = s;
+  }
+}
+__device__ __forceinline__ void v6_to_9(const double *s, double *S) {
+  S[0] = s[0]; S[4] = s[1]; S[8] = s[2];
+  S[1] = S[3] = s[3];
+  S[5] = S[7] = s[4];
+  S[2] = S[6] = s[5];
+}
+
+// 9x9 LU with partial pivoting; row swaps are predicated so every index is static.
+__device__ __forceinline__ bool lu9(double (&A)[81], int (&piv)[9]) {
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    int p = k;
+    double amax = fabs(A[k * 9 + k]);
+#pragma unroll
+    for (int i = k + 1; i < 9; ++i) {
+      const double v = fabs(A[i * 9 + k]);
+      if (v > amax) {
+        amax = v;
+        p = i;
+      }
+    }
+    if (!(amax > 0.0)) return false;
+    piv[k] = p;
+#pragma unroll
+    for (int i = k + 1; i < 9; ++i) {
+      const bool sw = (i == p);
+#pragma unroll
+      for (int j = 0; j < 9; ++j) {
+        const double t = A[k * 9 + j];
+        A[k * 9 + j] = sw ? A[i * 9 + j] : t;
+        A[i * 9 + j] = sw ? t : A[i * 9 + j];
+      }
+    }
+    const double r = 1.0 / A[k * 9 + k];
+#pragma unroll
+    for (int i = k + 1; i < 9; ++i) {
+      const double l = A[i * 9 + k] * r;
+      A[i * 9 + k] = l;
+#pragma unroll
+      for (int j = k + 1; j < 9; ++j) A[i * 9 + j] -= l * A[k * 9 + j];
+    }
+  }
+  return true;
+}
+__device__ __forceinline__ void lu9_solve(const double (&A)[81], const int (&piv)[9], double (&b)[9]) {
+#pragma unroll
+  for (int k = 0; k < 9; ++k)
+#pragma unroll
+    for (int i = k + 1; i < 9; ++i) {
+      const bool sw = (i == piv[k]);
+      const double t = b[k];
+      b[k] = sw ? b[i] : t;
+      b[i] = sw ? t : b[i];
+    }
+#pragma unroll
+  for (int k = 0; k < 9; ++k)
+#pragma unroll
+    for (int i = k + 1; i < 9; ++i) b[i] -= A[i * 9 + k] * b[k];
+#pragma unroll
+  for (int i = 8; i >= 0; --i) {
+    double s = b[i];
+#pragma unroll
+    for (int j = i + 1; j < 9; ++j) s -= A[i * 9 + j] * b[j];
+    b[i] = s / A[i * 9 + i];
+  }
+}
+
+// Material-point evaluation at x = (M, lambda, g).  Returns false outside the domain.
+struct CPPoint {
+  double Mi[9], pJ, eF[9], Ce[9], S[6], Sig[9], tau[12], gdot[12], dgdt[12], dgdg[12];
+  double gsum, gs, dgs, R[11];
+};
+
+__device__ __forceinline__ bool cp_point(const CPMaterial &cm, const DevGrain &gr, const double *A,
+                                         double det_pFn, double g_n, double dt, const double *x,
+                                         CPPoint &p) {
+  const double *M = x;
+  const double lam = x[9], g = x[10];
+  if (!(g > 0.0)) return false;
+  const double dM = det3(M);
+  if (!(dM > 0.0)) return false;
+  if (!inv3(M, p.Mi)) return false;
+  p.pJ = det_pFn / dM;
+  mm3(A, M, p.eF);
+  mtm3(p.eF, p.eF, p.Ce);
+  double E[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) E[i] = 0.5 * (p.Ce[i] - ((i % 4 == 0) ? 1.0 : 0.0));
+  lv_dot(gr.Lv, E, p.S);
+  double S9[9];
+  v6_to_9(p.S, S9);
+  mm3(p.Ce, S9, p.Sig);
+  const double expo = 1.0 / cm.m - 1.0;
+  p.gsum = 0.0;
+#pragma unroll
+  for (int a = 0; a < 12; ++a) {
+    double t = 0.0;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) t += p.Sig[i] * gr.P[a][i];
+    p.tau[a] = t;
+    const double r = t / g;
+    const double pw = pow(fabs(r), expo);
+    p.gdot[a] = cm.gd0 * r * pw;
+    p.dgdt[a] = cm.gd0 / (cm.m * g) * pw;
+    p.dgdg[a] = -p.dgdt[a] * t / g;
+    p.gsum += fabs(p.gdot[a]);
+  }
+  if (cm.w == 0.0) {
+    p.gs = cm.gs0;
+    p.dgs = 0.0;
+  } else if (p.gsum / cm.gds > 1.0e-15) {
+    p.gs = cm.gs0 * pow(p.gsum / cm.gds, cm.w);
+    p.dgs = cm.w * p.gs / p.gsum;
+  } else {
+    p.gs = 0.0;
+    p.dgs = 0.0;
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      double s = 0.0;
+#pragma unroll
+      for (int a = 0; a < 12; ++a) s += p.gdot[a] * gr.P[a][i * 3 + j];
+      p.R[i * 3 + j] = M[i * 3 + j] - ((i == j) ? 1.0 : 0.0) + dt * s - dt * lam * p.pJ * p.Mi[j * 3 + i];
+    }
+  p.R[9] = p.pJ - 1.0;
+  p.R[10] = g * (p.gs - cm.g0 + dt * cm.h0 * p.gsum) - (p.gs - cm.g0) * g_n - dt * cm.h0 * p.gs * p.gsum;
+  return true;
+}
+
+// d tau_a / d M = 2 A^T eF sym(P_a S + 1/2 L:(C_e P_a))
+__device__ __forceinline__ void dtau_dM(const DevGrain &gr, int a, const double *AtE, const CPPoint &p,
+                                        double *D) {
+  const double *P = gr.P[a];
+  double S9[9], PS[9], CP[9];
+  v6_to_9(p.S, S9);
+  mm3(P, S9, PS);
+  mm3(p.Ce, P, CP);
+  double LCP[6];
+  lv_dot(gr.Lv, CP, LCP);
+  double sQ[9];
+#pragma unroll
+  for (int I = 0; I < 6; ++I) {
+    const int i = vi(I), j = vj(I);
+    const double v = 0.5 * (PS[i * 3 + j] + PS[j * 3 + i]) + 0.5 * LCP[I];
+    sQ[i * 3 + j] = v;
+    sQ[j * 3 + i] = v;
+  }
+  double T[9];
+  mm3(AtE, sQ, T);
+#pragma unroll
+  for (int i = 0; i < 9; ++i) D[i] = 2.0 * T[i];
+}
+
+// Bordered Jacobian [Ah b c; d^T 0 0; e^T 0 f] at p.  With coupled == false the g row and
+// column are replaced by the identity (stage 1).
+struct Border {
+  double b[9], c[9], d[9], e[9], f, dRg;
+};
+
+__device__ __forceinline__ void cp_jacobian(const CPMaterial &cm, const DevGrain &gr, double dt, const double *x,
+                                            const double *A, const CPPoint &p, double g_n, bool coupled,
+                                            double (&Ah)[81], Border &B) {
+  const double lam = x[9], g = x[10];
+  double AtE[9];
+  mtm3(A, p.eF, AtE);
+#pragma unroll
+  for (int i = 0; i < 81; ++i) Ah[i] = 0.0;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    B.c[i] = 0.0;
+    B.e[i] = 0.0;
+  }
+  double sg = 0.0;
+  for (int a = 0; a < 12; ++a) {
+    double D[9];
+    dtau_dM(gr, a, AtE, p, D);
+    const double co = dt * p.dgdt[a];
+    const double sgn = (p.gdot[a] < 0) ? -1.0 : 1.0;
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+      const double pr = gr.P[a][r] * co;
+#pragma unroll
+      for (int c = 0; c < 9; ++c) Ah[r * 9 + c] += pr * D[c];
+      B.c[r] += dt * gr.P[a][r] * p.dgdg[a];
+    }
+#pragma unroll
+    for (int c = 0; c < 9; ++c) B.e[c] += sgn * p.dgdt[a] * D[c];
+    sg += sgn * p.dgdg[a];
+  }
+  const double *Mi = p.Mi;
+  const double lp = dt * lam * p.pJ;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int l = 0; l < 3; ++l)
+          Ah[(i * 3 + j) * 9 + k * 3 + l] +=
+              ((i == k && j == l) ? 1.0 : 0.0) + lp * (Mi[j * 3 + i] * Mi[l * 3 + k] + Mi[l * 3 + i] * Mi[j * 3 + k]);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      B.b[i * 3 + j] = -dt * p.pJ * Mi[j * 3 + i];
+      B.d[i * 3 + j] = -p.pJ * Mi[j * 3 + i];
+    }
+  B.dRg = g * (p.dgs + dt * cm.h0) - p.dgs * g_n - dt * cm.h0 * (p.dgs * p.gsum + p.gs);
+  if (coupled) {
+#pragma unroll
+    for (int c = 0; c < 9; ++c) B.e[c] *= B.dRg;
+    B.f = (p.gs - cm.g0 + dt * cm.h0 * p.gsum) + B.dRg * sg;
+  } else {
+#pragma unroll
+    for (int c = 0; c < 9; ++c) {
+      B.c[c] = 0.0;
+      B.e[c] = 0.0;
+    }
+    B.f = 1.0;
+  }
+  // Schur: Ah <- A - c e^T / f
+  const double rf = 1.0 / B.f;
+#pragma unroll
+  for (int r = 0; r < 9; ++r)
+#pragma unroll
+    for (int c = 0; c < 9; ++c) Ah[r * 9 + c] -= B.c[r] * B.e[c] * rf;
+}
+
+// Solve [A b c; d^T 0 0; e^T 0 f] [u; y; z] = [r; s; t] given LU(Ah) and v = Ah^-1 b.
+__device__ __forceinline__ void bordered_solve(const double (&LU)[81], const int (&piv)[9], const Border &B,
+                                               const double (&v)[9], double dTv, const double *r, double s,
+                                               double t, double *out) {
+  double u[9];
+  const double tf = t / B.f;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) u[i] = r[i] - B.c[i] * tf;
+  lu9_solve(LU, piv, u);
+  double dTu = 0.0;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) dTu += B.d[i] * u[i];
+  const double y = (dTu - s) / dTv;
+  double eTx = 0.0;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    out[i] = u[i] - y * v[i];
+    eTx += B.e[i] * out[i];
+  }
+  out[9] = y;
+  out[10] = (t - eTx) / B.f;
+}
+
+__device__ __forceinline__ double cp_merit(const CPPoint &p, double gscale) {
+  double r = 0.0;
+#pragma unroll
+  for (int i = 0; i < 10; ++i) r += p.R[i] * p.R[i];
+  const double rg = p.R[10] * gscale;
+  return 0.5 * (r + rg * rg);
+}
+
+// ------------------------------------------------------------------- kinematics kernel
+template <int NNE>
+__global__ __launch_bounds__(256) void cp_kinematics(CPArgs a) {
+  constexpr int NQP = nqp_of(NNE);
+  const int64_t tq = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (tq >= a.ne * NQP) return;
+  const int64_t e = tq / NQP;
+  const int ip = int(tq % NQP);
+  double x[NNE * 3], u[NNE * 3];
+#pragma unroll
+  for (int n = 0; n < NNE; ++n) {
+    const int32_t node = a.conn[e * NNE + n];
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      x[n * 3 + b] = a.X[int64_t(node) * 3 + b];
+      const int32_t d = a.dofid[int64_t(node) * 3 + b];
+      double uu = 0.0;  // def_elem_total, src/utils.cc:1391-1412
+      if (d > 0) uu = a.r[d - 1] + (a.d_r ? a.d_r[d - 1] : 0.0);
+      else if (d < 0) uu = (a.defl ? a.defl[-d - 1] : 0.0) + (a.defl_d ? a.defl_d[-d - 1] : 0.0);
+      u[n * 3 + b] = uu;
+    }
+  }
+  double k, ee, z, wt;
+  int_point(NNE, ip, k, ee, z, wt);
+  double dN[NNE * 3];
+  shape_dN<NNE>(k, ee, z, dN);
+  double dX[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+  for (int n = 0; n < NNE; ++n)
+#pragma unroll
+    for (int ib = 0; ib < 3; ++ib)
+#pragma unroll
+      for (int ic = 0; ic < 3; ++ic) dX[ib * 3 + ic] += dN[n * 3 + ic] * x[n * 3 + ib];
+  double Xi[9];
+  const bool ok = det3(dX) > 0.0 && inv3(dX, Xi);
+  double F[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+#pragma unroll
+  for (int n = 0; n < NNE; ++n) {
+    double g[3];
+#pragma unroll
+    for (int ib = 0; ib < 3; ++ib)
+      g[ib] = dN[n * 3 + 0] * Xi[0 * 3 + ib] + dN[n * 3 + 1] * Xi[1 * 3 + ib] + dN[n * 3 + 2] * Xi[2 * 3 + ib];
+#pragma unroll
+    for (int ib = 0; ib < 3; ++ib)
+#pragma unroll
+      for (int ic = 0; ic < 3; ++ic) F[ib * 3 + ic] += u[n * 3 + ib] * g[ic];
+  }
+  const int64_t q = a.qp_base + tq;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) a.F_np1[i * a.n_qp + q] = F[i];
+  if (!ok || !(det3(F) > 0.0)) atomicAdd(a.err, 1);  // getJacobian, src/utils.cc:513-527
+}
+
+// ------------------------------------------------------------------- Newton kernel
+// One wave per workgroup; wave w owns QPs [w*chunk, (w+1)*chunk) of the block.
+__global__ __launch_bounds__(64) void cp_newton(CPArgs a, int64_t chunk) {
+  const int lane = threadIdx.x;
+  const int64_t q_lo = int64_t(blockIdx.x) * chunk;
+  const int64_t q_hi = (q_lo + chunk < a.nq) ? q_lo + chunk : a.nq;
+  int64_t cursor = q_lo;  // wave-uniform
+  const int64_t NQ = a.n_qp;
+  bool need = true, done = false;
+  int64_t q = -1;
+  double x[11], A[9], det_pFn = 1.0, g_n = 1.0, gscale = 1.0, dt = a.dt;
+  int it = 0, gid = 0;
+  bool coupled = false;
+  CPMaterial cm{};
+  for (;;) {
+    // refill the lanes that need a QP, in lane order, from the wave's range
+    const uint64_t mask = __ballot(need && !done);
+    const int nneed = __popcll(mask);
+    if (need && !done) {
+      const int rank = __popcll(mask & ((uint64_t(1) << lane) - 1));
+      const int64_t cand = cursor + rank;
+      if (cand >= q_hi) {
+        done = true;
+      } else {
+        q = cand;
+        need = false;
+        const int64_t qg = a.qp_base + q;
+        double F[9], pFn[9], pFi[9];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+          F[i] = a.F_np1[i * NQ + qg];
+          pFn[i] = a.pF_n[i * NQ + qg];
+        }
+        inv3(pFn, pFi);
+        mm3(F, pFi, A);
+        det_pFn = det3(pFn);
+        g_n = a.g_n[qg];
+        gid = a.qgrain[q];
+        cm = a.cpmats[0];
+        gscale = 1.0 / (fabs(cm.gs0 - cm.g0) * g_n + 1e-300);
+        it = 0;
+        // start: caller's warm start, else the slip rates of the last converged step, else the
+        // elastic predictor (M = I) with the staged Newton
+        bool have = false;
+        if (a.warm) {
+#pragma unroll
+          for (int i = 0; i < 11; ++i) x[i] = a.warm[i * NQ + qg];
+          have = true;
+        } else {
+          double any = 0.0;
+#pragma unroll
+          for (int al = 0; al < 12; ++al) any += fabs(a.gdot_n[al * NQ + qg]);
+          const DevGrain &gr = a.grains[gid];
+#pragma unroll
+          for (int i = 0; i < 9; ++i) {
+            double s = 0.0;
+            for (int al = 0; al < 12; ++al) s += a.gdot_n[al * NQ + qg] * gr.P[al][i];
+            x[i] = ((i % 4 == 0) ? 1.0 : 0.0) - dt * s;
+          }
+          have = any > 0.0;
+          if (!have) {
+#pragma unroll
+            for (int i = 0; i < 9; ++i) x[i] = (i % 4 == 0) ? 1.0 : 0.0;
+          }
+          x[9] = a.lam_n[qg];
+          x[10] = g_n;
+        }
+        coupled = have;
+      }
+    }
+    cursor += nneed;
+    if (__all(done)) break;
+    if (done || need) continue;
+
+    // ---- one Newton iteration of this lane's QP
+    const DevGrain &gr = a.grains[gid];
+    const int64_t qg = a.qp_base + q;
+    ++it;
+    bool fail = false, conv = false;
+    CPPoint p;
+    if (!cp_point(cm, gr, A, det_pFn, g_n, dt, x, p)) {
+      fail = true;
+    } else {
+      double Ah[81];
+      Border B;
+      cp_jacobian(cm, gr, dt, x, A, p, g_n, coupled, Ah, B);
+      int piv[9];
+      if (!lu9(Ah, piv)) {
+        fail = true;
+      } else {
+        double v[9];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) v[i] = B.b[i];
+        lu9_solve(Ah, piv, v);
+        double dTv = 0.0;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) dTv += B.d[i] * v[i];
+        double r9[9];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) r9[i] = -p.R[i];
+        double dx[11];
+        bordered_solve(Ah, piv, B, v, dTv, r9, -p.R[9], coupled ? -p.R[10] : 0.0, dx);
+        double du = 0.0;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) du = fmax(du, fabs(dx[i]));
+        du = fmax(du, fabs(dx[9]) * dt);
+        du = fmax(du, fabs(dx[10]) / g_n);
+        if (coupled && du < cm.tol) {
+#pragma unroll
+          for (int i = 0; i < 11; ++i) x[i] += dx[i];
+          conv = true;
+        } else {
+          const double gsu = coupled ? gscale : 0.0;
+          const double f0 = cp_merit(p, gsu);
+          double alpha = 1.0, xn[11];
+          bool ok = false;
+          CPPoint pn;
+          for (int ls = 0; ls < 40; ++ls) {
+#pragma unroll
+            for (int i = 0; i < 11; ++i) xn[i] = x[i] + alpha * dx[i];
+            if (cp_point(cm, gr, A, det_pFn, g_n, dt, xn, pn) && cp_merit(pn, gsu) <= (1.0 - 1e-4 * alpha) * f0) {
+              ok = true;
+              break;
+            }
+            alpha *= 0.5;
+          }
+          if (!ok) {
+#pragma unroll
+            for (int i = 0; i < 11; ++i) xn[i] = x[i] + dx[i];
+            if (!cp_point(cm, gr, A, det_pFn, g_n, dt, xn, pn)) fail = true;
+          }
+          if (!fail) {
+#pragma unroll
+            for (int i = 0; i < 11; ++i) x[i] = xn[i];
+            if (!coupled && du < 1.0e-6) {  // stage 1 done: closed-form hardening update
+              const double num = (pn.gs - cm.g0) * g_n + dt * cm.h0 * pn.gs * pn.gsum;
+              const double den = pn.gs - cm.g0 + dt * cm.h0 * pn.gsum;
+              x[10] = num / den;
+              coupled = true;
+            }
+          }
+        }
+      }
+    }
+    if (!conv && !fail && it >= cm.maxit) fail = true;
+    if (conv || fail) {
+      a.iters[qg] = it;
+      if (fail) {
+        atomicAdd(a.err, 1);
+      } else {
+        // pF(n+1) = M^-1 pF(n)
+        double Mi[9], pFn[9], pF1[9];
+        inv3(x, Mi);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) pFn[i] = a.pF_n[i * NQ + qg];
+        mm3(Mi, pFn, pF1);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) a.pF_np1[i * NQ + qg] = pF1[i];
+        a.Sc_np1[qg] = x[10];
+        a.Sc_np1[NQ + qg] = x[9];
+      }
+      need = true;
+    }
+  }
+}
+
+// ------------------------------------------------------------------- tangent kernel
+// At the stored converged state (F, pF(n), pF(n+1), g, lambda): S, tau, gdot, Me, pJ and,
+// with WITH_D, dMe/dF = pF(n)^-1 dM/dF from the implicit derivative of the local system.
+template <bool WITH_D>
+__global__ __launch_bounds__(64) void cp_tangent(CPArgs a, int64_t nqb) {
+  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= nqb) return;
+  const int64_t NQ = a.n_qp;
+  const int64_t qg = a.qp_base + t;
+  const CPMaterial cm = a.cpmats[0];
+  const DevGrain &gr = a.grains[a.qgrain[t]];
+  double F[9], pFn[9], pF1[9], pFi[9], A[9], Me[9], x[11];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    F[i] = a.F_np1[i * NQ + qg];
+    pFn[i] = a.pF_n[i * NQ + qg];
+    pF1[i] = a.pF_np1[i * NQ + qg];
+  }
+  bool ok = inv3(pFn, pFi) && inv3(pF1, Me);
+  mm3(F, pFi, A);
+  mm3(pFn, Me, x);  // M = pF(n) pF(n+1)^-1
+  x[9] = a.Sc_np1[NQ + qg];
+  x[10] = a.Sc_np1[qg];
+  const double g_n = a.g_n[qg];
+  CPPoint p;
+  ok = ok && cp_point(cm, gr, A, det3(pFn), g_n, a.dt, x, p);
+  double *out = a.qpbuf + t * kCPBuf;
+  if (!ok) {
+    atomicAdd(a.err, 1);
+#pragma unroll
+    for (int i = 0; i < kCPBuf; ++i) out[i] = 0.0;
+    return;
+  }
+#pragma unroll
+  for (int al = 0; al < 12; ++al) {
+    a.tau[al * NQ + qg] = p.tau[al];
+    a.gdot[al * NQ + qg] = p.gdot[al];
+  }
+  // elastic PK2 from the stored state (eF = F pF(n+1)^-1)
+  double eF[9], Ce[9], E[9], S[6];
+  mm3(F, Me, eF);
+  mtm3(eF, eF, Ce);
+#pragma unroll
+  for (int i = 0; i < 9; ++i) E[i] = 0.5 * (Ce[i] - ((i % 4 == 0) ? 1.0 : 0.0));
+  lv_dot(gr.Lv, E, S);
+#pragma unroll
+  for (int i = 0; i < 9; ++i) out[i] = Me[i];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) out[9 + i] = S[i];
+  out[15] = det3(pF1);
+  if constexpr (WITH_D) {
+    double Ah[81];
+    Border B;
+    cp_jacobian(cm, gr, a.dt, x, A, p, g_n, true, Ah, B);
+    int piv[9];
+    if (!lu9(Ah, piv)) {
+      atomicAdd(a.err, 1);
+      return;
+    }
+    double v[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) v[i] = B.b[i];
+    lu9_solve(Ah, piv, v);
+    double dTv = 0.0;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) dTv += B.d[i] * v[i];
+    // dtau_a/dF = A^-T Dtau_a (pF(n)^-1 M)^T
+    double Ai[9], AiT[9], BM[9], AtE[9];
+    inv3(A, Ai);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) AiT[i * 3 + j] = Ai[j * 3 + i];
+    mm3(pFi, x, BM);
+    mtm3(A, p.eF, AtE);
+    double rhs[9][9];  // [F column][row of R_M]
+    double rg[9];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) {
+      rg[c] = 0.0;
+#pragma unroll
+      for (int r = 0; r < 9; ++r) rhs[c][r] = 0.0;
+    }
+    for (int al = 0; al < 12; ++al) {
+      double D[9], T[9], dtdF[9];
+      dtau_dM(gr, al, AtE, p, D);
+      mm3(AiT, D, T);
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+          dtdF[i * 3 + j] = T[i * 3 + 0] * BM[j * 3 + 0] + T[i * 3 + 1] * BM[j * 3 + 1] + T[i * 3 + 2] * BM[j * 3 + 2];
+      const double co = -a.dt * p.dgdt[al];
+      const double sg = ((p.gdot[al] < 0) ? -1.0 : 1.0) * p.dgdt[al];
+#pragma unroll
+      for (int c = 0; c < 9; ++c) {
+#pragma unroll
+        for (int r = 0; r < 9; ++r) rhs[c][r] += co * gr.P[al][r] * dtdF[c];
+        rg[c] -= sg * dtdF[c];
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 9; ++c) {
+      double sol[11];
+      bordered_solve(Ah, piv, B, v, dTv, rhs[c], 0.0, B.dRg * rg[c], sol);
+      // dMe/dF[:, c] = pF(n)^-1 dM/dF[:, c]
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+          out[16 + (i * 3 + j) * 9 + c] = pFi[i * 3 + 0] * sol[0 * 3 + j] + pFi[i * 3 + 1] * sol[1 * 3 + j] +
+                                          pFi[i * 3 + 2] * sol[2 * 3 + j];
+    }
+  }
+}
+
+// ------------------------------------------------------------------- element kernel
+template <int NNE, int MODE>
+__global__ __launch_bounds__(256) void cp_elem(CPElemArgs a, int64_t e_begin, int64_t e_end) {
+  constexpr int NQP = nqp_of(NNE);
+  constexpr int NDE = NNE * 3;
+  constexpr int TPE = (NNE == 4) ? 16 : 32;
+  constexpr int EPB = (NNE == 8) ? 4 : 256 / TPE;
+  constexpr bool RES = (MODE & MODE_RESIDUAL) != 0;
+  constexpr bool TAN = (MODE & MODE_TANGENT) != 0;
+  __shared__ double s_x[EPB][NDE], s_u[EPB][NDE];
+  __shared__ int32_t s_dof[EPB][NDE], s_node[EPB][NNE];
+  __shared__ double s_dN[EPB][NQP][NDE];
+  __shared__ double s_Fr[EPB][NQP][9], s_Me[EPB][NQP][9], s_MeS[EPB][NQP][9], s_CrMe[EPB][NQP][9];
+  __shared__ double s_MiT[EPB][NQP][9], s_Sh[EPB][NQP][6], s_S[EPB][NQP][6], s_w[EPB][NQP];
+  __shared__ double s_D[EPB][NQP][TAN ? 81 : 1];
+
+  const int slot = threadIdx.x / TPE, lane = threadIdx.x % TPE;
+  const int64_t e = e_begin + int64_t(blockIdx.x) * EPB + slot;
+  const bool active = (threadIdx.x < EPB * TPE) && e < e_end;
+  if (active && lane < NNE) {
+    const int32_t node = a.conn[e * NNE + lane];
+    s_node[slot][lane] = node;
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      s_x[slot][lane * 3 + b] = a.X[int64_t(node) * 3 + b];
+      const int32_t d = a.dofid[int64_t(node) * 3 + b];
+      s_dof[slot][lane * 3 + b] = d;
+      double u = 0.0;
+      if (d > 0) u = a.r[d - 1] + (a.d_r ? a.d_r[d - 1] : 0.0);
+      else if (d < 0) u = (a.defl ? a.defl[-d - 1] : 0.0) + (a.defl_d ? a.defl_d[-d - 1] : 0.0);
+      s_u[slot][lane * 3 + b] = u;
+    }
+  }
+  __syncthreads();
+  if (active && lane < NQP) {
+    const int ip = lane;
+    double k, ee, z, wt;
+    int_point(NNE, ip, k, ee, z, wt);
+    double dN[NDE];
+    shape_dN<NNE>(k, ee, z, dN);
+    double dX[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int n = 0; n < NNE; ++n)
+#pragma unroll
+      for (int ib = 0; ib < 3; ++ib)
+#pragma unroll
+        for (int ic = 0; ic < 3; ++ic) dX[ib * 3 + ic] += dN[n * 3 + ic] * s_x[slot][n * 3 + ib];
+    const double J0 = det3(dX);
+    double Xi[9];
+    const bool ok = J0 > 0.0 && inv3(dX, Xi);
+    if (!ok) atomicAdd(a.err, 1);
+    double Fr[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+#pragma unroll
+    for (int n = 0; n < NNE; ++n) {
+      double g[3];
+#pragma unroll
+      for (int ib = 0; ib < 3; ++ib)
+        g[ib] = dN[n * 3 + 0] * Xi[0 * 3 + ib] + dN[n * 3 + 1] * Xi[1 * 3 + ib] + dN[n * 3 + 2] * Xi[2 * 3 + ib];
+#pragma unroll
+      for (int ib = 0; ib < 3; ++ib) {
+        s_dN[slot][ip][n * 3 + ib] = g[ib];
+#pragma unroll
+        for (int ic = 0; ic < 3; ++ic) Fr[ib * 3 + ic] += s_u[slot][n * 3 + ib] * g[ic];
+      }
+    }
+    const double *qb = a.qpbuf + (e * NQP + ip) * kCPBuf;
+    double Me[9], S9[9], Mi[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Me[i] = qb[i];
+    double S6[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) S6[i] = qb[9 + i];
+    v6_to_9(S6, S9);
+    const double pJ = qb[15];
+    inv3(Me, Mi);
+    double MeS[9], Sh9[9], Cr[9], CrMe[9];
+    mm3(Me, S9, MeS);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+        Sh9[i * 3 + j] = MeS[i * 3 + 0] * Me[j * 3 + 0] + MeS[i * 3 + 1] * Me[j * 3 + 1] + MeS[i * 3 + 2] * Me[j * 3 + 2];
+    mtm3(Fr, Fr, Cr);
+    mm3(Cr, Me, CrMe);
+    s_w[slot][ip] = ok ? J0 * wt * pJ : 0.0;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+      s_Fr[slot][ip][i] = Fr[i];
+      s_Me[slot][ip][i] = Me[i];
+      s_MeS[slot][ip][i] = MeS[i];
+      s_CrMe[slot][ip][i] = CrMe[i];
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) s_MiT[slot][ip][i * 3 + j] = Mi[j * 3 + i];
+#pragma unroll
+    for (int I = 0; I < 6; ++I) {
+      s_S[slot][ip][I] = S6[I];
+      s_Sh[slot][ip][I] = Sh9[vi(I) * 3 + vj(I)];
+    }
+    if constexpr (TAN) {
+#pragma unroll
+      for (int i = 0; i < 81; ++i) s_D[slot][ip][i] = qb[16 + i];
+    }
+  }
+  __syncthreads();
+  if (!(active && lane < NDE)) return;
+  const int ra = lane / 3, rb = lane % 3;
+  const double eb[3] = {rb == 0 ? 1.0 : 0.0, rb == 1 ? 1.0 : 0.0, rb == 2 ? 1.0 : 0.0};
+  const DevGrain &gr = a.grains[a.grain[e]];
+  double K[TAN ? NDE : 1];
+  if constexpr (TAN) {
+#pragma unroll
+    for (int i = 0; i < NDE; ++i) K[i] = 0.0;
+  }
+  double fe = 0.0;
+#pragma unroll
+  for (int ip = 0; ip < NQP; ++ip) {
+    const double wq = s_w[slot][ip];
+    const double *dN = s_dN[slot][ip];
+    const double *Fr = s_Fr[slot][ip];
+    const double ga[3] = {dN[ra * 3], dN[ra * 3 + 1], dN[ra * 3 + 2]};
+    double Sh9[9];
+    v6_to_9(s_Sh[slot][ip], Sh9);
+    double ta[3];  // Sh dN_a
+#pragma unroll
+    for (int i = 0; i < 3; ++i) ta[i] = Sh9[i * 3] * ga[0] + Sh9[i * 3 + 1] * ga[1] + Sh9[i * 3 + 2] * ga[2];
+    if constexpr (RES) fe += wq * (Fr[rb * 3] * ta[0] + Fr[rb * 3 + 1] * ta[1] + Fr[rb * 3 + 2] * ta[2]);
+    if constexpr (TAN) {
+      const double *Me = s_Me[slot][ip];
+      // sAA = sym(Fr^T (e_b (x) dN_a)): sAA_ij = (Fr_bi g_j + Fr_bj g_i)/2
+      double sAA[9];
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) sAA[i * 3 + j] = 0.5 * (Fr[rb * 3 + i] * ga[j] + Fr[rb * 3 + j] * ga[i]);
+      double T[9], Bm[9];
+      mm3(sAA, Me, T);
+      mtm3(Me, T, Bm);  // Me^T sAA Me
+      double Y6[6];
+      lv_dot(gr.Lv, Bm, Y6);
+      double Y9[9];
+      v6_to_9(Y6, Y9);
+      double BS = 0.0;
+      {
+        const double *S6 = s_S[slot][ip];
+        BS = Bm[0] * S6[0] + Bm[4] * S6[1] + Bm[8] * S6[2] +
+             2.0 * (0.5 * (Bm[1] + Bm[3]) * S6[3] + 0.5 * (Bm[5] + Bm[7]) * S6[4] + 0.5 * (Bm[2] + Bm[6]) * S6[5]);
+      }
+      // Z = CrMe Y + 2 sAA MeS - (B:S) Me^-T
+      double Z[9], T2[9];
+      mm3(s_CrMe[slot][ip], Y9, Z);
+      mm3(sAA, s_MeS[slot][ip], T2);
+      const double *MiT = s_MiT[slot][ip];
+#pragma unroll
+      for (int i = 0; i < 9; ++i) Z[i] += 2.0 * T2[i] - BS * MiT[i];
+      // z[g][L] = sum_ij Z_ij D[ij][g L]
+      const double *D = s_D[slot][ip];
+      double zz[9];
+#pragma unroll
+      for (int c = 0; c < 9; ++c) {
+        double s = 0.0;
+#pragma unroll
+        for (int r = 0; r < 9; ++r) s += Z[r] * D[r * 9 + c];
+        zz[c] = s;
+      }
+      // W = Me Y Me^T, FrW = Fr W
+      double T3[9], W[9], FrW[9];
+      mm3(Me, Y9, T3);
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+          W[i * 3 + j] = T3[i * 3 + 0] * Me[j * 3 + 0] + T3[i * 3 + 1] * Me[j * 3 + 1] + T3[i * 3 + 2] * Me[j * 3 + 2];
+      mm3(Fr, W, FrW);
+#pragma unroll
+      for (int i = 0; i < 9; ++i) zz[i] = wq * (zz[i] + FrW[i]);
+      const double wt0 = wq * ta[0], wt1 = wq * ta[1], wt2 = wq * ta[2];
+#pragma unroll
+      for (int w = 0; w < NNE; ++w) {
+        const double h0 = dN[w * 3], h1 = dN[w * 3 + 1], h2 = dN[w * 3 + 2];
+        const double geo = wt0 * h0 + wt1 * h1 + wt2 * h2;
+#pragma unroll
+        for (int g = 0; g < 3; ++g) K[w * 3 + g] += zz[g * 3] * h0 + zz[g * 3 + 1] * h1 + zz[g * 3 + 2] * h2 + eb[g] * geo;
+      }
+    }
+  }
+  if constexpr (TAN) {
+    double diag = 0.0;
+#pragma unroll
+    for (int i = 0; i < NDE; ++i) diag = (i == lane) ? K[i] : diag;
+    if (!isnormal(diag)) atomicAdd(a.err, 1);
+  }
+  const int32_t dof = s_dof[slot][lane];
+  if (dof <= 0) return;
+  const int16_t *off = a.off + (e * NNE + ra) * NNE;
+  if constexpr (RES) {
+    if (off[ra] <= -2) a.f[dof - 1] = fe;
+    else a.f[dof - 1] += fe;
+  }
+  if constexpr (TAN) {
+    const int32_t node = s_node[slot][ra];
+    int bp = 0;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) bp += (j < rb && s_dof[slot][ra * 3 + j] > 0) ? 1 : 0;
+    double *row = a.vals + (int64_t(a.row_start[node]) + int64_t(bp) * a.rowlen[node]);
+    constexpr int H1 = (NNE + 1) / 2;
+#pragma unroll
+    for (int half = 0; half < 2; ++half) {
+      const int w0 = half ? H1 : 0, w1 = half ? NNE : H1;
+      int pos[3 * H1];
+      double old[3 * H1];
+#pragma unroll
+      for (int w = w0; w < w1; ++w) {
+        const int o = off[w];
+        const bool first = o <= -2;
+        const int base = first ? (-o - 2) : o;
+        int gp = 0;
+#pragma unroll
+        for (int gg = 0; gg < 3; ++gg) {
+          const bool fr = (o != -1) && s_dof[slot][w * 3 + gg] > 0;
+          pos[(w - w0) * 3 + gg] = fr ? base + gp : -1;
+          gp += fr ? 1 : 0;
+          old[(w - w0) * 3 + gg] = (fr && !first) ? row[base + gp - 1] : 0.0;
+        }
+      }
+#pragma unroll
+      for (int w = w0; w < w1; ++w)
+#pragma unroll
+        for (int gg = 0; gg < 3; ++gg) {
+          const int pp = pos[(w - w0) * 3 + gg];
+          if (pp >= 0) row[pp] = old[(w - w0) * 3 + gg] + K[w * 3 + gg];
+        }
+    }
+  }
+}
+
+template <int NNE>
+void launch_kin(const CPArgs &a, hipStream_t s) {
+  const int64_t n = a.ne * nqp_of(NNE);
+  if (n <= 0) return;
+  cp_kinematics<NNE><<<unsigned((n + 255) / 256), 256, 0, s>>>(a);
+}
+
+template <int NNE, int MODE>
+void launch_ce(const CPElemArgs &a, int64_t e0, int64_t e1, hipStream_t s) {
+  constexpr int TPE = (NNE == 4) ? 16 : 32;
+  constexpr int EPB = (NNE == 8) ? 4 : 256 / TPE;
+  const int64_t n = e1 - e0;
+  if (n <= 0) return;
+  cp_elem<NNE, MODE><<<unsigned((n + EPB - 1) / EPB), EPB * TPE, 0, s>>>(a, e0, e1);
+}
+
+template <int NNE>
+void launch_ce_mode(int mode, const CPElemArgs &a, int64_t e0, int64_t e1, hipStream_t s) {
+  switch (mode & (MODE_RESIDUAL | MODE_TANGENT)) {
+    case MODE_RESIDUAL: launch_ce<NNE, MODE_RESIDUAL>(a, e0, e1, s); break;
+    case MODE_TANGENT: launch_ce<NNE, MODE_TANGENT>(a, e0, e1, s); break;
+    case MODE_RESIDUAL | MODE_TANGENT: launch_ce<NNE, MODE_RESIDUAL | MODE_TANGENT>(a, e0, e1, s); break;
+    default: break;
+  }
+}
+
+}  // namespace
+
+void launch_cp_kinematics(int nne, const CPArgs &a, hipStream_t s) {
+  if (nne == 4) launch_kin<4>(a, s);
+  else if (nne == 10) launch_kin<10>(a, s);
+  else if (nne == 8) launch_kin<8>(a, s);
+}
+
+void launch_cp_newton(int nne, const CPArgs &a, hipStream_t s) {
+  (void)nne;
+  const int64_t nq = a.nq;
+  if (nq <= 0) return;
+  const int64_t chunk = 256;
+  cp_newton<<<unsigned((nq + chunk - 1) / chunk), 64, 0, s>>>(a, chunk);
+}
+
+void launch_cp_tangent(int nne, const CPArgs &a, bool with_D, hipStream_t s) {
+  (void)nne;
+  const int64_t nq = a.nq;
+  if (nq <= 0) return;
+  if (with_D) cp_tangent<true><<<unsigned((nq + 63) / 64), 64, 0, s>>>(a, nq);
+  else cp_tangent<false><<<unsigned((nq + 63) / 64), 64, 0, s>>>(a, nq);
+}
+
+void launch_cp_elem(int nne, int mode, const CPElemArgs &a, int64_t e0, int64_t e1, hipStream_t s) {
+  if (nne == 4) launch_ce_mode<4>(mode, a, e0, e1, s);
+  else if (nne == 10) launch_ce_mode<10>(mode, a, e0, e1, s);
+  else if (nne == 8) launch_ce_mode<8>(mode, a, e0, e1, s);
+}
+
+}  // namespace pgf

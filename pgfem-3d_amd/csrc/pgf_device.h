@@ -51,6 +51,67 @@ struct ElemArgs {
 // Mode bits of the element kernel.
 enum : int { MODE_INTEGRATE = 1, MODE_RESIDUAL = 2, MODE_TANGENT = 4 };
 
+// ---------------------------------------------------------------- crystal plasticity
+// Per-grain tables: Schmid tensors P_a = s_a (x) m_a rotated to the sample frame (12 x 9,
+// row-major) and the rotated cubic stiffness in Voigt form Lv[I][J] (I, J in xx,yy,zz,xy,yz,
+// xz) such that S_I = sum_J Lv[I][J] E_J with E_J the tensor components of a symmetric E.
+struct DevGrain {
+  double P[12][9];
+  double Lv[36];
+};
+
+struct CPMaterial {
+  double gd0, m, h0, g0, gs0, gds, w, tol;
+  int32_t maxit, pad;
+};
+
+// Arguments shared by the crystal-plasticity QP kernels (device pointers, SoA [comp][n_qp]).
+struct CPArgs {
+  // kinematics input
+  const double *X;
+  const int32_t *dofid;
+  const int32_t *conn;      // block connectivity
+  const int32_t *qgrain;    // [nq] grain id of the block's QPs
+  const DevGrain *grains;
+  const CPMaterial *cpmats;
+  int64_t qp_base, n_qp, ne, nq;  // block QP offset, total QPs, block elements, block QPs
+  const double *r, *d_r, *defl, *defl_d;
+  double dt;
+  // state
+  double *F_np1;
+  const double *pF_n;
+  double *pF_np1;
+  const double *g_n, *lam_n;  // Sc_n comps 0 and 1
+  double *Sc_np1;             // g, lambda at n+1
+  const double *gdot_n;       // 12 comps (plain start)
+  double *tau, *gdot;         // 12 comps at n+1
+  int32_t *iters;             // [n_qp]
+  const double *warm;         // optional start (M 9, lambda, g) [11][n_qp]
+  // tangent-pass output for the element kernel: Me 9, S 6, pJ 1, D = dMe/dF 81 (block-local)
+  double *qpbuf;
+  int32_t *err;
+  int32_t *queue;             // work-queue counter
+};
+constexpr int kCPBuf = 97;  // doubles per QP in qpbuf
+
+void launch_cp_kinematics(int nne, const CPArgs &a, hipStream_t s);
+void launch_cp_newton(int nne, const CPArgs &a, hipStream_t s);
+void launch_cp_tangent(int nne, const CPArgs &a, bool with_D, hipStream_t s);
+
+// Element kernel of the crystal-plasticity blocks: residual and / or tangent from qpbuf.
+struct CPElemArgs {
+  const double *X;
+  const int32_t *dofid, *row_start, *rowlen, *conn, *grain;
+  const int16_t *off;
+  const DevGrain *grains;
+  const double *qpbuf;
+  int64_t qp_base;
+  const double *r, *d_r, *defl, *defl_d;
+  double *f, *vals;
+  int32_t *err;
+};
+void launch_cp_elem(int nne, int mode, const CPElemArgs &a, int64_t e_begin, int64_t e_end, hipStream_t s);
+
 void launch_elem(int nne, int mode, const ElemArgs &a, int64_t e_begin, int64_t e_end,
                  hipStream_t s);
 void launch_fill(double *p, int64_t n, double v, hipStream_t s);

@@ -27,6 +27,7 @@ MODEL_HYPER_ELASTICITY, MODEL_CRYSTAL_PLASTICITY, MODEL_J2_PLASTICITY = 0, 1, 4
 STATE_F_NP1, STATE_F_N, STATE_F_NM1 = 0, 1, 2
 STATE_PLASTIC_NP1, STATE_PLASTIC_N, STATE_PLASTIC_NM1 = 3, 4, 5
 STATE_SCALAR_NP1, STATE_SCALAR_N, STATE_SCALAR_NM1 = 6, 7, 8
+STATE_TAU, STATE_TAU_N, STATE_GDOT, STATE_GDOT_N, STATE_ITERS = 9, 10, 11, 12, 13
 
 ERRORS = {-1: "PGF_EINVAL", -2: "PGF_ENOMEM", -3: "PGF_EHIP", -4: "PGF_ENOTSUP", -5: "PGF_ENODEV"}
 
@@ -91,6 +92,11 @@ def lib() -> C.CDLL:
             "pgf_state_ncomp": [P, I32, P],
             "pgf_get_state": [P, I32, P],
             "pgf_set_state": [P, I32, P],
+            "pgf_set_orientations": [P, I32, P, P],
+            "pgf_set_time_step": [P, D],
+            "pgf_set_warm_start": [P, P],
+            "pgf_get_qp_order": [P, P],
+            "pgf_state_device_ptr": [P, I32, P],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -106,7 +112,8 @@ def exported_symbols():
         "pgf_generate_dof_ids", "pgf_psparse_apai", "pgf_mesh_box", "pgf_create", "pgf_destroy",
         "pgf_set_stream", "pgf_set_profiling", "pgf_last_pass_ms", "pgf_synchronize", "pgf_get_sizes", "pgf_residual", "pgf_tangent",
         "pgf_residual_tangent", "pgf_commit_step", "pgf_reset_step", "pgf_state_ncomp",
-        "pgf_get_state", "pgf_set_state",
+        "pgf_get_state", "pgf_set_state", "pgf_set_orientations", "pgf_set_time_step",
+        "pgf_set_warm_start", "pgf_get_qp_order", "pgf_state_device_ptr",
     ]
 
 
@@ -314,6 +321,36 @@ class Handle:
     def set_state(self, field: int, values: np.ndarray):
         v = np.ascontiguousarray(values, dtype=np.float64)
         _check(self._lib.pgf_set_state(self.h, field, _ptr(v)), "pgf_set_state")
+
+    def state_device_ptr(self, field: int) -> int:
+        """Device address of a state field ([ncomp][n_qp], device QP order)."""
+        p = C.c_void_p()
+        _check(self._lib.pgf_state_device_ptr(self.h, field, C.byref(p)), "pgf_state_device_ptr")
+        return p.value
+
+    def qp_order(self) -> np.ndarray:
+        """mesh QP index (element-major, QP-minor) of every device QP."""
+        out = np.zeros(self.n_qp, dtype=np.int64)
+        _check(self._lib.pgf_get_qp_order(self.h, _ptr(out)), "pgf_get_qp_order")
+        return out
+
+    def set_orientations(self, R: np.ndarray, elem_grain: np.ndarray | None):
+        """Per-grain rotations (crystal -> sample, [n_grains,3,3]) and the grain of each element."""
+        Rm = np.ascontiguousarray(R, dtype=np.float64).reshape(-1, 9)
+        eg = None if elem_grain is None else np.ascontiguousarray(elem_grain, dtype=np.int32)
+        _check(self._lib.pgf_set_orientations(self.h, Rm.shape[0], _ptr(Rm), _ptr(eg)), "pgf_set_orientations")
+        self._grain_keep = (Rm, eg)
+
+    def set_time_step(self, dt: float):
+        _check(self._lib.pgf_set_time_step(self.h, float(dt)), "pgf_set_time_step")
+
+    def set_warm_start(self, warm):
+        """Start the next integration's local Newton from warm ([11][n_qp] float64 GPU tensor in
+        device QP order: M row-major, lambda, g); None returns to the plain start."""
+        if warm is not None:
+            self._check_vec(warm, 11 * self.n_qp, "warm")
+        self._warm = warm
+        _check(self._lib.pgf_set_warm_start(self.h, _ptr(warm)), "pgf_set_warm_start")
 
 
 # ------------------------------------------------------------------ crystal plasticity inputs

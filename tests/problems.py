@@ -28,3 +28,19 @@ class BoxProblem:
         mask = self.ids > 0
         r[self.ids[mask] - 1] = u[mask]
         return r, np.array([stretch])
+
+
+class CPBoxProblem(BoxProblem):
+    """Box mesh with crystal plasticity on Voronoi grains of random orientation."""
+
+    def __init__(self, n, etype, material, n_grains=3, jitter=0.1, seed=1):
+        super().__init__(n, etype, material, jitter, seed)
+        self.grain, q = P.voronoi_grains(self.mesh, n_grains, seed)
+        self.R = P.quat_to_rot(q)
+        tabs = [O.cp_grain(material, self.R[g]) for g in range(n_grains)]
+        self.gP = np.ascontiguousarray(np.stack([t[0] for t in tabs]).reshape(n_grains, 108))
+        self.gL = np.ascontiguousarray(np.stack([t[1] for t in tabs]).reshape(n_grains, 81))
+
+    def assemble(self, st, r, defl, **kw):
+        return O.assemble_cp(self.mesh, self.mats, self.grain, self.gP, self.gL, self.ids, self.ndof, self.Ap,
+                             self.Ai, st, r, defl=defl, **kw)

@@ -1,0 +1,177 @@
+"""GPU parity of the crystal-plasticity path (local return map per QP, exact dM/dF, element
+residual and tangent, coloured assembly) against the CPU oracle on the same seeded inputs.
+
+Tolerance (FP64): 1e-10 relative in the infinity norm of every output array per step
+(residual, CSR values, F, pF, g, lambda, tau, gamma_dot).  Both sides stop the local Newton
+on a full-step size below tol_M = 1e-12, after which a quadratically converging iteration
+sits at round-off, so the two solutions agree to far below the tolerance.
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+import pgfem3d_amd as P
+from conftest import rel_inf
+from problems import CPBoxProblem
+
+TOL = 1e-10
+MAT = P.fcc_crystal()
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev(x, device):
+    return torch.from_numpy(np.ascontiguousarray(x)).to(device)
+
+
+def _handle(prob):
+    h = P.Handle(prob.mesh, prob.ids, prob.ndof, prob.Ap, prob.Ai, prob.mats)
+    h.set_orientations(prob.R, prob.grain)
+    return h
+
+
+def _compare_state(h, st, step):
+    assert rel_inf(h.get_state(P.STATE_F_NP1), st.F_np1) < TOL, step
+    assert rel_inf(h.get_state(P.STATE_PLASTIC_NP1), st.pF_np1) < TOL, step
+    sc = h.get_state(P.STATE_SCALAR_NP1)
+    assert rel_inf(sc[0], st.g_np1) < TOL, step
+    if np.abs(st.lam_np1).max() > 0:
+        assert np.abs(sc[1] - st.lam_np1).max() <= TOL * max(1.0, np.abs(st.lam_np1).max()), step
+    assert rel_inf(h.get_state(P.STATE_TAU), st.tau) < TOL, step
+    gd = h.get_state(P.STATE_GDOT)
+    assert np.abs(gd - st.gdot).max() <= TOL * max(np.abs(st.gdot).max(), 1e-300), step
+
+
+def _run(prob, device, stretches, dt=1.0, noise=2e-5, check_modes=True):
+    h = _handle(prob)
+    st = O.CPState(prob.nqp, MAT.g0)
+    f = torch.empty(prob.ndof, dtype=torch.float64, device=device)
+    vals = torch.empty(int(prob.Ap[-1]), dtype=torch.float64, device=device)
+    plastic = False
+    for step, s in enumerate(stretches):
+        r, defl = prob.displacement(s, noise, seed=200 + step)
+        rd, dd = _dev(r, device), _dev(defl, device)
+        err = h.residual_tangent(f, vals, rd, defl=dd, dt=dt)
+        fo, vo, erro = prob.assemble(st, r, defl, dt=dt)
+        assert err == erro == 0, (step, err, erro)
+        assert rel_inf(f.cpu().numpy(), fo) < TOL, (step, rel_inf(f.cpu().numpy(), fo))
+        assert rel_inf(vals.cpu().numpy(), vo) < TOL, (step, rel_inf(vals.cpu().numpy(), vo))
+        _compare_state(h, st, step)
+        its = h.get_state(P.STATE_ITERS)[0]
+        # same algorithm on both sides: iteration counts agree except where a convergence
+        # test lands within round-off of its threshold
+        assert np.mean(its == st.iters) > 0.95, step
+        plastic |= np.abs(st.gdot).max() > 1e-5
+        if check_modes:
+            f2 = torch.empty_like(f)
+            v2 = torch.empty_like(vals)
+            assert h.residual(f2, rd, defl=dd, dt=dt, run_integration=True) == 0
+            assert h.tangent(v2, rd, defl=dd, dt=dt) == 0
+            assert rel_inf(f2.cpu().numpy(), fo) < TOL
+            assert rel_inf(v2.cpu().numpy(), vo) < TOL
+        h.commit_step()
+        st.commit()
+    h.close()
+    return plastic
+
+
+@pytest.mark.parametrize("etype", [P.PGF_TET4, P.PGF_TET10, P.PGF_HEX8])
+def test_cp_multistep_into_plastic_range(device, etype):
+    """Voronoi polycrystal (4 grains), uniaxial stretch increments of 1.5e-4 with commits,
+    beyond the resolved yield g0 = 16 MPa."""
+    n = {P.PGF_TET4: 3, P.PGF_TET10: 2, P.PGF_HEX8: 3}[etype]
+    prob = CPBoxProblem(n, etype, MAT, n_grains=4, seed=3)
+    assert _run(prob, device, [1.5e-4 * s for s in range(1, 9)])
+
+
+def test_cp_tangent_at_other_displacement(device):
+    """CP tangent at a displacement other than the last integration's (Fr != stored F)."""
+    prob = CPBoxProblem(2, P.PGF_TET10, MAT, n_grains=3, seed=5)
+    h = _handle(prob)
+    st = O.CPState(prob.nqp, MAT.g0)
+    f = torch.empty(prob.ndof, dtype=torch.float64, device=device)
+    vals = torch.empty(int(prob.Ap[-1]), dtype=torch.float64, device=device)
+    for step in range(1, 7):
+        r, defl = prob.displacement(1.5e-4 * step, 2e-5, seed=step)
+        assert h.residual(f, _dev(r, device), defl=_dev(defl, device)) == 0
+        prob.assemble(st, r, defl, tangent=False)
+        h.commit_step()
+        st.commit()
+    r, defl = prob.displacement(1.5e-4 * 7, 2e-5, seed=7)
+    assert h.residual(f, _dev(r, device), defl=_dev(defl, device)) == 0
+    prob.assemble(st, r, defl, tangent=False)
+    r2 = r + 1e-5 * np.random.default_rng(9).standard_normal(r.shape)
+    assert h.tangent(vals, _dev(r2, device), defl=_dev(defl, device)) == 0
+    _, vo, erro = prob.assemble(st, r2, defl, residual=False, integrate=False, tangent=True)
+    assert erro == 0
+    assert rel_inf(vals.cpu().numpy(), vo) < TOL
+    h.close()
+
+
+def test_cp_warm_start_and_qp_order(device):
+    """A caller-supplied start (device QP order, [11][n_qp]) converges to the same state, and in
+    fewer iterations when it is the converged point."""
+    prob = CPBoxProblem(2, P.PGF_TET10, MAT, n_grains=3, seed=7)
+    h = _handle(prob)
+    order = h.qp_order()
+    assert np.array_equal(np.sort(order), np.arange(prob.nqp))
+    st = O.CPState(prob.nqp, MAT.g0)
+    f = torch.empty(prob.ndof, dtype=torch.float64, device=device)
+    for step in range(1, 7):
+        r, defl = prob.displacement(1.5e-4 * step, 2e-5, seed=step)
+        assert h.residual(f, _dev(r, device), defl=_dev(defl, device)) == 0
+        prob.assemble(st, r, defl, tangent=False)
+        h.commit_step()
+        st.commit()
+    r, defl = prob.displacement(1.5e-4 * 7, 2e-5, seed=7)
+    st_ref = O.CPState(prob.nqp, MAT.g0)
+    for k in ("pF_n", "g_n", "lam_n", "gdot_n"):
+        getattr(st_ref, k)[:] = getattr(st, k)
+    prob.assemble(st_ref, r, defl, tangent=False)
+    # converged point in mesh order: M = pF_n pF_np1^-1, lambda, g
+    pFn = st.pF_n.T.reshape(-1, 3, 3)
+    pF1 = st_ref.pF_np1.T.reshape(-1, 3, 3)
+    M = pFn @ np.linalg.inv(pF1)
+    warm_mesh = np.concatenate([M.reshape(-1, 9).T, st_ref.lam_np1[None], st_ref.g_np1[None]], axis=0)
+    warm = _dev(np.ascontiguousarray(warm_mesh[:, order]), device)
+    h.set_warm_start(warm)
+    assert h.residual(f, _dev(r, device), defl=_dev(defl, device)) == 0
+    h.set_warm_start(None)
+    its = h.get_state(P.STATE_ITERS)[0]
+    assert its.max() <= 2
+    assert rel_inf(h.get_state(P.STATE_PLASTIC_NP1), st_ref.pF_np1) < TOL
+    assert rel_inf(h.get_state(P.STATE_TAU), st_ref.tau) < TOL
+    # device pointer of a state field addresses the same data
+    p = h.state_device_ptr(P.STATE_SCALAR_NP1)
+    assert p
+    h.close()
+
+
+def test_cp_bitwise_repeat(device):
+    prob = CPBoxProblem(3, P.PGF_TET4, MAT, n_grains=4, seed=11)
+    h = _handle(prob)
+    f = torch.empty(prob.ndof, dtype=torch.float64, device=device)
+    vals = torch.empty(int(prob.Ap[-1]), dtype=torch.float64, device=device)
+    r, defl = prob.displacement(4e-4, 2e-5, seed=1)
+    out = []
+    for _ in range(2):
+        h.reset_step()
+        assert h.residual_tangent(f, vals, _dev(r, device), defl=_dev(defl, device)) == 0
+        out.append((f.cpu().numpy().copy(), vals.cpu().numpy().copy()))
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+    h.close()
+
+
+def test_cp_local_failure_is_reported(device):
+    """A local Newton that cannot converge in its iteration budget is reported as a positive
+    INFO (the caller cuts the load step), not a fault: budget of one iteration."""
+    mat = P.fcc_crystal(max_itr=1)
+    prob = CPBoxProblem(2, P.PGF_TET4, mat, n_grains=2, seed=13)
+    h = P.Handle(prob.mesh, prob.ids, prob.ndof, prob.Ap, prob.Ai, prob.mats)
+    h.set_orientations(prob.R, prob.grain)
+    f = torch.empty(prob.ndof, dtype=torch.float64, device=device)
+    r, defl = prob.displacement(2e-3, 2e-5, seed=1)
+    err = h.residual(f, _dev(r, device), defl=_dev(defl, device))
+    assert err > 0
+    h.close()

@@ -148,21 +148,13 @@ def test_fcc_slip_systems():
 
 
 def _cp_problem(n, etype, seed=1, n_grains=3):
-    from problems import BoxProblem
+    from problems import CPBoxProblem
 
-    prob = BoxProblem(n, etype, MAT, seed=seed)
-    grain, q = P.voronoi_grains(prob.mesh, n_grains, seed)
-    R = P.quat_to_rot(q)
-    tabs = [O.cp_grain(MAT, R[g]) for g in range(n_grains)]
-    prob.grain = grain
-    prob.gP = np.ascontiguousarray(np.stack([t[0] for t in tabs]).reshape(n_grains, 108))
-    prob.gL = np.ascontiguousarray(np.stack([t[1] for t in tabs]).reshape(n_grains, 81))
-    return prob
+    return CPBoxProblem(n, etype, MAT, n_grains=n_grains, seed=seed)
 
 
 def _cp_assemble(prob, st, r, defl, **kw):
-    return O.assemble_cp(prob.mesh, prob.mats, prob.grain, prob.gP, prob.gL, prob.ids, prob.ndof, prob.Ap, prob.Ai,
-                         st, r, defl=defl, **kw)
+    return prob.assemble(st, r, defl, **kw)
 
 
 @pytest.mark.parametrize("etype", [4, 10])
