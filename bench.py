@@ -1,14 +1,22 @@
 """Benchmark of the finite-strain element hot path on one MI355X.
 
-Workload (default): J2 finite-strain plasticity with linear isotropic hardening on a unit cube
-of 48^3 cells, each split into 6 quadratic 10-node tets (663,552 elements, 2,654,208 QPs,
-4-point quadrature), E=70 GPa, nu=0.33, sigma_y=250 MPa, H=1 GPa, FP64.  Bottom face fixed,
-top face stretched; the displacement is a seeded 1.5% uniaxial stretch (plastic at every QP).
+Headline workload (value): J2 finite-strain plasticity with linear isotropic hardening on a
+unit cube of 48^3 cells, each split into 6 quadratic 10-node tets (663,552 elements,
+2,654,208 QPs, 4-point quadrature), E=70 GPa, nu=0.33, sigma_y=250 MPa, H=1 GPa, FP64.
+Bottom face fixed, top face stretched; the displacement is a seeded 1.5% uniaxial stretch
+(plastic at every QP).
 
-One step = one global Newton iteration's element work: deformation gradient, J2 return map
-and state update at every QP, element residual and tangent, and their assembly into the
-residual vector and the CSR values (Psparse_ApAi pattern), all resident in HBM.  The linear
-solve is not part of the step.  value = QPs / second of wall time per step.
+Second workload (the "cp" object): FCC crystal plasticity on 55^3 cells x 6 linear tets
+(998,250 elements and QPs), 64 Voronoi grains with uniformly random orientations, power-law
+flow (gamma_dot0=1e-3/s, m=0.05), saturation hardening (g0=16, gs=148, h0=180 MPa), cubic
+elasticity (168.4/121.4/75.4 GPa), strain rate 1e-3/s, dt = 1 s, plain local-Newton start.
+The load history is driven to 0.5% (five committed steps), then the sixth step is timed.
+
+One step = one global Newton iteration's element work: deformation gradient, constitutive
+update (return map with its local Newton) and state update at every QP, element residual
+and tangent, and their assembly into the residual vector and the CSR values (Psparse_ApAi
+pattern), all resident in HBM.  The linear solve is not part of the step.
+value = QPs / second of wall time per step.
 """
 import argparse
 import json
@@ -25,10 +33,11 @@ for sub in ("pgfem-3d_amd", "oracle", "tests"):
 METRIC = "QP constitutive updates + tangent assembly /sec (FP64); surrogate vs plain speedup"
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+CP_STEP = 1e-3  # strain per 1 s step at 1e-3/s
 
 
 def algorithmic_bytes(n_qp, nnz, ndof, n_nodes, n_elem, nne):
-    """Unique HBM traffic of one step (bytes): J2 state read (Cp^-1_n 6 + ep_n 1 doubles) and
+    """Unique HBM traffic of one J2 step (bytes): state read (Cp^-1_n 6 + ep_n 1 doubles) and
     written (F 9 + Cp^-1 6 + ep, gam 2 doubles), each CSR value and residual entry written
     once, coordinates and displacements (6 doubles per node) and connectivity (4 B per
     element node) read once."""
@@ -37,7 +46,7 @@ def algorithmic_bytes(n_qp, nnz, ndof, n_nodes, n_elem, nne):
 
 
 def flops_per_qp(nne):
-    """FP64 operations per QP of the fused kernel, counted from the source (FMA = 2):
+    """FP64 operations per QP of the fused J2 kernel, counted from the source (FMA = 2):
     phase 1 ~ (9*nne*2 kinematics + 9*nne h_a + ~700 constitutive/tangent), phase 2 per
     element row: 36 MACs (b^T c) + 9 (tau h_a) + 12*nne (R h_w and geometric term)."""
     ndofe = 3 * nne
@@ -46,7 +55,40 @@ def flops_per_qp(nne):
     return phase1 + phase2
 
 
-def cpu_baseline(n_cells, timeout_s=30.0):
+def cp_flops_per_qp(mean_iters, nne=4):
+    """FP64 operations per QP of the crystal-plasticity pass, counted from the kernels (FMA = 2,
+    divide, pow and cbrt as one each):
+      material-point evaluation (cp_point) ~1.4e3: eF, C_e, S = L:E, Mandel stress, 12 tau_a,
+        12 power-law rates, R_M;
+      Jacobian ~6.2e3: 12 x (d tau_a/dM 260 + rank-1 update of the 9x9 block 162 + borders
+        36), lambda terms 400, Schur complement 162;
+      9x9 LU 500 and two solves 324; one line-search evaluation 1.4e3;
+      => Newton iteration ~9.8e3 (the evaluation at the top of the iteration included);
+      start ~3.5e3 (up to 10 tau-ratio evaluations of ~350);
+      tangent pass ~1.2e4 (evaluation, Jacobian, LU, nine bordered solves, d tau/dF);
+      element rows: 3*nne rows x ~800 (B, Y, Z, z = Z:D, W, Fr W, the row of K)."""
+    return 9.8e3 * mean_iters + 3.5e3 + 1.2e4 + 3 * nne * 800
+
+
+def _timed_passes(h, f, vals, rd, dd, steps, warmup, dt=1.0):
+    import torch
+
+    for _ in range(warmup):
+        err = h.residual_tangent(f, vals, rd, defl=dd, dt=dt)
+        assert err == 0, err
+    torch.cuda.synchronize()
+    pass_ms = []
+    t1 = time.perf_counter()
+    for _ in range(steps):
+        err = h.residual_tangent(f, vals, rd, defl=dd, dt=dt)
+        pass_ms.append(h.last_pass_ms()[0])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t1
+    assert err == 0, err
+    return wall / steps, float(np.mean(pass_ms)), h.last_pass_ms()[1]
+
+
+def cpu_baseline(n_cells):
     """Oracle (CPU restatement of the reference path, one thread) on a bounded sample: the
     same element type and material on an n_cells^3 mesh; one fused residual + tangent pass."""
     import oracle as O
@@ -64,14 +106,112 @@ def cpu_baseline(n_cells, timeout_s=30.0):
                       f"({prob.mesh.n_elem} elements, {prob.nqp} QPs), {dt:.1f} s on 1 host thread"}
 
 
+def cpu_baseline_cp(n_cells, n_grains=64):
+    """Oracle crystal plasticity on an n_cells^3 P1 sample with the same grains and load
+    history as the GPU run: five committed steps untimed, the sixth timed (one thread)."""
+    import oracle as O
+    import pgfem3d_amd as P
+    from problems import CPBoxProblem
+
+    mat = P.fcc_crystal()
+    prob = CPBoxProblem(n_cells, P.PGF_TET4, mat, n_grains=n_grains, seed=2)
+    st = O.CPState(prob.nqp, mat.g0)
+    for s in range(1, 6):
+        r, defl = prob.displacement(CP_STEP * s, 0.0, seed=s)
+        _, _, err = prob.assemble(st, r, defl, tangent=False)
+        assert err == 0
+        st.commit()
+    r, defl = prob.displacement(CP_STEP * 6, 0.0, seed=6)
+    t0 = time.perf_counter()
+    _, _, err = prob.assemble(st, r, defl)
+    dt = time.perf_counter() - t0
+    assert err == 0
+    return {"value": prob.nqp / dt, "unit": "QP updates/s", "cores": 1, "kind": "port",
+            "mean_local_iters": float(np.mean(st.iters)),
+            "sample": f"oracle orc_assemble_cp, one fused residual+tangent pass at 0.6% strain, CP P1 "
+                      f"{n_cells}^3 cells ({prob.nqp} QPs, {n_grains} grains), {dt:.1f} s on 1 host thread"}
+
+
+def run_j2(args, dev):
+    import torch
+
+    import pgfem3d_amd as P
+    from problems import BoxProblem
+
+    t0 = time.perf_counter()
+    prob = BoxProblem(args.n, P.PGF_TET10, P.j2_linear(70e3, 0.33, 250.0, 1000.0))
+    t_setup = time.perf_counter() - t0
+    h = P.Handle(prob.mesh, prob.ids, prob.ndof, prob.Ap, prob.Ai, prob.mats)
+    t_create = time.perf_counter() - t0 - t_setup
+    print(f"[bench] J2: {prob.mesh.n_elem} P2 tets, {prob.mesh.n_nodes} nodes, ndof {prob.ndof}, nnz {h.nnz}, "
+          f"QPs {h.n_qp}, colours {h.n_colors}; host set-up {t_setup:.1f}s, create {t_create:.1f}s",
+          file=sys.stderr, flush=True)
+    r, defl = prob.displacement(0.015, 2e-4, seed=11)
+    rd = torch.from_numpy(r).to(dev)
+    dd = torch.from_numpy(defl).to(dev)
+    f = torch.empty(prob.ndof, dtype=torch.float64, device=dev)
+    vals = torch.empty(h.nnz, dtype=torch.float64, device=dev)
+    h.set_profiling(True)
+    s_step, kern_ms, n_launch = _timed_passes(h, f, vals, rd, dd, args.steps, args.warmup)
+    nbytes = algorithmic_bytes(h.n_qp, h.nnz, prob.ndof, prob.mesh.n_nodes, prob.mesh.n_elem, 10)
+    res = dict(n_qp=h.n_qp, nnz=h.nnz, ndof=prob.ndof, n_elem=prob.mesh.n_elem, s_step=s_step, kern_ms=kern_ms,
+               n_launch=n_launch, achieved=nbytes / (kern_ms * 1e-3) / 1e9,
+               tflops=flops_per_qp(10) * h.n_qp / (kern_ms * 1e-3) / 1e12)
+    print(f"[bench] J2: {1e3 * s_step:.3f} ms/step wall, element kernels {kern_ms:.3f} ms/pass over {n_launch} "
+          f"launches, {res['achieved']:.0f} GB/s algorithmic, {res['tflops']:.2f} TFLOP/s", file=sys.stderr, flush=True)
+    h.close()
+    return res
+
+
+def run_cp(args, dev):
+    import torch
+
+    import pgfem3d_amd as P
+    from problems import CPBoxProblem
+
+    mat = P.fcc_crystal()
+    t0 = time.perf_counter()
+    prob = CPBoxProblem(args.cp_n, P.PGF_TET4, mat, n_grains=64, seed=2)
+    h = P.Handle(prob.mesh, prob.ids, prob.ndof, prob.Ap, prob.Ai, prob.mats)
+    h.set_orientations(prob.R, prob.grain)
+    print(f"[bench] CP: {prob.mesh.n_elem} P1 tets, ndof {prob.ndof}, nnz {h.nnz}, QPs {h.n_qp}, colours "
+          f"{h.n_colors}; set-up {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
+    f = torch.empty(prob.ndof, dtype=torch.float64, device=dev)
+    vals = torch.empty(h.nnz, dtype=torch.float64, device=dev)
+    for s in range(1, 6):  # load history to 0.5%
+        r, defl = prob.displacement(CP_STEP * s, 0.0, seed=s)
+        err = h.residual(f, torch.from_numpy(r).to(dev), defl=torch.from_numpy(defl).to(dev), dt=1.0)
+        assert err == 0, (s, err)
+        h.commit_step()
+    r, defl = prob.displacement(CP_STEP * 6, 0.0, seed=6)
+    rd = torch.from_numpy(r).to(dev)
+    dd = torch.from_numpy(defl).to(dev)
+    h.set_profiling(True)
+    s_step, kern_ms, n_launch = _timed_passes(h, f, vals, rd, dd, args.cp_steps, 1)
+    its = h.get_state(P.STATE_ITERS)[0]
+    mean_it = float(its.mean())
+    tflops = cp_flops_per_qp(mean_it) * h.n_qp / (kern_ms * 1e-3) / 1e12
+    res = dict(n_qp=h.n_qp, s_step=s_step, kern_ms=kern_ms, n_launch=n_launch, mean_iters=mean_it,
+               max_iters=int(its.max()), tflops=tflops)
+    print(f"[bench] CP: {1e3 * s_step:.3f} ms/step wall, kernels {kern_ms:.3f} ms/pass over {n_launch} launches, "
+          f"local Newton iterations mean {mean_it:.2f} max {int(its.max())}, {tflops:.2f} TFLOP/s",
+          file=sys.stderr, flush=True)
+    h.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--n", type=int, default=48, help="cells per cube edge")
-    ap.add_argument("--cpu-n", type=int, default=18, help="cells per edge of the CPU baseline sample")
+    ap.add_argument("--n", type=int, default=48, help="cells per cube edge (J2 workload)")
+    ap.add_argument("--cp-n", type=int, default=55, help="cells per cube edge (crystal-plasticity workload)")
+    ap.add_argument("--cp-steps", type=int, default=10)
+    ap.add_argument("--cpu-n", type=int, default=18, help="cells per edge of the J2 CPU baseline sample")
+    ap.add_argument("--cpu-cp-n", type=int, default=34, help="cells per edge of the CP CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cp", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (multi-GPU sharding is a "
@@ -80,62 +220,27 @@ def main():
 
     import torch
 
-    import pgfem3d_amd as P
-    from problems import BoxProblem
-
     dev = torch.device("cuda:0")
-    t0 = time.perf_counter()
-    mat = P.j2_linear(70e3, 0.33, 250.0, 1000.0)
-    prob = BoxProblem(args.n, P.PGF_TET10, mat)
-    t_setup = time.perf_counter() - t0
-    h = P.Handle(prob.mesh, prob.ids, prob.ndof, prob.Ap, prob.Ai, prob.mats)
-    t_create = time.perf_counter() - t0 - t_setup
-    print(f"[bench] mesh {prob.mesh.n_elem} P2 tets, {prob.mesh.n_nodes} nodes, ndof {prob.ndof}, "
-          f"nnz {h.nnz}, QPs {h.n_qp}, colours {h.n_colors}; host set-up {t_setup:.1f}s, create {t_create:.1f}s",
-          file=sys.stderr, flush=True)
-    r, defl = prob.displacement(0.015, 2e-4, seed=11)
-    rd = torch.from_numpy(r).to(dev)
-    dd = torch.from_numpy(defl).to(dev)
-    f = torch.empty(prob.ndof, dtype=torch.float64, device=dev)
-    vals = torch.empty(h.nnz, dtype=torch.float64, device=dev)
-    h.set_profiling(True)
-    for _ in range(args.warmup):
-        err = h.residual_tangent(f, vals, rd, defl=dd)
-        assert err == 0, err
-    torch.cuda.synchronize()
-    pass_ms = []
-    t1 = time.perf_counter()
-    for _ in range(args.steps):
-        err = h.residual_tangent(f, vals, rd, defl=dd)
-        pass_ms.append(h.last_pass_ms()[0])
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t1
-    n_launch = h.last_pass_ms()[1]
-    assert err == 0
-    ms_step = 1e3 * wall / args.steps
-    value = h.n_qp / (wall / args.steps)
-    kern_ms = float(np.mean(pass_ms))  # GPU time of the element launches of one pass (HIP events)
-    nbytes = algorithmic_bytes(h.n_qp, h.nnz, prob.ndof, prob.mesh.n_nodes, prob.mesh.n_elem, 10)
-    achieved = nbytes / (kern_ms * 1e-3) / 1e9
-    tflops = flops_per_qp(10) * h.n_qp / (kern_ms * 1e-3) / 1e12
-    print(f"[bench] {ms_step:.3f} ms/step wall, element kernels {kern_ms:.3f} ms/pass over {n_launch} launches, "
-          f"{achieved:.0f} GB/s algorithmic, {tflops:.2f} TFLOP/s", file=sys.stderr, flush=True)
-    h.close()
-    del vals
+    j2 = run_j2(args, dev)
+    cp = None if args.no_cp else run_cp(args, dev)
 
-    cpu = None
+    cpu = cpu_cp = None
     if not args.no_cpu:
         cpu = cpu_baseline(args.cpu_n)
         print(f"[bench] cpu baseline {cpu['value']:.0f} QP/s ({cpu['sample']})", file=sys.stderr, flush=True)
+        if cp is not None:
+            cpu_cp = cpu_baseline_cp(args.cpu_cp_n)
+            print(f"[bench] cpu baseline CP {cpu_cp['value']:.0f} QP/s ({cpu_cp['sample']})", file=sys.stderr,
+                  flush=True)
 
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": j2["n_qp"] / j2["s_step"],
         "unit": "QP updates/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_step,
+        "ms_per_step": 1e3 * j2["s_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -143,7 +248,7 @@ def main():
         "data": "synthetic (seeded jittered box mesh and displacement)",
         "config": {
             "workload": f"J2 plasticity (linear isotropic hardening), {args.n}^3 cells x 6 quadratic tets "
-                        f"({prob.mesh.n_elem} elements, {h.n_qp} QPs, {prob.ndof} free DOFs, nnz {h.nnz}); "
+                        f"({j2['n_elem']} elements, {j2['n_qp']} QPs, {j2['ndof']} free DOFs, nnz {j2['nnz']}); "
                         "one fused residual + tangent + CSR assembly pass per step",
             "element": "tet10, 4 QPs",
             "model": "J2 E=70GPa nu=0.33 sigma_y=250MPa H=1GPa",
@@ -151,17 +256,33 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "achieved": achieved,
+            "achieved": j2["achieved"],
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
+            "frac": j2["achieved"] / HBM_PEAK_GBS,
             "traffic": None,
         },
-        "fp64": {"achieved_tflops": tflops, "peak_tflops": FP64_PEAK_TFLOPS, "frac": tflops / FP64_PEAK_TFLOPS},
-        "kernel_ms_per_pass": kern_ms,
-        "launches_per_pass": n_launch,
+        "fp64": {"achieved_tflops": j2["tflops"], "peak_tflops": FP64_PEAK_TFLOPS,
+                 "frac": j2["tflops"] / FP64_PEAK_TFLOPS},
+        "kernel_ms_per_pass": j2["kern_ms"],
+        "launches_per_pass": j2["n_launch"],
         "cpu_baseline": cpu,
     }
+    if cp is not None:
+        out["cp"] = {
+            "workload": f"FCC crystal plasticity, {args.cp_n}^3 cells x 6 linear tets ({cp['n_qp']} QPs), 64 "
+                        "Voronoi grains, step 6 of 1e-3/s at dt = 1 s, plain local-Newton start",
+            "value": cp["n_qp"] / cp["s_step"],
+            "unit": "QP updates/s",
+            "ms_per_step": 1e3 * cp["s_step"],
+            "kernel_ms_per_pass": cp["kern_ms"],
+            "mean_local_iters": cp["mean_iters"],
+            "max_local_iters": cp["max_iters"],
+            "roofline": {"bound": "fp64", "achieved": cp["tflops"], "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": cp["tflops"] / FP64_PEAK_TFLOPS},
+            "surrogate_speedup": None,
+            "cpu_baseline": cpu_cp,
+        }
     print(json.dumps(out))
 
 

@@ -50,7 +50,7 @@ void orc_stiffness_contract(double *lk, int nne, const double *ST, double detJxW
 void orc_cp_slip_systems(double *P0);
 void orc_cp_grain(const orc_material *mat, const double *R, double *P_out, double *L_out);
 int orc_cp_update(const orc_material *mat, const double *P_grain, const double *L_grain, const double *F,
-                  const double *pFn, double g_n, double lam_n, double dt, const double *x0, int solve,
+                  const double *Fn, const double *pFn, double g_n, double lam_n, double dt, const double *x0, int solve,
                   double *pFnp1, double *g_np1, double *lam_np1, double *S, double *tau, double *gdot,
                   double *dMdF, int *iters);
 

@@ -173,7 +173,7 @@ def _cp_lib():
         L.orc_cp_slip_systems.restype = None
         L.orc_cp_grain.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_cp_grain.restype = None
-        L.orc_cp_update.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
+        L.orc_cp_update.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
                                     C.c_double, C.c_double, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
                                     C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_cp_update.restype = C.c_int
@@ -200,7 +200,7 @@ def cp_grain(mat, R):
     return P, Lg
 
 
-def cp_update(mat, P, Lg, F, pFn, g_n, lam_n, dt, x0=None, solve=True, tangent=True):
+def cp_update(mat, P, Lg, F, pFn, g_n, lam_n, dt, x0=None, solve=True, tangent=True, Fn=None):
     om = material(mat)
     pF1 = np.zeros(9)
     S = np.zeros(9)
@@ -211,6 +211,7 @@ def cp_update(mat, P, Lg, F, pFn, g_n, lam_n, dt, x0=None, solve=True, tangent=T
     x0a = None if x0 is None else np.ascontiguousarray(x0, dtype=np.float64)
     rc = _cp_lib().orc_cp_update(C.byref(om), _p(np.ascontiguousarray(P)), _p(np.ascontiguousarray(Lg)),
                                  _p(np.ascontiguousarray(F, dtype=np.float64).reshape(9)),
+                                 _p(None if Fn is None else np.ascontiguousarray(Fn, dtype=np.float64).reshape(9)),
                                  _p(np.ascontiguousarray(pFn, dtype=np.float64).reshape(9)), g_n, lam_n, dt, _p(x0a),
                                  int(solve), _p(pF1), C.byref(g1), C.byref(l1), _p(S), _p(tau), _p(gdot), _p(dMdF),
                                  C.byref(its))
@@ -225,6 +226,7 @@ class CPState:
         eye = np.array([1.0, 0, 0, 0, 1, 0, 0, 0, 1])
         self.n_qp = n_qp
         self.F_np1 = np.ascontiguousarray(np.repeat(eye[:, None], n_qp, axis=1))
+        self.F_n = self.F_np1.copy()
         self.pF_n = self.F_np1.copy()
         self.pF_np1 = self.F_np1.copy()
         self.g_n = np.full(n_qp, float(g0))
@@ -237,6 +239,7 @@ class CPState:
         self.iters = np.zeros(n_qp, dtype=np.int32)
 
     def commit(self):
+        self.F_n[:] = self.F_np1
         self.pF_n[:] = self.pF_np1
         self.g_n[:] = self.g_np1
         self.lam_n[:] = self.lam_np1
@@ -257,5 +260,5 @@ def assemble_cp(mesh, mats, elem_grain, grain_P, grain_L, ids, ndof, Ap, Ai, st:
                             _p(eg), _p(gP), _p(gL), _p(ids), ndof, _p(Ap), _p(Ai), _p(r), _p(d_r), _p(defl),
                             _p(defl_d), dt, int(residual), int(integrate), int(tangent), _p(st.F_np1), _p(st.pF_n),
                             _p(st.pF_np1), _p(st.g_n), _p(st.g_np1), _p(st.lam_n), _p(st.lam_np1), _p(st.tau),
-                            _p(st.gdot), _p(st.iters), _p(st.gdot_n), _p(warm), _p(f), _p(vals))
+                            _p(st.gdot), _p(st.iters), _p(st.F_n), _p(warm), _p(f), _p(vals))
     return f, vals, err

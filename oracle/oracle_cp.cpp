@@ -16,7 +16,11 @@
 // with P_a = s_a (x) m_a of the 12 FCC {111}<110> systems rotated by the grain rotation R,
 // and L the cubic (C11, C12, C44) stiffness rotated by R.  The three equations are solved
 // together by Newton with backtracking; GCM staggers g (max_itr_stag) around the M solve,
-// which has the same converged point.  The consistent tangent dM/dF is the implicit
+// which has the same converged point.  Plain start: the reference's fully plastic
+// predictor (initial_guess, src/initial_guess.cc:100-110: the elastic part of step n is
+// kept, M_p = pF(n) F(n+1)^-1 F(n) pF(n)^-1 scaled to det 1), pulled back along
+// M(a) = I + a (M_p - I) to the first a at which max_a |tau_a|/g_n = 1 (8 bisection steps
+// on [0, 1], upper end kept); a = 0 when the elastic trial state is below yield.  The consistent tangent dM/dF is the implicit
 // derivative of the converged system (the role of plasticity_compute_dMdu,
 // src/plasticity_model.cc:578-699).
 //
@@ -233,6 +237,61 @@ void cp_jacobian(const orc_material &m, const Grain &gr, double dt, const double
   J[10 * kN + 10] = (ev.gs - g0 + dt * h0 * ev.gsum) + dRg_dgsum * sg;
 }
 
+// max_a |tau_a| / g at M (elastic part eF = A M).
+double tau_ratio_max(const Grain &gr, const double *A, const double *M, double g) {
+  double eF[9], Ce[9], E[9], S[9], Sig[9];
+  mm(A, M, eF);
+  mtm(eF, eF, Ce);
+  for (int i = 0; i < 9; ++i) E[i] = 0.5 * (Ce[i] - ((i % 4 == 0) ? 1.0 : 0.0));
+  L_dot(gr.L, E, S);
+  mm(Ce, S, Sig);
+  double mx = 0.0;
+  for (int a = 0; a < 12; ++a) {
+    double t = 0.0;
+    for (int i = 0; i < 9; ++i) t += Sig[i] * gr.P[a][i];
+    mx = std::fmax(mx, std::fabs(t));
+  }
+  return mx / g;
+}
+
+// Plain start x = (M, lambda_n, g_n): the fully plastic predictor pulled back to yield.
+bool cp_start(const Grain &gr, const double *A, const double *F, const double *Fn, const double *pFn,
+              const double *pFni, double g_n, double lam_n, double *x) {
+  double Fi[9], T[9], Mp[9];
+  if (!inv3(F, Fi)) return false;
+  mm(pFn, Fi, T);
+  mm(T, Fn, Mp);
+  mm(Mp, pFni, T);
+  const double dp = det3(T);
+  if (!(dp > 0.0)) return false;
+  const double c = 1.0 / std::cbrt(dp);
+  for (int i = 0; i < 9; ++i) Mp[i] = T[i] * c;
+  const double I9[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+  double al = 0.0;
+  if (tau_ratio_max(gr, A, I9, g_n) > 1.0) {
+    al = 1.0;
+    double lo = 0.0, hi = 1.0;
+    if (!(tau_ratio_max(gr, A, Mp, g_n) > 1.0)) {
+      for (int k = 0; k < 8; ++k) {
+        const double mid = 0.5 * (lo + hi);
+        double Mm[9];
+        for (int i = 0; i < 9; ++i) Mm[i] = I9[i] + mid * (Mp[i] - I9[i]);
+        if (tau_ratio_max(gr, A, Mm, g_n) > 1.0) lo = mid;
+        else hi = mid;
+      }
+      al = hi;
+    }
+  }
+  for (int i = 0; i < 9; ++i) x[i] = I9[i] + al * (Mp[i] - I9[i]);
+  if (al > 0.0) {
+    const double cm = 1.0 / std::cbrt(det3(x));
+    for (int i = 0; i < 9; ++i) x[i] *= cm;
+  }
+  x[9] = lam_n;
+  x[10] = g_n;
+  return true;
+}
+
 // Line-search merit: half the squared 2-norm of the scaled residual (the Newton direction
 // is a descent direction for it).
 double merit(const CPEval &ev, double gscale) {
@@ -297,7 +356,7 @@ void orc_cp_grain(const orc_material *mat, const double *R, double *P_out, doubl
 // x0 is taken as the converged point (tangent from a stored state).
 // Returns 0 on success, 1 on failure (non-invertible or no convergence).
 int orc_cp_update(const orc_material *mat, const double *P_grain, const double *L_grain, const double *F,
-                  const double *pFn, double g_n, double lam_n, double dt, const double *x0, int solve,
+                  const double *Fn, const double *pFn, double g_n, double lam_n, double dt, const double *x0, int solve,
                   double *pFnp1, double *g_np1, double *lam_np1, double *S, double *tau, double *gdot,
                   double *dMdF, int *iters) {
   const orc_material &m = *mat;
@@ -312,9 +371,8 @@ int orc_cp_update(const orc_material *mat, const double *P_grain, const double *
   if (x0) {
     std::memcpy(x, x0, sizeof(x));
   } else {
-    for (int i = 0; i < 9; ++i) x[i] = (i % 4 == 0) ? 1.0 : 0.0;
-    x[9] = lam_n;
-    x[10] = g_n;
+    const double I9[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    if (!cp_start(gr, A, F, Fn ? Fn : I9, pFn, pFni, g_n, lam_n, x)) return 1;
   }
   const double tol = (m.tol_M > 0.0) ? m.tol_M : 1.0e-10;
   const int maxit = (m.max_itr_M > 0) ? m.max_itr_M : 50;
@@ -325,22 +383,11 @@ int orc_cp_update(const orc_material *mat, const double *P_grain, const double *
   int piv[kN];
   int it = 0;
   bool conv = !solve;  // solve == 0: x0 is the converged point, only the tangent is wanted
-  // Stage 1 (plain start): Newton on (M, lambda) with g frozen at its start value, as the
-  // reference staggers the hardening around the M solve; once the M update is below
-  // kStage1 the hardening relation is solved in closed form for g (int_HA) and stage 2
-  // continues with the coupled (M, lambda, g) Newton.  A warm start begins in stage 2.
-  const double kStage1 = 1.0e-6;
-  bool coupled = (x0 != nullptr);
   while (solve && it < maxit) {
     ++it;
     cp_jacobian(m, gr, dt, x, ev, g_n, J);
     double rhs[kN];
     for (int i = 0; i < kN; ++i) rhs[i] = -ev.R[i];
-    if (!coupled) {  // freeze g: row and column 10 become the identity
-      for (int i = 0; i < kN; ++i) J[10 * kN + i] = J[i * kN + 10] = 0.0;
-      J[10 * kN + 10] = 1.0;
-      rhs[10] = 0.0;
-    }
     if (!lu_factor(J, piv, kN)) return 1;
     double dx[kN];
     for (int i = 0; i < kN; ++i) dx[i] = rhs[i];
@@ -351,21 +398,20 @@ int orc_cp_update(const orc_material *mat, const double *P_grain, const double *
     for (int i = 0; i < 9; ++i) du_full = std::fmax(du_full, std::fabs(dx[i]));
     du_full = std::fmax(du_full, std::fabs(dx[9]) * dt);
     du_full = std::fmax(du_full, std::fabs(dx[10]) / g_n);
-    if (coupled && du_full < tol) {
+    if (du_full < tol) {
       for (int i = 0; i < kN; ++i) x[i] += dx[i];
       if (!cp_eval(m, gr, A, det_pFn, &g_n, dt, x, ev)) return 1;
       conv = true;
       break;
     }
-    const double gs_used = coupled ? gscale : 0.0;
-    const double f0 = merit(ev, gs_used);
+    const double f0 = merit(ev, gscale);
     double alpha = 1.0;
     double xn[kN];
     CPEval evn;
     bool ok = false;
     for (int ls = 0; ls < 40; ++ls) {
       for (int i = 0; i < kN; ++i) xn[i] = x[i] + alpha * dx[i];
-      if (cp_eval(m, gr, A, det_pFn, &g_n, dt, xn, evn) && merit(evn, gs_used) <= (1.0 - 1e-4 * alpha) * f0) {
+      if (cp_eval(m, gr, A, det_pFn, &g_n, dt, xn, evn) && merit(evn, gscale) <= (1.0 - 1e-4 * alpha) * f0) {
         ok = true;
         break;
       }
@@ -378,14 +424,6 @@ int orc_cp_update(const orc_material *mat, const double *P_grain, const double *
     }
     std::memcpy(x, xn, sizeof(x));
     ev = evn;
-    if (!coupled && du_full < kStage1) {
-      // closed-form hardening update g = ((gs-g0) g_n + dt h0 gs gdot)/(gs-g0+dt h0 gdot)
-      const double num = (ev.gs - m.g0) * g_n + dt * m.G0 * ev.gs * ev.gsum;
-      const double den = ev.gs - m.g0 + dt * m.G0 * ev.gsum;
-      x[10] = num / den;
-      if (!cp_eval(m, gr, A, det_pFn, &g_n, dt, x, ev)) return 1;
-      coupled = true;
-    }
   }
   if (iters) *iters = it;
   if (!conv) return 1;
@@ -451,7 +489,7 @@ int orc_assemble_cp(int64_t n_nodes, const double *X, int64_t n_elem, const int6
                     const double *d_r, const double *defl, const double *defl_d, double dt, int do_residual,
                     int run_integration, int do_tangent, double *F_np1, const double *pF_n, double *pF_np1,
                     const double *g_n, double *g_np1, const double *lam_n, double *lam_np1, double *tau_out,
-                    double *gdot_out, int32_t *iters_out, const double *gdot_n, const double *warm, double *f, double *vals);
+                    double *gdot_out, int32_t *iters_out, const double *F_n, const double *warm, double *f, double *vals);
 }
 
 namespace {
@@ -466,7 +504,7 @@ extern "C" int orc_assemble_cp(int64_t n_nodes, const double *X, int64_t n_elem,
                                double dt, int do_residual, int run_integration, int do_tangent, double *F_np1,
                                const double *pF_n, double *pF_np1, const double *g_n, double *g_np1,
                                const double *lam_n, double *lam_np1, double *tau_out, double *gdot_out,
-                               int32_t *iters_out, const double *gdot_n, const double *warm, double *f, double *vals) {
+                               int32_t *iters_out, const double *F_n, const double *warm, double *f, double *vals) {
   (void)n_nodes;
   int64_t NQ = 0;
   for (int64_t e = 0; e < n_elem; ++e) NQ += orc_nint(int(elem_ptr[e + 1] - elem_ptr[e]));
@@ -521,26 +559,15 @@ extern "C" int orc_assemble_cp(int64_t n_nodes, const double *X, int64_t n_elem,
             for (int i = 0; i < 9; ++i) F_np1[i * NQ + q] = Fr[i];
             double S0[9], t12[12], gd12[12], g1, l1;
             int its = 0;
-            double x0[11];
+            double x0[11], Fn[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
             const double *x0p = nullptr;
             if (warm) {  // caller-provided start (M, lambda, g)
               for (int i = 0; i < 11; ++i) x0[i] = warm[i * NQ + q];
               x0p = x0;
-            } else if (gdot_n) {  // plain start: slip rates of the last converged step
-              double any = 0.0;
-              for (int a = 0; a < 12; ++a) any += std::fabs(gdot_n[a * NQ + q]);
-              if (any > 0.0) {
-                for (int i = 0; i < 9; ++i) {
-                  double s = 0.0;
-                  for (int a = 0; a < 12; ++a) s += gdot_n[a * NQ + q] * gP[a * 9 + i];
-                  x0[i] = ((i % 4 == 0) ? 1.0 : 0.0) - dt * s;
-                }
-                x0[9] = lam_n[q];
-                x0[10] = g_n[q];
-                x0p = x0;
-              }
             }
-            if (orc_cp_update(&mat, gP, gL, Fr, pFn, g_n[q], lam_n[q], dt, x0p, 1, pF1, &g1, &l1, S0, t12, gd12,
+            if (F_n)
+              for (int i = 0; i < 9; ++i) Fn[i] = F_n[i * NQ + q];
+            if (orc_cp_update(&mat, gP, gL, Fr, Fn, pFn, g_n[q], lam_n[q], dt, x0p, 1, pF1, &g1, &l1, S0, t12, gd12,
                               nullptr, &its)) {
               ++err;
               if (iters_out) iters_out[q] = its;
@@ -594,7 +621,7 @@ extern "C" int orc_assemble_cp(int64_t n_nodes, const double *X, int64_t n_elem,
           for (int i = 0; i < 9; ++i) x0[i] = Mcp[i];
           x0[9] = lam_np1[q];
           x0[10] = g_np1[q];
-          if (orc_cp_update(&mat, gP, gL, Fs, pFn, g_n[q], lam_n[q], dt, x0, 0, pF2, &g1, &l1, S1, t12, gd12, dMdF,
+          if (orc_cp_update(&mat, gP, gL, Fs, nullptr, pFn, g_n[q], lam_n[q], dt, x0, 0, pF2, &g1, &l1, S1, t12, gd12, dMdF,
                             nullptr)) { ++err; continue; }
           double dMdu_all[270];
           for (int w = 0; w < nne; ++w)

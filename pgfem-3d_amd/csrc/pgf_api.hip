@@ -170,7 +170,7 @@ void cp_prepass(pgf_handle h, const Block &b, const pgf_nodal_input *in, int mod
   c.g_n = h->d_Sc[1];
   c.lam_n = h->d_Sc[1] + NQ;
   c.Sc_np1 = h->d_Sc[0];
-  c.gdot_n = h->d_gdot[1];
+  c.F_n = h->d_F[1];
   c.tau = h->d_tau[0];
   c.gdot = h->d_gdot[0];
   c.iters = h->d_iters;

@@ -8,9 +8,12 @@
 //   R_g      = g (gs - g0 + dt h0 gdot) - (gs - g0) g_n - dt h0 gs gdot (:395-403)
 //   gdot_a   = gdot0 (tau_a/g)|tau_a/g|^(1/m-1), tau_a = (C_e S):P_a,
 //   S = L : (C_e - I)/2, C_e = eF^T eF, eF = F pF(n)^-1 M           (src/TA_GA.cc:8-41)
-// Newton: stage 1 freezes g until the (M, lambda) update is below 1e-6, then g jumps to the
-// closed-form hardening value and stage 2 solves all three; backtracking on half the squared
-// residual norm; converged when the full Newton step is below tol.  The 11x11 system is
+// Plain start: the reference's fully plastic predictor (initial_guess,
+// src/initial_guess.cc:100-110: elastic part of step n kept, M_p = pF(n) F^-1 F(n) pF(n)^-1
+// scaled to det 1) pulled back along M(a) = I + a (M_p - I) to max |tau_a|/g_n = 1 (8
+// bisection steps, upper end kept; a = 0 below yield).  Newton on all eleven unknowns with
+// backtracking on half the squared residual norm; converged when the full Newton step is
+// below tol.  The 11x11 system is
 // solved through the Schur complement of its (M, M) block: with the bordered structure
 //   [A b c; d^T 0 0; e^T 0 f]
 // one 9x9 LU (register-resident, pivoting by predicated row swaps so nothing is indexed at
@@ -223,14 +226,13 @@ __device__ __forceinline__ void dtau_dM(const DevGrain &gr, int a, const double 
   for (int i = 0; i < 9; ++i) D[i] = 2.0 * T[i];
 }
 
-// Bordered Jacobian [Ah b c; d^T 0 0; e^T 0 f] at p.  With coupled == false the g row and
-// column are replaced by the identity (stage 1).
+// Bordered Jacobian [Ah b c; d^T 0 0; e^T 0 f] at p.
 struct Border {
   double b[9], c[9], d[9], e[9], f, dRg;
 };
 
 __device__ __forceinline__ void cp_jacobian(const CPMaterial &cm, const DevGrain &gr, double dt, const double *x,
-                                            const double *A, const CPPoint &p, double g_n, bool coupled,
+                                            const double *A, const CPPoint &p, double g_n,
                                             double (&Ah)[81], Border &B) {
   const double lam = x[9], g = x[10];
   double AtE[9];
@@ -279,18 +281,9 @@ __device__ __forceinline__ void cp_jacobian(const CPMaterial &cm, const DevGrain
       B.d[i * 3 + j] = -p.pJ * Mi[j * 3 + i];
     }
   B.dRg = g * (p.dgs + dt * cm.h0) - p.dgs * g_n - dt * cm.h0 * (p.dgs * p.gsum + p.gs);
-  if (coupled) {
 #pragma unroll
-    for (int c = 0; c < 9; ++c) B.e[c] *= B.dRg;
-    B.f = (p.gs - cm.g0 + dt * cm.h0 * p.gsum) + B.dRg * sg;
-  } else {
-#pragma unroll
-    for (int c = 0; c < 9; ++c) {
-      B.c[c] = 0.0;
-      B.e[c] = 0.0;
-    }
-    B.f = 1.0;
-  }
+  for (int c = 0; c < 9; ++c) B.e[c] *= B.dRg;
+  B.f = (p.gs - cm.g0 + dt * cm.h0 * p.gsum) + B.dRg * sg;
   // Schur: Ah <- A - c e^T / f
   const double rf = 1.0 / B.f;
 #pragma unroll
@@ -328,6 +321,71 @@ __device__ __forceinline__ double cp_merit(const CPPoint &p, double gscale) {
   for (int i = 0; i < 10; ++i) r += p.R[i] * p.R[i];
   const double rg = p.R[10] * gscale;
   return 0.5 * (r + rg * rg);
+}
+
+// max_a |tau_a| / g at M (elastic part eF = A M)
+__device__ __forceinline__ double tau_ratio_max(const DevGrain &gr, const double *A, const double *M, double rg) {
+  double eF[9], Ce[9], E[9], S[6], S9[9], Sig[9];
+  mm3(A, M, eF);
+  mtm3(eF, eF, Ce);
+#pragma unroll
+  for (int i = 0; i < 9; ++i) E[i] = 0.5 * (Ce[i] - ((i % 4 == 0) ? 1.0 : 0.0));
+  lv_dot(gr.Lv, E, S);
+  v6_to_9(S, S9);
+  mm3(Ce, S9, Sig);
+  double mx = 0.0;
+#pragma unroll
+  for (int a = 0; a < 12; ++a) {
+    double t = 0.0;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) t += Sig[i] * gr.P[a][i];
+    mx = fmax(mx, fabs(t));
+  }
+  return mx * rg;
+}
+
+// Plain start x = (M, lambda_n, g_n): the fully plastic predictor pulled back to yield.
+__device__ __forceinline__ bool cp_start(const DevGrain &gr, const double *A, const double *F, const double *Fn,
+                                         const double *pFn, const double *pFi, double g_n, double lam_n,
+                                         double *x) {
+  double Fi[9], T[9], Mp[9];
+  if (!inv3(F, Fi)) return false;
+  mm3(pFn, Fi, T);
+  mm3(T, Fn, Mp);
+  mm3(Mp, pFi, T);
+  const double dp = det3(T);
+  if (!(dp > 0.0)) return false;
+  const double c = 1.0 / cbrt(dp);
+#pragma unroll
+  for (int i = 0; i < 9; ++i) Mp[i] = T[i] * c;
+  const double rg = 1.0 / g_n;
+  const double I9[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+  double al = 0.0;
+  if (tau_ratio_max(gr, A, I9, rg) > 1.0) {
+    al = 1.0;
+    if (!(tau_ratio_max(gr, A, Mp, rg) > 1.0)) {
+      double lo = 0.0, hi = 1.0;
+      for (int k = 0; k < 8; ++k) {
+        const double mid = 0.5 * (lo + hi);
+        double Mm[9];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) Mm[i] = I9[i] + mid * (Mp[i] - I9[i]);
+        if (tau_ratio_max(gr, A, Mm, rg) > 1.0) lo = mid;
+        else hi = mid;
+      }
+      al = hi;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 9; ++i) x[i] = I9[i] + al * (Mp[i] - I9[i]);
+  if (al > 0.0) {
+    const double cm = 1.0 / cbrt(det3(x));
+#pragma unroll
+    for (int i = 0; i < 9; ++i) x[i] *= cm;
+  }
+  x[9] = lam_n;
+  x[10] = g_n;
+  return true;
 }
 
 // ------------------------------------------------------------------- kinematics kernel
@@ -395,7 +453,7 @@ __global__ __launch_bounds__(64) void cp_newton(CPArgs a, int64_t chunk) {
   int64_t q = -1;
   double x[11], A[9], det_pFn = 1.0, g_n = 1.0, gscale = 1.0, dt = a.dt;
   int it = 0, gid = 0;
-  bool coupled = false;
+  bool bad_start = false;
   CPMaterial cm{};
   for (;;) {
     // refill the lanes that need a QP, in lane order, from the wave's range
@@ -424,33 +482,17 @@ __global__ __launch_bounds__(64) void cp_newton(CPArgs a, int64_t chunk) {
         cm = a.cpmats[0];
         gscale = 1.0 / (fabs(cm.gs0 - cm.g0) * g_n + 1e-300);
         it = 0;
-        // start: caller's warm start, else the slip rates of the last converged step, else the
-        // elastic predictor (M = I) with the staged Newton
-        bool have = false;
+        // start: the caller's warm start, else the plain start
+        bad_start = false;
         if (a.warm) {
 #pragma unroll
           for (int i = 0; i < 11; ++i) x[i] = a.warm[i * NQ + qg];
-          have = true;
         } else {
-          double any = 0.0;
+          double Fn[9];
 #pragma unroll
-          for (int al = 0; al < 12; ++al) any += fabs(a.gdot_n[al * NQ + qg]);
-          const DevGrain &gr = a.grains[gid];
-#pragma unroll
-          for (int i = 0; i < 9; ++i) {
-            double s = 0.0;
-            for (int al = 0; al < 12; ++al) s += a.gdot_n[al * NQ + qg] * gr.P[al][i];
-            x[i] = ((i % 4 == 0) ? 1.0 : 0.0) - dt * s;
-          }
-          have = any > 0.0;
-          if (!have) {
-#pragma unroll
-            for (int i = 0; i < 9; ++i) x[i] = (i % 4 == 0) ? 1.0 : 0.0;
-          }
-          x[9] = a.lam_n[qg];
-          x[10] = g_n;
+          for (int i = 0; i < 9; ++i) Fn[i] = a.F_n[i * NQ + qg];
+          bad_start = !cp_start(a.grains[gid], A, F, Fn, pFn, pFi, g_n, a.lam_n[qg], x);
         }
-        coupled = have;
       }
     }
     cursor += nneed;
@@ -463,12 +505,12 @@ __global__ __launch_bounds__(64) void cp_newton(CPArgs a, int64_t chunk) {
     ++it;
     bool fail = false, conv = false;
     CPPoint p;
-    if (!cp_point(cm, gr, A, det_pFn, g_n, dt, x, p)) {
+    if (bad_start || !cp_point(cm, gr, A, det_pFn, g_n, dt, x, p)) {
       fail = true;
     } else {
       double Ah[81];
       Border B;
-      cp_jacobian(cm, gr, dt, x, A, p, g_n, coupled, Ah, B);
+      cp_jacobian(cm, gr, dt, x, A, p, g_n, Ah, B);
       int piv[9];
       if (!lu9(Ah, piv)) {
         fail = true;
@@ -484,26 +526,25 @@ __global__ __launch_bounds__(64) void cp_newton(CPArgs a, int64_t chunk) {
 #pragma unroll
         for (int i = 0; i < 9; ++i) r9[i] = -p.R[i];
         double dx[11];
-        bordered_solve(Ah, piv, B, v, dTv, r9, -p.R[9], coupled ? -p.R[10] : 0.0, dx);
+        bordered_solve(Ah, piv, B, v, dTv, r9, -p.R[9], -p.R[10], dx);
         double du = 0.0;
 #pragma unroll
         for (int i = 0; i < 9; ++i) du = fmax(du, fabs(dx[i]));
         du = fmax(du, fabs(dx[9]) * dt);
         du = fmax(du, fabs(dx[10]) / g_n);
-        if (coupled && du < cm.tol) {
+        if (du < cm.tol) {
 #pragma unroll
           for (int i = 0; i < 11; ++i) x[i] += dx[i];
           conv = true;
         } else {
-          const double gsu = coupled ? gscale : 0.0;
-          const double f0 = cp_merit(p, gsu);
+          const double f0 = cp_merit(p, gscale);
           double alpha = 1.0, xn[11];
           bool ok = false;
           CPPoint pn;
           for (int ls = 0; ls < 40; ++ls) {
 #pragma unroll
             for (int i = 0; i < 11; ++i) xn[i] = x[i] + alpha * dx[i];
-            if (cp_point(cm, gr, A, det_pFn, g_n, dt, xn, pn) && cp_merit(pn, gsu) <= (1.0 - 1e-4 * alpha) * f0) {
+            if (cp_point(cm, gr, A, det_pFn, g_n, dt, xn, pn) && cp_merit(pn, gscale) <= (1.0 - 1e-4 * alpha) * f0) {
               ok = true;
               break;
             }
@@ -517,12 +558,6 @@ __global__ __launch_bounds__(64) void cp_newton(CPArgs a, int64_t chunk) {
           if (!fail) {
 #pragma unroll
             for (int i = 0; i < 11; ++i) x[i] = xn[i];
-            if (!coupled && du < 1.0e-6) {  // stage 1 done: closed-form hardening update
-              const double num = (pn.gs - cm.g0) * g_n + dt * cm.h0 * pn.gs * pn.gsum;
-              const double den = pn.gs - cm.g0 + dt * cm.h0 * pn.gsum;
-              x[10] = num / den;
-              coupled = true;
-            }
           }
         }
       }
@@ -602,7 +637,7 @@ __global__ __launch_bounds__(64) void cp_tangent(CPArgs a, int64_t nqb) {
   if constexpr (WITH_D) {
     double Ah[81];
     Border B;
-    cp_jacobian(cm, gr, a.dt, x, A, p, g_n, true, Ah, B);
+    cp_jacobian(cm, gr, a.dt, x, A, p, g_n, Ah, B);
     int piv[9];
     if (!lu9(Ah, piv)) {
       atomicAdd(a.err, 1);

@@ -83,7 +83,7 @@ struct CPArgs {
   double *pF_np1;
   const double *g_n, *lam_n;  // Sc_n comps 0 and 1
   double *Sc_np1;             // g, lambda at n+1
-  const double *gdot_n;       // 12 comps (plain start)
+  const double *F_n;          // 9 comps (plain start: plastic predictor)
   double *tau, *gdot;         // 12 comps at n+1
   int32_t *iters;             // [n_qp]
   const double *warm;         // optional start (M 9, lambda, g) [11][n_qp]

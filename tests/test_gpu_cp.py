@@ -126,7 +126,7 @@ def test_cp_warm_start_and_qp_order(device):
         st.commit()
     r, defl = prob.displacement(1.5e-4 * 7, 2e-5, seed=7)
     st_ref = O.CPState(prob.nqp, MAT.g0)
-    for k in ("pF_n", "g_n", "lam_n", "gdot_n"):
+    for k in ("F_n", "pF_n", "g_n", "lam_n"):
         getattr(st_ref, k)[:] = getattr(st, k)
     prob.assemble(st_ref, r, defl, tangent=False)
     # converged point in mesh order: M = pF_n pF_np1^-1, lambda, g

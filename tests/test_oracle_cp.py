@@ -14,19 +14,16 @@ def _rot(seed):
 
 
 def _march(Pg, Lg, F_of_step, nsteps, mat=MAT, dt=1.0):
-    """Single material point driven through F(step) with the plain start (previous rates)."""
-    pFn, g_n, lam_n, gd_n = np.eye(3), mat.g0, 0.0, np.zeros(12)
+    """Single material point driven through F(step) with the plain start (plastic predictor
+    pulled back to yield)."""
+    pFn, g_n, lam_n, Fn = np.eye(3), mat.g0, 0.0, np.eye(3)
     out = []
     for s in range(1, nsteps + 1):
         F = F_of_step(s)
-        x0 = None
-        if np.any(gd_n):
-            M0 = np.eye(3) - dt * np.einsum("a,aij->ij", gd_n, Pg)
-            x0 = np.r_[M0.reshape(9), lam_n, g_n]
-        r = O.cp_update(mat, Pg, Lg, F, pFn, g_n, lam_n, dt, x0=x0)
+        r = O.cp_update(mat, Pg, Lg, F, pFn, g_n, lam_n, dt, Fn=Fn)
         assert r["rc"] == 0, (s, r["iters"])
         out.append((F, pFn.copy(), g_n, lam_n, r))
-        pFn, g_n, lam_n, gd_n = r["pF"], r["g"], r["lam"], r["gdot"]
+        pFn, g_n, lam_n, Fn = r["pF"], r["g"], r["lam"], F
     return out
 
 
@@ -46,8 +43,8 @@ def test_isochoric_and_hardening_relation(seed):
         g_closed = ((gs - g0) * g_n + h0 * gs * gdot) / (gs - g0 + h0 * gdot)  # int_HA, dt = 1
         assert abs(r["g"] - g_closed) < 1e-9 * g_closed
     assert hist[-1][4]["g"] > MAT.g0  # it hardened
-    # steady plastic steps converge in a handful of iterations from the previous rates
-    assert max(h[4]["iters"] for h in hist[-6:]) <= 12
+    # steady plastic steps converge in a handful of iterations from the plain start
+    assert max(h[4]["iters"] for h in hist[-6:]) <= 8
 
 
 def test_flow_rule_at_convergence():
@@ -167,7 +164,7 @@ def test_cp_element_tangent_is_fd_of_residual(etype):
         assert err == 0, s
         st.commit()
     r, defl = prob.displacement(1.5e-4 * 11, 2e-5, seed=11)
-    n_state = {k: getattr(st, k).copy() for k in ("pF_n", "g_n", "lam_n", "gdot_n")}
+    n_state = {k: getattr(st, k).copy() for k in ("F_n", "pF_n", "g_n", "lam_n")}
     f0, vals, err = _cp_assemble(prob, st, r, defl)
     assert err == 0 and np.abs(st.gdot).max() > 1e-5  # plastic
     K = np.zeros((prob.ndof, prob.ndof))
@@ -188,3 +185,18 @@ def test_cp_element_tangent_is_fd_of_residual(etype):
             col.append(fp)
         Kfd[:, j] = (col[0] - col[1]) / (2 * h)
     assert np.abs(K - Kfd).max() / np.abs(K).max() < 1e-6
+
+
+@pytest.mark.parametrize("de", [1e-3, 5e-3])
+def test_plain_start_converges_from_rest(de):
+    """Strain increments of the polycrystal workload (1e-3 per 1 s step at 1e-3/s) and five
+    times that, from rest, for 40 random orientations: the plain start reaches tol_M on every
+    step without a step cut, and steady plastic steps take a handful of iterations."""
+    R = P.quat_to_rot(P.random_quaternions(40, 21))
+    worst = 0
+    for k in range(40):
+        Pg, Lg = O.cp_grain(MAT, R[k])
+        hist = _march(Pg, Lg, lambda s: np.diag([1 - 0.3 * de * s, 1 - 0.3 * de * s, 1 + de * s]), 6)
+        worst = max(worst, max(h[4]["iters"] for h in hist[2:]))
+        assert np.abs(hist[-1][4]["gdot"]).max() > 1e-4  # plastic
+    assert worst <= 10
