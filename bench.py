@@ -70,6 +70,16 @@ def cp_flops_per_qp(mean_iters, nne=4):
     return 9.8e3 * mean_iters + 3.5e3 + 1.2e4 + 3 * nne * 800
 
 
+def measured_traffic(key):
+    """HBM bytes per pass of the dominant kernel from the committed PMC summary (FETCH_SIZE and
+    WRITE_SIZE, calibrated against a copy of known size), or None."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "hbm_traffic.json")) as fh:
+            return json.load(fh)[key]["elem_kernel_bytes_per_pass"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def _timed_passes(h, f, vals, rd, dd, steps, warmup, dt=1.0):
     import torch
 
@@ -260,7 +270,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": j2["achieved"] / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(f"j2_tet10_{args.n}cube"),
         },
         "fp64": {"achieved_tflops": j2["tflops"], "peak_tflops": FP64_PEAK_TFLOPS,
                  "frac": j2["tflops"] / FP64_PEAK_TFLOPS},
