@@ -310,6 +310,7 @@ int pgf_create(const pgf_mesh *mesh, const int64_t *id, int64_t ndof, const int3
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return PGF_ENODEV;
   // --- validate
+  if (m.n_coel > 0) return PGF_ENOTSUP;  // interface elements are not on the device path yet
   for (int64_t e = 0; e < m.n_elem; ++e) {
     const int64_t nne = m.elem_ptr[e + 1] - m.elem_ptr[e];
     if (!(nne == 4 || nne == 10 || nne == 8)) return PGF_ENOTSUP;
