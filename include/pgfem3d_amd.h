@@ -87,8 +87,17 @@ typedef struct pgf_mesh {
   const int32_t *elem_mat;   /* [n_elem] material index, NULL = all 0 */
   int64_t n_coel;            /* cohesive elements (COEL): they enter the DOF / CSR graph */
   const int64_t *coel_ptr;   /* [n_coel+1] or NULL */
-  const int64_t *coel_conn;  /* node ids or NULL */
+  const int64_t *coel_conn;  /* node ids or NULL; 6 (3 + 3) or 8 (4 + 4) nodes per element,
+                                - side first (COEL::nod, include/cohesive_element.h:17-60) */
 } pgf_mesh;
+
+/* Interface (cohesive) element properties for the exponential traction-separation law,
+ * in the reference's property order (Needleman_pkey, include/cohesive_potentials.h:38-41;
+ * read_cohesive_properties, src/cohesive_potentials.cc:19-135): peak traction sig_c at the
+ * effective opening chi_c, shear weight beta, contact penalty shape alpha. */
+typedef struct pgf_cohesive {
+  double sig_c, chi_c, beta, alpha;
+} pgf_cohesive;
 
 /* ==================================================================== set-up (host) */
 
@@ -150,12 +159,23 @@ int pgf_get_sizes(pgf_handle h, int64_t *n_qp, int64_t *nnz, int64_t *ndof,
  * every element uses the crystal frame. */
 int pgf_set_orientations(pgf_handle h, int32_t n_grains, const double *R, const int32_t *elem_grain);
 
+/* Interface element properties: n_props records and the property index of every interface
+ * element ([n_coel], host; NULL = all 0).  Required before a pass when the mesh has
+ * interface elements (COEL::props, reset_coel_props, src/cohesive_element.cc:50-54). */
+int pgf_set_cohesive_props(pgf_handle h, int32_t n_props, const pgf_cohesive *props,
+                           const int32_t *coel_mat);
+
+/* Interface sizes: QPs (3 per triangle, 4 per quad element) and colours of the interface
+ * elements (launched after the volume colours). */
+int pgf_get_cohesive_sizes(pgf_handle h, int64_t *n_cqp, int32_t *n_colors);
+
 /* Time step size dt = t(n+1) - t(n) used by rate-dependent models (CM_Ctx::dt). */
 int pgf_set_time_step(pgf_handle h, double dt);
 
 /* Crystal plasticity: start the next integrations' local Newton from a caller-provided point
  * (device pointer, [11][n_qp] in DEVICE QP order: M = pF(n) pF(n+1)^-1 (9), lambda, g), e.g. a
- * learned prediction; NULL restores the plain start (slip rates of the last converged step).
+ * learned prediction; NULL restores the plain start (the fully plastic predictor pulled back
+ * to yield).
  * No counterpart in the reference; the converged result does not depend on the start. */
 int pgf_set_warm_start(pgf_handle h, const double *warm);
 
@@ -225,6 +245,9 @@ int pgf_reset_step(pgf_handle h);
 #define PGF_STATE_GDOT 11        /* CP: 12 slip rates at n+1 (TENSOR_gamma_dot) */
 #define PGF_STATE_GDOT_N 12      /* CP: at n (TENSOR_gamma_dot_n) */
 #define PGF_STATE_ITERS 13       /* CP: local Newton iterations of the last integration */
+#define PGF_STATE_COHESIVE_NP1 14 /* interface QPs: X_eff_max, t_max at n+1, [2][n_cqp] in
+                                     interface order (element order, then QP) */
+#define PGF_STATE_COHESIVE_N 15   /* at n (COEL::vars, src/cohesive_element.cc:152-156) */
 /* Number of components of a field (0 if the field does not exist for the mesh). */
 int pgf_state_ncomp(pgf_handle h, int32_t field, int32_t *ncomp);
 /* Copy a field to / from host memory laid out [comp][n_qp]. */

@@ -63,4 +63,20 @@ int orc_assemble(int64_t n_nodes, const double *X, int64_t n_elem, const int64_t
                  double dt, int do_residual, int run_integration, int do_tangent,
                  double *F_np1, const double *plastic_n, double *plastic_np1,
                  const double *scalar_n, double *scalar_np1, double *f, double *vals);
+
+// --- cohesive interface elements (oracle_coh.cpp): Needleman exponential law,
+// props = (sig_c, chi_c, beta, alpha) in Needleman_pkey order
+struct orc_cohesive {
+  double sig_c, chi_c, beta, alpha;
+};
+int orc_coh_traction(const orc_cohesive *p, const double *jump, const double *n, const double *vars,
+                     double *t);
+int orc_coh_tangents(const orc_cohesive *p, const double *jump, const double *n, const double *vars,
+                     double *mat_tan, double *geo_tan);
+int orc_coh_element(const orc_cohesive *p, int nn, const double *x0, const double *u, const double *vars_n,
+                    double *fe, double *K, double *vars_np1);
+int orc_assemble_coh(const double *X, int64_t n_coel, const int64_t *coel_ptr, const int64_t *coel_conn,
+                     const int32_t *coel_mat, const orc_cohesive *props, const int64_t *id, int64_t ndof,
+                     const int32_t *Ap, const int32_t *Ai, const double *r, const double *d_r, const double *defl,
+                     const double *defl_d, const double *vars_n, double *vars_np1, double *f, double *vals);
 }

@@ -262,3 +262,87 @@ def assemble_cp(mesh, mats, elem_grain, grain_P, grain_L, ids, ndof, Ap, Ai, st:
                             _p(st.pF_np1), _p(st.g_n), _p(st.g_np1), _p(st.lam_n), _p(st.lam_np1), _p(st.tau),
                             _p(st.gdot), _p(st.iters), _p(st.F_n), _p(warm), _p(f), _p(vals))
     return f, vals, err
+
+
+# ----------------------------------------------------------------------------- cohesive
+class OrcCohesive(C.Structure):
+    _fields_ = [("sig_c", C.c_double), ("chi_c", C.c_double), ("beta", C.c_double), ("alpha", C.c_double)]
+
+
+def _coh_lib():
+    L = lib()
+    if not getattr(L, "_coh_ready", False):
+        L.orc_coh_traction.argtypes = [C.c_void_p] * 5
+        L.orc_coh_traction.restype = C.c_int
+        L.orc_coh_tangents.argtypes = [C.c_void_p] * 6
+        L.orc_coh_tangents.restype = C.c_int
+        L.orc_coh_element.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 6
+        L.orc_coh_element.restype = C.c_int
+        L.orc_assemble_coh.argtypes = ([C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                        C.c_void_p, C.c_int64] + [C.c_void_p] * 10)
+        L.orc_assemble_coh.restype = C.c_int
+        L._coh_ready = True
+    return L
+
+
+def _coh(props):
+    return OrcCohesive(props.sig_c, props.chi_c, props.beta, props.alpha)
+
+
+def coh_traction(props, jump, n, vars_n=(0.0, 0.0)):
+    t = np.zeros(3)
+    v = np.ascontiguousarray(vars_n, dtype=np.float64)
+    p = _coh(props)
+    rc = _coh_lib().orc_coh_traction(C.byref(p), _p(np.ascontiguousarray(jump, dtype=np.float64)),
+                                     _p(np.ascontiguousarray(n, dtype=np.float64)), _p(v), _p(t))
+    return t, rc
+
+
+def coh_tangents(props, jump, n, vars_n=(0.0, 0.0)):
+    mt = np.zeros(9)
+    gt = np.zeros(9)
+    v = np.ascontiguousarray(vars_n, dtype=np.float64)
+    p = _coh(props)
+    _coh_lib().orc_coh_tangents(C.byref(p), _p(np.ascontiguousarray(jump, dtype=np.float64)),
+                                _p(np.ascontiguousarray(n, dtype=np.float64)), _p(v), _p(mt), _p(gt))
+    return mt.reshape(3, 3), gt.reshape(3, 3)
+
+
+def coh_element(props, x0, u, vars_n=None):
+    """One interface element with nn = len(x0)/2 nodes per side:
+    (fe[6nn], K[6nn,6nn], vars_np1[nn,2], n_failed)."""
+    x0 = np.ascontiguousarray(x0, dtype=np.float64).reshape(-1, 3)
+    nn = x0.shape[0] // 2
+    nd = 6 * nn
+    fe = np.zeros(nd)
+    K = np.zeros(nd * nd)
+    v1 = np.zeros(2 * nn)
+    vn = np.zeros(2 * nn) if vars_n is None else np.ascontiguousarray(vars_n, dtype=np.float64).reshape(2 * nn)
+    p = _coh(props)
+    err = _coh_lib().orc_coh_element(C.byref(p), nn, _p(x0), _p(np.ascontiguousarray(u, dtype=np.float64).reshape(nd)),
+                                     _p(vn), _p(fe), _p(K), _p(v1))
+    return fe, K.reshape(nd, nd), v1.reshape(nn, 2), err
+
+
+class CohState:
+    """Interface state [2][n_cqp] (X_eff_max, t_max) at n and n+1, cohesive-QP order (nn QPs
+    per element, element by element)."""
+
+    def __init__(self, n_cqp):
+        self.vars_n = np.zeros((2, n_cqp))
+        self.vars_np1 = np.zeros((2, n_cqp))
+
+    def commit(self):
+        self.vars_n[:] = self.vars_np1
+
+
+def assemble_coh(mesh, props_list, coel_mat, ids, ndof, Ap, Ai, st: CohState, r, f, vals, d_r=None, defl=None,
+                 defl_d=None):
+    """Add the interface elements' residual into f and tangent into vals (in place)."""
+    L = _coh_lib()
+    pr = (OrcCohesive * len(props_list))(*[_coh(p) for p in props_list])
+    cm = None if coel_mat is None else np.ascontiguousarray(coel_mat, dtype=np.int32)
+    err = L.orc_assemble_coh(_p(mesh.X), mesh.n_coel, _p(mesh.coel_ptr), _p(mesh.coel_conn), _p(cm), pr, _p(ids),
+                             ndof, _p(Ap), _p(Ai), _p(r), _p(d_r), _p(defl), _p(defl_d), _p(st.vars_n),
+                             _p(st.vars_np1), _p(f), _p(vals))
+    return err

@@ -31,6 +31,16 @@ struct Block {
   int32_t *d_qgrain = nullptr;
 };
 
+// Interface elements of one kind (nn nodes per side), colour-sorted.
+struct CohBlock {
+  int nn = 0;
+  int64_t nc = 0, cqp_base = 0;
+  std::vector<int64_t> elems;      // mesh interface-element ids in device order
+  std::vector<int64_t> color_ptr;  // [n_ccolors+1]
+  int32_t *d_conn = nullptr, *d_mat = nullptr;
+  int16_t *d_off = nullptr;
+};
+
 template <class T>
 int dalloc(T **p, size_t n) {
   *p = nullptr;
@@ -112,6 +122,13 @@ struct pgf_handle_s {
   const double *warm = nullptr;
   double cp_g0 = 0.0, dt = 1.0;
   std::vector<double> cp_mat_host;  // C11, C12, C44
+  // interface elements
+  std::vector<pgf::CohBlock> cblocks;
+  int32_t n_ccolors = 0;
+  int64_t n_coel = 0, n_cqp = 0;
+  std::vector<int64_t> cqp_mesh;  // device interface QP -> mesh interface QP
+  double *d_cv[3] = {nullptr, nullptr, nullptr};  // [2][n_cqp] (X_eff_max, t_max): n+1, n, n-1
+  pgf::DevCohesive *d_cprops = nullptr;
   bool profiling = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_ms = 0.0;
@@ -210,6 +227,7 @@ pgf::ElemArgs base_args(pgf_handle h, const pgf_nodal_input *in) {
 // One pass over all elements in the coloured schedule.
 int run_pass(pgf_handle h, const pgf_nodal_input *in, int mode, double *f, double *vals) {
   if (!in || !in->r) return PGF_EINVAL;
+  if (h->n_coel && !h->d_cprops) return PGF_EINVAL;  // interface properties not set
   if (hipMemsetAsync(h->d_err, 0, sizeof(int32_t), h->stream) != hipSuccess) return PGF_EHIP;
   pgf::ElemArgs a = base_args(h, in);
   a.f = f;
@@ -251,6 +269,35 @@ int run_pass(pgf_handle h, const pgf_nodal_input *in, int mode, double *f, doubl
       }
       ++launches;
     }
+  if (h->n_coel) {  // interface elements after every volume colour
+    pgf::CohArgs ca{};
+    ca.X = h->d_X;
+    ca.dofid = h->d_dofid;
+    ca.row_start = h->d_row_start;
+    ca.rowlen = h->d_rowlen;
+    ca.props = h->d_cprops;
+    ca.n_cqp = h->n_cqp;
+    ca.r = in->r;
+    ca.d_r = in->d_r;
+    ca.defl = in->defl;
+    ca.defl_d = in->defl_d;
+    ca.vars_n = h->d_cv[1];
+    ca.vars_np1 = h->d_cv[0];
+    ca.f = f;
+    ca.vals = vals;
+    ca.err = h->d_err;
+    for (int32_t c = 0; c < h->n_ccolors; ++c)
+      for (auto &cb : h->cblocks) {
+        const int64_t e0 = cb.color_ptr[c], e1 = cb.color_ptr[c + 1];
+        if (e1 <= e0) continue;
+        ca.conn = cb.d_conn;
+        ca.mat = cb.d_mat;
+        ca.off = cb.d_off;
+        ca.cqp_base = cb.cqp_base;
+        pgf::launch_coh_elem(cb.nn, mode, ca, e0, e1, h->stream);
+        ++launches;
+      }
+  }
   if (h->profiling) hipEventRecord(h->ev1, h->stream);
   h->last_launches = launches;
   if (hipGetLastError() != hipSuccess) return PGF_EHIP;
@@ -310,7 +357,12 @@ int pgf_create(const pgf_mesh *mesh, const int64_t *id, int64_t ndof, const int3
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return PGF_ENODEV;
   // --- validate
-  if (m.n_coel > 0) return PGF_ENOTSUP;  // interface elements are not on the device path yet
+  for (int64_t c = 0; c < m.n_coel; ++c) {
+    const int64_t nt = m.coel_ptr[c + 1] - m.coel_ptr[c];
+    if (!(nt == 6 || nt == 8)) return PGF_ENOTSUP;  // 3 + 3 or 4 + 4 node interfaces
+    for (int64_t k = m.coel_ptr[c]; k < m.coel_ptr[c + 1]; ++k)
+      if (m.coel_conn[k] < 0 || m.coel_conn[k] >= m.n_nodes) return PGF_EINVAL;
+  }
   for (int64_t e = 0; e < m.n_elem; ++e) {
     const int64_t nne = m.elem_ptr[e + 1] - m.elem_ptr[e];
     if (!(nne == 4 || nne == 10 || nne == 8)) return PGF_ENOTSUP;
@@ -445,8 +497,83 @@ int pgf_create(const pgf_mesh *mesh, const int64_t *id, int64_t ndof, const int3
           }
       }
     }
+  // --- interface elements: own colours after the volume ones, first-touch marks continued
+  std::vector<std::vector<int16_t>> cboff;
+  if (m.n_coel > 0) {
+    std::vector<int32_t> ccolor;
+    const int nccol = pgf::color_coels(m, ccolor);
+    if (nccol < 0) return fail(PGF_ENOTSUP);
+    h->n_coel = m.n_coel;
+    h->n_ccolors = nccol;
+    for (int nn : {3, 4}) {
+      pgf::CohBlock cb;
+      cb.nn = nn;
+      for (int64_t c = 0; c < m.n_coel; ++c)
+        if (m.coel_ptr[c + 1] - m.coel_ptr[c] == 2 * nn) cb.elems.push_back(c);
+      if (cb.elems.empty()) continue;
+      std::stable_sort(cb.elems.begin(), cb.elems.end(), [&](int64_t x, int64_t y) { return ccolor[x] < ccolor[y]; });
+      cb.nc = int64_t(cb.elems.size());
+      cb.color_ptr.assign(nccol + 1, 0);
+      for (int64_t c : cb.elems) cb.color_ptr[ccolor[c] + 1]++;
+      for (int c = 0; c < nccol; ++c) cb.color_ptr[c + 1] += cb.color_ptr[c];
+      cb.cqp_base = h->n_cqp;
+      h->n_cqp += cb.nc * nn;
+      h->cblocks.push_back(std::move(cb));
+    }
+    std::vector<int64_t> cq_of(m.n_coel + 1, 0);
+    for (int64_t c = 0; c < m.n_coel; ++c) cq_of[c + 1] = cq_of[c] + (m.coel_ptr[c + 1] - m.coel_ptr[c]) / 2;
+    h->cqp_mesh.resize(h->n_cqp);
+    for (auto &cb : h->cblocks)
+      for (int64_t k = 0; k < cb.nc; ++k)
+        for (int q = 0; q < cb.nn; ++q) h->cqp_mesh[cb.cqp_base + k * cb.nn + q] = cq_of[cb.elems[k]] + q;
+    std::vector<int64_t> cpair_ptr;
+    std::vector<int16_t> coff;
+    if (pgf::build_coel_offsets(m, id, Ap, Ai, cpair_ptr, coff) != PGF_OK) return fail(PGF_EINVAL);
+    cboff.resize(h->cblocks.size());
+    for (size_t bi = 0; bi < h->cblocks.size(); ++bi) {
+      const int nt = 2 * h->cblocks[bi].nn;
+      cboff[bi].resize(h->cblocks[bi].nc * nt * nt);
+    }
+    for (int32_t c = 0; c < nccol; ++c)
+      for (size_t bi = 0; bi < h->cblocks.size(); ++bi) {
+        pgf::CohBlock &cb = h->cblocks[bi];
+        const int nt = 2 * cb.nn;
+        for (int64_t k = cb.color_ptr[c]; k < cb.color_ptr[c + 1]; ++k) {
+          const int64_t ce = cb.elems[k];
+          const int64_t *conn = m.coel_conn + m.coel_ptr[ce];
+          for (int a = 0; a < nt; ++a)
+            for (int w = 0; w < nt; ++w) {
+              int16_t o = coff[cpair_ptr[ce] + a * nt + w];
+              if (o >= 0) {
+                const int64_t pos = int64_t(row_start[conn[a]]) + o;
+                if (!touched[pos]) {
+                  touched[pos] = 1;
+                  o = int16_t(-int(o) - 2);
+                }
+              }
+              cboff[bi][(k * nt + a) * nt + w] = o;
+            }
+        }
+      }
+  }
   // --- upload
   int rc = PGF_OK;
+  for (size_t bi = 0; bi < h->cblocks.size(); ++bi) {
+    pgf::CohBlock &cb = h->cblocks[bi];
+    const int nt = 2 * cb.nn;
+    std::vector<int32_t> conn(cb.nc * nt), zero(cb.nc, 0);
+    for (int64_t k = 0; k < cb.nc; ++k)
+      for (int a = 0; a < nt; ++a) conn[k * nt + a] = int32_t(m.coel_conn[m.coel_ptr[cb.elems[k]] + a]);
+    if ((rc = pgf::upload(&cb.d_conn, conn.data(), conn.size()))) return fail(rc);
+    if ((rc = pgf::upload(&cb.d_mat, zero.data(), zero.size()))) return fail(rc);
+    if ((rc = pgf::upload(&cb.d_off, cboff[bi].data(), cboff[bi].size()))) return fail(rc);
+  }
+  if (h->n_cqp) {
+    for (int t = 0; t < 3; ++t) {
+      if ((rc = pgf::dalloc(&h->d_cv[t], size_t(h->n_cqp) * 2))) return fail(rc);
+      pgf::launch_fill(h->d_cv[t], 2 * h->n_cqp, 0.0, h->stream);
+    }
+  }
   std::vector<pgf::DevMaterial> dm(n_mat);
   for (int32_t i = 0; i < n_mat; ++i) {
     dm[i].model = mats[i].model;
@@ -534,6 +661,13 @@ int pgf_destroy(pgf_handle h) {
     hipFree(b.d_egrain);
     hipFree(b.d_qgrain);
   }
+  for (auto &cb : h->cblocks) {
+    hipFree(cb.d_conn);
+    hipFree(cb.d_mat);
+    hipFree(cb.d_off);
+  }
+  for (int t = 0; t < 3; ++t) hipFree(h->d_cv[t]);
+  hipFree(h->d_cprops);
   hipFree(h->d_cpmat);
   hipFree(h->d_grains);
   for (int t = 0; t < 2; ++t) {
@@ -626,6 +760,10 @@ int pgf_commit_step(pgf_handle h) {
     pgf::launch_copy(h->d_tau[1], h->d_tau[0], 12 * NQ, h->stream);
     pgf::launch_copy(h->d_gdot[1], h->d_gdot[0], 12 * NQ, h->stream);
   }
+  if (h->n_cqp) {  // interface history (increment_cohesive_elements, src/cohesive_element.cc:645-700)
+    pgf::launch_copy(h->d_cv[2], h->d_cv[1], 2 * h->n_cqp, h->stream);
+    pgf::launch_copy(h->d_cv[1], h->d_cv[0], 2 * h->n_cqp, h->stream);
+  }
   return hipStreamSynchronize(h->stream) == hipSuccess ? PGF_OK : PGF_EHIP;
 }
 
@@ -641,11 +779,16 @@ int pgf_reset_step(pgf_handle h) {
     pgf::launch_copy(h->d_tau[0], h->d_tau[1], 12 * NQ, h->stream);
     pgf::launch_copy(h->d_gdot[0], h->d_gdot[1], 12 * NQ, h->stream);
   }
+  if (h->n_cqp) pgf::launch_copy(h->d_cv[0], h->d_cv[1], 2 * h->n_cqp, h->stream);
   return hipStreamSynchronize(h->stream) == hipSuccess ? PGF_OK : PGF_EHIP;
 }
 
 int pgf_state_ncomp(pgf_handle h, int32_t field, int32_t *ncomp) {
   if (!h || !ncomp) return PGF_EINVAL;
+  if (field == PGF_STATE_COHESIVE_NP1 || field == PGF_STATE_COHESIVE_N) {
+    *ncomp = h->n_cqp ? 2 : 0;
+    return PGF_OK;
+  }
   if (field == PGF_STATE_ITERS) {
     *ncomp = h->has_cp ? 1 : 0;
     return PGF_OK;
@@ -656,6 +799,18 @@ int pgf_state_ncomp(pgf_handle h, int32_t field, int32_t *ncomp) {
 
 int pgf_get_state(pgf_handle h, int32_t field, double *host_out) {
   if (!h || !host_out) return PGF_EINVAL;
+  if (field == PGF_STATE_COHESIVE_NP1 || field == PGF_STATE_COHESIVE_N) {
+    if (!h->n_cqp) return PGF_EINVAL;
+    const int64_t NC = h->n_cqp;
+    std::vector<double> tmp(2 * NC);
+    if (hipStreamSynchronize(h->stream) != hipSuccess) return PGF_EHIP;
+    if (hipMemcpy(tmp.data(), h->d_cv[field == PGF_STATE_COHESIVE_NP1 ? 0 : 1], sizeof(double) * 2 * NC,
+                  hipMemcpyDeviceToHost) != hipSuccess)
+      return PGF_EHIP;
+    for (int c = 0; c < 2; ++c)
+      for (int64_t q = 0; q < NC; ++q) host_out[c * NC + h->cqp_mesh[q]] = tmp[c * NC + q];
+    return PGF_OK;
+  }
   if (field == PGF_STATE_ITERS) {
     if (!h->has_cp) return PGF_EINVAL;
     std::vector<int32_t> it(h->n_qp);
@@ -680,6 +835,18 @@ int pgf_get_state(pgf_handle h, int32_t field, double *host_out) {
 
 int pgf_set_state(pgf_handle h, int32_t field, const double *host_in) {
   if (!h || !host_in) return PGF_EINVAL;
+  if (field == PGF_STATE_COHESIVE_NP1 || field == PGF_STATE_COHESIVE_N) {
+    if (!h->n_cqp) return PGF_EINVAL;
+    const int64_t NC = h->n_cqp;
+    std::vector<double> tmp(2 * NC);
+    for (int c = 0; c < 2; ++c)
+      for (int64_t q = 0; q < NC; ++q) tmp[c * NC + q] = host_in[c * NC + h->cqp_mesh[q]];
+    if (hipStreamSynchronize(h->stream) != hipSuccess) return PGF_EHIP;
+    if (hipMemcpy(h->d_cv[field == PGF_STATE_COHESIVE_NP1 ? 0 : 1], tmp.data(), sizeof(double) * 2 * NC,
+                  hipMemcpyHostToDevice) != hipSuccess)
+      return PGF_EHIP;
+    return PGF_OK;
+  }
   double *buf;
   int32_t nc;
   int rc = field_buffer(h, field, &buf, &nc);
@@ -741,6 +908,36 @@ int pgf_set_orientations(pgf_handle h, int32_t n_grains, const double *R, const 
         hipMemcpy(b.d_qgrain, qg.data(), qg.size() * sizeof(int32_t), hipMemcpyHostToDevice) != hipSuccess)
       return PGF_EHIP;
   }
+  return PGF_OK;
+}
+
+int pgf_set_cohesive_props(pgf_handle h, int32_t n_props, const pgf_cohesive *props, const int32_t *coel_mat) {
+  if (!h || n_props < 1 || !props) return PGF_EINVAL;
+  if (!h->n_coel) return PGF_EINVAL;
+  std::vector<pgf::DevCohesive> dp(n_props);
+  for (int32_t i = 0; i < n_props; ++i) {
+    if (!(props[i].chi_c > 0.0) || !(props[i].alpha != 0.0)) return PGF_EINVAL;
+    dp[i] = {props[i].sig_c, props[i].chi_c, props[i].beta, props[i].alpha};
+  }
+  for (auto &cb : h->cblocks) {
+    std::vector<int32_t> mat(cb.nc);
+    for (int64_t k = 0; k < cb.nc; ++k) {
+      const int32_t mi = coel_mat ? coel_mat[cb.elems[k]] : 0;
+      if (mi < 0 || mi >= n_props) return PGF_EINVAL;
+      mat[k] = mi;
+    }
+    if (hipMemcpy(cb.d_mat, mat.data(), mat.size() * sizeof(int32_t), hipMemcpyHostToDevice) != hipSuccess)
+      return PGF_EHIP;
+  }
+  hipFree(h->d_cprops);
+  h->d_cprops = nullptr;
+  return pgf::upload(&h->d_cprops, dp.data(), dp.size());
+}
+
+int pgf_get_cohesive_sizes(pgf_handle h, int64_t *n_cqp, int32_t *n_colors) {
+  if (!h) return PGF_EINVAL;
+  if (n_cqp) *n_cqp = h->n_cqp;
+  if (n_colors) *n_colors = h->n_ccolors;
   return PGF_OK;
 }
 

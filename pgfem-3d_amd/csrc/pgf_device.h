@@ -112,6 +112,28 @@ struct CPElemArgs {
 };
 void launch_cp_elem(int nne, int mode, const CPElemArgs &a, int64_t e_begin, int64_t e_end, hipStream_t s);
 
+// ---------------------------------------------------------------- interface elements
+// Exponential traction-separation law parameters (Needleman_pkey order).
+struct DevCohesive {
+  double sig_c, chi_c, beta, alpha;
+};
+
+struct CohArgs {
+  const double *X;
+  const int32_t *dofid, *row_start, *rowlen;
+  const int32_t *conn;     // [nc][2*nn] block connectivity (- side nodes, then + side)
+  const int32_t *mat;      // [nc] property index
+  const int16_t *off;      // [nc][2nn][2nn] scatter offsets (same encoding as ElemArgs::off)
+  const DevCohesive *props;
+  int64_t cqp_base, n_cqp; // first QP of the block, total interface QPs (SoA stride)
+  const double *r, *d_r, *defl, *defl_d;
+  const double *vars_n;    // [2][n_cqp] X_eff_max, t_max at n
+  double *vars_np1;        // [2][n_cqp] at n+1 (written by integrating passes)
+  double *f, *vals;
+  int32_t *err;
+};
+void launch_coh_elem(int nn, int mode, const CohArgs &a, int64_t e_begin, int64_t e_end, hipStream_t s);
+
 void launch_elem(int nne, int mode, const ElemArgs &a, int64_t e_begin, int64_t e_end,
                  hipStream_t s);
 void launch_fill(double *p, int64_t n, double v, hipStream_t s);

@@ -27,4 +27,12 @@ int build_scatter_offsets(const pgf_mesh &m, const int64_t *id, int64_t ndof,
                           const int32_t *Ap, const int32_t *Ai,
                           std::vector<int64_t> &pair_ptr, std::vector<int16_t> &off);
 
+// The same for the interface elements (pair_ptr over coel_ptr).
+int build_coel_offsets(const pgf_mesh &m, const int64_t *id, const int32_t *Ap, const int32_t *Ai,
+                       std::vector<int64_t> &pair_ptr, std::vector<int16_t> &off);
+
+// Greedy colouring of the interface elements among themselves (they are launched after all
+// volume colours, so they never race with volume elements).
+int color_coels(const pgf_mesh &m, std::vector<int32_t> &color);
+
 }  // namespace pgf

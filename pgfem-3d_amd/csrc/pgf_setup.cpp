@@ -121,20 +121,21 @@ static void node_columns(const pgf_mesh &m, const Incidence &inc, const int64_t 
   }
 }
 
-int build_scatter_offsets(const pgf_mesh &m, const int64_t *id, int64_t ndof,
-                          const int32_t *Ap, const int32_t *Ai,
-                          std::vector<int64_t> &pair_ptr, std::vector<int16_t> &off) {
-  pair_ptr.assign(m.n_elem + 1, 0);
-  for (int64_t e = 0; e < m.n_elem; ++e) {
-    const int64_t nne = nodes_of(m, e);
+// Node-pair offsets of elements given by (ptr, conn): see build_scatter_offsets.
+static int pair_offsets(int64_t n_el, const int64_t *ptr, const int64_t *conn_all, const int64_t *id,
+                        const int32_t *Ap, const int32_t *Ai, std::vector<int64_t> &pair_ptr,
+                        std::vector<int16_t> &off) {
+  pair_ptr.assign(n_el + 1, 0);
+  for (int64_t e = 0; e < n_el; ++e) {
+    const int64_t nne = ptr[e + 1] - ptr[e];
     pair_ptr[e + 1] = pair_ptr[e] + nne * nne;
   }
-  off.assign(pair_ptr[m.n_elem], int16_t(-1));
+  off.assign(pair_ptr[n_el], int16_t(-1));
   int bad = 0;
 #pragma omp parallel for schedule(static) reduction(+ : bad)
-  for (int64_t e = 0; e < m.n_elem; ++e) {
-    const int64_t nne = nodes_of(m, e);
-    const int64_t *conn = m.elem_conn + m.elem_ptr[e];
+  for (int64_t e = 0; e < n_el; ++e) {
+    const int64_t nne = ptr[e + 1] - ptr[e];
+    const int64_t *conn = conn_all + ptr[e];
     for (int64_t a = 0; a < nne; ++a) {
       int64_t ra = -1;
       for (int j = 0; j < 3 && ra < 0; ++j)
@@ -155,8 +156,40 @@ int build_scatter_offsets(const pgf_mesh &m, const int64_t *id, int64_t ndof,
       }
     }
   }
-  (void)ndof;
   return bad ? PGF_EINVAL : PGF_OK;
+}
+
+int build_scatter_offsets(const pgf_mesh &m, const int64_t *id, int64_t ndof,
+                          const int32_t *Ap, const int32_t *Ai,
+                          std::vector<int64_t> &pair_ptr, std::vector<int16_t> &off) {
+  (void)ndof;
+  return pair_offsets(m.n_elem, m.elem_ptr, m.elem_conn, id, Ap, Ai, pair_ptr, off);
+}
+
+int build_coel_offsets(const pgf_mesh &m, const int64_t *id, const int32_t *Ap, const int32_t *Ai,
+                       std::vector<int64_t> &pair_ptr, std::vector<int16_t> &off) {
+  return pair_offsets(m.n_coel, m.coel_ptr, m.coel_conn, id, Ap, Ai, pair_ptr, off);
+}
+
+int color_coels(const pgf_mesh &m, std::vector<int32_t> &color) {
+  constexpr int kWords = 4;
+  std::vector<uint64_t> used(size_t(m.n_nodes) * kWords, 0);
+  color.assign(m.n_coel, -1);
+  int ncol = 0;
+  for (int64_t c = 0; c < m.n_coel; ++c) {
+    uint64_t forb[kWords] = {0, 0, 0, 0};
+    for (int64_t k = m.coel_ptr[c]; k < m.coel_ptr[c + 1]; ++k)
+      for (int w = 0; w < kWords; ++w) forb[w] |= used[m.coel_conn[k] * kWords + w];
+    int col = -1;
+    for (int w = 0; w < kWords && col < 0; ++w)
+      if (~forb[w]) col = w * 64 + __builtin_ctzll(~forb[w]);
+    if (col < 0) return -1;
+    color[c] = col;
+    ncol = std::max(ncol, col + 1);
+    for (int64_t k = m.coel_ptr[c]; k < m.coel_ptr[c + 1]; ++k)
+      used[m.coel_conn[k] * kWords + col / 64] |= (uint64_t(1) << (col % 64));
+  }
+  return ncol;
 }
 
 }  // namespace pgf

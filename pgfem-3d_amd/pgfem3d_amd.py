@@ -28,6 +28,7 @@ STATE_F_NP1, STATE_F_N, STATE_F_NM1 = 0, 1, 2
 STATE_PLASTIC_NP1, STATE_PLASTIC_N, STATE_PLASTIC_NM1 = 3, 4, 5
 STATE_SCALAR_NP1, STATE_SCALAR_N, STATE_SCALAR_NM1 = 6, 7, 8
 STATE_TAU, STATE_TAU_N, STATE_GDOT, STATE_GDOT_N, STATE_ITERS = 9, 10, 11, 12, 13
+STATE_COHESIVE_NP1, STATE_COHESIVE_N = 14, 15
 
 ERRORS = {-1: "PGF_EINVAL", -2: "PGF_ENOMEM", -3: "PGF_EHIP", -4: "PGF_ENOTSUP", -5: "PGF_ENODEV"}
 
@@ -57,6 +58,12 @@ class Mesh(C.Structure):
         ("elem_conn", C.c_void_p), ("elem_mat", C.c_void_p), ("n_coel", C.c_int64),
         ("coel_ptr", C.c_void_p), ("coel_conn", C.c_void_p),
     ]
+
+
+class Cohesive(C.Structure):
+    """pgf_cohesive: exponential traction-separation law (sig_c, chi_c, beta, alpha)."""
+
+    _fields_ = [("sig_c", C.c_double), ("chi_c", C.c_double), ("beta", C.c_double), ("alpha", C.c_double)]
 
 
 class NodalInput(C.Structure):
@@ -97,6 +104,8 @@ def lib() -> C.CDLL:
             "pgf_set_warm_start": [P, P],
             "pgf_get_qp_order": [P, P],
             "pgf_state_device_ptr": [P, I32, P],
+            "pgf_set_cohesive_props": [P, I32, P, P],
+            "pgf_get_cohesive_sizes": [P, P, P],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -113,7 +122,8 @@ def exported_symbols():
         "pgf_set_stream", "pgf_set_profiling", "pgf_last_pass_ms", "pgf_synchronize", "pgf_get_sizes", "pgf_residual", "pgf_tangent",
         "pgf_residual_tangent", "pgf_commit_step", "pgf_reset_step", "pgf_state_ncomp",
         "pgf_get_state", "pgf_set_state", "pgf_set_orientations", "pgf_set_time_step",
-        "pgf_set_warm_start", "pgf_get_qp_order", "pgf_state_device_ptr",
+        "pgf_set_warm_start", "pgf_get_qp_order", "pgf_state_device_ptr", "pgf_set_cohesive_props",
+        "pgf_get_cohesive_sizes",
     ]
 
 
@@ -149,6 +159,10 @@ class MeshArrays:
     @property
     def n_elem(self) -> int:
         return self.elem_ptr.shape[0] - 1
+
+    @property
+    def n_coel(self) -> int:
+        return 0 if self.coel_ptr is None else self.coel_ptr.shape[0] - 1
 
     def struct(self) -> Mesh:
         m = Mesh()
@@ -241,6 +255,9 @@ class Handle:
         nq, nz, nd, nc = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int32()
         _check(self._lib.pgf_get_sizes(h, C.byref(nq), C.byref(nz), C.byref(nd), C.byref(nc)), "pgf_get_sizes")
         self.n_qp, self.nnz, self.ndof, self.n_colors = nq.value, nz.value, nd.value, nc.value
+        ncq, ncc = C.c_int64(), C.c_int32()
+        _check(self._lib.pgf_get_cohesive_sizes(h, C.byref(ncq), C.byref(ncc)), "pgf_get_cohesive_sizes")
+        self.n_cqp, self.n_ccolors = ncq.value, ncc.value
         self.n_prescribed = int(max(0, -int(np.min(ids)))) if ids.size else 0
 
     def close(self):
@@ -314,13 +331,22 @@ class Handle:
     def get_state(self, field: int) -> np.ndarray:
         nc = C.c_int32()
         _check(self._lib.pgf_state_ncomp(self.h, field, C.byref(nc)), "pgf_state_ncomp")
-        out = np.zeros((nc.value, self.n_qp))
+        npts = self.n_cqp if field in (STATE_COHESIVE_NP1, STATE_COHESIVE_N) else self.n_qp
+        out = np.zeros((nc.value, npts))
         _check(self._lib.pgf_get_state(self.h, field, _ptr(out)), "pgf_get_state")
         return out
 
     def set_state(self, field: int, values: np.ndarray):
         v = np.ascontiguousarray(values, dtype=np.float64)
         _check(self._lib.pgf_set_state(self.h, field, _ptr(v)), "pgf_set_state")
+
+    def set_cohesive_props(self, props, coel_mat=None):
+        """Interface properties (a list of Cohesive) and the property index of each interface
+        element (None = all 0)."""
+        arr = (Cohesive * len(props))(*props)
+        cm = None if coel_mat is None else np.ascontiguousarray(coel_mat, dtype=np.int32)
+        _check(self._lib.pgf_set_cohesive_props(self.h, len(props), arr, _ptr(cm)), "pgf_set_cohesive_props")
+        self._coh_keep = (arr, cm)
 
     def state_device_ptr(self, field: int) -> int:
         """Device address of a state field ([ncomp][n_qp], device QP order)."""
@@ -395,3 +421,55 @@ def voronoi_grains(mesh: MeshArrays, n_grains: int, seed: int):
         d = ((cen[s:s + 65536, None, :] - pts[None, :, :]) ** 2).sum(-1)
         grain[s:s + 65536] = d.argmin(axis=1)
     return grain, random_quaternions(n_grains, seed + 1)
+
+
+# ------------------------------------------------------------------ interface elements
+_TET_FACES = np.array([[0, 1, 2], [0, 1, 3], [0, 2, 3], [1, 2, 3]])
+_TET_OPP = np.array([3, 2, 1, 0])
+_HEX_FACES = np.array([[0, 1, 2, 3], [4, 5, 6, 7], [0, 1, 5, 4], [1, 2, 6, 5], [2, 3, 7, 6], [3, 0, 4, 7]])
+
+
+def split_grains(mesh: MeshArrays, grain: np.ndarray) -> MeshArrays:
+    """Insert zero-thickness interface elements on every face shared by two elements of
+    different grains: each node gets one copy per grain that uses it, and each such face
+    becomes an interface element (- side: the copies of one grain in an order whose normal
+    points into the other grain; + side: the other grain's copies in the same order).  Linear
+    tets give 3 + 3 node interfaces, hexes 4 + 4; a mesh of one element type is required."""
+    nne = np.diff(mesh.elem_ptr)
+    if not np.all(nne == nne[0]) or nne[0] not in (4, 8):
+        raise PGFError("split_grains needs a mesh of 4-node tets or 8-node hexes only")
+    k = int(nne[0])
+    conn = mesh.elem_conn.reshape(-1, k)
+    grain = np.asarray(grain, dtype=np.int64)
+    pairs = np.stack([conn.ravel(), np.repeat(grain, k)], axis=1)
+    uniq, inv = np.unique(pairs, axis=0, return_inverse=True)
+    new_X = np.ascontiguousarray(mesh.X[uniq[:, 0]])
+    new_conn = inv.reshape(-1, k).astype(np.int64)
+    copy = {(int(a), int(b)): i for i, (a, b) in enumerate(uniq)}
+    faces = _TET_FACES if k == 4 else _HEX_FACES
+    fn = conn[:, faces]                                   # [ne, nf, nv] original node ids
+    key = np.sort(fn, axis=2).reshape(-1, faces.shape[1])
+    order = np.lexsort(key.T[::-1])
+    ks = key[order]
+    same = np.all(ks[1:] == ks[:-1], axis=1)
+    idx = np.where(same)[0]
+    nf = faces.shape[0]
+    coel = []
+    for i in idx:
+        fa, fb = order[i], order[i + 1]
+        ea, eb = fa // nf, fb // nf
+        if grain[ea] == grain[eb]:
+            continue
+        nodes = fn[ea, fa % nf].copy()
+        x = mesh.X[nodes]
+        nrm = np.cross(x[1] - x[0], x[2] - x[0])
+        inner = mesh.X[conn[ea]].mean(axis=0)          # centroid of the - side element
+        if np.dot(nrm, x.mean(axis=0) - inner) < 0:     # make the normal point out of ea
+            nodes = nodes[::-1].copy()
+        minus = [copy[(int(v), int(grain[ea]))] for v in nodes]
+        plus = [copy[(int(v), int(grain[eb]))] for v in nodes]
+        coel.append(minus + plus)
+    coel = np.asarray(coel, dtype=np.int64).reshape(-1, 2 * faces.shape[1])
+    cptr = np.arange(coel.shape[0] + 1, dtype=np.int64) * coel.shape[1]
+    return MeshArrays(new_X, mesh.elem_ptr.copy(), np.ascontiguousarray(new_conn.ravel()),
+                      None if mesh.elem_mat is None else mesh.elem_mat.copy(), cptr, np.ascontiguousarray(coel.ravel()))

@@ -44,3 +44,25 @@ class CPBoxProblem(BoxProblem):
     def assemble(self, st, r, defl, **kw):
         return O.assemble_cp(self.mesh, self.mats, self.grain, self.gP, self.gL, self.ids, self.ndof, self.Ap,
                              self.Ai, st, r, defl=defl, **kw)
+
+
+class InterfaceBoxProblem(BoxProblem):
+    """Box mesh split along Voronoi grain boundaries with zero-thickness interface elements
+    (exponential traction-separation law) between the grains."""
+
+    def __init__(self, n, etype, material, coh, n_grains=4, jitter=0.1, seed=1):
+        self.mesh0 = P.mesh_box(n, etype, jitter, seed)
+        self.grain, q = P.voronoi_grains(self.mesh0, n_grains, seed)
+        self.R = P.quat_to_rot(q)
+        self.mesh = P.split_grains(self.mesh0, self.grain)
+        self.supports = P.box_supports(self.mesh)
+        self.ids, self.ndof = P.generate_dof_ids(self.mesh, self.supports)
+        self.Ap, self.Ai = P.psparse_apai(self.mesh, self.ids, self.ndof)
+        self.mats = [material]
+        self.coh = [coh]
+        self.nqp = self.mesh.n_elem * O.nint(etype)
+        self.ncqp = int(np.sum(np.diff(self.mesh.coel_ptr) // 2))
+
+    def assemble_coh(self, st, r, defl, f, vals):
+        return O.assemble_coh(self.mesh, self.coh, None, self.ids, self.ndof, self.Ap, self.Ai, st, r, f, vals,
+                              defl=defl)

@@ -22,6 +22,34 @@ def test_product_equals_oracle(n, etype):
     assert np.array_equal(Ap, Ap2) and np.array_equal(Ai, Ai2)
 
 
+@pytest.mark.parametrize("n,etype,ng", [(5, P.PGF_TET4, 6), (3, P.PGF_HEX8, 4)])
+def test_product_equals_oracle_with_interface_elements(n, etype, ng):
+    """Interface (COEL) elements enter the DOF numbering after the volume elements and couple
+    their two sides in the pattern (countDof/mapDof over COEL, src/Psparse_ApAi.cc:160-213)."""
+    m0 = P.mesh_box(n, etype, 0.1, 2)
+    g, _ = P.voronoi_grains(m0, ng, 4)
+    m = P.split_grains(m0, g)
+    assert m.n_coel > 0
+    sup = P.box_supports(m)
+    ids, nd = P.generate_dof_ids(m, sup)
+    ids2, nd2 = O.generate_dof_ids(m.X, m.elem_ptr, m.elem_conn, sup, m.coel_ptr, m.coel_conn)
+    assert nd == nd2 and np.array_equal(ids, ids2)
+    Ap, Ai = P.psparse_apai(m, ids, nd)
+    Ap2, Ai2 = O.psparse_apai(m.X, m.elem_ptr, m.elem_conn, ids2, nd2, m.coel_ptr, m.coel_conn)
+    assert np.array_equal(Ap, Ap2) and np.array_equal(Ai, Ai2)
+    # the two sides of every interface are coupled
+    k = m.coel_conn.reshape(m.n_coel, -1)
+    half = k.shape[1] // 2
+    ids = ids.reshape(-1)
+    for c in range(0, m.n_coel, max(1, m.n_coel // 20)):
+        a, b = k[c, 0], k[c, half]
+        ra = [ids[a * 3 + j] - 1 for j in range(3) if ids[a * 3 + j] > 0]
+        cb = [ids[b * 3 + j] - 1 for j in range(3) if ids[b * 3 + j] > 0]
+        for r in ra:
+            row = Ai[Ap[r]:Ap[r + 1]]
+            assert np.isin(np.asarray(cb), row).all()
+
+
 def _dense_pattern(Ap, Ai, nd):
     A = np.zeros((nd, nd), dtype=bool)
     for r in range(nd):
