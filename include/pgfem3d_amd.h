@@ -254,6 +254,38 @@ int pgf_state_ncomp(pgf_handle h, int32_t field, int32_t *ncomp);
 int pgf_get_state(pgf_handle h, int32_t field, double *host_out);
 int pgf_set_state(pgf_handle h, int32_t field, const double *host_in);
 
+/* ================================================== learned warm start (no reference
+ * counterpart: the reference has no learned component).  A small MLP (hidden widths <= 128)
+ * on the matrix cores, BF16 weights and activations with FP32 accumulation, trained by the
+ * library's own forward / backward / Adam kernels; it predicts the 12 slip increments of a
+ * crystal-plasticity QP, from which the local Newton starts (falling back to the plain
+ * start if that Newton fails).  The FP64 Newton still converges to tol_M. */
+typedef struct pgf_mlp_s *pgf_mlp;
+/* widths[0] inputs, widths[n_layers] outputs; GELU after every layer but the last. */
+int pgf_mlp_create(int32_t n_layers, const int32_t *widths, uint64_t seed, pgf_mlp *out);
+int pgf_mlp_destroy(pgf_mlp m);
+/* Parameters (FP32 master copy): for each layer W [Np][Kp] row-major then b [Np], with Kp, Np
+ * the widths rounded up to multiples of 16 (padding is zero). */
+int pgf_mlp_param_count(pgf_mlp m, int64_t *n);
+int pgf_mlp_get_params(pgf_mlp m, float *host);
+int pgf_mlp_set_params(pgf_mlp m, const float *host);
+int pgf_mlp_get_grads(pgf_mlp m, float *host);
+/* Device pointers: X [B][widths[0]], Y [B][widths[n_layers]] (FP32). */
+int pgf_mlp_forward(pgf_mlp m, const float *X, int64_t B, float *Y);
+/* Mean-squared-error forward + backward over a batch (B a multiple of 16); gradients are
+ * added to the gradient buffer, which pgf_mlp_adam_step consumes and clears. */
+int pgf_mlp_backward(pgf_mlp m, const float *X, const float *T, int64_t B, float *loss);
+int pgf_mlp_adam_step(pgf_mlp m, float lr, float beta1, float beta2, float eps);
+
+/* Crystal-plasticity features and targets of the last integration, device QP order, FP32:
+ * X [n_qp][34] = (F(n) pF(n)^-1 (9), F(n+1) - F(n) (9), 12 x g(n)/g0, grain orientation
+ * quaternion (w, x, y, z) with w >= 0), T [n_qp][12] = dt * gamma_dot(n+1). */
+int pgf_cp_log_samples(pgf_handle h, float *X, float *T);
+/* Attach a trained MLP (NULL detaches): integrating passes then build the features, run the
+ * MLP and start the local Newton from its prediction.  Normalisation (host arrays): the MLP
+ * sees (x - shift) * scale and predicts targets times out_scale. */
+int pgf_set_surrogate(pgf_handle h, pgf_mlp m, const float *shift, const float *scale, const float *out_scale);
+
 #ifdef __cplusplus
 }
 #endif

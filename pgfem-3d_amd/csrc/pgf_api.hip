@@ -57,6 +57,39 @@ int upload(T **p, const T *h, size_t n) {
 
 }  // namespace
 
+// Unit quaternion (w, x, y, z), w >= 0, of a rotation matrix (Shepperd's method: the
+// largest of the four squared components is taken from the trace or a diagonal entry).
+void rot_to_quat(const double *R, float *q) {
+  const double t = R[0] + R[4] + R[8];
+  double w, x, y, z;
+  if (t >= R[0] && t >= R[4] && t >= R[8]) {
+    w = 0.5 * std::sqrt(1.0 + t);
+    x = (R[7] - R[5]) / (4 * w);
+    y = (R[2] - R[6]) / (4 * w);
+    z = (R[3] - R[1]) / (4 * w);
+  } else if (R[0] >= R[4] && R[0] >= R[8]) {
+    x = 0.5 * std::sqrt(1.0 + 2 * R[0] - t);
+    w = (R[7] - R[5]) / (4 * x);
+    y = (R[1] + R[3]) / (4 * x);
+    z = (R[2] + R[6]) / (4 * x);
+  } else if (R[4] >= R[8]) {
+    y = 0.5 * std::sqrt(1.0 + 2 * R[4] - t);
+    w = (R[2] - R[6]) / (4 * y);
+    x = (R[1] + R[3]) / (4 * y);
+    z = (R[5] + R[7]) / (4 * y);
+  } else {
+    z = 0.5 * std::sqrt(1.0 + 2 * R[8] - t);
+    w = (R[3] - R[1]) / (4 * z);
+    x = (R[2] + R[6]) / (4 * z);
+    y = (R[5] + R[7]) / (4 * z);
+  }
+  const double sg = (w < 0) ? -1.0 : 1.0;
+  q[0] = float(sg * w);
+  q[1] = float(sg * x);
+  q[2] = float(sg * y);
+  q[3] = float(sg * z);
+}
+
 // Host tables of one grain: the 12 FCC {111}<110> systems (unit slip direction s and plane
 // normal m, Schmid tensor s (x) m) and the cubic stiffness, both rotated by R (crystal ->
 // sample frame).
@@ -121,6 +154,12 @@ struct pgf_handle_s {
   double *d_qpbuf = nullptr;
   const double *warm = nullptr;
   double cp_g0 = 0.0, dt = 1.0;
+  float *d_gquat = nullptr;  // [n_grains][4] orientation quaternions (learned warm start features)
+  // learned warm start
+  pgf_mlp sur = nullptr;
+  float *d_norm = nullptr;     // shift [34], scale [34], out_scale [12]
+  float *d_feat = nullptr;     // [n_qp][34]
+  double *d_sur_warm = nullptr;  // [11][n_qp]
   std::vector<double> cp_mat_host;  // C11, C12, C44
   // interface elements
   std::vector<pgf::CohBlock> cblocks;
@@ -163,7 +202,7 @@ int set_identity_state(pgf_handle h) {
 
 // Crystal-plasticity pre-pass of one block: F(n+1), the return map, and the per-QP data of
 // the element kernel.
-void cp_prepass(pgf_handle h, const Block &b, const pgf_nodal_input *in, int mode) {
+pgf::CPArgs cp_args(pgf_handle h, const Block &b, const pgf_nodal_input *in) {
   pgf::CPArgs c{};
   c.X = h->d_X;
   c.dofid = h->d_dofid;
@@ -175,10 +214,12 @@ void cp_prepass(pgf_handle h, const Block &b, const pgf_nodal_input *in, int mod
   c.n_qp = h->n_qp;
   c.ne = b.ne;
   c.nq = b.ne * b.nqp;
-  c.r = in->r;
-  c.d_r = in->d_r;
-  c.defl = in->defl;
-  c.defl_d = in->defl_d;
+  if (in) {
+    c.r = in->r;
+    c.d_r = in->d_r;
+    c.defl = in->defl;
+    c.defl_d = in->defl_d;
+  }
   c.dt = h->dt;
   const int64_t NQ = h->n_qp;
   c.F_np1 = h->d_F[0];
@@ -194,8 +235,22 @@ void cp_prepass(pgf_handle h, const Block &b, const pgf_nodal_input *in, int mod
   c.warm = h->warm;
   c.qpbuf = h->d_qpbuf + b.qp_base * pgf::kCPBuf;
   c.err = h->d_err;
+  return c;
+}
+
+void cp_prepass(pgf_handle h, const Block &b, const pgf_nodal_input *in, int mode) {
+  pgf::CPArgs c = cp_args(h, b, in);
   if (mode & pgf::MODE_INTEGRATE) {
     pgf::launch_cp_kinematics(b.nne, c, h->stream);
+    if (h->sur && !h->warm) {  // learned start: features -> MLP -> (M, lambda, g)
+      pgf::launch_cp_features(c, h->d_gquat, h->d_norm, h->d_norm + 34, h->d_feat, h->stream);
+      const float *Y = nullptr;
+      int ldy = 0;
+      if (pgf::mlp_forward_device(h->sur, h->d_feat + b.qp_base * 34, c.nq, h->stream, &Y, &ldy) == PGF_OK) {
+        pgf::launch_cp_warm_from_slip(c, Y, ldy, h->d_norm + 68, h->d_sur_warm, h->stream);
+        c.warm = h->d_sur_warm;
+      }
+    }
     pgf::launch_cp_newton(b.nne, c, h->stream);
   }
   pgf::launch_cp_tangent(b.nne, c, (mode & pgf::MODE_TANGENT) != 0, h->stream);
@@ -670,6 +725,10 @@ int pgf_destroy(pgf_handle h) {
   hipFree(h->d_cprops);
   hipFree(h->d_cpmat);
   hipFree(h->d_grains);
+  hipFree(h->d_gquat);
+  hipFree(h->d_norm);
+  hipFree(h->d_feat);
+  hipFree(h->d_sur_warm);
   for (int t = 0; t < 2; ++t) {
     hipFree(h->d_tau[t]);
     hipFree(h->d_gdot[t]);
@@ -889,11 +948,17 @@ int pgf_set_orientations(pgf_handle h, int32_t n_grains, const double *R, const 
   if (!h || n_grains < 1 || !R) return PGF_EINVAL;
   if (!h->has_cp) return PGF_ENOTSUP;
   std::vector<pgf::DevGrain> g(n_grains);
-  for (int32_t k = 0; k < n_grains; ++k)
+  std::vector<float> q(size_t(n_grains) * 4);
+  for (int32_t k = 0; k < n_grains; ++k) {
     pgf::make_grain(R + 9 * k, h->cp_mat_host[0], h->cp_mat_host[1], h->cp_mat_host[2], g[k]);
+    pgf::rot_to_quat(R + 9 * k, q.data() + 4 * k);
+  }
   hipFree(h->d_grains);
+  hipFree(h->d_gquat);
+  h->d_gquat = nullptr;
   int rc = pgf::upload(&h->d_grains, g.data(), g.size());
   if (rc) return rc;
+  if ((rc = pgf::upload(&h->d_gquat, q.data(), q.size()))) return rc;
   if (!elem_grain) return PGF_OK;
   for (auto &b : h->blocks) {
     if (b.model != PGF_MODEL_CRYSTAL_PLASTICITY) continue;
@@ -938,6 +1003,42 @@ int pgf_get_cohesive_sizes(pgf_handle h, int64_t *n_cqp, int32_t *n_colors) {
   if (!h) return PGF_EINVAL;
   if (n_cqp) *n_cqp = h->n_cqp;
   if (n_colors) *n_colors = h->n_ccolors;
+  return PGF_OK;
+}
+
+int pgf_cp_log_samples(pgf_handle h, float *X, float *T) {
+  if (!h || !X || !T) return PGF_EINVAL;
+  if (!h->has_cp) return PGF_ENOTSUP;
+  for (auto &b : h->blocks) {
+    if (b.model != PGF_MODEL_CRYSTAL_PLASTICITY) continue;
+    pgf::CPArgs c = cp_args(h, b, nullptr);
+    pgf::launch_cp_features(c, h->d_gquat, nullptr, nullptr, X, h->stream);
+    pgf::launch_cp_targets(c, T, h->stream);
+  }
+  if (hipGetLastError() != hipSuccess) return PGF_EHIP;
+  return hipStreamSynchronize(h->stream) == hipSuccess ? PGF_OK : PGF_EHIP;
+}
+
+int pgf_set_surrogate(pgf_handle h, pgf_mlp m, const float *shift, const float *scale, const float *out_scale) {
+  if (!h) return PGF_EINVAL;
+  if (!m) {
+    h->sur = nullptr;
+    return PGF_OK;
+  }
+  if (!h->has_cp) return PGF_ENOTSUP;
+  if (!shift || !scale || !out_scale) return PGF_EINVAL;
+  std::vector<float> nrm(80);
+  std::copy(shift, shift + 34, nrm.begin());
+  std::copy(scale, scale + 34, nrm.begin() + 34);
+  std::copy(out_scale, out_scale + 12, nrm.begin() + 68);
+  for (int i = 68; i < 80; ++i)
+    if (!(nrm[i] != 0.0f)) return PGF_EINVAL;
+  int rc;
+  if (!h->d_norm && (rc = pgf::dalloc(&h->d_norm, 80))) return rc;
+  if (!h->d_feat && (rc = pgf::dalloc(&h->d_feat, size_t(h->n_qp) * 34))) return rc;
+  if (!h->d_sur_warm && (rc = pgf::dalloc(&h->d_sur_warm, size_t(h->n_qp) * 11))) return rc;
+  if (hipMemcpy(h->d_norm, nrm.data(), 80 * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) return PGF_EHIP;
+  h->sur = m;
   return PGF_OK;
 }
 

@@ -452,8 +452,8 @@ __global__ __launch_bounds__(64) void cp_newton(CPArgs a, int64_t chunk) {
   bool need = true, done = false;
   int64_t q = -1;
   double x[11], A[9], det_pFn = 1.0, g_n = 1.0, gscale = 1.0, dt = a.dt;
-  int it = 0, gid = 0;
-  bool bad_start = false;
+  int it = 0, it_att = 0, gid = 0;
+  bool bad_start = false, from_warm = false;
   CPMaterial cm{};
   for (;;) {
     // refill the lanes that need a QP, in lane order, from the wave's range
@@ -482,16 +482,30 @@ __global__ __launch_bounds__(64) void cp_newton(CPArgs a, int64_t chunk) {
         cm = a.cpmats[0];
         gscale = 1.0 / (fabs(cm.gs0 - cm.g0) * g_n + 1e-300);
         it = 0;
+        it_att = 0;
         // start: the caller's warm start, else the plain start
         bad_start = false;
-        if (a.warm) {
-#pragma unroll
-          for (int i = 0; i < 11; ++i) x[i] = a.warm[i * NQ + qg];
-        } else {
+        from_warm = false;
+        {
           double Fn[9];
 #pragma unroll
           for (int i = 0; i < 9; ++i) Fn[i] = a.F_n[i * NQ + qg];
           bad_start = !cp_start(a.grains[gid], A, F, Fn, pFn, pFi, g_n, a.lam_n[qg], x);
+        }
+        if (a.warm) {
+          // a caller's start is taken when its residual merit is below the plain start's
+          double xw[11];
+#pragma unroll
+          for (int i = 0; i < 11; ++i) xw[i] = a.warm[i * NQ + qg];
+          CPPoint pw, pp;
+          const bool okw = cp_point(cm, a.grains[gid], A, det_pFn, g_n, dt, xw, pw);
+          const bool okp = !bad_start && cp_point(cm, a.grains[gid], A, det_pFn, g_n, dt, x, pp);
+          if (okw && (!okp || cp_merit(pw, gscale) < cp_merit(pp, gscale))) {
+#pragma unroll
+            for (int i = 0; i < 11; ++i) x[i] = xw[i];
+            from_warm = true;
+            bad_start = false;
+          }
         }
       }
     }
@@ -503,6 +517,7 @@ __global__ __launch_bounds__(64) void cp_newton(CPArgs a, int64_t chunk) {
     const DevGrain &gr = a.grains[gid];
     const int64_t qg = a.qp_base + q;
     ++it;
+    ++it_att;
     bool fail = false, conv = false;
     CPPoint p;
     if (bad_start || !cp_point(cm, gr, A, det_pFn, g_n, dt, x, p)) {
@@ -562,7 +577,23 @@ __global__ __launch_bounds__(64) void cp_newton(CPArgs a, int64_t chunk) {
         }
       }
     }
-    if (!conv && !fail && it >= cm.maxit) fail = true;
+    if (!conv && !fail && it_att >= cm.maxit) fail = true;
+    if (fail && from_warm) {
+      // the warm start did not converge: fall back to the plain start with a fresh budget
+      // (iterations keep counting)
+      from_warm = false;
+      it_att = 0;
+      double F[9], pFn[9], pFi[9], Fn[9];
+#pragma unroll
+      for (int i = 0; i < 9; ++i) {
+        F[i] = a.F_np1[i * NQ + qg];
+        pFn[i] = a.pF_n[i * NQ + qg];
+        Fn[i] = a.F_n[i * NQ + qg];
+      }
+      inv3(pFn, pFi);
+      bad_start = !cp_start(gr, A, F, Fn, pFn, pFi, g_n, a.lam_n[qg], x);
+      fail = false;
+    }
     if (conv || fail) {
       a.iters[qg] = it;
       if (fail) {
@@ -964,7 +995,95 @@ void launch_ce_mode(int mode, const CPElemArgs &a, int64_t e0, int64_t e1, hipSt
   }
 }
 
+// ------------------------------------------------------------------- learned warm start
+// Features of one QP (device QP order rows of 34): F(n) pF(n)^-1, F(n+1) - F(n), 12 x g(n)/g0,
+// grain quaternion; with shift/scale the row is normalised as (x - shift) * scale.
+__global__ void cp_features(CPArgs a, const float *gquat, const float *shift, const float *scale, float *X) {
+  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= a.nq) return;
+  const int64_t NQ = a.n_qp, qg = a.qp_base + t;
+  double Fn[9], pFn[9], pFi[9], Fe[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    Fn[i] = a.F_n[i * NQ + qg];
+    pFn[i] = a.pF_n[i * NQ + qg];
+  }
+  if (!inv3(pFn, pFi)) {
+#pragma unroll
+    for (int i = 0; i < 9; ++i) pFi[i] = (i % 4 == 0) ? 1.0 : 0.0;
+  }
+  mm3(Fn, pFi, Fe);
+  float f[34];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    f[i] = float(Fe[i]);
+    f[9 + i] = float(a.F_np1[i * NQ + qg] - Fn[i]);
+  }
+  const float gr = float(a.g_n[qg] / a.cpmats[0].g0);
+#pragma unroll
+  for (int i = 0; i < 12; ++i) f[18 + i] = gr;
+  const int gid = a.qgrain[t];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) f[30 + i] = gquat[gid * 4 + i];
+  float *row = X + qg * 34;
+#pragma unroll
+  for (int i = 0; i < 34; ++i) row[i] = shift ? (f[i] - shift[i]) * scale[i] : f[i];
+}
+
+// Targets: dt * gamma_dot(n+1), rows of 12.
+__global__ void cp_targets(CPArgs a, float *T) {
+  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= a.nq) return;
+  const int64_t NQ = a.n_qp, qg = a.qp_base + t;
+#pragma unroll
+  for (int i = 0; i < 12; ++i) T[qg * 12 + i] = float(a.dt * a.gdot[i * NQ + qg]);
+}
+
+// Warm start from predicted slip increments (block-local rows of Y, ldy apart):
+// M = I - sum_a dgamma_a P_a (first order in the increments), lambda = lambda(n) and g from
+// the closed-form hardening update at the predicted total slip rate.
+__global__ void cp_warm_from_slip(CPArgs a, const float *Y, int ldy, const float *out_scale, double *warm) {
+  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= a.nq) return;
+  const int64_t NQ = a.n_qp, qg = a.qp_base + t;
+  const DevGrain &gr = a.grains[a.qgrain[t]];
+  const CPMaterial cm = a.cpmats[0];
+  double dg[12], gsum = 0.0;
+#pragma unroll
+  for (int i = 0; i < 12; ++i) {
+    dg[i] = double(Y[t * ldy + i]) / double(out_scale[i]);
+    gsum += fabs(dg[i]);
+  }
+  gsum /= a.dt;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    double s = 0.0;
+#pragma unroll
+    for (int al = 0; al < 12; ++al) s += dg[al] * gr.P[al][i];
+    warm[i * NQ + qg] = ((i % 4 == 0) ? 1.0 : 0.0) - s;
+  }
+  double gs = cm.gs0;
+  if (cm.w != 0.0) gs = (gsum / cm.gds > 1.0e-15) ? cm.gs0 * pow(gsum / cm.gds, cm.w) : 0.0;
+  const double g_n = a.g_n[qg];
+  const double num = (gs - cm.g0) * g_n + a.dt * cm.h0 * gs * gsum;
+  const double den = gs - cm.g0 + a.dt * cm.h0 * gsum;
+  warm[9 * NQ + qg] = a.lam_n[qg];
+  warm[10 * NQ + qg] = (den != 0.0 && num / den > 0.0) ? num / den : g_n;
+}
+
 }  // namespace
+
+void launch_cp_features(const CPArgs &a, const float *gquat, const float *shift, const float *scale, float *X,
+                        hipStream_t s) {
+  if (a.nq > 0) cp_features<<<unsigned((a.nq + 255) / 256), 256, 0, s>>>(a, gquat, shift, scale, X);
+}
+void launch_cp_targets(const CPArgs &a, float *T, hipStream_t s) {
+  if (a.nq > 0) cp_targets<<<unsigned((a.nq + 255) / 256), 256, 0, s>>>(a, T);
+}
+void launch_cp_warm_from_slip(const CPArgs &a, const float *Y, int ldy, const float *out_scale, double *warm,
+                              hipStream_t s) {
+  if (a.nq > 0) cp_warm_from_slip<<<unsigned((a.nq + 255) / 256), 256, 0, s>>>(a, Y, ldy, out_scale, warm);
+}
 
 void launch_cp_kinematics(int nne, const CPArgs &a, hipStream_t s) {
   if (nne == 4) launch_kin<4>(a, s);

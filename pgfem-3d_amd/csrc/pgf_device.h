@@ -112,6 +112,15 @@ struct CPElemArgs {
 };
 void launch_cp_elem(int nne, int mode, const CPElemArgs &a, int64_t e_begin, int64_t e_end, hipStream_t s);
 
+// Learned warm start: features / targets of a block's QPs (rows indexed by device QP) and the
+// Newton start from predicted slip increments (block-local rows of Y).
+void launch_cp_features(const CPArgs &a, const float *gquat, const float *shift, const float *scale, float *X,
+                        hipStream_t s);
+void launch_cp_targets(const CPArgs &a, float *T, hipStream_t s);
+void launch_cp_warm_from_slip(const CPArgs &a, const float *Y, int ldy, const float *out_scale, double *warm,
+                              hipStream_t s);
+int mlp_forward_device(pgf_mlp m, const float *X, int64_t B, hipStream_t s, const float **Y, int *ldy);
+
 // ---------------------------------------------------------------- interface elements
 // Exponential traction-separation law parameters (Needleman_pkey order).
 struct DevCohesive {

@@ -106,6 +106,17 @@ def lib() -> C.CDLL:
             "pgf_state_device_ptr": [P, I32, P],
             "pgf_set_cohesive_props": [P, I32, P, P],
             "pgf_get_cohesive_sizes": [P, P, P],
+            "pgf_mlp_create": [I32, P, C.c_uint64, P],
+            "pgf_mlp_destroy": [P],
+            "pgf_mlp_param_count": [P, P],
+            "pgf_mlp_get_params": [P, P],
+            "pgf_mlp_set_params": [P, P],
+            "pgf_mlp_get_grads": [P, P],
+            "pgf_mlp_forward": [P, P, I64, P],
+            "pgf_mlp_backward": [P, P, P, I64, P],
+            "pgf_mlp_adam_step": [P, C.c_float, C.c_float, C.c_float, C.c_float],
+            "pgf_cp_log_samples": [P, P, P],
+            "pgf_set_surrogate": [P, P, P, P, P],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -123,7 +134,9 @@ def exported_symbols():
         "pgf_residual_tangent", "pgf_commit_step", "pgf_reset_step", "pgf_state_ncomp",
         "pgf_get_state", "pgf_set_state", "pgf_set_orientations", "pgf_set_time_step",
         "pgf_set_warm_start", "pgf_get_qp_order", "pgf_state_device_ptr", "pgf_set_cohesive_props",
-        "pgf_get_cohesive_sizes",
+        "pgf_get_cohesive_sizes", "pgf_mlp_create", "pgf_mlp_destroy", "pgf_mlp_param_count",
+        "pgf_mlp_get_params", "pgf_mlp_set_params", "pgf_mlp_get_grads", "pgf_mlp_forward", "pgf_mlp_backward",
+        "pgf_mlp_adam_step", "pgf_cp_log_samples", "pgf_set_surrogate",
     ]
 
 
@@ -348,6 +361,33 @@ class Handle:
         _check(self._lib.pgf_set_cohesive_props(self.h, len(props), arr, _ptr(cm)), "pgf_set_cohesive_props")
         self._coh_keep = (arr, cm)
 
+    def log_samples(self, X, T):
+        """Features [n_qp, 34] and targets [n_qp, 12] of the last integration into float32 GPU
+        tensors (device QP order)."""
+        self._check_f32(X, self.n_qp * 34, "X")
+        self._check_f32(T, self.n_qp * 12, "T")
+        _check(self._lib.pgf_cp_log_samples(self.h, _ptr(X), _ptr(T)), "pgf_cp_log_samples")
+
+    def set_surrogate(self, mlp, shift=None, scale=None, out_scale=None):
+        """Attach a trained MLP as the local-Newton warm start (None detaches)."""
+        if mlp is None:
+            _check(self._lib.pgf_set_surrogate(self.h, None, None, None, None), "pgf_set_surrogate")
+            self._sur = None
+            return
+        nrm = [np.ascontiguousarray(v, dtype=np.float32) for v in (shift, scale, out_scale)]
+        _check(self._lib.pgf_set_surrogate(self.h, mlp.m, _ptr(nrm[0]), _ptr(nrm[1]), _ptr(nrm[2])),
+               "pgf_set_surrogate")
+        self._sur = (mlp, nrm)
+
+    @staticmethod
+    def _check_f32(t, n, what):
+        import torch
+
+        if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.float32 and t.is_contiguous()):
+            raise PGFError(f"{what} must be a contiguous float32 GPU tensor")
+        if t.numel() != n:
+            raise PGFError(f"{what} has {t.numel()} entries, expected {n}")
+
     def state_device_ptr(self, field: int) -> int:
         """Device address of a state field ([ncomp][n_qp], device QP order)."""
         p = C.c_void_p()
@@ -473,3 +513,60 @@ def split_grains(mesh: MeshArrays, grain: np.ndarray) -> MeshArrays:
     cptr = np.arange(coel.shape[0] + 1, dtype=np.int64) * coel.shape[1]
     return MeshArrays(new_X, mesh.elem_ptr.copy(), np.ascontiguousarray(new_conn.ravel()),
                       None if mesh.elem_mat is None else mesh.elem_mat.copy(), cptr, np.ascontiguousarray(coel.ravel()))
+
+
+# ------------------------------------------------------------------------------- MLP
+class MLP:
+    """The library's matrix-core MLP (pgf_mlp_*): BF16 weights/activations, FP32 accumulation,
+    GELU hidden layers, trained by its own backward and Adam kernels."""
+
+    def __init__(self, widths, seed=0):
+        self._lib = lib()
+        self.widths = list(widths)
+        w = (C.c_int32 * len(widths))(*widths)
+        m = C.c_void_p()
+        _check(self._lib.pgf_mlp_create(len(widths) - 1, w, seed, C.byref(m)), "pgf_mlp_create")
+        self.m = m
+        n = C.c_int64()
+        _check(self._lib.pgf_mlp_param_count(m, C.byref(n)), "pgf_mlp_param_count")
+        self.n_param = n.value
+
+    def close(self):
+        if self.m:
+            self._lib.pgf_mlp_destroy(self.m)
+            self.m = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_params(self):
+        p = np.zeros(self.n_param, dtype=np.float32)
+        _check(self._lib.pgf_mlp_get_params(self.m, _ptr(p)), "pgf_mlp_get_params")
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        if p.size != self.n_param:
+            raise PGFError("parameter vector has the wrong length")
+        _check(self._lib.pgf_mlp_set_params(self.m, _ptr(p)), "pgf_mlp_set_params")
+
+    def get_grads(self):
+        g = np.zeros(self.n_param, dtype=np.float32)
+        _check(self._lib.pgf_mlp_get_grads(self.m, _ptr(g)), "pgf_mlp_get_grads")
+        return g
+
+    def forward(self, X, Y):
+        """X [B, in], Y [B, out]: float32 GPU tensors."""
+        _check(self._lib.pgf_mlp_forward(self.m, _ptr(X), X.shape[0], _ptr(Y)), "pgf_mlp_forward")
+
+    def backward(self, X, T):
+        """Adds the gradient of the batch MSE to the gradient buffer; returns the loss."""
+        loss = C.c_float()
+        _check(self._lib.pgf_mlp_backward(self.m, _ptr(X), _ptr(T), X.shape[0], C.byref(loss)), "pgf_mlp_backward")
+        return loss.value
+
+    def adam_step(self, lr, beta1=0.9, beta2=0.999, eps=1e-8):
+        _check(self._lib.pgf_mlp_adam_step(self.m, lr, beta1, beta2, eps), "pgf_mlp_adam_step")

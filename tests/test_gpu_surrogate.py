@@ -1,0 +1,86 @@
+"""The learned warm start on the GPU: trained with the library's MLP kernels on QP states
+logged from plain crystal-plasticity passes, it must leave the converged state unchanged
+(1e-10 relative against the plain start) and must not cost iterations (a QP keeps the
+plain start when the learned start's residual is larger); an untrained network must give
+the same state through the fallback to the plain start.  The measured gain is printed."""
+import numpy as np
+import pytest
+import torch
+
+import pgfem3d_amd as P
+import surrogate as S
+from conftest import rel_inf
+from problems import CPBoxProblem
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-10
+MAT = P.fcc_crystal()
+STEP = 1e-3
+
+
+def _dev(x, device):
+    return torch.from_numpy(np.ascontiguousarray(x)).to(device)
+
+
+def _history(prob, device, n_steps, log=None):
+    h = P.Handle(prob.mesh, prob.ids, prob.ndof, prob.Ap, prob.Ai, prob.mats)
+    h.set_orientations(prob.R, prob.grain)
+    f = torch.empty(prob.ndof, dtype=torch.float64, device=device)
+    row = 0
+    for s in range(1, n_steps + 1):
+        r, defl = prob.displacement(STEP * s, 0.0, seed=s)
+        assert h.residual(f, _dev(r, device), defl=_dev(defl, device)) == 0
+        if log is not None:
+            row = S.log_step(h, log[0], log[1], row)
+        h.commit_step()
+    return h, f
+
+
+def _step_states(h, prob, device, s):
+    r, defl = prob.displacement(STEP * s, 0.0, seed=s)
+    f = torch.empty(prob.ndof, dtype=torch.float64, device=device)
+    vals = torch.empty(int(prob.Ap[-1]), dtype=torch.float64, device=device)
+    assert h.residual_tangent(f, vals, _dev(r, device), defl=_dev(defl, device)) == 0
+    out = {k: h.get_state(k) for k in (P.STATE_PLASTIC_NP1, P.STATE_SCALAR_NP1, P.STATE_TAU, P.STATE_ITERS)}
+    out["f"] = f.cpu().numpy()
+    out["vals"] = vals.cpu().numpy()
+    return out
+
+
+def test_trained_warm_start_same_state_fewer_iterations(device):
+    prob = CPBoxProblem(8, P.PGF_TET4, MAT, n_grains=8, seed=4)
+    n_log = 8
+    X = torch.empty((n_log * prob.nqp, S.N_FEAT), dtype=torch.float32, device=device)
+    T = torch.empty((n_log * prob.nqp, S.N_OUT), dtype=torch.float32, device=device)
+    h, _ = _history(prob, device, n_log, (X, T))
+    assert torch.isfinite(X).all() and torch.isfinite(T).all() and float(T.abs().max()) > 0
+    mlp, nrm, hist = S.train(X, T, epochs=25, batch=1024, lr=2e-3, seed=0)
+    assert hist[-1][2] < 0.2 * hist[0][2] or hist[-1][2] < 0.05  # validation loss fell
+    plain = _step_states(h, prob, device, n_log + 1)
+    h.reset_step()
+    S.attach(h, mlp, nrm)
+    warm = _step_states(h, prob, device, n_log + 1)
+    for k in (P.STATE_PLASTIC_NP1, P.STATE_SCALAR_NP1, P.STATE_TAU, "f", "vals"):
+        assert rel_inf(warm[k], plain[k]) < TOL, k
+    it_p, it_w = plain[P.STATE_ITERS].mean(), warm[P.STATE_ITERS].mean()
+    print(f"[surrogate] mean local iterations: plain {it_p:.3f}, learned start {it_w:.3f}; "
+          f"validation loss {hist[0][2]:.3e} -> {hist[-1][2]:.3e}")
+    assert it_w <= 1.05 * it_p, (it_w, it_p)
+    h.close()
+    mlp.close()
+
+
+def test_untrained_network_falls_back_to_plain_start(device):
+    prob = CPBoxProblem(4, P.PGF_TET4, MAT, n_grains=4, seed=6)
+    h, _ = _history(prob, device, 4)
+    plain = _step_states(h, prob, device, 5)
+    h.reset_step()
+    mlp = P.MLP(S.WIDTHS, seed=3)
+    # a wild output scale makes the predicted increments far too large
+    S.attach(h, mlp, S.Normalisation(np.zeros(34, np.float32), np.ones(34, np.float32), np.full(12, 1e-2, np.float32)))
+    warm = _step_states(h, prob, device, 5)
+    for k in (P.STATE_PLASTIC_NP1, P.STATE_SCALAR_NP1, P.STATE_TAU, "f", "vals"):
+        assert rel_inf(warm[k], plain[k]) < TOL, k
+    h.set_surrogate(None)
+    h.close()
+    mlp.close()
