@@ -11,6 +11,10 @@ Second workload (the "cp" object): FCC crystal plasticity on 55^3 cells x 6 line
 flow (gamma_dot0=1e-3/s, m=0.05), saturation hardening (g0=16, gs=148, h0=180 MPa), cubic
 elasticity (168.4/121.4/75.4 GPa), strain rate 1e-3/s, dt = 1 s, plain local-Newton start.
 The load history is driven to 0.5% (five committed steps), then the sixth step is timed.
+The learned warm start ("cp.surrogate"): the MLP (34 -> 4 x 128 GELU -> 12, BF16 on the
+matrix cores) is trained by the library's own kernels on the QP states logged over the five
+history steps (seeded 90/10 split), attached, and the sixth step is timed again; the
+converged state is compared with the plain start's.
 
 One step = one global Newton iteration's element work: deformation gradient, constitutive
 update (return map with its local Newton) and state update at every QP, element residual
@@ -188,10 +192,20 @@ def run_cp(args, dev):
           f"{h.n_colors}; set-up {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
     f = torch.empty(prob.ndof, dtype=torch.float64, device=dev)
     vals = torch.empty(h.nnz, dtype=torch.float64, device=dev)
-    for s in range(1, 6):  # load history to 0.5%
+    n_hist = 5
+    log = None
+    if not args.no_surrogate:
+        import surrogate as S
+
+        log = (torch.empty((n_hist * h.n_qp, S.N_FEAT), dtype=torch.float32, device=dev),
+               torch.empty((n_hist * h.n_qp, S.N_OUT), dtype=torch.float32, device=dev))
+    row = 0
+    for s in range(1, n_hist + 1):  # load history to 0.5%, logged for the learned start
         r, defl = prob.displacement(CP_STEP * s, 0.0, seed=s)
         err = h.residual(f, torch.from_numpy(r).to(dev), defl=torch.from_numpy(defl).to(dev), dt=1.0)
         assert err == 0, (s, err)
+        if log is not None:
+            row = S.log_step(h, log[0], log[1], row)
         h.commit_step()
     r, defl = prob.displacement(CP_STEP * 6, 0.0, seed=6)
     rd = torch.from_numpy(r).to(dev)
@@ -202,10 +216,36 @@ def run_cp(args, dev):
     mean_it = float(its.mean())
     tflops = cp_flops_per_qp(mean_it) * h.n_qp / (kern_ms * 1e-3) / 1e12
     res = dict(n_qp=h.n_qp, s_step=s_step, kern_ms=kern_ms, n_launch=n_launch, mean_iters=mean_it,
-               max_iters=int(its.max()), tflops=tflops)
+               max_iters=int(its.max()), tflops=tflops, surrogate=None)
     print(f"[bench] CP: {1e3 * s_step:.3f} ms/step wall, kernels {kern_ms:.3f} ms/pass over {n_launch} launches, "
           f"local Newton iterations mean {mean_it:.2f} max {int(its.max())}, {tflops:.2f} TFLOP/s",
           file=sys.stderr, flush=True)
+    if log is not None:
+        # learned warm start: trained on the logged plain history by the library's kernels,
+        # then the same step timed with the network in the loop
+        tau_plain = h.get_state(P.STATE_TAU)
+        pF_plain = h.get_state(P.STATE_PLASTIC_NP1)
+        t0 = time.perf_counter()
+        mlp, nrm, hist = S.train(log[0], log[1], epochs=args.sur_epochs, batch=8192, lr=2e-3, seed=0)
+        t_train = time.perf_counter() - t0
+        del log
+        S.attach(h, mlp, nrm)
+        s_sur, kern_sur, _ = _timed_passes(h, f, vals, rd, dd, args.cp_steps, 1)
+        its_s = h.get_state(P.STATE_ITERS)[0]
+        tau_s = h.get_state(P.STATE_TAU)
+        dtau = float(np.abs(tau_s - tau_plain).max() / np.abs(tau_plain).max())
+        dpf = float(np.abs(h.get_state(P.STATE_PLASTIC_NP1) - pF_plain).max())
+        res["surrogate"] = dict(
+            value=h.n_qp / s_sur, ms_per_step=1e3 * s_sur, kernel_ms_per_pass=kern_sur, speedup=s_step / s_sur,
+            mean_local_iters=float(its_s.mean()), iterations_saved=mean_it - float(its_s.mean()),
+            max_rel_tau_diff_vs_plain=dtau, max_abs_pF_diff_vs_plain=dpf, train_s=t_train,
+            train_samples=int(n_hist * h.n_qp), epochs=args.sur_epochs, val_loss=[float(x[2]) for x in hist])
+        print(f"[bench] CP learned start: {1e3 * s_sur:.3f} ms/step, speedup {s_step / s_sur:.3f}x, iterations "
+              f"mean {its_s.mean():.2f} (plain {mean_it:.2f}), max rel tau diff {dtau:.1e}, trained on "
+              f"{n_hist * h.n_qp} states in {t_train:.1f}s, val loss {hist[0][2]:.3e}->{hist[-1][2]:.3e}",
+              file=sys.stderr, flush=True)
+        h.set_surrogate(None)
+        mlp.close()
     h.close()
     return res
 
@@ -222,6 +262,8 @@ def main():
     ap.add_argument("--cpu-cp-n", type=int, default=34, help="cells per edge of the CP CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-cp", action="store_true")
+    ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--sur-epochs", type=int, default=3)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (multi-GPU sharding is a "
@@ -290,7 +332,8 @@ def main():
             "max_local_iters": cp["max_iters"],
             "roofline": {"bound": "fp64", "achieved": cp["tflops"], "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                          "frac": cp["tflops"] / FP64_PEAK_TFLOPS},
-            "surrogate_speedup": None,
+            "surrogate_speedup": None if cp["surrogate"] is None else cp["surrogate"]["speedup"],
+            "surrogate": cp["surrogate"],
             "cpu_baseline": cpu_cp,
         }
     print(json.dumps(out))

@@ -20,7 +20,7 @@ from dataclasses import dataclass
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "_build", "libpgfem3d_amd.so")
+LIB_PATH = os.environ.get("PGFEM3D_AMD_LIB") or os.path.join(_HERE, "_build", "libpgfem3d_amd.so")
 
 PGF_TET4, PGF_TET10, PGF_HEX8 = 4, 10, 8
 MODEL_HYPER_ELASTICITY, MODEL_CRYSTAL_PLASTICITY, MODEL_J2_PLASTICITY = 0, 1, 4
