@@ -254,6 +254,15 @@ int pgf_state_ncomp(pgf_handle h, int32_t field, int32_t *ncomp);
 int pgf_get_state(pgf_handle h, int32_t field, double *host_out);
 int pgf_set_state(pgf_handle h, int32_t field, const double *host_in);
 
+/* ================================================================ linear solve */
+/* Solve K x = b on the device for the tangent values vals[nnz] in the handle's pattern:
+ * Jacobi-preconditioned BiCGStab from the initial x (device vectors [ndof]) to
+ * ||b - K x|| <= tol ||b||.  Plays the role of the reference's per-iteration hypre solve
+ * (SparseSystem::solveSystem, src/solvers/hypre/Hypre.cc; called from Newton_Raphson,
+ * src/Newton_Raphson.cc:980-1012).  Returns 0 when converged, 1 when maxit was reached. */
+int pgf_solve(pgf_handle h, const double *vals, const double *b, double *x, double tol, int32_t maxit,
+              int32_t *iters, double *relres);
+
 /* ================================================== learned warm start (no reference
  * counterpart: the reference has no learned component).  A small MLP (hidden widths <= 128)
  * on the matrix cores, BF16 weights and activations with FP32 accumulation, trained by the

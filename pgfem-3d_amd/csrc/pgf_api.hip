@@ -161,6 +161,9 @@ struct pgf_handle_s {
   float *d_feat = nullptr;     // [n_qp][34]
   double *d_sur_warm = nullptr;  // [11][n_qp]
   std::vector<double> cp_mat_host;  // C11, C12, C44
+  // linear solve
+  int32_t *d_Ap = nullptr, *d_Ai = nullptr;
+  double *d_solve_work = nullptr;
   // interface elements
   std::vector<pgf::CohBlock> cblocks;
   int32_t n_ccolors = 0;
@@ -643,6 +646,8 @@ int pgf_create(const pgf_mesh *mesh, const int64_t *id, int64_t ndof, const int3
   if ((rc = pgf::upload(&h->d_dofid, dofid.data(), dofid.size()))) return fail(rc);
   if ((rc = pgf::upload(&h->d_row_start, row_start.data(), row_start.size()))) return fail(rc);
   if ((rc = pgf::upload(&h->d_rowlen, rowlen.data(), rowlen.size()))) return fail(rc);
+  if ((rc = pgf::upload(&h->d_Ap, Ap, size_t(ndof) + 1))) return fail(rc);
+  if ((rc = pgf::upload(&h->d_Ai, Ai, size_t(h->nnz)))) return fail(rc);
   for (size_t bi = 0; bi < h->blocks.size(); ++bi) {
     Block &b = h->blocks[bi];
     std::vector<int32_t> conn(b.ne * b.nne), mat(b.ne);
@@ -722,6 +727,9 @@ int pgf_destroy(pgf_handle h) {
     hipFree(cb.d_off);
   }
   for (int t = 0; t < 3; ++t) hipFree(h->d_cv[t]);
+  hipFree(h->d_Ap);
+  hipFree(h->d_Ai);
+  hipFree(h->d_solve_work);
   hipFree(h->d_cprops);
   hipFree(h->d_cpmat);
   hipFree(h->d_grains);
@@ -1040,6 +1048,24 @@ int pgf_set_surrogate(pgf_handle h, pgf_mlp m, const float *shift, const float *
   if (hipMemcpy(h->d_norm, nrm.data(), 80 * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) return PGF_EHIP;
   h->sur = m;
   return PGF_OK;
+}
+
+int pgf_solve(pgf_handle h, const double *vals, const double *b, double *x, double tol, int32_t maxit,
+              int32_t *iters, double *relres) {
+  if (!h || !vals || !b || !x || !(tol > 0.0) || maxit < 1) return PGF_EINVAL;
+  const int64_t n = h->ndof;
+  if (!h->d_solve_work) {
+    int rc = pgf::dalloc(&h->d_solve_work, size_t(9 * n + 1025));
+    if (rc) return rc;
+  }
+  int it = 0;
+  double rr = 0.0;
+  if (pgf::bicgstab(h->stream, h->d_Ap, h->d_Ai, vals, b, x, n, tol, maxit, &it, &rr, h->d_solve_work))
+    return PGF_EHIP;
+  if (hipStreamSynchronize(h->stream) != hipSuccess) return PGF_EHIP;
+  if (iters) *iters = it;
+  if (relres) *relres = rr;
+  return (rr <= tol) ? PGF_OK : 1;  // 1 = not converged within maxit
 }
 
 }  // extern "C"

@@ -145,6 +145,10 @@ void launch_coh_elem(int nn, int mode, const CohArgs &a, int64_t e_begin, int64_
 
 void launch_elem(int nne, int mode, const ElemArgs &a, int64_t e_begin, int64_t e_end,
                  hipStream_t s);
+// Jacobi-preconditioned BiCGStab on a CSR matrix; work holds 9 n + 1025 doubles.
+int bicgstab(hipStream_t s, const int32_t *Ap, const int32_t *Ai, const double *vals, const double *b, double *x,
+             int64_t n, double tol, int maxit, int *iters, double *relres, double *work);
+
 void launch_fill(double *p, int64_t n, double v, hipStream_t s);
 void launch_copy(double *dst, const double *src, int64_t n, hipStream_t s);
 

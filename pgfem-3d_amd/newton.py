@@ -1,0 +1,88 @@
+"""Global Newton driver over the device element path (the role of Newton_Raphson,
+src/Newton_Raphson.cc:886-1430, for the total Lagrangian CM analysis on one GPU).
+
+Per load step: the prescribed values of step n+1 are applied; every iteration runs the
+fused integration + residual + tangent pass (pgf_residual_tangent), forms the residual
+R = RR - f_int (RR = 0: the workloads are displacement driven), checks the normalised norm
+||R|| / NORM against the tolerance (NORM = the largest first-iteration norm seen, as
+Newton_Raphson keeps fv->NORM, :1207-1216), and solves K du = R for the free DOFs -- on the
+device with pgf_solve (Jacobi-preconditioned BiCGStab), or on the host with a sparse direct
+solve.  A step that fails (positive INFO from the element pass, a diverging or non-finite
+residual, or no convergence in max_iter) returns its status so the caller can subdivide and
+pgf_reset_step, as the reference does.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+import pgfem3d_amd as P
+
+
+@dataclass
+class StepResult:
+    converged: bool
+    iterations: int
+    norms: list = field(default_factory=list)  # normalised residual norm per iteration
+    info: int = 0                              # element-pass INFO of the failing iteration
+    solver_iters: list = field(default_factory=list)
+
+
+class NewtonDriver:
+    def __init__(self, h: P.Handle, Ap, Ai, tol=1e-10, max_iter=25, solver="device", lin_tol=1e-13,
+                 lin_maxit=20000, computer_zero=1e-15):
+        import torch
+
+        self.h, self.tol, self.max_iter = h, tol, max_iter
+        self.solver, self.lin_tol, self.lin_maxit = solver, lin_tol, lin_maxit
+        self.computer_zero = computer_zero
+        self.Ap, self.Ai = Ap, Ai
+        dev = torch.device("cuda:0")
+        self.f = torch.empty(h.ndof, dtype=torch.float64, device=dev)
+        self.vals = torch.empty(h.nnz, dtype=torch.float64, device=dev)
+        self.du = torch.zeros(h.ndof, dtype=torch.float64, device=dev)
+        self.NORM = 0.0
+        self.first = True
+
+    def _solve(self, R):
+        import torch
+
+        if self.solver == "device":
+            self.du.zero_()
+            it, rr = self.h.solve(self.vals, R, self.du, tol=self.lin_tol, maxit=self.lin_maxit)
+            return self.du, it
+        import scipy.sparse as sp
+        import scipy.sparse.linalg as spla
+
+        K = sp.csr_matrix((self.vals.cpu().numpy(), self.Ai, self.Ap), shape=(self.h.ndof, self.h.ndof))
+        x = spla.spsolve(K.tocsc(), R.cpu().numpy())
+        return torch.from_numpy(np.ascontiguousarray(x)).to(R.device), 0
+
+    def load_step(self, r, defl, d_r=None, defl_d=None, dt=1.0) -> StepResult:
+        """Solve one load step in place: r (free DOFs, device float64) enters as the converged
+        u(n) and leaves as u(n+1); defl holds the prescribed values of step n+1."""
+        res = StepResult(False, 0)
+        for it in range(self.max_iter):
+            info = self.h.residual_tangent(self.f, self.vals, r, d_r=d_r, defl=defl, defl_d=defl_d, dt=dt)
+            if info:
+                res.info = info
+                return res
+            R = -self.f
+            nor2 = float(R.norm())
+            if self.first or self.NORM < self.tol:  # normalisation of the residual norm
+                self.NORM = max(self.NORM, nor2, self.computer_zero)
+                self.first = False
+            nor = nor2 / self.NORM
+            res.norms.append(nor)
+            res.iterations = it
+            if not np.isfinite(nor):
+                return res
+            if nor <= self.tol or nor2 <= self.computer_zero:
+                res.converged = True
+                self.h.commit_step()
+                return res
+            du, lit = self._solve(R)
+            res.solver_iters.append(lit)
+            r += du
+        return res

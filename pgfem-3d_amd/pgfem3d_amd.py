@@ -116,6 +116,7 @@ def lib() -> C.CDLL:
             "pgf_mlp_backward": [P, P, P, I64, P],
             "pgf_mlp_adam_step": [P, C.c_float, C.c_float, C.c_float, C.c_float],
             "pgf_cp_log_samples": [P, P, P],
+            "pgf_solve": [P, P, P, P, D, I32, P, P],
             "pgf_set_surrogate": [P, P, P, P, P],
         }
         for name, args in sig.items():
@@ -136,7 +137,7 @@ def exported_symbols():
         "pgf_set_warm_start", "pgf_get_qp_order", "pgf_state_device_ptr", "pgf_set_cohesive_props",
         "pgf_get_cohesive_sizes", "pgf_mlp_create", "pgf_mlp_destroy", "pgf_mlp_param_count",
         "pgf_mlp_get_params", "pgf_mlp_set_params", "pgf_mlp_get_grads", "pgf_mlp_forward", "pgf_mlp_backward",
-        "pgf_mlp_adam_step", "pgf_cp_log_samples", "pgf_set_surrogate",
+        "pgf_mlp_adam_step", "pgf_cp_log_samples", "pgf_set_surrogate", "pgf_solve",
     ]
 
 
@@ -360,6 +361,14 @@ class Handle:
         cm = None if coel_mat is None else np.ascontiguousarray(coel_mat, dtype=np.int32)
         _check(self._lib.pgf_set_cohesive_props(self.h, len(props), arr, _ptr(cm)), "pgf_set_cohesive_props")
         self._coh_keep = (arr, cm)
+
+    def solve(self, vals, b, x, tol=1e-12, maxit=5000):
+        """Device BiCGStab: x <- K^-1 b (x holds the initial guess); returns (iters, relres)."""
+        it, rr = C.c_int32(), C.c_double()
+        rc = self._lib.pgf_solve(self.h, _ptr(vals), _ptr(b), _ptr(x), tol, maxit, C.byref(it), C.byref(rr))
+        if rc < 0:
+            _check(rc, "pgf_solve")
+        return it.value, rr.value
 
     def log_samples(self, X, T):
         """Features [n_qp, 34] and targets [n_qp, 12] of the last integration into float32 GPU

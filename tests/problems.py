@@ -66,3 +66,30 @@ class InterfaceBoxProblem(BoxProblem):
     def assemble_coh(self, st, r, defl, f, vals):
         return O.assemble_coh(self.mesh, self.coh, None, self.ids, self.ndof, self.Ap, self.Ai, st, r, f, vals,
                               defl=defl)
+
+
+def oracle_newton_step(prob, st, r, defl, state, tol=1e-12, max_iter=25, cp_tables=None, dt=1.0):
+    """The Newton load step of pgfem-3d_amd/newton.py restated over the CPU oracle with a
+    sparse direct solve: returns (r_np1, iterations).  state = dict(NORM=..., first=...)."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+
+    r = r.copy()
+    for it in range(max_iter):
+        if cp_tables is None:
+            f, vals, err = O.assemble(prob.mesh, prob.mats, prob.ids, prob.ndof, prob.Ap, prob.Ai, st, r, defl=defl)
+        else:
+            f, vals, err = O.assemble_cp(prob.mesh, prob.mats, prob.grain, cp_tables[0], cp_tables[1], prob.ids,
+                                         prob.ndof, prob.Ap, prob.Ai, st, r, defl=defl, dt=dt)
+        assert err == 0
+        R = -f
+        nor2 = float(np.linalg.norm(R))
+        if state["first"] or state["NORM"] < tol:
+            state["NORM"] = max(state["NORM"], nor2, 1e-15)
+            state["first"] = False
+        if nor2 / state["NORM"] <= tol or nor2 <= 1e-15:
+            st.commit()
+            return r, it
+        K = sp.csr_matrix((vals, prob.Ai, prob.Ap), shape=(prob.ndof, prob.ndof))
+        r = r + spla.spsolve(K.tocsc(), R)
+    raise AssertionError("oracle Newton did not converge")
