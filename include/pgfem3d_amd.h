@@ -219,6 +219,16 @@ int pgf_tangent(pgf_handle h, const pgf_nodal_input *in, double dt, double *vals
 int pgf_residual_tangent(pgf_handle h, const pgf_nodal_input *in, double dt, double *f,
                          double *vals);
 
+/* Capture each pass (all of its launches: every element type, the crystal-plasticity QP
+ * kernels, the interface elements) into one hipGraph and replay it while the pass mode, the
+ * vector pointers, dt and the tuning stay the same; on = 0 returns to eager launches.  (No
+ * reference counterpart: a launch-overhead measure for the per-iteration element pass.) */
+int pgf_set_graph(pgf_handle h, int on);
+
+/* Autotune the crystal-plasticity local-Newton work queue (QPs per wave range, 64..2048) by
+ * timing integrating passes at the given input; *chunk = the chosen value. */
+int pgf_autotune(pgf_handle h, const pgf_nodal_input *in, double dt, int64_t *chunk);
+
 /* ======================================================================= state */
 /* After convergence: n+1 -> n, n -> n-1 for every QP (constitutive_model_update_time_steps,
  * src/constitutive_model.cc:1442-1497; CM_J2P_PARAM::update_state_vars,

@@ -129,6 +129,18 @@ void make_grain(const double *R, double C11, double C12, double C44, DevGrain &g
 
 }  // namespace pgf
 
+// What a captured pass depends on: a new key re-captures.
+struct GraphKey {
+  int mode;
+  const void *r, *d_r, *defl, *defl_d, *f, *vals, *warm;
+  double dt;
+  int64_t chunk;
+  bool operator==(const GraphKey &o) const {
+    return mode == o.mode && r == o.r && d_r == o.d_r && defl == o.defl && defl_d == o.defl_d && f == o.f &&
+           vals == o.vals && warm == o.warm && dt == o.dt && chunk == o.chunk;
+  }
+};
+
 struct pgf_handle_s {
   int device = 0;
   hipStream_t own_stream = nullptr, stream = nullptr;
@@ -154,6 +166,9 @@ struct pgf_handle_s {
   double *d_qpbuf = nullptr;
   const double *warm = nullptr;
   double cp_g0 = 0.0, dt = 1.0;
+  int64_t cp_nq = 0;         // crystal-plasticity QPs: device QPs [0, cp_nq)
+  int64_t cp_chunk = 256;    // QPs per wave range of the local-Newton work queue (autotuned)
+  int32_t *d_qgrain_all = nullptr;  // [n_qp] grain of every device QP
   float *d_gquat = nullptr;  // [n_grains][4] orientation quaternions (learned warm start features)
   // learned warm start
   pgf_mlp sur = nullptr;
@@ -161,6 +176,11 @@ struct pgf_handle_s {
   float *d_feat = nullptr;     // [n_qp][34]
   double *d_sur_warm = nullptr;  // [11][n_qp]
   std::vector<double> cp_mat_host;  // C11, C12, C44
+  // hipGraph of a whole pass
+  bool use_graph = false;
+  hipGraphExec_t gexec = nullptr;
+  GraphKey gkey{};
+  int32_t glaunches = 0;
   // linear solve
   int32_t *d_Ap = nullptr, *d_Ai = nullptr;
   double *d_solve_work = nullptr;
@@ -241,22 +261,39 @@ pgf::CPArgs cp_args(pgf_handle h, const Block &b, const pgf_nodal_input *in) {
   return c;
 }
 
-void cp_prepass(pgf_handle h, const Block &b, const pgf_nodal_input *in, int mode) {
-  pgf::CPArgs c = cp_args(h, b, in);
+// All crystal-plasticity QPs as one range (device QPs [0, cp_nq)), for the kernels that do
+// not depend on the element type.
+pgf::CPArgs cp_args_all(pgf_handle h, const pgf_nodal_input *in) {
+  Block all;
+  all.qp_base = 0;
+  all.ne = h->cp_nq;
+  all.nqp = 1;
+  all.d_qgrain = h->d_qgrain_all;
+  pgf::CPArgs c = cp_args(h, all, in);
+  c.conn = nullptr;
+  return c;
+}
+
+// Crystal-plasticity pre-pass: F(n+1) per element type, then the local Newton (with the
+// learned start when attached) and the per-QP tangent data over all CP QPs at once.
+void cp_prepass(pgf_handle h, const pgf_nodal_input *in, int mode) {
+  if (!h->cp_nq) return;
+  pgf::CPArgs all = cp_args_all(h, in);
   if (mode & pgf::MODE_INTEGRATE) {
-    pgf::launch_cp_kinematics(b.nne, c, h->stream);
+    for (auto &b : h->blocks)
+      if (b.model == PGF_MODEL_CRYSTAL_PLASTICITY) pgf::launch_cp_kinematics(b.nne, cp_args(h, b, in), h->stream);
     if (h->sur && !h->warm) {  // learned start: features -> MLP -> (M, lambda, g)
-      pgf::launch_cp_features(c, h->d_gquat, h->d_norm, h->d_norm + 34, h->d_feat, h->stream);
+      pgf::launch_cp_features(all, h->d_gquat, h->d_norm, h->d_norm + 34, h->d_feat, h->stream);
       const float *Y = nullptr;
       int ldy = 0;
-      if (pgf::mlp_forward_device(h->sur, h->d_feat + b.qp_base * 34, c.nq, h->stream, &Y, &ldy) == PGF_OK) {
-        pgf::launch_cp_warm_from_slip(c, Y, ldy, h->d_norm + 68, h->d_sur_warm, h->stream);
-        c.warm = h->d_sur_warm;
+      if (pgf::mlp_forward_device(h->sur, h->d_feat, all.nq, h->stream, &Y, &ldy) == PGF_OK) {
+        pgf::launch_cp_warm_from_slip(all, Y, ldy, h->d_norm + 68, h->d_sur_warm, h->stream);
+        all.warm = h->d_sur_warm;
       }
     }
-    pgf::launch_cp_newton(b.nne, c, h->stream);
+    pgf::launch_cp_newton(all, h->cp_chunk, h->stream);
   }
-  pgf::launch_cp_tangent(b.nne, c, (mode & pgf::MODE_TANGENT) != 0, h->stream);
+  pgf::launch_cp_tangent(all, (mode & pgf::MODE_TANGENT) != 0, h->stream);
 }
 
 pgf::ElemArgs base_args(pgf_handle h, const pgf_nodal_input *in) {
@@ -283,17 +320,15 @@ pgf::ElemArgs base_args(pgf_handle h, const pgf_nodal_input *in) {
 }
 
 // One pass over all elements in the coloured schedule.
-int run_pass(pgf_handle h, const pgf_nodal_input *in, int mode, double *f, double *vals) {
-  if (!in || !in->r) return PGF_EINVAL;
-  if (h->n_coel && !h->d_cprops) return PGF_EINVAL;  // interface properties not set
-  if (hipMemsetAsync(h->d_err, 0, sizeof(int32_t), h->stream) != hipSuccess) return PGF_EHIP;
+// Every launch of one pass, in order, on the handle's stream (also what a captured graph
+// holds; the error counter is cleared by a kernel, not a memset node).
+int32_t enqueue_pass(pgf_handle h, const pgf_nodal_input *in, int mode, double *f, double *vals) {
+  pgf::launch_fill_i32(h->d_err, 1, 0, h->stream);
   pgf::ElemArgs a = base_args(h, in);
   a.f = f;
   a.vals = vals;
-  if (h->profiling) hipEventRecord(h->ev0, h->stream);
   int32_t launches = 0;
-  for (auto &b : h->blocks)
-    if (b.model == PGF_MODEL_CRYSTAL_PLASTICITY) cp_prepass(h, b, in, mode);
+  cp_prepass(h, in, mode);
   for (int32_t c = 0; c < h->n_colors; ++c)
     for (auto &b : h->blocks) {
       const int64_t e0 = b.color_ptr[c], e1 = b.color_ptr[c + 1];
@@ -356,8 +391,39 @@ int run_pass(pgf_handle h, const pgf_nodal_input *in, int mode, double *f, doubl
         ++launches;
       }
   }
+  return launches;
+}
+
+int run_pass(pgf_handle h, const pgf_nodal_input *in, int mode, double *f, double *vals) {
+  if (!in || !in->r) return PGF_EINVAL;
+  if (h->n_coel && !h->d_cprops) return PGF_EINVAL;  // interface properties not set
+  const bool graph = h->use_graph && !h->sur;  // the learned start may size buffers: eager
+  GraphKey key{mode, in->r, in->d_r, in->defl, in->defl_d, f, vals, h->warm, h->dt, h->cp_chunk};
+  if (graph && !(h->gexec && key == h->gkey)) {
+    if (h->gexec) {
+      hipGraphExecDestroy(h->gexec);
+      h->gexec = nullptr;
+    }
+    hipGraph_t g = nullptr;
+    if (hipStreamBeginCapture(h->stream, hipStreamCaptureModeRelaxed) != hipSuccess) return PGF_EHIP;
+    h->glaunches = enqueue_pass(h, in, mode, f, vals);
+    if (hipStreamEndCapture(h->stream, &g) != hipSuccess) return PGF_EHIP;
+    const hipError_t ie = hipGraphInstantiate(&h->gexec, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (ie != hipSuccess) {
+      h->gexec = nullptr;
+      return PGF_EHIP;
+    }
+    h->gkey = key;
+  }
+  if (h->profiling) hipEventRecord(h->ev0, h->stream);
+  if (graph) {
+    if (hipGraphLaunch(h->gexec, h->stream) != hipSuccess) return PGF_EHIP;
+    h->last_launches = h->glaunches;
+  } else {
+    h->last_launches = enqueue_pass(h, in, mode, f, vals);
+  }
   if (h->profiling) hipEventRecord(h->ev1, h->stream);
-  h->last_launches = launches;
   if (hipGetLastError() != hipSuccess) return PGF_EHIP;
   int32_t err = 0;
   if (hipMemcpyAsync(&err, h->d_err, sizeof(int32_t), hipMemcpyDeviceToHost, h->stream) != hipSuccess)
@@ -501,15 +567,24 @@ int pgf_create(const pgf_mesh *mesh, const int64_t *id, int64_t ndof, const int3
     if (md == PGF_MODEL_J2_PLASTICITY) h->has_j2 = true;
     if (md == PGF_MODEL_CRYSTAL_PLASTICITY) h->has_cp = true;
   }
+  // crystal-plasticity blocks first: their QPs then form one contiguous range that the
+  // element-type-agnostic QP kernels (local Newton, tangent) cover in single launches
+  std::vector<int> border(keys.size());
+  for (size_t i = 0; i < keys.size(); ++i) border[i] = int(i);
+  std::stable_sort(border.begin(), border.end(), [&](int x, int y) {
+    return (keys[x].second == PGF_MODEL_CRYSTAL_PLASTICITY) > (keys[y].second == PGF_MODEL_CRYSTAL_PLASTICITY);
+  });
+  std::vector<int> block_of_key(keys.size());
+  for (size_t i = 0; i < keys.size(); ++i) block_of_key[border[i]] = int(i);
   h->blocks.resize(keys.size());
   int64_t qp = 0;
   for (size_t bi = 0; bi < keys.size(); ++bi) {
     Block &b = h->blocks[bi];
-    b.nne = keys[bi].first;
-    b.model = keys[bi].second;
+    b.nne = keys[border[bi]].first;
+    b.model = keys[border[bi]].second;
     b.nqp = pgf::nqp_of(b.nne);
     for (int64_t e = 0; e < m.n_elem; ++e)
-      if (elem_block[e] == int(bi)) b.elems.push_back(e);
+      if (block_of_key[elem_block[e]] == int(bi)) b.elems.push_back(e);
     std::stable_sort(b.elems.begin(), b.elems.end(), [&](int64_t x, int64_t y) { return color[x] < color[y]; });
     b.ne = int64_t(b.elems.size());
     b.color_ptr.assign(ncol + 1, 0);
@@ -519,6 +594,8 @@ int pgf_create(const pgf_mesh *mesh, const int64_t *id, int64_t ndof, const int3
     qp += b.ne * b.nqp;
   }
   h->n_qp = qp;
+  for (auto &b : h->blocks)
+    if (b.model == PGF_MODEL_CRYSTAL_PLASTICITY) h->cp_nq += b.ne * b.nqp;
   // mesh QP numbering: elements in mesh order, QPs in element order
   std::vector<int64_t> qp_of_elem(m.n_elem + 1, 0);
   for (int64_t e = 0; e < m.n_elem; ++e)
@@ -689,13 +766,13 @@ int pgf_create(const pgf_mesh *mesh, const int64_t *id, int64_t ndof, const int3
     if ((rc = pgf::dalloc(&h->d_iters, size_t(h->n_qp)))) return fail(rc);
     if ((rc = pgf::dalloc(&h->d_qpbuf, size_t(h->n_qp) * pgf::kCPBuf))) return fail(rc);
     if (hipMemset(h->d_iters, 0, sizeof(int32_t) * h->n_qp) != hipSuccess) return fail(PGF_EHIP);
+    if ((rc = pgf::dalloc(&h->d_qgrain_all, size_t(h->n_qp)))) return fail(rc);
+    if (hipMemset(h->d_qgrain_all, 0, sizeof(int32_t) * h->n_qp) != hipSuccess) return fail(PGF_EHIP);
     for (auto &b : h->blocks) {
       if (b.model != PGF_MODEL_CRYSTAL_PLASTICITY) continue;
       if ((rc = pgf::dalloc(&b.d_egrain, size_t(b.ne)))) return fail(rc);
-      if ((rc = pgf::dalloc(&b.d_qgrain, size_t(b.ne * b.nqp)))) return fail(rc);
-      if (hipMemset(b.d_egrain, 0, sizeof(int32_t) * b.ne) != hipSuccess ||
-          hipMemset(b.d_qgrain, 0, sizeof(int32_t) * b.ne * b.nqp) != hipSuccess)
-        return fail(PGF_EHIP);
+      if (hipMemset(b.d_egrain, 0, sizeof(int32_t) * b.ne) != hipSuccess) return fail(PGF_EHIP);
+      b.d_qgrain = h->d_qgrain_all + b.qp_base;  // view into the handle's array
     }
     const double I9[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
     if ((rc = pgf_set_orientations(h, 1, I9, nullptr))) return fail(rc);
@@ -719,7 +796,6 @@ int pgf_destroy(pgf_handle h) {
     hipFree(b.d_off);
     hipFree(b.d_mat);
     hipFree(b.d_egrain);
-    hipFree(b.d_qgrain);
   }
   for (auto &cb : h->cblocks) {
     hipFree(cb.d_conn);
@@ -727,6 +803,7 @@ int pgf_destroy(pgf_handle h) {
     hipFree(cb.d_off);
   }
   for (int t = 0; t < 3; ++t) hipFree(h->d_cv[t]);
+  if (h->gexec) hipGraphExecDestroy(h->gexec);
   hipFree(h->d_Ap);
   hipFree(h->d_Ai);
   hipFree(h->d_solve_work);
@@ -734,6 +811,7 @@ int pgf_destroy(pgf_handle h) {
   hipFree(h->d_cpmat);
   hipFree(h->d_grains);
   hipFree(h->d_gquat);
+  hipFree(h->d_qgrain_all);
   hipFree(h->d_norm);
   hipFree(h->d_feat);
   hipFree(h->d_sur_warm);
@@ -1017,12 +1095,9 @@ int pgf_get_cohesive_sizes(pgf_handle h, int64_t *n_cqp, int32_t *n_colors) {
 int pgf_cp_log_samples(pgf_handle h, float *X, float *T) {
   if (!h || !X || !T) return PGF_EINVAL;
   if (!h->has_cp) return PGF_ENOTSUP;
-  for (auto &b : h->blocks) {
-    if (b.model != PGF_MODEL_CRYSTAL_PLASTICITY) continue;
-    pgf::CPArgs c = cp_args(h, b, nullptr);
-    pgf::launch_cp_features(c, h->d_gquat, nullptr, nullptr, X, h->stream);
-    pgf::launch_cp_targets(c, T, h->stream);
-  }
+  pgf::CPArgs c = cp_args_all(h, nullptr);
+  pgf::launch_cp_features(c, h->d_gquat, nullptr, nullptr, X, h->stream);
+  pgf::launch_cp_targets(c, T, h->stream);
   if (hipGetLastError() != hipSuccess) return PGF_EHIP;
   return hipStreamSynchronize(h->stream) == hipSuccess ? PGF_OK : PGF_EHIP;
 }
@@ -1066,6 +1141,54 @@ int pgf_solve(pgf_handle h, const double *vals, const double *b, double *x, doub
   if (iters) *iters = it;
   if (relres) *relres = rr;
   return (rr <= tol) ? PGF_OK : 1;  // 1 = not converged within maxit
+}
+
+int pgf_set_graph(pgf_handle h, int on) {
+  if (!h) return PGF_EINVAL;
+  h->use_graph = on != 0;
+  if (!h->use_graph && h->gexec) {
+    hipGraphExecDestroy(h->gexec);
+    h->gexec = nullptr;
+  }
+  return PGF_OK;
+}
+
+int pgf_autotune(pgf_handle h, const pgf_nodal_input *in, double dt, int64_t *chunk_out) {
+  if (!h || !in || !in->r) return PGF_EINVAL;
+  if (dt > 0.0) h->dt = dt;
+  if (!h->cp_nq) {
+    if (chunk_out) *chunk_out = h->cp_chunk;
+    return PGF_OK;
+  }
+  double *f = nullptr;
+  int rc = pgf::dalloc(&f, size_t(h->ndof));
+  if (rc) return rc;
+  const bool prof = h->profiling, graph = h->use_graph;
+  h->profiling = true;
+  h->use_graph = false;
+  const int64_t cand[] = {64, 128, 256, 512, 1024, 2048};
+  int64_t best = h->cp_chunk;
+  double best_ms = 1e300;
+  for (int64_t c : cand) {
+    h->cp_chunk = c;
+    double ms = 1e300;
+    for (int rep = 0; rep < 3; ++rep) {
+      rc = run_pass(h, in, pgf::MODE_INTEGRATE | pgf::MODE_RESIDUAL, f, nullptr);
+      if (rc < 0) break;
+      ms = std::min(ms, h->last_ms);
+    }
+    if (rc < 0) break;
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = c;
+    }
+  }
+  hipFree(f);
+  h->cp_chunk = best;
+  h->profiling = prof;
+  h->use_graph = graph;
+  if (chunk_out) *chunk_out = best;
+  return rc < 0 ? rc : PGF_OK;
 }
 
 }  // extern "C"

@@ -1091,16 +1091,14 @@ void launch_cp_kinematics(int nne, const CPArgs &a, hipStream_t s) {
   else if (nne == 8) launch_kin<8>(a, s);
 }
 
-void launch_cp_newton(int nne, const CPArgs &a, hipStream_t s) {
-  (void)nne;
+void launch_cp_newton(const CPArgs &a, int64_t chunk, hipStream_t s) {
   const int64_t nq = a.nq;
   if (nq <= 0) return;
-  const int64_t chunk = 256;
+  if (chunk < 64) chunk = 64;
   cp_newton<<<unsigned((nq + chunk - 1) / chunk), 64, 0, s>>>(a, chunk);
 }
 
-void launch_cp_tangent(int nne, const CPArgs &a, bool with_D, hipStream_t s) {
-  (void)nne;
+void launch_cp_tangent(const CPArgs &a, bool with_D, hipStream_t s) {
   const int64_t nq = a.nq;
   if (nq <= 0) return;
   if (with_D) cp_tangent<true><<<unsigned((nq + 63) / 64), 64, 0, s>>>(a, nq);

@@ -95,8 +95,8 @@ struct CPArgs {
 constexpr int kCPBuf = 97;  // doubles per QP in qpbuf
 
 void launch_cp_kinematics(int nne, const CPArgs &a, hipStream_t s);
-void launch_cp_newton(int nne, const CPArgs &a, hipStream_t s);
-void launch_cp_tangent(int nne, const CPArgs &a, bool with_D, hipStream_t s);
+void launch_cp_newton(const CPArgs &a, int64_t chunk, hipStream_t s);
+void launch_cp_tangent(const CPArgs &a, bool with_D, hipStream_t s);
 
 // Element kernel of the crystal-plasticity blocks: residual and / or tangent from qpbuf.
 struct CPElemArgs {
@@ -150,6 +150,7 @@ int bicgstab(hipStream_t s, const int32_t *Ap, const int32_t *Ai, const double *
              int64_t n, double tol, int maxit, int *iters, double *relres, double *work);
 
 void launch_fill(double *p, int64_t n, double v, hipStream_t s);
+void launch_fill_i32(int32_t *p, int64_t n, int32_t v, hipStream_t s);
 void launch_copy(double *dst, const double *src, int64_t n, hipStream_t s);
 
 }  // namespace pgf

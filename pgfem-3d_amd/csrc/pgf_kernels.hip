@@ -449,6 +449,10 @@ __global__ void fill_kernel(double *p, int64_t n, double v) {
   for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
     p[i] = v;
 }
+__global__ void fill_i32_kernel(int32_t *p, int64_t n, int32_t v) {
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+    p[i] = v;
+}
 __global__ void copy_kernel(double *dst, const double *src, int64_t n) {
   for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
     dst[i] = src[i];
@@ -492,6 +496,12 @@ void launch_fill(double *p, int64_t n, double v, hipStream_t s) {
   if (n <= 0) return;
   const int64_t blocks = std::min<int64_t>((n + 255) / 256, 8192);
   fill_kernel<<<unsigned(blocks), 256, 0, s>>>(p, n, v);
+}
+
+void launch_fill_i32(int32_t *p, int64_t n, int32_t v, hipStream_t s) {
+  if (n <= 0) return;
+  const int64_t blocks = std::min<int64_t>((n + 255) / 256, 8192);
+  fill_i32_kernel<<<unsigned(blocks), 256, 0, s>>>(p, n, v);
 }
 
 void launch_copy(double *dst, const double *src, int64_t n, hipStream_t s) {

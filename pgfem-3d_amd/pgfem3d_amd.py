@@ -117,6 +117,8 @@ def lib() -> C.CDLL:
             "pgf_mlp_adam_step": [P, C.c_float, C.c_float, C.c_float, C.c_float],
             "pgf_cp_log_samples": [P, P, P],
             "pgf_solve": [P, P, P, P, D, I32, P, P],
+            "pgf_set_graph": [P, C.c_int],
+            "pgf_autotune": [P, P, D, P],
             "pgf_set_surrogate": [P, P, P, P, P],
         }
         for name, args in sig.items():
@@ -137,7 +139,8 @@ def exported_symbols():
         "pgf_set_warm_start", "pgf_get_qp_order", "pgf_state_device_ptr", "pgf_set_cohesive_props",
         "pgf_get_cohesive_sizes", "pgf_mlp_create", "pgf_mlp_destroy", "pgf_mlp_param_count",
         "pgf_mlp_get_params", "pgf_mlp_set_params", "pgf_mlp_get_grads", "pgf_mlp_forward", "pgf_mlp_backward",
-        "pgf_mlp_adam_step", "pgf_cp_log_samples", "pgf_set_surrogate", "pgf_solve",
+        "pgf_mlp_adam_step", "pgf_cp_log_samples", "pgf_set_surrogate", "pgf_solve", "pgf_set_graph",
+        "pgf_autotune",
     ]
 
 
@@ -314,6 +317,15 @@ class Handle:
         ms, nl = C.c_double(), C.c_int32()
         _check(self._lib.pgf_last_pass_ms(self.h, C.byref(ms), C.byref(nl)), "pgf_last_pass_ms")
         return ms.value, nl.value
+
+    def set_graph(self, on: bool):
+        _check(self._lib.pgf_set_graph(self.h, int(on)), "pgf_set_graph")
+
+    def autotune(self, r, d_r=None, defl=None, defl_d=None, dt=1.0) -> int:
+        inp = self._input(r, d_r, defl, defl_d)
+        c = C.c_int64()
+        _check(self._lib.pgf_autotune(self.h, C.byref(inp), dt, C.byref(c)), "pgf_autotune")
+        return c.value
 
     def set_stream(self, stream_ptr: int | None):
         _check(self._lib.pgf_set_stream(self.h, stream_ptr), "pgf_set_stream")

@@ -93,3 +93,66 @@ def oracle_newton_step(prob, st, r, defl, state, tol=1e-12, max_iter=25, cp_tabl
         K = sp.csr_matrix((vals, prob.Ai, prob.Ap), shape=(prob.ndof, prob.ndof))
         r = r + spla.spsolve(K.tocsc(), R)
     raise AssertionError("oracle Newton did not converge")
+
+
+def mixed_mesh(n_tet4, n_tet10, n_hex8, n_grains, jitter=0.1, seed=1, interfaces=True):
+    """One mesh holding three disjoint boxes side by side along x -- linear tets, quadratic
+    tets and hexes -- in Voronoi grains drawn over the whole bounding box, with interface
+    elements on the grain boundaries of the tet4 and hex8 boxes (the reference's interface
+    elements are linear triangles or quads, so the quadratic box keeps its grains bonded).
+    Returns (mesh, elem_grain, quaternions)."""
+    rng = np.random.default_rng(seed)
+    pts = rng.random((n_grains, 3)) * np.array([3.5, 1.0, 1.0])
+    parts = []
+    for k, (n, et) in enumerate(((n_tet4, P.PGF_TET4), (n_tet10, P.PGF_TET10), (n_hex8, P.PGF_HEX8))):
+        if n <= 0:
+            continue
+        m = P.mesh_box(n, et, jitter, seed + k)
+        m.X[:, 0] += 1.25 * k
+        nne = np.diff(m.elem_ptr)[0]
+        conn = m.elem_conn.reshape(-1, nne)
+        cen = m.X[conn[:, :4 if nne == 10 else nne]].mean(axis=1)
+        grain = np.argmin(((cen[:, None, :] - pts[None]) ** 2).sum(-1), axis=1).astype(np.int32)
+        if interfaces and et != P.PGF_TET10:
+            m = P.split_grains(m, grain)
+        parts.append((m, grain))
+    X = np.concatenate([m.X for m, _ in parts])
+    off = np.cumsum([0] + [m.n_nodes for m, _ in parts])
+    conn = np.concatenate([m.elem_conn + off[i] for i, (m, _) in enumerate(parts)])
+    eptr = [np.zeros(1, dtype=np.int64)]
+    base = 0
+    for m, _ in parts:
+        eptr.append(m.elem_ptr[1:] + base)
+        base += m.elem_ptr[-1]
+    cc = [m.coel_conn + off[i] for i, (m, _) in enumerate(parts) if m.coel_ptr is not None]
+    cp = [np.diff(m.coel_ptr) for m, _ in parts if m.coel_ptr is not None]
+    coel_ptr = coel_conn = None
+    if cc:
+        coel_conn = np.ascontiguousarray(np.concatenate(cc))
+        coel_ptr = np.concatenate([[0], np.cumsum(np.concatenate(cp))]).astype(np.int64)
+    mesh = P.MeshArrays(np.ascontiguousarray(X), np.concatenate(eptr), np.ascontiguousarray(conn), None, coel_ptr,
+                        coel_conn)
+    grain = np.concatenate([g for _, g in parts])
+    return mesh, grain, P.random_quaternions(n_grains, seed + 7)
+
+
+class MixedProblem(BoxProblem):
+    """Crystal plasticity on the mixed mesh, interfaces on the tet4 / hex8 grain boundaries."""
+
+    def __init__(self, n_tet4, n_tet10, n_hex8, material, coh, n_grains=6, jitter=0.1, seed=1):
+        self.mesh, self.grain, q = mixed_mesh(n_tet4, n_tet10, n_hex8, n_grains, jitter, seed)
+        self.R = P.quat_to_rot(q)
+        self.supports = P.box_supports(self.mesh)
+        self.ids, self.ndof = P.generate_dof_ids(self.mesh, self.supports)
+        self.Ap, self.Ai = P.psparse_apai(self.mesh, self.ids, self.ndof)
+        self.mats = [material]
+        self.coh = [coh]
+        self.nqp = int(sum(O.nint(int(k)) for k in np.diff(self.mesh.elem_ptr)))
+        self.ncqp = 0 if self.mesh.coel_ptr is None else int(np.sum(np.diff(self.mesh.coel_ptr) // 2))
+        tabs = [O.cp_grain(material, self.R[g]) for g in range(n_grains)]
+        self.gP = np.ascontiguousarray(np.stack([t[0] for t in tabs]).reshape(n_grains, 108))
+        self.gL = np.ascontiguousarray(np.stack([t[1] for t in tabs]).reshape(n_grains, 81))
+
+    def assemble_coh(self, st, r, defl, f, vals):
+        return O.assemble_coh(self.mesh, self.coh, None, self.ids, self.ndof, self.Ap, self.Ai, st, r, f, vals,
+                              defl=defl)
