@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <numeric>
@@ -23,7 +24,7 @@ struct Block {
   int nne = 0, nqp = 0, model = 0;
   int64_t ne = 0, qp_base = 0;
   std::vector<int64_t> elems;      // mesh element ids in device order
-  std::vector<int64_t> color_ptr;  // [n_colors+1] segments of elems
+  std::vector<int64_t> color_ptr;  // [n_slabs*n_colors+1] (slab, colour) segments of elems
   int32_t *d_conn = nullptr;
   int16_t *d_off = nullptr;
   int32_t *d_mat = nullptr;
@@ -145,7 +146,7 @@ struct pgf_handle_s {
   int device = 0;
   hipStream_t own_stream = nullptr, stream = nullptr;
   int64_t n_nodes = 0, ndof = 0, nnz = 0, n_qp = 0;
-  int32_t n_colors = 0;
+  int32_t n_colors = 0, n_slabs = 1, n_segs = 0;
   bool has_j2 = false;
   double *d_X = nullptr;
   int32_t *d_dofid = nullptr, *d_row_start = nullptr, *d_rowlen = nullptr;
@@ -329,7 +330,7 @@ int32_t enqueue_pass(pgf_handle h, const pgf_nodal_input *in, int mode, double *
   a.vals = vals;
   int32_t launches = 0;
   cp_prepass(h, in, mode);
-  for (int32_t c = 0; c < h->n_colors; ++c)
+  for (int32_t c = 0; c < h->n_segs; ++c)  // (slab, colour) segments, slab-major
     for (auto &b : h->blocks) {
       const int64_t e0 = b.color_ptr[c], e1 = b.color_ptr[c + 1];
       if (e1 <= e0) continue;
@@ -576,6 +577,35 @@ int pgf_create(const pgf_mesh *mesh, const int64_t *id, int64_t ndof, const int3
   });
   std::vector<int> block_of_key(keys.size());
   for (size_t i = 0; i < keys.size(); ++i) block_of_key[border[i]] = int(i);
+  // slab-major schedule: elements are split into n_slabs equal-count slabs along the mesh's
+  // longest axis (by centroid) and passes launch (slab, colour) segments in that order, so the
+  // CSR rows a slab's colours revisit stay cache-resident between its launches
+  int n_slabs = int(std::min<int64_t>(8, std::max<int64_t>(1, (int64_t(Ap[ndof]) * 8) / (int64_t(512) << 20))));
+  if (const char *env = std::getenv("PGF_ASSEMBLY_SLABS")) n_slabs = std::max(1, std::min(64, std::atoi(env)));
+  std::vector<int32_t> slab(m.n_elem, 0);
+  if (n_slabs > 1 && m.n_elem > 0) {
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int64_t n = 0; n < m.n_nodes; ++n)
+      for (int d = 0; d < 3; ++d) {
+        lo[d] = std::min(lo[d], m.X[n * 3 + d]);
+        hi[d] = std::max(hi[d], m.X[n * 3 + d]);
+      }
+    int ax = 0;
+    for (int d = 1; d < 3; ++d)
+      if (hi[d] - lo[d] > hi[ax] - lo[ax]) ax = d;
+    std::vector<double> key(m.n_elem);
+    for (int64_t e = 0; e < m.n_elem; ++e) {
+      double c = 0.0;
+      for (int64_t k = m.elem_ptr[e]; k < m.elem_ptr[e + 1]; ++k) c += m.X[m.elem_conn[k] * 3 + ax];
+      key[e] = c / double(m.elem_ptr[e + 1] - m.elem_ptr[e]);
+    }
+    std::vector<int64_t> rank(m.n_elem);
+    for (int64_t e = 0; e < m.n_elem; ++e) rank[e] = e;
+    std::stable_sort(rank.begin(), rank.end(), [&](int64_t x, int64_t y) { return key[x] < key[y]; });
+    for (int64_t i = 0; i < m.n_elem; ++i) slab[rank[i]] = int32_t((i * n_slabs) / m.n_elem);
+  }
+  h->n_slabs = n_slabs;
+  h->n_segs = n_slabs * ncol;
   h->blocks.resize(keys.size());
   int64_t qp = 0;
   for (size_t bi = 0; bi < keys.size(); ++bi) {
@@ -585,11 +615,12 @@ int pgf_create(const pgf_mesh *mesh, const int64_t *id, int64_t ndof, const int3
     b.nqp = pgf::nqp_of(b.nne);
     for (int64_t e = 0; e < m.n_elem; ++e)
       if (block_of_key[elem_block[e]] == int(bi)) b.elems.push_back(e);
-    std::stable_sort(b.elems.begin(), b.elems.end(), [&](int64_t x, int64_t y) { return color[x] < color[y]; });
+    auto seg = [&](int64_t e) { return int64_t(slab[e]) * ncol + color[e]; };
+    std::stable_sort(b.elems.begin(), b.elems.end(), [&](int64_t x, int64_t y) { return seg(x) < seg(y); });
     b.ne = int64_t(b.elems.size());
-    b.color_ptr.assign(ncol + 1, 0);
-    for (int64_t e : b.elems) b.color_ptr[color[e] + 1]++;
-    for (int c = 0; c < ncol; ++c) b.color_ptr[c + 1] += b.color_ptr[c];
+    b.color_ptr.assign(h->n_segs + 1, 0);
+    for (int64_t e : b.elems) b.color_ptr[seg(e) + 1]++;
+    for (int c = 0; c < h->n_segs; ++c) b.color_ptr[c + 1] += b.color_ptr[c];
     b.qp_base = qp;
     qp += b.ne * b.nqp;
   }
@@ -611,7 +642,7 @@ int pgf_create(const pgf_mesh *mesh, const int64_t *id, int64_t ndof, const int3
   std::vector<uint8_t> touched(size_t(h->nnz), 0);
   std::vector<std::vector<int16_t>> boff(h->blocks.size());
   for (size_t bi = 0; bi < h->blocks.size(); ++bi) boff[bi].resize(h->blocks[bi].ne * h->blocks[bi].nne * h->blocks[bi].nne);
-  for (int32_t c = 0; c < ncol; ++c)
+  for (int32_t c = 0; c < h->n_segs; ++c)
     for (size_t bi = 0; bi < h->blocks.size(); ++bi) {
       Block &b = h->blocks[bi];
       const int nne = b.nne;
