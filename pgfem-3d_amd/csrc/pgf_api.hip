@@ -1195,7 +1195,10 @@ int pgf_autotune(pgf_handle h, const pgf_nodal_input *in, double dt, int64_t *ch
   int rc = pgf::dalloc(&f, size_t(h->ndof));
   if (rc) return rc;
   const bool prof = h->profiling, graph = h->use_graph;
-  h->profiling = true;
+  if ((rc = pgf_set_profiling(h, 1))) {  // creates the timing events on first use
+    hipFree(f);
+    return rc;
+  }
   h->use_graph = false;
   const int64_t cand[] = {64, 128, 256, 512, 1024, 2048};
   int64_t best = h->cp_chunk;

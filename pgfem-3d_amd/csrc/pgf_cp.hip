@@ -292,6 +292,79 @@ __device__ __forceinline__ void cp_jacobian(const CPMaterial &cm, const DevGrain
     for (int c = 0; c < 9; ++c) Ah[r * 9 + c] -= B.c[r] * B.e[c] * rf;
 }
 
+// cp_jacobian for callers that do not keep the per-system arrays of the point: they are
+// recomputed from the Mandel stress with the same arithmetic.
+__device__ __forceinline__ void cp_jacobian_re(const CPMaterial &cm, const DevGrain &gr, double dt, const double *x,
+                                            const double *A, const CPPoint &p, double g_n,
+                                            double (&Ah)[81], Border &B) {
+  const double lam = x[9], g = x[10];
+  double AtE[9];
+  mtm3(A, p.eF, AtE);
+#pragma unroll
+  for (int i = 0; i < 81; ++i) Ah[i] = 0.0;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    B.c[i] = 0.0;
+    B.e[i] = 0.0;
+  }
+  double sg = 0.0;
+  const double expo = 1.0 / cm.m - 1.0;
+  for (int a = 0; a < 12; ++a) {
+    // slip-system quantities recomputed from the Mandel stress (as cp_point evaluates them)
+    double t = 0.0;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) t += p.Sig[i] * gr.P[a][i];
+    const double r = t / g;
+    const double pw = pow(fabs(r), expo);
+    const double gd = cm.gd0 * r * pw;
+    const double dgdt = cm.gd0 / (cm.m * g) * pw;
+    const double dgdg = -dgdt * t / g;
+    double D[9];
+    dtau_dM(gr, a, AtE, p, D);
+    const double co = dt * dgdt;
+    const double sgn = (gd < 0) ? -1.0 : 1.0;
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+      const double pr = gr.P[a][r] * co;
+#pragma unroll
+      for (int c = 0; c < 9; ++c) Ah[r * 9 + c] += pr * D[c];
+      B.c[r] += dt * gr.P[a][r] * dgdg;
+    }
+#pragma unroll
+    for (int c = 0; c < 9; ++c) B.e[c] += sgn * dgdt * D[c];
+    sg += sgn * dgdg;
+  }
+  const double *Mi = p.Mi;
+  const double lp = dt * lam * p.pJ;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int l = 0; l < 3; ++l)
+          Ah[(i * 3 + j) * 9 + k * 3 + l] +=
+              ((i == k && j == l) ? 1.0 : 0.0) + lp * (Mi[j * 3 + i] * Mi[l * 3 + k] + Mi[l * 3 + i] * Mi[j * 3 + k]);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      B.b[i * 3 + j] = -dt * p.pJ * Mi[j * 3 + i];
+      B.d[i * 3 + j] = -p.pJ * Mi[j * 3 + i];
+    }
+  B.dRg = g * (p.dgs + dt * cm.h0) - p.dgs * g_n - dt * cm.h0 * (p.dgs * p.gsum + p.gs);
+#pragma unroll
+  for (int c = 0; c < 9; ++c) B.e[c] *= B.dRg;
+  B.f = (p.gs - cm.g0 + dt * cm.h0 * p.gsum) + B.dRg * sg;
+  // Schur: Ah <- A - c e^T / f
+  const double rf = 1.0 / B.f;
+#pragma unroll
+  for (int r = 0; r < 9; ++r)
+#pragma unroll
+    for (int c = 0; c < 9; ++c) Ah[r * 9 + c] -= B.c[r] * B.e[c] * rf;
+}
+
 // Solve [A b c; d^T 0 0; e^T 0 f] [u; y; z] = [r; s; t] given LU(Ah) and v = Ah^-1 b.
 __device__ __forceinline__ void bordered_solve(const double (&LU)[81], const int (&piv)[9], const Border &B,
                                                const double (&v)[9], double dTv, const double *r, double s,
@@ -321,6 +394,58 @@ __device__ __forceinline__ double cp_merit(const CPPoint &p, double gscale) {
   for (int i = 0; i < 10; ++i) r += p.R[i] * p.R[i];
   const double rg = p.R[10] * gscale;
   return 0.5 * (r + rg * rg);
+}
+
+// The merit of x without keeping the point: the same arithmetic as cp_point's residual (slip
+// contributions accumulated in slip-system order), so acceptance decisions match it.
+__device__ __forceinline__ bool cp_merit_at(const CPMaterial &cm, const DevGrain &gr, const double *A,
+                                            double det_pFn, double g_n, double dt, const double *x, double gscale,
+                                            double &merit) {
+  const double *M = x;
+  const double lam = x[9], g = x[10];
+  if (!(g > 0.0)) return false;
+  const double dM = det3(M);
+  if (!(dM > 0.0)) return false;
+  double Mi[9];
+  if (!inv3(M, Mi)) return false;
+  const double pJ = det_pFn / dM;
+  double eF[9], Ce[9], E[9], S[6], S9[9], Sig[9];
+  mm3(A, M, eF);
+  mtm3(eF, eF, Ce);
+#pragma unroll
+  for (int i = 0; i < 9; ++i) E[i] = 0.5 * (Ce[i] - ((i % 4 == 0) ? 1.0 : 0.0));
+  lv_dot(gr.Lv, E, S);
+  v6_to_9(S, S9);
+  mm3(Ce, S9, Sig);
+  const double expo = 1.0 / cm.m - 1.0;
+  double gsum = 0.0, sl[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int a = 0; a < 12; ++a) {
+    double t = 0.0;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) t += Sig[i] * gr.P[a][i];
+    const double r = t / g;
+    const double gd = cm.gd0 * r * pow(fabs(r), expo);
+    gsum += fabs(gd);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) sl[i] += gd * gr.P[a][i];
+  }
+  double gs;
+  if (cm.w == 0.0) gs = cm.gs0;
+  else if (gsum / cm.gds > 1.0e-15) gs = cm.gs0 * pow(gsum / cm.gds, cm.w);
+  else gs = 0.0;
+  double r2 = 0.0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const double R = M[i * 3 + j] - ((i == j) ? 1.0 : 0.0) + dt * sl[i * 3 + j] - dt * lam * pJ * Mi[j * 3 + i];
+      r2 += R * R;
+    }
+  const double R9 = pJ - 1.0;
+  r2 += R9 * R9;
+  const double R10 = (g * (gs - cm.g0 + dt * cm.h0 * gsum) - (gs - cm.g0) * g_n - dt * cm.h0 * gs * gsum) * gscale;
+  merit = 0.5 * (r2 + R10 * R10);
+  return true;
 }
 
 // max_a |tau_a| / g at M (elastic part eF = A M)
@@ -497,10 +622,10 @@ __global__ __launch_bounds__(64) void cp_newton(CPArgs a, int64_t chunk) {
           double xw[11];
 #pragma unroll
           for (int i = 0; i < 11; ++i) xw[i] = a.warm[i * NQ + qg];
-          CPPoint pw, pp;
-          const bool okw = cp_point(cm, a.grains[gid], A, det_pFn, g_n, dt, xw, pw);
-          const bool okp = !bad_start && cp_point(cm, a.grains[gid], A, det_pFn, g_n, dt, x, pp);
-          if (okw && (!okp || cp_merit(pw, gscale) < cp_merit(pp, gscale))) {
+          double mw = 0.0, mp = 0.0;
+          const bool okw = cp_merit_at(cm, a.grains[gid], A, det_pFn, g_n, dt, xw, gscale, mw);
+          const bool okp = !bad_start && cp_merit_at(cm, a.grains[gid], A, det_pFn, g_n, dt, x, gscale, mp);
+          if (okw && (!okp || mw < mp)) {
 #pragma unroll
             for (int i = 0; i < 11; ++i) x[i] = xw[i];
             from_warm = true;
@@ -525,14 +650,14 @@ __global__ __launch_bounds__(64) void cp_newton(CPArgs a, int64_t chunk) {
     } else {
       double Ah[81];
       Border B;
-      cp_jacobian(cm, gr, dt, x, A, p, g_n, Ah, B);
+      cp_jacobian_re(cm, gr, dt, x, A, p, g_n, Ah, B);
       int piv[9];
       if (!lu9(Ah, piv)) {
         fail = true;
       } else {
         double v[9];
 #pragma unroll
-        for (int i = 0; i < 9; ++i) v[i] = B.b[i];
+        for (int i = 0; i < 9; ++i) v[i] = dt * B.d[i];  // b = dt d
         lu9_solve(Ah, piv, v);
         double dTv = 0.0;
 #pragma unroll
@@ -555,11 +680,11 @@ __global__ __launch_bounds__(64) void cp_newton(CPArgs a, int64_t chunk) {
           const double f0 = cp_merit(p, gscale);
           double alpha = 1.0, xn[11];
           bool ok = false;
-          CPPoint pn;
           for (int ls = 0; ls < 40; ++ls) {
 #pragma unroll
             for (int i = 0; i < 11; ++i) xn[i] = x[i] + alpha * dx[i];
-            if (cp_point(cm, gr, A, det_pFn, g_n, dt, xn, pn) && cp_merit(pn, gscale) <= (1.0 - 1e-4 * alpha) * f0) {
+            double mn = 0.0;
+            if (cp_merit_at(cm, gr, A, det_pFn, g_n, dt, xn, gscale, mn) && mn <= (1.0 - 1e-4 * alpha) * f0) {
               ok = true;
               break;
             }
@@ -568,7 +693,8 @@ __global__ __launch_bounds__(64) void cp_newton(CPArgs a, int64_t chunk) {
           if (!ok) {
 #pragma unroll
             for (int i = 0; i < 11; ++i) xn[i] = x[i] + dx[i];
-            if (!cp_point(cm, gr, A, det_pFn, g_n, dt, xn, pn)) fail = true;
+            double mn = 0.0;
+            if (!cp_merit_at(cm, gr, A, det_pFn, g_n, dt, xn, gscale, mn)) fail = true;
           }
           if (!fail) {
 #pragma unroll
@@ -648,10 +774,14 @@ __global__ __launch_bounds__(64) void cp_tangent(CPArgs a, int64_t nqb) {
     for (int i = 0; i < kCPBuf; ++i) out[i] = 0.0;
     return;
   }
+  const double expo = 1.0 / cm.m - 1.0;
+  for (int al = 0; al < 12; ++al) {  // tau and gamma_dot as cp_point evaluates them
+    double tt = 0.0;
 #pragma unroll
-  for (int al = 0; al < 12; ++al) {
-    a.tau[al * NQ + qg] = p.tau[al];
-    a.gdot[al * NQ + qg] = p.gdot[al];
+    for (int i = 0; i < 9; ++i) tt += p.Sig[i] * gr.P[al][i];
+    const double r = tt / x[10];
+    a.tau[al * NQ + qg] = tt;
+    a.gdot[al * NQ + qg] = cm.gd0 * r * pow(fabs(r), expo);
   }
   // elastic PK2 from the stored state (eF = F pF(n+1)^-1)
   double eF[9], Ce[9], E[9], S[6];
@@ -668,7 +798,7 @@ __global__ __launch_bounds__(64) void cp_tangent(CPArgs a, int64_t nqb) {
   if constexpr (WITH_D) {
     double Ah[81];
     Border B;
-    cp_jacobian(cm, gr, a.dt, x, A, p, g_n, Ah, B);
+    cp_jacobian_re(cm, gr, a.dt, x, A, p, g_n, Ah, B);
     int piv[9];
     if (!lu9(Ah, piv)) {
       atomicAdd(a.err, 1);
@@ -707,8 +837,14 @@ __global__ __launch_bounds__(64) void cp_tangent(CPArgs a, int64_t nqb) {
 #pragma unroll
         for (int j = 0; j < 3; ++j)
           dtdF[i * 3 + j] = T[i * 3 + 0] * BM[j * 3 + 0] + T[i * 3 + 1] * BM[j * 3 + 1] + T[i * 3 + 2] * BM[j * 3 + 2];
-      const double co = -a.dt * p.dgdt[al];
-      const double sg = ((p.gdot[al] < 0) ? -1.0 : 1.0) * p.dgdt[al];
+      double tt = 0.0;
+#pragma unroll
+      for (int i = 0; i < 9; ++i) tt += p.Sig[i] * gr.P[al][i];
+      const double rr = tt / x[10];
+      const double pw = pow(fabs(rr), expo);
+      const double dgdt = cm.gd0 / (cm.m * x[10]) * pw;
+      const double co = -a.dt * dgdt;
+      const double sg = ((cm.gd0 * rr * pw < 0) ? -1.0 : 1.0) * dgdt;
 #pragma unroll
       for (int c = 0; c < 9; ++c) {
 #pragma unroll

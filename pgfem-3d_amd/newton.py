@@ -27,6 +27,7 @@ class StepResult:
     norms: list = field(default_factory=list)  # normalised residual norm per iteration
     info: int = 0                              # element-pass INFO of the failing iteration
     solver_iters: list = field(default_factory=list)
+    cuts: int = 0                              # subdivisions used (increment())
 
 
 class NewtonDriver:
@@ -58,6 +59,34 @@ class NewtonDriver:
         K = sp.csr_matrix((self.vals.cpu().numpy(), self.Ai, self.Ap), shape=(self.h.ndof, self.h.ndof))
         x = spla.spsolve(K.tocsc(), R.cpu().numpy())
         return torch.from_numpy(np.ascontiguousarray(x)).to(R.device), 0
+
+    def increment(self, r, defl_from, defl_to, dt=1.0, max_cuts=4) -> StepResult:
+        """Take the prescribed values from defl_from (converged) to defl_to, subdividing on
+        failure as the reference does (a failing step is reset with pgf_reset_step and the
+        remaining increment is applied in halves, up to max_cuts levels; the time step is
+        split with it)."""
+        out = StepResult(False, 0)
+        pieces = [(0.0, 1.0, 0)]
+        r_conv = r.clone()
+        while pieces:
+            a, b, lvl = pieces.pop(0)
+            defl = defl_from + b * (defl_to - defl_from)
+            res = self.load_step(r, defl, dt=dt * (b - a))
+            out.iterations += res.iterations
+            out.norms += res.norms
+            if res.converged:
+                r_conv.copy_(r)
+                continue
+            self.h.reset_step()
+            r.copy_(r_conv)
+            if lvl >= max_cuts:
+                out.info = res.info
+                return out
+            out.cuts += 1
+            m = 0.5 * (a + b)
+            pieces = [(a, m, lvl + 1), (m, b, lvl + 1)] + pieces
+        out.converged = True
+        return out
 
     def load_step(self, r, defl, d_r=None, defl_d=None, dt=1.0) -> StepResult:
         """Solve one load step in place: r (free DOFs, device float64) enters as the converged

@@ -1,7 +1,8 @@
 """Load steps solved end to end: the Newton driver over the device element path and the device
-linear solve against the same Newton over the CPU oracle with a sparse direct solve.  The
-nodal displacements must agree within 1e-9 relative after every load step (both Newton loops
-run to a 1e-12 normalised residual)."""
+linear solve against the same Newton over the CPU oracle with a sparse direct solve, both
+started from the same homogeneous predictor.  The nodal displacements must agree within 1e-9
+relative after every load step (both Newton loops run to a 1e-12 normalised residual); the
+driver's subdivision of a failing step is exercised separately."""
 import numpy as np
 import pytest
 import torch
@@ -34,7 +35,9 @@ def _run(prob, device, stretches, solver="device", cp=False):
     ost = dict(NORM=0.0, first=True)
     iters = []
     for s in stretches:
-        _, defl = prob.displacement(s, 0.0, seed=0)
+        guess, defl = prob.displacement(s, 0.0, seed=0)  # homogeneous predictor for both loops
+        r_gpu.copy_(_dev(guess, device))
+        r_cpu = guess.copy()
         res = drv.load_step(r_gpu, _dev(defl, device))
         assert res.converged, (s, res.norms, res.info)
         r_cpu, it_cpu = oracle_newton_step(prob, st, r_cpu, defl, ost, tol=1e-12, cp_tables=tables)
@@ -66,3 +69,19 @@ def test_host_direct_solve_option(device):
 def test_crystal_plasticity_steps(device):
     prob = CPBoxProblem(3, P.PGF_TET4, P.fcc_crystal(), n_grains=4, seed=2)
     _run(prob, device, [1e-3, 2e-3, 3e-3], cp=True)
+
+
+def test_subdivision_recovers_a_large_step(device):
+    """A crystal-plasticity step from rest with the whole increment applied at once fails in
+    the local Newton; the driver cuts it into halves until every piece converges."""
+    prob = CPBoxProblem(3, P.PGF_TET4, P.fcc_crystal(), n_grains=4, seed=2)
+    h = P.Handle(prob.mesh, prob.ids, prob.ndof, prob.Ap, prob.Ai, prob.mats)
+    h.set_orientations(prob.R, prob.grain)
+    drv = NW.NewtonDriver(h, prob.Ap, prob.Ai, tol=1e-10)
+    r = torch.zeros(prob.ndof, dtype=torch.float64, device=device)
+    _, d0 = prob.displacement(0.0, 0.0, seed=0)
+    _, d1 = prob.displacement(2e-3, 0.0, seed=0)
+    res = drv.increment(r, _dev(d0, device), _dev(d1, device), max_cuts=6)
+    assert res.converged, res
+    assert res.cuts >= 1
+    h.close()
