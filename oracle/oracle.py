@@ -346,3 +346,35 @@ def assemble_coh(mesh, props_list, coel_mat, ids, ndof, Ap, Ai, st: CohState, r,
                              ndof, _p(Ap), _p(Ai), _p(r), _p(d_r), _p(defl), _p(defl_d), _p(st.vars_n),
                              _p(st.vars_np1), _p(f), _p(vals))
     return err
+
+
+def line_search_gamma(gam, phi, prev=None):
+    """Next step length of the global Newton line search, LINE_S3_MP (src/LINE.cc:58-250).
+
+    phi is ||R(u + gam du)||^2 / ||R(u)||^2, the ratio LS2/LS1 of the reference (LS scaled by
+    LS1, so phi(0) = 1 and the slope of the Newton direction is -2).  The first cut minimises
+    the quadratic through phi(0), phi'(0), phi(gam = 1) (LINE.cc:101-102); later cuts minimise
+    the cubic through the last two trials (LINE.cc:104-116), capped at half the current step;
+    a proposal below a tenth of the current step is replaced by a tenth (LINE.cc:120-124).
+    prev = (gam, phi) of the trial before."""
+    slope = -2.0
+    if prev is None:
+        lam = -slope / (2.0 * (phi - 1.0 - slope))
+    else:
+        g2, f2 = prev
+        r1 = phi - 1.0 - gam * slope
+        r2 = f2 - 1.0 - g2 * slope
+        a = (r1 / (gam * gam) - r2 / (g2 * g2)) / (gam - g2)
+        b = (-g2 * r1 / (gam * gam) + gam * r2 / (g2 * g2)) / (gam - g2)
+        if a == 0.0:
+            lam = -slope / (2.0 * b)
+        else:
+            disc = b * b - 3.0 * a * slope
+            if disc < 0.0:
+                lam = 0.5 * gam
+            elif b <= 0.0:
+                lam = (-b + np.sqrt(disc)) / (3.0 * a)
+            else:
+                lam = -slope / (b + np.sqrt(disc))
+        lam = min(lam, 0.5 * gam)
+    return lam if lam >= 0.1 * gam else 0.1 * gam

@@ -28,23 +28,55 @@ class StepResult:
     info: int = 0                              # element-pass INFO of the failing iteration
     solver_iters: list = field(default_factory=list)
     cuts: int = 0                              # subdivisions used (increment())
+    gammas: list = field(default_factory=list)  # step length taken per iteration
+
+
+def backtrack_gamma(gam, phi, prev=None):
+    """Step length for the next trial of the line search (LINE.cc:99-124).  phi is
+    ||R(u + gam du)||^2 / ||R(u)||^2 at the rejected trial gam (so phi(0) = 1 and the Newton
+    direction's slope is -2); prev = (gam, phi) of the trial before, or None after the full
+    step.  The first cut is the minimiser of the quadratic through phi(0), phi'(0) and phi(1);
+    later cuts the minimiser of the cubic through phi(0), phi'(0) and the last two trials,
+    capped at half the current step.  No cut goes below a tenth of the current step."""
+    if prev is None:
+        lam = 1.0 / (phi + 1.0)  # -phi'(0) / (2 (phi(1) - phi(0) - phi'(0)))
+    else:
+        g1, p1 = prev
+        e0 = (phi - 1.0 + 2.0 * gam) / (gam * gam)  # excess over the linear model, / gam^2
+        e1 = (p1 - 1.0 + 2.0 * g1) / (g1 * g1)
+        a = (e0 - e1) / (gam - g1)  # phi(x) = 1 - 2 x + b x^2 + a x^3
+        b = (gam * e1 - g1 * e0) / (gam - g1)
+        if a == 0.0:
+            lam = 1.0 / b
+        else:
+            disc = b * b + 6.0 * a
+            if disc < 0.0:
+                lam = 0.5 * gam
+            elif b <= 0.0:
+                lam = (np.sqrt(disc) - b) / (3.0 * a)
+            else:
+                lam = 2.0 / (b + np.sqrt(disc))  # same root, without cancellation
+        lam = min(lam, 0.5 * gam)
+    return max(lam, 0.1 * gam)
 
 
 class NewtonDriver:
     def __init__(self, h: P.Handle, Ap, Ai, tol=1e-10, max_iter=25, solver="device", lin_tol=1e-13,
-                 lin_maxit=20000, computer_zero=1e-15):
+                 lin_maxit=20000, computer_zero=1e-15, line_search=True):
         import torch
 
         self.h, self.tol, self.max_iter = h, tol, max_iter
         self.solver, self.lin_tol, self.lin_maxit = solver, lin_tol, lin_maxit
-        self.computer_zero = computer_zero
+        self.computer_zero, self.line_search = computer_zero, line_search
         self.Ap, self.Ai = Ap, Ai
         dev = torch.device("cuda:0")
         self.f = torch.empty(h.ndof, dtype=torch.float64, device=dev)
         self.vals = torch.empty(h.nnz, dtype=torch.float64, device=dev)
         self.du = torch.zeros(h.ndof, dtype=torch.float64, device=dev)
+        self.rt = torch.empty(h.ndof, dtype=torch.float64, device=dev)  # line-search trial point
         self.NORM = 0.0
         self.first = True
+        self.last_relres = 0.0
 
     def _solve(self, R):
         import torch
@@ -52,6 +84,7 @@ class NewtonDriver:
         if self.solver == "device":
             self.du.zero_()
             it, rr = self.h.solve(self.vals, R, self.du, tol=self.lin_tol, maxit=self.lin_maxit)
+            self.last_relres = rr
             return self.du, it
         import scipy.sparse as sp
         import scipy.sparse.linalg as spla
@@ -91,9 +124,15 @@ class NewtonDriver:
     def load_step(self, r, defl, d_r=None, defl_d=None, dt=1.0) -> StepResult:
         """Solve one load step in place: r (free DOFs, device float64) enters as the converged
         u(n) and leaves as u(n+1); defl holds the prescribed values of step n+1."""
+        import torch
+
         res = StepResult(False, 0)
+
+        def run(x):
+            return self.h.residual_tangent(self.f, self.vals, x, d_r=d_r, defl=defl, defl_d=defl_d, dt=dt)
+
+        info = run(r)
         for it in range(self.max_iter):
-            info = self.h.residual_tangent(self.f, self.vals, r, d_r=d_r, defl=defl, defl_d=defl_d, dt=dt)
             if info:
                 res.info = info
                 return res
@@ -112,6 +151,25 @@ class NewtonDriver:
                 self.h.commit_step()
                 return res
             du, lit = self._solve(R)
-            res.solver_iters.append(lit)
-            r += du
+            res.solver_iters.append((lit, self.last_relres))
+            gam, prev = 1.0, None
+            while True:
+                torch.add(r, du, alpha=gam, out=self.rt)
+                info = run(self.rt)
+                if info or it == 0 or not self.line_search:
+                    break
+                phi = float(self.f.norm()) ** 2 / (nor2 * nor2)
+                if not np.isfinite(phi):
+                    return res
+                if phi <= 1.0 - 2e-4 * gam:  # sufficient decrease (LINE.cc:96)
+                    break
+                gam, prev = backtrack_gamma(gam, phi, prev), (gam, phi)
+                if gam < 0.1 * self.tol:  # the line search failed (LINE.cc:126-127)
+                    if nor < 5.0 * self.tol:  # soft convergence: keep u (Newton_Raphson.cc:1246)
+                        if run(r) == 0:
+                            res.converged = True
+                            self.h.commit_step()
+                    return res
+            res.gammas.append(gam)
+            r.copy_(self.rt)
         return res

@@ -68,20 +68,28 @@ class InterfaceBoxProblem(BoxProblem):
                               defl=defl)
 
 
-def oracle_newton_step(prob, st, r, defl, state, tol=1e-12, max_iter=25, cp_tables=None, dt=1.0):
+def oracle_newton_step(prob, st, r, defl, state, tol=1e-12, max_iter=25, cp_tables=None, dt=1.0,
+                       line_search=True):
     """The Newton load step of pgfem-3d_amd/newton.py restated over the CPU oracle with a
-    sparse direct solve: returns (r_np1, iterations).  state = dict(NORM=..., first=...)."""
+    sparse direct solve: returns (r_np1, iterations, step lengths).  From the second iteration
+    on, a step whose residual does not fall enough is cut back by the rule of LINE_S3_MP
+    (src/LINE.cc:58-250, restated in oracle.line_search_gamma).  state = dict(NORM=..., first=...)."""
     import scipy.sparse as sp
     import scipy.sparse.linalg as spla
 
-    r = r.copy()
-    for it in range(max_iter):
+    def ev(x):
         if cp_tables is None:
-            f, vals, err = O.assemble(prob.mesh, prob.mats, prob.ids, prob.ndof, prob.Ap, prob.Ai, st, r, defl=defl)
+            out = O.assemble(prob.mesh, prob.mats, prob.ids, prob.ndof, prob.Ap, prob.Ai, st, x, defl=defl)
         else:
-            f, vals, err = O.assemble_cp(prob.mesh, prob.mats, prob.grain, cp_tables[0], cp_tables[1], prob.ids,
-                                         prob.ndof, prob.Ap, prob.Ai, st, r, defl=defl, dt=dt)
-        assert err == 0
+            out = O.assemble_cp(prob.mesh, prob.mats, prob.grain, cp_tables[0], cp_tables[1], prob.ids,
+                                prob.ndof, prob.Ap, prob.Ai, st, x, defl=defl, dt=dt)
+        assert out[2] == 0
+        return out[0], out[1]
+
+    r = r.copy()
+    gams = []
+    f, vals = ev(r)
+    for it in range(max_iter):
         R = -f
         nor2 = float(np.linalg.norm(R))
         if state["first"] or state["NORM"] < tol:
@@ -89,9 +97,21 @@ def oracle_newton_step(prob, st, r, defl, state, tol=1e-12, max_iter=25, cp_tabl
             state["first"] = False
         if nor2 / state["NORM"] <= tol or nor2 <= 1e-15:
             st.commit()
-            return r, it
+            return r, it, gams
         K = sp.csr_matrix((vals, prob.Ai, prob.Ap), shape=(prob.ndof, prob.ndof))
-        r = r + spla.spsolve(K.tocsc(), R)
+        du = spla.spsolve(K.tocsc(), R)
+        gam, prev = 1.0, None
+        while True:
+            f, vals = ev(r + gam * du)
+            if it == 0 or not line_search:
+                break
+            phi = float(np.linalg.norm(f)) ** 2 / nor2 ** 2
+            if phi <= 1.0 - 2e-4 * gam:
+                break
+            gam, prev = O.line_search_gamma(gam, phi, prev), (gam, phi)
+            assert gam >= 0.1 * tol, "oracle line search failed"
+        gams.append(gam)
+        r = r + gam * du
     raise AssertionError("oracle Newton did not converge")
 
 
