@@ -166,13 +166,22 @@ def run_j2(args, dev):
     f = torch.empty(prob.ndof, dtype=torch.float64, device=dev)
     vals = torch.empty(h.nnz, dtype=torch.float64, device=dev)
     h.set_profiling(True)
+    modes = {"gather": P.ASSEMBLY_GATHER, "coloured": P.ASSEMBLY_COLOURED}
+    h.set_assembly(modes[args.assembly])
     s_step, kern_ms, n_launch = _timed_passes(h, f, vals, rd, dd, args.steps, args.warmup)
     nbytes = algorithmic_bytes(h.n_qp, h.nnz, prob.ndof, prob.mesh.n_nodes, prob.mesh.n_elem, 10)
     res = dict(n_qp=h.n_qp, nnz=h.nnz, ndof=prob.ndof, n_elem=prob.mesh.n_elem, s_step=s_step, kern_ms=kern_ms,
                n_launch=n_launch, achieved=nbytes / (kern_ms * 1e-3) / 1e9,
                tflops=flops_per_qp(10) * h.n_qp / (kern_ms * 1e-3) / 1e12)
-    print(f"[bench] J2: {1e3 * s_step:.3f} ms/step wall, element kernels {kern_ms:.3f} ms/pass over {n_launch} "
-          f"launches, {res['achieved']:.0f} GB/s algorithmic, {res['tflops']:.2f} TFLOP/s", file=sys.stderr, flush=True)
+    print(f"[bench] J2 ({args.assembly} assembly): {1e3 * s_step:.3f} ms/step wall, element pass {kern_ms:.3f} ms "
+          f"over {n_launch} launches, {res['achieved']:.0f} GB/s algorithmic, {res['tflops']:.2f} TFLOP/s",
+          file=sys.stderr, flush=True)
+    other = "coloured" if args.assembly == "gather" else "gather"
+    h.set_assembly(modes[other])
+    s2, k2, n2 = _timed_passes(h, f, vals, rd, dd, max(3, args.steps // 4), 1)
+    res["other"] = {"assembly": other, "ms_per_step": 1e3 * s2, "kernel_ms_per_pass": k2, "launches_per_pass": n2}
+    print(f"[bench] J2 ({other} assembly): {1e3 * s2:.3f} ms/step wall, element pass {k2:.3f} ms over {n2} "
+          "launches", file=sys.stderr, flush=True)
     h.close()
     return res
 
@@ -315,6 +324,8 @@ def main():
     ap.add_argument("--mix", type=int, nargs=3, default=[41, 32, 46],
                     help="cells per edge of the linear-tet, quadratic-tet and hex boxes of the mixed batch")
     ap.add_argument("--sur-epochs", type=int, default=3)
+    ap.add_argument("--assembly", choices=["gather", "coloured"], default="gather",
+                    help="assembly mode of the timed passes (the other mode is timed for comparison)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (multi-GPU sharding is a "
@@ -370,6 +381,8 @@ def main():
                  "frac": j2["tflops"] / FP64_PEAK_TFLOPS},
         "kernel_ms_per_pass": j2["kern_ms"],
         "launches_per_pass": j2["n_launch"],
+        "assembly": args.assembly,
+        "other_assembly": j2["other"],
         "cpu_baseline": cpu,
     }
     if cp is not None:

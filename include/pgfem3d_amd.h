@@ -229,6 +229,21 @@ int pgf_set_graph(pgf_handle h, int on);
  * timing integrating passes at the given input; *chunk = the chosen value. */
 int pgf_autotune(pgf_handle h, const pgf_nodal_input *in, double dt, int64_t *chunk);
 
+/* Assembly of element rows into f and the CSR values (the role of fd_res_assemble and
+ * PLoc_Sparse, src/fd_residuals.cc:58-73, src/PLoc_Sparse.cc:15-175).  Both modes are
+ * deterministic, use no atomics, and give the same sums up to the order of the additions:
+ *   PGF_ASSEMBLY_GATHER (default): every element writes its rows once into a node-major
+ *     staging buffer (3 (3 nne + 1) doubles per element node, allocated at the first such
+ *     pass); then one wavefront per node adds its elements' blocks in a fixed order and
+ *     writes the node's CSR rows and residual entries once, contiguously.
+ *   PGF_ASSEMBLY_COLOURED: elements of one colour share no node and add their rows into
+ *     the CSR values directly, colour by colour (no staging memory).
+ * The gather mode needs 3 * max row length * 8 bytes of LDS per node; a mesh whose rows are
+ * too long for it gets PGF_ENOTSUP here and is created in the coloured mode. */
+#define PGF_ASSEMBLY_GATHER 0
+#define PGF_ASSEMBLY_COLOURED 1
+int pgf_set_assembly(pgf_handle h, int32_t mode);
+
 /* ======================================================================= state */
 /* After convergence: n+1 -> n, n -> n-1 for every QP (constitutive_model_update_time_steps,
  * src/constitutive_model.cc:1442-1497; CM_J2P_PARAM::update_state_vars,

@@ -1,4 +1,5 @@
-// C ABI of the element hot path: device handle, schedule of coloured launches, state.
+// C ABI of the element hot path: device handle, the pass schedule (gather or coloured
+// assembly), state.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -24,9 +25,10 @@ struct Block {
   int nne = 0, nqp = 0, model = 0;
   int64_t ne = 0, qp_base = 0;
   std::vector<int64_t> elems;      // mesh element ids in device order
-  std::vector<int64_t> color_ptr;  // [n_slabs*n_colors+1] (slab, colour) segments of elems
+  std::vector<int64_t> color_ptr;  // [n_colors+1] colour segments of elems
   int32_t *d_conn = nullptr;
   int16_t *d_off = nullptr;
+  int64_t *d_soff = nullptr;       // gather mode: [ne*nne] staging offsets
   int32_t *d_mat = nullptr;
   int32_t *d_egrain = nullptr;  // crystal plasticity: grain per element / per QP (device order)
   int32_t *d_qgrain = nullptr;
@@ -40,6 +42,7 @@ struct CohBlock {
   std::vector<int64_t> color_ptr;  // [n_ccolors+1]
   int32_t *d_conn = nullptr, *d_mat = nullptr;
   int16_t *d_off = nullptr;
+  int64_t *d_soff = nullptr;
 };
 
 template <class T>
@@ -146,7 +149,7 @@ struct pgf_handle_s {
   int device = 0;
   hipStream_t own_stream = nullptr, stream = nullptr;
   int64_t n_nodes = 0, ndof = 0, nnz = 0, n_qp = 0;
-  int32_t n_colors = 0, n_slabs = 1, n_segs = 0;
+  int32_t n_colors = 0;
   bool has_j2 = false;
   double *d_X = nullptr;
   int32_t *d_dofid = nullptr, *d_row_start = nullptr, *d_rowlen = nullptr;
@@ -192,6 +195,14 @@ struct pgf_handle_s {
   std::vector<int64_t> cqp_mesh;  // device interface QP -> mesh interface QP
   double *d_cv[3] = {nullptr, nullptr, nullptr};  // [2][n_cqp] (X_eff_max, t_max): n+1, n, n-1
   pgf::DevCohesive *d_cprops = nullptr;
+  // gather assembly (PGF_ASSEMBLY_GATHER): per-node staging blocks, see pgf::GatherArgs
+  int32_t assembly = PGF_ASSEMBLY_COLOURED;
+  int32_t maxlen = 0;
+  int64_t stage_size = 0;  // doubles
+  int64_t *d_inc_ptr = nullptr, *d_inc_soff = nullptr, *d_inc_poff = nullptr;
+  int32_t *d_inc_nne = nullptr;
+  uint32_t *d_pos = nullptr;
+  double *d_stage = nullptr;
   bool profiling = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_ms = 0.0;
@@ -320,7 +331,75 @@ pgf::ElemArgs base_args(pgf_handle h, const pgf_nodal_input *in) {
   return a;
 }
 
-// One pass over all elements in the coloured schedule.
+// Gather tables (pgf::GatherArgs) of the element groups -- the volume blocks, then the
+// interface blocks, each in device order.  Every (element, node) pair whose node has a free
+// DOF gets a staging block; a node's blocks are contiguous and ordered by (group, element,
+// local node), which fixes the order of every sum.  off holds the groups' first-touch-marked
+// node-pair offsets; soff receives each group's [n*nne] staging offsets (-1: no block).
+struct GatherGroup {
+  int nne;
+  int64_t ne;
+  const int32_t *conn;  // [ne*nne] device order
+  const int16_t *off;   // [ne*nne*nne]
+  std::vector<int64_t> soff;
+};
+
+void build_gather(int64_t n_nodes, const int32_t *dofid, std::vector<GatherGroup> &groups,
+                  std::vector<int64_t> &inc_ptr, std::vector<int64_t> &inc_soff,
+                  std::vector<int64_t> &inc_poff, std::vector<int32_t> &inc_nne, std::vector<uint32_t> &pos,
+                  int64_t &stage_size) {
+  auto fmask = [&](int64_t n) {
+    return (dofid[n * 3] > 0 ? 1 : 0) | (dofid[n * 3 + 1] > 0 ? 2 : 0) | (dofid[n * 3 + 2] > 0 ? 4 : 0);
+  };
+  inc_ptr.assign(n_nodes + 1, 0);
+  for (auto &g : groups)
+    for (int64_t i = 0; i < g.ne * g.nne; ++i)
+      if (fmask(g.conn[i])) inc_ptr[g.conn[i] + 1]++;
+  for (int64_t n = 0; n < n_nodes; ++n) inc_ptr[n + 1] += inc_ptr[n];
+  const int64_t n_inc = inc_ptr[n_nodes];
+  std::vector<int32_t> s_group(n_inc);
+  std::vector<int64_t> s_pair(n_inc);  // k*nne + a of the slot's element node
+  std::vector<int64_t> cursor(inc_ptr.begin(), inc_ptr.end() - 1);
+  inc_nne.resize(n_inc);
+  for (size_t gi = 0; gi < groups.size(); ++gi) {
+    GatherGroup &g = groups[gi];
+    g.soff.assign(g.ne * g.nne, -1);
+    for (int64_t i = 0; i < g.ne * g.nne; ++i) {
+      const int32_t node = g.conn[i];
+      if (!fmask(node)) continue;
+      const int64_t sl = cursor[node]++;
+      s_group[sl] = int32_t(gi);
+      s_pair[sl] = i;
+      inc_nne[sl] = g.nne;
+    }
+  }
+  inc_soff.resize(n_inc);
+  inc_poff.resize(n_inc);
+  int64_t so = 0, po = 0;
+  for (int64_t sl = 0; sl < n_inc; ++sl) {
+    inc_soff[sl] = so;
+    inc_poff[sl] = po;
+    so += 3 * (3 * int64_t(inc_nne[sl]) + 1);
+    po += inc_nne[sl];
+  }
+  stage_size = so;
+  pos.assign(po, 0u);
+#pragma omp parallel for schedule(static)
+  for (int64_t sl = 0; sl < n_inc; ++sl) {
+    GatherGroup &g = groups[s_group[sl]];
+    const int64_t i = s_pair[sl], k = i / g.nne;
+    g.soff[i] = inc_soff[sl];
+    const int16_t *orow = g.off + i * g.nne;
+    for (int w = 0; w < g.nne; ++w) {
+      const int o = orow[w];
+      if (o == -1) continue;
+      const uint32_t base = uint32_t(o >= 0 ? o : -o - 2);
+      pos[inc_poff[sl] + w] = base | (uint32_t(fmask(g.conn[k * g.nne + w])) << 28);
+    }
+  }
+}
+
+// One pass over all elements in the pass schedule.
 // Every launch of one pass, in order, on the handle's stream (also what a captured graph
 // holds; the error counter is cleared by a kernel, not a memset node).
 int32_t enqueue_pass(pgf_handle h, const pgf_nodal_input *in, int mode, double *f, double *vals) {
@@ -330,9 +409,21 @@ int32_t enqueue_pass(pgf_handle h, const pgf_nodal_input *in, int mode, double *
   a.vals = vals;
   int32_t launches = 0;
   cp_prepass(h, in, mode);
-  for (int32_t c = 0; c < h->n_segs; ++c)  // (slab, colour) segments, slab-major
+  // gather mode: each block in one launch (all elements), then one gather over the nodes;
+  // coloured mode: colour by colour, every block's elements of the colour
+  const bool gather = h->assembly == PGF_ASSEMBLY_GATHER;
+  double *fout = f, *vout = vals;
+  if (gather) {
+    a.f = nullptr;
+    a.vals = nullptr;
+    a.stage = h->d_stage;
+    f = nullptr;
+    vals = nullptr;
+  }
+  const int32_t nseg = gather ? 1 : h->n_colors;
+  for (int32_t c = 0; c < nseg; ++c)
     for (auto &b : h->blocks) {
-      const int64_t e0 = b.color_ptr[c], e1 = b.color_ptr[c + 1];
+      const int64_t e0 = gather ? 0 : b.color_ptr[c], e1 = gather ? b.ne : b.color_ptr[c + 1];
       if (e1 <= e0) continue;
       if (b.model == PGF_MODEL_CRYSTAL_PLASTICITY) {
         pgf::CPElemArgs ce{};
@@ -352,11 +443,14 @@ int32_t enqueue_pass(pgf_handle h, const pgf_nodal_input *in, int mode, double *
         ce.defl_d = in->defl_d;
         ce.f = f;
         ce.vals = vals;
+        ce.soff = b.d_soff;
+        ce.stage = gather ? h->d_stage : nullptr;
         ce.err = h->d_err;
         pgf::launch_cp_elem(b.nne, mode, ce, e0, e1, h->stream);
       } else {
         a.conn = b.d_conn;
         a.off = b.d_off;
+        a.soff = b.d_soff;
         a.mat = b.d_mat;
         a.qp_base = b.qp_base;
         pgf::launch_elem(b.nne, mode, a, e0, e1, h->stream);
@@ -379,18 +473,39 @@ int32_t enqueue_pass(pgf_handle h, const pgf_nodal_input *in, int mode, double *
     ca.vars_np1 = h->d_cv[0];
     ca.f = f;
     ca.vals = vals;
+    ca.stage = gather ? h->d_stage : nullptr;
+    const int32_t ncseg = gather ? 1 : h->n_ccolors;
     ca.err = h->d_err;
-    for (int32_t c = 0; c < h->n_ccolors; ++c)
+    for (int32_t c = 0; c < ncseg; ++c)
       for (auto &cb : h->cblocks) {
-        const int64_t e0 = cb.color_ptr[c], e1 = cb.color_ptr[c + 1];
+        const int64_t e0 = gather ? 0 : cb.color_ptr[c], e1 = gather ? cb.nc : cb.color_ptr[c + 1];
         if (e1 <= e0) continue;
         ca.conn = cb.d_conn;
         ca.mat = cb.d_mat;
         ca.off = cb.d_off;
+        ca.soff = cb.d_soff;
         ca.cqp_base = cb.cqp_base;
         pgf::launch_coh_elem(cb.nn, mode, ca, e0, e1, h->stream);
         ++launches;
       }
+  }
+  if (gather) {
+    pgf::GatherArgs g{};
+    g.inc_ptr = h->d_inc_ptr;
+    g.inc_soff = h->d_inc_soff;
+    g.inc_poff = h->d_inc_poff;
+    g.inc_nne = h->d_inc_nne;
+    g.pos = h->d_pos;
+    g.dofid = h->d_dofid;
+    g.row_start = h->d_row_start;
+    g.rowlen = h->d_rowlen;
+    g.stage = h->d_stage;
+    g.f = fout;
+    g.vals = vout;
+    g.n_nodes = h->n_nodes;
+    g.maxlen = h->maxlen;
+    pgf::launch_gather(mode, g, h->stream);
+    ++launches;
   }
   return launches;
 }
@@ -398,6 +513,10 @@ int32_t enqueue_pass(pgf_handle h, const pgf_nodal_input *in, int mode, double *
 int run_pass(pgf_handle h, const pgf_nodal_input *in, int mode, double *f, double *vals) {
   if (!in || !in->r) return PGF_EINVAL;
   if (h->n_coel && !h->d_cprops) return PGF_EINVAL;  // interface properties not set
+  if (h->assembly == PGF_ASSEMBLY_GATHER && !h->d_stage && h->stage_size) {
+    const int rc = pgf::dalloc(&h->d_stage, size_t(h->stage_size));
+    if (rc) return rc;
+  }
   const bool graph = h->use_graph && !h->sur;  // the learned start may size buffers: eager
   GraphKey key{mode, in->r, in->d_r, in->defl, in->defl_d, f, vals, h->warm, h->dt, h->cp_chunk};
   if (graph && !(h->gexec && key == h->gkey)) {
@@ -577,35 +696,6 @@ int pgf_create(const pgf_mesh *mesh, const int64_t *id, int64_t ndof, const int3
   });
   std::vector<int> block_of_key(keys.size());
   for (size_t i = 0; i < keys.size(); ++i) block_of_key[border[i]] = int(i);
-  // slab-major schedule: elements are split into n_slabs equal-count slabs along the mesh's
-  // longest axis (by centroid) and passes launch (slab, colour) segments in that order, so the
-  // CSR rows a slab's colours revisit stay cache-resident between its launches
-  int n_slabs = int(std::min<int64_t>(8, std::max<int64_t>(1, (int64_t(Ap[ndof]) * 8) / (int64_t(512) << 20))));
-  if (const char *env = std::getenv("PGF_ASSEMBLY_SLABS")) n_slabs = std::max(1, std::min(64, std::atoi(env)));
-  std::vector<int32_t> slab(m.n_elem, 0);
-  if (n_slabs > 1 && m.n_elem > 0) {
-    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-    for (int64_t n = 0; n < m.n_nodes; ++n)
-      for (int d = 0; d < 3; ++d) {
-        lo[d] = std::min(lo[d], m.X[n * 3 + d]);
-        hi[d] = std::max(hi[d], m.X[n * 3 + d]);
-      }
-    int ax = 0;
-    for (int d = 1; d < 3; ++d)
-      if (hi[d] - lo[d] > hi[ax] - lo[ax]) ax = d;
-    std::vector<double> key(m.n_elem);
-    for (int64_t e = 0; e < m.n_elem; ++e) {
-      double c = 0.0;
-      for (int64_t k = m.elem_ptr[e]; k < m.elem_ptr[e + 1]; ++k) c += m.X[m.elem_conn[k] * 3 + ax];
-      key[e] = c / double(m.elem_ptr[e + 1] - m.elem_ptr[e]);
-    }
-    std::vector<int64_t> rank(m.n_elem);
-    for (int64_t e = 0; e < m.n_elem; ++e) rank[e] = e;
-    std::stable_sort(rank.begin(), rank.end(), [&](int64_t x, int64_t y) { return key[x] < key[y]; });
-    for (int64_t i = 0; i < m.n_elem; ++i) slab[rank[i]] = int32_t((i * n_slabs) / m.n_elem);
-  }
-  h->n_slabs = n_slabs;
-  h->n_segs = n_slabs * ncol;
   h->blocks.resize(keys.size());
   int64_t qp = 0;
   for (size_t bi = 0; bi < keys.size(); ++bi) {
@@ -615,12 +705,11 @@ int pgf_create(const pgf_mesh *mesh, const int64_t *id, int64_t ndof, const int3
     b.nqp = pgf::nqp_of(b.nne);
     for (int64_t e = 0; e < m.n_elem; ++e)
       if (block_of_key[elem_block[e]] == int(bi)) b.elems.push_back(e);
-    auto seg = [&](int64_t e) { return int64_t(slab[e]) * ncol + color[e]; };
-    std::stable_sort(b.elems.begin(), b.elems.end(), [&](int64_t x, int64_t y) { return seg(x) < seg(y); });
+    std::stable_sort(b.elems.begin(), b.elems.end(), [&](int64_t x, int64_t y) { return color[x] < color[y]; });
     b.ne = int64_t(b.elems.size());
-    b.color_ptr.assign(h->n_segs + 1, 0);
-    for (int64_t e : b.elems) b.color_ptr[seg(e) + 1]++;
-    for (int c = 0; c < h->n_segs; ++c) b.color_ptr[c + 1] += b.color_ptr[c];
+    b.color_ptr.assign(ncol + 1, 0);
+    for (int64_t e : b.elems) b.color_ptr[color[e] + 1]++;
+    for (int c = 0; c < ncol; ++c) b.color_ptr[c + 1] += b.color_ptr[c];
     b.qp_base = qp;
     qp += b.ne * b.nqp;
   }
@@ -642,7 +731,7 @@ int pgf_create(const pgf_mesh *mesh, const int64_t *id, int64_t ndof, const int3
   std::vector<uint8_t> touched(size_t(h->nnz), 0);
   std::vector<std::vector<int16_t>> boff(h->blocks.size());
   for (size_t bi = 0; bi < h->blocks.size(); ++bi) boff[bi].resize(h->blocks[bi].ne * h->blocks[bi].nne * h->blocks[bi].nne);
-  for (int32_t c = 0; c < h->n_segs; ++c)
+  for (int32_t c = 0; c < ncol; ++c)
     for (size_t bi = 0; bi < h->blocks.size(); ++bi) {
       Block &b = h->blocks[bi];
       const int nne = b.nne;
@@ -722,17 +811,47 @@ int pgf_create(const pgf_mesh *mesh, const int64_t *id, int64_t ndof, const int3
         }
       }
   }
-  // --- upload
-  int rc = PGF_OK;
+  // --- device-order connectivity and the gather tables
+  std::vector<std::vector<int32_t>> bconn(h->blocks.size()), cconn(h->cblocks.size());
+  for (size_t bi = 0; bi < h->blocks.size(); ++bi) {
+    Block &b = h->blocks[bi];
+    bconn[bi].resize(b.ne * b.nne);
+    for (int64_t k = 0; k < b.ne; ++k)
+      for (int a = 0; a < b.nne; ++a) bconn[bi][k * b.nne + a] = int32_t(m.elem_conn[m.elem_ptr[b.elems[k]] + a]);
+  }
   for (size_t bi = 0; bi < h->cblocks.size(); ++bi) {
     pgf::CohBlock &cb = h->cblocks[bi];
     const int nt = 2 * cb.nn;
-    std::vector<int32_t> conn(cb.nc * nt), zero(cb.nc, 0);
+    cconn[bi].resize(cb.nc * nt);
     for (int64_t k = 0; k < cb.nc; ++k)
-      for (int a = 0; a < nt; ++a) conn[k * nt + a] = int32_t(m.coel_conn[m.coel_ptr[cb.elems[k]] + a]);
-    if ((rc = pgf::upload(&cb.d_conn, conn.data(), conn.size()))) return fail(rc);
+      for (int a = 0; a < nt; ++a) cconn[bi][k * nt + a] = int32_t(m.coel_conn[m.coel_ptr[cb.elems[k]] + a]);
+  }
+  std::vector<GatherGroup> groups;
+  for (size_t bi = 0; bi < h->blocks.size(); ++bi)
+    groups.push_back({h->blocks[bi].nne, h->blocks[bi].ne, bconn[bi].data(), boff[bi].data(), {}});
+  for (size_t bi = 0; bi < h->cblocks.size(); ++bi)
+    groups.push_back({2 * h->cblocks[bi].nn, h->cblocks[bi].nc, cconn[bi].data(), cboff[bi].data(), {}});
+  std::vector<int64_t> inc_ptr, inc_soff, inc_poff;
+  std::vector<int32_t> inc_nne;
+  std::vector<uint32_t> gpos;
+  build_gather(m.n_nodes, dofid.data(), groups, inc_ptr, inc_soff, inc_poff, inc_nne, gpos, h->stage_size);
+  for (int64_t n = 0; n < m.n_nodes; ++n) h->maxlen = std::max(h->maxlen, rowlen[n]);
+  h->assembly = pgf::gather_waves(h->maxlen) >= 1 ? PGF_ASSEMBLY_GATHER : PGF_ASSEMBLY_COLOURED;
+  // --- upload
+  int rc = PGF_OK;
+  if ((rc = pgf::upload(&h->d_inc_ptr, inc_ptr.data(), inc_ptr.size()))) return fail(rc);
+  if ((rc = pgf::upload(&h->d_inc_soff, inc_soff.data(), inc_soff.size()))) return fail(rc);
+  if ((rc = pgf::upload(&h->d_inc_poff, inc_poff.data(), inc_poff.size()))) return fail(rc);
+  if ((rc = pgf::upload(&h->d_inc_nne, inc_nne.data(), inc_nne.size()))) return fail(rc);
+  if ((rc = pgf::upload(&h->d_pos, gpos.data(), gpos.size()))) return fail(rc);
+  for (size_t bi = 0; bi < h->cblocks.size(); ++bi) {
+    pgf::CohBlock &cb = h->cblocks[bi];
+    const std::vector<int32_t> zero(cb.nc, 0);
+    const std::vector<int64_t> &soff = groups[h->blocks.size() + bi].soff;
+    if ((rc = pgf::upload(&cb.d_conn, cconn[bi].data(), cconn[bi].size()))) return fail(rc);
     if ((rc = pgf::upload(&cb.d_mat, zero.data(), zero.size()))) return fail(rc);
     if ((rc = pgf::upload(&cb.d_off, cboff[bi].data(), cboff[bi].size()))) return fail(rc);
+    if ((rc = pgf::upload(&cb.d_soff, soff.data(), soff.size()))) return fail(rc);
   }
   if (h->n_cqp) {
     for (int t = 0; t < 3; ++t) {
@@ -758,15 +877,12 @@ int pgf_create(const pgf_mesh *mesh, const int64_t *id, int64_t ndof, const int3
   if ((rc = pgf::upload(&h->d_Ai, Ai, size_t(h->nnz)))) return fail(rc);
   for (size_t bi = 0; bi < h->blocks.size(); ++bi) {
     Block &b = h->blocks[bi];
-    std::vector<int32_t> conn(b.ne * b.nne), mat(b.ne);
-    for (int64_t k = 0; k < b.ne; ++k) {
-      const int64_t e = b.elems[k];
-      for (int a = 0; a < b.nne; ++a) conn[k * b.nne + a] = int32_t(m.elem_conn[m.elem_ptr[e] + a]);
-      mat[k] = m.elem_mat ? m.elem_mat[e] : 0;
-    }
-    if ((rc = pgf::upload(&b.d_conn, conn.data(), conn.size()))) return fail(rc);
+    std::vector<int32_t> mat(b.ne);
+    for (int64_t k = 0; k < b.ne; ++k) mat[k] = m.elem_mat ? m.elem_mat[b.elems[k]] : 0;
+    if ((rc = pgf::upload(&b.d_conn, bconn[bi].data(), bconn[bi].size()))) return fail(rc);
     if ((rc = pgf::upload(&b.d_mat, mat.data(), mat.size()))) return fail(rc);
     if ((rc = pgf::upload(&b.d_off, boff[bi].data(), boff[bi].size()))) return fail(rc);
+    if ((rc = pgf::upload(&b.d_soff, groups[bi].soff.data(), groups[bi].soff.size()))) return fail(rc);
   }
   for (int t = 0; t < 3; ++t) {
     if ((rc = pgf::dalloc(&h->d_F[t], size_t(h->n_qp) * 9))) return fail(rc);
@@ -825,6 +941,7 @@ int pgf_destroy(pgf_handle h) {
   for (auto &b : h->blocks) {
     hipFree(b.d_conn);
     hipFree(b.d_off);
+    hipFree(b.d_soff);
     hipFree(b.d_mat);
     hipFree(b.d_egrain);
   }
@@ -832,7 +949,14 @@ int pgf_destroy(pgf_handle h) {
     hipFree(cb.d_conn);
     hipFree(cb.d_mat);
     hipFree(cb.d_off);
+    hipFree(cb.d_soff);
   }
+  hipFree(h->d_inc_ptr);
+  hipFree(h->d_inc_soff);
+  hipFree(h->d_inc_poff);
+  hipFree(h->d_inc_nne);
+  hipFree(h->d_pos);
+  hipFree(h->d_stage);
   for (int t = 0; t < 3; ++t) hipFree(h->d_cv[t]);
   if (h->gexec) hipGraphExecDestroy(h->gexec);
   hipFree(h->d_Ap);
@@ -1181,6 +1305,23 @@ int pgf_set_graph(pgf_handle h, int on) {
     hipGraphExecDestroy(h->gexec);
     h->gexec = nullptr;
   }
+  return PGF_OK;
+}
+
+int pgf_set_assembly(pgf_handle h, int32_t mode) {
+  if (!h || (mode != PGF_ASSEMBLY_GATHER && mode != PGF_ASSEMBLY_COLOURED)) return PGF_EINVAL;
+  if (mode == PGF_ASSEMBLY_GATHER && pgf::gather_waves(h->maxlen) < 1) return PGF_ENOTSUP;
+  if (mode == h->assembly) return PGF_OK;
+  if (hipStreamSynchronize(h->stream) != hipSuccess) return PGF_EHIP;
+  if (h->gexec) {  // a captured pass holds the old schedule
+    hipGraphExecDestroy(h->gexec);
+    h->gexec = nullptr;
+  }
+  if (mode == PGF_ASSEMBLY_COLOURED && h->d_stage) {  // release the staging memory
+    hipFree(h->d_stage);
+    h->d_stage = nullptr;
+  }
+  h->assembly = mode;
   return PGF_OK;
 }
 

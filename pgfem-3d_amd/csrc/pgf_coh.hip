@@ -15,9 +15,10 @@
 //
 // Work mapping: one 32-lane group per element, 8 elements per 256-thread workgroup.
 // Phase 0: lanes < 2*NN gather nodes.  Phase 1: lanes < NN compute one QP each into LDS.
-// Phase 2: lane r < 6*NN owns element row r: its residual entry and its K row, scattered
-// with the same first-touch offsets as the volume elements (interface colours are
-// launched after the volume colours).
+// Phase 2: lane r < 6*NN owns element row r: its residual entry and its K row, written to
+// the node's staging block (gather mode) or scattered with the same first-touch offsets as
+// the volume elements (coloured mode: interface colours are launched after the volume
+// colours).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -293,9 +294,13 @@ __global__ __launch_bounds__(256) void coh_elem(CohArgs a, int64_t e_begin, int6
       fr += s_live[slot][q] ? qd[0] * qd[1 + NN + M] * qd[1 + P] : 0.0;
     }
     fr *= sr;
-    const bool first = off[ra] <= -2;
-    if (first) a.f[dof - 1] = fr;
-    else a.f[dof - 1] += fr;
+    if (a.stage) {
+      a.stage[a.soff[e * NT + ra] + M * (ND + 1) + ND] = fr;
+    } else {
+      const bool first = off[ra] <= -2;
+      if (first) a.f[dof - 1] = fr;
+      else a.f[dof - 1] += fr;
+    }
   }
   if constexpr (TAN) {
     double K[ND];
@@ -312,6 +317,12 @@ __global__ __launch_bounds__(256) void coh_elem(CohArgs a, int64_t e_begin, int6
         kn += 0.5 * wl * qd[13 + NN + (M * 3 + C) * NN + R];
       }
       K[c] = sr * sc * kt + sr * kn;
+    }
+    if (a.stage) {  // gather mode: row (ra, M) into the node's staging block
+      double *srow = a.stage + a.soff[e * NT + ra] + M * (ND + 1);
+#pragma unroll
+      for (int c = 0; c < ND; ++c) srow[c] = K[c];
+      return;
     }
     const int32_t node = s_node[slot][ra];
     int bp = 0;

@@ -1063,6 +1063,15 @@ __global__ __launch_bounds__(256) void cp_elem(CPElemArgs a, int64_t e_begin, in
   }
   const int32_t dof = s_dof[slot][lane];
   if (dof <= 0) return;
+  if (a.stage) {  // gather mode: row (ra, rb) into the node's staging block
+    double *srow = a.stage + a.soff[e * NNE + ra] + rb * (NDE + 1);
+    if constexpr (TAN) {
+#pragma unroll
+      for (int i = 0; i < NDE; ++i) srow[i] = K[i];
+    }
+    if constexpr (RES) srow[NDE] = fe;
+    return;
+  }
   const int16_t *off = a.off + (e * NNE + ra) * NNE;
   if constexpr (RES) {
     if (off[ra] <= -2) a.f[dof - 1] = fe;

@@ -42,9 +42,12 @@ struct ElemArgs {
   const double *Sc_n;       // scalars n   (J2: ep, gam)
   double *Sc_np1;           // scalars n+1
   double *cauchy;           // 6 comps (written by integration)
-  // outputs
+  // outputs: coloured mode adds into f / vals through off; gather mode (stage != null)
+  // writes row (a, b) of element e to stage[soff[e*nne + a] + b*(3nne+1) ...], residual last
   double *f;                // [ndof]
   double *vals;             // [nnz]
+  const int64_t *soff;      // [ne*nne] staging offsets of the element's node blocks
+  double *stage;
   int32_t *err;             // failure counter
 };
 
@@ -108,6 +111,8 @@ struct CPElemArgs {
   int64_t qp_base;
   const double *r, *d_r, *defl, *defl_d;
   double *f, *vals;
+  const int64_t *soff;  // gather mode: as ElemArgs
+  double *stage;
   int32_t *err;
 };
 void launch_cp_elem(int nne, int mode, const CPElemArgs &a, int64_t e_begin, int64_t e_end, hipStream_t s);
@@ -139,12 +144,33 @@ struct CohArgs {
   const double *vars_n;    // [2][n_cqp] X_eff_max, t_max at n
   double *vars_np1;        // [2][n_cqp] at n+1 (written by integrating passes)
   double *f, *vals;
+  const int64_t *soff;     // gather mode: as ElemArgs (node blocks of 2nn nodes)
+  double *stage;
   int32_t *err;
 };
 void launch_coh_elem(int nn, int mode, const CohArgs &a, int64_t e_begin, int64_t e_end, hipStream_t s);
 
 void launch_elem(int nne, int mode, const ElemArgs &a, int64_t e_begin, int64_t e_end,
                  hipStream_t s);
+
+// Gather assembly: node n owns the staging blocks [inc_ptr[n], inc_ptr[n+1]) (one per
+// incident element, in a fixed order); block k starts at inc_soff[k], comes from an element
+// of inc_nne[k] nodes, and pos[inc_poff[k] + w] packs, for the element's node w, the column
+// of its first free DOF inside node n's rows (bits 0-27) and its free-DOF mask (bits 28-30;
+// 0 = no column).
+struct GatherArgs {
+  const int64_t *inc_ptr, *inc_soff, *inc_poff;
+  const int32_t *inc_nne;
+  const uint32_t *pos;
+  const int32_t *dofid, *row_start, *rowlen;
+  const double *stage;
+  double *f, *vals;
+  int64_t n_nodes;
+  int32_t maxlen;  // longest row of any node (LDS per wavefront: 3 maxlen doubles)
+};
+// Wavefronts per workgroup for a given maxlen (0: rows too long for LDS).
+int gather_waves(int32_t maxlen);
+void launch_gather(int mode, const GatherArgs &g, hipStream_t s);
 // Jacobi-preconditioned BiCGStab on a CSR matrix; work holds 9 n + 1025 doubles.
 int bicgstab(hipStream_t s, const int32_t *Ap, const int32_t *Ai, const double *vals, const double *b, double *x,
              int64_t n, double tol, int maxit, int *iters, double *relres, double *work);

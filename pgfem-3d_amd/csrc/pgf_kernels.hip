@@ -5,12 +5,14 @@
 //   phase 0  lanes a < nne        gather node coordinates, DOF ids and displacements
 //   phase 1  lanes q < nqp        shape gradients, F, constitutive update, tau, c  -> LDS
 //   phase 2  lanes a*3+b < ndofe  one element row each: residual entry f_ab and the
-//                                 tangent row K_ab,wg summed over the QPs, then scattered
-//                                 into the CSR values by node-pair blocks.
-// Elements of one colour share no node, so the scatter is a plain read-modify-write (no
-// atomics); the first element (in launch order) to touch a node-pair block stores instead
-// of adding, so neither f nor the CSR values need zeroing and each entry's sum runs in the
-// same fixed order on every call.
+//                                 tangent row K_ab,wg summed over the QPs, then written out.
+// Output, gather mode: the row goes, whole and contiguous, to the element node's block of a
+// node-major staging buffer; gather_kernel then sums each node's blocks and writes the node's
+// CSR rows once.  Coloured mode: the launch covers one colour, whose elements share no node,
+// so the row is added into the CSR values by node-pair blocks with a plain read-modify-write
+// (no atomics); the first element (in launch order) to touch a node-pair block stores instead
+// of adding, so neither f nor the CSR values need zeroing.  Either way every entry's sum runs
+// in the same fixed order on every call.
 //
 // The contraction is the reference's compute_residual_vector / compute_stiffness_matrix
 // (src/constitutive_model.cc:1520-1662) for TL with M = I, written in the current
@@ -23,6 +25,7 @@
 // L = Fs^-1 Fs^-1 Fs^-1 Fs^-1 : c.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 
 #include "pgf_device.h"
@@ -172,7 +175,16 @@ __device__ __forceinline__ void j2_point(const DevMaterial &m, bool plastic, con
   }
 }
 
-template <int NNE, int MODE>
+// Elements per round of the LDS transpose that turns the lanes' element rows into
+// contiguous staging-block stores: the largest divisor of epb whose rows (nde x (nde+1)
+// doubles each) fit in the `room` doubles of the dead shape-gradient and tangent arrays.
+constexpr int stage_round(int epb, int nde, int room) {
+  int re = epb;
+  while (re > 1 && (epb % re != 0 || re * nde * (nde + 1) > room)) --re;
+  return re;
+}
+
+template <int NNE, int MODE, bool STG>
 __global__ __launch_bounds__(256) void elem_kernel(ElemArgs a, int64_t e_begin, int64_t e_end) {
   constexpr int NQP = nqp_of(NNE);
   constexpr int NDE = NNE * 3;
@@ -182,14 +194,20 @@ __global__ __launch_bounds__(256) void elem_kernel(ElemArgs a, int64_t e_begin, 
   constexpr bool RES = (MODE & MODE_RESIDUAL) != 0;
   constexpr bool TAN = (MODE & MODE_TANGENT) != 0;
   constexpr bool GEN = !INTEG;  // Fr may differ from the stored F(n+1)
+  constexpr int LD = NDE + 1;   // staging row: K row, then the residual entry
+  constexpr int kH = EPB * NQP * NDE, kC = EPB * NQP * (TAN ? 36 : 1);
+  constexpr int RE = stage_round(EPB, NDE, kH + kC);
+  static_assert(!TAN || RE * NDE * LD <= kH + kC, "staging round does not fit");
 
   __shared__ double s_u[EPB][NDE];
   __shared__ int32_t s_dof[EPB][NDE];
   __shared__ int32_t s_node[EPB][NNE];
   __shared__ double s_x[EPB][NDE];
-  __shared__ double s_h[EPB][NQP][NDE];
+  // shape gradients and tangents; reused as the staging transpose buffer after phase 2
+  __shared__ double s_work[kH + kC];
+  auto s_h = reinterpret_cast<double (*)[NQP][NDE]>(s_work);
+  auto s_c = reinterpret_cast<double (*)[NQP][TAN ? 36 : 1]>(s_work + kH);
   __shared__ double s_tau[EPB][NQP][6];
-  __shared__ double s_c[EPB][NQP][TAN ? 36 : 1];
   __shared__ double s_g[EPB][NQP][GEN ? 9 : 1];
   __shared__ double s_w[EPB][NQP];
 
@@ -328,8 +346,8 @@ __global__ __launch_bounds__(256) void elem_kernel(ElemArgs a, int64_t e_begin, 
   }
   __syncthreads();
 
-  // ---- phase 2: element rows
-  if (!(active && lane < NDE)) return;
+  // ---- phase 2: element rows (lanes a*3+b < NDE of live elements)
+  const bool rowlane = active && lane < NDE;
   const int ra = lane / 3, rb = lane % 3;
   const double eb0 = rb == 0 ? 1.0 : 0.0, eb1 = rb == 1 ? 1.0 : 0.0, eb2 = rb == 2 ? 1.0 : 0.0;
   double K[TAN ? NDE : 1];
@@ -338,68 +356,102 @@ __global__ __launch_bounds__(256) void elem_kernel(ElemArgs a, int64_t e_begin, 
     for (int i = 0; i < NDE; ++i) K[i] = 0.0;
   }
   double fe = 0.0;
+  if (rowlane) {
 #pragma unroll
-  for (int ip = 0; ip < NQP; ++ip) {
-    const double wq = s_w[slot][ip];
-    const double *h = s_h[slot][ip];
-    const double *tau = s_tau[slot][ip];
-    const double ha[3] = {h[ra * 3 + 0], h[ra * 3 + 1], h[ra * 3 + 2]};
-    double t[3];
-    symv(tau, ha, t);  // tau h_a
-    double g[3] = {eb0, eb1, eb2};
-    if constexpr (GEN) {
-      g[0] = s_g[slot][ip][rb * 3 + 0];
-      g[1] = s_g[slot][ip][rb * 3 + 1];
-      g[2] = s_g[slot][ip][rb * 3 + 2];
-    }
-    if constexpr (RES) fe += wq * (g[0] * t[0] + g[1] * t[1] + g[2] * t[2]);
-    if constexpr (TAN) {
-      const double *c = s_c[slot][ip];
-      // engineering strain vector of sym(g (x) h_a)
-      const double bv[6] = {g[0] * ha[0], g[1] * ha[1], g[2] * ha[2], g[0] * ha[1] + g[1] * ha[0],
-                            g[1] * ha[2] + g[2] * ha[1], g[0] * ha[2] + g[2] * ha[0]};
-      double r[6];
-#pragma unroll
-      for (int J = 0; J < 6; ++J) {
-        double acc = 0.0;
-#pragma unroll
-        for (int I = 0; I < 6; ++I) acc += bv[I] * c[I * 6 + J];
-        r[J] = wq * acc;
-      }
-      // M = Gr R with R = mat(r) (symmetric)
-      double M[9];
+    for (int ip = 0; ip < NQP; ++ip) {
+      const double wq = s_w[slot][ip];
+      const double *h = s_h[slot][ip];
+      const double *tau = s_tau[slot][ip];
+      const double ha[3] = {h[ra * 3 + 0], h[ra * 3 + 1], h[ra * 3 + 2]};
+      double t[3];
+      symv(tau, ha, t);  // tau h_a
+      double g[3] = {eb0, eb1, eb2};
       if constexpr (GEN) {
-        const double *Gm = s_g[slot][ip];
-#pragma unroll
-        for (int i = 0; i < 3; ++i) {
-          M[i * 3 + 0] = Gm[i * 3 + 0] * r[0] + Gm[i * 3 + 1] * r[3] + Gm[i * 3 + 2] * r[5];
-          M[i * 3 + 1] = Gm[i * 3 + 0] * r[3] + Gm[i * 3 + 1] * r[1] + Gm[i * 3 + 2] * r[4];
-          M[i * 3 + 2] = Gm[i * 3 + 0] * r[5] + Gm[i * 3 + 1] * r[4] + Gm[i * 3 + 2] * r[2];
-        }
-      } else {
-        M[0] = r[0]; M[1] = r[3]; M[2] = r[5];
-        M[3] = r[3]; M[4] = r[1]; M[5] = r[4];
-        M[6] = r[5]; M[7] = r[4]; M[8] = r[2];
+        g[0] = s_g[slot][ip][rb * 3 + 0];
+        g[1] = s_g[slot][ip][rb * 3 + 1];
+        g[2] = s_g[slot][ip][rb * 3 + 2];
       }
-      const double wt0 = wq * t[0], wt1 = wq * t[1], wt2 = wq * t[2];
+      if constexpr (RES) fe += wq * (g[0] * t[0] + g[1] * t[1] + g[2] * t[2]);
+      if constexpr (TAN) {
+        const double *c = s_c[slot][ip];
+        // engineering strain vector of sym(g (x) h_a)
+        const double bv[6] = {g[0] * ha[0], g[1] * ha[1], g[2] * ha[2], g[0] * ha[1] + g[1] * ha[0],
+                              g[1] * ha[2] + g[2] * ha[1], g[0] * ha[2] + g[2] * ha[0]};
+        double r[6];
 #pragma unroll
-      for (int w = 0; w < NNE; ++w) {
-        const double h0 = h[w * 3 + 0], h1 = h[w * 3 + 1], h2 = h[w * 3 + 2];
-        const double geo = wt0 * h0 + wt1 * h1 + wt2 * h2;  // delta_bg h_a.tau.h_w
-        K[w * 3 + 0] += M[0] * h0 + M[1] * h1 + M[2] * h2 + eb0 * geo;
-        K[w * 3 + 1] += M[3] * h0 + M[4] * h1 + M[5] * h2 + eb1 * geo;
-        K[w * 3 + 2] += M[6] * h0 + M[7] * h1 + M[8] * h2 + eb2 * geo;
+        for (int J = 0; J < 6; ++J) {
+          double acc = 0.0;
+#pragma unroll
+          for (int I = 0; I < 6; ++I) acc += bv[I] * c[I * 6 + J];
+          r[J] = wq * acc;
+        }
+        // M = Gr R with R = mat(r) (symmetric)
+        double M[9];
+        if constexpr (GEN) {
+          const double *Gm = s_g[slot][ip];
+#pragma unroll
+          for (int i = 0; i < 3; ++i) {
+            M[i * 3 + 0] = Gm[i * 3 + 0] * r[0] + Gm[i * 3 + 1] * r[3] + Gm[i * 3 + 2] * r[5];
+            M[i * 3 + 1] = Gm[i * 3 + 0] * r[3] + Gm[i * 3 + 1] * r[1] + Gm[i * 3 + 2] * r[4];
+            M[i * 3 + 2] = Gm[i * 3 + 0] * r[5] + Gm[i * 3 + 1] * r[4] + Gm[i * 3 + 2] * r[2];
+          }
+        } else {
+          M[0] = r[0]; M[1] = r[3]; M[2] = r[5];
+          M[3] = r[3]; M[4] = r[1]; M[5] = r[4];
+          M[6] = r[5]; M[7] = r[4]; M[8] = r[2];
+        }
+        const double wt0 = wq * t[0], wt1 = wq * t[1], wt2 = wq * t[2];
+#pragma unroll
+        for (int w = 0; w < NNE; ++w) {
+          const double h0 = h[w * 3 + 0], h1 = h[w * 3 + 1], h2 = h[w * 3 + 2];
+          const double geo = wt0 * h0 + wt1 * h1 + wt2 * h2;  // delta_bg h_a.tau.h_w
+          K[w * 3 + 0] += M[0] * h0 + M[1] * h1 + M[2] * h2 + eb0 * geo;
+          K[w * 3 + 1] += M[3] * h0 + M[4] * h1 + M[5] * h2 + eb1 * geo;
+          K[w * 3 + 2] += M[6] * h0 + M[7] * h1 + M[8] * h2 + eb2 * geo;
+        }
       }
     }
-  }
-  if constexpr (TAN) {  // non-normal diagonal, src/constitutive_model.cc:2682-2687
-    double diag = 0.0;
+    if constexpr (TAN) {  // non-normal diagonal, src/constitutive_model.cc:2682-2687
+      double diag = 0.0;
 #pragma unroll
-    for (int i = 0; i < NDE; ++i) diag = (i == lane) ? K[i] : diag;
-    if (!isnormal(diag)) atomicAdd(a.err, 1);
+      for (int i = 0; i < NDE; ++i) diag = (i == lane) ? K[i] : diag;
+      if (!isnormal(diag)) atomicAdd(a.err, 1);
+    }
   }
+  if constexpr (TAN && STG) {
+    // Gather mode: RE elements at a time, the rows go to LDS laid out as the staging blocks
+    // ([node a][row b][K row, residual]); then the whole workgroup copies each node block
+    // (3 (NDE+1) consecutive doubles) to its staging offset, consecutive threads on
+    // consecutive doubles.  Rows of prescribed DOFs are not stored (nothing reads them).
+    __syncthreads();  // every lane is done with s_h / s_c
+    const int64_t e_blk = e_begin + int64_t(blockIdx.x) * EPB;
+#pragma unroll 1
+    for (int r0 = 0; r0 < EPB; r0 += RE) {
+      if (rowlane && slot >= r0 && slot < r0 + RE) {
+        double *row = s_work + ((slot - r0) * NDE + lane) * LD;
+#pragma unroll
+        for (int i = 0; i < NDE; ++i) row[i] = K[i];
+        row[NDE] = fe;
+      }
+      __syncthreads();
+      for (int idx = threadIdx.x; idx < RE * NDE * LD; idx += 256) {
+        const int el = idx / (NDE * LD), o = idx - el * (NDE * LD);
+        const int na = o / (3 * LD), ob = o - na * 3 * LD;
+        const int64_t ee = e_blk + r0 + el;
+        if (ee >= e_end || s_dof[r0 + el][na * 3 + ob / LD] <= 0) continue;
+        a.stage[a.soff[ee * NNE + na] + ob] = s_work[idx];
+      }
+      __syncthreads();
+    }
+    return;
+  }
+  if (!rowlane) return;
   const int32_t dof = s_dof[slot][lane];
   if (dof <= 0) return;
+  if constexpr (STG) {  // gather mode, residual only: the row's entry into its staging block
+    a.stage[a.soff[e * NNE + ra] + rb * LD + NDE] = fe;
+    return;
+  }
   const int16_t *off = a.off + (e * NNE + ra) * NNE;
   if constexpr (RES) {
     const bool first = off[ra] <= -2;
@@ -445,6 +497,84 @@ __global__ __launch_bounds__(256) void elem_kernel(ElemArgs a, int64_t e_begin, 
   }
 }
 
+// Gather assembly, one wavefront per node n with free DOFs: the node's rows (nr free DOFs x
+// rowlen columns, contiguous in the CSR values) are summed in LDS over the node's staging
+// blocks, then stored once; lanes 0-2 sum the residual entries.  Blocks are taken four at a
+// time: every lane first loads its items (at most two per block: a block has 3 x 3nne <= 90
+// items) of all four, then the four blocks' LDS adds follow in block order.  Within one block
+// every (row, column) target is distinct, so the lanes' adds never collide, and the wave
+// barrier between blocks keeps their order -- the sums run in the same order on every call.
+constexpr int kGatherBatch = 4;
+
+template <bool RES, bool TAN>
+__global__ __launch_bounds__(256) void gather_kernel(GatherArgs g) {
+  extern __shared__ double s_acc[];
+  const int wpb = blockDim.x / 64;
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
+  const int lane = threadIdx.x % 64;
+  const int64_t node = int64_t(blockIdx.x) * wpb + wid;
+  if (node >= g.n_nodes) return;
+  int fm = 0;
+#pragma unroll
+  for (int b = 0; b < 3; ++b) fm |= (g.dofid[node * 3 + b] > 0) ? (1 << b) : 0;
+  if (fm == 0) return;
+  const int nr = __popc(fm);
+  const int len = g.rowlen[node];
+  double *acc = s_acc + int64_t(wid) * 3 * g.maxlen;
+  if constexpr (TAN) {
+    for (int i = lane; i < nr * len; i += 64) acc[i] = 0.0;
+    __builtin_amdgcn_wave_barrier();
+  }
+  const bool frow = lane < 3 && ((fm >> lane) & 1);
+  double fs = 0.0;
+  const int64_t k1 = g.inc_ptr[node + 1];
+  for (int64_t k0 = g.inc_ptr[node]; k0 < k1; k0 += kGatherBatch) {
+    double v[kGatherBatch][2], fv[kGatherBatch];
+    int dst[kGatherBatch][2];
+#pragma unroll
+    for (int j = 0; j < kGatherBatch; ++j) {
+      dst[j][0] = dst[j][1] = -1;
+      fv[j] = 0.0;
+      const int64_t k = k0 + j;
+      if (k >= k1) continue;
+      const double *blk = g.stage + g.inc_soff[k];
+      const int nde = 3 * g.inc_nne[k], ld = nde + 1;
+      if (RES && frow) fv[j] = blk[lane * ld + nde];
+      if constexpr (TAN) {
+        const uint32_t *pos = g.pos + g.inc_poff[k];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int it = lane + 64 * h;
+          v[j][h] = 0.0;
+          if (it >= 3 * nde) continue;
+          const int b = it / nde, col = it - b * nde;
+          const int w = col / 3, gg = col - 3 * w;
+          const uint32_t p = pos[w];
+          const int mw = int(p >> 28);
+          if (!((fm >> b) & 1) || !((mw >> gg) & 1)) continue;
+          dst[j][h] = __popc(fm & ((1 << b) - 1)) * len + int(p & 0x0fffffffu) + __popc(mw & ((1 << gg) - 1));
+          v[j][h] = blk[b * ld + col];
+        }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < kGatherBatch; ++j) {
+      if constexpr (RES) fs += fv[j];
+      if constexpr (TAN) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+          if (dst[j][h] >= 0) acc[dst[j][h]] += v[j][h];
+        __builtin_amdgcn_wave_barrier();
+      }
+    }
+  }
+  if constexpr (TAN) {
+    double *out = g.vals + g.row_start[node];
+    for (int i = lane; i < nr * len; i += 64) out[i] = acc[i];
+  }
+  if (RES && frow) g.f[g.dofid[node * 3 + lane] - 1] = fs;
+}
+
 __global__ void fill_kernel(double *p, int64_t n, double v) {
   for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
     p[i] = v;
@@ -458,7 +588,7 @@ __global__ void copy_kernel(double *dst, const double *src, int64_t n) {
     dst[i] = src[i];
 }
 
-template <int NNE>
+template <int NNE, bool STG>
 void launch_nne(int mode, const ElemArgs &a, int64_t e_begin, int64_t e_end, hipStream_t s) {
   constexpr int TPE = (NNE == 4) ? 16 : 32;
   constexpr int EPB = 256 / TPE;
@@ -467,16 +597,16 @@ void launch_nne(int mode, const ElemArgs &a, int64_t e_begin, int64_t e_end, hip
   const unsigned grid = unsigned((n + EPB - 1) / EPB);
   switch (mode) {
     case MODE_INTEGRATE | MODE_RESIDUAL:
-      elem_kernel<NNE, MODE_INTEGRATE | MODE_RESIDUAL><<<grid, 256, 0, s>>>(a, e_begin, e_end);
+      elem_kernel<NNE, MODE_INTEGRATE | MODE_RESIDUAL, STG><<<grid, 256, 0, s>>>(a, e_begin, e_end);
       break;
     case MODE_RESIDUAL:
-      elem_kernel<NNE, MODE_RESIDUAL><<<grid, 256, 0, s>>>(a, e_begin, e_end);
+      elem_kernel<NNE, MODE_RESIDUAL, STG><<<grid, 256, 0, s>>>(a, e_begin, e_end);
       break;
     case MODE_TANGENT:
-      elem_kernel<NNE, MODE_TANGENT><<<grid, 256, 0, s>>>(a, e_begin, e_end);
+      elem_kernel<NNE, MODE_TANGENT, STG><<<grid, 256, 0, s>>>(a, e_begin, e_end);
       break;
     case MODE_INTEGRATE | MODE_RESIDUAL | MODE_TANGENT:
-      elem_kernel<NNE, MODE_INTEGRATE | MODE_RESIDUAL | MODE_TANGENT><<<grid, 256, 0, s>>>(a, e_begin, e_end);
+      elem_kernel<NNE, MODE_INTEGRATE | MODE_RESIDUAL | MODE_TANGENT, STG><<<grid, 256, 0, s>>>(a, e_begin, e_end);
       break;
     default:
       break;
@@ -487,9 +617,27 @@ void launch_nne(int mode, const ElemArgs &a, int64_t e_begin, int64_t e_end, hip
 
 void launch_elem(int nne, int mode, const ElemArgs &a, int64_t e_begin, int64_t e_end,
                  hipStream_t s) {
-  if (nne == 4) launch_nne<4>(mode, a, e_begin, e_end, s);
-  else if (nne == 10) launch_nne<10>(mode, a, e_begin, e_end, s);
-  else if (nne == 8) launch_nne<8>(mode, a, e_begin, e_end, s);
+  const bool stg = a.stage != nullptr;
+  if (nne == 4) (stg ? launch_nne<4, true> : launch_nne<4, false>)(mode, a, e_begin, e_end, s);
+  else if (nne == 10) (stg ? launch_nne<10, true> : launch_nne<10, false>)(mode, a, e_begin, e_end, s);
+  else if (nne == 8) (stg ? launch_nne<8, true> : launch_nne<8, false>)(mode, a, e_begin, e_end, s);
+}
+
+int gather_waves(int32_t maxlen) {
+  const int64_t per_wave = int64_t(3) * std::max<int32_t>(maxlen, 1) * int64_t(sizeof(double));
+  const int64_t w = (int64_t(64) << 10) / per_wave;  // 64 KiB of LDS per workgroup
+  return int(std::min<int64_t>(4, w));
+}
+
+void launch_gather(int mode, const GatherArgs &g, hipStream_t s) {
+  const int wpb = gather_waves(g.maxlen);
+  if (g.n_nodes <= 0 || wpb < 1) return;
+  const unsigned grid = unsigned((g.n_nodes + wpb - 1) / wpb);
+  const size_t lds = size_t(wpb) * 3 * std::max<int32_t>(g.maxlen, 1) * sizeof(double);
+  const bool res = (mode & MODE_RESIDUAL) != 0, tan = (mode & MODE_TANGENT) != 0;
+  if (res && tan) gather_kernel<true, true><<<grid, 64 * wpb, lds, s>>>(g);
+  else if (res) gather_kernel<true, false><<<grid, 64 * wpb, lds, s>>>(g);
+  else if (tan) gather_kernel<false, true><<<grid, 64 * wpb, lds, s>>>(g);
 }
 
 void launch_fill(double *p, int64_t n, double v, hipStream_t s) {

@@ -24,6 +24,7 @@ LIB_PATH = os.environ.get("PGFEM3D_AMD_LIB") or os.path.join(_HERE, "_build", "l
 
 PGF_TET4, PGF_TET10, PGF_HEX8 = 4, 10, 8
 MODEL_HYPER_ELASTICITY, MODEL_CRYSTAL_PLASTICITY, MODEL_J2_PLASTICITY = 0, 1, 4
+ASSEMBLY_GATHER, ASSEMBLY_COLOURED = 0, 1
 STATE_F_NP1, STATE_F_N, STATE_F_NM1 = 0, 1, 2
 STATE_PLASTIC_NP1, STATE_PLASTIC_N, STATE_PLASTIC_NM1 = 3, 4, 5
 STATE_SCALAR_NP1, STATE_SCALAR_N, STATE_SCALAR_NM1 = 6, 7, 8
@@ -118,6 +119,7 @@ def lib() -> C.CDLL:
             "pgf_cp_log_samples": [P, P, P],
             "pgf_solve": [P, P, P, P, D, I32, P, P],
             "pgf_set_graph": [P, C.c_int],
+        "pgf_set_assembly": [P, C.c_int32],
             "pgf_autotune": [P, P, D, P],
             "pgf_set_surrogate": [P, P, P, P, P],
         }
@@ -139,7 +141,7 @@ def exported_symbols():
         "pgf_set_warm_start", "pgf_get_qp_order", "pgf_state_device_ptr", "pgf_set_cohesive_props",
         "pgf_get_cohesive_sizes", "pgf_mlp_create", "pgf_mlp_destroy", "pgf_mlp_param_count",
         "pgf_mlp_get_params", "pgf_mlp_set_params", "pgf_mlp_get_grads", "pgf_mlp_forward", "pgf_mlp_backward",
-        "pgf_mlp_adam_step", "pgf_cp_log_samples", "pgf_set_surrogate", "pgf_solve", "pgf_set_graph",
+        "pgf_mlp_adam_step", "pgf_cp_log_samples", "pgf_set_surrogate", "pgf_solve", "pgf_set_graph", "pgf_set_assembly",
         "pgf_autotune",
     ]
 
@@ -320,6 +322,11 @@ class Handle:
 
     def set_graph(self, on: bool):
         _check(self._lib.pgf_set_graph(self.h, int(on)), "pgf_set_graph")
+
+    def set_assembly(self, mode: int):
+        """ASSEMBLY_GATHER (default: staged element rows, one gather per node) or
+        ASSEMBLY_COLOURED (colour-by-colour adds into the CSR values, no staging memory)."""
+        _check(self._lib.pgf_set_assembly(self.h, int(mode)), "pgf_set_assembly")
 
     def autotune(self, r, d_r=None, defl=None, defl_d=None, dt=1.0) -> int:
         inp = self._input(r, d_r, defl, defl_d)
