@@ -180,6 +180,7 @@ def run_j2(args, dev):
     h.set_assembly(modes[other])
     s2, k2, n2 = _timed_passes(h, f, vals, rd, dd, max(3, args.steps // 4), 1)
     res["other"] = {"assembly": other, "ms_per_step": 1e3 * s2, "kernel_ms_per_pass": k2, "launches_per_pass": n2}
+    h.commit_step()  # state roll-over: copies of known size (calibration of HBM counters)
     print(f"[bench] J2 ({other} assembly): {1e3 * s2:.3f} ms/step wall, element pass {k2:.3f} ms over {n2} "
           "launches", file=sys.stderr, flush=True)
     h.close()

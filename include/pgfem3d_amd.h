@@ -233,8 +233,8 @@ int pgf_autotune(pgf_handle h, const pgf_nodal_input *in, double dt, int64_t *ch
  * PLoc_Sparse, src/fd_residuals.cc:58-73, src/PLoc_Sparse.cc:15-175).  Both modes are
  * deterministic, use no atomics, and give the same sums up to the order of the additions:
  *   PGF_ASSEMBLY_GATHER (default): every element writes its rows once into a node-major
- *     staging buffer (3 (3 nne + 1) doubles per element node, allocated at the first such
- *     pass); then one wavefront per node adds its elements' blocks in a fixed order and
+ *     staging buffer (3 (3 nne + 1) doubles per element node, rounded up to whole 128-byte
+ *     lines; allocated at the first such pass); then one wavefront per node adds its elements' blocks in a fixed order and
  *     writes the node's CSR rows and residual entries once, contiguously.
  *   PGF_ASSEMBLY_COLOURED: elements of one colour share no node and add their rows into
  *     the CSR values directly, colour by colour (no staging memory).

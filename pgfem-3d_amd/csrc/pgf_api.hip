@@ -19,13 +19,16 @@ namespace {
 
 constexpr int nqp_of(int nne) { return nne == 4 ? 1 : (nne == 10 ? 4 : 8); }
 
-// Elements of one (element type, material model) pair, in device order (sorted by colour,
-// then by mesh order).  Their QPs occupy [qp_base, qp_base + ne*nqp) of the state arrays.
+// Elements of one (element type, material model) pair, in device order (mesh order, which
+// keeps neighbouring elements together).  Their QPs occupy [qp_base, qp_base + ne*nqp) of
+// the state arrays.  The coloured mode launches colour by colour through clist.
 struct Block {
   int nne = 0, nqp = 0, model = 0;
   int64_t ne = 0, qp_base = 0;
   std::vector<int64_t> elems;      // mesh element ids in device order
-  std::vector<int64_t> color_ptr;  // [n_colors+1] colour segments of elems
+  std::vector<int32_t> clist;      // block-local element indices sorted by colour
+  std::vector<int64_t> color_ptr;  // [n_colors+1] colour segments of clist
+  int32_t *d_clist = nullptr;
   int32_t *d_conn = nullptr;
   int16_t *d_off = nullptr;
   int64_t *d_soff = nullptr;       // gather mode: [ne*nne] staging offsets
@@ -34,12 +37,14 @@ struct Block {
   int32_t *d_qgrain = nullptr;
 };
 
-// Interface elements of one kind (nn nodes per side), colour-sorted.
+// Interface elements of one kind (nn nodes per side), in mesh order; colour lists as Block.
 struct CohBlock {
   int nn = 0;
   int64_t nc = 0, cqp_base = 0;
   std::vector<int64_t> elems;      // mesh interface-element ids in device order
+  std::vector<int32_t> clist;
   std::vector<int64_t> color_ptr;  // [n_ccolors+1]
+  int32_t *d_clist = nullptr;
   int32_t *d_conn = nullptr, *d_mat = nullptr;
   int16_t *d_off = nullptr;
   int64_t *d_soff = nullptr;
@@ -334,7 +339,9 @@ pgf::ElemArgs base_args(pgf_handle h, const pgf_nodal_input *in) {
 // Gather tables (pgf::GatherArgs) of the element groups -- the volume blocks, then the
 // interface blocks, each in device order.  Every (element, node) pair whose node has a free
 // DOF gets a staging block; a node's blocks are contiguous and ordered by (group, element,
-// local node), which fixes the order of every sum.  off holds the groups' first-touch-marked
+// local node), which fixes the order of every sum.  Blocks start on 128-byte lines and are
+// padded to whole lines, so the element kernels' block stores never share a line with a
+// block written at another time (a partly written line costs a read-modify-write).  off holds the groups' first-touch-marked
 // node-pair offsets; soff receives each group's [n*nne] staging offsets (-1: no block).
 struct GatherGroup {
   int nne;
@@ -379,7 +386,7 @@ void build_gather(int64_t n_nodes, const int32_t *dofid, std::vector<GatherGroup
   for (int64_t sl = 0; sl < n_inc; ++sl) {
     inc_soff[sl] = so;
     inc_poff[sl] = po;
-    so += 3 * (3 * int64_t(inc_nne[sl]) + 1);
+    so += pgf::stage_block(inc_nne[sl]);
     po += inc_nne[sl];
   }
   stage_size = so;
@@ -445,12 +452,14 @@ int32_t enqueue_pass(pgf_handle h, const pgf_nodal_input *in, int mode, double *
         ce.vals = vals;
         ce.soff = b.d_soff;
         ce.stage = gather ? h->d_stage : nullptr;
+        ce.elist = gather ? nullptr : b.d_clist;
         ce.err = h->d_err;
         pgf::launch_cp_elem(b.nne, mode, ce, e0, e1, h->stream);
       } else {
         a.conn = b.d_conn;
         a.off = b.d_off;
         a.soff = b.d_soff;
+        a.elist = gather ? nullptr : b.d_clist;
         a.mat = b.d_mat;
         a.qp_base = b.qp_base;
         pgf::launch_elem(b.nne, mode, a, e0, e1, h->stream);
@@ -484,6 +493,7 @@ int32_t enqueue_pass(pgf_handle h, const pgf_nodal_input *in, int mode, double *
         ca.mat = cb.d_mat;
         ca.off = cb.d_off;
         ca.soff = cb.d_soff;
+        ca.elist = gather ? nullptr : cb.d_clist;
         ca.cqp_base = cb.cqp_base;
         pgf::launch_coh_elem(cb.nn, mode, ca, e0, e1, h->stream);
         ++launches;
@@ -705,8 +715,11 @@ int pgf_create(const pgf_mesh *mesh, const int64_t *id, int64_t ndof, const int3
     b.nqp = pgf::nqp_of(b.nne);
     for (int64_t e = 0; e < m.n_elem; ++e)
       if (block_of_key[elem_block[e]] == int(bi)) b.elems.push_back(e);
-    std::stable_sort(b.elems.begin(), b.elems.end(), [&](int64_t x, int64_t y) { return color[x] < color[y]; });
     b.ne = int64_t(b.elems.size());
+    b.clist.resize(b.ne);
+    std::iota(b.clist.begin(), b.clist.end(), 0);
+    std::stable_sort(b.clist.begin(), b.clist.end(),
+                     [&](int32_t x, int32_t y) { return color[b.elems[x]] < color[b.elems[y]]; });
     b.color_ptr.assign(ncol + 1, 0);
     for (int64_t e : b.elems) b.color_ptr[color[e] + 1]++;
     for (int c = 0; c < ncol; ++c) b.color_ptr[c + 1] += b.color_ptr[c];
@@ -735,8 +748,8 @@ int pgf_create(const pgf_mesh *mesh, const int64_t *id, int64_t ndof, const int3
     for (size_t bi = 0; bi < h->blocks.size(); ++bi) {
       Block &b = h->blocks[bi];
       const int nne = b.nne;
-      for (int64_t k = b.color_ptr[c]; k < b.color_ptr[c + 1]; ++k) {
-        const int64_t e = b.elems[k];
+      for (int64_t j = b.color_ptr[c]; j < b.color_ptr[c + 1]; ++j) {
+        const int64_t k = b.clist[j], e = b.elems[k];
         const int64_t *conn = m.elem_conn + m.elem_ptr[e];
         for (int a = 0; a < nne; ++a)
           for (int w = 0; w < nne; ++w) {
@@ -766,8 +779,11 @@ int pgf_create(const pgf_mesh *mesh, const int64_t *id, int64_t ndof, const int3
       for (int64_t c = 0; c < m.n_coel; ++c)
         if (m.coel_ptr[c + 1] - m.coel_ptr[c] == 2 * nn) cb.elems.push_back(c);
       if (cb.elems.empty()) continue;
-      std::stable_sort(cb.elems.begin(), cb.elems.end(), [&](int64_t x, int64_t y) { return ccolor[x] < ccolor[y]; });
       cb.nc = int64_t(cb.elems.size());
+      cb.clist.resize(cb.nc);
+      std::iota(cb.clist.begin(), cb.clist.end(), 0);
+      std::stable_sort(cb.clist.begin(), cb.clist.end(),
+                       [&](int32_t x, int32_t y) { return ccolor[cb.elems[x]] < ccolor[cb.elems[y]]; });
       cb.color_ptr.assign(nccol + 1, 0);
       for (int64_t c : cb.elems) cb.color_ptr[ccolor[c] + 1]++;
       for (int c = 0; c < nccol; ++c) cb.color_ptr[c + 1] += cb.color_ptr[c];
@@ -793,8 +809,8 @@ int pgf_create(const pgf_mesh *mesh, const int64_t *id, int64_t ndof, const int3
       for (size_t bi = 0; bi < h->cblocks.size(); ++bi) {
         pgf::CohBlock &cb = h->cblocks[bi];
         const int nt = 2 * cb.nn;
-        for (int64_t k = cb.color_ptr[c]; k < cb.color_ptr[c + 1]; ++k) {
-          const int64_t ce = cb.elems[k];
+        for (int64_t j = cb.color_ptr[c]; j < cb.color_ptr[c + 1]; ++j) {
+          const int64_t k = cb.clist[j], ce = cb.elems[k];
           const int64_t *conn = m.coel_conn + m.coel_ptr[ce];
           for (int a = 0; a < nt; ++a)
             for (int w = 0; w < nt; ++w) {
@@ -852,6 +868,7 @@ int pgf_create(const pgf_mesh *mesh, const int64_t *id, int64_t ndof, const int3
     if ((rc = pgf::upload(&cb.d_mat, zero.data(), zero.size()))) return fail(rc);
     if ((rc = pgf::upload(&cb.d_off, cboff[bi].data(), cboff[bi].size()))) return fail(rc);
     if ((rc = pgf::upload(&cb.d_soff, soff.data(), soff.size()))) return fail(rc);
+    if ((rc = pgf::upload(&cb.d_clist, cb.clist.data(), cb.clist.size()))) return fail(rc);
   }
   if (h->n_cqp) {
     for (int t = 0; t < 3; ++t) {
@@ -883,6 +900,7 @@ int pgf_create(const pgf_mesh *mesh, const int64_t *id, int64_t ndof, const int3
     if ((rc = pgf::upload(&b.d_mat, mat.data(), mat.size()))) return fail(rc);
     if ((rc = pgf::upload(&b.d_off, boff[bi].data(), boff[bi].size()))) return fail(rc);
     if ((rc = pgf::upload(&b.d_soff, groups[bi].soff.data(), groups[bi].soff.size()))) return fail(rc);
+    if ((rc = pgf::upload(&b.d_clist, b.clist.data(), b.clist.size()))) return fail(rc);
   }
   for (int t = 0; t < 3; ++t) {
     if ((rc = pgf::dalloc(&h->d_F[t], size_t(h->n_qp) * 9))) return fail(rc);
@@ -942,6 +960,7 @@ int pgf_destroy(pgf_handle h) {
     hipFree(b.d_conn);
     hipFree(b.d_off);
     hipFree(b.d_soff);
+    hipFree(b.d_clist);
     hipFree(b.d_mat);
     hipFree(b.d_egrain);
   }
@@ -950,6 +969,7 @@ int pgf_destroy(pgf_handle h) {
     hipFree(cb.d_mat);
     hipFree(cb.d_off);
     hipFree(cb.d_soff);
+    hipFree(cb.d_clist);
   }
   hipFree(h->d_inc_ptr);
   hipFree(h->d_inc_soff);

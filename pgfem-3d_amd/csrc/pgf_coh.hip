@@ -151,8 +151,9 @@ __global__ __launch_bounds__(256) void coh_elem(CohArgs a, int64_t e_begin, int6
 
   const int slot = threadIdx.x / 32;
   const int lane = threadIdx.x % 32;
-  const int64_t e = e_begin + int64_t(blockIdx.x) * EPB + slot;
-  const bool active = e < e_end;
+  const int64_t ie = e_begin + int64_t(blockIdx.x) * EPB + slot;
+  const bool active = ie < e_end;
+  const int64_t e = (a.elist && active) ? int64_t(a.elist[ie]) : ie;
 
   // ---- phase 0: gather
   if (active && lane < NT) {

@@ -884,8 +884,9 @@ __global__ __launch_bounds__(256) void cp_elem(CPElemArgs a, int64_t e_begin, in
   __shared__ double s_D[EPB][NQP][TAN ? 81 : 1];
 
   const int slot = threadIdx.x / TPE, lane = threadIdx.x % TPE;
-  const int64_t e = e_begin + int64_t(blockIdx.x) * EPB + slot;
-  const bool active = (threadIdx.x < EPB * TPE) && e < e_end;
+  const int64_t ie = e_begin + int64_t(blockIdx.x) * EPB + slot;
+  const bool active = (threadIdx.x < EPB * TPE) && ie < e_end;
+  const int64_t e = (a.elist && active) ? int64_t(a.elist[ie]) : ie;
   if (active && lane < NNE) {
     const int32_t node = a.conn[e * NNE + lane];
     s_node[slot][lane] = node;

@@ -12,6 +12,19 @@ namespace pgf {
 __host__ __device__ constexpr int voigt_i(int I) { return I < 3 ? I : (I == 3 ? 0 : (I == 4 ? 1 : 0)); }
 __host__ __device__ constexpr int voigt_j(int I) { return I < 3 ? I : (I == 3 ? 1 : 2); }
 
+// Workgroup barrier that orders LDS only: global stores still in flight keep draining
+// across it (__syncthreads would first wait for every outstanding global access).
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Doubles of one staging block (gather assembly): the 3 rows of an element node, each the
+// element's K row (3 nne) and residual entry, padded to whole 128-byte lines so every line
+// of a block is written whole by the element that owns it.
+__host__ __device__ constexpr int stage_block(int nne) { return (3 * (3 * nne + 1) + 15) & ~15; }
+
 // Device copy of the per-material constants used by the kernels.
 struct DevMaterial {
   int32_t model;
@@ -29,6 +42,7 @@ struct ElemArgs {
   const int32_t *rowlen;    // [n_nodes]
   const int32_t *conn;      // [ne*nne] block connectivity, device element order
   const int16_t *off;       // [ne*nne*nne] scatter offsets (>=0 add, <=-2 first touch, -1 none)
+  const int32_t *elist;     // coloured mode: block elements in colour order (launch ranges index it)
   const int32_t *mat;       // [ne] material index
   const DevMaterial *mats;
   int64_t qp_base;          // first QP of the block in device QP order
@@ -106,6 +120,7 @@ struct CPElemArgs {
   const double *X;
   const int32_t *dofid, *row_start, *rowlen, *conn, *grain;
   const int16_t *off;
+  const int32_t *elist;  // coloured mode: as ElemArgs
   const DevGrain *grains;
   const double *qpbuf;
   int64_t qp_base;
@@ -138,6 +153,7 @@ struct CohArgs {
   const int32_t *conn;     // [nc][2*nn] block connectivity (- side nodes, then + side)
   const int32_t *mat;      // [nc] property index
   const int16_t *off;      // [nc][2nn][2nn] scatter offsets (same encoding as ElemArgs::off)
+  const int32_t *elist;    // coloured mode: as ElemArgs
   const DevCohesive *props;
   int64_t cqp_base, n_cqp; // first QP of the block, total interface QPs (SoA stride)
   const double *r, *d_r, *defl, *defl_d;

@@ -213,8 +213,11 @@ __global__ __launch_bounds__(256) void elem_kernel(ElemArgs a, int64_t e_begin, 
 
   const int slot = threadIdx.x / TPE;
   const int lane = threadIdx.x % TPE;
-  const int64_t e = e_begin + int64_t(blockIdx.x) * EPB + slot;
-  const bool active = e < e_end;
+  // gather mode: all elements of the block; coloured mode: one colour through elist
+  const int64_t blk = blockIdx.x;
+  const int64_t ie = e_begin + blk * EPB + slot;
+  const bool active = ie < e_end;
+  const int64_t e = (a.elist && active) ? int64_t(a.elist[ie]) : ie;
 
   // ---- phase 0: gather
   if (active && lane < NNE) {
@@ -235,7 +238,7 @@ __global__ __launch_bounds__(256) void elem_kernel(ElemArgs a, int64_t e_begin, 
       s_u[slot][lane * 3 + b] = u;
     }
   }
-  __syncthreads();
+  lds_barrier();
 
   // ---- phase 1: quadrature points
   if (active && lane < NQP) {
@@ -344,7 +347,7 @@ __global__ __launch_bounds__(256) void elem_kernel(ElemArgs a, int64_t e_begin, 
       for (int i = 0; i < 3; ++i) hq[n * 3 + i] = Fi[0 * 3 + i] * d0 + Fi[1 * 3 + i] * d1 + Fi[2 * 3 + i] * d2;
     }
   }
-  __syncthreads();
+  lds_barrier();
 
   // ---- phase 2: element rows (lanes a*3+b < NDE of live elements)
   const bool rowlane = active && lane < NDE;
@@ -421,10 +424,12 @@ __global__ __launch_bounds__(256) void elem_kernel(ElemArgs a, int64_t e_begin, 
   if constexpr (TAN && STG) {
     // Gather mode: RE elements at a time, the rows go to LDS laid out as the staging blocks
     // ([node a][row b][K row, residual]); then the whole workgroup copies each node block
-    // (3 (NDE+1) consecutive doubles) to its staging offset, consecutive threads on
-    // consecutive doubles.  Rows of prescribed DOFs are not stored (nothing reads them).
-    __syncthreads();  // every lane is done with s_h / s_c
-    const int64_t e_blk = e_begin + int64_t(blockIdx.x) * EPB;
+    // (3 (NDE+1) doubles, zero-padded to whole lines) to its staging offset, consecutive
+    // threads on consecutive doubles.  Rows of prescribed DOFs are copied too (nothing reads
+    // them; it keeps every line whole); nodes without free DOFs have no block.
+    constexpr int PB = stage_block(NNE);
+    lds_barrier();  // every lane is done with s_h / s_c
+    const int64_t e_blk = e_begin + blk * EPB;
 #pragma unroll 1
     for (int r0 = 0; r0 < EPB; r0 += RE) {
       if (rowlane && slot >= r0 && slot < r0 + RE) {
@@ -433,15 +438,18 @@ __global__ __launch_bounds__(256) void elem_kernel(ElemArgs a, int64_t e_begin, 
         for (int i = 0; i < NDE; ++i) row[i] = K[i];
         row[NDE] = fe;
       }
-      __syncthreads();
-      for (int idx = threadIdx.x; idx < RE * NDE * LD; idx += 256) {
-        const int el = idx / (NDE * LD), o = idx - el * (NDE * LD);
-        const int na = o / (3 * LD), ob = o - na * 3 * LD;
+      lds_barrier();
+      for (int idx = threadIdx.x; idx < RE * NNE * PB; idx += 256) {
+        const int el = idx / (NNE * PB), o = idx - el * (NNE * PB);
+        const int na = o / PB, ob = o - na * PB;
         const int64_t ee = e_blk + r0 + el;
-        if (ee >= e_end || s_dof[r0 + el][na * 3 + ob / LD] <= 0) continue;
-        a.stage[a.soff[ee * NNE + na] + ob] = s_work[idx];
+        if (ee >= e_end) continue;
+        const int64_t so = a.soff[ee * NNE + na];
+        if (so < 0) continue;
+        // streamed past L2: the gather reads it in a later launch
+        __builtin_nontemporal_store((ob < 3 * LD) ? s_work[(el * NDE + na * 3) * LD + ob] : 0.0, a.stage + so + ob);
       }
-      __syncthreads();
+      lds_barrier();
     }
     return;
   }
@@ -498,12 +506,14 @@ __global__ __launch_bounds__(256) void elem_kernel(ElemArgs a, int64_t e_begin, 
 }
 
 // Gather assembly, one wavefront per node n with free DOFs: the node's rows (nr free DOFs x
-// rowlen columns, contiguous in the CSR values) are summed in LDS over the node's staging
-// blocks, then stored once; lanes 0-2 sum the residual entries.  Blocks are taken four at a
-// time: every lane first loads its items (at most two per block: a block has 3 x 3nne <= 90
-// items) of all four, then the four blocks' LDS adds follow in block order.  Within one block
-// every (row, column) target is distinct, so the lanes' adds never collide, and the wave
-// barrier between blocks keeps their order -- the sums run in the same order on every call.
+// rowlen columns, contiguous in the CSR values) and residual entries are summed in LDS over
+// the node's staging blocks, then stored once.  Each half-wave takes one block at a time
+// (blocks k, k+1 side by side); lane c < 3nne of a half owns element column c -- it decodes
+// the column's position once and loads the column's three rows -- and lane 3nne the three
+// residual entries.  Blocks are loaded kGatherBatch pairs ahead; the adds then follow in
+// block order (the lower half's block before the upper half's, with a wave barrier between),
+// and within one block every target is distinct: the sums run in the same order on every
+// call.  Elements have at most 10 nodes (3nne + 1 <= 32 lanes).
 constexpr int kGatherBatch = 4;
 
 template <bool RES, bool TAN>
@@ -520,59 +530,61 @@ __global__ __launch_bounds__(256) void gather_kernel(GatherArgs g) {
   if (fm == 0) return;
   const int nr = __popc(fm);
   const int len = g.rowlen[node];
-  double *acc = s_acc + int64_t(wid) * 3 * g.maxlen;
-  if constexpr (TAN) {
+  double *acc = s_acc + int64_t(wid) * (3 * g.maxlen + 3);
+  double *facc = acc + 3 * g.maxlen;  // residual entries of the node's rows
+  if constexpr (TAN)
     for (int i = lane; i < nr * len; i += 64) acc[i] = 0.0;
-    __builtin_amdgcn_wave_barrier();
-  }
-  const bool frow = lane < 3 && ((fm >> lane) & 1);
-  double fs = 0.0;
-  const int64_t k1 = g.inc_ptr[node + 1];
-  for (int64_t k0 = g.inc_ptr[node]; k0 < k1; k0 += kGatherBatch) {
-    double v[kGatherBatch][2], fv[kGatherBatch];
-    int dst[kGatherBatch][2];
+  if (lane < 3) facc[lane] = 0.0;
+  __builtin_amdgcn_wave_barrier();
+  int rowoff[3];  // offset of row b in the node's values (b free)
+#pragma unroll
+  for (int b = 0; b < 3; ++b) rowoff[b] = __popc(fm & ((1 << b) - 1)) * len;
+  const int half = lane >> 5, c = lane & 31;
+  const int64_t kb = g.inc_ptr[node], ke = g.inc_ptr[node + 1];
+  for (int64_t k0 = kb; k0 < ke; k0 += 2 * kGatherBatch) {
+    double v[kGatherBatch][3];
+    int dst[kGatherBatch];  // column in the rows, -2 = residual entries, -1 = nothing
 #pragma unroll
     for (int j = 0; j < kGatherBatch; ++j) {
-      dst[j][0] = dst[j][1] = -1;
-      fv[j] = 0.0;
-      const int64_t k = k0 + j;
-      if (k >= k1) continue;
-      const double *blk = g.stage + g.inc_soff[k];
-      const int nde = 3 * g.inc_nne[k], ld = nde + 1;
-      if (RES && frow) fv[j] = blk[lane * ld + nde];
-      if constexpr (TAN) {
-        const uint32_t *pos = g.pos + g.inc_poff[k];
+      dst[j] = -1;
+      const int64_t k = k0 + 2 * j + half;
+      if (k >= ke) continue;
+      const int nde = 3 * g.inc_nne[k];
+      const double *blk = g.stage + g.inc_soff[k] + c;
+      if (TAN && c < nde) {
+        const int w = c / 3, gg = c - 3 * w;
+        const uint32_t p = g.pos[g.inc_poff[k] + w];
+        const int mw = int(p >> 28);
+        if ((mw >> gg) & 1) dst[j] = int(p & 0x0fffffffu) + __popc(mw & ((1 << gg) - 1));
+      } else if (RES && c == nde) {
+        dst[j] = -2;
+      }
+      if (dst[j] != -1) {
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-          const int it = lane + 64 * h;
-          v[j][h] = 0.0;
-          if (it >= 3 * nde) continue;
-          const int b = it / nde, col = it - b * nde;
-          const int w = col / 3, gg = col - 3 * w;
-          const uint32_t p = pos[w];
-          const int mw = int(p >> 28);
-          if (!((fm >> b) & 1) || !((mw >> gg) & 1)) continue;
-          dst[j][h] = __popc(fm & ((1 << b) - 1)) * len + int(p & 0x0fffffffu) + __popc(mw & ((1 << gg) - 1));
-          v[j][h] = blk[b * ld + col];
-        }
+        for (int b = 0; b < 3; ++b)
+          v[j][b] = ((fm >> b) & 1) ? __builtin_nontemporal_load(blk + b * (nde + 1)) : 0.0;
       }
     }
 #pragma unroll
-    for (int j = 0; j < kGatherBatch; ++j) {
-      if constexpr (RES) fs += fv[j];
-      if constexpr (TAN) {
+    for (int j = 0; j < kGatherBatch; ++j)
 #pragma unroll
-        for (int h = 0; h < 2; ++h)
-          if (dst[j][h] >= 0) acc[dst[j][h]] += v[j][h];
+      for (int h = 0; h < 2; ++h) {
+        if (half == h && dst[j] != -1) {
+#pragma unroll
+          for (int b = 0; b < 3; ++b) {
+            if (!((fm >> b) & 1)) continue;
+            if (dst[j] >= 0) acc[rowoff[b] + dst[j]] += v[j][b];
+            else facc[b] += v[j][b];
+          }
+        }
         __builtin_amdgcn_wave_barrier();
       }
-    }
   }
   if constexpr (TAN) {
     double *out = g.vals + g.row_start[node];
-    for (int i = lane; i < nr * len; i += 64) out[i] = acc[i];
+    for (int i = lane; i < nr * len; i += 64) __builtin_nontemporal_store(acc[i], out + i);
   }
-  if (RES && frow) g.f[g.dofid[node * 3 + lane] - 1] = fs;
+  if (RES && lane < 3 && ((fm >> lane) & 1)) g.f[g.dofid[node * 3 + lane] - 1] = facc[lane];
 }
 
 __global__ void fill_kernel(double *p, int64_t n, double v) {
@@ -624,7 +636,7 @@ void launch_elem(int nne, int mode, const ElemArgs &a, int64_t e_begin, int64_t 
 }
 
 int gather_waves(int32_t maxlen) {
-  const int64_t per_wave = int64_t(3) * std::max<int32_t>(maxlen, 1) * int64_t(sizeof(double));
+  const int64_t per_wave = (int64_t(3) * std::max<int32_t>(maxlen, 1) + 3) * int64_t(sizeof(double));
   const int64_t w = (int64_t(64) << 10) / per_wave;  // 64 KiB of LDS per workgroup
   return int(std::min<int64_t>(4, w));
 }
@@ -633,7 +645,7 @@ void launch_gather(int mode, const GatherArgs &g, hipStream_t s) {
   const int wpb = gather_waves(g.maxlen);
   if (g.n_nodes <= 0 || wpb < 1) return;
   const unsigned grid = unsigned((g.n_nodes + wpb - 1) / wpb);
-  const size_t lds = size_t(wpb) * 3 * std::max<int32_t>(g.maxlen, 1) * sizeof(double);
+  const size_t lds = size_t(wpb) * (3 * size_t(std::max<int32_t>(g.maxlen, 1)) + 3) * sizeof(double);
   const bool res = (mode & MODE_RESIDUAL) != 0, tan = (mode & MODE_TANGENT) != 0;
   if (res && tan) gather_kernel<true, true><<<grid, 64 * wpb, lds, s>>>(g);
   else if (res) gather_kernel<true, false><<<grid, 64 * wpb, lds, s>>>(g);

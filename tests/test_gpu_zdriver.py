@@ -4,7 +4,8 @@ starts from the converged field of the step before plus the homogeneous incremen
 prescribed stretch, on both sides.  The nodal displacements must agree within 1e-9 relative
 after every load step (both Newton loops run to a 1e-12 normalised residual), and the two
 line searches must take the same step lengths (1e-6; the step length is a smooth function of
-residual norms that agree to ~1e-12) and iteration counts within one.  The driver's
+residual norms that agree to ~1e-12) wherever the residual is above rounding level, and
+iteration counts within one.  The driver's
 subdivision of a failing step is exercised separately."""
 import numpy as np
 import pytest
@@ -51,8 +52,11 @@ def _run(prob, device, stretches, solver="device", cp=False):
         assert rel < TOL_U, (s, rel)
         # an iterative solve at 1e-13 may move the last norm across the tolerance
         assert abs(res.iterations - it_cpu) <= 1, (s, res.norms, res.gammas, gam_cpu)
+        # step lengths taken from a residual at rounding level (< 1e-9 of the first) are
+        # decided by the rounding of the sums, which differs between the two assemblies
         k = min(len(res.gammas), len(gam_cpu))
-        np.testing.assert_allclose(res.gammas[:k], gam_cpu[:k], rtol=1e-6)
+        live = np.asarray(res.norms[:k]) > 1e-9
+        np.testing.assert_allclose(np.asarray(res.gammas[:k])[live], np.asarray(gam_cpu[:k])[live], rtol=1e-6)
         iters.append((res.iterations, it_cpu, min(gam_cpu, default=1.0)))
     h.close()
     return iters
