@@ -868,7 +868,15 @@ __global__ __launch_bounds__(64) void cp_tangent(CPArgs a, int64_t nqb) {
 }
 
 // ------------------------------------------------------------------- element kernel
-template <int NNE, int MODE>
+// Elements per round of the LDS transpose of the staged rows (gather mode): the largest
+// divisor of epb whose rows (nde x (nde+1) doubles each) fit in the dead dM/dF array.
+constexpr int cp_stage_round(int epb, int nde, int room) {
+  int re = epb;
+  while (re > 1 && (epb % re != 0 || re * nde * (nde + 1) > room)) --re;
+  return re;
+}
+
+template <int NNE, int MODE, bool STG>
 __global__ __launch_bounds__(256) void cp_elem(CPElemArgs a, int64_t e_begin, int64_t e_end) {
   constexpr int NQP = nqp_of(NNE);
   constexpr int NDE = NNE * 3;
@@ -975,17 +983,18 @@ __global__ __launch_bounds__(256) void cp_elem(CPElemArgs a, int64_t e_begin, in
     }
   }
   __syncthreads();
-  if (!(active && lane < NDE)) return;
+  const bool rowlane = active && lane < NDE;
   const int ra = lane / 3, rb = lane % 3;
   const double eb[3] = {rb == 0 ? 1.0 : 0.0, rb == 1 ? 1.0 : 0.0, rb == 2 ? 1.0 : 0.0};
-  const DevGrain &gr = a.grains[a.grain[e]];
   double K[TAN ? NDE : 1];
   if constexpr (TAN) {
 #pragma unroll
     for (int i = 0; i < NDE; ++i) K[i] = 0.0;
   }
   double fe = 0.0;
-#pragma unroll
+  if (rowlane) {
+  const DevGrain &gr = a.grains[a.grain[e]];
+#pragma unroll 1  // unrolled QP bodies interleave and exhaust the registers (occupancy 1)
   for (int ip = 0; ip < NQP; ++ip) {
     const double wq = s_w[slot][ip];
     const double *dN = s_dN[slot][ip];
@@ -1062,10 +1071,45 @@ __global__ __launch_bounds__(256) void cp_elem(CPElemArgs a, int64_t e_begin, in
     for (int i = 0; i < NDE; ++i) diag = (i == lane) ? K[i] : diag;
     if (!isnormal(diag)) atomicAdd(a.err, 1);
   }
+  }  // rowlane
+  constexpr int LD = NDE + 1;
+  if constexpr (TAN && STG && NNE != 4) {
+    // Gather mode: RE elements at a time, the rows go to LDS (in the dead dM/dF array) laid
+    // out as the staging blocks, then the workgroup copies each node block, zero-padded to
+    // whole lines, to its staging offset with consecutive threads on consecutive doubles.
+    // (Linear tets store their 13-double rows directly: the transpose rounds cost more.)
+    constexpr int PB = stage_block(NNE);
+    constexpr int RE = cp_stage_round(EPB, NDE, EPB * NQP * 81);
+    static_assert(RE * NDE * LD <= EPB * NQP * 81, "staging round does not fit");
+    double *s_st = &s_D[0][0][0];
+    lds_barrier();  // every lane is done with s_D
+#pragma unroll 1
+    for (int r0 = 0; r0 < EPB; r0 += RE) {
+      if (rowlane && slot >= r0 && slot < r0 + RE) {
+        double *row = s_st + ((slot - r0) * NDE + lane) * LD;
+#pragma unroll
+        for (int i = 0; i < NDE; ++i) row[i] = K[i];
+        row[NDE] = fe;
+      }
+      lds_barrier();
+      for (int idx = threadIdx.x; idx < RE * NNE * PB; idx += blockDim.x) {
+        const int el = idx / (NNE * PB), o = idx - el * (NNE * PB);
+        const int na = o / PB, ob = o - na * PB;
+        const int64_t ee = e_begin + int64_t(blockIdx.x) * EPB + r0 + el;
+        if (ee >= e_end) continue;
+        const int64_t so = a.soff[ee * NNE + na];
+        if (so < 0) continue;
+        __builtin_nontemporal_store((ob < 3 * LD) ? s_st[(el * NDE + na * 3) * LD + ob] : 0.0, a.stage + so + ob);
+      }
+      lds_barrier();
+    }
+    return;
+  }
+  if (!rowlane) return;
   const int32_t dof = s_dof[slot][lane];
   if (dof <= 0) return;
-  if (a.stage) {  // gather mode: row (ra, rb) into the node's staging block
-    double *srow = a.stage + a.soff[e * NNE + ra] + rb * (NDE + 1);
+  if constexpr (STG) {  // gather mode: the row straight into its staging block
+    double *srow = a.stage + a.soff[e * NNE + ra] + rb * LD;
     if constexpr (TAN) {
 #pragma unroll
       for (int i = 0; i < NDE; ++i) srow[i] = K[i];
@@ -1128,7 +1172,9 @@ void launch_ce(const CPElemArgs &a, int64_t e0, int64_t e1, hipStream_t s) {
   constexpr int EPB = (NNE == 8) ? 4 : 256 / TPE;
   const int64_t n = e1 - e0;
   if (n <= 0) return;
-  cp_elem<NNE, MODE><<<unsigned((n + EPB - 1) / EPB), EPB * TPE, 0, s>>>(a, e0, e1);
+  const unsigned grid = unsigned((n + EPB - 1) / EPB);
+  if (a.stage) cp_elem<NNE, MODE, true><<<grid, EPB * TPE, 0, s>>>(a, e0, e1);
+  else cp_elem<NNE, MODE, false><<<grid, EPB * TPE, 0, s>>>(a, e0, e1);
 }
 
 template <int NNE>
