@@ -127,8 +127,9 @@ int pgf_mesh_box(int32_t n, int32_t etype, double jitter, uint64_t seed, int64_t
 typedef struct pgf_handle_s *pgf_handle;
 
 /* Create the device-side model: copies mesh, DOF ids, CSR pattern and materials to HBM,
- * colours the elements (no two elements of one colour share a node) and builds the
- * element -> CSR scatter map.  The pattern must be the one pgf_psparse_apai returns.
+ * builds the node -> staging-block tables of the gather assembly, colours the elements (no
+ * two elements of one colour share a node) and builds the element -> CSR scatter map of the
+ * coloured assembly (see pgf_set_assembly).  The pattern must be the one pgf_psparse_apai returns.
  * Replaces the per-element set-up of stiffmat_fd_MP / fd_residuals_MP
  * (src/stiffmat_fd.cc:831-993, src/fd_residuals.cc:468-636) and the per-QP
  * Constitutive_model objects (include/constitutive_model.h:766-844). */
