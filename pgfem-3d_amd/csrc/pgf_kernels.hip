@@ -51,6 +51,15 @@ __device__ __forceinline__ void vol_pot(int flag, double J, double &dU, double &
   }
 }
 
+// Sum over aligned groups of W lanes (butterfly: every lane ends with the same bits, since
+// each step adds the same two values in either order).
+template <int W>
+__device__ __forceinline__ double group_sum(double v) {
+#pragma unroll
+  for (int o = W / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, W);
+  return v;
+}
+
 // symmetric 3x3 in Voigt (xx,yy,zz,xy,yz,xz) times vector
 __device__ __forceinline__ void symv(const double *s, const double *x, double *y) {
   y[0] = s[0] * x[0] + s[3] * x[1] + s[5] * x[2];
@@ -58,15 +67,42 @@ __device__ __forceinline__ void symv(const double *s, const double *x, double *y
   y[2] = s[5] * x[0] + s[4] * x[1] + s[2] * x[2];
 }
 
+// What the Voigt Kirchhoff tangent c of a material point is made of (j2_point):
+//   c_IK = cv1 oo_IK + cv2 Is_IK + a1 cbar_IK - tm b3 n_I n_K - tm b4 n_I dn2_K,
+//   cbar_IK = tm (Is_IK - oo_IK / 3) - 2/3 (s_tr_I 1_K + 1_I s_tr_K)
+// (oo: 1 (x) 1 on the normal components, Is: the symmetric identity in Voigt form).
+struct J2Tan {
+  double cv1, cv2, a1, tm, b3, b4, s_tr[6], n[6], dn2[6];
+};
+
+// v[I] for a run-time I (selects, so v stays in registers)
+__device__ __forceinline__ double pick6(double v0, double v1, double v2, double v3, double v4, double v5, int I) {
+  return I == 0 ? v0 : I == 1 ? v1 : I == 2 ? v2 : I == 3 ? v3 : I == 4 ? v4 : v5;
+}
+
+// Row I of c (row = stress component, column = engineering strain component).
+__device__ __forceinline__ void j2_c_row(const J2Tan &T, int I, double *row) {
+  const double nI = pick6(T.n[0], T.n[1], T.n[2], T.n[3], T.n[4], T.n[5], I);
+  const double sI = pick6(T.s_tr[0], T.s_tr[1], T.s_tr[2], T.s_tr[3], T.s_tr[4], T.s_tr[5], I);
+  const double nrm_I = I < 3 ? 1.0 : 0.0;
+#pragma unroll
+  for (int K = 0; K < 6; ++K) {
+    const double oo = (I < 3 && K < 3) ? 1.0 : 0.0;
+    const double Is = (I == K) ? (I < 3 ? 1.0 : 0.5) : 0.0;
+    const double cbar = T.tm * (Is - oo * (1.0 / 3.0)) - (2.0 / 3.0) * (sI * (K < 3 ? 1.0 : 0.0) + nrm_I * T.s_tr[K]);
+    row[K] = T.cv1 * oo + T.cv2 * Is + T.a1 * cbar - T.tm * T.b3 * nI * T.n[K] - T.tm * T.b4 * nI * T.dn2[K];
+  }
+}
+
 // Neo-Hookean / J2 material point in the current configuration.
-// In: Fs, its inverse and J; J2 state (Cp^-1 at n, ep_n).  Out: tau (Voigt), c (Voigt 6x6,
-// row I = stress component, column J = engineering strain component) and, when
-// integrating, the n+1 state.  HE is the J2 elastic branch with Cp^-1 = I.
+// In: Fs, its inverse and J; J2 state (Cp^-1 at n, ep_n).  Out: tau (Voigt), the pieces of
+// the tangent c (J2Tan, when WITH_C) and, when integrating, the n+1 state.  HE is the J2
+// elastic branch with Cp^-1 = I.
 template <bool WITH_C>
 __device__ __forceinline__ void j2_point(const DevMaterial &m, bool plastic, const double *Fs,
                                          const double *Fi, double J, const double *Cn,
                                          double ep_n, double *Cnp1, double *ep1, double *gam,
-                                         double *tau, double *c) {
+                                         double *tau, J2Tan &tg) {
   // b_bar trial = J^-2/3 Fs Cn Fs^T (Voigt)
   double T[9];
 #pragma unroll
@@ -161,17 +197,18 @@ __device__ __forceinline__ void j2_point(const DevMaterial &m, bool plastic, con
 #pragma unroll
       for (int I = 0; I < 6; ++I) dn2[I] = n2[I] - (I < 3 ? t3 : 0.0);
     }
-    const double a1 = 1.0 - beta1, tm = 2.0 * mubar;
+    tg.cv1 = cv1;
+    tg.cv2 = cv2;
+    tg.a1 = 1.0 - beta1;
+    tg.tm = 2.0 * mubar;
+    tg.b3 = beta3;
+    tg.b4 = beta4;
 #pragma unroll
-    for (int I = 0; I < 6; ++I)
-#pragma unroll
-      for (int K = 0; K < 6; ++K) {
-        const double oo = (I < 3 && K < 3) ? 1.0 : 0.0;
-        const double Is = (I == K) ? (I < 3 ? 1.0 : 0.5) : 0.0;
-        const double cbar = tm * (Is - oo * (1.0 / 3.0)) -
-                            (2.0 / 3.0) * (s_tr[I] * (K < 3 ? 1.0 : 0.0) + (I < 3 ? 1.0 : 0.0) * s_tr[K]);
-        c[I * 6 + K] = cv1 * oo + cv2 * Is + a1 * cbar - tm * beta3 * n[I] * n[K] - tm * beta4 * n[I] * dn2[K];
-      }
+    for (int I = 0; I < 6; ++I) {
+      tg.s_tr[I] = s_tr[I];
+      tg.n[I] = n[I];
+      tg.dn2[I] = dn2[I];
+    }
   }
 }
 
@@ -240,9 +277,14 @@ __global__ __launch_bounds__(256) void elem_kernel(ElemArgs a, int64_t e_begin, 
   }
   lds_barrier();
 
-  // ---- phase 1: quadrature points
-  if (active && lane < NQP) {
-    const int ip = lane;
+  // ---- phase 1: quadrature points, LPQ lanes per QP (lane = ip * LPQ + sub).  Node loops
+  // are split over the QP's lanes (node n on sub-lane n % LPQ, which alone reads and writes
+  // its entries of s_h) and summed with butterfly shuffles, which leave every lane with the
+  // same bits; the return map runs on every lane; row I of c goes to sub-lanes I, I+LPQ, ...
+  constexpr int LPQ = TPE / NQP;
+  static_assert(LPQ * NQP == TPE && (LPQ & (LPQ - 1)) == 0, "lanes per QP");
+  if (active) {
+    const int ip = lane / LPQ, sub = lane % LPQ;
     double k, ee, z, wt;
     int_point(NNE, ip, k, ee, z, wt);
     double *hq = s_h[slot][ip];  // dN/dX first, h_a = Fs^-T dN_a/dX at the end
@@ -250,13 +292,17 @@ __global__ __launch_bounds__(256) void elem_kernel(ElemArgs a, int64_t e_begin, 
       double dN[NDE];
       shape_dN<NNE>(k, ee, z, dN);
 #pragma unroll
-      for (int i = 0; i < NDE; ++i) hq[i] = dN[i];
+      for (int n = 0; n < NNE; ++n)
+        if (n % LPQ == sub) {
+          hq[n * 3 + 0] = dN[n * 3 + 0];
+          hq[n * 3 + 1] = dN[n * 3 + 1];
+          hq[n * 3 + 2] = dN[n * 3 + 2];
+        }
     }
     double dX[9];
 #pragma unroll
     for (int i = 0; i < 9; ++i) dX[i] = 0.0;
-#pragma unroll
-    for (int n = 0; n < NNE; ++n) {
+    for (int n = sub; n < NNE; n += LPQ) {
       const double d0 = hq[n * 3 + 0], d1 = hq[n * 3 + 1], d2 = hq[n * 3 + 2];
 #pragma unroll
       for (int ib = 0; ib < 3; ++ib) {
@@ -266,6 +312,8 @@ __global__ __launch_bounds__(256) void elem_kernel(ElemArgs a, int64_t e_begin, 
         dX[ib * 3 + 2] += d2 * xb;
       }
     }
+#pragma unroll
+    for (int i = 0; i < 9; ++i) dX[i] = group_sum<LPQ>(dX[i]);
     const double J0 = det3(dX);
     bool ok = J0 > 0.0;
     double cof[9];
@@ -279,9 +327,10 @@ __global__ __launch_bounds__(256) void elem_kernel(ElemArgs a, int64_t e_begin, 
     cof[7] = dX[1] * dX[6] - dX[0] * dX[7];
     cof[8] = dX[0] * dX[4] - dX[1] * dX[3];
     const double rJ0 = 1.0 / J0;
-    double Fr[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    double Fr[9];  // sum of u_n (x) dN_n/dX over the lane's nodes, then over the QP's lanes
 #pragma unroll
-    for (int n = 0; n < NNE; ++n) {
+    for (int i = 0; i < 9; ++i) Fr[i] = 0.0;
+    for (int n = sub; n < NNE; n += LPQ) {
       const double d0 = hq[n * 3 + 0], d1 = hq[n * 3 + 1], d2 = hq[n * 3 + 2];
       double g[3];
 #pragma unroll
@@ -294,6 +343,8 @@ __global__ __launch_bounds__(256) void elem_kernel(ElemArgs a, int64_t e_begin, 
         for (int ic = 0; ic < 3; ++ic) Fr[ib * 3 + ic] += uu * g[ic];
       }
     }
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Fr[i] = group_sum<LPQ>(Fr[i]) + ((i % 4 == 0) ? 1.0 : 0.0);
     const int64_t q = a.qp_base + e * NQP + ip;  // e is the block-local element index
     const int64_t NQ = a.n_qp;
     double Fs[9];
@@ -301,7 +352,7 @@ __global__ __launch_bounds__(256) void elem_kernel(ElemArgs a, int64_t e_begin, 
 #pragma unroll
       for (int i = 0; i < 9; ++i) {
         Fs[i] = Fr[i];
-        a.F_np1[i * NQ + q] = Fr[i];
+        if (i % LPQ == sub) a.F_np1[i * NQ + q] = Fr[i];
       }
     } else {
 #pragma unroll
@@ -311,7 +362,7 @@ __global__ __launch_bounds__(256) void elem_kernel(ElemArgs a, int64_t e_begin, 
     double Fi[9];
     ok = ok && (J > 0.0) && inv3(Fs, Fi);
     if (!ok) {
-      atomicAdd(a.err, 1);
+      if (sub == 0) atomicAdd(a.err, 1);
 #pragma unroll
       for (int i = 0; i < 9; ++i) Fi[i] = (i % 4 == 0) ? 1.0 : 0.0;
     }
@@ -323,25 +374,39 @@ __global__ __launch_bounds__(256) void elem_kernel(ElemArgs a, int64_t e_begin, 
       for (int I = 0; I < 6; ++I) Cn[I] = a.P_n[I * NQ + q];
       ep_n = a.Sc_n[q];
     }
-    double Cnp1[6], ep1, gm;
+    double Cnp1[6], ep1, gm, tau[6];
+    J2Tan T;
     j2_point<TAN>(m, is_j2, Fs, Fi, ok ? J : 1.0, Cn, ep_n, (INTEG && is_j2) ? Cnp1 : nullptr, &ep1,
-                  &gm, s_tau[slot][ip], TAN ? s_c[slot][ip] : nullptr);
-    if (INTEG && is_j2) {
+                  &gm, tau, T);
 #pragma unroll
-      for (int I = 0; I < 6; ++I) a.P_np1[I * NQ + q] = Cnp1[I];
-      a.Sc_np1[q] = ep1;
-      a.Sc_np1[NQ + q] = gm;
+    for (int I = 0; I < 6; ++I)
+      if (I % LPQ == sub) {
+        s_tau[slot][ip][I] = tau[I];
+        if (INTEG && is_j2) a.P_np1[I * NQ + q] = Cnp1[I];
+      }
+    if (sub == 0) {
+      if (INTEG && is_j2) {
+        a.Sc_np1[q] = ep1;
+        a.Sc_np1[NQ + q] = gm;
+      }
+      s_w[slot][ip] = ok ? J0 * wt : 0.0;
     }
-    s_w[slot][ip] = ok ? J0 * wt : 0.0;
+    if constexpr (TAN) {
+#pragma unroll
+      for (int t = 0; t < (6 + LPQ - 1) / LPQ; ++t) {
+        const int I = sub + t * LPQ;
+        if (I < 6) j2_c_row(T, I, &s_c[slot][ip][I * 6]);
+      }
+    }
     if constexpr (GEN) {  // Gr = Fr Fs^-1
 #pragma unroll
       for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int j = 0; j < 3; ++j)
-          s_g[slot][ip][i * 3 + j] = Fr[i * 3 + 0] * Fi[0 * 3 + j] + Fr[i * 3 + 1] * Fi[1 * 3 + j] + Fr[i * 3 + 2] * Fi[2 * 3 + j];
+          if ((i * 3 + j) % LPQ == sub)
+            s_g[slot][ip][i * 3 + j] = Fr[i * 3 + 0] * Fi[0 * 3 + j] + Fr[i * 3 + 1] * Fi[1 * 3 + j] + Fr[i * 3 + 2] * Fi[2 * 3 + j];
     }
-#pragma unroll
-    for (int n = 0; n < NNE; ++n) {
+    for (int n = sub; n < NNE; n += LPQ) {
       const double d0 = hq[n * 3 + 0], d1 = hq[n * 3 + 1], d2 = hq[n * 3 + 2];
 #pragma unroll
       for (int i = 0; i < 3; ++i) hq[n * 3 + i] = Fi[0 * 3 + i] * d0 + Fi[1 * 3 + i] * d1 + Fi[2 * 3 + i] * d2;
