@@ -63,7 +63,10 @@ typedef struct pgf_material {
   int32_t devPotFlag;  /* 1 = Mooney-Rivlin (src/new_potentials.cc:106-127) */
   int32_t volPotFlag;  /* 2 = Doll-Schweizerhof 7, 3 = DS 8, 99 = quadratic
                           (src/new_potentials.cc:184-210) */
-  int32_t pad0;
+  int32_t cp_elastic;  /* crystal plasticity, elastic law of the intermediate configuration:
+                          0 = cubic C11/C12/C44 St Venant-Kirchhoff (src/tensors.cc:49-75),
+                          1 = the devPotFlag/volPotFlag potentials on eF, as the CM model
+                          builds them (src/constitutive_model.cc:998-1014) */
   double E, nu, G, m10, m01;
   /* J2 plasticity, linear isotropic hardening (CM_J2P_PARAM PARAM_hp / PARAM_k0,
    * src/cm_j2_plasticity.cc:86-103): k0 = initial uniaxial yield stress, hp = linear

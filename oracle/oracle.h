@@ -11,7 +11,7 @@
 extern "C" {
 
 struct orc_material {  // same field order as pgf_material (include/pgfem3d_amd.h)
-  int32_t model, devPotFlag, volPotFlag, pad0;
+  int32_t model, devPotFlag, volPotFlag, cp_elastic;
   double E, nu, G, m10, m01;
   double k0, hp;
   double gamma_dot_0, m_rate, G0, g0, gs_0, gamma_dot_s, w_sat;

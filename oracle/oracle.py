@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
 
 class OrcMaterial(C.Structure):
     _fields_ = [
-        ("model", C.c_int32), ("devPotFlag", C.c_int32), ("volPotFlag", C.c_int32), ("pad0", C.c_int32),
+        ("model", C.c_int32), ("devPotFlag", C.c_int32), ("volPotFlag", C.c_int32), ("cp_elastic", C.c_int32),
         ("E", C.c_double), ("nu", C.c_double), ("G", C.c_double), ("m10", C.c_double), ("m01", C.c_double),
         ("k0", C.c_double), ("hp", C.c_double),
         ("gamma_dot_0", C.c_double), ("m_rate", C.c_double), ("G0", C.c_double), ("g0", C.c_double),

@@ -14,7 +14,11 @@
 //   tau_a    = (C_e S) : P_a, C_e = eF^T eF, eF = F pF(n+1)^-1, S = L : (C_e - I)/2
 //              (src/TA_GA.cc:21-31, get_SPK_stress src/stress_strain.cc:4-40)
 // with P_a = s_a (x) m_a of the 12 FCC {111}<110> systems rotated by the grain rotation R,
-// and L the cubic (C11, C12, C44) stiffness rotated by R.  The three equations are solved
+// and L the cubic (C11, C12, C44) stiffness rotated by R -- or, for a material with
+// cp_elastic = 1, S and L = dS/dE of the material's hyperelastic potentials (devPotFlag,
+// volPotFlag) evaluated at eF, as the CM model builds its elasticity
+// (src/constitutive_model.cc:998-1014, HyperElasticity::update_elasticity called on eF at
+// src/plasticity_model.cc:747-772; the potentials are orc_he_update's).  The three equations are solved
 // together by Newton with backtracking; GCM staggers g (max_itr_stag) around the M solve,
 // which has the same converged point.  Plain start: the reference's fully plastic
 // predictor (initial_guess, src/initial_guess.cc:100-110: the elastic part of step n is
@@ -120,6 +124,22 @@ struct Grain {
   double L[81];     // rotated cubic stiffness
 };
 
+// Elastic PK2 S and tangent dS/dE of the intermediate configuration at eF (returns the
+// tangent: the grain's rotated cubic stiffness, or Lbuf filled from the potentials).
+const double *elastic_SL(const orc_material &m, const Grain &gr, const double *eF, double *S, double *Lbuf,
+                         bool *ok) {
+  *ok = true;
+  if (m.cp_elastic == 1) {
+    *ok = orc_he_update(&m, eF, S, Lbuf) == 0;
+    return Lbuf;
+  }
+  double Ce[9], E[9];
+  mtm(eF, eF, Ce);
+  for (int i = 0; i < 9; ++i) E[i] = 0.5 * (Ce[i] - ((i % 4 == 0) ? 1.0 : 0.0));
+  L_dot(gr.L, E, S);
+  return gr.L;
+}
+
 struct CPEval {
   double eF[9], Ce[9], S[9], tau[12], gdot[12], dgdt[12], dgdg[12];
   double Dtau[12][9];  // d tau_a / d M
@@ -141,9 +161,10 @@ bool cp_eval(const orc_material &m, const Grain &gr, const double *A /*F pFn^-1*
   ev.pJ = det_pFn / dM;
   mm(A, M, ev.eF);
   mtm(ev.eF, ev.eF, ev.Ce);
-  double E[9];
-  for (int i = 0; i < 9; ++i) E[i] = 0.5 * (ev.Ce[i] - ((i % 4 == 0) ? 1.0 : 0.0));
-  L_dot(gr.L, E, ev.S);
+  double Lbuf[81];
+  bool sok = true;
+  const double *Lp = elastic_SL(m, gr, ev.eF, ev.S, Lbuf, &sok);
+  if (!sok) return false;
   double Sig[9];
   mm(ev.Ce, ev.S, Sig);  // Mandel stress C_e S
   const double expo = 1.0 / m.m_rate - 1.0;
@@ -164,7 +185,7 @@ bool cp_eval(const orc_material &m, const Grain &gr, const double *A /*F pFn^-1*
     double PS[9], CP[9], LCP[9], Q[9];
     mm(gr.P[a], ev.S, PS);
     mm(ev.Ce, gr.P[a], CP);
-    L_dot(gr.L, CP, LCP);
+    L_dot(Lp, CP, LCP);
     for (int i = 0; i < 9; ++i) Q[i] = PS[i] + 0.5 * LCP[i];
     double sQ[9];
     for (int i = 0; i < 3; ++i)
@@ -238,12 +259,13 @@ void cp_jacobian(const orc_material &m, const Grain &gr, double dt, const double
 }
 
 // max_a |tau_a| / g at M (elastic part eF = A M).
-double tau_ratio_max(const Grain &gr, const double *A, const double *M, double g) {
-  double eF[9], Ce[9], E[9], S[9], Sig[9];
+double tau_ratio_max(const orc_material &m, const Grain &gr, const double *A, const double *M, double g) {
+  double eF[9], Ce[9], S[9], Sig[9], Lbuf[81];
   mm(A, M, eF);
   mtm(eF, eF, Ce);
-  for (int i = 0; i < 9; ++i) E[i] = 0.5 * (Ce[i] - ((i % 4 == 0) ? 1.0 : 0.0));
-  L_dot(gr.L, E, S);
+  bool ok = true;
+  elastic_SL(m, gr, eF, S, Lbuf, &ok);
+  if (!ok) return 2.0;  // no elastic state: treat as beyond yield
   mm(Ce, S, Sig);
   double mx = 0.0;
   for (int a = 0; a < 12; ++a) {
@@ -255,8 +277,8 @@ double tau_ratio_max(const Grain &gr, const double *A, const double *M, double g
 }
 
 // Plain start x = (M, lambda_n, g_n): the fully plastic predictor pulled back to yield.
-bool cp_start(const Grain &gr, const double *A, const double *F, const double *Fn, const double *pFn,
-              const double *pFni, double g_n, double lam_n, double *x) {
+bool cp_start(const orc_material &m, const Grain &gr, const double *A, const double *F, const double *Fn,
+              const double *pFn, const double *pFni, double g_n, double lam_n, double *x) {
   double Fi[9], T[9], Mp[9];
   if (!inv3(F, Fi)) return false;
   mm(pFn, Fi, T);
@@ -268,15 +290,15 @@ bool cp_start(const Grain &gr, const double *A, const double *F, const double *F
   for (int i = 0; i < 9; ++i) Mp[i] = T[i] * c;
   const double I9[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
   double al = 0.0;
-  if (tau_ratio_max(gr, A, I9, g_n) > 1.0) {
+  if (tau_ratio_max(m, gr, A, I9, g_n) > 1.0) {
     al = 1.0;
     double lo = 0.0, hi = 1.0;
-    if (!(tau_ratio_max(gr, A, Mp, g_n) > 1.0)) {
+    if (!(tau_ratio_max(m, gr, A, Mp, g_n) > 1.0)) {
       for (int k = 0; k < 8; ++k) {
         const double mid = 0.5 * (lo + hi);
         double Mm[9];
         for (int i = 0; i < 9; ++i) Mm[i] = I9[i] + mid * (Mp[i] - I9[i]);
-        if (tau_ratio_max(gr, A, Mm, g_n) > 1.0) lo = mid;
+        if (tau_ratio_max(m, gr, A, Mm, g_n) > 1.0) lo = mid;
         else hi = mid;
       }
       al = hi;
@@ -372,7 +394,7 @@ int orc_cp_update(const orc_material *mat, const double *P_grain, const double *
     std::memcpy(x, x0, sizeof(x));
   } else {
     const double I9[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
-    if (!cp_start(gr, A, F, Fn ? Fn : I9, pFn, pFni, g_n, lam_n, x)) return 1;
+    if (!cp_start(m, gr, A, F, Fn ? Fn : I9, pFn, pFni, g_n, lam_n, x)) return 1;
   }
   const double tol = (m.tol_M > 0.0) ? m.tol_M : 1.0e-10;
   const int maxit = (m.max_itr_M > 0) ? m.max_itr_M : 50;
@@ -593,12 +615,14 @@ extern "C" int orc_assemble_cp(int64_t n_nodes, const double *X, int64_t n_elem,
         double MeT[9];
         for (int i = 0; i < 3; ++i)
           for (int j = 0; j < 3; ++j) MeT[i * 3 + j] = Me[j * 3 + i];
-        // elastic PK2 from the stored state
-        double eF[9], Ce[9], E[9], S[9];
+        // elastic PK2 and tangent from the stored state
+        double eF[9], S[9], Lbuf[81];
         matmul_plain(Fs, Me, eF);
-        mtm(eF, eF, Ce);
-        for (int i = 0; i < 9; ++i) E[i] = 0.5 * (Ce[i] - ((i % 4 == 0) ? 1.0 : 0.0));
-        L_dot(gL, E, S);
+        Grain gel;
+        std::memcpy(gel.L, gL, sizeof(gel.L));
+        bool sok = true;
+        const double *Lel = elastic_SL(mat, gel, eF, S, Lbuf, &sok);
+        if (!sok) { ++err; continue; }
         if (pass == 0) {
           for (int a = 0; a < nne; a++)  // compute_residual_vector
             for (int b = 0; b < 3; b++) {
@@ -638,7 +662,7 @@ extern "C" int orc_assemble_cp(int64_t n_nodes, const double *X, int64_t n_elem,
           const double I9[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
           double FrTFr[9];
           mtm(Fr, Fr, FrTFr);
-          orc_stiffness_contract(lk.data(), nne, ST, detJxW, Fr, MeT, I9, Me, FrTFr, Me, S, gL, dMdu_all, pJ);
+          orc_stiffness_contract(lk.data(), nne, ST, detJxW, Fr, MeT, I9, Me, FrTFr, Me, S, Lel, dMdu_all, pJ);
         }
       }
     }

@@ -636,6 +636,7 @@ int pgf_create(const pgf_mesh *mesh, const int64_t *id, int64_t ndof, const int3
       ++n_cp;
       cp_index = i;
       if (!(mats[i].m_rate > 0.0) || !(mats[i].g0 > 0.0)) return PGF_EINVAL;
+      if (mats[i].cp_elastic != 0) return PGF_ENOTSUP;
     } else {
       return PGF_ENOTSUP;
     }
