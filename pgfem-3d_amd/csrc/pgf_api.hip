@@ -175,6 +175,7 @@ struct pgf_handle_s {
   double *d_qpbuf = nullptr;
   const double *warm = nullptr;
   double cp_g0 = 0.0, dt = 1.0;
+  int32_t cp_law = 0;  // elastic law of the crystal-plasticity material (pgf_material.cp_elastic)
   int64_t cp_nq = 0;         // crystal-plasticity QPs: device QPs [0, cp_nq)
   int64_t cp_chunk = 256;    // QPs per wave range of the local-Newton work queue (autotuned)
   int32_t *d_qgrain_all = nullptr;  // [n_qp] grain of every device QP
@@ -275,6 +276,7 @@ pgf::CPArgs cp_args(pgf_handle h, const Block &b, const pgf_nodal_input *in) {
   c.warm = h->warm;
   c.qpbuf = h->d_qpbuf + b.qp_base * pgf::kCPBuf;
   c.err = h->d_err;
+  c.law = h->cp_law;
   return c;
 }
 
@@ -442,6 +444,8 @@ int32_t enqueue_pass(pgf_handle h, const pgf_nodal_input *in, int mode, double *
         ce.grain = b.d_egrain;
         ce.off = b.d_off;
         ce.grains = h->d_grains;
+        ce.cpmats = h->d_cpmat;
+        ce.law = h->cp_law;
         ce.qpbuf = h->d_qpbuf + b.qp_base * pgf::kCPBuf;
         ce.qp_base = b.qp_base;
         ce.r = in->r;
@@ -636,7 +640,13 @@ int pgf_create(const pgf_mesh *mesh, const int64_t *id, int64_t ndof, const int3
       ++n_cp;
       cp_index = i;
       if (!(mats[i].m_rate > 0.0) || !(mats[i].g0 > 0.0)) return PGF_EINVAL;
-      if (mats[i].cp_elastic != 0) return PGF_ENOTSUP;
+      if (mats[i].cp_elastic == 1) {
+        // neo-Hookean deviatoric part only, as for the hyperelastic model
+        if (mats[i].m01 != 0.0 || mats[i].devPotFlag != 1) return PGF_ENOTSUP;
+        if (!(mats[i].m10 > 0.0) || !(mats[i].E > 0.0) || !(mats[i].nu < 0.5)) return PGF_EINVAL;
+      } else if (mats[i].cp_elastic != 0) {
+        return PGF_EINVAL;
+      }
     } else {
       return PGF_ENOTSUP;
     }
@@ -922,7 +932,12 @@ int pgf_create(const pgf_mesh *mesh, const int64_t *id, int64_t ndof, const int3
     c.w = cpm.w_sat;
     c.tol = cpm.tol_M > 0.0 ? cpm.tol_M : 1.0e-10;
     c.maxit = cpm.max_itr_M > 0 ? cpm.max_itr_M : 50;
+    c.m10 = cpm.m10;
+    c.kappa = cpm.E / (3.0 * (1.0 - 2.0 * cpm.nu));
+    c.law = cpm.cp_elastic;
+    c.vol_flag = cpm.volPotFlag;
     h->cp_g0 = cpm.g0;
+    h->cp_law = cpm.cp_elastic;
     h->cp_mat_host = {cpm.C11, cpm.C12, cpm.C44};
     if ((rc = pgf::upload(&h->d_cpmat, &c, 1))) return fail(rc);
     for (int t = 0; t < 2; ++t) {

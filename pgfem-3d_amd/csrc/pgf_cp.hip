@@ -7,7 +7,9 @@
 //   R_lambda = pJ - 1,  pJ = det(pF(n)) / det(M)                      (:88-95)
 //   R_g      = g (gs - g0 + dt h0 gdot) - (gs - g0) g_n - dt h0 gs gdot (:395-403)
 //   gdot_a   = gdot0 (tau_a/g)|tau_a/g|^(1/m-1), tau_a = (C_e S):P_a,
-//   S = L : (C_e - I)/2, C_e = eF^T eF, eF = F pF(n)^-1 M           (src/TA_GA.cc:8-41)
+//   C_e = eF^T eF, eF = F pF(n)^-1 M, and S either the cubic S = L : (C_e - I)/2
+//   (src/TA_GA.cc:8-41) or the hyperelastic law of the material line (he_law below); the
+//   kernels are instantiated per law
 // Plain start: the reference's fully plastic predictor (initial_guess,
 // src/initial_guess.cc:100-110: elastic part of step n kept, M_p = pF(n) F^-1 F(n) pF(n)^-1
 // scaled to det 1) pulled back along M(a) = I + a (M_p - I) to max |tau_a|/g_n = 1 (8
@@ -78,6 +80,78 @@ __device__ __forceinline__ void v6_to_9(const double *s, double *S) {
   S[2] = S[6] = s[5];
 }
 
+// Hyperelastic law of the intermediate configuration (law 1, as the CM model's elasticity on
+// eF, src/constitutive_model.cc:998-1014): neo-Hookean deviatoric part (devStress_Mooney_Rivlin
+// with m01 = 0, src/new_potentials.cc:329-358) and the volumetric potential of volPotFlag
+// (:575-616), kappa = E / (3 (1 - 2 nu)):
+//   S = 2 m10 d (I - tr(Ce)/3 Ci) + kappa J U'(J) Ci,   Ci = Ce^-1, d = det(Ce)^(-1/3), J = det eF
+// Its tangent L = dS/dE (matStiff_Mooney_Rivlin, :376-432, with the volumetric terms) is used
+// as a directional derivative on the symmetric part of X:
+//   L : X = c1 (Ci:X) Ci + c2 ((Ci:X) I + tr(X) Ci) + c3 Ci X Ci
+//   c1 = 4 m10 d tr(Ce)/9 + kappa (J U' + J^2 U''),  c2 = -4 m10 d / 3,  c3 = 4 m10 d tr(Ce)/3 - 2 kappa J U'
+struct HeTan {
+  double Ci[6], c1, c2, c3;
+};
+
+__device__ __forceinline__ void he_law(const CPMaterial &cm, const double *eF, const double *Ce, double *S6,
+                                       HeTan &t) {
+  const double J = det3(eF);
+  double Ci9[9];
+  inv3(Ce, Ci9);
+  const double d = pow(det3(Ce), -1.0 / 3.0);
+  const double trC = Ce[0] + Ce[4] + Ce[8];
+  double dU, d2U;
+  vol_pot(cm.vol_flag, J, dU, d2U);
+  const double a = 2.0 * cm.m10 * d, pv = cm.kappa * J * dU;
+  const double ci = pv - a * trC * (1.0 / 3.0);
+#pragma unroll
+  for (int I = 0; I < 6; ++I) {
+    t.Ci[I] = Ci9[vi(I) * 3 + vj(I)];
+    S6[I] = (I < 3 ? a : 0.0) + ci * t.Ci[I];
+  }
+  t.c1 = 2.0 * a * trC * (1.0 / 9.0) + cm.kappa * (J * dU + J * J * d2U);
+  t.c2 = -2.0 * a * (1.0 / 3.0);
+  t.c3 = 2.0 * a * trC * (1.0 / 3.0) - 2.0 * pv;
+}
+
+__device__ __forceinline__ void he_dot(const HeTan &t, const double *X9, double *out6) {
+  const double x[6] = {X9[0], X9[4], X9[8], 0.5 * (X9[1] + X9[3]), 0.5 * (X9[5] + X9[7]), 0.5 * (X9[2] + X9[6])};
+  const double *C = t.Ci;
+  const double cx = C[0] * x[0] + C[1] * x[1] + C[2] * x[2] + 2.0 * (C[3] * x[3] + C[4] * x[4] + C[5] * x[5]);
+  const double tx = x[0] + x[1] + x[2];
+  double Ci9[9], X[9], T[9];
+  v6_to_9(C, Ci9);
+  v6_to_9(x, X);
+  mm3(Ci9, X, T);
+#pragma unroll
+  for (int I = 0; I < 6; ++I) {
+    const int i = vi(I), j = vj(I);
+    const double r = T[i * 3] * Ci9[j] + T[i * 3 + 1] * Ci9[3 + j] + T[i * 3 + 2] * Ci9[6 + j];
+    out6[I] = t.c1 * cx * C[I] + t.c2 * ((I < 3 ? cx : 0.0) + tx * C[I]) + t.c3 * r;
+  }
+}
+
+// Elastic PK2 from eF and Ce = eF^T eF: law 0 is S = Lv : (Ce - I)/2 with the grain's cubic
+// stiffness; law 1 is he_law, which also fills the tangent coefficients.
+template <bool HE>
+__device__ __forceinline__ void el_stress(const CPMaterial &cm, const DevGrain &gr, const double *eF,
+                                          const double *Ce, double *S6, HeTan &t) {
+  if constexpr (HE) {
+    he_law(cm, eF, Ce, S6, t);
+  } else {
+    double E[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) E[i] = 0.5 * (Ce[i] - ((i % 4 == 0) ? 1.0 : 0.0));
+    lv_dot(gr.Lv, E, S6);
+  }
+}
+// L : X at the state el_stress was evaluated at
+template <bool HE>
+__device__ __forceinline__ void el_dot(const DevGrain &gr, const HeTan &t, const double *X9, double *out6) {
+  if constexpr (HE) he_dot(t, X9, out6);
+  else lv_dot(gr.Lv, X9, out6);
+}
+
 // 9x9 LU with partial pivoting; row swaps are predicated so every index is static.
 __device__ __forceinline__ bool lu9(double (&A)[81], int (&piv)[9]) {
 #pragma unroll
@@ -142,8 +216,10 @@ __device__ __forceinline__ void lu9_solve(const double (&A)[81], const int (&piv
 struct CPPoint {
   double Mi[9], pJ, eF[9], Ce[9], S[6], Sig[9], tau[12], gdot[12], dgdt[12], dgdg[12];
   double gsum, gs, dgs, R[11];
+  HeTan he;  // law 1: tangent coefficients at Ce
 };
 
+template <bool HE>
 __device__ __forceinline__ bool cp_point(const CPMaterial &cm, const DevGrain &gr, const double *A,
                                          double det_pFn, double g_n, double dt, const double *x,
                                          CPPoint &p) {
@@ -156,10 +232,7 @@ __device__ __forceinline__ bool cp_point(const CPMaterial &cm, const DevGrain &g
   p.pJ = det_pFn / dM;
   mm3(A, M, p.eF);
   mtm3(p.eF, p.eF, p.Ce);
-  double E[9];
-#pragma unroll
-  for (int i = 0; i < 9; ++i) E[i] = 0.5 * (p.Ce[i] - ((i % 4 == 0) ? 1.0 : 0.0));
-  lv_dot(gr.Lv, E, p.S);
+  el_stress<HE>(cm, gr, p.eF, p.Ce, p.S, p.he);
   double S9[9];
   v6_to_9(p.S, S9);
   mm3(p.Ce, S9, p.Sig);
@@ -203,6 +276,7 @@ __device__ __forceinline__ bool cp_point(const CPMaterial &cm, const DevGrain &g
 }
 
 // d tau_a / d M = 2 A^T eF sym(P_a S + 1/2 L:(C_e P_a))
+template <bool HE>
 __device__ __forceinline__ void dtau_dM(const DevGrain &gr, int a, const double *AtE, const CPPoint &p,
                                         double *D) {
   const double *P = gr.P[a];
@@ -211,7 +285,7 @@ __device__ __forceinline__ void dtau_dM(const DevGrain &gr, int a, const double 
   mm3(P, S9, PS);
   mm3(p.Ce, P, CP);
   double LCP[6];
-  lv_dot(gr.Lv, CP, LCP);
+  el_dot<HE>(gr, p.he, CP, LCP);
   double sQ[9];
 #pragma unroll
   for (int I = 0; I < 6; ++I) {
@@ -231,69 +305,9 @@ struct Border {
   double b[9], c[9], d[9], e[9], f, dRg;
 };
 
-__device__ __forceinline__ void cp_jacobian(const CPMaterial &cm, const DevGrain &gr, double dt, const double *x,
-                                            const double *A, const CPPoint &p, double g_n,
-                                            double (&Ah)[81], Border &B) {
-  const double lam = x[9], g = x[10];
-  double AtE[9];
-  mtm3(A, p.eF, AtE);
-#pragma unroll
-  for (int i = 0; i < 81; ++i) Ah[i] = 0.0;
-#pragma unroll
-  for (int i = 0; i < 9; ++i) {
-    B.c[i] = 0.0;
-    B.e[i] = 0.0;
-  }
-  double sg = 0.0;
-  for (int a = 0; a < 12; ++a) {
-    double D[9];
-    dtau_dM(gr, a, AtE, p, D);
-    const double co = dt * p.dgdt[a];
-    const double sgn = (p.gdot[a] < 0) ? -1.0 : 1.0;
-#pragma unroll
-    for (int r = 0; r < 9; ++r) {
-      const double pr = gr.P[a][r] * co;
-#pragma unroll
-      for (int c = 0; c < 9; ++c) Ah[r * 9 + c] += pr * D[c];
-      B.c[r] += dt * gr.P[a][r] * p.dgdg[a];
-    }
-#pragma unroll
-    for (int c = 0; c < 9; ++c) B.e[c] += sgn * p.dgdt[a] * D[c];
-    sg += sgn * p.dgdg[a];
-  }
-  const double *Mi = p.Mi;
-  const double lp = dt * lam * p.pJ;
-#pragma unroll
-  for (int i = 0; i < 3; ++i)
-#pragma unroll
-    for (int j = 0; j < 3; ++j)
-#pragma unroll
-      for (int k = 0; k < 3; ++k)
-#pragma unroll
-        for (int l = 0; l < 3; ++l)
-          Ah[(i * 3 + j) * 9 + k * 3 + l] +=
-              ((i == k && j == l) ? 1.0 : 0.0) + lp * (Mi[j * 3 + i] * Mi[l * 3 + k] + Mi[l * 3 + i] * Mi[j * 3 + k]);
-#pragma unroll
-  for (int i = 0; i < 3; ++i)
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      B.b[i * 3 + j] = -dt * p.pJ * Mi[j * 3 + i];
-      B.d[i * 3 + j] = -p.pJ * Mi[j * 3 + i];
-    }
-  B.dRg = g * (p.dgs + dt * cm.h0) - p.dgs * g_n - dt * cm.h0 * (p.dgs * p.gsum + p.gs);
-#pragma unroll
-  for (int c = 0; c < 9; ++c) B.e[c] *= B.dRg;
-  B.f = (p.gs - cm.g0 + dt * cm.h0 * p.gsum) + B.dRg * sg;
-  // Schur: Ah <- A - c e^T / f
-  const double rf = 1.0 / B.f;
-#pragma unroll
-  for (int r = 0; r < 9; ++r)
-#pragma unroll
-    for (int c = 0; c < 9; ++c) Ah[r * 9 + c] -= B.c[r] * B.e[c] * rf;
-}
-
-// cp_jacobian for callers that do not keep the per-system arrays of the point: they are
-// recomputed from the Mandel stress with the same arithmetic.
+// Bordered Jacobian of the local system at p; the per-system slip quantities are recomputed
+// from the Mandel stress with cp_point's arithmetic.
+template <bool HE>
 __device__ __forceinline__ void cp_jacobian_re(const CPMaterial &cm, const DevGrain &gr, double dt, const double *x,
                                             const double *A, const CPPoint &p, double g_n,
                                             double (&Ah)[81], Border &B) {
@@ -320,7 +334,7 @@ __device__ __forceinline__ void cp_jacobian_re(const CPMaterial &cm, const DevGr
     const double dgdt = cm.gd0 / (cm.m * g) * pw;
     const double dgdg = -dgdt * t / g;
     double D[9];
-    dtau_dM(gr, a, AtE, p, D);
+    dtau_dM<HE>(gr, a, AtE, p, D);
     const double co = dt * dgdt;
     const double sgn = (gd < 0) ? -1.0 : 1.0;
 #pragma unroll
@@ -398,6 +412,7 @@ __device__ __forceinline__ double cp_merit(const CPPoint &p, double gscale) {
 
 // The merit of x without keeping the point: the same arithmetic as cp_point's residual (slip
 // contributions accumulated in slip-system order), so acceptance decisions match it.
+template <bool HE>
 __device__ __forceinline__ bool cp_merit_at(const CPMaterial &cm, const DevGrain &gr, const double *A,
                                             double det_pFn, double g_n, double dt, const double *x, double gscale,
                                             double &merit) {
@@ -409,12 +424,11 @@ __device__ __forceinline__ bool cp_merit_at(const CPMaterial &cm, const DevGrain
   double Mi[9];
   if (!inv3(M, Mi)) return false;
   const double pJ = det_pFn / dM;
-  double eF[9], Ce[9], E[9], S[6], S9[9], Sig[9];
+  double eF[9], Ce[9], S[6], S9[9], Sig[9];
   mm3(A, M, eF);
   mtm3(eF, eF, Ce);
-#pragma unroll
-  for (int i = 0; i < 9; ++i) E[i] = 0.5 * (Ce[i] - ((i % 4 == 0) ? 1.0 : 0.0));
-  lv_dot(gr.Lv, E, S);
+  HeTan ht;
+  el_stress<HE>(cm, gr, eF, Ce, S, ht);
   v6_to_9(S, S9);
   mm3(Ce, S9, Sig);
   const double expo = 1.0 / cm.m - 1.0;
@@ -449,13 +463,14 @@ __device__ __forceinline__ bool cp_merit_at(const CPMaterial &cm, const DevGrain
 }
 
 // max_a |tau_a| / g at M (elastic part eF = A M)
-__device__ __forceinline__ double tau_ratio_max(const DevGrain &gr, const double *A, const double *M, double rg) {
-  double eF[9], Ce[9], E[9], S[6], S9[9], Sig[9];
+template <bool HE>
+__device__ __forceinline__ double tau_ratio_max(const CPMaterial &cm, const DevGrain &gr, const double *A,
+                                                const double *M, double rg) {
+  double eF[9], Ce[9], S[6], S9[9], Sig[9];
   mm3(A, M, eF);
   mtm3(eF, eF, Ce);
-#pragma unroll
-  for (int i = 0; i < 9; ++i) E[i] = 0.5 * (Ce[i] - ((i % 4 == 0) ? 1.0 : 0.0));
-  lv_dot(gr.Lv, E, S);
+  HeTan ht;
+  el_stress<HE>(cm, gr, eF, Ce, S, ht);
   v6_to_9(S, S9);
   mm3(Ce, S9, Sig);
   double mx = 0.0;
@@ -470,7 +485,9 @@ __device__ __forceinline__ double tau_ratio_max(const DevGrain &gr, const double
 }
 
 // Plain start x = (M, lambda_n, g_n): the fully plastic predictor pulled back to yield.
-__device__ __forceinline__ bool cp_start(const DevGrain &gr, const double *A, const double *F, const double *Fn,
+template <bool HE>
+__device__ __forceinline__ bool cp_start(const CPMaterial &cm, const DevGrain &gr, const double *A,
+                                         const double *F, const double *Fn,
                                          const double *pFn, const double *pFi, double g_n, double lam_n,
                                          double *x) {
   double Fi[9], T[9], Mp[9];
@@ -486,16 +503,16 @@ __device__ __forceinline__ bool cp_start(const DevGrain &gr, const double *A, co
   const double rg = 1.0 / g_n;
   const double I9[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
   double al = 0.0;
-  if (tau_ratio_max(gr, A, I9, rg) > 1.0) {
+  if (tau_ratio_max<HE>(cm, gr, A, I9, rg) > 1.0) {
     al = 1.0;
-    if (!(tau_ratio_max(gr, A, Mp, rg) > 1.0)) {
+    if (!(tau_ratio_max<HE>(cm, gr, A, Mp, rg) > 1.0)) {
       double lo = 0.0, hi = 1.0;
       for (int k = 0; k < 8; ++k) {
         const double mid = 0.5 * (lo + hi);
         double Mm[9];
 #pragma unroll
         for (int i = 0; i < 9; ++i) Mm[i] = I9[i] + mid * (Mp[i] - I9[i]);
-        if (tau_ratio_max(gr, A, Mm, rg) > 1.0) lo = mid;
+        if (tau_ratio_max<HE>(cm, gr, A, Mm, rg) > 1.0) lo = mid;
         else hi = mid;
       }
       al = hi;
@@ -504,9 +521,9 @@ __device__ __forceinline__ bool cp_start(const DevGrain &gr, const double *A, co
 #pragma unroll
   for (int i = 0; i < 9; ++i) x[i] = I9[i] + al * (Mp[i] - I9[i]);
   if (al > 0.0) {
-    const double cm = 1.0 / cbrt(det3(x));
+    const double sc = 1.0 / cbrt(det3(x));
 #pragma unroll
-    for (int i = 0; i < 9; ++i) x[i] *= cm;
+    for (int i = 0; i < 9; ++i) x[i] *= sc;
   }
   x[9] = lam_n;
   x[10] = g_n;
@@ -568,6 +585,7 @@ __global__ __launch_bounds__(256) void cp_kinematics(CPArgs a) {
 
 // ------------------------------------------------------------------- Newton kernel
 // One wave per workgroup; wave w owns QPs [w*chunk, (w+1)*chunk) of the block.
+template <bool HE>
 __global__ __launch_bounds__(64) void cp_newton(CPArgs a, int64_t chunk) {
   const int lane = threadIdx.x;
   const int64_t q_lo = int64_t(blockIdx.x) * chunk;
@@ -615,7 +633,7 @@ __global__ __launch_bounds__(64) void cp_newton(CPArgs a, int64_t chunk) {
           double Fn[9];
 #pragma unroll
           for (int i = 0; i < 9; ++i) Fn[i] = a.F_n[i * NQ + qg];
-          bad_start = !cp_start(a.grains[gid], A, F, Fn, pFn, pFi, g_n, a.lam_n[qg], x);
+          bad_start = !cp_start<HE>(cm, a.grains[gid], A, F, Fn, pFn, pFi, g_n, a.lam_n[qg], x);
         }
         if (a.warm) {
           // a caller's start is taken when its residual merit is below the plain start's
@@ -623,8 +641,8 @@ __global__ __launch_bounds__(64) void cp_newton(CPArgs a, int64_t chunk) {
 #pragma unroll
           for (int i = 0; i < 11; ++i) xw[i] = a.warm[i * NQ + qg];
           double mw = 0.0, mp = 0.0;
-          const bool okw = cp_merit_at(cm, a.grains[gid], A, det_pFn, g_n, dt, xw, gscale, mw);
-          const bool okp = !bad_start && cp_merit_at(cm, a.grains[gid], A, det_pFn, g_n, dt, x, gscale, mp);
+          const bool okw = cp_merit_at<HE>(cm, a.grains[gid], A, det_pFn, g_n, dt, xw, gscale, mw);
+          const bool okp = !bad_start && cp_merit_at<HE>(cm, a.grains[gid], A, det_pFn, g_n, dt, x, gscale, mp);
           if (okw && (!okp || mw < mp)) {
 #pragma unroll
             for (int i = 0; i < 11; ++i) x[i] = xw[i];
@@ -645,12 +663,12 @@ __global__ __launch_bounds__(64) void cp_newton(CPArgs a, int64_t chunk) {
     ++it_att;
     bool fail = false, conv = false;
     CPPoint p;
-    if (bad_start || !cp_point(cm, gr, A, det_pFn, g_n, dt, x, p)) {
+    if (bad_start || !cp_point<HE>(cm, gr, A, det_pFn, g_n, dt, x, p)) {
       fail = true;
     } else {
       double Ah[81];
       Border B;
-      cp_jacobian_re(cm, gr, dt, x, A, p, g_n, Ah, B);
+      cp_jacobian_re<HE>(cm, gr, dt, x, A, p, g_n, Ah, B);
       int piv[9];
       if (!lu9(Ah, piv)) {
         fail = true;
@@ -684,7 +702,7 @@ __global__ __launch_bounds__(64) void cp_newton(CPArgs a, int64_t chunk) {
 #pragma unroll
             for (int i = 0; i < 11; ++i) xn[i] = x[i] + alpha * dx[i];
             double mn = 0.0;
-            if (cp_merit_at(cm, gr, A, det_pFn, g_n, dt, xn, gscale, mn) && mn <= (1.0 - 1e-4 * alpha) * f0) {
+            if (cp_merit_at<HE>(cm, gr, A, det_pFn, g_n, dt, xn, gscale, mn) && mn <= (1.0 - 1e-4 * alpha) * f0) {
               ok = true;
               break;
             }
@@ -694,7 +712,7 @@ __global__ __launch_bounds__(64) void cp_newton(CPArgs a, int64_t chunk) {
 #pragma unroll
             for (int i = 0; i < 11; ++i) xn[i] = x[i] + dx[i];
             double mn = 0.0;
-            if (!cp_merit_at(cm, gr, A, det_pFn, g_n, dt, xn, gscale, mn)) fail = true;
+            if (!cp_merit_at<HE>(cm, gr, A, det_pFn, g_n, dt, xn, gscale, mn)) fail = true;
           }
           if (!fail) {
 #pragma unroll
@@ -717,7 +735,7 @@ __global__ __launch_bounds__(64) void cp_newton(CPArgs a, int64_t chunk) {
         Fn[i] = a.F_n[i * NQ + qg];
       }
       inv3(pFn, pFi);
-      bad_start = !cp_start(gr, A, F, Fn, pFn, pFi, g_n, a.lam_n[qg], x);
+      bad_start = !cp_start<HE>(cm, gr, A, F, Fn, pFn, pFi, g_n, a.lam_n[qg], x);
       fail = false;
     }
     if (conv || fail) {
@@ -744,7 +762,7 @@ __global__ __launch_bounds__(64) void cp_newton(CPArgs a, int64_t chunk) {
 // ------------------------------------------------------------------- tangent kernel
 // At the stored converged state (F, pF(n), pF(n+1), g, lambda): S, tau, gdot, Me, pJ and,
 // with WITH_D, dMe/dF = pF(n)^-1 dM/dF from the implicit derivative of the local system.
-template <bool WITH_D>
+template <bool HE, bool WITH_D>
 __global__ __launch_bounds__(64) void cp_tangent(CPArgs a, int64_t nqb) {
   const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (t >= nqb) return;
@@ -766,7 +784,7 @@ __global__ __launch_bounds__(64) void cp_tangent(CPArgs a, int64_t nqb) {
   x[10] = a.Sc_np1[qg];
   const double g_n = a.g_n[qg];
   CPPoint p;
-  ok = ok && cp_point(cm, gr, A, det3(pFn), g_n, a.dt, x, p);
+  ok = ok && cp_point<HE>(cm, gr, A, det3(pFn), g_n, a.dt, x, p);
   double *out = a.qpbuf + t * kCPBuf;
   if (!ok) {
     atomicAdd(a.err, 1);
@@ -784,12 +802,13 @@ __global__ __launch_bounds__(64) void cp_tangent(CPArgs a, int64_t nqb) {
     a.gdot[al * NQ + qg] = cm.gd0 * r * pow(fabs(r), expo);
   }
   // elastic PK2 from the stored state (eF = F pF(n+1)^-1)
-  double eF[9], Ce[9], E[9], S[6];
+  double eF[9], Ce[9], S[6];
   mm3(F, Me, eF);
   mtm3(eF, eF, Ce);
-#pragma unroll
-  for (int i = 0; i < 9; ++i) E[i] = 0.5 * (Ce[i] - ((i % 4 == 0) ? 1.0 : 0.0));
-  lv_dot(gr.Lv, E, S);
+  {
+    HeTan ht;
+    el_stress<HE>(cm, gr, eF, Ce, S, ht);
+  }
 #pragma unroll
   for (int i = 0; i < 9; ++i) out[i] = Me[i];
 #pragma unroll
@@ -798,7 +817,7 @@ __global__ __launch_bounds__(64) void cp_tangent(CPArgs a, int64_t nqb) {
   if constexpr (WITH_D) {
     double Ah[81];
     Border B;
-    cp_jacobian_re(cm, gr, a.dt, x, A, p, g_n, Ah, B);
+    cp_jacobian_re<HE>(cm, gr, a.dt, x, A, p, g_n, Ah, B);
     int piv[9];
     if (!lu9(Ah, piv)) {
       atomicAdd(a.err, 1);
@@ -830,7 +849,7 @@ __global__ __launch_bounds__(64) void cp_tangent(CPArgs a, int64_t nqb) {
     }
     for (int al = 0; al < 12; ++al) {
       double D[9], T[9], dtdF[9];
-      dtau_dM(gr, al, AtE, p, D);
+      dtau_dM<HE>(gr, al, AtE, p, D);
       mm3(AiT, D, T);
 #pragma unroll
       for (int i = 0; i < 3; ++i)
@@ -876,7 +895,7 @@ constexpr int cp_stage_round(int epb, int nde, int room) {
   return re;
 }
 
-template <int NNE, int MODE, bool STG>
+template <bool HE, int NNE, int MODE, bool STG>
 __global__ __launch_bounds__(256) void cp_elem(CPElemArgs a, int64_t e_begin, int64_t e_end) {
   constexpr int NQP = nqp_of(NNE);
   constexpr int NDE = NNE * 3;
@@ -890,6 +909,7 @@ __global__ __launch_bounds__(256) void cp_elem(CPElemArgs a, int64_t e_begin, in
   __shared__ double s_Fr[EPB][NQP][9], s_Me[EPB][NQP][9], s_MeS[EPB][NQP][9], s_CrMe[EPB][NQP][9];
   __shared__ double s_MiT[EPB][NQP][9], s_Sh[EPB][NQP][6], s_S[EPB][NQP][6], s_w[EPB][NQP];
   __shared__ double s_D[EPB][NQP][TAN ? 81 : 1];
+  __shared__ double s_he[EPB][NQP][(HE && TAN) ? 9 : 1];  // law 1: Ci, c1, c2, c3 at the QP
 
   const int slot = threadIdx.x / TPE, lane = threadIdx.x % TPE;
   const int64_t ie = e_begin + int64_t(blockIdx.x) * EPB + slot;
@@ -960,6 +980,18 @@ __global__ __launch_bounds__(256) void cp_elem(CPElemArgs a, int64_t e_begin, in
         Sh9[i * 3 + j] = MeS[i * 3 + 0] * Me[j * 3 + 0] + MeS[i * 3 + 1] * Me[j * 3 + 1] + MeS[i * 3 + 2] * Me[j * 3 + 2];
     mtm3(Fr, Fr, Cr);
     mm3(Cr, Me, CrMe);
+    if constexpr (HE && TAN) {
+      double eF[9], Ce[9], Sx[6];
+      HeTan ht;
+      mm3(Fr, Me, eF);
+      mtm3(eF, eF, Ce);
+      he_law(a.cpmats[0], eF, Ce, Sx, ht);
+#pragma unroll
+      for (int i = 0; i < 6; ++i) s_he[slot][ip][i] = ht.Ci[i];
+      s_he[slot][ip][6] = ht.c1;
+      s_he[slot][ip][7] = ht.c2;
+      s_he[slot][ip][8] = ht.c3;
+    }
     s_w[slot][ip] = ok ? J0 * wt * pJ : 0.0;
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
@@ -1018,7 +1050,18 @@ __global__ __launch_bounds__(256) void cp_elem(CPElemArgs a, int64_t e_begin, in
       mm3(sAA, Me, T);
       mtm3(Me, T, Bm);  // Me^T sAA Me
       double Y6[6];
-      lv_dot(gr.Lv, Bm, Y6);
+      if constexpr (HE) {
+        const double *hv = s_he[slot][ip];
+        HeTan ht;
+#pragma unroll
+        for (int i = 0; i < 6; ++i) ht.Ci[i] = hv[i];
+        ht.c1 = hv[6];
+        ht.c2 = hv[7];
+        ht.c3 = hv[8];
+        he_dot(ht, Bm, Y6);
+      } else {
+        lv_dot(gr.Lv, Bm, Y6);
+      }
       double Y9[9];
       v6_to_9(Y6, Y9);
       double BS = 0.0;
@@ -1173,8 +1216,13 @@ void launch_ce(const CPElemArgs &a, int64_t e0, int64_t e1, hipStream_t s) {
   const int64_t n = e1 - e0;
   if (n <= 0) return;
   const unsigned grid = unsigned((n + EPB - 1) / EPB);
-  if (a.stage) cp_elem<NNE, MODE, true><<<grid, EPB * TPE, 0, s>>>(a, e0, e1);
-  else cp_elem<NNE, MODE, false><<<grid, EPB * TPE, 0, s>>>(a, e0, e1);
+  if (a.law == 1) {
+    if (a.stage) cp_elem<true, NNE, MODE, true><<<grid, EPB * TPE, 0, s>>>(a, e0, e1);
+    else cp_elem<true, NNE, MODE, false><<<grid, EPB * TPE, 0, s>>>(a, e0, e1);
+  } else {
+    if (a.stage) cp_elem<false, NNE, MODE, true><<<grid, EPB * TPE, 0, s>>>(a, e0, e1);
+    else cp_elem<false, NNE, MODE, false><<<grid, EPB * TPE, 0, s>>>(a, e0, e1);
+  }
 }
 
 template <int NNE>
@@ -1287,14 +1335,22 @@ void launch_cp_newton(const CPArgs &a, int64_t chunk, hipStream_t s) {
   const int64_t nq = a.nq;
   if (nq <= 0) return;
   if (chunk < 64) chunk = 64;
-  cp_newton<<<unsigned((nq + chunk - 1) / chunk), 64, 0, s>>>(a, chunk);
+  const unsigned grid = unsigned((nq + chunk - 1) / chunk);
+  if (a.law == 1) cp_newton<true><<<grid, 64, 0, s>>>(a, chunk);
+  else cp_newton<false><<<grid, 64, 0, s>>>(a, chunk);
 }
 
 void launch_cp_tangent(const CPArgs &a, bool with_D, hipStream_t s) {
   const int64_t nq = a.nq;
   if (nq <= 0) return;
-  if (with_D) cp_tangent<true><<<unsigned((nq + 63) / 64), 64, 0, s>>>(a, nq);
-  else cp_tangent<false><<<unsigned((nq + 63) / 64), 64, 0, s>>>(a, nq);
+  const unsigned grid = unsigned((nq + 63) / 64);
+  if (a.law == 1) {
+    if (with_D) cp_tangent<true, true><<<grid, 64, 0, s>>>(a, nq);
+    else cp_tangent<true, false><<<grid, 64, 0, s>>>(a, nq);
+  } else {
+    if (with_D) cp_tangent<false, true><<<grid, 64, 0, s>>>(a, nq);
+    else cp_tangent<false, false><<<grid, 64, 0, s>>>(a, nq);
+  }
 }
 
 void launch_cp_elem(int nne, int mode, const CPElemArgs &a, int64_t e0, int64_t e1, hipStream_t s) {

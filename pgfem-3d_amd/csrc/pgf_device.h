@@ -77,9 +77,12 @@ struct DevGrain {
   double Lv[36];
 };
 
+// law 0: cubic St Venant-Kirchhoff through the grain's Lv; law 1: neo-Hookean deviatoric part
+// (m10) and the volumetric potential vol_flag (kappa) on the elastic part eF.
 struct CPMaterial {
   double gd0, m, h0, g0, gs0, gds, w, tol;
-  int32_t maxit, pad;
+  double m10, kappa;
+  int32_t maxit, law, vol_flag, pad;
 };
 
 // Arguments shared by the crystal-plasticity QP kernels (device pointers, SoA [comp][n_qp]).
@@ -108,6 +111,7 @@ struct CPArgs {
   double *qpbuf;
   int32_t *err;
   int32_t *queue;             // work-queue counter
+  int32_t law;                // cpmats[0].law, read on the host to pick the kernels
 };
 constexpr int kCPBuf = 97;  // doubles per QP in qpbuf
 
@@ -122,6 +126,8 @@ struct CPElemArgs {
   const int16_t *off;
   const int32_t *elist;  // coloured mode: as ElemArgs
   const DevGrain *grains;
+  const CPMaterial *cpmats;
+  int32_t law;  // cpmats[0].law
   const double *qpbuf;
   int64_t qp_base;
   const double *r, *d_r, *defl, *defl_d;

@@ -36,21 +36,6 @@ namespace {
 
 using namespace dev;
 
-// volumetric potential derivatives (src/new_potentials.cc:575-616)
-__device__ __forceinline__ void vol_pot(int flag, double J, double &dU, double &d2U) {
-  if (flag == 3) {
-    dU = (-1 + J) / (2. * J) + log(J) * .5;
-    d2U = 1. / J - (-1 + J) / (2. * J * J);
-  } else if (flag == 99) {
-    dU = J - 1;
-    d2U = 1.0;
-  } else {
-    const double ex = exp(J - 1.0);
-    dU = (ex - 1. / J) * .5;
-    d2U = (ex + 1. / (J * J)) * .5;
-  }
-}
-
 // Sum over aligned groups of W lanes (butterfly: every lane ends with the same bits, since
 // each step adds the same two values in either order).
 template <int W>

@@ -8,6 +8,21 @@
 namespace pgf {
 namespace dev {
 
+// volumetric potential derivatives (src/new_potentials.cc:575-616)
+__device__ __forceinline__ void vol_pot(int flag, double J, double &dU, double &d2U) {
+  if (flag == 3) {
+    dU = (-1 + J) / (2. * J) + log(J) * .5;
+    d2U = 1. / J - (-1 + J) / (2. * J * J);
+  } else if (flag == 99) {
+    dU = J - 1;
+    d2U = 1.0;
+  } else {
+    const double ex = exp(J - 1.0);
+    dU = (ex - 1. / J) * .5;
+    d2U = (ex + 1. / (J * J)) * .5;
+  }
+}
+
 __host__ __device__ constexpr int nqp_of(int nne) { return nne == 4 ? 1 : (nne == 10 ? 4 : 8); }
 
 // FEMLIB quadrature (FEMLIB/src/femlib.cc:544-588), truncated constants kept.

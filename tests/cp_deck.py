@@ -12,6 +12,7 @@ import numpy as np
 import oracle as O
 import pgfem3d_amd as P
 import ref_io as R
+from problems import oracle_newton_step
 
 DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "cp_box_tension")
 NPARTS = 8
@@ -71,3 +72,23 @@ def compare_with_fixture(prob, u, F_n, pF_n, g_n):
             dpF = max(dpF, float(np.abs(pF_n[:, q] - rec.pF_n).max()))
             dg = max(dg, abs(g_n[q] - rec.g_n) / rec.g_n)
     return du, dF, dpF, dg
+
+
+def run_oracle(prob, rot, nsteps, tol=1e-10):
+    """The deck's load steps over the oracle (Newton to tol on the normalised residual, sparse
+    direct solve): (nodal u, CPState, prescribed values, Newton iterations per step)."""
+    tabs = [O.cp_grain(prob.mat, x) for x in rot]
+    gP = np.stack([t[0] for t in tabs])
+    gL = np.stack([t[1] for t in tabs])
+    st = O.CPState(prob.ne, prob.mat.g0)
+    defl = np.zeros(len(prob.deck.defl))
+    r = np.zeros(prob.ndof)
+    state = dict(NORM=0.0, first=True)
+    sf = prob.sf
+    its = []
+    for step in range(nsteps):
+        defl = defl + sf.increment(step, prob.deck.defl)
+        r, it, _ = oracle_newton_step(prob, st, r, defl, state, tol=tol, max_iter=40, cp_tables=(gP, gL),
+                                      dt=sf.times[step + 1] - sf.times[step])
+        its.append(it)
+    return nodal_u(prob, r, defl), st, defl, its

@@ -30,6 +30,18 @@ class BoxProblem:
         return r, np.array([stretch])
 
 
+def fcc_crystal_he(volPotFlag=2, **kw):
+    """FCC crystal plasticity with the hyperelastic elastic law (cp_elastic = 1): neo-Hookean
+    deviatoric part m10 = 24 GPa (shear modulus 48 GPa) and the volumetric potential volPotFlag
+    with E = 128.6 GPa, nu = 0.34 -- close to the cubic constants of fcc_crystal in average --
+    and fcc_crystal's slip and hardening parameters."""
+    m = P.fcc_crystal(**kw)
+    m.cp_elastic, m.devPotFlag, m.volPotFlag = 1, 1, volPotFlag
+    m.E, m.nu, m.m10, m.m01 = 128.6e3, 0.34, 24.0e3, 0.0
+    m.C11 = m.C12 = m.C44 = 0.0
+    return m
+
+
 class CPBoxProblem(BoxProblem):
     """Box mesh with crystal plasticity on Voronoi grains of random orientation."""
 

@@ -13,10 +13,11 @@ import torch
 import oracle as O
 import pgfem3d_amd as P
 from conftest import rel_inf
-from problems import CPBoxProblem
+from problems import CPBoxProblem, fcc_crystal_he
 
 TOL = 1e-10
 MAT = P.fcc_crystal()
+MAT_HE = fcc_crystal_he()
 
 pytestmark = pytest.mark.gpu
 
@@ -45,7 +46,7 @@ def _compare_state(h, st, step):
 
 def _run(prob, device, stretches, dt=1.0, noise=2e-5, check_modes=True):
     h = _handle(prob)
-    st = O.CPState(prob.nqp, MAT.g0)
+    st = O.CPState(prob.nqp, prob.mats[0].g0)
     f = torch.empty(prob.ndof, dtype=torch.float64, device=device)
     vals = torch.empty(int(prob.Ap[-1]), dtype=torch.float64, device=device)
     plastic = False
@@ -77,12 +78,21 @@ def _run(prob, device, stretches, dt=1.0, noise=2e-5, check_modes=True):
 
 
 @pytest.mark.parametrize("etype", [P.PGF_TET4, P.PGF_TET10, P.PGF_HEX8])
-def test_cp_multistep_into_plastic_range(device, etype):
+@pytest.mark.parametrize("law", ["cubic", "hyperelastic"])
+def test_cp_multistep_into_plastic_range(device, etype, law):
     """Voronoi polycrystal (4 grains), uniaxial stretch increments of 1.5e-4 with commits,
-    beyond the resolved yield g0 = 16 MPa."""
+    beyond the resolved yield g0 = 16 MPa; the elastic law of the intermediate configuration
+    is the cubic St Venant-Kirchhoff one or the hyperelastic one (cp_elastic = 1)."""
     n = {P.PGF_TET4: 3, P.PGF_TET10: 2, P.PGF_HEX8: 3}[etype]
-    prob = CPBoxProblem(n, etype, MAT, n_grains=4, seed=3)
+    prob = CPBoxProblem(n, etype, MAT if law == "cubic" else MAT_HE, n_grains=4, seed=3)
     assert _run(prob, device, [1.5e-4 * s for s in range(1, 9)])
+
+
+@pytest.mark.parametrize("vol", [3, 99])
+def test_cp_hyperelastic_volumetric_potentials(device, vol):
+    """The other volumetric potentials of the hyperelastic law, linear tets."""
+    prob = CPBoxProblem(3, P.PGF_TET4, fcc_crystal_he(volPotFlag=vol), n_grains=4, seed=4)
+    assert _run(prob, device, [1.5e-4 * s for s in range(1, 7)], check_modes=False)
 
 
 def test_cp_tangent_at_other_displacement(device):

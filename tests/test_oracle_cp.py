@@ -5,8 +5,10 @@ from scipy.optimize import fsolve
 
 import oracle as O
 import pgfem3d_amd as P
+from problems import fcc_crystal_he
 
 MAT = P.fcc_crystal()
+MAT_HE = fcc_crystal_he()
 
 
 def _rot(seed):
@@ -96,14 +98,15 @@ def test_scipy_solves_same_system():
     np.testing.assert_allclose(x_sp[10], x_or[10], rtol=1e-10)
 
 
-def test_dMdF_matches_fd():
-    Pg, Lg = O.cp_grain(MAT, _rot(3))
-    F, pFn, g_n, lam_n, r = _march(Pg, Lg, _uniax, 10)[-1]
+@pytest.mark.parametrize("mat", [MAT, MAT_HE], ids=["cubic", "hyperelastic"])
+def test_dMdF_matches_fd(mat):
+    Pg, Lg = O.cp_grain(mat, _rot(3))
+    F, pFn, g_n, lam_n, r = _march(Pg, Lg, _uniax, 10, mat=mat)[-1]
     assert r["gdot"].any()
     x_conv = np.r_[(pFn @ np.linalg.inv(r["pF"])).reshape(9), r["lam"], r["g"]]
 
     def M_of(Fp):
-        rr = O.cp_update(MAT, Pg, Lg, Fp, pFn, g_n, lam_n, 1.0, x0=x_conv, tangent=False)
+        rr = O.cp_update(mat, Pg, Lg, Fp, pFn, g_n, lam_n, 1.0, x0=x_conv, tangent=False)
         assert rr["rc"] == 0
         return (pFn @ np.linalg.inv(rr["pF"])).reshape(9)
 
@@ -133,6 +136,23 @@ def test_slip_off_is_st_venant_kirchhoff():
     assert abs(L0[0, 1, 0, 1] - mat.C44) < 1e-8 and abs(L0[0, 0, 0, 1]) < 1e-8
 
 
+@pytest.mark.parametrize("vol", [2, 3, 99])
+def test_slip_off_is_the_hyperelastic_model(vol):
+    """gdot0 -> 0 with the hyperelastic law: pF stays I and S is the hyperelastic model's own
+    update at F (orc_he_update, pinned by its finite differences and closed forms)."""
+    mat = fcc_crystal_he(volPotFlag=vol, gamma_dot_0=1e-100)
+    Pg, Lg = O.cp_grain(mat, _rot(2))
+    F = np.eye(3) + 0.02 * np.random.default_rng(4).standard_normal((3, 3))
+    r = O.cp_update(mat, Pg, Lg, F, np.eye(3), mat.g0, 0.0, 1.0)
+    assert r["rc"] == 0
+    np.testing.assert_allclose(r["pF"], np.eye(3), atol=1e-14)
+    S_he, _, rc = O.he_update(mat, F, want_L=False)
+    assert rc == 0
+    np.testing.assert_allclose(r["S"], S_he, rtol=1e-12, atol=1e-9)
+    # and it is not the cubic law of the same material line
+    assert np.abs(S_he).max() > 100.0
+
+
 def test_fcc_slip_systems():
     P0 = O.cp_slip_systems()
     for a in range(12):
@@ -144,20 +164,21 @@ def test_fcc_slip_systems():
     assert np.linalg.matrix_rank(P0.reshape(12, 9), tol=1e-10) == 8
 
 
-def _cp_problem(n, etype, seed=1, n_grains=3):
+def _cp_problem(n, etype, seed=1, n_grains=3, mat=MAT):
     from problems import CPBoxProblem
 
-    return CPBoxProblem(n, etype, MAT, n_grains=n_grains, seed=seed)
+    return CPBoxProblem(n, etype, mat, n_grains=n_grains, seed=seed)
 
 
 def _cp_assemble(prob, st, r, defl, **kw):
     return prob.assemble(st, r, defl, **kw)
 
 
-@pytest.mark.parametrize("etype", [4, 10])
-def test_cp_element_tangent_is_fd_of_residual(etype):
-    prob = _cp_problem(1, etype)
-    st = O.CPState(prob.nqp, MAT.g0)
+@pytest.mark.parametrize("etype,mat", [(4, MAT), (10, MAT), (4, MAT_HE), (10, MAT_HE)],
+                         ids=["tet4-cubic", "tet10-cubic", "tet4-hyperelastic", "tet10-hyperelastic"])
+def test_cp_element_tangent_is_fd_of_residual(etype, mat):
+    prob = _cp_problem(1, etype, mat=mat)
+    st = O.CPState(prob.nqp, mat.g0)
     for s in range(1, 11):
         r, defl = prob.displacement(1.5e-4 * s, 2e-5, seed=s)
         f, vals, err = _cp_assemble(prob, st, r, defl)
@@ -175,7 +196,7 @@ def test_cp_element_tangent_is_fd_of_residual(etype):
     for j in range(prob.ndof):
         col = []
         for sgn in (1, -1):
-            st2 = O.CPState(prob.nqp, MAT.g0)
+            st2 = O.CPState(prob.nqp, mat.g0)
             for k, v in n_state.items():
                 getattr(st2, k)[:] = v
             rp = r.copy()

@@ -10,30 +10,11 @@ So agreement is held to the scale those tolerances leave: displacement 1e-5 (lar
 value 2e-4), F and pF 5e-5, hardening 2e-4 relative.  Measured: |du| 1.3e-6, |dF| 6.7e-6,
 |dpF| 1.7e-5, relative dg 6.5e-5.  A wrong Euler-angle convention (R transposed) misses by
 4e-3 in F."""
-import numpy as np
 import pytest
 
-import oracle as O
 import cp_deck
-from problems import oracle_newton_step
 
 TOL_U, TOL_F, TOL_PF, TOL_G = 1e-5, 5e-5, 5e-5, 2e-4
-
-
-def run_oracle(prob, rot, nsteps):
-    tabs = [O.cp_grain(prob.mat, x) for x in rot]
-    gP = np.stack([t[0] for t in tabs])
-    gL = np.stack([t[1] for t in tabs])
-    st = O.CPState(prob.ne, prob.mat.g0)
-    defl = np.zeros(len(prob.deck.defl))
-    r = np.zeros(prob.ndof)
-    state = dict(NORM=0.0, first=True)
-    sf = prob.sf
-    for step in range(nsteps):
-        defl = defl + sf.increment(step, prob.deck.defl)
-        r, _, _ = oracle_newton_step(prob, st, r, defl, state, tol=1e-10, max_iter=40, cp_tables=(gP, gL),
-                                     dt=sf.times[step + 1] - sf.times[step])
-    return cp_deck.nodal_u(prob, r, defl), st
 
 
 @pytest.fixture(scope="module")
@@ -48,7 +29,7 @@ def test_deck_shape(prob):
 
 
 def test_oracle_matches_reference_restart(prob):
-    u, st = run_oracle(prob, prob.rot, cp_deck.FIXTURE_STEP + 1)
+    u, st, _, _ = cp_deck.run_oracle(prob, prob.rot, cp_deck.FIXTURE_STEP + 1)
     du, dF, dpF, dg = cp_deck.compare_with_fixture(prob, u, st.F_n, st.pF_n, st.g_n)
     assert du < TOL_U and dF < TOL_F and dpF < TOL_PF and dg < TOL_G, (du, dF, dpF, dg)
     # the stretch is large enough for the fixture to tell the state apart from elastic response
@@ -58,6 +39,6 @@ def test_oracle_matches_reference_restart(prob):
 def test_transposed_orientation_misses(prob):
     """The same run with every rotation transposed fails the bar: the comparison has the power
     to see the orientation convention."""
-    u, st = run_oracle(prob, prob.rot.transpose(0, 2, 1), cp_deck.FIXTURE_STEP + 1)
+    u, st, _, _ = cp_deck.run_oracle(prob, prob.rot.transpose(0, 2, 1), cp_deck.FIXTURE_STEP + 1)
     du, dF, dpF, dg = cp_deck.compare_with_fixture(prob, u, st.F_n, st.pF_n, st.g_n)
     assert dF > 10 * TOL_F, (du, dF, dpF, dg)
