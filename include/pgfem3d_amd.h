@@ -217,6 +217,15 @@ int pgf_residual(pgf_handle h, const pgf_nodal_input *in, double dt, int run_int
  * PLoc_Sparse (src/PLoc_Sparse.cc:15-175). */
 int pgf_tangent(pgf_handle h, const pgf_nodal_input *in, double dt, double *vals);
 
+/* Load vector of a prescribed increment: f_defl[ndof] (zeroed first) = sum over elements of
+ * K_e u_e, K_e the element tangent at (r + d_r, defl) -- the in->defl_d increment left out, as
+ * pgf_tangent at in->defl_d = 0 would form it -- and u_e the element's values of in->defl_d on
+ * its prescribed DOFs (zero on the others), kept on the free rows.  Its negative is the
+ * right-hand side of the first Newton iteration of a load step.  Replaces
+ * compute_load_vector_for_prescribed_BC (src/load.cc:363-417) -> momentum_equation_load4pBCs
+ * (:102-220), whose stiffness is el_compute_stiffmat_MP at def_elem(TOTAL) values. */
+int pgf_prescribed_load(pgf_handle h, const pgf_nodal_input *in, double dt, double *f_defl);
+
 /* Both of the above at one displacement in one pass over the elements (integration
  * run, then residual and tangent from the updated state).  Equivalent to pgf_residual
  * (run_integration=1) followed by pgf_tangent at the same input. */

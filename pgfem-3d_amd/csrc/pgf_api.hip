@@ -1074,6 +1074,12 @@ int pgf_tangent(pgf_handle h, const pgf_nodal_input *in, double dt, double *vals
   return run_pass(h, in, pgf::MODE_TANGENT, nullptr, vals);
 }
 
+int pgf_prescribed_load(pgf_handle h, const pgf_nodal_input *in, double dt, double *f_defl) {
+  if (!h || !f_defl || !in || !in->defl_d) return PGF_EINVAL;
+  if (dt > 0.0) h->dt = dt;
+  return run_pass(h, in, pgf::MODE_TANGENT | pgf::MODE_DEFL_LOAD, f_defl, nullptr);
+}
+
 int pgf_residual_tangent(pgf_handle h, const pgf_nodal_input *in, double dt, double *f, double *vals) {
   if (!h || !f || !vals) return PGF_EINVAL;
   if (dt > 0.0) h->dt = dt;

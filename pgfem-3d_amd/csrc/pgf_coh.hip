@@ -142,8 +142,11 @@ __global__ __launch_bounds__(256) void coh_elem(CohArgs a, int64_t e_begin, int6
   constexpr bool INTEG = (MODE & MODE_INTEGRATE) != 0;
   constexpr bool RES = (MODE & MODE_RESIDUAL) != 0;
   constexpr bool TAN = (MODE & MODE_TANGENT) != 0;
+  constexpr bool DFL = (MODE & MODE_DEFL_LOAD) != 0;  // K times the prescribed increments
+  static_assert(!DFL || (TAN && !RES && !INTEG), "MODE_DEFL_LOAD goes with MODE_TANGENT alone");
 
   __shared__ double s_x[EPB][ND], s_u[EPB][ND];
+  __shared__ double s_dl[EPB][DFL ? ND : 1];
   __shared__ int32_t s_dof[EPB][ND];
   __shared__ int32_t s_node[EPB][NT];
   __shared__ double s_q[EPB][NN][QD];
@@ -165,13 +168,16 @@ __global__ __launch_bounds__(256) void coh_elem(CohArgs a, int64_t e_begin, int6
       const int32_t d = a.dofid[int64_t(node) * 3 + b];
       s_dof[slot][lane * 3 + b] = d;
       double u = 0.0;  // def_elem_total, src/utils.cc:1391-1412
+      double dl = 0.0;
       if (d > 0) {
         u = a.r[d - 1] + (a.d_r ? a.d_r[d - 1] : 0.0);
       } else if (d < 0) {
         const int k = -d - 1;
-        u = (a.defl ? a.defl[k] : 0.0) + (a.defl_d ? a.defl_d[k] : 0.0);
+        dl = a.defl_d ? a.defl_d[k] : 0.0;
+        u = (a.defl ? a.defl[k] : 0.0) + (DFL ? 0.0 : dl);  // the prescribed-load pass: K at defl
       }
       s_u[slot][lane * 3 + b] = u;
+      if constexpr (DFL) s_dl[slot][lane * 3 + b] = dl;
     }
   }
   __syncthreads();
@@ -319,6 +325,19 @@ __global__ __launch_bounds__(256) void coh_elem(CohArgs a, int64_t e_begin, int6
       }
       K[c] = sr * sc * kt + sr * kn;
     }
+    if constexpr (DFL) {  // f_defl row: K row . prescribed increments (src/load.cc:190-215)
+      double fl = 0.0;
+#pragma unroll
+      for (int c = 0; c < ND; ++c) fl += K[c] * s_dl[slot][c];
+      if (a.stage) {
+        a.stage[a.soff[e * NT + ra] + M * (ND + 1) + ND] = fl;
+      } else {
+        const bool first = off[ra] <= -2;
+        if (first) a.f[dof - 1] = fl;
+        else a.f[dof - 1] += fl;
+      }
+      return;
+    }
     if (a.stage) {  // gather mode: row (ra, M) into the node's staging block
       double *srow = a.stage + a.soff[e * NT + ra] + M * (ND + 1);
 #pragma unroll
@@ -366,6 +385,9 @@ void launch_nn(int mode, const CohArgs &a, int64_t e0, int64_t e1, hipStream_t s
       break;
     case MODE_INTEGRATE | MODE_RESIDUAL | MODE_TANGENT:
       coh_elem<NN, MODE_INTEGRATE | MODE_RESIDUAL | MODE_TANGENT><<<grid, 256, 0, s>>>(a, e0, e1);
+      break;
+    case MODE_TANGENT | MODE_DEFL_LOAD:
+      coh_elem<NN, MODE_TANGENT | MODE_DEFL_LOAD><<<grid, 256, 0, s>>>(a, e0, e1);
       break;
     default:
       break;

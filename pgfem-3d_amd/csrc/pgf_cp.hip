@@ -903,7 +903,12 @@ __global__ __launch_bounds__(256) void cp_elem(CPElemArgs a, int64_t e_begin, in
   constexpr int EPB = (NNE == 8) ? 4 : 256 / TPE;
   constexpr bool RES = (MODE & MODE_RESIDUAL) != 0;
   constexpr bool TAN = (MODE & MODE_TANGENT) != 0;
+  constexpr bool DFL = (MODE & MODE_DEFL_LOAD) != 0;  // K times the prescribed increments
+  static_assert(!DFL || (TAN && !RES), "MODE_DEFL_LOAD goes with MODE_TANGENT alone");
+  constexpr bool WK = TAN && !DFL;  // K rows are written
+  constexpr bool WR = RES || DFL;   // the residual entry is written
   __shared__ double s_x[EPB][NDE], s_u[EPB][NDE];
+  __shared__ double s_dl[EPB][DFL ? NDE : 1];
   __shared__ int32_t s_dof[EPB][NDE], s_node[EPB][NNE];
   __shared__ double s_dN[EPB][NQP][NDE];
   __shared__ double s_Fr[EPB][NQP][9], s_Me[EPB][NQP][9], s_MeS[EPB][NQP][9], s_CrMe[EPB][NQP][9];
@@ -923,10 +928,15 @@ __global__ __launch_bounds__(256) void cp_elem(CPElemArgs a, int64_t e_begin, in
       s_x[slot][lane * 3 + b] = a.X[int64_t(node) * 3 + b];
       const int32_t d = a.dofid[int64_t(node) * 3 + b];
       s_dof[slot][lane * 3 + b] = d;
-      double u = 0.0;
-      if (d > 0) u = a.r[d - 1] + (a.d_r ? a.d_r[d - 1] : 0.0);
-      else if (d < 0) u = (a.defl ? a.defl[-d - 1] : 0.0) + (a.defl_d ? a.defl_d[-d - 1] : 0.0);
+      double u = 0.0, dl = 0.0;
+      if (d > 0) {
+        u = a.r[d - 1] + (a.d_r ? a.d_r[d - 1] : 0.0);
+      } else if (d < 0) {
+        dl = a.defl_d ? a.defl_d[-d - 1] : 0.0;
+        u = (a.defl ? a.defl[-d - 1] : 0.0) + (DFL ? 0.0 : dl);  // the prescribed-load pass: K at defl
+      }
       s_u[slot][lane * 3 + b] = u;
+      if constexpr (DFL) s_dl[slot][lane * 3 + b] = dl;
     }
   }
   __syncthreads();
@@ -1114,9 +1124,15 @@ __global__ __launch_bounds__(256) void cp_elem(CPElemArgs a, int64_t e_begin, in
     for (int i = 0; i < NDE; ++i) diag = (i == lane) ? K[i] : diag;
     if (!isnormal(diag)) atomicAdd(a.err, 1);
   }
+  if constexpr (DFL) {  // f_defl row: K row . prescribed increments (src/load.cc:190-215)
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < NDE; ++i) s += K[i] * s_dl[slot][i];
+    fe = s;
+  }
   }  // rowlane
   constexpr int LD = NDE + 1;
-  if constexpr (TAN && STG && NNE != 4) {
+  if constexpr (WK && STG && NNE != 4) {
     // Gather mode: RE elements at a time, the rows go to LDS (in the dead dM/dF array) laid
     // out as the staging blocks, then the workgroup copies each node block, zero-padded to
     // whole lines, to its staging offset with consecutive threads on consecutive doubles.
@@ -1153,19 +1169,19 @@ __global__ __launch_bounds__(256) void cp_elem(CPElemArgs a, int64_t e_begin, in
   if (dof <= 0) return;
   if constexpr (STG) {  // gather mode: the row straight into its staging block
     double *srow = a.stage + a.soff[e * NNE + ra] + rb * LD;
-    if constexpr (TAN) {
+    if constexpr (WK) {
 #pragma unroll
       for (int i = 0; i < NDE; ++i) srow[i] = K[i];
     }
-    if constexpr (RES) srow[NDE] = fe;
+    if constexpr (WR) srow[NDE] = fe;
     return;
   }
   const int16_t *off = a.off + (e * NNE + ra) * NNE;
-  if constexpr (RES) {
+  if constexpr (WR) {
     if (off[ra] <= -2) a.f[dof - 1] = fe;
     else a.f[dof - 1] += fe;
   }
-  if constexpr (TAN) {
+  if constexpr (WK) {
     const int32_t node = s_node[slot][ra];
     int bp = 0;
 #pragma unroll
@@ -1227,10 +1243,11 @@ void launch_ce(const CPElemArgs &a, int64_t e0, int64_t e1, hipStream_t s) {
 
 template <int NNE>
 void launch_ce_mode(int mode, const CPElemArgs &a, int64_t e0, int64_t e1, hipStream_t s) {
-  switch (mode & (MODE_RESIDUAL | MODE_TANGENT)) {
+  switch (mode & (MODE_RESIDUAL | MODE_TANGENT | MODE_DEFL_LOAD)) {
     case MODE_RESIDUAL: launch_ce<NNE, MODE_RESIDUAL>(a, e0, e1, s); break;
     case MODE_TANGENT: launch_ce<NNE, MODE_TANGENT>(a, e0, e1, s); break;
     case MODE_RESIDUAL | MODE_TANGENT: launch_ce<NNE, MODE_RESIDUAL | MODE_TANGENT>(a, e0, e1, s); break;
+    case MODE_TANGENT | MODE_DEFL_LOAD: launch_ce<NNE, MODE_TANGENT | MODE_DEFL_LOAD>(a, e0, e1, s); break;
     default: break;
   }
 }

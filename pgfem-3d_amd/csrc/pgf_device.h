@@ -66,7 +66,9 @@ struct ElemArgs {
 };
 
 // Mode bits of the element kernel.
-enum : int { MODE_INTEGRATE = 1, MODE_RESIDUAL = 2, MODE_TANGENT = 4 };
+// MODE_DEFL_LOAD (with MODE_TANGENT alone): the element tangent at (r + d_r, defl) times the
+// element's prescribed increments defl_d, summed into f in place of the residual; no K output.
+enum : int { MODE_INTEGRATE = 1, MODE_RESIDUAL = 2, MODE_TANGENT = 4, MODE_DEFL_LOAD = 8 };
 
 // ---------------------------------------------------------------- crystal plasticity
 // Per-grain tables: Schmid tensors P_a = s_a (x) m_a rotated to the sample frame (12 x 9,

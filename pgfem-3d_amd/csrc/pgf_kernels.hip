@@ -215,6 +215,10 @@ __global__ __launch_bounds__(256) void elem_kernel(ElemArgs a, int64_t e_begin, 
   constexpr bool INTEG = (MODE & MODE_INTEGRATE) != 0;
   constexpr bool RES = (MODE & MODE_RESIDUAL) != 0;
   constexpr bool TAN = (MODE & MODE_TANGENT) != 0;
+  constexpr bool DFL = (MODE & MODE_DEFL_LOAD) != 0;  // K times the prescribed increments
+  static_assert(!DFL || (TAN && !RES && !INTEG), "MODE_DEFL_LOAD goes with MODE_TANGENT alone");
+  constexpr bool WK = TAN && !DFL;  // K rows are written
+  constexpr bool WR = RES || DFL;   // the residual entry is written
   constexpr bool GEN = !INTEG;  // Fr may differ from the stored F(n+1)
   constexpr int LD = NDE + 1;   // staging row: K row, then the residual entry
   constexpr int kH = EPB * NQP * NDE, kC = EPB * NQP * (TAN ? 36 : 1);
@@ -232,6 +236,7 @@ __global__ __launch_bounds__(256) void elem_kernel(ElemArgs a, int64_t e_begin, 
   __shared__ double s_tau[EPB][NQP][6];
   __shared__ double s_g[EPB][NQP][GEN ? 9 : 1];
   __shared__ double s_w[EPB][NQP];
+  __shared__ double s_dl[EPB][DFL ? NDE : 1];  // prescribed increments of the element DOFs
 
   const int slot = threadIdx.x / TPE;
   const int lane = threadIdx.x % TPE;
@@ -251,13 +256,17 @@ __global__ __launch_bounds__(256) void elem_kernel(ElemArgs a, int64_t e_begin, 
       const int32_t d = a.dofid[int64_t(node) * 3 + b];
       s_dof[slot][lane * 3 + b] = d;
       double u = 0.0;  // def_elem_total, src/utils.cc:1391-1412
+      double dl = 0.0;
       if (d > 0) {
         u = a.r[d - 1] + (a.d_r ? a.d_r[d - 1] : 0.0);
       } else if (d < 0) {
         const int k = -d - 1;
-        u = (a.defl ? a.defl[k] : 0.0) + (a.defl_d ? a.defl_d[k] : 0.0);
+        dl = a.defl_d ? a.defl_d[k] : 0.0;
+        // the prescribed-load pass takes K at defl (defl_d zeroed, src/stiffmat_fd.cc:722-745)
+        u = (a.defl ? a.defl[k] : 0.0) + (DFL ? 0.0 : dl);
       }
       s_u[slot][lane * 3 + b] = u;
+      if constexpr (DFL) s_dl[slot][lane * 3 + b] = dl;
     }
   }
   lds_barrier();
@@ -470,8 +479,14 @@ __global__ __launch_bounds__(256) void elem_kernel(ElemArgs a, int64_t e_begin, 
       for (int i = 0; i < NDE; ++i) diag = (i == lane) ? K[i] : diag;
       if (!isnormal(diag)) atomicAdd(a.err, 1);
     }
+    if constexpr (DFL) {  // f_defl row: K row . prescribed increments (src/load.cc:190-215)
+      double s = 0.0;
+#pragma unroll
+      for (int i = 0; i < NDE; ++i) s += K[i] * s_dl[slot][i];
+      fe = s;
+    }
   }
-  if constexpr (TAN && STG) {
+  if constexpr (WK && STG) {
     // Gather mode: RE elements at a time, the rows go to LDS laid out as the staging blocks
     // ([node a][row b][K row, residual]); then the whole workgroup copies each node block
     // (3 (NDE+1) doubles, zero-padded to whole lines) to its staging offset, consecutive
@@ -511,12 +526,12 @@ __global__ __launch_bounds__(256) void elem_kernel(ElemArgs a, int64_t e_begin, 
     return;
   }
   const int16_t *off = a.off + (e * NNE + ra) * NNE;
-  if constexpr (RES) {
+  if constexpr (WR) {
     const bool first = off[ra] <= -2;
     if (first) a.f[dof - 1] = fe;
     else a.f[dof - 1] += fe;
   }
-  if constexpr (TAN) {
+  if constexpr (WK) {
     const int32_t node = s_node[slot][ra];
     int bp = 0;
 #pragma unroll
@@ -670,6 +685,9 @@ void launch_nne(int mode, const ElemArgs &a, int64_t e_begin, int64_t e_end, hip
     case MODE_INTEGRATE | MODE_RESIDUAL | MODE_TANGENT:
       elem_kernel<NNE, MODE_INTEGRATE | MODE_RESIDUAL | MODE_TANGENT, STG><<<grid, 256, 0, s>>>(a, e_begin, e_end);
       break;
+    case MODE_TANGENT | MODE_DEFL_LOAD:
+      elem_kernel<NNE, MODE_TANGENT | MODE_DEFL_LOAD, STG><<<grid, 256, 0, s>>>(a, e_begin, e_end);
+      break;
     default:
       break;
   }
@@ -696,7 +714,9 @@ void launch_gather(int mode, const GatherArgs &g, hipStream_t s) {
   if (g.n_nodes <= 0 || wpb < 1) return;
   const unsigned grid = unsigned((g.n_nodes + wpb - 1) / wpb);
   const size_t lds = size_t(wpb) * (3 * size_t(std::max<int32_t>(g.maxlen, 1)) + 3) * sizeof(double);
-  const bool res = (mode & MODE_RESIDUAL) != 0, tan = (mode & MODE_TANGENT) != 0;
+  // the prescribed-load pass stages its vector in the residual entries and no K rows
+  const bool dfl = (mode & MODE_DEFL_LOAD) != 0;
+  const bool res = (mode & MODE_RESIDUAL) != 0 || dfl, tan = (mode & MODE_TANGENT) != 0 && !dfl;
   if (res && tan) gather_kernel<true, true><<<grid, 64 * wpb, lds, s>>>(g);
   else if (res) gather_kernel<true, false><<<grid, 64 * wpb, lds, s>>>(g);
   else if (tan) gather_kernel<false, true><<<grid, 64 * wpb, lds, s>>>(g);

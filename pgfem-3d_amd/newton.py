@@ -1,14 +1,19 @@
 """Global Newton driver over the device element path (the role of Newton_Raphson,
 src/Newton_Raphson.cc:886-1430, for the total Lagrangian CM analysis on one GPU).
 
-Per load step: the prescribed values of step n+1 are applied; every iteration runs the
-fused integration + residual + tangent pass (pgf_residual_tangent), forms the residual
-R = RR - f_int (RR = 0: the workloads are displacement driven), checks the normalised norm
-||R|| / NORM against the tolerance (NORM = the largest first-iteration norm seen, as
-Newton_Raphson keeps fv->NORM, :1207-1216), and solves K du = R for the free DOFs -- on the
-device with pgf_solve (Jacobi-preconditioned BiCGStab), or on the host with a sparse direct
-solve.  A step that fails (positive INFO from the element pass, a diverging or non-finite
-residual, or no convergence in max_iter) returns its status so the caller can subdivide and
+Per load step, as the reference takes it when given the step's prescribed increment
+(load_step(r, defl_n, defl_d=increment)): the first iteration is the reference's predictor --
+the tangent and the load vector of the prescribed increment f_defl (pgf_prescribed_load) at
+(u_n, defl_n), K du = -f_defl (Newton_Raphson.cc:1700-1741, stiffmat_fd.cc:722-745) -- after
+which every iteration runs the fused integration + residual + tangent pass
+(pgf_residual_tangent) at defl_n + increment, forms R = RR - f_int (RR = 0: displacement
+driven), checks ||R|| / NORM against the tolerance and solves K du = R.  NORM is the largest
+norm of a step's first residual (the one after the predictor) seen so far, re-taken while it
+is below the tolerance (fv->NORM, :1207-1216).  Called with the full prescribed values of step
+n+1 instead (defl_d None), the step starts from them at u_n with no predictor.  Linear solves
+run on the device with pgf_solve (Jacobi-preconditioned BiCGStab), or on the host with a sparse
+direct solve.  A step that fails (positive INFO from the element pass, a diverging or
+non-finite residual, the iteration cap) returns its status so the caller can subdivide and
 pgf_reset_step, as the reference does.
 """
 from __future__ import annotations
@@ -62,12 +67,15 @@ def backtrack_gamma(gam, phi, prev=None):
 
 class NewtonDriver:
     def __init__(self, h: P.Handle, Ap, Ai, tol=1e-10, max_iter=25, solver="device", lin_tol=1e-13,
-                 lin_maxit=20000, computer_zero=1e-15, line_search=True):
+                 lin_maxit=20000, computer_zero=1e-15, line_search=True, iter_max=None):
+        """tol is the reference's nor_min; iter_max its iteration cap (None: max_iter only):
+        past iter_max iterations a residual above 20 tol, or any past iter_max + 3, ends the
+        step -- converged if below 5 tol, else failed (Newton_Raphson.cc:1333-1352)."""
         import torch
 
         self.h, self.tol, self.max_iter = h, tol, max_iter
         self.solver, self.lin_tol, self.lin_maxit = solver, lin_tol, lin_maxit
-        self.computer_zero, self.line_search = computer_zero, line_search
+        self.computer_zero, self.line_search, self.iter_max = computer_zero, line_search, iter_max
         self.Ap, self.Ai = Ap, Ai
         dev = torch.device("cuda:0")
         self.f = torch.empty(h.ndof, dtype=torch.float64, device=dev)
@@ -93,18 +101,21 @@ class NewtonDriver:
         x = spla.spsolve(K.tocsc(), R.cpu().numpy())
         return torch.from_numpy(np.ascontiguousarray(x)).to(R.device), 0
 
-    def increment(self, r, defl_from, defl_to, dt=1.0, max_cuts=4) -> StepResult:
+    def increment(self, r, defl_from, defl_to, dt=1.0, max_cuts=4, predictor=True) -> StepResult:
         """Take the prescribed values from defl_from (converged) to defl_to, subdividing on
         failure as the reference does (a failing step is reset with pgf_reset_step and the
         remaining increment is applied in halves, up to max_cuts levels; the time step is
-        split with it)."""
+        split with it).  predictor: each piece starts with the reference's predictor."""
         out = StepResult(False, 0)
         pieces = [(0.0, 1.0, 0)]
         r_conv = r.clone()
         while pieces:
             a, b, lvl = pieces.pop(0)
-            defl = defl_from + b * (defl_to - defl_from)
-            res = self.load_step(r, defl, dt=dt * (b - a))
+            if predictor:
+                res = self.load_step(r, defl_from + a * (defl_to - defl_from), dt=dt * (b - a),
+                                     defl_d=(b - a) * (defl_to - defl_from))
+            else:
+                res = self.load_step(r, defl_from + b * (defl_to - defl_from), dt=dt * (b - a))
             out.iterations += res.iterations
             out.norms += res.norms
             if res.converged:
@@ -123,28 +134,52 @@ class NewtonDriver:
 
     def load_step(self, r, defl, d_r=None, defl_d=None, dt=1.0) -> StepResult:
         """Solve one load step in place: r (free DOFs, device float64) enters as the converged
-        u(n) and leaves as u(n+1); defl holds the prescribed values of step n+1."""
+        u(n) and leaves as u(n+1).  With defl_d (device, the step's prescribed increment) defl
+        holds the prescribed values of step n and the step opens with the predictor; without
+        it defl holds those of step n+1."""
         import torch
 
         res = StepResult(False, 0)
+        it0 = 0
+        if defl_d is not None:
+            # predictor: K and f_defl at (u_n, defl_n), K du = -f_defl, no line search
+            info = self.h.tangent(self.vals, r, d_r=d_r, defl=defl, dt=dt)
+            if not info:
+                info = self.h.prescribed_load(self.f, r, defl_d, d_r=d_r, defl=defl, dt=dt)
+            if info:
+                res.info = info
+                return res
+            du, lit = self._solve(-self.f)
+            res.solver_iters.append((lit, self.last_relres))
+            r.add_(du)
+            res.gammas.append(1.0)
+            it0 = 1
 
         def run(x):
             return self.h.residual_tangent(self.f, self.vals, x, d_r=d_r, defl=defl, defl_d=defl_d, dt=dt)
 
         info = run(r)
-        for it in range(self.max_iter):
+        for it in range(it0, self.max_iter):
             if info:
                 res.info = info
                 return res
             R = -self.f
             nor2 = float(R.norm())
-            if self.first or self.NORM < self.tol:  # normalisation of the residual norm
+            # NORM from a step's first residual (Newton_Raphson.cc:1207-1216)
+            if it == it0 and (self.first or self.NORM < self.tol):
                 self.NORM = max(self.NORM, nor2, self.computer_zero)
                 self.first = False
             nor = nor2 / self.NORM
             res.norms.append(nor)
             res.iterations = it
             if not np.isfinite(nor):
+                return res
+            # iteration cap: this residual follows the reference's iteration it - 1
+            if (self.iter_max is not None and it > self.iter_max and nor > self.tol
+                    and (nor > 20.0 * self.tol or it > self.iter_max + 3)):
+                if nor < 5.0 * self.tol:
+                    res.converged = True
+                    self.h.commit_step()
                 return res
             if nor <= self.tol or nor2 <= self.computer_zero:
                 res.converged = True

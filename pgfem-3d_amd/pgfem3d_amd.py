@@ -94,6 +94,7 @@ def lib() -> C.CDLL:
             "pgf_get_sizes": [P, P, P, P, P],
             "pgf_residual": [P, P, D, C.c_int, P],
             "pgf_tangent": [P, P, D, P],
+            "pgf_prescribed_load": [P, P, D, P],
             "pgf_residual_tangent": [P, P, D, P, P],
             "pgf_commit_step": [P],
             "pgf_reset_step": [P],
@@ -135,8 +136,8 @@ def exported_symbols():
     """Names the C ABI exports (all functions declared in include/pgfem3d_amd.h)."""
     return [
         "pgf_generate_dof_ids", "pgf_psparse_apai", "pgf_mesh_box", "pgf_create", "pgf_destroy",
-        "pgf_set_stream", "pgf_set_profiling", "pgf_last_pass_ms", "pgf_synchronize", "pgf_get_sizes", "pgf_residual", "pgf_tangent",
-        "pgf_residual_tangent", "pgf_commit_step", "pgf_reset_step", "pgf_state_ncomp",
+        "pgf_set_stream", "pgf_set_profiling", "pgf_last_pass_ms", "pgf_synchronize", "pgf_get_sizes", "pgf_residual",
+        "pgf_tangent", "pgf_prescribed_load", "pgf_residual_tangent", "pgf_commit_step", "pgf_reset_step", "pgf_state_ncomp",
         "pgf_get_state", "pgf_set_state", "pgf_set_orientations", "pgf_set_time_step",
         "pgf_set_warm_start", "pgf_get_qp_order", "pgf_state_device_ptr", "pgf_set_cohesive_props",
         "pgf_get_cohesive_sizes", "pgf_mlp_create", "pgf_mlp_destroy", "pgf_mlp_param_count",
@@ -347,6 +348,14 @@ class Handle:
         inp = self._input(r, d_r, defl, defl_d)
         self._check_vec(vals, self.nnz, "vals")
         return _check(self._lib.pgf_tangent(self.h, C.byref(inp), dt, _ptr(vals)), "pgf_tangent")
+
+    def prescribed_load(self, f_defl, r, defl_d, d_r=None, defl=None, dt=1.0) -> int:
+        """f_defl = K(r + d_r, defl) . (defl_d on the prescribed DOFs), on the free rows."""
+        if defl_d is None:
+            raise PGFError("defl_d is required")
+        inp = self._input(r, d_r, defl, defl_d)
+        self._check_vec(f_defl, self.ndof, "f_defl")
+        return _check(self._lib.pgf_prescribed_load(self.h, C.byref(inp), dt, _ptr(f_defl)), "pgf_prescribed_load")
 
     def residual_tangent(self, f, vals, r, d_r=None, defl=None, defl_d=None, dt=1.0) -> int:
         inp = self._input(r, d_r, defl, defl_d)

@@ -74,9 +74,10 @@ def compare_with_fixture(prob, u, F_n, pF_n, g_n):
     return du, dF, dpF, dg
 
 
-def run_oracle(prob, rot, nsteps, tol=1e-10):
+def run_oracle(prob, rot, nsteps, tol=1e-10, predictor=False, iter_max=None, computer_zero=1e-15):
     """The deck's load steps over the oracle (Newton to tol on the normalised residual, sparse
-    direct solve): (nodal u, CPState, prescribed values, Newton iterations per step)."""
+    direct solve; with predictor each step opens with K du = -f_defl at (u_n, defl_n), as the
+    reference's Newton does): (nodal u, CPState, prescribed values, iterations per step)."""
     tabs = [O.cp_grain(prob.mat, x) for x in rot]
     gP = np.stack([t[0] for t in tabs])
     gL = np.stack([t[1] for t in tabs])
@@ -87,8 +88,15 @@ def run_oracle(prob, rot, nsteps, tol=1e-10):
     sf = prob.sf
     its = []
     for step in range(nsteps):
-        defl = defl + sf.increment(step, prob.deck.defl)
-        r, it, _ = oracle_newton_step(prob, st, r, defl, state, tol=tol, max_iter=40, cp_tables=(gP, gL),
-                                      dt=sf.times[step + 1] - sf.times[step])
+        inc = sf.increment(step, prob.deck.defl)
+        dt = sf.times[step + 1] - sf.times[step]
+        if predictor:
+            r, it, _ = oracle_newton_step(prob, st, r, defl, state, tol=tol, max_iter=40, cp_tables=(gP, gL),
+                                          dt=dt, defl_d=inc, iter_max=iter_max, computer_zero=computer_zero)
+            defl = defl + inc
+        else:
+            defl = defl + inc
+            r, it, _ = oracle_newton_step(prob, st, r, defl, state, tol=tol, max_iter=40, cp_tables=(gP, gL),
+                                          dt=dt, iter_max=iter_max, computer_zero=computer_zero)
         its.append(it)
     return nodal_u(prob, r, defl), st, defl, its

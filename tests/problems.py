@@ -80,41 +80,104 @@ class InterfaceBoxProblem(BoxProblem):
                               defl=defl)
 
 
+def expand_prescribed(prob, n_prescribed):
+    """The mesh's DOF numbering with every prescribed DOF k (id -k-1) made the free DOF
+    ndof + k + 1: (ids2, ndof2, Ap2, Ai2)."""
+    ids2 = prob.ids.copy()
+    pres = ids2 < 0
+    ids2[pres] = prob.ndof - ids2[pres]
+    ndof2 = prob.ndof + n_prescribed
+    m = prob.mesh
+    Ap2, Ai2 = O.psparse_apai(m.X, m.elem_ptr, m.elem_conn, ids2, ndof2, getattr(m, "coel_ptr", None),
+                              getattr(m, "coel_conn", None))
+    return ids2, ndof2, Ap2, Ai2
+
+
+def oracle_tangent_full(prob, st, r, defl, cp_tables=None, cst=None, dt=1.0):
+    """The oracle tangent at (r, defl) from the stored state (no integration) over the free and
+    the prescribed DOFs (prescribed k at column ndof + k): (CSR matrix, ndof)."""
+    import copy
+
+    import scipy.sparse as sp
+
+    ids2, ndof2, Ap2, Ai2 = expand_prescribed(prob, len(defl))
+    r2 = np.concatenate([r, defl])
+    if cp_tables is None:
+        _, vals2, err = O.assemble(prob.mesh, prob.mats, ids2, ndof2, Ap2, Ai2, copy.deepcopy(st), r2, dt=dt,
+                                   residual=False, integrate=False, tangent=True)
+    else:
+        _, vals2, err = O.assemble_cp(prob.mesh, prob.mats, prob.grain, cp_tables[0], cp_tables[1], ids2, ndof2,
+                                      Ap2, Ai2, copy.deepcopy(st), r2, dt=dt, residual=False, integrate=False,
+                                      tangent=True)
+    assert err == 0
+    if cst is not None:
+        f2 = np.zeros(ndof2)
+        assert O.assemble_coh(prob.mesh, prob.coh, None, ids2, ndof2, Ap2, Ai2, copy.deepcopy(cst), r2, f2,
+                              vals2) == 0
+    return sp.csr_matrix((vals2, Ai2, Ap2), shape=(ndof2, ndof2)), prob.ndof
+
+
+def oracle_prescribed_load(prob, st, r, defl, defl_d, cp_tables=None, cst=None, dt=1.0):
+    """f_defl of the reference's load step (momentum_equation_load4pBCs, src/load.cc:102-220):
+    the element tangents at (r, defl) applied to the prescribed increments defl_d, on the free
+    rows -- the free-row, prescribed-column block of oracle_tangent_full times defl_d."""
+    K2, n = oracle_tangent_full(prob, st, r, defl, cp_tables, cst, dt)
+    return K2[:n, n:] @ np.asarray(defl_d, dtype=np.float64)
+
+
 def oracle_newton_step(prob, st, r, defl, state, tol=1e-12, max_iter=25, cp_tables=None, dt=1.0,
-                       line_search=True):
+                       line_search=True, defl_d=None, iter_max=None, computer_zero=1e-15):
     """The Newton load step of pgfem-3d_amd/newton.py restated over the CPU oracle with a
     sparse direct solve: returns (r_np1, iterations, step lengths).  From the second iteration
     on, a step whose residual does not fall enough is cut back by the rule of LINE_S3_MP
-    (src/LINE.cc:58-250, restated in oracle.line_search_gamma).  state = dict(NORM=..., first=...)."""
+    (src/LINE.cc:58-250, restated in oracle.line_search_gamma).  state = dict(NORM=..., first=...).
+    With defl_d (the step's prescribed increment; defl then holds the values at n) the first
+    iteration is the reference's predictor: K and f_defl at (u_n, defl_n), K du = -f_defl
+    (src/Newton_Raphson.cc:1700-1741, src/stiffmat_fd.cc:722-745), and the residuals after it
+    are taken at defl + defl_d.  iter_max: the reference's iteration cap with its soft
+    acceptance (src/Newton_Raphson.cc:1333-1352)."""
     import scipy.sparse as sp
     import scipy.sparse.linalg as spla
 
-    def ev(x):
+    def ev(x, dfl):
         if cp_tables is None:
-            out = O.assemble(prob.mesh, prob.mats, prob.ids, prob.ndof, prob.Ap, prob.Ai, st, x, defl=defl)
+            out = O.assemble(prob.mesh, prob.mats, prob.ids, prob.ndof, prob.Ap, prob.Ai, st, x, defl=dfl, dt=dt)
         else:
             out = O.assemble_cp(prob.mesh, prob.mats, prob.grain, cp_tables[0], cp_tables[1], prob.ids,
-                                prob.ndof, prob.Ap, prob.Ai, st, x, defl=defl, dt=dt)
+                                prob.ndof, prob.Ap, prob.Ai, st, x, defl=dfl, dt=dt)
         assert out[2] == 0
         return out[0], out[1]
 
     r = r.copy()
     gams = []
-    f, vals = ev(r)
-    for it in range(max_iter):
+    it0 = 0
+    if defl_d is not None:
+        K2, n = oracle_tangent_full(prob, st, r, defl, cp_tables, dt=dt)
+        K = K2[:n, :n].tocsc()
+        r = r + spla.spsolve(K, -(K2[:n, n:] @ np.asarray(defl_d, dtype=np.float64)))
+        gams.append(1.0)
+        defl = defl + defl_d
+        it0 = 1
+    f, vals = ev(r, defl)
+    for it in range(it0, max_iter):
         R = -f
         nor2 = float(np.linalg.norm(R))
-        if state["first"] or state["NORM"] < tol:
-            state["NORM"] = max(state["NORM"], nor2, 1e-15)
+        if it == it0 and (state["first"] or state["NORM"] < tol):
+            state["NORM"] = max(state["NORM"], nor2, computer_zero)
             state["first"] = False
-        if nor2 / state["NORM"] <= tol or nor2 <= 1e-15:
+        nor = nor2 / state["NORM"]
+        if iter_max is not None and it > iter_max and nor > tol and (nor > 20 * tol or it > iter_max + 3):
+            assert nor < 5 * tol, "oracle Newton hit the iteration cap"
+            st.commit()
+            return r, it, gams
+        if nor <= tol or nor2 <= computer_zero:
             st.commit()
             return r, it, gams
         K = sp.csr_matrix((vals, prob.Ai, prob.Ap), shape=(prob.ndof, prob.ndof))
         du = spla.spsolve(K.tocsc(), R)
         gam, prev = 1.0, None
         while True:
-            f, vals = ev(r + gam * du)
+            f, vals = ev(r + gam * du, defl)
             if it == 0 or not line_search:
                 break
             phi = float(np.linalg.norm(f)) ** 2 / nor2 ** 2

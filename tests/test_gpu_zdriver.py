@@ -6,7 +6,12 @@ after every load step (both Newton loops run to a 1e-12 normalised residual), an
 line searches must take the same step lengths (1e-6; the step length is a smooth function of
 residual norms that agree to ~1e-12) wherever the residual is above rounding level, and
 iteration counts within one.  The driver's
-subdivision of a failing step is exercised separately."""
+subdivision of a failing step is exercised separately.
+
+With the reference's predictor (load_step(r, defl_n, defl_d=increment): K du = -f_defl at
+(u_n, defl_n) first), NORM is the residual left by the first step's predictor, which for an
+elastic first step is near round-off; those runs stop on the absolute criterion
+||R|| <= computer_zero as well, on both sides."""
 import numpy as np
 import pytest
 import torch
@@ -24,7 +29,7 @@ def _dev(x, device):
     return torch.from_numpy(np.ascontiguousarray(x)).to(device)
 
 
-def _run(prob, device, stretches, solver="device", cp=False):
+def _run(prob, device, stretches, solver="device", cp=False, predictor=False, computer_zero=1e-15):
     h = P.Handle(prob.mesh, prob.ids, prob.ndof, prob.Ap, prob.Ai, prob.mats)
     tables = None
     if cp:
@@ -33,20 +38,28 @@ def _run(prob, device, stretches, solver="device", cp=False):
         tables = (prob.gP, prob.gL)
     else:
         st = O.OracleState(prob.nqp, prob.mats[0].model == P.MODEL_J2_PLASTICITY)
-    drv = NW.NewtonDriver(h, prob.Ap, prob.Ai, tol=1e-12, solver=solver)
+    drv = NW.NewtonDriver(h, prob.Ap, prob.Ai, tol=1e-12, solver=solver, computer_zero=computer_zero)
     r_gpu = torch.zeros(prob.ndof, dtype=torch.float64, device=device)
     r_cpu = np.zeros(prob.ndof)
     g_prev = np.zeros(prob.ndof)
     ost = dict(NORM=0.0, first=True)
     iters = []
+    defl_n = None
     for s in stretches:
         guess, defl = prob.displacement(s, 0.0, seed=0)
-        r_gpu += _dev(guess - g_prev, device)  # converged u(n) + homogeneous increment
-        res = drv.load_step(r_gpu, _dev(defl, device))
+        if predictor:  # from u(n) at defl(n), the predictor takes the increment
+            dn = np.zeros_like(defl) if defl_n is None else defl_n
+            res = drv.load_step(r_gpu, _dev(dn, device), defl_d=_dev(defl - dn, device))
+            r_cpu, it_cpu, gam_cpu = oracle_newton_step(prob, st, r_cpu, dn, ost, tol=1e-12, cp_tables=tables,
+                                                         defl_d=defl - dn, computer_zero=computer_zero)
+            defl_n = defl
+        else:
+            r_gpu += _dev(guess - g_prev, device)  # converged u(n) + homogeneous increment
+            res = drv.load_step(r_gpu, _dev(defl, device))
+            r_cpu, it_cpu, gam_cpu = oracle_newton_step(prob, st, r_cpu + guess - g_prev, defl, ost, tol=1e-12,
+                                                         cp_tables=tables, computer_zero=computer_zero)
+            g_prev = guess
         assert res.converged, (s, res.norms, res.info, res.solver_iters, res.gammas)
-        r_cpu, it_cpu, gam_cpu = oracle_newton_step(prob, st, r_cpu + guess - g_prev, defl, ost, tol=1e-12,
-                                                     cp_tables=tables)
-        g_prev = guess
         u = r_gpu.cpu().numpy()
         rel = np.abs(u - r_cpu).max() / np.abs(r_cpu).max()
         assert rel < TOL_U, (s, rel)
@@ -87,6 +100,19 @@ def test_crystal_plasticity_steps(device):
     assert all(g < 1.0 for _, _, g in iters), iters  # every step needed a cut
 
 
+def test_predictor_j2_tet10_into_plasticity(device):
+    """The reference's predictor on the J2 steps; the first (elastic) step's predictor leaves
+    a residual near round-off, so the absolute criterion 1e-7 (forces of order 1e3) ends it."""
+    prob = BoxProblem(3, P.PGF_TET10, P.j2_linear(70e3, 0.33, 250.0, 1000.0))
+    iters = _run(prob, device, [0.003, 0.006, 0.01], predictor=True, computer_zero=1e-7)
+    assert all(a <= 8 for a, _, _ in iters), iters
+
+
+def test_predictor_crystal_plasticity(device):
+    prob = CPBoxProblem(3, P.PGF_TET4, P.fcc_crystal(), n_grains=4, seed=2)
+    _run(prob, device, [1e-3, 2e-3, 3e-3, 4e-3], cp=True, predictor=True, computer_zero=1e-9)
+
+
 def test_subdivision_recovers_a_large_step(device):
     """A 4% crystal-plasticity stretch from rest applied at once fails in the local Newton of
     its first pass (7 QPs on the oracle); the driver cuts it into halves until every piece
@@ -98,7 +124,7 @@ def test_subdivision_recovers_a_large_step(device):
     r = torch.zeros(prob.ndof, dtype=torch.float64, device=device)
     _, d0 = prob.displacement(0.0, 0.0, seed=0)
     _, d1 = prob.displacement(4e-2, 0.0, seed=0)
-    res = drv.increment(r, _dev(d0, device), _dev(d1, device), max_cuts=6)
+    res = drv.increment(r, _dev(d0, device), _dev(d1, device), max_cuts=6, predictor=False)
     assert res.converged, res
     assert res.cuts >= 1
     h.close()
