@@ -267,6 +267,7 @@ class Handle:
         self._lib = lib()
         mats = (Material * len(materials))(*materials)
         self._mats = mats
+        self.materials = list(materials)
         h = C.c_void_p()
         m = mesh.struct()
         _check(self._lib.pgf_create(C.byref(m), _ptr(ids), ndof, _ptr(Ap), _ptr(Ai), len(materials), mats,

@@ -17,7 +17,11 @@ Formats (one rank per partition file, as PGFem3D reads them):
   <base>_<p>_<step>.res  restart: nodal u(n-1), u(n), then per element its QP count and each
                      QP's model record (write_restart_constitutive_model,
                      src/restart.cc:187-245; CP_PARAM::write_restart,
-                     src/plasticity_model.cc:883-912)
+                     src/plasticity_model.cc:883-912; HE_PARAM::write_restart,
+                     src/plasticity_model_none.cc:300-312), kept under
+                     <opath>/restart/<physics>/STEP_<step>/ (:199-208)
+  time_step_info_<step>.res  t(n-1) t(n) t(n+1), then NORM and t(n) per physics
+                     (write_time_step_info / read_time_step_info, src/restart.cc:82-165)
 
 Partitions are merged into one mesh: a node with a global number (Gnn >= 0) is the same node
 in every partition that lists it; a node with Gnn = -1 belongs to its partition alone.
@@ -294,8 +298,26 @@ class CPRecord:
     lam_nm1: float
 
 
-def read_restart_cp(path, nn, ndofn, ne):
-    """(u_nm1 [nn, ndofn], u_n [nn, ndofn], [[CPRecord per QP] per element])."""
+@dataclass
+class HERecord:
+    """One QP's hyperelastic restart record."""
+    F_n: np.ndarray
+    F_nm1: np.ndarray
+
+
+def restart_file(opath, ofname, rank, step, physics="Mechanical"):
+    """Where the reference keeps a rank's restart file of a step."""
+    return os.path.join(opath, "restart", physics, f"STEP_{step:06d}", f"{ofname}_{rank}_{step}.res")
+
+
+def time_step_info_file(opath, step):
+    return os.path.join(opath, "restart", f"time_step_info_{step:06d}.res")
+
+
+def read_restart(path, nn, ndofn, kinds, nqp=None):
+    """(u_nm1 [nn, ndofn], u_n [nn, ndofn], [[record per QP] per element]); kinds[e] is "cp"
+    or "he", the model of element e; nqp[e], when given, its QP count, which the file must
+    hold (read_restart_constitutive_model checks it, src/restart.cc:300-309)."""
     t = _Tokens(path)
     u_nm1 = np.zeros((nn, ndofn))
     u_n = np.zeros((nn, ndofn))
@@ -303,20 +325,38 @@ def read_restart_cp(path, nn, ndofn, ne):
         for b in range(ndofn):
             u_nm1[a, b], u_n[a, b] = t.float(), t.float()
     elems = []
-    for _ in range(ne):
+    for e, kind in enumerate(kinds):
+        n_ip = t.int()
+        if nqp is not None and n_ip != nqp[e]:
+            raise ValueError(f"{path}: element {e} has {n_ip} QPs in the file, {nqp[e]} in the mesh")
         recs = []
-        for _ in range(t.int()):
-            F_n, F_nm1, pF_n, pF_nm1 = (np.array(t.floats(9)) for _ in range(4))
-            tau, gd = np.array(t.floats(12)), np.array(t.floats(12))
-            recs.append(CPRecord(F_n, F_nm1, pF_n, pF_nm1, tau, gd, *t.floats(4)))
+        for _ in range(n_ip):
+            if kind == "cp":
+                F_n, F_nm1, pF_n, pF_nm1 = (np.array(t.floats(9)) for _ in range(4))
+                tau, gd = np.array(t.floats(12)), np.array(t.floats(12))
+                recs.append(CPRecord(F_n, F_nm1, pF_n, pF_nm1, tau, gd, *t.floats(4)))
+            elif kind == "he":
+                recs.append(HERecord(np.array(t.floats(9)), np.array(t.floats(9))))
+            else:
+                raise ValueError(f"no restart record of model kind {kind!r}")
         elems.append(recs)
     return u_nm1, u_n, elems
 
 
-def write_restart_cp(path, u_nm1, u_n, elems):
+def read_restart_cp(path, nn, ndofn, ne):
+    """read_restart with crystal plasticity on every element."""
+    return read_restart(path, nn, ndofn, ["cp"] * ne)
+
+
+def _row(v, trail):
+    return " ".join(f"{x:.17e}" for x in v) + (" \n" if trail else "\n")
+
+
+def write_restart(path, u_nm1, u_n, elems):
     """The same layout, %.17e as the reference writes it (tensors one row each with no
-    trailing blank, slip values with one): a file read by read_restart_cp is written back
+    trailing blank, slip values with one): a file read by read_restart is written back
     byte for byte."""
+    os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
     with open(path, "w") as fh:
         for a in range(u_n.shape[0]):
             for b in range(u_n.shape[1]):
@@ -325,11 +365,39 @@ def write_restart_cp(path, u_nm1, u_n, elems):
         for recs in elems:
             fh.write(f"{len(recs)}\n")
             for r in recs:
-                for v in (r.F_n, r.F_nm1, r.pF_n, r.pF_nm1):
-                    fh.write(" ".join(f"{x:.17e}" for x in v) + "\n")
-                fh.write(" ".join(f"{x:.17e}" for x in r.tau_n) + " \n")
-                fh.write(" ".join(f"{x:.17e}" for x in r.gdot_n) + " \n")
-                fh.write(f"{r.g_n:.17e} {r.g_nm1:.17e} {r.lam_n:.17e} {r.lam_nm1:.17e}\n")
+                if isinstance(r, CPRecord):
+                    for v in (r.F_n, r.F_nm1, r.pF_n, r.pF_nm1):
+                        fh.write(_row(v, False))
+                    fh.write(_row(r.tau_n, True))
+                    fh.write(_row(r.gdot_n, True))
+                    fh.write(f"{r.g_n:.17e} {r.g_nm1:.17e} {r.lam_n:.17e} {r.lam_nm1:.17e}\n")
+                else:
+                    fh.write(_row(r.F_n, False))
+                    fh.write(_row(r.F_nm1, False))
+
+
+write_restart_cp = write_restart
+
+
+def write_time_step_info(path, t3, norms, tns):
+    """t3 = (t(n-1), t(n), t(n+1)); norms and tns per physics."""
+    os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+    with open(path, "w") as fh:
+        fh.write(f"{t3[0]:.17e} {t3[1]:.17e} {t3[2]:.17e} ")
+        for nv, tv in zip(norms, tns):
+            fh.write(f"\n{nv:.17e} {tv:.17e}")
+        fh.write("\n")
+
+
+def read_time_step_info(path, nphysics=1):
+    """(t3, norms, tns); a file without t(n) per physics (an older layout) gives tns None."""
+    t = _Tokens(path)
+    t3 = tuple(t.floats(3))
+    norms, tns = [], []
+    for _ in range(nphysics):
+        norms.append(t.float())
+        tns.append(t.float() if t.more() else None)
+    return t3, norms, tns
 
 
 def euler_rotation(phi, theta, psi):

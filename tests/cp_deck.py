@@ -100,3 +100,11 @@ def run_oracle(prob, rot, nsteps, tol=1e-10, predictor=False, iter_max=None, com
                                           dt=dt, iter_max=iter_max, computer_zero=computer_zero)
         its.append(it)
     return nodal_u(prob, r, defl), st, defl, its
+
+
+def free_dofs(prob, u):
+    """The free-DOF vector of nodal values u [n_nodes, 3]."""
+    r = np.zeros(prob.ndof)
+    free = prob.ids > 0
+    r[prob.ids[free] - 1] = u[free]
+    return r

@@ -119,3 +119,34 @@ def test_euler_rotation():
     assert np.allclose(R.euler_rotation(0, 0, np.pi / 2) @ [1, 0, 0], [0, 1, 0], atol=1e-15)
     assert np.allclose(R.euler_rotation(np.pi / 2, 0, 0) @ [0, 1, 0], [0, 0, 1], atol=1e-15)
     assert np.allclose(R.euler_rotation(0, np.pi / 2, 0) @ [0, 0, 1], [1, 0, 0], atol=1e-15)
+
+
+def test_time_step_info_of_the_fixture():
+    """The fixture's file holds t(n-1), t(n), t(n+1) and NORM (an older layout without t(n)
+    per physics)."""
+    t3, norms, tns = R.read_time_step_info(os.path.join(D, "ref_out", "time_step_info.res"))
+    assert t3 == (9.79794966462717645e-03, 9.99800463196023372e-03, 1.00000000000000002e-02)
+    assert norms == [2.25794289851497188e-04] and tns == [None]
+
+
+def test_time_step_info_round_trip(tmp_path):
+    path = R.time_step_info_file(str(tmp_path), 7)
+    assert path.endswith(os.path.join("restart", "time_step_info_000007.res"))
+    R.write_time_step_info(path, (0.1, 0.2, 0.3), [1.5e-3], [0.2])
+    assert R.read_time_step_info(path) == ((0.1, 0.2, 0.3), [1.5e-3], [0.2])
+
+
+def test_hyperelastic_records_round_trip(tmp_path):
+    rng = np.random.default_rng(2)
+    u1, u2 = rng.standard_normal((5, 3)), rng.standard_normal((5, 3))
+    elems = [[R.HERecord(rng.standard_normal(9), rng.standard_normal(9)) for _ in range(n)] for n in (1, 4, 8)]
+    path = R.restart_file(str(tmp_path), "box", 3, 12)
+    assert path.endswith(os.path.join("restart", "Mechanical", "STEP_000012", "box_3_12.res"))
+    R.write_restart(path, u1, u2, elems)
+    a, b, back = R.read_restart(path, 5, 3, ["he"] * 3, nqp=[1, 4, 8])
+    assert np.array_equal(a, u1) and np.array_equal(b, u2)
+    for e0, e1 in zip(elems, back):
+        for r0, r1 in zip(e0, e1):
+            assert np.array_equal(r0.F_n, r1.F_n) and np.array_equal(r0.F_nm1, r1.F_nm1)
+    with pytest.raises(ValueError, match="QPs"):
+        R.read_restart(path, 5, 3, ["he"] * 3, nqp=[1, 4, 4])
