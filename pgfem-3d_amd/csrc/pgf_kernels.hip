@@ -36,6 +36,8 @@ namespace {
 
 using namespace dev;
 
+typedef double dbl2 __attribute__((ext_vector_type(2)));
+
 // Sum over aligned groups of W lanes (butterfly: every lane ends with the same bits, since
 // each step adds the same two values in either order).
 template <int W>
@@ -66,7 +68,7 @@ __device__ __forceinline__ double pick6(double v0, double v1, double v2, double 
 }
 
 // Row I of c (row = stress component, column = engineering strain component).
-__device__ __forceinline__ void j2_c_row(const J2Tan &T, int I, double *row) {
+__device__ __forceinline__ void j2_c_row(const J2Tan &T, int I, double w, double *row) {
   const double nI = pick6(T.n[0], T.n[1], T.n[2], T.n[3], T.n[4], T.n[5], I);
   const double sI = pick6(T.s_tr[0], T.s_tr[1], T.s_tr[2], T.s_tr[3], T.s_tr[4], T.s_tr[5], I);
   const double nrm_I = I < 3 ? 1.0 : 0.0;
@@ -75,7 +77,7 @@ __device__ __forceinline__ void j2_c_row(const J2Tan &T, int I, double *row) {
     const double oo = (I < 3 && K < 3) ? 1.0 : 0.0;
     const double Is = (I == K) ? (I < 3 ? 1.0 : 0.5) : 0.0;
     const double cbar = T.tm * (Is - oo * (1.0 / 3.0)) - (2.0 / 3.0) * (sI * (K < 3 ? 1.0 : 0.0) + nrm_I * T.s_tr[K]);
-    row[K] = T.cv1 * oo + T.cv2 * Is + T.a1 * cbar - T.tm * T.b3 * nI * T.n[K] - T.tm * T.b4 * nI * T.dn2[K];
+    row[K] = w * (T.cv1 * oo + T.cv2 * Is + T.a1 * cbar - T.tm * T.b3 * nI * T.n[K] - T.tm * T.b4 * nI * T.dn2[K]);
   }
 }
 
@@ -207,7 +209,7 @@ constexpr int stage_round(int epb, int nde, int room) {
 }
 
 template <int NNE, int MODE, bool STG>
-__global__ __launch_bounds__(256) void elem_kernel(ElemArgs a, int64_t e_begin, int64_t e_end) {
+__global__ __launch_bounds__(256, STG ? 4 : 1) void elem_kernel(ElemArgs a, int64_t e_begin, int64_t e_end) {
   constexpr int NQP = nqp_of(NNE);
   constexpr int NDE = NNE * 3;
   constexpr int TPE = (NNE == 4) ? 16 : 32;
@@ -235,7 +237,6 @@ __global__ __launch_bounds__(256) void elem_kernel(ElemArgs a, int64_t e_begin, 
   auto s_c = reinterpret_cast<double (*)[NQP][TAN ? 36 : 1]>(s_work + kH);
   __shared__ double s_tau[EPB][NQP][6];
   __shared__ double s_g[EPB][NQP][GEN ? 9 : 1];
-  __shared__ double s_w[EPB][NQP];
   __shared__ double s_dl[EPB][DFL ? NDE : 1];  // prescribed increments of the element DOFs
 
   const int slot = threadIdx.x / TPE;
@@ -271,14 +272,22 @@ __global__ __launch_bounds__(256) void elem_kernel(ElemArgs a, int64_t e_begin, 
   }
   lds_barrier();
 
-  // ---- phase 1: quadrature points, LPQ lanes per QP (lane = ip * LPQ + sub).  Node loops
-  // are split over the QP's lanes (node n on sub-lane n % LPQ, which alone reads and writes
-  // its entries of s_h) and summed with butterfly shuffles, which leave every lane with the
-  // same bits; the return map runs on every lane; row I of c goes to sub-lanes I, I+LPQ, ...
-  constexpr int LPQ = TPE / NQP;
-  static_assert(LPQ * NQP == TPE && (LPQ & (LPQ - 1)) == 0, "lanes per QP");
-  if (active) {
-    const int ip = lane / LPQ, sub = lane % LPQ;
+  // ---- phase 1: quadrature points on two of the four waves (the other two skip it, so a
+  // workgroup issues the QP stream twice, not four times), LPQ lanes per QP: thread
+  // p = slot * NQP * LPQ + ip * LPQ + sub.  Blocks alternate which pair of waves works, so
+  // the QP work spreads over the SIMDs.  Node loops are split over the QP's lanes (node n on
+  // sub-lane n % LPQ, which alone reads and writes its entries of s_h) and summed with
+  // butterfly shuffles, which leave every lane with the same bits; the return map runs on
+  // every lane; row I of c goes to sub-lanes I, I+LPQ, ...
+  constexpr int P1T = 64;
+  constexpr int LPQ = P1T / (EPB * NQP);
+  static_assert(LPQ * NQP * EPB == P1T && (LPQ & (LPQ - 1)) == 0 && LPQ <= 64, "lanes per QP");
+  const int p1 = int(threadIdx.x) - int(blk & 3) * 64;
+  if (p1 >= 0 && p1 < P1T && e_begin + blk * EPB + p1 / (NQP * LPQ) < e_end) {
+    const int slot = p1 / (NQP * LPQ);  // the QP's element (shadows the row lanes' slot)
+    const int64_t ie1 = e_begin + blk * EPB + slot;
+    const int64_t e = a.elist ? int64_t(a.elist[ie1]) : ie1;
+    const int ip = (p1 / LPQ) % NQP, sub = p1 % LPQ;
     double k, ee, z, wt;
     int_point(NNE, ip, k, ee, z, wt);
     double *hq = s_h[slot][ip];  // dN/dX first, h_a = Fs^-T dN_a/dX at the end
@@ -372,24 +381,23 @@ __global__ __launch_bounds__(256) void elem_kernel(ElemArgs a, int64_t e_begin, 
     J2Tan T;
     j2_point<TAN>(m, is_j2, Fs, Fi, ok ? J : 1.0, Cn, ep_n, (INTEG && is_j2) ? Cnp1 : nullptr, &ep1,
                   &gm, tau, T);
+    // tau and c go to LDS weighted by the QP's detJ w (zero for a failed QP)
+    const double wq = ok ? J0 * wt : 0.0;
 #pragma unroll
     for (int I = 0; I < 6; ++I)
       if (I % LPQ == sub) {
-        s_tau[slot][ip][I] = tau[I];
+        s_tau[slot][ip][I] = wq * tau[I];
         if (INTEG && is_j2) a.P_np1[I * NQ + q] = Cnp1[I];
       }
-    if (sub == 0) {
-      if (INTEG && is_j2) {
-        a.Sc_np1[q] = ep1;
-        a.Sc_np1[NQ + q] = gm;
-      }
-      s_w[slot][ip] = ok ? J0 * wt : 0.0;
+    if (sub == 0 && INTEG && is_j2) {
+      a.Sc_np1[q] = ep1;
+      a.Sc_np1[NQ + q] = gm;
     }
     if constexpr (TAN) {
 #pragma unroll
       for (int t = 0; t < (6 + LPQ - 1) / LPQ; ++t) {
         const int I = sub + t * LPQ;
-        if (I < 6) j2_c_row(T, I, &s_c[slot][ip][I * 6]);
+        if (I < 6) j2_c_row(T, I, wq, &s_c[slot][ip][I * 6]);
       }
     }
     if constexpr (GEN) {  // Gr = Fr Fs^-1
@@ -419,35 +427,46 @@ __global__ __launch_bounds__(256) void elem_kernel(ElemArgs a, int64_t e_begin, 
   }
   double fe = 0.0;
   if (rowlane) {
+    // sym(e_b (x) h_a) in engineering Voigt order has three entries: h_a[b] at (b,b) and
+    // h_a[j1], h_a[j2] at the shears (b,j1), (b,j2)
+    const int j1 = (rb + 1) % 3, j2 = (rb + 2) % 3;
+    const int s1 = 3 + rb, s2 = 3 + j2;
 #pragma unroll
     for (int ip = 0; ip < NQP; ++ip) {
-      const double wq = s_w[slot][ip];
       const double *h = s_h[slot][ip];
-      const double *tau = s_tau[slot][ip];
+      const double *tau = s_tau[slot][ip];  // detJ w tau
       const double ha[3] = {h[ra * 3 + 0], h[ra * 3 + 1], h[ra * 3 + 2]};
       double t[3];
-      symv(tau, ha, t);  // tau h_a
+      symv(tau, ha, t);  // detJ w tau h_a
       double g[3] = {eb0, eb1, eb2};
       if constexpr (GEN) {
         g[0] = s_g[slot][ip][rb * 3 + 0];
         g[1] = s_g[slot][ip][rb * 3 + 1];
         g[2] = s_g[slot][ip][rb * 3 + 2];
       }
-      if constexpr (RES) fe += wq * (g[0] * t[0] + g[1] * t[1] + g[2] * t[2]);
+      if constexpr (RES) fe += g[0] * t[0] + g[1] * t[1] + g[2] * t[2];
       if constexpr (TAN) {
-        const double *c = s_c[slot][ip];
-        // engineering strain vector of sym(g (x) h_a)
-        const double bv[6] = {g[0] * ha[0], g[1] * ha[1], g[2] * ha[2], g[0] * ha[1] + g[1] * ha[0],
-                              g[1] * ha[2] + g[2] * ha[1], g[0] * ha[2] + g[2] * ha[0]};
+        const double *c = s_c[slot][ip];  // detJ w c
         double r[6];
+        if constexpr (GEN) {
+          // engineering strain vector of sym(g (x) h_a)
+          const double bv[6] = {g[0] * ha[0], g[1] * ha[1], g[2] * ha[2], g[0] * ha[1] + g[1] * ha[0],
+                                g[1] * ha[2] + g[2] * ha[1], g[0] * ha[2] + g[2] * ha[0]};
 #pragma unroll
-        for (int J = 0; J < 6; ++J) {
-          double acc = 0.0;
+          for (int J = 0; J < 6; ++J) {
+            double acc = 0.0;
 #pragma unroll
-          for (int I = 0; I < 6; ++I) acc += bv[I] * c[I * 6 + J];
-          r[J] = wq * acc;
+            for (int I = 0; I < 6; ++I) acc += bv[I] * c[I * 6 + J];
+            r[J] = acc;
+          }
+        } else {
+          const double hb = h[ra * 3 + rb], hj1 = h[ra * 3 + j1], hj2 = h[ra * 3 + j2];
+          const double *cb = c + rb * 6, *c1 = c + s1 * 6, *c2 = c + s2 * 6;
+#pragma unroll
+          for (int J = 0; J < 6; ++J) r[J] = hb * cb[J] + hj1 * c1[J] + hj2 * c2[J];
         }
-        // M = Gr R with R = mat(r) (symmetric)
+        // M = Gr R with R = mat(r) (symmetric), plus the geometric term on row b:
+        // K_(ab)(wg) = sum_k M_gk h_w,k with M_bk += (detJ w tau h_a)_k
         double M[9];
         if constexpr (GEN) {
           const double *Gm = s_g[slot][ip];
@@ -462,14 +481,18 @@ __global__ __launch_bounds__(256) void elem_kernel(ElemArgs a, int64_t e_begin, 
           M[3] = r[3]; M[4] = r[1]; M[5] = r[4];
           M[6] = r[5]; M[7] = r[4]; M[8] = r[2];
         }
-        const double wt0 = wq * t[0], wt1 = wq * t[1], wt2 = wq * t[2];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          M[0 + k] += eb0 * t[k];
+          M[3 + k] += eb1 * t[k];
+          M[6 + k] += eb2 * t[k];
+        }
 #pragma unroll
         for (int w = 0; w < NNE; ++w) {
           const double h0 = h[w * 3 + 0], h1 = h[w * 3 + 1], h2 = h[w * 3 + 2];
-          const double geo = wt0 * h0 + wt1 * h1 + wt2 * h2;  // delta_bg h_a.tau.h_w
-          K[w * 3 + 0] += M[0] * h0 + M[1] * h1 + M[2] * h2 + eb0 * geo;
-          K[w * 3 + 1] += M[3] * h0 + M[4] * h1 + M[5] * h2 + eb1 * geo;
-          K[w * 3 + 2] += M[6] * h0 + M[7] * h1 + M[8] * h2 + eb2 * geo;
+          K[w * 3 + 0] += M[0] * h0 + M[1] * h1 + M[2] * h2;
+          K[w * 3 + 1] += M[3] * h0 + M[4] * h1 + M[5] * h2;
+          K[w * 3 + 2] += M[6] * h0 + M[7] * h1 + M[8] * h2;
         }
       }
     }
@@ -504,15 +527,21 @@ __global__ __launch_bounds__(256) void elem_kernel(ElemArgs a, int64_t e_begin, 
         row[NDE] = fe;
       }
       lds_barrier();
-      for (int idx = threadIdx.x; idx < RE * NNE * PB; idx += 256) {
-        const int el = idx / (NNE * PB), o = idx - el * (NNE * PB);
-        const int na = o / PB, ob = o - na * PB;
+      // two doubles per thread (blocks are whole 128-byte lines, so pairs are 16-byte aligned)
+      constexpr int PB2 = PB / 2;
+      for (int idx = threadIdx.x; idx < RE * NNE * PB2; idx += 256) {
+        const int el = idx / (NNE * PB2), o = idx - el * (NNE * PB2);
+        const int na = o / PB2, ob = 2 * (o - na * PB2);
         const int64_t ee = e_blk + r0 + el;
         if (ee >= e_end) continue;
         const int64_t so = a.soff[ee * NNE + na];
         if (so < 0) continue;
+        const double *src = s_work + (el * NDE + na * 3) * LD + ob;
+        dbl2 v;
+        v.x = (ob < 3 * LD) ? src[0] : 0.0;
+        v.y = (ob + 1 < 3 * LD) ? src[1] : 0.0;
         // streamed past L2: the gather reads it in a later launch
-        __builtin_nontemporal_store((ob < 3 * LD) ? s_work[(el * NDE + na * 3) * LD + ob] : 0.0, a.stage + so + ob);
+        __builtin_nontemporal_store(v, reinterpret_cast<dbl2 *>(a.stage + so + ob));
       }
       lds_barrier();
     }
