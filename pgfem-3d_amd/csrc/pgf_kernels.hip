@@ -83,13 +83,13 @@ __device__ __forceinline__ void j2_c_row(const J2Tan &T, int I, double w, double
 
 // Neo-Hookean / J2 material point in the current configuration.
 // In: Fs, its inverse and J; J2 state (Cp^-1 at n, ep_n).  Out: tau (Voigt), the pieces of
-// the tangent c (J2Tan, when WITH_C) and, when integrating, the n+1 state.  HE is the J2
-// elastic branch with Cp^-1 = I.
+// the tangent c (J2Tan, when WITH_C) and, when integ, the n+1 state (outputs by reference, so
+// they stay in registers).  HE is the J2 elastic branch with Cp^-1 = I.
 template <bool WITH_C>
 __device__ __forceinline__ void j2_point(const DevMaterial &m, bool plastic, const double *Fs,
                                          const double *Fi, double J, const double *Cn,
-                                         double ep_n, double *Cnp1, double *ep1, double *gam,
-                                         double *tau, J2Tan &tg) {
+                                         double ep_n, bool integ, double (&Cnp1)[6], double &ep1,
+                                         double &gam, double *tau, J2Tan &tg) {
   // b_bar trial = J^-2/3 Fs Cn Fs^T (Voigt)
   double T[9];
 #pragma unroll
@@ -131,12 +131,13 @@ __device__ __forceinline__ void j2_point(const DevMaterial &m, bool plastic, con
 #pragma unroll
     for (int I = 0; I < 6; ++I) s[I] = s_tr[I];
   }
-  if (Cnp1) {
+  if (integ) {
     if (yield) {
       // Cp^-1(n+1) = J^2/3 Fi (s/G + Ibar 1) Fi^T
       double be[6];
+      const double rG = 1.0 / G;
 #pragma unroll
-      for (int I = 0; I < 6; ++I) be[I] = s[I] / G + (I < 3 ? Ibar : 0.0);
+      for (int I = 0; I < 6; ++I) be[I] = s[I] * rG + (I < 3 ? Ibar : 0.0);
       double U[9];
 #pragma unroll
       for (int i = 0; i < 3; ++i) {
@@ -155,8 +156,8 @@ __device__ __forceinline__ void j2_point(const DevMaterial &m, bool plastic, con
 #pragma unroll
       for (int I = 0; I < 6; ++I) Cnp1[I] = Cn[I];
     }
-    *ep1 = ep_n + sqrt(2.0 / 3.0) * dg;
-    *gam = dg;
+    ep1 = ep_n + sqrt(2.0 / 3.0) * dg;
+    gam = dg;
   }
   double dU, d2U;
   vol_pot(m.volPotFlag, J, dU, d2U);
@@ -167,11 +168,13 @@ __device__ __forceinline__ void j2_point(const DevMaterial &m, bool plastic, con
     const double cv1 = m.kappa * (J * J * d2U + J * dU), cv2 = -2.0 * m.kappa * J * dU;
     double beta1 = 0.0, beta3 = 0.0, beta4 = 0.0, dn2[6] = {0, 0, 0, 0, 0, 0};
     if (yield) {
-      const double beta0 = 1.0 + m.hp / (3.0 * mubar);
-      beta1 = 2.0 * mubar * dg / nrm;
-      const double beta2 = (1.0 - 1.0 / beta0) * (2.0 / 3.0) * (nrm / mubar) * dg;
-      beta3 = 1.0 / beta0 - beta1 + beta2;
-      beta4 = (1.0 / beta0 - beta1) * nrm / mubar;
+      // 1 / beta0 with beta0 = 1 + hp / (3 mubar)
+      const double rb0 = (3.0 * mubar) / (3.0 * mubar + m.hp);
+      const double nm = nrm / mubar;
+      beta1 = 2.0 * dg / nm;
+      const double beta2 = (1.0 - rb0) * (2.0 / 3.0) * nm * dg;
+      beta3 = rb0 - beta1 + beta2;
+      beta4 = (rb0 - beta1) * nm;
       // n^2 (symmetric), then its deviator
       double n2[6];
       n2[0] = n[0] * n[0] + n[3] * n[3] + n[5] * n[5];
@@ -379,8 +382,7 @@ __global__ __launch_bounds__(256, STG ? 4 : 1) void elem_kernel(ElemArgs a, int6
     }
     double Cnp1[6], ep1, gm, tau[6];
     J2Tan T;
-    j2_point<TAN>(m, is_j2, Fs, Fi, ok ? J : 1.0, Cn, ep_n, (INTEG && is_j2) ? Cnp1 : nullptr, &ep1,
-                  &gm, tau, T);
+    j2_point<TAN>(m, is_j2, Fs, Fi, ok ? J : 1.0, Cn, ep_n, INTEG && is_j2, Cnp1, ep1, gm, tau, T);
     // tau and c go to LDS weighted by the QP's detJ w (zero for a failed QP)
     const double wq = ok ? J0 * wt : 0.0;
 #pragma unroll

@@ -17,9 +17,9 @@ __device__ __forceinline__ void vol_pot(int flag, double J, double &dU, double &
     dU = J - 1;
     d2U = 1.0;
   } else {
-    const double ex = exp(J - 1.0);
-    dU = (ex - 1. / J) * .5;
-    d2U = (ex + 1. / (J * J)) * .5;
+    const double ex = exp(J - 1.0), rJ = 1.0 / J;
+    dU = (ex - rJ) * .5;
+    d2U = (ex + rJ * rJ) * .5;
   }
 }
 
