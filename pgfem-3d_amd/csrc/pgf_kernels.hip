@@ -661,17 +661,19 @@ __global__ __launch_bounds__(256) void gather_kernel(GatherArgs g) {
           v[j][b] = ((fm >> b) & 1) ? __builtin_nontemporal_load(blk + b * (nde + 1)) : 0.0;
       }
     }
+    // the three rows of one block are distinct targets: read the three sums, then write them
 #pragma unroll
     for (int j = 0; j < kGatherBatch; ++j)
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
         if (half == h && dst[j] != -1) {
-#pragma unroll
-          for (int b = 0; b < 3; ++b) {
-            if (!((fm >> b) & 1)) continue;
-            if (dst[j] >= 0) acc[rowoff[b] + dst[j]] += v[j][b];
-            else facc[b] += v[j][b];
-          }
+          double *t0 = (dst[j] >= 0) ? acc + rowoff[0] + dst[j] : facc + 0;
+          double *t1 = (dst[j] >= 0) ? acc + rowoff[1] + dst[j] : facc + 1;
+          double *t2 = (dst[j] >= 0) ? acc + rowoff[2] + dst[j] : facc + 2;
+          const double o0 = (fm & 1) ? *t0 : 0.0, o1 = (fm & 2) ? *t1 : 0.0, o2 = (fm & 4) ? *t2 : 0.0;
+          if (fm & 1) *t0 = o0 + v[j][0];
+          if (fm & 2) *t1 = o1 + v[j][1];
+          if (fm & 4) *t2 = o2 + v[j][2];
         }
         __builtin_amdgcn_wave_barrier();
       }
