@@ -52,10 +52,11 @@ def algorithmic_bytes(n_qp, nnz, ndof, n_nodes, n_elem, nne):
 def flops_per_qp(nne):
     """FP64 operations per QP of the fused J2 kernel, counted from the source (FMA = 2):
     phase 1 ~ (9*nne*2 kinematics + 9*nne h_a + ~700 constitutive/tangent), phase 2 per
-    element row: 36 MACs (b^T c) + 9 (tau h_a) + 12*nne (R h_w and geometric term)."""
+    element row: 18 MACs (the three strain entries of the row against c) + 9 (tau h_a) + 3
+    (residual) + 9 (geometric term folded into the row block) + 9*nne (the block times h_w)."""
     ndofe = 3 * nne
     phase1 = 2 * (9 * nne * 2 + 9 * nne) + 700
-    phase2 = ndofe * 2 * (36 + 9 + 3 + 12 * nne)
+    phase2 = ndofe * 2 * (18 + 9 + 3 + 9 + 9 * nne)
     return phase1 + phase2
 
 

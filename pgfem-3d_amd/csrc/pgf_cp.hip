@@ -45,6 +45,7 @@ namespace {
 using namespace dev;
 
 constexpr int kWave = 64;
+typedef double dbl2 __attribute__((ext_vector_type(2)));
 
 // Voigt helpers: index I -> (i,j) in xx,yy,zz,xy,yz,xz
 __device__ __forceinline__ int vi(int I) { return I < 3 ? I : (I == 3 ? 0 : (I == 4 ? 1 : 0)); }
@@ -1151,14 +1152,20 @@ __global__ __launch_bounds__(256) void cp_elem(CPElemArgs a, int64_t e_begin, in
         row[NDE] = fe;
       }
       lds_barrier();
-      for (int idx = threadIdx.x; idx < RE * NNE * PB; idx += blockDim.x) {
-        const int el = idx / (NNE * PB), o = idx - el * (NNE * PB);
-        const int na = o / PB, ob = o - na * PB;
+      // two doubles per thread (blocks are whole 128-byte lines, so pairs are 16-byte aligned)
+      constexpr int PB2 = PB / 2;
+      for (int idx = threadIdx.x; idx < RE * NNE * PB2; idx += blockDim.x) {
+        const int el = idx / (NNE * PB2), o = idx - el * (NNE * PB2);
+        const int na = o / PB2, ob = 2 * (o - na * PB2);
         const int64_t ee = e_begin + int64_t(blockIdx.x) * EPB + r0 + el;
         if (ee >= e_end) continue;
         const int64_t so = a.soff[ee * NNE + na];
         if (so < 0) continue;
-        __builtin_nontemporal_store((ob < 3 * LD) ? s_st[(el * NDE + na * 3) * LD + ob] : 0.0, a.stage + so + ob);
+        const double *src = s_st + (el * NDE + na * 3) * LD + ob;
+        dbl2 v;
+        v.x = (ob < 3 * LD) ? src[0] : 0.0;
+        v.y = (ob + 1 < 3 * LD) ? src[1] : 0.0;
+        __builtin_nontemporal_store(v, reinterpret_cast<dbl2 *>(a.stage + so + ob));
       }
       lds_barrier();
     }
