@@ -251,8 +251,9 @@ int pgf_autotune(pgf_handle h, const pgf_nodal_input *in, double dt, int64_t *ch
  *     writes the node's CSR rows and residual entries once, contiguously.
  *   PGF_ASSEMBLY_COLOURED: elements of one colour share no node and add their rows into
  *     the CSR values directly, colour by colour (no staging memory).
- * The gather mode needs 3 * max row length * 8 bytes of LDS per node; a mesh whose rows are
- * too long for it gets PGF_ENOTSUP here and is created in the coloured mode. */
+ * The gather mode needs 3 * max row length * 8 bytes of LDS per node, and packs each staging
+ * block's offsets into 64 bits (staging offset < 2^36 doubles, element-node pairs < 2^28); a
+ * mesh outside either limit gets PGF_ENOTSUP here and is created in the coloured mode. */
 #define PGF_ASSEMBLY_GATHER 0
 #define PGF_ASSEMBLY_COLOURED 1
 int pgf_set_assembly(pgf_handle h, int32_t mode);

@@ -205,8 +205,9 @@ struct pgf_handle_s {
   int32_t assembly = PGF_ASSEMBLY_COLOURED;
   int32_t maxlen = 0;
   int64_t stage_size = 0;  // doubles
-  int64_t *d_inc_ptr = nullptr, *d_inc_soff = nullptr, *d_inc_poff = nullptr;
-  int32_t *d_inc_nne = nullptr;
+  int64_t *d_inc_ptr = nullptr;
+  uint64_t *d_inc_rec = nullptr;  // packed staging offset, pos offset and node count per block
+  bool gather_fits = true;        // the packed fields hold the mesh's offsets
   uint32_t *d_pos = nullptr;
   double *d_stage = nullptr;
   bool profiling = false;
@@ -506,9 +507,7 @@ int32_t enqueue_pass(pgf_handle h, const pgf_nodal_input *in, int mode, double *
   if (gather) {
     pgf::GatherArgs g{};
     g.inc_ptr = h->d_inc_ptr;
-    g.inc_soff = h->d_inc_soff;
-    g.inc_poff = h->d_inc_poff;
-    g.inc_nne = h->d_inc_nne;
+    g.inc_rec = h->d_inc_rec;
     g.pos = h->d_pos;
     g.dofid = h->d_dofid;
     g.row_start = h->d_row_start;
@@ -863,13 +862,17 @@ int pgf_create(const pgf_mesh *mesh, const int64_t *id, int64_t ndof, const int3
   std::vector<uint32_t> gpos;
   build_gather(m.n_nodes, dofid.data(), groups, inc_ptr, inc_soff, inc_poff, inc_nne, gpos, h->stage_size);
   for (int64_t n = 0; n < m.n_nodes; ++n) h->maxlen = std::max(h->maxlen, rowlen[n]);
-  h->assembly = pgf::gather_waves(h->maxlen) >= 1 ? PGF_ASSEMBLY_GATHER : PGF_ASSEMBLY_COLOURED;
+  std::vector<uint64_t> inc_rec(inc_nne.size());
+  for (size_t sl = 0; sl < inc_rec.size(); ++sl) {
+    const uint64_t s16 = uint64_t(inc_soff[sl]) / 16, po = uint64_t(inc_poff[sl]);
+    if (inc_soff[sl] % 16 != 0 || s16 >> 32 || po >> 28 || inc_nne[sl] > 15) h->gather_fits = false;
+    inc_rec[sl] = (s16 & 0xffffffffull) | ((po & 0x0fffffffull) << 32) | (uint64_t(inc_nne[sl]) << 60);
+  }
+  h->assembly = (pgf::gather_waves(h->maxlen) >= 1 && h->gather_fits) ? PGF_ASSEMBLY_GATHER : PGF_ASSEMBLY_COLOURED;
   // --- upload
   int rc = PGF_OK;
   if ((rc = pgf::upload(&h->d_inc_ptr, inc_ptr.data(), inc_ptr.size()))) return fail(rc);
-  if ((rc = pgf::upload(&h->d_inc_soff, inc_soff.data(), inc_soff.size()))) return fail(rc);
-  if ((rc = pgf::upload(&h->d_inc_poff, inc_poff.data(), inc_poff.size()))) return fail(rc);
-  if ((rc = pgf::upload(&h->d_inc_nne, inc_nne.data(), inc_nne.size()))) return fail(rc);
+  if ((rc = pgf::upload(&h->d_inc_rec, inc_rec.data(), inc_rec.size()))) return fail(rc);
   if ((rc = pgf::upload(&h->d_pos, gpos.data(), gpos.size()))) return fail(rc);
   for (size_t bi = 0; bi < h->cblocks.size(); ++bi) {
     pgf::CohBlock &cb = h->cblocks[bi];
@@ -988,9 +991,7 @@ int pgf_destroy(pgf_handle h) {
     hipFree(cb.d_clist);
   }
   hipFree(h->d_inc_ptr);
-  hipFree(h->d_inc_soff);
-  hipFree(h->d_inc_poff);
-  hipFree(h->d_inc_nne);
+  hipFree(h->d_inc_rec);
   hipFree(h->d_pos);
   hipFree(h->d_stage);
   for (int t = 0; t < 3; ++t) hipFree(h->d_cv[t]);
@@ -1352,7 +1353,7 @@ int pgf_set_graph(pgf_handle h, int on) {
 
 int pgf_set_assembly(pgf_handle h, int32_t mode) {
   if (!h || (mode != PGF_ASSEMBLY_GATHER && mode != PGF_ASSEMBLY_COLOURED)) return PGF_EINVAL;
-  if (mode == PGF_ASSEMBLY_GATHER && pgf::gather_waves(h->maxlen) < 1) return PGF_ENOTSUP;
+  if (mode == PGF_ASSEMBLY_GATHER && (pgf::gather_waves(h->maxlen) < 1 || !h->gather_fits)) return PGF_ENOTSUP;
   if (mode == h->assembly) return PGF_OK;
   if (hipStreamSynchronize(h->stream) != hipSuccess) return PGF_EHIP;
   if (h->gexec) {  // a captured pass holds the old schedule

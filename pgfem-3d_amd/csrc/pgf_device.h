@@ -182,9 +182,12 @@ void launch_elem(int nne, int mode, const ElemArgs &a, int64_t e_begin, int64_t 
 // of inc_nne[k] nodes, and pos[inc_poff[k] + w] packs, for the element's node w, the column
 // of its first free DOF inside node n's rows (bits 0-27) and its free-DOF mask (bits 28-30;
 // 0 = no column).
+// inc_rec: per staging block (node-major), bits 0-31 its staging offset in units of 16 doubles,
+// bits 32-59 the offset of its element's column positions in pos, bits 60-63 the element's
+// node count.
 struct GatherArgs {
-  const int64_t *inc_ptr, *inc_soff, *inc_poff;
-  const int32_t *inc_nne;
+  const int64_t *inc_ptr;
+  const uint64_t *inc_rec;
   const uint32_t *pos;
   const int32_t *dofid, *row_start, *rowlen;
   const double *stage;

@@ -645,11 +645,12 @@ __global__ __launch_bounds__(256) void gather_kernel(GatherArgs g) {
       dst[j] = -1;
       const int64_t k = k0 + 2 * j + half;
       if (k >= ke) continue;
-      const int nde = 3 * g.inc_nne[k];
-      const double *blk = g.stage + g.inc_soff[k] + c;
+      const uint64_t rec = g.inc_rec[k];
+      const int nde = 3 * int(rec >> 60);
+      const double *blk = g.stage + int64_t(rec & 0xffffffffu) * 16 + c;
       if (TAN && c < nde) {
         const int w = c / 3, gg = c - 3 * w;
-        const uint32_t p = g.pos[g.inc_poff[k] + w];
+        const uint32_t p = g.pos[int64_t((rec >> 32) & 0x0fffffffu) + w];
         const int mw = int(p >> 28);
         if ((mw >> gg) & 1) dst[j] = int(p & 0x0fffffffu) + __popc(mw & ((1 << gg) - 1));
       } else if (RES && c == nde) {
