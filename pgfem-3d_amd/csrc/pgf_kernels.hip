@@ -250,6 +250,30 @@ __global__ __launch_bounds__(256, STG ? 4 : 1) void elem_kernel(ElemArgs a, int6
   const bool active = ie < e_end;
   const int64_t e = (a.elist && active) ? int64_t(a.elist[ie]) : ie;
 
+  // The QP phase (phase 1 below) runs on one wave: the workgroup's QPs, LPQ lanes per QP,
+  // thread p = slot * NQP * LPQ + ip * LPQ + sub of the wave (blk & 3).  Its lanes load their
+  // QP's material and state now, so those loads are in flight during phase 0.
+  constexpr int P1T = 64;
+  constexpr int LPQ = P1T / (EPB * NQP);
+  static_assert(LPQ * NQP * EPB == P1T && (LPQ & (LPQ - 1)) == 0 && LPQ <= 64, "lanes per QP");
+  const int p1 = int(threadIdx.x) - int(blk & 3) * 64;
+  const bool p1live = p1 >= 0 && p1 < P1T && e_begin + blk * EPB + p1 / (NQP * LPQ) < e_end;
+  const int p1slot = p1live ? p1 / (NQP * LPQ) : 0, p1ip = p1live ? (p1 / LPQ) % NQP : 0;
+  int64_t p1e = 0;
+  DevMaterial pm{};
+  double pCn[6] = {1, 1, 1, 0, 0, 0}, pep = 0.0;
+  if (p1live) {
+    const int64_t ie1 = e_begin + blk * EPB + p1slot;
+    p1e = a.elist ? int64_t(a.elist[ie1]) : ie1;
+    pm = a.mats[a.mat[p1e]];
+    if (pm.model == PGF_MODEL_J2_PLASTICITY) {
+      const int64_t q = a.qp_base + p1e * NQP + p1ip;
+#pragma unroll
+      for (int I = 0; I < 6; ++I) pCn[I] = a.P_n[I * a.n_qp + q];
+      pep = a.Sc_n[q];
+    }
+  }
+
   // ---- phase 0: gather
   if (active && lane < NNE) {
     const int32_t node = a.conn[e * NNE + lane];
@@ -275,22 +299,16 @@ __global__ __launch_bounds__(256, STG ? 4 : 1) void elem_kernel(ElemArgs a, int6
   }
   lds_barrier();
 
-  // ---- phase 1: quadrature points on two of the four waves (the other two skip it, so a
-  // workgroup issues the QP stream twice, not four times), LPQ lanes per QP: thread
-  // p = slot * NQP * LPQ + ip * LPQ + sub.  Blocks alternate which pair of waves works, so
-  // the QP work spreads over the SIMDs.  Node loops are split over the QP's lanes (node n on
-  // sub-lane n % LPQ, which alone reads and writes its entries of s_h) and summed with
-  // butterfly shuffles, which leave every lane with the same bits; the return map runs on
+  // ---- phase 1: quadrature points on one of the four waves (the others skip it, so a
+  // workgroup issues the QP stream once, not four times); which wave works rotates with the
+  // block, so the QP work spreads over the SIMDs.  Node loops are split over the QP's lanes
+  // (node n on sub-lane n % LPQ, which alone reads and writes its entries of s_h) and summed
+  // with butterfly shuffles, which leave every lane with the same bits; the return map runs on
   // every lane; row I of c goes to sub-lanes I, I+LPQ, ...
-  constexpr int P1T = 64;
-  constexpr int LPQ = P1T / (EPB * NQP);
-  static_assert(LPQ * NQP * EPB == P1T && (LPQ & (LPQ - 1)) == 0 && LPQ <= 64, "lanes per QP");
-  const int p1 = int(threadIdx.x) - int(blk & 3) * 64;
-  if (p1 >= 0 && p1 < P1T && e_begin + blk * EPB + p1 / (NQP * LPQ) < e_end) {
-    const int slot = p1 / (NQP * LPQ);  // the QP's element (shadows the row lanes' slot)
-    const int64_t ie1 = e_begin + blk * EPB + slot;
-    const int64_t e = a.elist ? int64_t(a.elist[ie1]) : ie1;
-    const int ip = (p1 / LPQ) % NQP, sub = p1 % LPQ;
+  if (p1live) {
+    const int slot = p1slot;  // the QP's element (shadows the row lanes' slot)
+    const int64_t e = p1e;
+    const int ip = p1ip, sub = p1 % LPQ;
     double k, ee, z, wt;
     int_point(NNE, ip, k, ee, z, wt);
     double *hq = s_h[slot][ip];  // dN/dX first, h_a = Fs^-T dN_a/dX at the end
@@ -372,14 +390,12 @@ __global__ __launch_bounds__(256, STG ? 4 : 1) void elem_kernel(ElemArgs a, int6
 #pragma unroll
       for (int i = 0; i < 9; ++i) Fi[i] = (i % 4 == 0) ? 1.0 : 0.0;
     }
-    const DevMaterial m = a.mats[a.mat[e]];
+    const DevMaterial m = pm;
     const bool is_j2 = (m.model == PGF_MODEL_J2_PLASTICITY);
-    double Cn[6] = {1, 1, 1, 0, 0, 0}, ep_n = 0.0;
-    if (is_j2) {
+    const double ep_n = pep;
+    double Cn[6];
 #pragma unroll
-      for (int I = 0; I < 6; ++I) Cn[I] = a.P_n[I * NQ + q];
-      ep_n = a.Sc_n[q];
-    }
+    for (int I = 0; I < 6; ++I) Cn[I] = pCn[I];
     double Cnp1[6], ep1, gm, tau[6];
     J2Tan T;
     j2_point<TAN>(m, is_j2, Fs, Fi, ok ? J : 1.0, Cn, ep_n, INTEG && is_j2, Cnp1, ep1, gm, tau, T);
