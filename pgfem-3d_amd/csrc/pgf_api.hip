@@ -30,6 +30,7 @@ struct Block {
   std::vector<int64_t> color_ptr;  // [n_colors+1] colour segments of clist
   int32_t *d_clist = nullptr;
   int32_t *d_conn = nullptr;
+  int32_t *d_edof = nullptr;  // [ne*3nne] DOF ids of the element DOFs
   int16_t *d_off = nullptr;
   int64_t *d_soff = nullptr;       // gather mode: [ne*nne] staging offsets
   int32_t *d_mat = nullptr;
@@ -462,6 +463,7 @@ int32_t enqueue_pass(pgf_handle h, const pgf_nodal_input *in, int mode, double *
         pgf::launch_cp_elem(b.nne, mode, ce, e0, e1, h->stream);
       } else {
         a.conn = b.d_conn;
+        a.edof = b.d_edof;
         a.off = b.d_off;
         a.soff = b.d_soff;
         a.elist = gather ? nullptr : b.d_clist;
@@ -911,6 +913,12 @@ int pgf_create(const pgf_mesh *mesh, const int64_t *id, int64_t ndof, const int3
     std::vector<int32_t> mat(b.ne);
     for (int64_t k = 0; k < b.ne; ++k) mat[k] = m.elem_mat ? m.elem_mat[b.elems[k]] : 0;
     if ((rc = pgf::upload(&b.d_conn, bconn[bi].data(), bconn[bi].size()))) return fail(rc);
+    if (b.model != PGF_MODEL_CRYSTAL_PLASTICITY) {  // the element kernel's DOF ids, element-major
+      std::vector<int32_t> edof(bconn[bi].size() * 3);
+      for (size_t i = 0; i < bconn[bi].size(); ++i)
+        for (int c = 0; c < 3; ++c) edof[i * 3 + c] = dofid[size_t(bconn[bi][i]) * 3 + c];
+      if ((rc = pgf::upload(&b.d_edof, edof.data(), edof.size()))) return fail(rc);
+    }
     if ((rc = pgf::upload(&b.d_mat, mat.data(), mat.size()))) return fail(rc);
     if ((rc = pgf::upload(&b.d_off, boff[bi].data(), boff[bi].size()))) return fail(rc);
     if ((rc = pgf::upload(&b.d_soff, groups[bi].soff.data(), groups[bi].soff.size()))) return fail(rc);
@@ -977,6 +985,7 @@ int pgf_destroy(pgf_handle h) {
   hipFree(h->d_mats);
   for (auto &b : h->blocks) {
     hipFree(b.d_conn);
+    hipFree(b.d_edof);
     hipFree(b.d_off);
     hipFree(b.d_soff);
     hipFree(b.d_clist);

@@ -41,6 +41,7 @@ struct ElemArgs {
   const int32_t *row_start; // [n_nodes] Ap of the node's first free DOF, -1 if none
   const int32_t *rowlen;    // [n_nodes]
   const int32_t *conn;      // [ne*nne] block connectivity, device element order
+  const int32_t *edof;      // [ne*3nne] the element DOFs' ids (dofid of each node component)
   const int16_t *off;       // [ne*nne*nne] scatter offsets (>=0 add, <=-2 first touch, -1 none)
   const int32_t *elist;     // coloured mode: block elements in colour order (launch ranges index it)
   const int32_t *mat;       // [ne] material index

@@ -274,28 +274,27 @@ __global__ __launch_bounds__(256, STG ? 4 : 1) void elem_kernel(ElemArgs a, int6
     }
   }
 
-  // ---- phase 0: gather
-  if (active && lane < NNE) {
-    const int32_t node = a.conn[e * NNE + lane];
-    s_node[slot][lane] = node;
-#pragma unroll
-    for (int b = 0; b < 3; ++b) {
-      s_x[slot][lane * 3 + b] = a.X[int64_t(node) * 3 + b];
-      const int32_t d = a.dofid[int64_t(node) * 3 + b];
-      s_dof[slot][lane * 3 + b] = d;
-      double u = 0.0;  // def_elem_total, src/utils.cc:1391-1412
-      double dl = 0.0;
-      if (d > 0) {
-        u = a.r[d - 1] + (a.d_r ? a.d_r[d - 1] : 0.0);
-      } else if (d < 0) {
-        const int k = -d - 1;
-        dl = a.defl_d ? a.defl_d[k] : 0.0;
-        // the prescribed-load pass takes K at defl (defl_d zeroed, src/stiffmat_fd.cc:722-745)
-        u = (a.defl ? a.defl[k] : 0.0) + (DFL ? 0.0 : dl);
-      }
-      s_u[slot][lane * 3 + b] = u;
-      if constexpr (DFL) s_dl[slot][lane * 3 + b] = dl;
+  // ---- phase 0: gather, one lane per element DOF; the DOF ids come from the element's own
+  // table (not through the node), so the displacements are two loads deep, not three
+  if (active && lane < NDE) {
+    const int nl = lane / 3, b = lane - 3 * nl;
+    const int32_t node = a.conn[e * NNE + nl];
+    const int32_t d = a.edof[e * NDE + lane];
+    if (b == 0) s_node[slot][nl] = node;
+    s_x[slot][lane] = a.X[int64_t(node) * 3 + b];
+    s_dof[slot][lane] = d;
+    double u = 0.0;  // def_elem_total, src/utils.cc:1391-1412
+    double dl = 0.0;
+    if (d > 0) {
+      u = a.r[d - 1] + (a.d_r ? a.d_r[d - 1] : 0.0);
+    } else if (d < 0) {
+      const int k = -d - 1;
+      dl = a.defl_d ? a.defl_d[k] : 0.0;
+      // the prescribed-load pass takes K at defl (defl_d zeroed, src/stiffmat_fd.cc:722-745)
+      u = (a.defl ? a.defl[k] : 0.0) + (DFL ? 0.0 : dl);
     }
+    s_u[slot][lane] = u;
+    if constexpr (DFL) s_dl[slot][lane] = dl;
   }
   lds_barrier();
 
