@@ -71,8 +71,10 @@ def cp_flops_per_qp(mean_iters, nne=4):
       => Newton iteration ~9.8e3 (the evaluation at the top of the iteration included);
       start ~3.5e3 (up to 10 tau-ratio evaluations of ~350);
       tangent pass ~1.2e4 (evaluation, Jacobian, LU, nine bordered solves, d tau/dF);
-      element rows: 3*nne rows x ~800 (B, Y, Z, z = Z:D, W, Fr W, the row of K)."""
-    return 9.8e3 * mean_iters + 3.5e3 + 1.2e4 + 3 * nne * 800
+      element rows: nine row-basis vectors per QP x ~650 (B, Y = L:B, Z, z = Z:D, W, Fr W),
+        then 3*nne rows x 2 (50 + 9 nne) (S g_a, residual, the three-basis combination, the
+        geometric term and the row of K)."""
+    return 9.8e3 * mean_iters + 3.5e3 + 1.2e4 + 9 * 650 + 3 * nne * 2 * (50 + 9 * nne)
 
 
 def measured_traffic(key):

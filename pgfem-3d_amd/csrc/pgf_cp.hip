@@ -1026,6 +1026,92 @@ __global__ __launch_bounds__(256) void cp_elem(CPElemArgs a, int64_t e_begin, in
     }
   }
   __syncthreads();
+  if constexpr (TAN) {
+    // The row-dependent part of a K row is linear in sAA = sym(Fr^T (e_b (x) g_a)), so in the
+    // row's g_a: per QP, the nine vectors z_bj of z = detJ w (Z : dMe/dF + Fr W) for
+    // sAA = sym(Fr^T (e_b (x) e_j)) (b the row's component, j a component of g_a) replace the
+    // per-row evaluation of B, L:B, Z, Z:dMe/dF and W; a row combines three of them.  They
+    // are computed from dMe/dF and then overwrite it in LDS.
+    constexpr int NT = EPB * TPE, NIT = EPB * NQP * 9, NPASS = (NIT + NT - 1) / NT;
+    double zr[NPASS][9];
+#pragma unroll
+    for (int p = 0; p < NPASS; ++p) {
+      const int t = int(threadIdx.x) + p * NT;
+      const int sl = t / (NQP * 9), ip = (t / 9) % NQP, bj = t % 9, b = bj / 3, j = bj - 3 * b;
+      const int64_t iet = e_begin + int64_t(blockIdx.x) * EPB + sl;
+#pragma unroll
+      for (int c = 0; c < 9; ++c) zr[p][c] = 0.0;
+      if (t < NIT && iet < e_end) {
+        const double *Fr = s_Fr[sl][ip], *Me = s_Me[sl][ip];
+        double sAA[9];  // sym(Fr^T (e_b (x) e_j))_ik = (Fr_bi d_jk + Fr_bk d_ij) / 2
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int k = 0; k < 3; ++k)
+            sAA[i * 3 + k] = 0.5 * ((k == j ? Fr[b * 3 + i] : 0.0) + (i == j ? Fr[b * 3 + k] : 0.0));
+        double T[9], Bm[9];
+        mm3(sAA, Me, T);
+        mtm3(Me, T, Bm);  // Me^T sAA Me
+        double Y6[6];
+        if constexpr (HE) {
+          const double *hv = s_he[sl][ip];
+          HeTan ht;
+#pragma unroll
+          for (int i = 0; i < 6; ++i) ht.Ci[i] = hv[i];
+          ht.c1 = hv[6];
+          ht.c2 = hv[7];
+          ht.c3 = hv[8];
+          he_dot(ht, Bm, Y6);
+        } else {
+          const int64_t et = a.elist ? int64_t(a.elist[iet]) : iet;
+          lv_dot(a.grains[a.grain[et]].Lv, Bm, Y6);
+        }
+        double Y9[9];
+        v6_to_9(Y6, Y9);
+        const double *S6 = s_S[sl][ip];
+        const double BS = Bm[0] * S6[0] + Bm[4] * S6[1] + Bm[8] * S6[2] +
+                          2.0 * (0.5 * (Bm[1] + Bm[3]) * S6[3] + 0.5 * (Bm[5] + Bm[7]) * S6[4] +
+                                 0.5 * (Bm[2] + Bm[6]) * S6[5]);
+        // Z = CrMe Y + 2 sAA MeS - (B:S) Me^-T
+        double Z[9], T2[9];
+        mm3(s_CrMe[sl][ip], Y9, Z);
+        mm3(sAA, s_MeS[sl][ip], T2);
+        const double *MiT = s_MiT[sl][ip];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) Z[i] += 2.0 * T2[i] - BS * MiT[i];
+        // W = Me Y Me^T, FrW = Fr W
+        double T3[9], W[9], FrW[9];
+        mm3(Me, Y9, T3);
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int k = 0; k < 3; ++k)
+            W[i * 3 + k] = T3[i * 3 + 0] * Me[k * 3 + 0] + T3[i * 3 + 1] * Me[k * 3 + 1] + T3[i * 3 + 2] * Me[k * 3 + 2];
+        mm3(Fr, W, FrW);
+        // z[g L] = sum_ij Z_ij D[ij][g L] + (Fr W)[g L]
+        const double *D = s_D[sl][ip];
+        const double wq = s_w[sl][ip];
+#pragma unroll
+        for (int c = 0; c < 9; ++c) {
+          double sum = 0.0;
+#pragma unroll
+          for (int r = 0; r < 9; ++r) sum += Z[r] * D[r * 9 + c];
+          zr[p][c] = wq * (sum + FrW[c]);
+        }
+      }
+    }
+    lds_barrier();  // every z_bj is computed from dMe/dF before any overwrites it
+#pragma unroll
+    for (int p = 0; p < NPASS; ++p) {
+      const int t = int(threadIdx.x) + p * NT;
+      if (t < NIT) {
+        const int sl = t / (NQP * 9), ip = (t / 9) % NQP, bj = t % 9;
+#pragma unroll
+        for (int c = 0; c < 9; ++c) s_D[sl][ip][bj * 9 + c] = zr[p][c];
+      }
+    }
+    lds_barrier();
+  }
   const bool rowlane = active && lane < NDE;
   const int ra = lane / 3, rb = lane % 3;
   const double eb[3] = {rb == 0 ? 1.0 : 0.0, rb == 1 ? 1.0 : 0.0, rb == 2 ? 1.0 : 0.0};
@@ -1036,8 +1122,7 @@ __global__ __launch_bounds__(256) void cp_elem(CPElemArgs a, int64_t e_begin, in
   }
   double fe = 0.0;
   if (rowlane) {
-  const DevGrain &gr = a.grains[a.grain[e]];
-#pragma unroll 1  // unrolled QP bodies interleave and exhaust the registers (occupancy 1)
+#pragma unroll 1
   for (int ip = 0; ip < NQP; ++ip) {
     const double wq = s_w[slot][ip];
     const double *dN = s_dN[slot][ip];
@@ -1050,72 +1135,23 @@ __global__ __launch_bounds__(256) void cp_elem(CPElemArgs a, int64_t e_begin, in
     for (int i = 0; i < 3; ++i) ta[i] = Sh9[i * 3] * ga[0] + Sh9[i * 3 + 1] * ga[1] + Sh9[i * 3 + 2] * ga[2];
     if constexpr (RES) fe += wq * (Fr[rb * 3] * ta[0] + Fr[rb * 3 + 1] * ta[1] + Fr[rb * 3 + 2] * ta[2]);
     if constexpr (TAN) {
-      const double *Me = s_Me[slot][ip];
-      // sAA = sym(Fr^T (e_b (x) dN_a)): sAA_ij = (Fr_bi g_j + Fr_bj g_i)/2
-      double sAA[9];
-#pragma unroll
-      for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j) sAA[i * 3 + j] = 0.5 * (Fr[rb * 3 + i] * ga[j] + Fr[rb * 3 + j] * ga[i]);
-      double T[9], Bm[9];
-      mm3(sAA, Me, T);
-      mtm3(Me, T, Bm);  // Me^T sAA Me
-      double Y6[6];
-      if constexpr (HE) {
-        const double *hv = s_he[slot][ip];
-        HeTan ht;
-#pragma unroll
-        for (int i = 0; i < 6; ++i) ht.Ci[i] = hv[i];
-        ht.c1 = hv[6];
-        ht.c2 = hv[7];
-        ht.c3 = hv[8];
-        he_dot(ht, Bm, Y6);
-      } else {
-        lv_dot(gr.Lv, Bm, Y6);
-      }
-      double Y9[9];
-      v6_to_9(Y6, Y9);
-      double BS = 0.0;
-      {
-        const double *S6 = s_S[slot][ip];
-        BS = Bm[0] * S6[0] + Bm[4] * S6[1] + Bm[8] * S6[2] +
-             2.0 * (0.5 * (Bm[1] + Bm[3]) * S6[3] + 0.5 * (Bm[5] + Bm[7]) * S6[4] + 0.5 * (Bm[2] + Bm[6]) * S6[5]);
-      }
-      // Z = CrMe Y + 2 sAA MeS - (B:S) Me^-T
-      double Z[9], T2[9];
-      mm3(s_CrMe[slot][ip], Y9, Z);
-      mm3(sAA, s_MeS[slot][ip], T2);
-      const double *MiT = s_MiT[slot][ip];
-#pragma unroll
-      for (int i = 0; i < 9; ++i) Z[i] += 2.0 * T2[i] - BS * MiT[i];
-      // z[g][L] = sum_ij Z_ij D[ij][g L]
-      const double *D = s_D[slot][ip];
+      // z = sum_j g_a,j z_bj, plus the geometric term of row b: detJ w (Sh g_a)
+      const double *zb = &s_D[slot][ip][rb * 27];
       double zz[9];
 #pragma unroll
-      for (int c = 0; c < 9; ++c) {
-        double s = 0.0;
+      for (int c = 0; c < 9; ++c) zz[c] = ga[0] * zb[c] + ga[1] * zb[9 + c] + ga[2] * zb[18 + c];
 #pragma unroll
-        for (int r = 0; r < 9; ++r) s += Z[r] * D[r * 9 + c];
-        zz[c] = s;
+      for (int k = 0; k < 3; ++k) {
+        const double wt = wq * ta[k];
+        zz[0 + k] += eb[0] * wt;
+        zz[3 + k] += eb[1] * wt;
+        zz[6 + k] += eb[2] * wt;
       }
-      // W = Me Y Me^T, FrW = Fr W
-      double T3[9], W[9], FrW[9];
-      mm3(Me, Y9, T3);
-#pragma unroll
-      for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j)
-          W[i * 3 + j] = T3[i * 3 + 0] * Me[j * 3 + 0] + T3[i * 3 + 1] * Me[j * 3 + 1] + T3[i * 3 + 2] * Me[j * 3 + 2];
-      mm3(Fr, W, FrW);
-#pragma unroll
-      for (int i = 0; i < 9; ++i) zz[i] = wq * (zz[i] + FrW[i]);
-      const double wt0 = wq * ta[0], wt1 = wq * ta[1], wt2 = wq * ta[2];
 #pragma unroll
       for (int w = 0; w < NNE; ++w) {
         const double h0 = dN[w * 3], h1 = dN[w * 3 + 1], h2 = dN[w * 3 + 2];
-        const double geo = wt0 * h0 + wt1 * h1 + wt2 * h2;
 #pragma unroll
-        for (int g = 0; g < 3; ++g) K[w * 3 + g] += zz[g * 3] * h0 + zz[g * 3 + 1] * h1 + zz[g * 3 + 2] * h2 + eb[g] * geo;
+        for (int g = 0; g < 3; ++g) K[w * 3 + g] += zz[g * 3] * h0 + zz[g * 3 + 1] * h1 + zz[g * 3 + 2] * h2;
       }
     }
   }
