@@ -132,6 +132,15 @@ __device__ __forceinline__ void he_dot(const HeTan &t, const double *X9, double 
   }
 }
 
+// The grain's 6x6 stiffness is loop-invariant in the tangent kernel's slip-system loop; hiding
+// its address there keeps the compiler from hoisting its 36 loads (72 registers live across
+// the loop, which spill) and has it read them from cache where they are used.  (In the local
+// Newton the hoisted copy serves many evaluations and pays for itself.)
+__device__ __forceinline__ const double *opaque(const double *p) {
+  asm volatile("" : "+v"(p));
+  return p;
+}
+
 // Elastic PK2 from eF and Ce = eF^T eF: law 0 is S = Lv : (Ce - I)/2 with the grain's cubic
 // stiffness; law 1 is he_law, which also fills the tangent coefficients.
 template <bool HE>
@@ -276,25 +285,38 @@ __device__ __forceinline__ bool cp_point(const CPMaterial &cm, const DevGrain &g
   return true;
 }
 
-// d tau_a / d M = 2 A^T eF sym(P_a S + 1/2 L:(C_e P_a))
+// Q_a = sym(P_a S + 1/2 L:(C_e P_a)) in Voigt order, so that d tau_a / d M = 2 A^T eF Q_a
 template <bool HE>
-__device__ __forceinline__ void dtau_dM(const DevGrain &gr, int a, const double *AtE, const CPPoint &p,
-                                        double *D) {
+__device__ __forceinline__ void qsym(const DevGrain &gr, const double *Lv, int a, const CPPoint &p, double *Q6) {
   const double *P = gr.P[a];
   double S9[9], PS[9], CP[9];
   v6_to_9(p.S, S9);
   mm3(P, S9, PS);
   mm3(p.Ce, P, CP);
   double LCP[6];
-  el_dot<HE>(gr, p.he, CP, LCP);
-  double sQ[9];
+  if constexpr (HE) he_dot(p.he, CP, LCP);
+  else lv_dot(Lv, CP, LCP);
 #pragma unroll
   for (int I = 0; I < 6; ++I) {
     const int i = vi(I), j = vj(I);
-    const double v = 0.5 * (PS[i * 3 + j] + PS[j * 3 + i]) + 0.5 * LCP[I];
-    sQ[i * 3 + j] = v;
-    sQ[j * 3 + i] = v;
+    Q6[I] = 0.5 * (PS[i * 3 + j] + PS[j * 3 + i]) + 0.5 * LCP[I];
   }
+}
+
+// u^T Q w for Q symmetric in Voigt order
+__device__ __forceinline__ double quad_form(const double *Q, const double *u, const double *w) {
+  return u[0] * (Q[0] * w[0] + Q[3] * w[1] + Q[5] * w[2]) + u[1] * (Q[3] * w[0] + Q[1] * w[1] + Q[4] * w[2]) +
+         u[2] * (Q[5] * w[0] + Q[4] * w[1] + Q[2] * w[2]);
+}
+
+// d tau_a / d M = 2 A^T eF sym(P_a S + 1/2 L:(C_e P_a)); Lv: the grain's stiffness (its address
+// made opaque by callers that must not hoist its loads)
+template <bool HE>
+__device__ __forceinline__ void dtau_dM(const DevGrain &gr, const double *Lv, int a, const double *AtE,
+                                        const CPPoint &p, double *D) {
+  double Q6[6], sQ[9];
+  qsym<HE>(gr, Lv, a, p, Q6);
+  v6_to_9(Q6, sQ);
   double T[9];
   mm3(AtE, sQ, T);
 #pragma unroll
@@ -307,8 +329,9 @@ struct Border {
 };
 
 // Bordered Jacobian of the local system at p; the per-system slip quantities are recomputed
-// from the Mandel stress with cp_point's arithmetic.
-template <bool HE>
+// from the Mandel stress with cp_point's arithmetic.  OPQ: read the grain stiffness per slip
+// system rather than hoisting it (the tangent kernel, where the hoisted copy spills).
+template <bool HE, bool OPQ>
 __device__ __forceinline__ void cp_jacobian_re(const CPMaterial &cm, const DevGrain &gr, double dt, const double *x,
                                             const double *A, const CPPoint &p, double g_n,
                                             double (&Ah)[81], Border &B) {
@@ -335,7 +358,7 @@ __device__ __forceinline__ void cp_jacobian_re(const CPMaterial &cm, const DevGr
     const double dgdt = cm.gd0 / (cm.m * g) * pw;
     const double dgdg = -dgdt * t / g;
     double D[9];
-    dtau_dM<HE>(gr, a, AtE, p, D);
+    dtau_dM<HE>(gr, OPQ ? opaque(gr.Lv) : gr.Lv, a, AtE, p, D);
     const double co = dt * dgdt;
     const double sgn = (gd < 0) ? -1.0 : 1.0;
 #pragma unroll
@@ -669,7 +692,7 @@ __global__ __launch_bounds__(64) void cp_newton(CPArgs a, int64_t chunk) {
     } else {
       double Ah[81];
       Border B;
-      cp_jacobian_re<HE>(cm, gr, dt, x, A, p, g_n, Ah, B);
+      cp_jacobian_re<HE, false>(cm, gr, dt, x, A, p, g_n, Ah, B);
       int piv[9];
       if (!lu9(Ah, piv)) {
         fail = true;
@@ -816,9 +839,43 @@ __global__ __launch_bounds__(64) void cp_tangent(CPArgs a, int64_t nqb) {
   for (int i = 0; i < 6; ++i) out[9 + i] = S[i];
   out[15] = det3(pF1);
   if constexpr (WITH_D) {
+    // dtau_a/dF = A^-T D_a (pF(n)^-1 M)^T with D_a = 2 A^T eF Q_a, so dtau_a/dF = 2 eF Q_a BM^T
+    // (BM = pF(n)^-1 M).  The right-hand sides are linear in Q_a: with R_r = sum_a co_a P_a[r] Q_a
+    // and Rg = -sum_a sg_a Q_a, column c = (i, j) of F has rhs[r] = 2 (eF R_r BM^T)_ij.
+    double BM[9];
+    mm3(pFi, x, BM);
+    double R[9][6], Rg[6];
+#pragma unroll
+    for (int I = 0; I < 6; ++I) {
+      Rg[I] = 0.0;
+#pragma unroll
+      for (int r = 0; r < 9; ++r) R[r][I] = 0.0;
+    }
+    for (int al = 0; al < 12; ++al) {
+      double Q[6];
+      qsym<HE>(gr, opaque(gr.Lv), al, p, Q);
+      double tt = 0.0;
+#pragma unroll
+      for (int i = 0; i < 9; ++i) tt += p.Sig[i] * gr.P[al][i];
+      const double rr = tt / x[10];
+      const double pw = pow(fabs(rr), expo);
+      const double dgdt = cm.gd0 / (cm.m * x[10]) * pw;
+      const double co = -a.dt * dgdt;
+      const double sg = ((cm.gd0 * rr * pw < 0) ? -1.0 : 1.0) * dgdt;
+#pragma unroll
+      for (int r = 0; r < 9; ++r) {
+        const double cr = co * gr.P[al][r];
+#pragma unroll
+        for (int I = 0; I < 6; ++I) R[r][I] += cr * Q[I];
+      }
+#pragma unroll
+      for (int I = 0; I < 6; ++I) Rg[I] -= sg * Q[I];
+    }
+    // the local Jacobian at the converged point, factored once for the ten solves (after the
+    // slip loop, so the factors and the accumulators are not live together)
     double Ah[81];
     Border B;
-    cp_jacobian_re<HE>(cm, gr, a.dt, x, A, p, g_n, Ah, B);
+    cp_jacobian_re<HE, true>(cm, gr, a.dt, x, A, p, g_n, Ah, B);
     int piv[9];
     if (!lu9(Ah, piv)) {
       atomicAdd(a.err, 1);
@@ -831,51 +888,15 @@ __global__ __launch_bounds__(64) void cp_tangent(CPArgs a, int64_t nqb) {
     double dTv = 0.0;
 #pragma unroll
     for (int i = 0; i < 9; ++i) dTv += B.d[i] * v[i];
-    // dtau_a/dF = A^-T Dtau_a (pF(n)^-1 M)^T
-    double Ai[9], AiT[9], BM[9], AtE[9];
-    inv3(A, Ai);
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-      for (int j = 0; j < 3; ++j) AiT[i * 3 + j] = Ai[j * 3 + i];
-    mm3(pFi, x, BM);
-    mtm3(A, p.eF, AtE);
-    double rhs[9][9];  // [F column][row of R_M]
-    double rg[9];
 #pragma unroll
     for (int c = 0; c < 9; ++c) {
-      rg[c] = 0.0;
+      const double *u = p.eF + (c / 3) * 3, *w = BM + (c % 3) * 3;
+      double rhs[9];
 #pragma unroll
-      for (int r = 0; r < 9; ++r) rhs[c][r] = 0.0;
-    }
-    for (int al = 0; al < 12; ++al) {
-      double D[9], T[9], dtdF[9];
-      dtau_dM<HE>(gr, al, AtE, p, D);
-      mm3(AiT, D, T);
-#pragma unroll
-      for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j)
-          dtdF[i * 3 + j] = T[i * 3 + 0] * BM[j * 3 + 0] + T[i * 3 + 1] * BM[j * 3 + 1] + T[i * 3 + 2] * BM[j * 3 + 2];
-      double tt = 0.0;
-#pragma unroll
-      for (int i = 0; i < 9; ++i) tt += p.Sig[i] * gr.P[al][i];
-      const double rr = tt / x[10];
-      const double pw = pow(fabs(rr), expo);
-      const double dgdt = cm.gd0 / (cm.m * x[10]) * pw;
-      const double co = -a.dt * dgdt;
-      const double sg = ((cm.gd0 * rr * pw < 0) ? -1.0 : 1.0) * dgdt;
-#pragma unroll
-      for (int c = 0; c < 9; ++c) {
-#pragma unroll
-        for (int r = 0; r < 9; ++r) rhs[c][r] += co * gr.P[al][r] * dtdF[c];
-        rg[c] -= sg * dtdF[c];
-      }
-    }
-#pragma unroll
-    for (int c = 0; c < 9; ++c) {
+      for (int r = 0; r < 9; ++r) rhs[r] = 2.0 * quad_form(R[r], u, w);
+      const double rgc = 2.0 * quad_form(Rg, u, w);
       double sol[11];
-      bordered_solve(Ah, piv, B, v, dTv, rhs[c], 0.0, B.dRg * rg[c], sol);
+      bordered_solve(Ah, piv, B, v, dTv, rhs, 0.0, B.dRg * rgc, sol);
       // dMe/dF[:, c] = pF(n)^-1 dM/dF[:, c]
 #pragma unroll
       for (int i = 0; i < 3; ++i)
