@@ -185,3 +185,24 @@ def test_cp_local_failure_is_reported(device):
     err = h.residual(f, _dev(r, device), defl=_dev(defl, device))
     assert err > 0
     h.close()
+
+
+def test_elastic_law_validation(device):
+    """pgf_create refuses what the crystal-plasticity kernels do not implement: an unknown
+    elastic law (PGF_EINVAL), the hyperelastic law with a Mooney-Rivlin m01 or another
+    deviatoric potential (PGF_ENOTSUP: neo-Hookean only), and nu = 0.5 (no bulk modulus)."""
+    prob = CPBoxProblem(1, P.PGF_TET4, MAT, n_grains=1, seed=1)
+    cases = [(dict(cp_elastic=2), "PGF_EINVAL"), (dict(m01=100.0), "PGF_ENOTSUP"),
+             (dict(devPotFlag=2), "PGF_ENOTSUP"), (dict(nu=0.5), "PGF_EINVAL")]
+    for change, code in cases:
+        m = fcc_crystal_he()
+        for k, v in change.items():
+            setattr(m, k, v)
+        with pytest.raises(P.PGFError, match=code):
+            P.Handle(prob.mesh, prob.ids, prob.ndof, prob.Ap, prob.Ai, [m])
+    h = _handle(CPBoxProblem(1, P.PGF_TET4, MAT_HE, n_grains=1, seed=1))
+    f = torch.zeros(h.ndof, dtype=torch.float64, device=device)
+    r = torch.zeros(h.ndof, dtype=torch.float64, device=device)
+    with pytest.raises(P.PGFError):
+        h.prescribed_load(f, r, None)
+    h.close()
