@@ -793,3 +793,285 @@ int orc_assemble(int64_t n_nodes, const double *X, int64_t n_elem, const int64_t
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------ three-field (CM3F)
+// The CM3F mixed formulation of the hyperelastic model: per element one volume variable theta
+// and one pressure P (get_3f_pressure_volume_number, src/three_field_element.cc:1399-1416:
+// npres = nVol = 1, constant shape functions Np = Nt = 1, femlib.cc:1189-1193), total
+// Lagrangian.  For this model pF = I (HE_PARAM::get_pF), so M = I, dM/du = dM/dtheta = 0
+// (Model_parameters::compute_dMdu/dMdt, include/constitutive_model.h:662-698), eFn = I and
+// Jn = tJn = theta_n = 1 (ConstitutiveModelIntregrate::integrate_ss,
+// src/constitutive_model.cc:794-868).  The element quantities are those of CM_ThreeField,
+// src/constitutive_model_3f.cc:189-470, written here in their material form:
+//   S, L   deviatoric PK2 and tangent at eF = Fr (update_elasticity_dev, constitutive_model.h:413-434)
+//   dU,ddU kappa U'(theta), kappa U''(theta) (compute_dudj/d2udj2, constitutive_model.h:319-384)
+// Checks: Kuu, Kup, Ktt, Ktp against central differences of Ru, Rt, Rp; at theta = J and
+// P = kappa U'(J) on linear tets Ru equals the displacement element's residual; the
+// condensation equals a monolithic solve of the element's (u, theta, P) system; a converged
+// CM3F Newton on linear tets equals the displacement solution -> test_oracle_three_field.py.
+namespace {
+
+struct TfElem {
+  double Kuu[900], Ru[30], Kup[30], Kut[30], Ktu[30];
+  double Rp, Rt, Ktt, Ktp;
+};
+
+void tf_dof_grad(const double *dNdX, int k, double *G) {  // Grad_du = ST_(a,b): e_b (x) dN_a
+  const int a = k / 3, b = k % 3;
+  for (int i = 0; i < 9; ++i) G[i] = 0.0;
+  for (int J = 0; J < 3; ++J) G[b * 3 + J] = dNdX[a * 3 + J];
+}
+
+int tf_element(const orc_material &mat, int nne, const double *x, const double *u, double theta, double P,
+               bool want_K, TfElem &E, double *Fbar_out /* [nint][9] or null */) {
+  const int nd = nne * 3;
+  std::memset(&E, 0, sizeof(E));
+  const double kappa = kappa_of(mat);
+  double dU, ddU, U;
+  vol_pot(mat.volPotFlag, theta, &dU, &ddU, &U);
+  dU *= kappa;
+  ddU *= kappa;
+  const double I9[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+  int err = 0;
+  for (int ip = 0; ip < nint_of(nne); ++ip) {
+    double N[10], dNdX[30], w;
+    if (orc_basis(nne, x, ip, N, dNdX, &w)) { ++err; continue; }
+    double Fr[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    for (int a = 0; a < nne; ++a)
+      for (int b = 0; b < 3; ++b)
+        for (int c = 0; c < 3; ++c) Fr[b * 3 + c] += u[a * 3 + b] * dNdX[a * 3 + c];
+    const double tJr = det3x3(Fr);
+    double Fi[9];
+    if (!(tJr > 0.0) || inv3x3(Fr, Fi)) { ++err; continue; }
+    if (Fbar_out) {  // the model's stored F: (theta/J)^(1/3) Fr (constitutive_model.cc:815, :1115)
+      const double fac = std::cbrt(theta / tJr);
+      for (int i = 0; i < 9; ++i) Fbar_out[ip * 9 + i] = fac * Fr[i];
+    }
+    double C[9], S[9], L[81];
+    matmul_tn(Fr, Fr, C);
+    dev_stress_mr(C, mat.m10, mat.m01, S);
+    if (want_K) mat_stiff_mr(C, mat.m10, mat.m01, L);
+    const double *M = I9, *eFn = I9;
+    double Psi[30][9], trGF[30];
+    for (int k = 0; k < nd; ++k) {
+      double G[9], A[9], As[9], Phi[9], T[9];
+      tf_dof_grad(dNdX, k, G);
+      matmul_tn(G, Fr, A);  // A = Grad_du^T tFr
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) As[i * 3 + j] = 0.5 * (A[i * 3 + j] + A[j * 3 + i]);
+      for (int i = 0; i < 9; ++i) T[i] = 0.0;  // Phi = eFn^T As eFn, Psi = M^T Phi M
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+          for (int kk = 0; kk < 3; ++kk)
+            for (int l = 0; l < 3; ++l) T[i * 3 + j] += eFn[kk * 3 + i] * As[kk * 3 + l] * eFn[l * 3 + j];
+      for (int i = 0; i < 9; ++i) Phi[i] = T[i];
+      for (int i = 0; i < 9; ++i) Psi[k][i] = 0.0;
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+          for (int kk = 0; kk < 3; ++kk)
+            for (int l = 0; l < 3; ++l) Psi[k][i * 3 + j] += M[kk * 3 + i] * Phi[kk * 3 + l] * M[l * 3 + j];
+      double tr = 0.0;  // Grad_du(j,i) tFrI(i,j)
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) tr += G[j * 3 + i] * Fi[i * 3 + j];
+      trGF[k] = tr;
+      double PsiS = 0.0;
+      for (int i = 0; i < 9; ++i) PsiS += Psi[k][i] * S[i];
+      E.Ru[k] += (PsiS + P * tJr * tr) * w;          // compute_Ru, :190-212
+      E.Kup[k] += tJr * tr * w;                       // compute_Kup, :292-316 (dt factor -1)
+    }
+    E.Rp += (tJr - theta) * w;                        // compute_Rp, :219-226
+    E.Rt += (dU - P) * w;                             // compute_Rt, :237-246
+    E.Ktt += ddU * w;                                 // compute_Ktt, :438-466 (dM/dtheta = 0)
+    E.Ktp -= w;                                       // compute_Ktp, :418-430
+    if (!want_K) continue;
+    for (int k = 0; k < nd; ++k) {
+      double Gk[9];
+      tf_dof_grad(dNdX, k, Gk);
+      for (int l = 0; l < nd; ++l) {
+        double Gl[9];
+        tf_dof_grad(dNdX, l, Gl);
+        double PLP = 0.0;
+        for (int i = 0; i < 3; ++i)
+          for (int j = 0; j < 3; ++j)
+            for (int kk = 0; kk < 3; ++kk)
+              for (int ll = 0; ll < 3; ++ll) PLP += Psi[k][i * 3 + j] * L[idx4(i, j, kk, ll)] * Psi[l][kk * 3 + ll];
+        double tr2 = 0.0;  // Grad_du(j,i) tFrI(i,k) Grad_tu(k,l) tFrI(l,j)
+        for (int i = 0; i < 3; ++i)
+          for (int j = 0; j < 3; ++j)
+            for (int kk = 0; kk < 3; ++kk)
+              for (int ll = 0; ll < 3; ++ll) tr2 += Gk[j * 3 + i] * Fi[i * 3 + kk] * Gl[kk * 3 + ll] * Fi[ll * 3 + j];
+        const double pJ = P * tJr * (trGF[k] * trGF[l] - tr2);
+        double GG[9], DS = 0.0;  // DPsi = eFnM^T sym(Grad_du^T Grad_tu) eFnM, eFnM = I
+        matmul_tn(Gk, Gl, GG);
+        for (int i = 0; i < 3; ++i)
+          for (int j = 0; j < 3; ++j) DS += 0.5 * (GG[i * 3 + j] + GG[j * 3 + i]) * S[i * 3 + j];
+        E.Kuu[k * nd + l] += w * (PLP + pJ + DS);  // compute_Kuu, :249-286 (dt factor -1)
+      }
+    }
+  }
+  return err;
+}
+
+// condense_K_3F_to_1F / condense_F_3F_to_1F, src/condense.cc:304-385, Pno = Vno = 1
+// (Kpt = Ktp^T, Kpu = Kup^T: IntegrateThreeFieldStiffness::out_loop, constitutive_model.cc:531-555)
+void tf_condense(int nd, const TfElem &E, double *K, double *f) {
+  const double Kpt = E.Ktp;
+  for (int i = 0; i < nd; ++i) {
+    if (K)
+      for (int j = 0; j < nd; ++j)
+        K[i * nd + j] = E.Kuu[i * nd + j] - E.Kup[i] / E.Ktp * E.Ktu[j] +
+                        E.Kup[i] / E.Ktp * E.Ktt / Kpt * E.Kup[j] - E.Kut[i] / Kpt * E.Kup[j];
+    if (f) f[i] = E.Ru[i] + E.Kup[i] / E.Ktp * E.Ktt / Kpt * E.Rp - E.Kup[i] / E.Ktp * E.Rt - E.Kut[i] / Kpt * E.Rp;
+  }
+}
+
+void tf_gather(const double *X, const int64_t *nod, int nne, const int64_t *id, const double *r, const double *d_r,
+               const double *defl, const double *defl_d, double *x, double *u, int64_t *cn) {
+  for (int a = 0; a < nne; ++a)
+    for (int b = 0; b < 3; ++b) {  // def_elem_total, src/utils.cc:1391-1412
+      x[a * 3 + b] = X[nod[a] * 3 + b];
+      const int64_t dof = id[nod[a] * 3 + b];
+      cn[a * 3 + b] = dof;
+      const int64_t aid = std::llabs(dof) - 1;
+      if (dof == 0) u[a * 3 + b] = 0.0;
+      else if (dof > 0) u[a * 3 + b] = r[aid] + (d_r ? d_r[aid] : 0.0);
+      else u[a * 3 + b] = (defl ? defl[aid] : 0.0) + (defl_d ? defl_d[aid] : 0.0);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+// One CM3F element (material form) for tests: the blocks of CM_ThreeField (Kuu row-major
+// [3nne][3nne], Ru, Kup, Kut, Ktu; sc = Rp, Rt, Ktt, Ktp) and the condensed K and f.
+int orc_tf_element(const orc_material *mat, int nne, const double *x, const double *u, double theta, double P,
+                   double *Kuu, double *Ru, double *Kup, double *sc, double *Kc, double *fc) {
+  TfElem E;
+  const int err = tf_element(*mat, nne, x, u, theta, P, Kuu || Kc, E, nullptr);
+  const int nd = 3 * nne;
+  if (Kuu) std::memcpy(Kuu, E.Kuu, sizeof(double) * nd * nd);
+  if (Ru) std::memcpy(Ru, E.Ru, sizeof(double) * nd);
+  if (Kup) std::memcpy(Kup, E.Kup, sizeof(double) * nd);
+  if (sc) { sc[0] = E.Rp; sc[1] = E.Rt; sc[2] = E.Ktt; sc[3] = E.Ktp; }
+  if (Kc || fc) tf_condense(nd, E, Kc, fc);
+  return err;
+}
+
+// fd_residuals_MP + stiffmat_fd_MP of the CM3F analysis (residuals_el_constitutive_model_3f /
+// stiffness_el_constitutive_model_3f, src/constitutive_model.cc:2714-2735, :3565-3590):
+// theta[e] = V_np1 + dV and P[e] = P_np1 + dP of each element; with run_integration the
+// model's F(n+1) (per QP) is stored.
+int orc_assemble_tf(int64_t n_elem, const double *X, const int64_t *elem_ptr, const int64_t *elem_conn,
+                    const int32_t *elem_mat, const orc_material *mats, const int64_t *id, int64_t ndof,
+                    const int32_t *Ap, const int32_t *Ai, const double *r, const double *d_r, const double *defl,
+                    const double *defl_d, const double *theta, const double *P, int run_integration,
+                    double *F_np1, int64_t n_qp, double *f, double *vals) {
+  if (f) std::memset(f, 0, sizeof(double) * ndof);
+  if (vals) std::memset(vals, 0, sizeof(double) * Ap[ndof]);
+  int err = 0;
+  int64_t qp0 = 0;
+  for (int64_t e = 0; e < n_elem; ++e) {
+    const int nne = int(elem_ptr[e + 1] - elem_ptr[e]), nd = 3 * nne;
+    double x[30], u[30], Fb[72];
+    int64_t cn[30];
+    tf_gather(X, elem_conn + elem_ptr[e], nne, id, r, d_r, defl, defl_d, x, u, cn);
+    TfElem E;
+    const orc_material &mat = mats[elem_mat ? elem_mat[e] : 0];
+    err += tf_element(mat, nne, x, u, theta[e], P[e], vals != nullptr, E, Fb);
+    if (run_integration && F_np1)
+      for (int ip = 0; ip < nint_of(nne); ++ip)
+        for (int i = 0; i < 9; ++i) F_np1[i * n_qp + qp0 + ip] = Fb[ip * 9 + i];
+    std::vector<double> K(size_t(nd) * nd);
+    double fe[30];
+    tf_condense(nd, E, K.data(), fe);
+    if (vals)
+      for (int a = 0; a < nd; ++a)
+        if (!std::isnormal(K[a * nd + a])) ++err;  // IntegrateThreeFieldStiffness::out_loop
+    for (int i = 0; i < nd; ++i) {
+      const int64_t row = cn[i] - 1;
+      if (row < 0) continue;
+      if (f) f[row] += fe[i];
+      if (vals)
+        for (int j = 0; j < nd; ++j) {
+          const int64_t col = cn[j] - 1;
+          if (col < 0) continue;
+          const int32_t *p = Ai + Ap[row], *en = Ai + Ap[row + 1];
+          while (p < en && *p != col) ++p;
+          if (p == en) return -1;
+          vals[p - Ai] += K[i * nd + j];
+        }
+    }
+    qp0 += nint_of(nne);
+  }
+  return err;
+}
+
+// constitutive_model_update_NR for CM3F without inertia (src/constitutive_model.cc:3916-3999,
+// IntegrateThreeFieldUpdate :595-656, compute_d_theta_dP src/constitutive_model_3f.cc:494-549):
+// at u = r + d_r (defl + defl_d), theta, P, the increments of theta and P that go with the
+// displacement increment du (prescribed DOFs zero: def_elem TYPE 2, src/utils.cc:1365-1389).
+int orc_tf_update(int64_t n_elem, const double *X, const int64_t *elem_ptr, const int64_t *elem_conn,
+                  const int32_t *elem_mat, const orc_material *mats, const int64_t *id, const double *r,
+                  const double *d_r, const double *defl, const double *defl_d, const double *theta, const double *P,
+                  const double *du, double *ddV, double *ddP) {
+  int err = 0;
+  for (int64_t e = 0; e < n_elem; ++e) {
+    const int nne = int(elem_ptr[e + 1] - elem_ptr[e]), nd = 3 * nne;
+    double x[30], u[30];
+    int64_t cn[30];
+    tf_gather(X, elem_conn + elem_ptr[e], nne, id, r, d_r, defl, defl_d, x, u, cn);
+    TfElem E;
+    err += tf_element(mats[elem_mat ? elem_mat[e] : 0], nne, x, u, theta[e], P[e], false, E, nullptr);
+    const double Kpt = E.Ktp;
+    double Kpu_du = 0.0, Ktu_du = 0.0;
+    for (int k = 0; k < nd; ++k) {
+      const double d = cn[k] > 0 ? du[cn[k] - 1] : 0.0;
+      Kpu_du += E.Kup[k] * d;
+      Ktu_du += E.Ktu[k] * d;
+    }
+    const double dth = -(Kpu_du + E.Rp) / Kpt;
+    ddV[e] = dth;
+    ddP[e] = -(E.Rt + Ktu_du + E.Ktt * dth) / E.Ktp;
+  }
+  return err;
+}
+
+// compute_3f_initial_conditions, src/three_field_element.cc:1328-1386: per element the
+// volume averages of J and of kappa U'(J) at the nodal displacements u ([n_nodes][3]).
+int orc_tf_initial(int64_t n_elem, const double *X, const int64_t *elem_ptr, const int64_t *elem_conn,
+                   const int32_t *elem_mat, const orc_material *mats, const double *u_nodal, double *V,
+                   double *Pr) {
+  int err = 0;
+  for (int64_t e = 0; e < n_elem; ++e) {
+    const int nne = int(elem_ptr[e + 1] - elem_ptr[e]);
+    const int64_t *nod = elem_conn + elem_ptr[e];
+    const orc_material &mat = mats[elem_mat ? elem_mat[e] : 0];
+    double x[30], u[30];
+    for (int a = 0; a < nne; ++a)
+      for (int b = 0; b < 3; ++b) {
+        x[a * 3 + b] = X[nod[a] * 3 + b];
+        u[a * 3 + b] = u_nodal[nod[a] * 3 + b];
+      }
+    double vol = 0.0, Jv = 0.0, Up = 0.0;
+    for (int ip = 0; ip < nint_of(nne); ++ip) {
+      double N[10], dNdX[30], w;
+      if (orc_basis(nne, x, ip, N, dNdX, &w)) { ++err; continue; }
+      double F[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+      for (int a = 0; a < nne; ++a)
+        for (int b = 0; b < 3; ++b)
+          for (int c = 0; c < 3; ++c) F[b * 3 + c] += u[a * 3 + b] * dNdX[a * 3 + c];
+      const double J = det3x3(F);
+      double dU, d2U, U;
+      vol_pot(mat.volPotFlag, J, &dU, &d2U, &U);
+      vol += w;
+      Jv += w * J;
+      Up += w * kappa_of(mat) * dU;
+    }
+    V[e] = Jv / vol;
+    Pr[e] = Up / vol;
+  }
+  return err;
+}
+
+}  // extern "C"

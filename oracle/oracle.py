@@ -378,3 +378,75 @@ def line_search_gamma(gam, phi, prev=None):
                 lam = -slope / (b + np.sqrt(disc))
         lam = min(lam, 0.5 * gam)
     return lam if lam >= 0.1 * gam else 0.1 * gam
+
+
+# ----------------------------------------------------------------------------- three-field (CM3F)
+def _tf_lib():
+    L = lib()
+    if not getattr(L, "_tf_ready", False):
+        vp, i64 = C.c_void_p, C.c_int64
+        L.orc_tf_element.argtypes = [vp, C.c_int, vp, vp, C.c_double, C.c_double, vp, vp, vp, vp, vp, vp]
+        L.orc_assemble_tf.argtypes = [i64, vp, vp, vp, vp, vp, vp, i64, vp, vp, vp, vp, vp, vp, vp, vp, C.c_int, vp, i64,
+                                      vp, vp]
+        L.orc_tf_update.argtypes = [i64] + [vp] * 15
+        L.orc_tf_initial.argtypes = [i64] + [vp] * 8
+        for fn in ("orc_tf_element", "orc_assemble_tf", "orc_tf_update", "orc_tf_initial"):
+            getattr(L, fn).restype = C.c_int
+        L._tf_ready = True
+    return L
+
+
+def tf_element(mat, nne, x, u, theta, P):
+    """One CM3F element (material form): its blocks and the condensed K and f."""
+    nd = 3 * nne
+    out = dict(Kuu=np.zeros((nd, nd)), Ru=np.zeros(nd), Kup=np.zeros(nd), sc=np.zeros(4), Kc=np.zeros((nd, nd)),
+               fc=np.zeros(nd))
+    om = material(mat)
+    rc = _tf_lib().orc_tf_element(C.byref(om), nne, _p(np.ascontiguousarray(x, dtype=np.float64)),
+                                  _p(np.ascontiguousarray(u, dtype=np.float64)), float(theta), float(P),
+                                  *[_p(out[k]) for k in ("Kuu", "Ru", "Kup", "sc", "Kc", "fc")])
+    out["Rp"], out["Rt"], out["Ktt"], out["Ktp"] = out.pop("sc")
+    out["rc"] = rc
+    return out
+
+
+def _mesh_args(mesh, mats):
+    om = (OrcMaterial * len(mats))(*[material(m) for m in mats])
+    em = mesh.elem_mat if mesh.elem_mat is not None else None
+    return om, em
+
+
+def assemble_tf(mesh, mats, ids, ndof, Ap, Ai, theta, P, r, d_r=None, defl=None, defl_d=None, integrate=True,
+                F_np1=None, tangent=True):
+    """Condensed CM3F residual and tangent over the mesh. Returns (f, vals, err)."""
+    om, em = _mesh_args(mesh, mats)
+    f = np.zeros(ndof)
+    vals = np.zeros(Ap[-1]) if tangent else None
+    nq = F_np1.shape[1] if F_np1 is not None else 0
+    err = _tf_lib().orc_assemble_tf(mesh.n_elem, _p(mesh.X), _p(mesh.elem_ptr), _p(mesh.elem_conn), _p(em), om,
+                                    _p(ids), ndof, _p(Ap), _p(Ai), _p(r), _p(d_r), _p(defl), _p(defl_d),
+                                    _p(np.ascontiguousarray(theta, dtype=np.float64)),
+                                    _p(np.ascontiguousarray(P, dtype=np.float64)), int(integrate), _p(F_np1), nq,
+                                    _p(f), _p(vals))
+    return f, vals, err
+
+
+def tf_update(mesh, mats, ids, theta, P, du, r, d_r=None, defl=None, defl_d=None):
+    """Increments (ddV, ddP) of the elements' volume and pressure for the displacement increment du."""
+    om, em = _mesh_args(mesh, mats)
+    ddV, ddP = np.zeros(mesh.n_elem), np.zeros(mesh.n_elem)
+    err = _tf_lib().orc_tf_update(mesh.n_elem, _p(mesh.X), _p(mesh.elem_ptr), _p(mesh.elem_conn), _p(em), om,
+                                  _p(ids), _p(r), _p(d_r), _p(defl), _p(defl_d),
+                                  _p(np.ascontiguousarray(theta, dtype=np.float64)),
+                                  _p(np.ascontiguousarray(P, dtype=np.float64)),
+                                  _p(np.ascontiguousarray(du, dtype=np.float64)), _p(ddV), _p(ddP))
+    return ddV, ddP, err
+
+
+def tf_initial(mesh, mats, u_nodal):
+    """Per element volume-averaged J and kappa U'(J) at nodal displacements [n_nodes, 3]."""
+    om, em = _mesh_args(mesh, mats)
+    V, Pr = np.zeros(mesh.n_elem), np.zeros(mesh.n_elem)
+    err = _tf_lib().orc_tf_initial(mesh.n_elem, _p(mesh.X), _p(mesh.elem_ptr), _p(mesh.elem_conn), _p(em), om,
+                                   _p(np.ascontiguousarray(u_nodal, dtype=np.float64)), _p(V), _p(Pr))
+    return V, Pr, err

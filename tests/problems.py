@@ -251,3 +251,35 @@ class MixedProblem(BoxProblem):
     def assemble_coh(self, st, r, defl, f, vals):
         return O.assemble_coh(self.mesh, self.coh, None, self.ids, self.ndof, self.Ap, self.Ai, st, r, f, vals,
                               defl=defl)
+
+
+def oracle_tf_newton_step(prob, r, defl, tf, tol=1e-12, max_iter=25, F_np1=None, state=None):
+    """One load step of the CM3F analysis over the CPU oracle (sparse direct solve, full
+    steps): each iteration takes the condensed residual and tangent at (x, theta, P), solves
+    K du = -f, takes the elements' volume and pressure increments for du
+    (constitutive_model_update_NR, src/constitutive_model.cc:3916-3999) and steps all three.
+    tf = dict(V=V_np1, P=P_np1) is advanced in place (the increments folded in, as
+    update_np1_from_increments does at the end of a step).  Returns (r, iterations)."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+
+    r = r.copy()
+    th, pr = tf["V"].copy(), tf["P"].copy()
+    state = state if state is not None else {"NORM": 0.0, "first": True}
+    for it in range(max_iter):
+        f, vals, err = O.assemble_tf(prob.mesh, prob.mats, prob.ids, prob.ndof, prob.Ap, prob.Ai, th, pr, r,
+                                     defl=defl, F_np1=F_np1)
+        assert err == 0
+        nor2 = float(np.linalg.norm(f))
+        if it == 0 and (state["first"] or state["NORM"] < tol):
+            state["NORM"] = max(state["NORM"], nor2, 1e-15)
+            state["first"] = False
+        if nor2 / state["NORM"] <= tol or nor2 <= 1e-15:
+            tf["V"], tf["P"] = th, pr
+            return r, it
+        K = sp.csr_matrix((vals, prob.Ai, prob.Ap), shape=(prob.ndof, prob.ndof))
+        du = spla.spsolve(K.tocsc(), -f)
+        ddV, ddP, err = O.tf_update(prob.mesh, prob.mats, prob.ids, th, pr, du, r, defl=defl)
+        assert err == 0
+        r, th, pr = r + du, th + ddV, pr + ddP
+    raise AssertionError("oracle CM3F Newton did not converge")

@@ -1,0 +1,154 @@
+"""Pins of the oracle's CM3F three-field formulation of the hyperelastic model
+(CM_ThreeField, src/constitutive_model_3f.cc; condensation src/condense.cc:304-385; the
+element update src/constitutive_model.cc:595-656, 3916-3999):
+
+  * each block is the derivative of the residual it belongs to (central differences):
+    Kuu = dRu/du, Kup = dRu/dP, Kup = dRp/du, Ktt = dRt/dtheta, Ktp = dRt/dP = dRp/dtheta;
+  * the condensed K and f equal a monolithic solve of the element's (u, theta, P) system;
+  * on linear tets (one QP, J constant over the element) at theta = J and P = kappa U'(J), the
+    condensed residual and tangent are those of the displacement element, and a converged
+    CM3F load step lands on the displacement solution;
+  * on quadratic tets and hexes a converged step has theta = the volume average of J and
+    P = kappa U'(theta) in every element;
+  * the initial conditions of a homogeneous stretch are theta = det F, P = kappa U'(det F).
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+import pgfem3d_amd as P
+from problems import BoxProblem, oracle_newton_step, oracle_tf_newton_step
+
+HE = P.neo_hookean(200e3, 0.3)
+KAPPA = 200e3 / (3 * (1 - 2 * 0.3))
+
+
+def _dU(J, flag=2):  # Doll-Schweizerhof 7 (vol_pot flag 2), U'(J) and U''(J)
+    return 0.5 * (np.exp(J - 1) - 1 / J), 0.5 * (np.exp(J - 1) + 1 / J ** 2)
+
+
+def _element(nne, seed):
+    rng = np.random.default_rng(seed)
+    prob = BoxProblem(1, nne, HE, jitter=0.1, seed=seed)
+    e = 0
+    nod = prob.mesh.elem_conn[prob.mesh.elem_ptr[e]:prob.mesh.elem_ptr[e + 1]]
+    x = prob.mesh.X[nod].reshape(-1)
+    u = 0.03 * rng.standard_normal(x.shape)
+    return x, u
+
+
+@pytest.mark.parametrize("nne", [4, 10, 8])
+def test_blocks_are_derivatives(nne):
+    x, u = _element(nne, 3)
+    th, pr = 1.01, 150.0
+    E = O.tf_element(HE, nne, x, u, th, pr)
+    assert E["rc"] == 0
+    nd = 3 * nne
+    h = 1e-6
+    Kfd, kpu = np.zeros((nd, nd)), np.zeros(nd)
+    for k in range(nd):
+        up, um = u.copy(), u.copy()
+        up[k] += h
+        um[k] -= h
+        a, b = O.tf_element(HE, nne, x, up, th, pr), O.tf_element(HE, nne, x, um, th, pr)
+        Kfd[:, k] = (a["Ru"] - b["Ru"]) / (2 * h)
+        kpu[k] = (a["Rp"] - b["Rp"]) / (2 * h)
+    scale = np.abs(E["Kuu"]).max()
+    assert np.abs(Kfd - E["Kuu"]).max() < 1e-6 * scale
+    assert np.abs(kpu - E["Kup"]).max() < 1e-6 * np.abs(E["Kup"]).max()
+    hp = 1e-3
+    a, b = O.tf_element(HE, nne, x, u, th, pr + hp), O.tf_element(HE, nne, x, u, th, pr - hp)
+    assert np.allclose((a["Ru"] - b["Ru"]) / (2 * hp), E["Kup"], rtol=1e-7, atol=1e-9 * np.abs(E["Kup"]).max())
+    assert np.isclose((a["Rt"] - b["Rt"]) / (2 * hp), E["Ktp"], rtol=1e-9)
+    ht = 1e-6
+    a, b = O.tf_element(HE, nne, x, u, th + ht, pr), O.tf_element(HE, nne, x, u, th - ht, pr)
+    assert np.isclose((a["Rt"] - b["Rt"]) / (2 * ht), E["Ktt"], rtol=1e-6)
+    assert np.isclose((a["Rp"] - b["Rp"]) / (2 * ht), E["Ktp"], rtol=1e-9)
+    assert np.abs(a["Ru"] - b["Ru"]).max() == 0.0  # Kut = 0: the deviatoric stress does not see theta
+
+
+@pytest.mark.parametrize("nne", [4, 10, 8])
+def test_condensation_is_the_monolithic_solve(nne):
+    x, u = _element(nne, 5)
+    E = O.tf_element(HE, nne, x, u, 0.995, -80.0)
+    nd = 3 * nne
+    A = np.zeros((nd + 2, nd + 2))
+    A[:nd, :nd] = E["Kuu"]
+    A[:nd, nd + 1] = E["Kup"]            # u rows: Kut = 0, Kup
+    A[nd, nd] = E["Ktt"]                 # theta row: Ktu = 0, Ktt, Ktp
+    A[nd, nd + 1] = E["Ktp"]
+    A[nd + 1, :nd] = E["Kup"]            # P row: Kpu = Kup^T, Kpt = Ktp, Kpp = 0
+    A[nd + 1, nd] = E["Ktp"]
+    rhs = -np.concatenate([E["Ru"], [E["Rt"], E["Rp"]]])
+    # hold the first node fixed so both systems are regular
+    free = np.r_[3:nd, nd, nd + 1]
+    sol = np.zeros(nd + 2)
+    sol[free] = np.linalg.solve(A[np.ix_(free, free)], rhs[free])
+    fu = np.arange(3, nd)
+    du = np.zeros(nd)
+    du[fu] = np.linalg.solve(E["Kc"][np.ix_(fu, fu)], -E["fc"][fu])
+    assert np.abs(du - sol[:nd]).max() < 1e-10 * np.abs(du).max()
+
+
+def test_linear_tets_reduce_to_the_displacement_element():
+    prob = BoxProblem(3, 4, HE, jitter=0.1, seed=2)
+    r, defl = prob.displacement(0.03, 0.01, seed=4)
+    u = np.zeros_like(prob.mesh.X)
+    m = prob.ids > 0
+    u[m] = r[prob.ids[m] - 1]
+    u[prob.ids < 0] = defl[-prob.ids[prob.ids < 0] - 1]
+    V, Pr, err = O.tf_initial(prob.mesh, prob.mats, u)  # one QP: theta = J, P = kappa U'(J)
+    assert err == 0
+    f3, K3, err = O.assemble_tf(prob.mesh, prob.mats, prob.ids, prob.ndof, prob.Ap, prob.Ai, V, Pr, r, defl=defl)
+    assert err == 0
+    st = O.OracleState(prob.nqp, False)
+    f1, K1, err = O.assemble(prob.mesh, prob.mats, prob.ids, prob.ndof, prob.Ap, prob.Ai, st, r, defl=defl)
+    assert err == 0
+    assert np.abs(f3 - f1).max() < 1e-10 * np.abs(f1).max()
+    assert np.abs(K3 - K1).max() < 1e-10 * np.abs(K1).max()
+
+
+def test_converged_tet4_step_is_the_displacement_solution():
+    prob = BoxProblem(3, 4, HE, jitter=0.1, seed=2)
+    defl = np.array([0.04])
+    r0 = np.zeros(prob.ndof)
+    tf = dict(V=np.ones(prob.mesh.n_elem), P=np.zeros(prob.mesh.n_elem))
+    r3, its = oracle_tf_newton_step(prob, r0, defl, tf, tol=1e-12)
+    st = O.OracleState(prob.nqp, False)
+    r1, _, _ = oracle_newton_step(prob, st, r0, defl, {"NORM": 0.0, "first": True}, tol=1e-12)
+    assert np.abs(r3 - r1).max() < 1e-9 * np.abs(r1).max()
+    assert its <= 6
+
+
+@pytest.mark.parametrize("nne", [10, 8])
+def test_converged_step_satisfies_the_constraints(nne):
+    prob = BoxProblem(2, nne, HE, jitter=0.1, seed=3)
+    tf = dict(V=np.ones(prob.mesh.n_elem), P=np.zeros(prob.mesh.n_elem))
+    r, its = oracle_tf_newton_step(prob, np.zeros(prob.ndof), np.array([0.05]), tf, tol=1e-12)
+    assert its <= 7
+    u = np.zeros_like(prob.mesh.X)
+    m = prob.ids > 0
+    u[m] = r[prob.ids[m] - 1]
+    u[prob.ids < 0] = 0.05
+    for e in range(prob.mesh.n_elem):  # Rp = 0: theta = mean J;  Rt = 0: P = kappa U'(theta)
+        nod = prob.mesh.elem_conn[prob.mesh.elem_ptr[e]:prob.mesh.elem_ptr[e + 1]]
+        E = O.tf_element(HE, nne, prob.mesh.X[nod].reshape(-1), u[nod].reshape(-1), tf["V"][e], tf["P"][e])
+        assert abs(E["Rp"]) < 1e-9 * abs(E["Ktp"]) and abs(E["Rt"]) < 1e-9 * abs(E["Ktp"]) * KAPPA
+        assert np.isclose(tf["P"][e], KAPPA * _dU(tf["V"][e])[0], rtol=1e-9, atol=1e-9)
+    # the u residual of the converged fields (uncondensed) vanishes at the free DOFs
+    f, _, err = O.assemble_tf(prob.mesh, prob.mats, prob.ids, prob.ndof, prob.Ap, prob.Ai, tf["V"], tf["P"], r,
+                              defl=np.array([0.05]), tangent=False)
+    assert err == 0 and np.abs(f).max() < 1e-7 * KAPPA
+
+
+@pytest.mark.parametrize("nne", [4, 10, 8])
+def test_initial_conditions_of_a_homogeneous_stretch(nne):
+    prob = BoxProblem(2, nne, HE, jitter=0.1, seed=1)
+    F = np.array([[1.02, 0.01, 0.0], [0.0, 0.99, 0.02], [0.01, 0.0, 1.03]])
+    u = prob.mesh.X @ (F - np.eye(3)).T
+    V, Pr, err = O.tf_initial(prob.mesh, prob.mats, u)
+    assert err == 0
+    J = np.linalg.det(F)
+    assert np.allclose(V, J, rtol=1e-13) and np.allclose(Pr, KAPPA * _dU(J)[0], rtol=1e-11)
+    V0, P0, _ = O.tf_initial(prob.mesh, prob.mats, np.zeros_like(u))
+    assert np.all(V0 == 1.0) and np.abs(P0).max() < 1e-12
