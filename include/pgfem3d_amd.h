@@ -268,6 +268,46 @@ int pgf_commit_step(pgf_handle h);
  * constitutive_model_reset_state_using_temporal, src/constitutive_model.cc:1421-1440). */
 int pgf_reset_step(pgf_handle h);
 
+/* ============================================================ three-field (CM3F) */
+/* The CM3F analysis (-cm3f: analysis_type CM3F, include/enumerations.h:16) for meshes whose
+ * materials are all hyperelastic: one volume variable theta and one pressure P per element
+ * (get_3f_pressure_volume_number, src/three_field_element.cc:1399-1416), condensed out of
+ * the element residual and tangent (condense_K_3F_to_1F / condense_F_3F_to_1F,
+ * src/condense.cc:304-385).  With the mode on, pgf_residual / pgf_tangent /
+ * pgf_residual_tangent / pgf_prescribed_load evaluate the condensed element of
+ * residuals_el_constitutive_model_3f / stiffness_el_constitutive_model_3f
+ * (src/constitutive_model.cc:2714-2735, :3565-3590) at theta = V(n+1) + dV, P = P(n+1) + dP.
+ * pgf_commit_step folds the increments in and rolls the variables over
+ * (update_np1_from_increments + update_for_next_time_step, Newton_Raphson.cc:683-689);
+ * pgf_reset_step zeroes the increments (ThreeFieldVariables::reset).
+ * on = 0 turns the mode off and frees its memory.  PGF_ENOTSUP if a material is not
+ * hyperelastic. */
+int pgf_set_three_field(pgf_handle h, int32_t on);
+/* compute_3f_initial_conditions (src/three_field_element.cc:1328-1386): theta and P at n (and
+ * n+1) from the nodal input in_n, at n-1 from in_nm1 (NULL: in_n); increments zeroed. */
+int pgf_tf_initial(pgf_handle h, const pgf_nodal_input *in_n, const pgf_nodal_input *in_nm1);
+/* constitutive_model_update_NR without inertia (src/constitutive_model.cc:3916-3999,
+ * IntegrateThreeFieldUpdate :595-656): ddV, ddP of every element for the Newton increment
+ * du (device [ndof]), from the element data of the last integrating residual pass, which
+ * must have been taken at the state the increment starts from. */
+int pgf_tf_update(pgf_handle h, const double *du);
+/* ThreeFieldVariables::update_increments_from_NR (include/PGFem3D_data_structure.h:197-213):
+ * dV += gamma ddV, dP += gamma ddP. */
+int pgf_tf_accept(pgf_handle h, double gamma);
+/* Per-element variables, [n_elem] in mesh element order. */
+#define PGF_TF_V_NP1 0
+#define PGF_TF_V_N 1
+#define PGF_TF_V_NM1 2
+#define PGF_TF_P_NP1 3
+#define PGF_TF_P_N 4
+#define PGF_TF_P_NM1 5
+#define PGF_TF_DV 6
+#define PGF_TF_DDV 7
+#define PGF_TF_DP 8
+#define PGF_TF_DDP 9
+int pgf_tf_get(pgf_handle h, int32_t field, double *host_out);
+int pgf_tf_set(pgf_handle h, int32_t field, const double *host_in);
+
 /* State fields, stored structure-of-arrays [component][n_qp] in QP order (element order,
  * then QP order inside the element). */
 #define PGF_STATE_F_NP1 0     /* 9 comps: F at n+1 */

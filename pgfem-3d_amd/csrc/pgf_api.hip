@@ -25,6 +25,7 @@ constexpr int nqp_of(int nne) { return nne == 4 ? 1 : (nne == 10 ? 4 : 8); }
 struct Block {
   int nne = 0, nqp = 0, model = 0;
   int64_t ne = 0, qp_base = 0;
+  int64_t e_base = 0;              // first element of the block in per-element device arrays
   std::vector<int64_t> elems;      // mesh element ids in device order
   std::vector<int32_t> clist;      // block-local element indices sorted by colour
   std::vector<int64_t> color_ptr;  // [n_colors+1] colour segments of clist
@@ -154,8 +155,14 @@ struct GraphKey {
 struct pgf_handle_s {
   int device = 0;
   hipStream_t own_stream = nullptr, stream = nullptr;
-  int64_t n_nodes = 0, ndof = 0, nnz = 0, n_qp = 0;
+  int64_t n_nodes = 0, ndof = 0, nnz = 0, n_qp = 0, n_elem = 0;
   int32_t n_colors = 0;
+  // three-field (CM3F) mode: per element in device order, d_tf holds [10][n_elem] = V at
+  // n+1, n, n-1; P at n+1, n, n-1; dV, ddV, dP, ddP (ThreeFieldVariables,
+  // include/PGFem3D_data_structure.h:130-235); d_tfbuf the element data of the last residual
+  // pass, [n_elem][kTfBuf]
+  bool tf = false;
+  double *d_tf = nullptr, *d_tfbuf = nullptr;
   bool has_j2 = false;
   double *d_X = nullptr;
   int32_t *d_dofid = nullptr, *d_row_start = nullptr, *d_rowlen = nullptr;
@@ -469,6 +476,14 @@ int32_t enqueue_pass(pgf_handle h, const pgf_nodal_input *in, int mode, double *
         a.elist = gather ? nullptr : b.d_clist;
         a.mat = b.d_mat;
         a.qp_base = b.qp_base;
+        if (h->tf) {
+          const int64_t NE = h->n_elem;
+          a.tf_V = h->d_tf + b.e_base;
+          a.tf_P = h->d_tf + 3 * NE + b.e_base;
+          a.tf_dV = h->d_tf + 6 * NE + b.e_base;
+          a.tf_dP = h->d_tf + 8 * NE + b.e_base;
+          a.tf_buf = h->d_tfbuf + b.e_base * pgf::kTfBuf;
+        }
         pgf::launch_elem(b.nne, mode, a, e0, e1, h->stream);
       }
       ++launches;
@@ -737,6 +752,8 @@ int pgf_create(const pgf_mesh *mesh, const int64_t *id, int64_t ndof, const int3
     for (int c = 0; c < ncol; ++c) b.color_ptr[c + 1] += b.color_ptr[c];
     b.qp_base = qp;
     qp += b.ne * b.nqp;
+    b.e_base = h->n_elem;
+    h->n_elem += b.ne;
   }
   h->n_qp = qp;
   for (auto &b : h->blocks)
@@ -1004,6 +1021,8 @@ int pgf_destroy(pgf_handle h) {
   hipFree(h->d_pos);
   hipFree(h->d_stage);
   for (int t = 0; t < 3; ++t) hipFree(h->d_cv[t]);
+  hipFree(h->d_tf);
+  hipFree(h->d_tfbuf);
   if (h->gexec) hipGraphExecDestroy(h->gexec);
   hipFree(h->d_Ap);
   hipFree(h->d_Ai);
@@ -1116,6 +1135,17 @@ int pgf_commit_step(pgf_handle h) {
     pgf::launch_copy(h->d_cv[2], h->d_cv[1], 2 * h->n_cqp, h->stream);
     pgf::launch_copy(h->d_cv[1], h->d_cv[0], 2 * h->n_cqp, h->stream);
   }
+  if (h->tf) {  // update_np1_from_increments, then update_for_next_time_step (TL)
+    const int64_t NE = h->n_elem;
+    double *t = h->d_tf;
+    for (int v = 0; v < 2; ++v) {  // V, then P
+      double *np1 = t + 3 * v * NE, *n = np1 + NE, *nm1 = n + NE, *inc = t + (6 + 2 * v) * NE;
+      pgf::launch_axpy(np1, inc, 1.0, NE, h->stream);
+      pgf::launch_copy(nm1, n, NE, h->stream);
+      pgf::launch_copy(n, np1, NE, h->stream);
+    }
+    pgf::launch_fill(t + 6 * NE, 4 * NE, 0.0, h->stream);
+  }
   return hipStreamSynchronize(h->stream) == hipSuccess ? PGF_OK : PGF_EHIP;
 }
 
@@ -1132,7 +1162,124 @@ int pgf_reset_step(pgf_handle h) {
     pgf::launch_copy(h->d_gdot[0], h->d_gdot[1], 12 * NQ, h->stream);
   }
   if (h->n_cqp) pgf::launch_copy(h->d_cv[0], h->d_cv[1], 2 * h->n_cqp, h->stream);
+  if (h->tf) pgf::launch_fill(h->d_tf + 6 * h->n_elem, 4 * h->n_elem, 0.0, h->stream);  // tf.reset()
   return hipStreamSynchronize(h->stream) == hipSuccess ? PGF_OK : PGF_EHIP;
+}
+
+// ------------------------------------------------------------------ three-field (CM3F)
+int pgf_set_three_field(pgf_handle h, int32_t on) {
+  if (!h) return PGF_EINVAL;
+  if (hipStreamSynchronize(h->stream) != hipSuccess) return PGF_EHIP;
+  if (h->gexec) {  // a captured pass holds the old kernels
+    hipGraphExecDestroy(h->gexec);
+    h->gexec = nullptr;
+  }
+  if (!on) {
+    hipFree(h->d_tf);
+    hipFree(h->d_tfbuf);
+    h->d_tf = h->d_tfbuf = nullptr;
+    h->tf = false;
+    return PGF_OK;
+  }
+  for (auto &b : h->blocks)
+    if (b.model != PGF_MODEL_HYPER_ELASTICITY) return PGF_ENOTSUP;
+  if (h->tf) return PGF_OK;
+  const int64_t NE = h->n_elem;
+  int rc = pgf::dalloc(&h->d_tf, size_t(10 * NE));
+  if (!rc) rc = pgf::dalloc(&h->d_tfbuf, size_t(NE * pgf::kTfBuf));
+  if (rc) {
+    hipFree(h->d_tf);
+    h->d_tf = nullptr;
+    return rc;
+  }
+  // ThreeFieldVariables::construct: V = 1, P = 0, increments 0
+  pgf::launch_fill(h->d_tf, 3 * NE, 1.0, h->stream);
+  pgf::launch_fill(h->d_tf + 3 * NE, 7 * NE, 0.0, h->stream);
+  pgf::launch_fill(h->d_tfbuf, NE * pgf::kTfBuf, 0.0, h->stream);
+  h->tf = true;
+  return hipStreamSynchronize(h->stream) == hipSuccess ? PGF_OK : PGF_EHIP;
+}
+
+namespace {
+pgf::TfArgs tf_args(pgf_handle h, const Block &b) {
+  pgf::TfArgs t{};
+  t.X = h->d_X;
+  t.conn = b.d_conn;
+  t.edof = b.d_edof;
+  t.mat = b.d_mat;
+  t.mats = h->d_mats;
+  t.ne = b.ne;
+  t.buf = h->d_tfbuf + b.e_base * pgf::kTfBuf;
+  return t;
+}
+}  // namespace
+
+int pgf_tf_initial(pgf_handle h, const pgf_nodal_input *in_n, const pgf_nodal_input *in_nm1) {
+  if (!h || !h->tf || !in_n || !in_n->r) return PGF_EINVAL;
+  const int64_t NE = h->n_elem;
+  for (int s = 0; s < 2; ++s) {
+    const pgf_nodal_input *in = (s == 1 && in_nm1) ? in_nm1 : in_n;
+    if (!in->r) return PGF_EINVAL;
+    for (auto &b : h->blocks) {
+      pgf::TfArgs t = tf_args(h, b);
+      t.r = in->r;
+      t.d_r = in->d_r;
+      t.defl = in->defl;
+      t.defl_d = in->defl_d;
+      t.V = h->d_tf + (s == 0 ? 1 : 2) * NE + b.e_base;      // V(n) / V(n-1)
+      t.P = h->d_tf + (s == 0 ? 4 : 5) * NE + b.e_base;      // P(n) / P(n-1)
+      pgf::launch_tf_initial(b.nne, t, h->stream);
+    }
+  }
+  pgf::launch_copy(h->d_tf, h->d_tf + NE, NE, h->stream);           // V(n+1) = V(n)
+  pgf::launch_copy(h->d_tf + 3 * NE, h->d_tf + 4 * NE, NE, h->stream);  // P(n+1) = P(n)
+  pgf::launch_fill(h->d_tf + 6 * NE, 4 * NE, 0.0, h->stream);
+  return hipStreamSynchronize(h->stream) == hipSuccess ? PGF_OK : PGF_EHIP;
+}
+
+int pgf_tf_update(pgf_handle h, const double *du) {
+  if (!h || !h->tf || !du) return PGF_EINVAL;
+  const int64_t NE = h->n_elem;
+  for (auto &b : h->blocks) {
+    pgf::TfArgs t = tf_args(h, b);
+    t.du = du;
+    t.ddV = h->d_tf + 7 * NE + b.e_base;
+    t.ddP = h->d_tf + 9 * NE + b.e_base;
+    pgf::launch_tf_update(b.nne, t, h->stream);
+  }
+  return hipStreamSynchronize(h->stream) == hipSuccess ? PGF_OK : PGF_EHIP;
+}
+
+int pgf_tf_accept(pgf_handle h, double gamma) {
+  if (!h || !h->tf) return PGF_EINVAL;
+  const int64_t NE = h->n_elem;
+  pgf::launch_axpy(h->d_tf + 6 * NE, h->d_tf + 7 * NE, gamma, NE, h->stream);  // dV += gamma ddV
+  pgf::launch_axpy(h->d_tf + 8 * NE, h->d_tf + 9 * NE, gamma, NE, h->stream);  // dP += gamma ddP
+  return hipStreamSynchronize(h->stream) == hipSuccess ? PGF_OK : PGF_EHIP;
+}
+
+int pgf_tf_get(pgf_handle h, int32_t field, double *host_out) {
+  if (!h || !h->tf || !host_out || field < 0 || field > PGF_TF_DDP) return PGF_EINVAL;
+  std::vector<double> tmp(h->n_elem);
+  if (hipStreamSynchronize(h->stream) != hipSuccess) return PGF_EHIP;
+  if (hipMemcpy(tmp.data(), h->d_tf + int64_t(field) * h->n_elem, sizeof(double) * h->n_elem,
+                hipMemcpyDeviceToHost) != hipSuccess)
+    return PGF_EHIP;
+  for (auto &b : h->blocks)
+    for (int64_t k = 0; k < b.ne; ++k) host_out[b.elems[k]] = tmp[b.e_base + k];
+  return PGF_OK;
+}
+
+int pgf_tf_set(pgf_handle h, int32_t field, const double *host_in) {
+  if (!h || !h->tf || !host_in || field < 0 || field > PGF_TF_DDP) return PGF_EINVAL;
+  std::vector<double> tmp(h->n_elem);
+  for (auto &b : h->blocks)
+    for (int64_t k = 0; k < b.ne; ++k) tmp[b.e_base + k] = host_in[b.elems[k]];
+  if (hipStreamSynchronize(h->stream) != hipSuccess) return PGF_EHIP;
+  if (hipMemcpy(h->d_tf + int64_t(field) * h->n_elem, tmp.data(), sizeof(double) * h->n_elem,
+                hipMemcpyHostToDevice) != hipSuccess)
+    return PGF_EHIP;
+  return PGF_OK;
 }
 
 int pgf_state_ncomp(pgf_handle h, int32_t field, int32_t *ncomp) {

@@ -64,6 +64,26 @@ struct ElemArgs {
   const int64_t *soff;      // [ne*nne] staging offsets of the element's node blocks
   double *stage;
   int32_t *err;             // failure counter
+  // three-field (CM3F) blocks, per block element (block-local index): theta = V(n+1) + dV and
+  // P = P(n+1) + dP, and the element data of the last residual pass for the update (kTfBuf
+  // doubles: kup[3nne], fp, ft, Ktt, V).  Null on a displacement-only handle.
+  const double *tf_V, *tf_dV, *tf_P, *tf_dP;
+  double *tf_buf;
+};
+constexpr int kTfBuf = 34;
+
+// Element update and initial conditions of the three-field formulation, one thread per
+// element of a block (block-local arrays).
+struct TfArgs {
+  const double *X;
+  const int32_t *conn, *edof, *mat;
+  const DevMaterial *mats;
+  int64_t ne;
+  const double *buf;         // [ne][kTfBuf] from the last residual pass
+  const double *du;          // free-DOF increments (the linear solve's solution)
+  double *ddV, *ddP;         // out: the element's volume and pressure increments
+  const double *r, *d_r, *defl, *defl_d;  // initial conditions: nodal input
+  double *V, *P;             // out: volume-averaged J and kappa U'(J)
 };
 
 // Mode bits of the element kernel.
@@ -203,6 +223,9 @@ void launch_gather(int mode, const GatherArgs &g, hipStream_t s);
 int bicgstab(hipStream_t s, const int32_t *Ap, const int32_t *Ai, const double *vals, const double *b, double *x,
              int64_t n, double tol, int maxit, int *iters, double *relres, double *work);
 
+void launch_tf_update(int nne, const TfArgs &t, hipStream_t s);
+void launch_tf_initial(int nne, const TfArgs &t, hipStream_t s);
+void launch_axpy(double *y, const double *x, double g, int64_t n, hipStream_t s);  // y += g x
 void launch_fill(double *p, int64_t n, double v, hipStream_t s);
 void launch_fill_i32(int32_t *p, int64_t n, int32_t v, hipStream_t s);
 void launch_copy(double *dst, const double *src, int64_t n, hipStream_t s);

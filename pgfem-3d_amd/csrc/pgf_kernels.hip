@@ -85,11 +85,14 @@ __device__ __forceinline__ void j2_c_row(const J2Tan &T, int I, double w, double
 // In: Fs, its inverse and J; J2 state (Cp^-1 at n, ep_n).  Out: tau (Voigt), the pieces of
 // the tangent c (J2Tan, when WITH_C) and, when integ, the n+1 state (outputs by reference, so
 // they stay in registers).  HE is the J2 elastic branch with Cp^-1 = I.
-template <bool WITH_C>
+// TFV (three-field): the volumetric Kirchhoff stress is J P with the element pressure Ptf in
+// place of J kappa U'(J), and its tangent J P (1 x 1 - 2 I) (CM_ThreeField::compute_Ru/Kuu,
+// src/constitutive_model_3f.cc:190-286: P tJr Grad_du : tFr^-T and the pJrJn term).
+template <bool WITH_C, bool TFV = false>
 __device__ __forceinline__ void j2_point(const DevMaterial &m, bool plastic, const double *Fs,
                                          const double *Fi, double J, const double *Cn,
                                          double ep_n, bool integ, double (&Cnp1)[6], double &ep1,
-                                         double &gam, double *tau, J2Tan &tg) {
+                                         double &gam, double *tau, J2Tan &tg, double Ptf = 0.0) {
   // b_bar trial = J^-2/3 Fs Cn Fs^T (Voigt)
   double T[9];
 #pragma unroll
@@ -159,13 +162,14 @@ __device__ __forceinline__ void j2_point(const DevMaterial &m, bool plastic, con
     ep1 = ep_n + sqrt(2.0 / 3.0) * dg;
     gam = dg;
   }
-  double dU, d2U;
-  vol_pot(m.volPotFlag, J, dU, d2U);
-  const double Jp = J * m.kappa * dU;
+  double dU = 0.0, d2U = 0.0;
+  if constexpr (!TFV) vol_pot(m.volPotFlag, J, dU, d2U);
+  const double Jp = TFV ? J * Ptf : J * m.kappa * dU;
 #pragma unroll
   for (int I = 0; I < 6; ++I) tau[I] = s[I] + (I < 3 ? Jp : 0.0);
   if constexpr (WITH_C) {
-    const double cv1 = m.kappa * (J * J * d2U + J * dU), cv2 = -2.0 * m.kappa * J * dU;
+    const double cv1 = TFV ? Jp : m.kappa * (J * J * d2U + J * dU);
+    const double cv2 = TFV ? -2.0 * Jp : -2.0 * m.kappa * J * dU;
     double beta1 = 0.0, beta3 = 0.0, beta4 = 0.0, dn2[6] = {0, 0, 0, 0, 0, 0};
     if (yield) {
       // 1 / beta0 with beta0 = 1 + hp / (3 mubar)
@@ -211,7 +215,10 @@ constexpr int stage_round(int epb, int nde, int room) {
   return re;
 }
 
-template <int NNE, int MODE, bool STG>
+// TF: the CM3F three-field element of the hyperelastic model (one volume and one pressure
+// variable per element, condensed): F(n+1) is always Fr (the model stores (theta/J)^1/3 Fr),
+// tau and c take the element pressure, and the rows get the condensation terms.
+template <int NNE, int MODE, bool STG, bool TF>
 __global__ __launch_bounds__(256, STG ? 4 : 1) void elem_kernel(ElemArgs a, int64_t e_begin, int64_t e_end) {
   constexpr int NQP = nqp_of(NNE);
   constexpr int NDE = NNE * 3;
@@ -224,7 +231,7 @@ __global__ __launch_bounds__(256, STG ? 4 : 1) void elem_kernel(ElemArgs a, int6
   static_assert(!DFL || (TAN && !RES && !INTEG), "MODE_DEFL_LOAD goes with MODE_TANGENT alone");
   constexpr bool WK = TAN && !DFL;  // K rows are written
   constexpr bool WR = RES || DFL;   // the residual entry is written
-  constexpr bool GEN = !INTEG;  // Fr may differ from the stored F(n+1)
+  constexpr bool GEN = !INTEG && !TF;  // Fr may differ from the stored F(n+1)
   constexpr int LD = NDE + 1;   // staging row: K row, then the residual entry
   constexpr int kH = EPB * NQP * NDE, kC = EPB * NQP * (TAN ? 36 : 1);
   constexpr int RE = stage_round(EPB, NDE, kH + kC);
@@ -241,6 +248,7 @@ __global__ __launch_bounds__(256, STG ? 4 : 1) void elem_kernel(ElemArgs a, int6
   __shared__ double s_tau[EPB][NQP][6];
   __shared__ double s_g[EPB][NQP][GEN ? 9 : 1];
   __shared__ double s_dl[EPB][DFL ? NDE : 1];  // prescribed increments of the element DOFs
+  __shared__ double s_tf[EPB][NQP][TF ? 2 : 1];  // three-field: detJ w J, detJ w
 
   const int slot = threadIdx.x / TPE;
   const int lane = threadIdx.x % TPE;
@@ -261,11 +269,15 @@ __global__ __launch_bounds__(256, STG ? 4 : 1) void elem_kernel(ElemArgs a, int6
   const int p1slot = p1live ? p1 / (NQP * LPQ) : 0, p1ip = p1live ? (p1 / LPQ) % NQP : 0;
   int64_t p1e = 0;
   DevMaterial pm{};
-  double pCn[6] = {1, 1, 1, 0, 0, 0}, pep = 0.0;
+  double pCn[6] = {1, 1, 1, 0, 0, 0}, pep = 0.0, pth = 1.0, ppr = 0.0;
   if (p1live) {
     const int64_t ie1 = e_begin + blk * EPB + p1slot;
     p1e = a.elist ? int64_t(a.elist[ie1]) : ie1;
     pm = a.mats[a.mat[p1e]];
+    if constexpr (TF) {
+      pth = a.tf_V[p1e] + a.tf_dV[p1e];
+      ppr = a.tf_P[p1e] + a.tf_dP[p1e];
+    }
     if (pm.model == PGF_MODEL_J2_PLASTICITY) {
       const int64_t q = a.qp_base + p1e * NQP + p1ip;
 #pragma unroll
@@ -371,11 +383,14 @@ __global__ __launch_bounds__(256, STG ? 4 : 1) void elem_kernel(ElemArgs a, int6
     const int64_t q = a.qp_base + e * NQP + ip;  // e is the block-local element index
     const int64_t NQ = a.n_qp;
     double Fs[9];
-    if (INTEG) {
+    if (INTEG || TF) {
+      // three-field: the model keeps (theta/J)^1/3 Fr (run_integration_algorithm's tf_factor,
+      // src/constitutive_model.cc:815, :1115); the element works with Fr
+      const double fac = TF ? cbrt(pth / det3(Fr)) : 1.0;
 #pragma unroll
       for (int i = 0; i < 9; ++i) {
         Fs[i] = Fr[i];
-        if (i % LPQ == sub) a.F_np1[i * NQ + q] = Fr[i];
+        if (INTEG && i % LPQ == sub) a.F_np1[i * NQ + q] = TF ? fac * Fr[i] : Fr[i];
       }
     } else {
 #pragma unroll
@@ -397,9 +412,15 @@ __global__ __launch_bounds__(256, STG ? 4 : 1) void elem_kernel(ElemArgs a, int6
     for (int I = 0; I < 6; ++I) Cn[I] = pCn[I];
     double Cnp1[6], ep1, gm, tau[6];
     J2Tan T;
-    j2_point<TAN>(m, is_j2, Fs, Fi, ok ? J : 1.0, Cn, ep_n, INTEG && is_j2, Cnp1, ep1, gm, tau, T);
+    j2_point<TAN, TF>(m, is_j2, Fs, Fi, ok ? J : 1.0, Cn, ep_n, INTEG && is_j2, Cnp1, ep1, gm, tau, T, ppr);
     // tau and c go to LDS weighted by the QP's detJ w (zero for a failed QP)
     const double wq = ok ? J0 * wt : 0.0;
+    if constexpr (TF) {
+      if (sub == 0) {
+        s_tf[slot][ip][0] = wq * J;
+        s_tf[slot][ip][1] = wq;
+      }
+    }
 #pragma unroll
     for (int I = 0; I < 6; ++I)
       if (I % LPQ == sub) {
@@ -510,6 +531,50 @@ __global__ __launch_bounds__(256, STG ? 4 : 1) void elem_kernel(ElemArgs a, int6
           K[w * 3 + 0] += M[0] * h0 + M[1] * h1 + M[2] * h2;
           K[w * 3 + 1] += M[3] * h0 + M[4] * h1 + M[5] * h2;
           K[w * 3 + 2] += M[6] * h0 + M[7] * h1 + M[8] * h2;
+        }
+      }
+    }
+    if constexpr (TF) {
+      // Condensation of the element's volume and pressure (condense_K_3F_to_1F /
+      // condense_F_3F_to_1F, src/condense.cc:304-385, with Pno = Vno = 1): Kpt = Ktp = -V,
+      // Kup_(a,b) = sum detJ w J h_a[b] = Kpu, Kut = Ktu = 0 for this model, so
+      //   K += Ktt / V^2 Kup Kup^T,   f += Kup (Ktt fp / V^2 + ft / V)
+      // with fp = sum detJ w (J - theta), ft = V (kappa U'(theta) - P), Ktt = V kappa U''(theta)
+      // (compute_Rp/Rt/Ktt/Ktp, src/constitutive_model_3f.cc:219-246, :418-466).
+      const DevMaterial me = a.mats[a.mat[e]];
+      const double th = a.tf_V[e] + a.tf_dV[e], pr = a.tf_P[e] + a.tf_dP[e];
+      double V = 0.0, Jw = 0.0, kup = 0.0;
+#pragma unroll
+      for (int ip = 0; ip < NQP; ++ip) {
+        Jw += s_tf[slot][ip][0];
+        V += s_tf[slot][ip][1];
+        kup += s_tf[slot][ip][0] * s_h[slot][ip][lane];
+      }
+      double dU, d2U;
+      vol_pot(me.volPotFlag, th, dU, d2U);
+      const double rV = V > 0.0 ? 1.0 / V : 0.0;  // V = 0 only when every QP failed (counted)
+      const double Ktt = V * me.kappa * d2U, ft = V * (me.kappa * dU - pr), fp = Jw - th * V;
+      if constexpr (RES) {
+        fe += kup * (Ktt * fp * rV * rV + ft * rV);
+        if constexpr (INTEG) {  // the element data of the update (IntegrateThreeFieldUpdate)
+          double *tb = a.tf_buf + e * kTfBuf;
+          tb[lane] = kup;
+          if (lane == 0) {
+            tb[NDE] = fp;
+            tb[NDE + 1] = ft;
+            tb[NDE + 2] = Ktt;
+            tb[NDE + 3] = V;
+          }
+        }
+      }
+      if constexpr (TAN) {
+        const double al = Ktt * rV * rV * kup;
+#pragma unroll
+        for (int c = 0; c < NDE; ++c) {
+          double kc = 0.0;
+#pragma unroll
+          for (int ip = 0; ip < NQP; ++ip) kc += s_tf[slot][ip][0] * s_h[slot][ip][c];
+          K[c] += al * kc;
         }
       }
     }
@@ -714,7 +779,102 @@ __global__ void copy_kernel(double *dst, const double *src, int64_t n) {
     dst[i] = src[i];
 }
 
-template <int NNE, bool STG>
+// Element update of the three-field formulation (constitutive_model_update_NR without inertia,
+// src/constitutive_model.cc:3916-3999; compute_d_theta_dP, src/constitutive_model_3f.cc:494-549),
+// one thread per element, from the element data of the last residual pass and the
+// displacement increment du (prescribed DOFs zero: def_elem TYPE 2, src/utils.cc:1365-1389):
+//   d_theta = -(Kpu du + fp) / Kpt = (Kup . du + fp) / V,   dP = -(ft + Ktt d_theta) / Ktp.
+template <int NNE>
+__global__ __launch_bounds__(256) void tf_update_kernel(TfArgs t) {
+  constexpr int NDE = 3 * NNE;
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= t.ne) return;
+  const double *b = t.buf + e * kTfBuf;
+  double s = 0.0;
+#pragma unroll
+  for (int k = 0; k < NDE; ++k) {
+    const int32_t d = t.edof[e * NDE + k];
+    if (d > 0) s += b[k] * t.du[d - 1];
+  }
+  const double V = b[NDE + 3];
+  const double rV = V > 0.0 ? 1.0 / V : 0.0;
+  const double dth = (s + b[NDE]) * rV;
+  t.ddV[e] = dth;
+  t.ddP[e] = (b[NDE + 1] + b[NDE + 2] * dth) * rV;
+}
+
+// Initial volume and pressure of each element (compute_3f_initial_conditions,
+// src/three_field_element.cc:1328-1386): the volume averages of J and kappa U'(J) at the
+// nodal input, one thread per element.
+template <int NNE>
+__global__ __launch_bounds__(256) void tf_initial_kernel(TfArgs t) {
+  constexpr int NQP = nqp_of(NNE), NDE = 3 * NNE;
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= t.ne) return;
+  double x[NDE], u[NDE];
+#pragma unroll
+  for (int k = 0; k < NDE; ++k) {
+    const int32_t node = t.conn[e * NNE + k / 3];
+    x[k] = t.X[int64_t(node) * 3 + k % 3];
+    const int32_t d = t.edof[e * NDE + k];
+    u[k] = d > 0 ? t.r[d - 1] + (t.d_r ? t.d_r[d - 1] : 0.0)
+                 : (d < 0 ? (t.defl ? t.defl[-d - 1] : 0.0) + (t.defl_d ? t.defl_d[-d - 1] : 0.0) : 0.0);
+  }
+  const DevMaterial m = t.mats[t.mat[e]];
+  double vol = 0.0, Jv = 0.0, Up = 0.0;
+  for (int ip = 0; ip < NQP; ++ip) {
+    double k, ee, z, wt, dN[NDE], dX[9], cof[9], F[9];
+    int_point(NNE, ip, k, ee, z, wt);
+    shape_dN<NNE>(k, ee, z, dN);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) dX[i] = 0.0;
+#pragma unroll
+    for (int n = 0; n < NNE; ++n)
+#pragma unroll
+      for (int ib = 0; ib < 3; ++ib)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) dX[ib * 3 + j] += dN[n * 3 + j] * x[n * 3 + ib];
+    const double J0 = det3(dX);
+    cof[0] = dX[4] * dX[8] - dX[5] * dX[7];
+    cof[1] = dX[2] * dX[7] - dX[1] * dX[8];
+    cof[2] = dX[1] * dX[5] - dX[2] * dX[4];
+    cof[3] = dX[5] * dX[6] - dX[3] * dX[8];
+    cof[4] = dX[0] * dX[8] - dX[2] * dX[6];
+    cof[5] = dX[2] * dX[3] - dX[0] * dX[5];
+    cof[6] = dX[3] * dX[7] - dX[4] * dX[6];
+    cof[7] = dX[1] * dX[6] - dX[0] * dX[7];
+    cof[8] = dX[0] * dX[4] - dX[1] * dX[3];
+    const double rJ0 = 1.0 / J0;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) F[i] = (i % 4 == 0) ? 1.0 : 0.0;
+#pragma unroll
+    for (int n = 0; n < NNE; ++n) {
+      double g[3];
+#pragma unroll
+      for (int ib = 0; ib < 3; ++ib)
+        g[ib] = rJ0 * (dN[n * 3 + 0] * cof[0 * 3 + ib] + dN[n * 3 + 1] * cof[1 * 3 + ib] + dN[n * 3 + 2] * cof[2 * 3 + ib]);
+#pragma unroll
+      for (int ib = 0; ib < 3; ++ib)
+#pragma unroll
+        for (int ic = 0; ic < 3; ++ic) F[ib * 3 + ic] += u[n * 3 + ib] * g[ic];
+    }
+    double dU, d2U;
+    vol_pot(m.volPotFlag, det3(F), dU, d2U);
+    const double w = J0 * wt;
+    vol += w;
+    Jv += w * det3(F);
+    Up += w * m.kappa * dU;
+  }
+  t.V[e] = Jv / vol;
+  t.P[e] = Up / vol;
+}
+
+__global__ void axpy_kernel(double *y, const double *x, double g, int64_t n) {
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+    y[i] += g * x[i];
+}
+
+template <int NNE, bool STG, bool TF>
 void launch_nne(int mode, const ElemArgs &a, int64_t e_begin, int64_t e_end, hipStream_t s) {
   constexpr int TPE = (NNE == 4) ? 16 : 32;
   constexpr int EPB = 256 / TPE;
@@ -723,19 +883,19 @@ void launch_nne(int mode, const ElemArgs &a, int64_t e_begin, int64_t e_end, hip
   const unsigned grid = unsigned((n + EPB - 1) / EPB);
   switch (mode) {
     case MODE_INTEGRATE | MODE_RESIDUAL:
-      elem_kernel<NNE, MODE_INTEGRATE | MODE_RESIDUAL, STG><<<grid, 256, 0, s>>>(a, e_begin, e_end);
+      elem_kernel<NNE, MODE_INTEGRATE | MODE_RESIDUAL, STG, TF><<<grid, 256, 0, s>>>(a, e_begin, e_end);
       break;
     case MODE_RESIDUAL:
-      elem_kernel<NNE, MODE_RESIDUAL, STG><<<grid, 256, 0, s>>>(a, e_begin, e_end);
+      elem_kernel<NNE, MODE_RESIDUAL, STG, TF><<<grid, 256, 0, s>>>(a, e_begin, e_end);
       break;
     case MODE_TANGENT:
-      elem_kernel<NNE, MODE_TANGENT, STG><<<grid, 256, 0, s>>>(a, e_begin, e_end);
+      elem_kernel<NNE, MODE_TANGENT, STG, TF><<<grid, 256, 0, s>>>(a, e_begin, e_end);
       break;
     case MODE_INTEGRATE | MODE_RESIDUAL | MODE_TANGENT:
-      elem_kernel<NNE, MODE_INTEGRATE | MODE_RESIDUAL | MODE_TANGENT, STG><<<grid, 256, 0, s>>>(a, e_begin, e_end);
+      elem_kernel<NNE, MODE_INTEGRATE | MODE_RESIDUAL | MODE_TANGENT, STG, TF><<<grid, 256, 0, s>>>(a, e_begin, e_end);
       break;
     case MODE_TANGENT | MODE_DEFL_LOAD:
-      elem_kernel<NNE, MODE_TANGENT | MODE_DEFL_LOAD, STG><<<grid, 256, 0, s>>>(a, e_begin, e_end);
+      elem_kernel<NNE, MODE_TANGENT | MODE_DEFL_LOAD, STG, TF><<<grid, 256, 0, s>>>(a, e_begin, e_end);
       break;
     default:
       break;
@@ -747,9 +907,15 @@ void launch_nne(int mode, const ElemArgs &a, int64_t e_begin, int64_t e_end, hip
 void launch_elem(int nne, int mode, const ElemArgs &a, int64_t e_begin, int64_t e_end,
                  hipStream_t s) {
   const bool stg = a.stage != nullptr;
-  if (nne == 4) (stg ? launch_nne<4, true> : launch_nne<4, false>)(mode, a, e_begin, e_end, s);
-  else if (nne == 10) (stg ? launch_nne<10, true> : launch_nne<10, false>)(mode, a, e_begin, e_end, s);
-  else if (nne == 8) (stg ? launch_nne<8, true> : launch_nne<8, false>)(mode, a, e_begin, e_end, s);
+  if (a.tf_V) {
+    if (nne == 4) (stg ? launch_nne<4, true, true> : launch_nne<4, false, true>)(mode, a, e_begin, e_end, s);
+    else if (nne == 10) (stg ? launch_nne<10, true, true> : launch_nne<10, false, true>)(mode, a, e_begin, e_end, s);
+    else if (nne == 8) (stg ? launch_nne<8, true, true> : launch_nne<8, false, true>)(mode, a, e_begin, e_end, s);
+    return;
+  }
+  if (nne == 4) (stg ? launch_nne<4, true, false> : launch_nne<4, false, false>)(mode, a, e_begin, e_end, s);
+  else if (nne == 10) (stg ? launch_nne<10, true, false> : launch_nne<10, false, false>)(mode, a, e_begin, e_end, s);
+  else if (nne == 8) (stg ? launch_nne<8, true, false> : launch_nne<8, false, false>)(mode, a, e_begin, e_end, s);
 }
 
 int gather_waves(int32_t maxlen) {
@@ -769,6 +935,28 @@ void launch_gather(int mode, const GatherArgs &g, hipStream_t s) {
   if (res && tan) gather_kernel<true, true><<<grid, 64 * wpb, lds, s>>>(g);
   else if (res) gather_kernel<true, false><<<grid, 64 * wpb, lds, s>>>(g);
   else if (tan) gather_kernel<false, true><<<grid, 64 * wpb, lds, s>>>(g);
+}
+
+void launch_tf_update(int nne, const TfArgs &t, hipStream_t s) {
+  if (t.ne <= 0) return;
+  const unsigned grid = unsigned((t.ne + 255) / 256);
+  if (nne == 4) tf_update_kernel<4><<<grid, 256, 0, s>>>(t);
+  else if (nne == 10) tf_update_kernel<10><<<grid, 256, 0, s>>>(t);
+  else if (nne == 8) tf_update_kernel<8><<<grid, 256, 0, s>>>(t);
+}
+
+void launch_tf_initial(int nne, const TfArgs &t, hipStream_t s) {
+  if (t.ne <= 0) return;
+  const unsigned grid = unsigned((t.ne + 255) / 256);
+  if (nne == 4) tf_initial_kernel<4><<<grid, 256, 0, s>>>(t);
+  else if (nne == 10) tf_initial_kernel<10><<<grid, 256, 0, s>>>(t);
+  else if (nne == 8) tf_initial_kernel<8><<<grid, 256, 0, s>>>(t);
+}
+
+void launch_axpy(double *y, const double *x, double g, int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  const int64_t blocks = std::min<int64_t>((n + 255) / 256, 8192);
+  axpy_kernel<<<unsigned(blocks), 256, 0, s>>>(y, x, g, n);
 }
 
 void launch_fill(double *p, int64_t n, double v, hipStream_t s) {

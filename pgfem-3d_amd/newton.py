@@ -141,6 +141,7 @@ class NewtonDriver:
 
         res = StepResult(False, 0)
         it0 = 0
+        tf = getattr(self.h, "three_field", False)
         if defl_d is not None:
             # predictor: K and f_defl at (u_n, defl_n), K du = -f_defl, no line search
             info = self.h.tangent(self.vals, r, d_r=d_r, defl=defl, dt=dt)
@@ -187,9 +188,13 @@ class NewtonDriver:
                 return res
             du, lit = self._solve(R)
             res.solver_iters.append((lit, self.last_relres))
+            if tf:  # the elements' volume and pressure increments that go with du
+                self.h.tf_update(du)
             gam, prev = 1.0, None
             while True:
                 torch.add(r, du, alpha=gam, out=self.rt)
+                if tf:
+                    self.h.tf_accept(gam)
                 info = run(self.rt)
                 if info or it == 0 or not self.line_search:
                     break
@@ -198,6 +203,8 @@ class NewtonDriver:
                     return res
                 if phi <= 1.0 - 2e-4 * gam:  # sufficient decrease (LINE.cc:96)
                     break
+                if tf:
+                    self.h.tf_accept(-gam)
                 gam, prev = backtrack_gamma(gam, phi, prev), (gam, phi)
                 if gam < 0.1 * self.tol:  # the line search failed (LINE.cc:126-127)
                     if nor < 5.0 * self.tol:  # soft convergence: keep u (Newton_Raphson.cc:1246)

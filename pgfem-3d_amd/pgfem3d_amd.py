@@ -30,6 +30,7 @@ STATE_PLASTIC_NP1, STATE_PLASTIC_N, STATE_PLASTIC_NM1 = 3, 4, 5
 STATE_SCALAR_NP1, STATE_SCALAR_N, STATE_SCALAR_NM1 = 6, 7, 8
 STATE_TAU, STATE_TAU_N, STATE_GDOT, STATE_GDOT_N, STATE_ITERS = 9, 10, 11, 12, 13
 STATE_COHESIVE_NP1, STATE_COHESIVE_N = 14, 15
+TF_V_NP1, TF_V_N, TF_V_NM1, TF_P_NP1, TF_P_N, TF_P_NM1, TF_DV, TF_DDV, TF_DP, TF_DDP = range(10)
 
 ERRORS = {-1: "PGF_EINVAL", -2: "PGF_ENOMEM", -3: "PGF_EHIP", -4: "PGF_ENOTSUP", -5: "PGF_ENODEV"}
 
@@ -123,6 +124,12 @@ def lib() -> C.CDLL:
         "pgf_set_assembly": [P, C.c_int32],
             "pgf_autotune": [P, P, D, P],
             "pgf_set_surrogate": [P, P, P, P, P],
+            "pgf_set_three_field": [P, I32],
+            "pgf_tf_initial": [P, P, P],
+            "pgf_tf_update": [P, P],
+            "pgf_tf_accept": [P, D],
+            "pgf_tf_get": [P, I32, P],
+            "pgf_tf_set": [P, I32, P],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -143,6 +150,7 @@ def exported_symbols():
         "pgf_get_cohesive_sizes", "pgf_mlp_create", "pgf_mlp_destroy", "pgf_mlp_param_count",
         "pgf_mlp_get_params", "pgf_mlp_set_params", "pgf_mlp_get_grads", "pgf_mlp_forward", "pgf_mlp_backward",
         "pgf_mlp_adam_step", "pgf_cp_log_samples", "pgf_set_surrogate", "pgf_solve", "pgf_set_graph", "pgf_set_assembly",
+        "pgf_set_three_field", "pgf_tf_initial", "pgf_tf_update", "pgf_tf_accept", "pgf_tf_get", "pgf_tf_set",
         "pgf_autotune",
     ]
 
@@ -280,6 +288,8 @@ class Handle:
         _check(self._lib.pgf_get_cohesive_sizes(h, C.byref(ncq), C.byref(ncc)), "pgf_get_cohesive_sizes")
         self.n_cqp, self.n_ccolors = ncq.value, ncc.value
         self.n_prescribed = int(max(0, -int(np.min(ids)))) if ids.size else 0
+        self.n_elem = mesh.n_elem
+        self.three_field = False
 
     def close(self):
         if self.h:
@@ -367,6 +377,38 @@ class Handle:
 
     def commit_step(self):
         _check(self._lib.pgf_commit_step(self.h), "pgf_commit_step")
+
+    # -- three-field (CM3F) analysis of hyperelastic meshes
+    def set_three_field(self, on=True):
+        """Switch the CM3F mode (per-element volume and pressure, condensed) on or off."""
+        _check(self._lib.pgf_set_three_field(self.h, int(bool(on))), "pgf_set_three_field")
+        self.three_field = bool(on)
+
+    def tf_initial(self, r, defl=None, r_nm1=None, defl_nm1=None):
+        """compute_3f_initial_conditions at (r, defl) for n and n+1, (r_nm1, defl_nm1) for n-1."""
+        a = self._input(r, None, defl, None)
+        b = self._input(r_nm1, None, defl_nm1, None) if r_nm1 is not None else None
+        _check(self._lib.pgf_tf_initial(self.h, C.byref(a), C.byref(b) if b is not None else None),
+               "pgf_tf_initial")
+
+    def tf_update(self, du):
+        """Element volume and pressure increments for the Newton increment du (device [ndof])."""
+        self._check_vec(du, self.ndof, "du")
+        _check(self._lib.pgf_tf_update(self.h, _ptr(du)), "pgf_tf_update")
+
+    def tf_accept(self, gamma=1.0):
+        _check(self._lib.pgf_tf_accept(self.h, float(gamma)), "pgf_tf_accept")
+
+    def tf_get(self, field: int) -> np.ndarray:
+        out = np.zeros(self.n_elem)
+        _check(self._lib.pgf_tf_get(self.h, field, _ptr(out)), "pgf_tf_get")
+        return out
+
+    def tf_set(self, field: int, values):
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        if v.shape != (self.n_elem,):
+            raise PGFError(f"tf_set: expected {self.n_elem} values")
+        _check(self._lib.pgf_tf_set(self.h, field, _ptr(v)), "pgf_tf_set")
 
     def reset_step(self):
         _check(self._lib.pgf_reset_step(self.h), "pgf_reset_step")
