@@ -314,10 +314,12 @@ def time_step_info_file(opath, step):
     return os.path.join(opath, "restart", f"time_step_info_{step:06d}.res")
 
 
-def read_restart(path, nn, ndofn, kinds, nqp=None):
+def read_restart(path, nn, ndofn, kinds, nqp=None, three_field=False):
     """(u_nm1 [nn, ndofn], u_n [nn, ndofn], [[record per QP] per element]); kinds[e] is "cp"
     or "he", the model of element e; nqp[e], when given, its QP count, which the file must
-    hold (read_restart_constitutive_model checks it, src/restart.cc:300-309)."""
+    hold (read_restart_constitutive_model checks it, src/restart.cc:300-309).  three_field
+    (a CM3F run): the per-element volume and pressure records follow (src/restart.cc:324-340,
+    one of each per element), returned fourth as (V_n, V_nm1, P_n, P_nm1) arrays [ne]."""
     t = _Tokens(path)
     u_nm1 = np.zeros((nn, ndofn))
     u_n = np.zeros((nn, ndofn))
@@ -340,7 +342,13 @@ def read_restart(path, nn, ndofn, kinds, nqp=None):
             else:
                 raise ValueError(f"no restart record of model kind {kind!r}")
         elems.append(recs)
-    return u_nm1, u_n, elems
+    if not three_field:
+        return u_nm1, u_n, elems
+    tf = np.zeros((4, len(kinds)))
+    for e in range(len(kinds)):
+        tf[0, e], tf[1, e] = t.float(), t.float()
+        tf[2, e], tf[3, e] = t.float(), t.float()
+    return u_nm1, u_n, elems, (tf[0], tf[1], tf[2], tf[3])
 
 
 def read_restart_cp(path, nn, ndofn, ne):
@@ -352,10 +360,11 @@ def _row(v, trail):
     return " ".join(f"{x:.17e}" for x in v) + (" \n" if trail else "\n")
 
 
-def write_restart(path, u_nm1, u_n, elems):
+def write_restart(path, u_nm1, u_n, elems, tf=None):
     """The same layout, %.17e as the reference writes it (tensors one row each with no
     trailing blank, slip values with one): a file read by read_restart is written back
-    byte for byte."""
+    byte for byte.  tf = (V_n, V_nm1, P_n, P_nm1) per element appends the CM3F records
+    (write_restart_constitutive_model, src/restart.cc:241-254)."""
     os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
     with open(path, "w") as fh:
         for a in range(u_n.shape[0]):
@@ -374,6 +383,9 @@ def write_restart(path, u_nm1, u_n, elems):
                 else:
                     fh.write(_row(r.F_n, False))
                     fh.write(_row(r.F_nm1, False))
+        if tf is not None:
+            for e in range(len(elems)):
+                fh.write(f"{tf[0][e]:.17e} {tf[1][e]:.17e} \n{tf[2][e]:.17e} {tf[3][e]:.17e} \n")
 
 
 write_restart_cp = write_restart

@@ -8,6 +8,9 @@ handle: the records set the state at n and n-1 and, as the models' read_restart 
 (CP_PARAM::read_restart, src/plasticity_model.cc:913-942; HE_PARAM::read_restart,
 src/plasticity_model_none.cc:314-327), the state at n+1 equal to n.
 
+With the handle in the three-field (CM3F) mode, each file also holds the elements' volume and
+pressure at n and n-1 (src/restart.cc:241-254, read back at :324-340 with n+1 = n).
+
 Records exist for crystal plasticity (F, pF at n and n-1; tau and gamma_dot at n; g and lambda
 at n and n-1) and for the hyperelastic model (F at n and n-1).  The reference's J2 record
 (CM_J2P_PARAM::write_restart, src/cm_j2_plasticity.cc:784-808) holds the un-vendored GCM
@@ -92,10 +95,11 @@ def save(opath, ofname, step, deck, h, u_nm1, u_n, t3, norm, physics="Mechanical
     """Write the step's restart files of every partition of deck (u_nm1, u_n: nodal [n_nodes, 3]
     on the merged mesh) and its time-step info (t3 = t(n-1), t(n), t(n+1); norm = NORM)."""
     elems = records(h, deck.elem_ptr)
+    tf = [h.tf_get(f) for f in (P.TF_V_N, P.TF_V_NM1, P.TF_P_N, P.TF_P_NM1)] if h.three_field else None
     for p, part in enumerate(deck.parts):
         nodes, els = deck.node_of[p], deck.elem_of[p]
         R.write_restart(R.restart_file(opath, ofname, p, step, physics), u_nm1[nodes], u_n[nodes],
-                        [elems[e] for e in els])
+                        [elems[e] for e in els], None if tf is None else [a[els] for a in tf])
     R.write_time_step_info(R.time_step_info_file(opath, step), t3, [norm], [t3[1]])
 
 
@@ -108,15 +112,26 @@ def load(opath, ofname, step, deck, h, physics="Mechanical"):
     u_n = np.zeros((n_nodes, 3))
     ne = len(deck.elem_ptr) - 1
     elems = [None] * ne
+    tf = np.zeros((4, ne))
     _, nqp = _qp_ptr(deck.elem_ptr)
     for p, part in enumerate(deck.parts):
         els = deck.elem_of[p]
-        a, b, recs = R.read_restart(R.restart_file(opath, ofname, p, step, physics), part.nn, 3,
-                                    [kind] * part.ne, nqp[els])
+        out = R.read_restart(R.restart_file(opath, ofname, p, step, physics), part.nn, 3,
+                             [kind] * part.ne, nqp[els], three_field=h.three_field)
+        a, b, recs = out[:3]
         u_nm1[deck.node_of[p]] = a
         u_n[deck.node_of[p]] = b
         for k, e in enumerate(els):
             elems[e] = recs[k]
+        if h.three_field:
+            for i in range(4):
+                tf[i, els] = out[3][i]
     load_records(h, deck.elem_ptr, elems)
+    if h.three_field:  # n+1 = n, increments zero
+        zero = np.zeros(ne)
+        for f, v in ((P.TF_V_NP1, tf[0]), (P.TF_V_N, tf[0]), (P.TF_V_NM1, tf[1]), (P.TF_P_NP1, tf[2]),
+                     (P.TF_P_N, tf[2]), (P.TF_P_NM1, tf[3]), (P.TF_DV, zero), (P.TF_DDV, zero), (P.TF_DP, zero),
+                     (P.TF_DDP, zero)):
+            h.tf_set(f, v)
     t3, norms, _ = R.read_time_step_info(R.time_step_info_file(opath, step))
     return u_nm1, u_n, t3, norms[0]

@@ -183,3 +183,58 @@ def test_newton_load_steps(device, etype, n):
             assert d1.load_step(r1, _dev(np.array([0.02 * (s + 1)]), device)).converged
         h1.close()
         assert (r1 - r).abs().max().item() <= 1e-9 * r1.abs().max().item()
+
+
+def test_checkpoint_and_resume(device, tmp_path):
+    """Three CM3F load steps, stopped after the second: the state written as the reference's
+    restart files (with the volume and pressure records), read into a fresh handle and driver
+    and run on, ends bit for bit on the uninterrupted run."""
+    from types import SimpleNamespace
+
+    import restart as RS
+
+    prob = BoxProblem(2, 10, HE, seed=5)
+    m = prob.mesh
+    deck = SimpleNamespace(parts=[SimpleNamespace(nn=m.n_nodes, ne=m.n_elem)], node_of=[np.arange(m.n_nodes)],
+                           elem_of=[np.arange(m.n_elem)], elem_ptr=m.elem_ptr, X=m.X)
+
+    def new():
+        h = _handle(prob, device)
+        return h, NW.NewtonDriver(h, prob.Ap, prob.Ai, tol=1e-12, max_iter=30)
+
+    def steps(drv, r, s0, s1):
+        for s in range(s0, s1):
+            assert drv.load_step(r, _dev(np.array([0.02 * (s + 1)]), device)).converged
+
+    h, drv = new()
+    r = torch.zeros(prob.ndof, dtype=torch.float64, device=device)
+    steps(drv, r, 0, 3)
+    full = (r.cpu().numpy(), h.tf_get(P.TF_V_N), h.tf_get(P.TF_P_N), h.get_state(P.STATE_F_N))
+    h.close()
+
+    def nodal(rr, d):
+        u = np.zeros_like(m.X)
+        msk = prob.ids > 0
+        u[msk] = rr[prob.ids[msk] - 1]
+        u[prob.ids < 0] = d
+        return u
+
+    h, drv = new()
+    r = torch.zeros(prob.ndof, dtype=torch.float64, device=device)
+    steps(drv, r, 0, 1)
+    r1 = r.cpu().numpy()
+    steps(drv, r, 1, 2)
+    RS.save(str(tmp_path), "box", 1, deck, h, nodal(r1, 0.02), nodal(r.cpu().numpy(), 0.04), (0.0, 1.0, 2.0), drv.NORM)
+    norm = drv.NORM
+    h.close()
+
+    h, drv = new()
+    _, u_n, _, norm2 = RS.load(str(tmp_path), "box", 1, deck, h)
+    assert norm2 == norm
+    drv.NORM, drv.first = norm2, False
+    r = _dev(u_n[prob.ids > 0][np.argsort(prob.ids[prob.ids > 0])], device)
+    steps(drv, r, 2, 3)
+    assert np.array_equal(r.cpu().numpy(), full[0])
+    assert np.array_equal(h.tf_get(P.TF_V_N), full[1]) and np.array_equal(h.tf_get(P.TF_P_N), full[2])
+    assert np.array_equal(h.get_state(P.STATE_F_N), full[3])
+    h.close()

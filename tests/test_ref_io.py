@@ -108,6 +108,27 @@ def test_restart_round_trip_is_byte_exact(deck, p, tmp_path):
         assert a.read() == b.read()
 
 
+def test_three_field_restart_records(tmp_path):
+    """A CM3F file (write_restart_constitutive_model, src/restart.cc:241-254): the hyperelastic
+    records, then per element "V_n V_nm1 \n P_n P_nm1 \n"; read back and written again byte for
+    byte, values exact."""
+    rng = np.random.default_rng(1)
+    nn, ne = 5, 3
+    u1, un = rng.standard_normal((nn, 3)), rng.standard_normal((nn, 3))
+    el = [[R.HERecord(rng.standard_normal(9), rng.standard_normal(9)) for _ in range(k)] for k in (1, 4, 1)]
+    tf = tuple(rng.standard_normal(ne) for _ in range(4))
+    out = tmp_path / "a.res"
+    R.write_restart(out, u1, un, el, tf)
+    text = open(out).read().splitlines()
+    assert text[-2] == f"{tf[0][2]:.17e} {tf[1][2]:.17e} " and text[-1] == f"{tf[2][2]:.17e} {tf[3][2]:.17e} "
+    a, b, recs, tf2 = R.read_restart(out, nn, 3, ["he"] * ne, [1, 4, 1], three_field=True)
+    assert np.array_equal(a, u1) and np.array_equal(b, un) and [len(r) for r in recs] == [1, 4, 1]
+    assert all(np.array_equal(x, y) for x, y in zip(tf, tf2))
+    out2 = tmp_path / "b.res"
+    R.write_restart(out2, a, b, recs, tf2)
+    assert open(out).read() == open(out2).read()
+
+
 def test_euler_rotation():
     rng = np.random.default_rng(3)
     for _ in range(20):
