@@ -190,6 +190,41 @@ def run_j2(args, dev):
     return res
 
 
+def run_tf(args, dev):
+    """Three-field (CM3F) passes of a neo-Hookean mesh of the J2 workload's size (48^3 P2 tets,
+    E=70 GPa, nu=0.33), theta and P at their initial conditions for the displacement; the same
+    mesh's displacement-formulation pass is timed beside it.  Algorithmic bytes per pass: F(n+1)
+    written (9 doubles per QP), per element theta, P and their increments read and the update
+    data written (4 + 34 doubles), the CSR values, residual, nodes and connectivity."""
+    import torch
+
+    import pgfem3d_amd as P
+    from problems import BoxProblem
+
+    prob = BoxProblem(args.n, P.PGF_TET10, P.neo_hookean(70e3, 0.33))
+    h = P.Handle(prob.mesh, prob.ids, prob.ndof, prob.Ap, prob.Ai, prob.mats)
+    r, defl = prob.displacement(0.015, 2e-4, seed=11)
+    rd = torch.from_numpy(r).to(dev)
+    dd = torch.from_numpy(defl).to(dev)
+    f = torch.empty(prob.ndof, dtype=torch.float64, device=dev)
+    vals = torch.empty(h.nnz, dtype=torch.float64, device=dev)
+    h.set_profiling(True)
+    steps = max(3, args.steps // 2)
+    s_1f, k_1f, _ = _timed_passes(h, f, vals, rd, dd, steps, 1)
+    h.set_three_field(True)
+    h.tf_initial(rd, dd)
+    s_tf, k_tf, n_l = _timed_passes(h, f, vals, rd, dd, steps, 1)
+    ne, nq = prob.mesh.n_elem, h.n_qp
+    nbytes = 8 * (9 * nq + 38 * ne + h.nnz + prob.ndof + 6 * prob.mesh.n_nodes) + 4 * 10 * ne
+    res = dict(n_qp=nq, n_elem=ne, s_step=s_tf, kern_ms=k_tf, n_launch=n_l, kern_ms_1f=k_1f, s_step_1f=s_1f,
+               achieved=nbytes / (k_tf * 1e-3) / 1e9)
+    print(f"[bench] CM3F neo-Hookean {ne} P2 tets: {1e3 * s_tf:.3f} ms/step wall, pass {k_tf:.3f} ms over {n_l} "
+          f"launches ({res['achieved']:.0f} GB/s algorithmic); displacement formulation {k_1f:.3f} ms",
+          file=sys.stderr, flush=True)
+    h.close()
+    return res
+
+
 def run_cp(args, dev):
     import torch
 
@@ -325,6 +360,7 @@ def main():
     ap.add_argument("--no-cp", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-tf", action="store_true", help="skip the three-field (CM3F) passes")
     ap.add_argument("--mix", type=int, nargs=3, default=[41, 32, 46],
                     help="cells per edge of the linear-tet, quadratic-tet and hex boxes of the mixed batch")
     ap.add_argument("--sur-epochs", type=int, default=3)
@@ -342,6 +378,7 @@ def main():
     j2 = run_j2(args, dev)
     cp = None if args.no_cp else run_cp(args, dev)
     mixed = None if args.no_mixed else run_mixed(args, dev)
+    tf = None if args.no_tf else run_tf(args, dev)
 
     cpu = cpu_cp = None
     if not args.no_cpu:
@@ -412,6 +449,17 @@ def main():
             "value": mixed["graph"]["value"], "unit": "QP updates/s (volume + interface QPs)",
             "eager": mixed["eager"], "graph": mixed["graph"], "launches_per_pass": mixed["launches"],
             "autotuned_chunk": mixed["chunk"], "mean_local_iters": mixed["mean_local_iters"],
+        }
+    if tf is not None:
+        out["three_field"] = {
+            "workload": f"CM3F (one volume and one pressure per element, condensed) neo-Hookean E=70GPa nu=0.33, "
+                        f"{args.n}^3 cells x 6 quadratic tets ({tf['n_elem']} elements, {tf['n_qp']} QPs); one fused "
+                        "residual + tangent + CSR assembly pass per step",
+            "value": tf["n_qp"] / tf["s_step"], "unit": "QP updates/s", "ms_per_step": 1e3 * tf["s_step"],
+            "kernel_ms_per_pass": tf["kern_ms"], "launches_per_pass": tf["n_launch"],
+            "displacement_formulation_kernel_ms_per_pass": tf["kern_ms_1f"],
+            "roofline": {"bound": "hbm", "achieved": tf["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": tf["achieved"] / HBM_PEAK_GBS},
         }
     print(json.dumps(out))
 

@@ -248,7 +248,9 @@ __global__ __launch_bounds__(256, STG ? 4 : 1) void elem_kernel(ElemArgs a, int6
   __shared__ double s_tau[EPB][NQP][6];
   __shared__ double s_g[EPB][NQP][GEN ? 9 : 1];
   __shared__ double s_dl[EPB][DFL ? NDE : 1];  // prescribed increments of the element DOFs
-  __shared__ double s_tf[EPB][NQP][TF ? 2 : 1];  // three-field: detJ w J, detJ w
+  __shared__ double s_tf[EPB][NQP][TF ? 1 : 1];  // three-field: detJ w J per QP
+  __shared__ double s_tfe[EPB][TF ? 6 : 1];      // three-field, per element: Ktt/V^2, the residual
+                                                 // coefficient, fp, ft, Ktt, V
 
   const int slot = threadIdx.x / TPE;
   const int lane = threadIdx.x % TPE;
@@ -416,9 +418,32 @@ __global__ __launch_bounds__(256, STG ? 4 : 1) void elem_kernel(ElemArgs a, int6
     // tau and c go to LDS weighted by the QP's detJ w (zero for a failed QP)
     const double wq = ok ? J0 * wt : 0.0;
     if constexpr (TF) {
-      if (sub == 0) {
-        s_tf[slot][ip][0] = wq * J;
-        s_tf[slot][ip][1] = wq;
+      // element sums over the QP lanes of the element (NQP * LPQ consecutive lanes), then the
+      // element's condensation coefficients (compute_Rp/Rt/Ktt/Ktp,
+      // src/constitutive_model_3f.cc:219-246, :418-466): fp = sum detJ w (J - theta),
+      // ft = V (kappa U'(theta) - P), Ktt = V kappa U''(theta), Ktp = Kpt = -V
+      const double wJ = wq * (ok ? J : 0.0);
+      if (sub == 0) s_tf[slot][ip][0] = wJ;
+      const double Jw = group_sum<NQP * LPQ>(sub == 0 ? wJ : 0.0);
+      const double V = group_sum<NQP * LPQ>(sub == 0 ? wq : 0.0);
+      if (ip == 0 && sub == 0) {
+        double dU, d2U;
+        vol_pot(m.volPotFlag, pth, dU, d2U);
+        const double rV = V > 0.0 ? 1.0 / V : 0.0;  // V = 0 only when every QP failed (counted)
+        const double Ktt = V * m.kappa * d2U, ft = V * (m.kappa * dU - ppr), fp = Jw - pth * V;
+        s_tfe[slot][0] = Ktt * rV * rV;
+        s_tfe[slot][1] = Ktt * fp * rV * rV + ft * rV;
+        s_tfe[slot][2] = fp;
+        s_tfe[slot][3] = ft;
+        s_tfe[slot][4] = Ktt;
+        s_tfe[slot][5] = V;
+        if (INTEG && RES) {  // the element data of the update (IntegrateThreeFieldUpdate)
+          double *tb = a.tf_buf + e * kTfBuf + NDE;
+          tb[0] = fp;
+          tb[1] = ft;
+          tb[2] = Ktt;
+          tb[3] = V;
+        }
       }
     }
 #pragma unroll
@@ -469,6 +494,16 @@ __global__ __launch_bounds__(256, STG ? 4 : 1) void elem_kernel(ElemArgs a, int6
     // h_a[j1], h_a[j2] at the shears (b,j1), (b,j2)
     const int j1 = (rb + 1) % 3, j2 = (rb + 2) % 3;
     const int s1 = 3 + rb, s2 = 3 + j2;
+    // three-field: this row's Kup entry, sum detJ w J h_a[b]; the condensation's rank-one
+    // term Ktt/V^2 Kup_row Kup_(w,g) is added per QP through the row block M (+ q 1 with
+    // q = Ktt/V^2 Kup_row detJ w J), so it costs no extra pass over the columns
+    double kup = 0.0;
+    if constexpr (TF) {
+#pragma unroll
+      for (int ip = 0; ip < NQP; ++ip) kup += s_tf[slot][ip][0] * s_h[slot][ip][lane];
+      if constexpr (INTEG && RES) a.tf_buf[e * kTfBuf + lane] = kup;  // Kup row of the update data
+    }
+    const double al = TF ? s_tfe[slot][0] * kup : 0.0;
 #pragma unroll
     for (int ip = 0; ip < NQP; ++ip) {
       const double *h = s_h[slot][ip];
@@ -525,6 +560,12 @@ __global__ __launch_bounds__(256, STG ? 4 : 1) void elem_kernel(ElemArgs a, int6
           M[3 + k] += eb1 * t[k];
           M[6 + k] += eb2 * t[k];
         }
+        if constexpr (TF) {
+          const double q = al * s_tf[slot][ip][0];
+          M[0] += q;
+          M[4] += q;
+          M[8] += q;
+        }
 #pragma unroll
         for (int w = 0; w < NNE; ++w) {
           const double h0 = h[w * 3 + 0], h1 = h[w * 3 + 1], h2 = h[w * 3 + 2];
@@ -541,42 +582,7 @@ __global__ __launch_bounds__(256, STG ? 4 : 1) void elem_kernel(ElemArgs a, int6
       //   K += Ktt / V^2 Kup Kup^T,   f += Kup (Ktt fp / V^2 + ft / V)
       // with fp = sum detJ w (J - theta), ft = V (kappa U'(theta) - P), Ktt = V kappa U''(theta)
       // (compute_Rp/Rt/Ktt/Ktp, src/constitutive_model_3f.cc:219-246, :418-466).
-      const DevMaterial me = a.mats[a.mat[e]];
-      const double th = a.tf_V[e] + a.tf_dV[e], pr = a.tf_P[e] + a.tf_dP[e];
-      double V = 0.0, Jw = 0.0, kup = 0.0;
-#pragma unroll
-      for (int ip = 0; ip < NQP; ++ip) {
-        Jw += s_tf[slot][ip][0];
-        V += s_tf[slot][ip][1];
-        kup += s_tf[slot][ip][0] * s_h[slot][ip][lane];
-      }
-      double dU, d2U;
-      vol_pot(me.volPotFlag, th, dU, d2U);
-      const double rV = V > 0.0 ? 1.0 / V : 0.0;  // V = 0 only when every QP failed (counted)
-      const double Ktt = V * me.kappa * d2U, ft = V * (me.kappa * dU - pr), fp = Jw - th * V;
-      if constexpr (RES) {
-        fe += kup * (Ktt * fp * rV * rV + ft * rV);
-        if constexpr (INTEG) {  // the element data of the update (IntegrateThreeFieldUpdate)
-          double *tb = a.tf_buf + e * kTfBuf;
-          tb[lane] = kup;
-          if (lane == 0) {
-            tb[NDE] = fp;
-            tb[NDE + 1] = ft;
-            tb[NDE + 2] = Ktt;
-            tb[NDE + 3] = V;
-          }
-        }
-      }
-      if constexpr (TAN) {
-        const double al = Ktt * rV * rV * kup;
-#pragma unroll
-        for (int c = 0; c < NDE; ++c) {
-          double kc = 0.0;
-#pragma unroll
-          for (int ip = 0; ip < NQP; ++ip) kc += s_tf[slot][ip][0] * s_h[slot][ip][c];
-          K[c] += al * kc;
-        }
-      }
+      if constexpr (RES) fe += kup * s_tfe[slot][1];
     }
     if constexpr (TAN) {  // non-normal diagonal, src/constitutive_model.cc:2682-2687
       double diag = 0.0;
