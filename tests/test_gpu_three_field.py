@@ -11,7 +11,9 @@ test_oracle_three_field.py), at the same per-element volume and pressure variabl
   * Newton load steps (device driver, device linear solve) against the oracle's CM3F Newton:
     displacements within 1e-9 relative, theta and P within 1e-9; on linear tets the converged
     steps equal the displacement formulation's;
-  * commit and reset: n+1 <- n+1 + increments, n <- n+1, n-1 <- n, increments zeroed.
+  * commit and reset: n+1 <- n+1 + increments, n <- n+1, n-1 <- n, increments zeroed;
+  * a mesh of all three element types (three device blocks: block offsets, device order);
+  * checkpoint/resume through the reference's restart files, and hipGraph replays.
 """
 import numpy as np
 import pytest
@@ -237,4 +239,85 @@ def test_checkpoint_and_resume(device, tmp_path):
     assert np.array_equal(r.cpu().numpy(), full[0])
     assert np.array_equal(h.tf_get(P.TF_V_N), full[1]) and np.array_equal(h.tf_get(P.TF_P_N), full[2])
     assert np.array_equal(h.get_state(P.STATE_F_N), full[3])
+    h.close()
+
+
+class _MixedHE(BoxProblem):
+    """Linear tets, quadratic tets and hexes side by side in one neo-Hookean mesh: three device
+    blocks, so the per-element variables sit at block offsets and in device order."""
+
+    def __init__(self):
+        from problems import mixed_mesh
+
+        self.mesh, _, _ = mixed_mesh(2, 2, 2, 3, seed=4, interfaces=False)
+        self.supports = P.box_supports(self.mesh)
+        self.ids, self.ndof = P.generate_dof_ids(self.mesh, self.supports)
+        self.Ap, self.Ai = P.psparse_apai(self.mesh, self.ids, self.ndof)
+        self.mats = [HE]
+        self.nqp = int(sum(O.nint(int(k)) for k in np.diff(self.mesh.elem_ptr)))
+
+
+def test_mixed_element_types(device):
+    prob = _MixedHE()
+    r, defl = prob.displacement(0.03, 0.01, seed=6)
+    V, Pr, dV, dP = _fields(prob.mesh.n_elem, 13)
+    h = _handle(prob, device)
+    for fld, v in ((P.TF_V_NP1, V), (P.TF_P_NP1, Pr), (P.TF_DV, dV), (P.TF_DP, dP)):
+        h.tf_set(fld, v)
+    assert np.array_equal(h.tf_get(P.TF_V_NP1), V) and np.array_equal(h.tf_get(P.TF_DP), dP)
+    f = torch.zeros(prob.ndof, dtype=torch.float64, device=device)
+    vals = torch.zeros(h.nnz, dtype=torch.float64, device=device)
+    assert h.residual_tangent(f, vals, _dev(r, device), defl=_dev(defl, device)) == 0
+    Fo = np.zeros((9, prob.nqp))
+    fo, vo, err = O.assemble_tf(prob.mesh, prob.mats, prob.ids, prob.ndof, prob.Ap, prob.Ai, V + dV, Pr + dP, r,
+                                defl=defl, F_np1=Fo)
+    assert err == 0
+    assert np.abs(f.cpu().numpy() - fo).max() <= 1e-10 * np.abs(fo).max()
+    assert np.abs(vals.cpu().numpy() - vo).max() <= 1e-10 * np.abs(vo).max()
+    assert np.abs(h.get_state(P.STATE_F_NP1) - Fo).max() < 1e-12
+    du = 1e-3 * np.random.default_rng(2).standard_normal(prob.ndof)
+    h.tf_update(_dev(du, device))
+    ddV, ddP, _ = O.tf_update(prob.mesh, prob.mats, prob.ids, V + dV, Pr + dP, du, r, defl=defl)
+    assert np.abs(h.tf_get(P.TF_DDV) - ddV).max() <= 1e-10 * np.abs(ddV).max()
+    assert np.abs(h.tf_get(P.TF_DDP) - ddP).max() <= 1e-10 * np.abs(ddP).max()
+    h.close()
+    # Newton load steps from rest
+    h = _handle(prob, device)
+    drv = NW.NewtonDriver(h, prob.Ap, prob.Ai, tol=1e-12, max_iter=30)
+    rr = torch.zeros(prob.ndof, dtype=torch.float64, device=device)
+    ro = np.zeros(prob.ndof)
+    tf = dict(V=np.ones(prob.mesh.n_elem), P=np.zeros(prob.mesh.n_elem))
+    state = {"NORM": 0.0, "first": True}
+    for s in range(2):
+        d = np.array([0.02 * (s + 1)])
+        assert drv.load_step(rr, _dev(d, device)).converged
+        ro, _ = oracle_tf_newton_step(prob, ro, d, tf, tol=1e-12, state=state)
+        assert np.abs(rr.cpu().numpy() - ro).max() <= 1e-9 * np.abs(ro).max()
+        assert np.abs(h.tf_get(P.TF_V_N) - tf["V"]).max() < 1e-9
+        assert np.abs(h.tf_get(P.TF_P_N) - tf["P"]).max() < 1e-9 * np.abs(tf["P"]).max()
+    h.close()
+
+
+def test_graph_replays_match_eager(device):
+    """The condensed pass captured in a hipGraph and replayed (with eager work between the
+    replays) gives the eager pass's values on every replay."""
+    prob = BoxProblem(2, 10, HE, seed=8)
+    r, defl = prob.displacement(0.02, 0.01, seed=3)
+    V, Pr, dV, dP = _fields(prob.mesh.n_elem, 3)
+    h = _handle(prob, device)
+    for fld, v in ((P.TF_V_NP1, V), (P.TF_P_NP1, Pr), (P.TF_DV, dV), (P.TF_DP, dP)):
+        h.tf_set(fld, v)
+    rd, dd = _dev(r, device), _dev(defl, device)
+    f0 = torch.zeros(prob.ndof, dtype=torch.float64, device=device)
+    v0 = torch.zeros(h.nnz, dtype=torch.float64, device=device)
+    assert h.residual_tangent(f0, v0, rd, defl=dd) == 0
+    h.set_graph(True)
+    f = torch.zeros_like(f0)
+    v = torch.zeros_like(v0)
+    for _ in range(3):
+        f.fill_(7.0)
+        v.fill_(-1.0)
+        assert h.residual_tangent(f, v, rd, defl=dd) == 0
+        torch.cuda.synchronize()
+        assert torch.equal(f, f0) and torch.equal(v, v0)
     h.close()
