@@ -807,7 +807,8 @@ int orc_assemble(int64_t n_nodes, const double *X, int64_t n_elem, const int64_t
 //   dU,ddU kappa U'(theta), kappa U''(theta) (compute_dudj/d2udj2, constitutive_model.h:319-384)
 // Checks: Kuu, Kup, Ktt, Ktp against central differences of Ru, Rt, Rp; at theta = J and
 // P = kappa U'(J) on linear tets Ru equals the displacement element's residual; the
-// condensation equals a monolithic solve of the element's (u, theta, P) system; a converged
+// condensation equals a monolithic solve of the element's (u, theta, P) system, and the
+// condensed global solve with orc_tf_update that of a whole mesh's system; a converged
 // CM3F Newton on linear tets equals the displacement solution -> test_oracle_three_field.py.
 namespace {
 

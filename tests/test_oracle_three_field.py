@@ -4,7 +4,9 @@ element update src/constitutive_model.cc:595-656, 3916-3999):
 
   * each block is the derivative of the residual it belongs to (central differences):
     Kuu = dRu/du, Kup = dRu/dP, Kup = dRp/du, Ktt = dRt/dtheta, Ktp = dRt/dP = dRp/dtheta;
-  * the condensed K and f equal a monolithic solve of the element's (u, theta, P) system;
+  * the condensed K and f equal a monolithic solve of the element's (u, theta, P) system, and
+    over a whole mesh the condensed global solve plus the element update (orc_tf_update)
+    equal the monolithic solve of the mesh's (u, theta_e, P_e) system;
   * on linear tets (one QP, J constant over the element) at theta = J and P = kappa U'(J), the
     condensed residual and tangent are those of the displacement element, and a converged
     CM3F load step lands on the displacement solution;
@@ -152,3 +154,47 @@ def test_initial_conditions_of_a_homogeneous_stretch(nne):
     assert np.allclose(V, J, rtol=1e-13) and np.allclose(Pr, KAPPA * _dU(J)[0], rtol=1e-11)
     V0, P0, _ = O.tf_initial(prob.mesh, prob.mats, np.zeros_like(u))
     assert np.all(V0 == 1.0) and np.abs(P0).max() < 1e-12
+
+
+@pytest.mark.parametrize("nne,n", [(10, 1), (8, 2)])
+def test_mesh_condensation_and_update_are_the_monolithic_solve(nne, n):
+    """Over a whole mesh: the (u, theta_e, P_e) Newton system assembled from the element blocks
+    and solved at once gives the displacement increment of the condensed global solve
+    (orc_assemble_tf) and the volume and pressure increments of orc_tf_update."""
+    prob = BoxProblem(n, nne, HE, jitter=0.1, seed=6)
+    r, defl = prob.displacement(0.03, 0.01, seed=2)
+    ne, nd = prob.mesh.n_elem, prob.ndof
+    rng = np.random.default_rng(4)
+    th, pr = 1.0 + 0.01 * rng.standard_normal(ne), 200.0 * rng.standard_normal(ne)
+    u = np.zeros_like(prob.mesh.X)
+    m = prob.ids > 0
+    u[m] = r[prob.ids[m] - 1]
+    u[prob.ids < 0] = defl[-prob.ids[prob.ids < 0] - 1]
+    N = nd + 2 * ne
+    A, rhs = np.zeros((N, N)), np.zeros(N)
+    for e in range(ne):
+        nod = prob.mesh.elem_conn[prob.mesh.elem_ptr[e]:prob.mesh.elem_ptr[e + 1]]
+        E = O.tf_element(HE, nne, prob.mesh.X[nod].reshape(-1), u[nod].reshape(-1), th[e], pr[e])
+        dof = prob.ids[nod].reshape(-1)
+        fr = np.nonzero(dof > 0)[0]
+        g = dof[fr] - 1
+        it, ip = nd + 2 * e, nd + 2 * e + 1
+        A[np.ix_(g, g)] += E["Kuu"][np.ix_(fr, fr)]
+        A[g, ip] += E["Kup"][fr]          # Kup (Kut = 0)
+        A[ip, g] += E["Kup"][fr]          # Kpu = Kup^T (Ktu = 0)
+        A[it, it] += E["Ktt"]
+        A[it, ip] += E["Ktp"]
+        A[ip, it] += E["Ktp"]             # Kpt = Ktp
+        rhs[g] -= E["Ru"][fr]
+        rhs[it] -= E["Rt"]
+        rhs[ip] -= E["Rp"]
+    sol = np.linalg.solve(A, rhs)
+    f, vals, err = O.assemble_tf(prob.mesh, prob.mats, prob.ids, nd, prob.Ap, prob.Ai, th, pr, r, defl=defl)
+    assert err == 0
+    import scipy.sparse as sp
+    du = np.linalg.solve(sp.csr_matrix((vals, prob.Ai, prob.Ap), shape=(nd, nd)).toarray(), -f)
+    assert np.abs(du - sol[:nd]).max() < 1e-9 * np.abs(du).max()
+    ddV, ddP, err = O.tf_update(prob.mesh, prob.mats, prob.ids, th, pr, du, r, defl=defl)
+    assert err == 0
+    assert np.abs(ddV - sol[nd::2]).max() < 1e-9 * np.abs(ddV).max()
+    assert np.abs(ddP - sol[nd + 1::2]).max() < 1e-9 * np.abs(ddP).max()
