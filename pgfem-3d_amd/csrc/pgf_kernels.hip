@@ -217,9 +217,10 @@ constexpr int stage_round(int epb, int nde, int room) {
 
 // TF: the CM3F three-field element of the hyperelastic model (one volume and one pressure
 // variable per element, condensed): F(n+1) is always Fr (the model stores (theta/J)^1/3 Fr),
-// tau and c take the element pressure, and the rows get the condensation terms.
+// tau and c take the element pressure, and the rows get the condensation terms.  Its staged
+// variant runs at three waves per SIMD: at four (128 registers) the extra element terms spill.
 template <int NNE, int MODE, bool STG, bool TF>
-__global__ __launch_bounds__(256, STG ? 4 : 1) void elem_kernel(ElemArgs a, int64_t e_begin, int64_t e_end) {
+__global__ __launch_bounds__(256, STG ? (TF ? 3 : 4) : 1) void elem_kernel(ElemArgs a, int64_t e_begin, int64_t e_end) {
   constexpr int NQP = nqp_of(NNE);
   constexpr int NDE = NNE * 3;
   constexpr int TPE = (NNE == 4) ? 16 : 32;
