@@ -13,7 +13,8 @@ test_oracle_three_field.py), at the same per-element volume and pressure variabl
     steps equal the displacement formulation's;
   * commit and reset: n+1 <- n+1 + increments, n <- n+1, n-1 <- n, increments zeroed;
   * a mesh of all three element types (three device blocks: block offsets, device order);
-  * checkpoint/resume through the reference's restart files, and hipGraph replays.
+  * checkpoint/resume through the reference's restart files, and hipGraph replays;
+  * load steps opened by the reference's predictor land on the steps without it.
 """
 import numpy as np
 import pytest
@@ -321,3 +322,26 @@ def test_graph_replays_match_eager(device):
         torch.cuda.synchronize()
         assert torch.equal(f, f0) and torch.equal(v, v0)
     h.close()
+
+
+def test_predictor_steps_reach_the_same_solution(device):
+    """Load steps that open with the reference's predictor (K du = -f_defl at (u_n, defl_n);
+    theta and P follow in the iterations after it) converge to the steps taken from the full
+    prescribed values."""
+    prob = BoxProblem(2, 10, HE, seed=9)
+    out = []
+    for pred in (False, True):
+        h = _handle(prob, device)
+        drv = NW.NewtonDriver(h, prob.Ap, prob.Ai, tol=1e-12, max_iter=30, computer_zero=1e-9)
+        r = torch.zeros(prob.ndof, dtype=torch.float64, device=device)
+        for s in range(3):
+            if pred:
+                res = drv.load_step(r, _dev(np.array([0.02 * s]), device), defl_d=_dev(np.array([0.02]), device))
+            else:
+                res = drv.load_step(r, _dev(np.array([0.02 * (s + 1)]), device))
+            assert res.converged, (pred, s, res.norms)
+        out.append((r.cpu().numpy(), h.tf_get(P.TF_V_N), h.tf_get(P.TF_P_N)))
+        h.close()
+    (r0, V0, P0), (r1, V1, P1) = out
+    assert np.abs(r1 - r0).max() <= 1e-8 * np.abs(r0).max()
+    assert np.abs(V1 - V0).max() < 1e-9 and np.abs(P1 - P0).max() <= 1e-8 * np.abs(P0).max()
