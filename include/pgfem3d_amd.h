@@ -77,6 +77,10 @@ typedef struct pgf_material {
   double C11, C12, C44;           /* cubic elasticity in the crystal frame */
   double tol_M, tol_hardening;    /* local Newton tolerances */
   int32_t max_itr_M, max_itr_stag;
+  /* J2 continuum damage (CM_J2P_PARAM damage properties, src/cm_j2_plasticity.cc:865-932):
+   * viscosity mu, cap w_max (>= 1: 0.9999), Weibull P1, P2 and threshold Yin.  mu <= 0 turns
+   * the damage off. */
+  double mu, w_max, p1, p2, Yin;
 } pgf_material;
 
 /* Mesh record (host memory).  Element connectivity is CSR-like so that several element
@@ -232,10 +236,25 @@ int pgf_prescribed_load(pgf_handle h, const pgf_nodal_input *in, double dt, doub
 int pgf_residual_tangent(pgf_handle h, const pgf_nodal_input *in, double dt, double *f,
                          double *vals);
 
+/* Physics-based subdivision measure of the last integration (cm_get_subdivision_parameter,
+ * src/constitutive_model.cc:1980-2001): the largest per-QP value of the models'
+ * get_subdiv_param over the mesh.  Crystal plasticity (CP_PARAM::get_subdiv_param,
+ * src/plasticity_model.cc:1110-1125): dt sum_a |gamma_dot_a(n+1)| / 0.005; J2 with damage
+ * (CM_J2P_PARAM::get_subdiv_param, src/cm_j2_plasticity.cc:523-531): (w(n+1) - w(n)) / 0.05,
+ * the damage increment over DELTA_W_MAX (:55; the formula itself is in the absent library);
+ * the hyperelastic model gives 0 (Model_parameters::get_subdiv_param,
+ * include/constitutive_model.h:617-624).  dt <= 0 uses the handle's time step.  The caller
+ * rejects a converged step whose value exceeds 1.25 and subdivides it, as Newton_Raphson does
+ * (src/Newton_Raphson.cc:1364-1398). */
+int pgf_subdivision_param(pgf_handle h, double dt, double *alpha);
+
 /* Capture each pass (all of its launches: every element type, the crystal-plasticity QP
  * kernels, the interface elements) into one hipGraph and replay it while the pass mode, the
- * vector pointers, dt and the tuning stay the same; on = 0 returns to eager launches.  (No
- * reference counterpart: a launch-overhead measure for the per-iteration element pass.) */
+ * vector pointers, dt and the tuning stay the same; on = 0 returns to eager launches.
+ * pgf_set_orientations, pgf_set_cohesive_props, pgf_set_three_field and pgf_set_assembly
+ * replace buffers or kernels a captured pass holds: they finish the handle's work and drop
+ * the graph, and the next pass captures again.  (No reference counterpart: a launch-overhead
+ * measure for the per-iteration element pass.) */
 int pgf_set_graph(pgf_handle h, int on);
 
 /* Autotune the crystal-plasticity local-Newton work queue (QPs per wave range, 64..2048) by
@@ -283,8 +302,11 @@ int pgf_reset_step(pgf_handle h);
  * on = 0 turns the mode off and frees its memory.  PGF_ENOTSUP if a material is not
  * hyperelastic. */
 int pgf_set_three_field(pgf_handle h, int32_t on);
-/* compute_3f_initial_conditions (src/three_field_element.cc:1328-1386): theta and P at n (and
- * n+1) from the nodal input in_n, at n-1 from in_nm1 (NULL: in_n); increments zeroed. */
+/* compute_cm_initial_conditions (src/constitutive_model.cc:4035-4160) for the three-field
+ * analysis, with compute_3f_initial_conditions (src/three_field_element.cc:1328-1386): theta
+ * and P at n (and n+1) from the nodal input in_n, at n-1 from in_nm1 (NULL: in_n); the
+ * model's F(n) and F(n-1) from the same displacements (set_F, :4097-4105); increments
+ * zeroed.  Returns the number of QPs with J <= 0 or a singular dX/dxi. */
 int pgf_tf_initial(pgf_handle h, const pgf_nodal_input *in_n, const pgf_nodal_input *in_nm1);
 /* constitutive_model_update_NR without inertia (src/constitutive_model.cc:3916-3999,
  * IntegrateThreeFieldUpdate :595-656): ddV, ddP of every element for the Newton increment
@@ -292,7 +314,10 @@ int pgf_tf_initial(pgf_handle h, const pgf_nodal_input *in_n, const pgf_nodal_in
  * must have been taken at the state the increment starts from. */
 int pgf_tf_update(pgf_handle h, const double *du);
 /* ThreeFieldVariables::update_increments_from_NR (include/PGFem3D_data_structure.h:197-213):
- * dV += gamma ddV, dP += gamma ddP. */
+ * ddV = gamma ddV, dV += ddV (and the same for P), taken from the ddV, ddP and dV, dP that
+ * the last pgf_tf_update left, so that a line-search trial is taken back exactly by
+ * gamma = 0 and a second trial replaces the first (the reference scales ddV in place and
+ * applies the update once). */
 int pgf_tf_accept(pgf_handle h, double gamma);
 /* Per-element variables, [n_elem] in mesh element order. */
 #define PGF_TF_V_NP1 0
@@ -316,7 +341,8 @@ int pgf_tf_set(pgf_handle h, int32_t field, const double *host_in);
 #define PGF_STATE_PLASTIC_NP1 3  /* J2: 6 comps Cp^-1 (xx,yy,zz,xy,yz,xz); CP: 9 comps pF */
 #define PGF_STATE_PLASTIC_N 4
 #define PGF_STATE_PLASTIC_NM1 5
-#define PGF_STATE_SCALAR_NP1 6   /* J2: ep, gam; CP: g (slip resistance), lambda */
+#define PGF_STATE_SCALAR_NP1 6   /* J2: ep, gam (with damage also w, X, H, damaged flag);
+                                    CP: g (slip resistance), lambda */
 #define PGF_STATE_SCALAR_N 7
 #define PGF_STATE_SCALAR_NM1 8
 #define PGF_STATE_TAU 9          /* CP: 12 resolved shear stresses at n+1 (TENSOR_tau) */

@@ -444,6 +444,93 @@ int j2_update(const orc_material &m, const double *F, const double *Cpinv_n, dou
   return 0;
 }
 
+// ------------------------------------------------------------------ J2 + continuum damage
+// CM_J2P_PARAM (src/cm_j2_plasticity.cc:184-212, 287-347): after the J2 return map the
+// damage is integrated with the driving force Y0 = G/2 (tr bbar - 3) + kappa U(J), the
+// undamaged elastic energy of the total F (j2d_compute_Y0, :162-182, bbar = J^-2/3 F F^T),
+// then S = (1 - w) Sbar (apply_damage_on_stress) and, when the point is damaging,
+// L = (1 - w) Lbar - evo/2 (Sbar (x) S0 + S0 (x) Sbar) with evo = dt mu H / (1 + dt mu)
+// (:364-382).  S0 = 2 dY0/dC is the stress Y0 derives from.  The damage update,
+// continuum_damage_integration_alg_public, lives in the absent library; it is restated from
+// the in-tree viscous damage of the same parameters (damage_int_alg with the Weibull
+// functions, src/volumetric_damage.cc:196-262; parameters mu, w_max, P1, P2, Yin read by
+// CM_J2P_PARAM::read_param, :865-932, w_max >= 1 taken as DAMAGE_THRESH): with dmu = dt mu,
+// g = G(Y0) - X_n; if g > 0: w = w_n + dmu/(1 + dmu) g, X = max(X_n, (X_n + dmu G)/(1 + dmu)),
+// H = G'(Y0); else w = w_n, X = X_n, H = 0.  w is capped at w_max.
+constexpr double kDamageThresh = 0.9999;  // DAMAGE_THRESH, src/volumetric_damage.cc:16
+
+// weibull_function / weibull_evolution, src/volumetric_damage.cc:236-262
+void weibull(const orc_material &m, double Y, double *G, double *H) {
+  if (Y <= m.Yin) {
+    *G = 0.0;
+    *H = 0.0;
+    return;
+  }
+  const double z = (Y - m.Yin) / (m.p1 * m.Yin);
+  const double e = std::exp(-std::pow(z, m.p2));
+  *G = kDamageThresh - kDamageThresh * e;
+  *H = kDamageThresh * m.p2 / (m.p1 * m.Yin) * e * std::pow(z, m.p2 - 1.0);
+}
+
+double j2d_Y0(const orc_material &m, const double *F, double *S0) {
+  const double J = det3x3(F);
+  double C[9], CI[9];
+  matmul_tn(F, F, C);
+  inv3x3(C, CI);
+  const double Jm23 = std::pow(J, -2.0 / 3.0);
+  const double trC = C[0] + C[4] + C[8];
+  double dU, d2U, U;
+  vol_pot(m.volPotFlag, J, &dU, &d2U, &U);
+  const double kappa = kappa_of(m);
+  if (S0) {
+    const double id[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    for (int i = 0; i < 9; ++i)
+      S0[i] = m.G * Jm23 * (id[i] - trC / 3.0 * CI[i]) + kappa * J * dU * CI[i];
+  }
+  return 0.5 * m.G * (Jm23 * trC - 3.0) + kappa * U;
+}
+
+int j2d_update(const orc_material &m, const double *F, const double *Cpinv_n, double ep_n, double w_n, double X_n,
+               double dt, double *Cpinv_np1, double *ep_np1, double *gam, double *dam, double *S, double *L,
+               double *S0_out) {
+  double Sb[9], Lb[81], S0[9];
+  const int err = j2_update(m, F, Cpinv_n, ep_n, Cpinv_np1, ep_np1, gam, Sb, L ? Lb : nullptr);
+  if (err) return err;
+  double w = w_n, X = X_n, H = 0.0;
+  int damaged = 0;
+  const double Y = j2d_Y0(m, F, S0);
+  const double dmu = dt * m.mu;
+  if (m.mu > 0.0) {
+    double G, Gp;
+    weibull(m, Y, &G, &Gp);
+    const double g = G - X_n;
+    if (g > 0.0) {
+      damaged = 1;
+      w = w_n + dmu / (1.0 + dmu) * g;
+      X = std::max(X_n, (X_n + dmu * G) / (1.0 + dmu));
+      H = Gp;
+    }
+    const double wmax = m.w_max >= 1.0 ? kDamageThresh : m.w_max;
+    w = std::min(w, wmax);
+  }
+  if (dam) {
+    dam[0] = w;
+    dam[1] = X;
+    dam[2] = H;
+    dam[3] = damaged;
+  }
+  if (S0_out) std::memcpy(S0_out, S0, sizeof(S0));
+  const double om = 1.0 - w;
+  if (S)
+    for (int i = 0; i < 9; ++i) S[i] = om * Sb[i];
+  if (L) {
+    const double evo = damaged ? dmu * H / (1.0 + dmu) : 0.0;
+    for (int i = 0; i < 9; ++i)
+      for (int k = 0; k < 9; ++k) L[i * 9 + k] = om * Lb[i * 9 + k] - 0.5 * evo * (Sb[i] * S0[k] + S0[i] * Sb[k]);
+  }
+  return 0;
+}
+
 // ------------------------------------------------------------------ element contractions
 // compute_residual_vector, src/constitutive_model.cc:1628-1662
 void compute_residual_vector(double *f, int nne, const double *ST, double detJxW,
@@ -564,6 +651,16 @@ int orc_j2_update(const orc_material *mat, const double *F, const double *Cpinv_
   return j2_update(*mat, F, Cpinv_n, ep_n, Cpinv_np1, ep_np1, gam, S, L);
 }
 
+int orc_j2d_update(const orc_material *mat, const double *F, const double *Cpinv_n, double ep_n, double w_n,
+                   double X_n, double dt, double *Cpinv_np1, double *ep_np1, double *gam, double *dam_np1,
+                   double *S, double *L, double *S0) {
+  return j2d_update(*mat, F, Cpinv_n, ep_n, w_n, X_n, dt, Cpinv_np1, ep_np1, gam, dam_np1, S, L, S0);
+}
+
+void orc_weibull(const orc_material *mat, double Y, double *G, double *H) { weibull(*mat, Y, G, H); }
+
+double orc_j2d_Y0(const orc_material *mat, const double *F) { return j2d_Y0(*mat, F, nullptr); }
+
 // generate_local_dof_ids (+ _on_elem), src/generate_dof_ids.cc:69-171,306-412, without
 // periodic nodes, element DOFs or bounding elements.
 int orc_generate_local_dof_ids(int64_t n_nodes, int64_t n_elem, const int64_t *elem_ptr,
@@ -683,7 +780,6 @@ int orc_assemble(int64_t n_nodes, const double *X, int64_t n_elem, const int64_t
                  double *F_np1, const double *plastic_n, double *plastic_np1,
                  const double *scalar_n, double *scalar_np1, double *f, double *vals) {
   (void)n_nodes;
-  (void)dt;
   int64_t nqp_total = 0;
   for (int64_t e = 0; e < n_elem; ++e) nqp_total += nint_of(int(elem_ptr[e + 1] - elem_ptr[e]));
   const int64_t NQ = nqp_total;
@@ -741,13 +837,16 @@ int orc_assemble(int64_t n_nodes, const double *X, int64_t n_elem, const int64_t
         double *Lp = (pass == 1) ? L : nullptr;
         int merr = 0;
         if (mat.model == 4) {  // J2: integration stores the n+1 state, S/L from the state
-          double cn6[6], c6[6], ep_n = scalar_n[0 * NQ + q], ep1, gm;
+          double cn6[6], c6[6], ep_n = scalar_n[0 * NQ + q], ep1, gm, dam[4];
           for (int i = 0; i < 6; ++i) cn6[i] = plastic_n[i * NQ + q];
-          merr = j2_update(mat, Fs, cn6, ep_n, c6, &ep1, &gm, S, Lp);
+          // scalars (ep, gam, w, X, H, damaged); damage from (w_n, X_n) when mu > 0
+          merr = j2d_update(mat, Fs, cn6, ep_n, scalar_n[2 * NQ + q], scalar_n[3 * NQ + q], dt, c6, &ep1, &gm, dam,
+                            S, Lp, nullptr);
           if (pass == 0 && run_integration && !merr) {
             for (int i = 0; i < 6; ++i) plastic_np1[i * NQ + q] = c6[i];
             scalar_np1[0 * NQ + q] = ep1;
             scalar_np1[1 * NQ + q] = gm;
+            for (int i = 0; i < 4; ++i) scalar_np1[(2 + i) * NQ + q] = dam[i];
           }
         } else {
           merr = he_update(mat, Fs, S, Lp);

@@ -18,6 +18,7 @@ struct orc_material {  // same field order as pgf_material (include/pgfem3d_amd.
   double C11, C12, C44;
   double tol_M, tol_hardening;
   int32_t max_itr_M, max_itr_stag;
+  double mu, w_max, p1, p2, Yin;  // J2 continuum damage (off when mu <= 0)
 };
 
 // --- DOF numbering / sparsity (src/generate_dof_ids.cc, src/Psparse_ApAi.cc)
@@ -39,6 +40,17 @@ int orc_he_update(const orc_material *mat, const double *F, double *S, double *L
 int orc_j2_update(const orc_material *mat, const double *F, const double *Cpinv_n, double ep_n,
                   double *Cpinv_np1, double *ep_np1, double *gam, double *S, double *L);
 
+// J2 with continuum damage: state in also (w_n, X_n); out also dam_np1 = (w, X, H, damaged);
+// S = (1 - w) Sbar, L = (1 - w) Lbar - evo/2 (Sbar (x) S0 + S0 (x) Sbar) when damaged.
+// S0 (optional) = the undamaged elastic PK2 of the total F that drives the damage.
+int orc_j2d_update(const orc_material *mat, const double *F, const double *Cpinv_n, double ep_n, double w_n,
+                   double X_n, double dt, double *Cpinv_np1, double *ep_np1, double *gam, double *dam_np1,
+                   double *S, double *L, double *S0);
+// the Weibull damage function G(Y) and its derivative H(Y) = dG/dY of the J2 damage
+void orc_weibull(const orc_material *mat, double Y, double *G, double *H);
+// undamaged elastic energy Y0 of the total F that drives the J2 damage
+double orc_j2d_Y0(const orc_material *mat, const double *F);
+
 // --- element contraction of the reference (compute_stiffness_matrix,
 // src/constitutive_model.cc:1520-1616)
 void orc_stiffness_contract(double *lk, int nne, const double *ST, double detJxW, const double *Fr,
@@ -55,7 +67,8 @@ int orc_cp_update(const orc_material *mat, const double *P_grain, const double *
                   double *dMdF, int *iters);
 
 // --- whole mesh: fd_residuals_MP / stiffmat_fd_MP restated for one rank (TL, CM path)
-// state arrays are [comp][n_qp]; plastic = Cpinv (6 comps) for J2, scalar = (ep, gam).
+// state arrays are [comp][n_qp]; plastic = Cpinv (6 comps) for J2, scalar = (ep, gam, w, X, H,
+// damaged): 6 comps.
 int orc_assemble(int64_t n_nodes, const double *X, int64_t n_elem, const int64_t *elem_ptr,
                  const int64_t *elem_conn, const int32_t *elem_mat, const orc_material *mats,
                  const int64_t *id, int64_t ndof, const int32_t *Ap, const int32_t *Ai,

@@ -24,6 +24,7 @@ class OrcMaterial(C.Structure):
         ("C11", C.c_double), ("C12", C.c_double), ("C44", C.c_double),
         ("tol_M", C.c_double), ("tol_hardening", C.c_double),
         ("max_itr_M", C.c_int32), ("max_itr_stag", C.c_int32),
+        ("mu", C.c_double), ("w_max", C.c_double), ("p1", C.c_double), ("p2", C.c_double), ("Yin", C.c_double),
     ]
 
 
@@ -41,6 +42,13 @@ def lib():
             getattr(_lib, fn).restype = C.c_int
         _lib.orc_j2_update.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_void_p, C.c_void_p,
                                        C.c_void_p, C.c_void_p, C.c_void_p]
+        _lib.orc_j2d_update.restype = C.c_int
+        _lib.orc_j2d_update.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_double,
+                                        C.c_double] + [C.c_void_p] * 7
+        _lib.orc_weibull.restype = None
+        _lib.orc_weibull.argtypes = [C.c_void_p, C.c_double, C.c_void_p, C.c_void_p]
+        _lib.orc_j2d_Y0.restype = C.c_double
+        _lib.orc_j2d_Y0.argtypes = [C.c_void_p, C.c_void_p]
         _lib.orc_assemble.argtypes = [C.c_int64, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
                                       C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
                                       C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_int,
@@ -132,6 +140,35 @@ def j2_update(mat, F, Cpinv_n, ep_n, want_L=True):
                 gam=gam.value, rc=rc)
 
 
+def j2d_update(mat, F, Cpinv_n, ep_n, w_n, X_n, dt, want_L=True):
+    """J2 with continuum damage at one point: dict of S, L, S0, Cpinv, ep, gam, w, X, H,
+    damaged, rc."""
+    S = np.zeros(9)
+    S0 = np.zeros(9)
+    L = np.zeros(81) if want_L else None
+    c6 = np.zeros(6)
+    dam = np.zeros(4)
+    ep1, gam = C.c_double(), C.c_double()
+    F = np.ascontiguousarray(F, dtype=np.float64).reshape(9)
+    cn = np.ascontiguousarray(Cpinv_n, dtype=np.float64)
+    om = material(mat)
+    rc = lib().orc_j2d_update(C.byref(om), _p(F), _p(cn), float(ep_n), float(w_n), float(X_n), float(dt), _p(c6),
+                              C.byref(ep1), C.byref(gam), _p(dam), _p(S), _p(L), _p(S0))
+    return dict(S=S.reshape(3, 3), L=(L.reshape(3, 3, 3, 3) if want_L else None), S0=S0.reshape(3, 3), Cpinv=c6,
+                ep=ep1.value, gam=gam.value, w=dam[0], X=dam[1], H=dam[2], damaged=int(dam[3]), rc=rc)
+
+
+def weibull(mat, Y):
+    """(G(Y), H(Y)) of the J2 damage's Weibull law."""
+    g, h = C.c_double(), C.c_double()
+    lib().orc_weibull(C.byref(material(mat)), float(Y), C.byref(g), C.byref(h))
+    return g.value, h.value
+
+
+def j2d_Y0(mat, F):
+    return lib().orc_j2d_Y0(C.byref(material(mat)), _p(np.ascontiguousarray(F, dtype=np.float64).reshape(9)))
+
+
 class OracleState:
     """SoA state arrays [comp][n_qp] in mesh QP order, as the oracle's assembly expects."""
 
@@ -142,8 +179,9 @@ class OracleState:
         c6 = np.array([1.0, 1, 1, 0, 0, 0])
         self.P_n = np.ascontiguousarray(np.repeat(c6[:, None], n_qp, axis=1))
         self.P_np1 = self.P_n.copy()
-        self.S_n = np.zeros((2, n_qp))
-        self.S_np1 = np.zeros((2, n_qp))
+        # scalars: ep, gam, then the J2 damage variables w, X, H, damaged
+        self.S_n = np.zeros((6, n_qp))
+        self.S_np1 = np.zeros((6, n_qp))
 
     def commit(self):
         self.P_n[:] = self.P_np1

@@ -157,9 +157,10 @@ struct pgf_handle_s {
   hipStream_t own_stream = nullptr, stream = nullptr;
   int64_t n_nodes = 0, ndof = 0, nnz = 0, n_qp = 0, n_elem = 0;
   int32_t n_colors = 0;
-  // three-field (CM3F) mode: per element in device order, d_tf holds [10][n_elem] = V at
-  // n+1, n, n-1; P at n+1, n, n-1; dV, ddV, dP, ddP (ThreeFieldVariables,
-  // include/PGFem3D_data_structure.h:130-235); d_tfbuf the element data of the last residual
+  // three-field (CM3F) mode: per element in device order, d_tf holds [kTfSlots][n_elem] = V
+  // at n+1, n, n-1; P at n+1, n, n-1; dV, ddV, dP, ddP (ThreeFieldVariables,
+  // include/PGFem3D_data_structure.h:130-235); then the last update's full ddV, ddP and the
+  // dV, dP its line-search trials start from; d_tfbuf the element data of the last residual
   // pass, [n_elem][kTfBuf]
   bool tf = false;
   double *d_tf = nullptr, *d_tfbuf = nullptr;
@@ -172,8 +173,11 @@ struct pgf_handle_s {
   // state, SoA [comp][n_qp]; index 0 = n+1, 1 = n, 2 = n-1
   double *d_F[3] = {nullptr, nullptr, nullptr};
   double *d_P[3] = {nullptr, nullptr, nullptr};   // J2: Cp^-1 (6)
-  double *d_Sc[3] = {nullptr, nullptr, nullptr};  // J2: ep, gam
+  double *d_Sc[3] = {nullptr, nullptr, nullptr};  // J2: ep, gam (+ w, X, H, damaged); CP: g, lambda
+  int nsc = 2;       // scalar components per QP of d_Sc
+  bool j2_damage = false;  // a J2 material with continuum damage (mu > 0)
   int32_t *d_err = nullptr;
+  double *d_red = nullptr;  // one scalar of a device reduction (pgf_subdivision_param)
   // crystal plasticity
   bool has_cp = false;
   pgf::CPMaterial *d_cpmat = nullptr;
@@ -234,7 +238,7 @@ int set_identity_state(pgf_handle h) {
     for (int i = 0; i < 9; ++i) pgf::launch_fill(h->d_F[t] + i * NQ, NQ, (i % 4 == 0) ? 1.0 : 0.0, h->stream);
     if (h->has_j2) {
       for (int I = 0; I < 6; ++I) pgf::launch_fill(h->d_P[t] + I * NQ, NQ, I < 3 ? 1.0 : 0.0, h->stream);
-      pgf::launch_fill(h->d_Sc[t], 2 * NQ, 0.0, h->stream);
+      pgf::launch_fill(h->d_Sc[t], h->nsc * NQ, 0.0, h->stream);
     }
     if (h->has_cp) {  // pF = I, g = g0, lambda = 0
       for (int i = 0; i < 9; ++i) pgf::launch_fill(h->d_P[t] + i * NQ, NQ, (i % 4 == 0) ? 1.0 : 0.0, h->stream);
@@ -343,6 +347,7 @@ pgf::ElemArgs base_args(pgf_handle h, const pgf_nodal_input *in) {
   a.P_np1 = h->d_P[0];
   a.Sc_n = h->d_Sc[1];
   a.Sc_np1 = h->d_Sc[0];
+  a.dt = h->dt;
   a.err = h->d_err;
   return a;
 }
@@ -420,7 +425,9 @@ void build_gather(int64_t n_nodes, const int32_t *dofid, std::vector<GatherGroup
 // One pass over all elements in the pass schedule.
 // Every launch of one pass, in order, on the handle's stream (also what a captured graph
 // holds; the error counter is cleared by a kernel, not a memset node).
-int32_t enqueue_pass(pgf_handle h, const pgf_nodal_input *in, int mode, double *f, double *vals) {
+// stage: the gather mode's staging blocks (stage_size doubles); nullptr selects the coloured
+// mode.
+int32_t enqueue_pass(pgf_handle h, const pgf_nodal_input *in, int mode, double *f, double *vals, double *stage) {
   pgf::launch_fill_i32(h->d_err, 1, 0, h->stream);
   pgf::ElemArgs a = base_args(h, in);
   a.f = f;
@@ -429,12 +436,12 @@ int32_t enqueue_pass(pgf_handle h, const pgf_nodal_input *in, int mode, double *
   cp_prepass(h, in, mode);
   // gather mode: each block in one launch (all elements), then one gather over the nodes;
   // coloured mode: colour by colour, every block's elements of the colour
-  const bool gather = h->assembly == PGF_ASSEMBLY_GATHER;
+  const bool gather = stage != nullptr;
   double *fout = f, *vout = vals;
   if (gather) {
     a.f = nullptr;
     a.vals = nullptr;
-    a.stage = h->d_stage;
+    a.stage = stage;
     f = nullptr;
     vals = nullptr;
   }
@@ -464,7 +471,7 @@ int32_t enqueue_pass(pgf_handle h, const pgf_nodal_input *in, int mode, double *
         ce.f = f;
         ce.vals = vals;
         ce.soff = b.d_soff;
-        ce.stage = gather ? h->d_stage : nullptr;
+        ce.stage = gather ? stage : nullptr;
         ce.elist = gather ? nullptr : b.d_clist;
         ce.err = h->d_err;
         pgf::launch_cp_elem(b.nne, mode, ce, e0, e1, h->stream);
@@ -484,7 +491,7 @@ int32_t enqueue_pass(pgf_handle h, const pgf_nodal_input *in, int mode, double *
           a.tf_dP = h->d_tf + 8 * NE + b.e_base;
           a.tf_buf = h->d_tfbuf + b.e_base * pgf::kTfBuf;
         }
-        pgf::launch_elem(b.nne, mode, a, e0, e1, h->stream);
+        pgf::launch_elem(b.nne, mode, a, e0, e1, h->stream, h->j2_damage && b.model == PGF_MODEL_J2_PLASTICITY);
       }
       ++launches;
     }
@@ -504,7 +511,7 @@ int32_t enqueue_pass(pgf_handle h, const pgf_nodal_input *in, int mode, double *
     ca.vars_np1 = h->d_cv[0];
     ca.f = f;
     ca.vals = vals;
-    ca.stage = gather ? h->d_stage : nullptr;
+    ca.stage = gather ? stage : nullptr;
     const int32_t ncseg = gather ? 1 : h->n_ccolors;
     ca.err = h->d_err;
     for (int32_t c = 0; c < ncseg; ++c)
@@ -529,7 +536,7 @@ int32_t enqueue_pass(pgf_handle h, const pgf_nodal_input *in, int mode, double *
     g.dofid = h->d_dofid;
     g.row_start = h->d_row_start;
     g.rowlen = h->d_rowlen;
-    g.stage = h->d_stage;
+    g.stage = stage;
     g.f = fout;
     g.vals = vout;
     g.n_nodes = h->n_nodes;
@@ -539,6 +546,19 @@ int32_t enqueue_pass(pgf_handle h, const pgf_nodal_input *in, int mode, double *
   }
   return launches;
 }
+
+// Finish the handle's work and forget its captured pass: for setters that free or replace a
+// buffer whose address the graph's kernels hold.
+int drop_graph(pgf_handle h) {
+  if (hipStreamSynchronize(h->stream) != hipSuccess) return PGF_EHIP;
+  if (h->gexec) {
+    hipGraphExecDestroy(h->gexec);
+    h->gexec = nullptr;
+  }
+  return PGF_OK;
+}
+
+double *pass_stage(pgf_handle h) { return h->assembly == PGF_ASSEMBLY_GATHER ? h->d_stage : nullptr; }
 
 int run_pass(pgf_handle h, const pgf_nodal_input *in, int mode, double *f, double *vals) {
   if (!in || !in->r) return PGF_EINVAL;
@@ -556,7 +576,7 @@ int run_pass(pgf_handle h, const pgf_nodal_input *in, int mode, double *f, doubl
     }
     hipGraph_t g = nullptr;
     if (hipStreamBeginCapture(h->stream, hipStreamCaptureModeRelaxed) != hipSuccess) return PGF_EHIP;
-    h->glaunches = enqueue_pass(h, in, mode, f, vals);
+    h->glaunches = enqueue_pass(h, in, mode, f, vals, pass_stage(h));
     if (hipStreamEndCapture(h->stream, &g) != hipSuccess) return PGF_EHIP;
     const hipError_t ie = hipGraphInstantiate(&h->gexec, g, nullptr, nullptr, 0);
     hipGraphDestroy(g);
@@ -571,7 +591,7 @@ int run_pass(pgf_handle h, const pgf_nodal_input *in, int mode, double *f, doubl
     if (hipGraphLaunch(h->gexec, h->stream) != hipSuccess) return PGF_EHIP;
     h->last_launches = h->glaunches;
   } else {
-    h->last_launches = enqueue_pass(h, in, mode, f, vals);
+    h->last_launches = enqueue_pass(h, in, mode, f, vals, pass_stage(h));
   }
   if (h->profiling) hipEventRecord(h->ev1, h->stream);
   if (hipGetLastError() != hipSuccess) return PGF_EHIP;
@@ -603,7 +623,7 @@ int field_buffer(pgf_handle h, int32_t field, double **buf, int32_t *ncomp) {
     case PGF_STATE_SCALAR_NP1: case PGF_STATE_SCALAR_N: case PGF_STATE_SCALAR_NM1:
       if (!h->has_j2 && !h->has_cp) return PGF_OK;
       *buf = h->d_Sc[field - PGF_STATE_SCALAR_NP1];
-      *ncomp = 2;
+      *ncomp = h->nsc;
       return PGF_OK;
     case PGF_STATE_TAU: case PGF_STATE_TAU_N:
       if (!h->has_cp) return PGF_OK;
@@ -645,13 +665,18 @@ int pgf_create(const pgf_mesh *mesh, const int64_t *id, int64_t ndof, const int3
     for (int64_t k = m.elem_ptr[e]; k < m.elem_ptr[e + 1]; ++k)
       if (m.elem_conn[k] < 0 || m.elem_conn[k] >= m.n_nodes) return PGF_EINVAL;
   }
-  int n_cp = 0, n_j2 = 0, cp_index = -1;
+  int n_cp = 0, n_j2 = 0, n_dmg = 0, cp_index = -1;
   for (int32_t i = 0; i < n_mat; ++i) {
     const int md = mats[i].model;
     if (md == PGF_MODEL_HYPER_ELASTICITY) {
       if (mats[i].m01 != 0.0 || mats[i].devPotFlag != 1) return PGF_ENOTSUP;  // neo-Hookean only
     } else if (md == PGF_MODEL_J2_PLASTICITY) {
       ++n_j2;
+      if (mats[i].mu > 0.0) {  // Weibull law: positive threshold and scales
+        if (!(mats[i].Yin > 0.0) || !(mats[i].p1 > 0.0) || !(mats[i].p2 > 0.0) || !(mats[i].w_max > 0.0))
+          return PGF_EINVAL;
+        ++n_dmg;
+      }
     } else if (md == PGF_MODEL_CRYSTAL_PLASTICITY) {
       ++n_cp;
       cp_index = i;
@@ -724,6 +749,8 @@ int pgf_create(const pgf_mesh *mesh, const int64_t *id, int64_t ndof, const int3
     if (md == PGF_MODEL_J2_PLASTICITY) h->has_j2 = true;
     if (md == PGF_MODEL_CRYSTAL_PLASTICITY) h->has_cp = true;
   }
+  h->j2_damage = h->has_j2 && n_dmg > 0;
+  h->nsc = h->j2_damage ? 6 : 2;
   // crystal-plasticity blocks first: their QPs then form one contiguous range that the
   // element-type-agnostic QP kernels (local Newton, tangent) cover in single launches
   std::vector<int> border(keys.size());
@@ -917,6 +944,11 @@ int pgf_create(const pgf_mesh *mesh, const int64_t *id, int64_t ndof, const int3
     dm[i].kappa = mats[i].E / (3.0 * (1.0 - 2.0 * mats[i].nu));
     dm[i].k0 = mats[i].k0;
     dm[i].hp = mats[i].hp;
+    dm[i].mu = mats[i].mu;
+    dm[i].wmax = mats[i].w_max;
+    dm[i].p1 = mats[i].p1;
+    dm[i].p2 = mats[i].p2;
+    dm[i].Yin = mats[i].Yin;
   }
   if ((rc = pgf::upload(&h->d_mats, dm.data(), dm.size()))) return fail(rc);
   if ((rc = pgf::upload(&h->d_X, m.X, size_t(m.n_nodes) * 3))) return fail(rc);
@@ -945,7 +977,7 @@ int pgf_create(const pgf_mesh *mesh, const int64_t *id, int64_t ndof, const int3
     if ((rc = pgf::dalloc(&h->d_F[t], size_t(h->n_qp) * 9))) return fail(rc);
     if (h->has_j2 || h->has_cp) {
       if ((rc = pgf::dalloc(&h->d_P[t], size_t(h->n_qp) * (h->has_cp ? 9 : 6)))) return fail(rc);
-      if ((rc = pgf::dalloc(&h->d_Sc[t], size_t(h->n_qp) * 2))) return fail(rc);
+      if ((rc = pgf::dalloc(&h->d_Sc[t], size_t(h->n_qp) * h->nsc))) return fail(rc);
     }
   }
   if (h->has_cp) {
@@ -987,6 +1019,7 @@ int pgf_create(const pgf_mesh *mesh, const int64_t *id, int64_t ndof, const int3
     if ((rc = pgf_set_orientations(h, 1, I9, nullptr))) return fail(rc);
   }
   if ((rc = pgf::dalloc(&h->d_err, 1))) return fail(rc);
+  if ((rc = pgf::dalloc(&h->d_red, 1))) return fail(rc);
   if ((rc = set_identity_state(h))) return fail(rc);
   *out = h;
   return PGF_OK;
@@ -1047,6 +1080,7 @@ int pgf_destroy(pgf_handle h) {
     hipFree(h->d_Sc[t]);
   }
   hipFree(h->d_err);
+  hipFree(h->d_red);
   if (h->ev0) hipEventDestroy(h->ev0);
   if (h->ev1) hipEventDestroy(h->ev1);
   if (h->own_stream) hipStreamDestroy(h->own_stream);
@@ -1115,6 +1149,24 @@ int pgf_residual_tangent(pgf_handle h, const pgf_nodal_input *in, double dt, dou
   return run_pass(h, in, pgf::MODE_INTEGRATE | pgf::MODE_RESIDUAL | pgf::MODE_TANGENT, f, vals);
 }
 
+int pgf_subdivision_param(pgf_handle h, double dt, double *alpha) {
+  if (!h || !alpha) return PGF_EINVAL;
+  if (!(dt > 0.0)) dt = h->dt;
+  *alpha = 0.0;
+  if (!h->has_cp && !h->j2_damage) return PGF_OK;  // the hyperelastic and undamaged models give 0
+  pgf::launch_fill(h->d_red, 1, 0.0, h->stream);
+  auto *out = reinterpret_cast<unsigned long long *>(h->d_red);
+  if (h->has_cp)  // CP_PARAM::get_subdiv_param: dt sum_a |gamma_dot_a(n+1)| / MAX_D_GAMMA (0.005)
+    pgf::launch_rate_max(h->d_gdot[0], 12, h->n_qp, h->cp_nq, dt / 0.005, nullptr, nullptr, 0.0, out, h->stream);
+  if (h->j2_damage)  // CM_J2P_PARAM::get_subdiv_param: (w(n+1) - w(n)) / DELTA_W_MAX (0.05)
+    pgf::launch_rate_max(nullptr, 0, 0, h->n_qp, 0.0, h->d_Sc[0] + 2 * h->n_qp, h->d_Sc[1] + 2 * h->n_qp,
+                         1.0 / 0.05, out, h->stream);
+  if (hipGetLastError() != hipSuccess) return PGF_EHIP;
+  if (hipMemcpyAsync(alpha, h->d_red, sizeof(double), hipMemcpyDeviceToHost, h->stream) != hipSuccess)
+    return PGF_EHIP;
+  return hipStreamSynchronize(h->stream) == hipSuccess ? PGF_OK : PGF_EHIP;
+}
+
 int pgf_commit_step(pgf_handle h) {
   if (!h) return PGF_EINVAL;
   const int64_t NQ = h->n_qp;
@@ -1124,8 +1176,8 @@ int pgf_commit_step(pgf_handle h) {
     const int np = h->has_cp ? 9 : 6;
     pgf::launch_copy(h->d_P[2], h->d_P[1], np * NQ, h->stream);
     pgf::launch_copy(h->d_P[1], h->d_P[0], np * NQ, h->stream);
-    pgf::launch_copy(h->d_Sc[2], h->d_Sc[1], 2 * NQ, h->stream);
-    pgf::launch_copy(h->d_Sc[1], h->d_Sc[0], 2 * NQ, h->stream);
+    pgf::launch_copy(h->d_Sc[2], h->d_Sc[1], h->nsc * NQ, h->stream);
+    pgf::launch_copy(h->d_Sc[1], h->d_Sc[0], h->nsc * NQ, h->stream);
   }
   if (h->has_cp) {  // tau_n, gamma_dot_n (src/plasticity_model.cc:174-192)
     pgf::launch_copy(h->d_tau[1], h->d_tau[0], 12 * NQ, h->stream);
@@ -1144,7 +1196,7 @@ int pgf_commit_step(pgf_handle h) {
       pgf::launch_copy(nm1, n, NE, h->stream);
       pgf::launch_copy(n, np1, NE, h->stream);
     }
-    pgf::launch_fill(t + 6 * NE, 4 * NE, 0.0, h->stream);
+    pgf::launch_fill(t + 6 * NE, (pgf::kTfSlots - 6) * NE, 0.0, h->stream);
   }
   return hipStreamSynchronize(h->stream) == hipSuccess ? PGF_OK : PGF_EHIP;
 }
@@ -1155,25 +1207,21 @@ int pgf_reset_step(pgf_handle h) {
   pgf::launch_copy(h->d_F[0], h->d_F[1], 9 * NQ, h->stream);
   if (h->has_j2 || h->has_cp) {
     pgf::launch_copy(h->d_P[0], h->d_P[1], (h->has_cp ? 9 : 6) * NQ, h->stream);
-    pgf::launch_copy(h->d_Sc[0], h->d_Sc[1], 2 * NQ, h->stream);
+    pgf::launch_copy(h->d_Sc[0], h->d_Sc[1], h->nsc * NQ, h->stream);
   }
   if (h->has_cp) {
     pgf::launch_copy(h->d_tau[0], h->d_tau[1], 12 * NQ, h->stream);
     pgf::launch_copy(h->d_gdot[0], h->d_gdot[1], 12 * NQ, h->stream);
   }
   if (h->n_cqp) pgf::launch_copy(h->d_cv[0], h->d_cv[1], 2 * h->n_cqp, h->stream);
-  if (h->tf) pgf::launch_fill(h->d_tf + 6 * h->n_elem, 4 * h->n_elem, 0.0, h->stream);  // tf.reset()
+  if (h->tf) pgf::launch_fill(h->d_tf + 6 * h->n_elem, (pgf::kTfSlots - 6) * h->n_elem, 0.0, h->stream);  // tf.reset()
   return hipStreamSynchronize(h->stream) == hipSuccess ? PGF_OK : PGF_EHIP;
 }
 
 // ------------------------------------------------------------------ three-field (CM3F)
 int pgf_set_three_field(pgf_handle h, int32_t on) {
   if (!h) return PGF_EINVAL;
-  if (hipStreamSynchronize(h->stream) != hipSuccess) return PGF_EHIP;
-  if (h->gexec) {  // a captured pass holds the old kernels
-    hipGraphExecDestroy(h->gexec);
-    h->gexec = nullptr;
-  }
+  if (int rc = drop_graph(h)) return rc;  // a captured pass holds the old kernels
   if (!on) {
     hipFree(h->d_tf);
     hipFree(h->d_tfbuf);
@@ -1185,7 +1233,7 @@ int pgf_set_three_field(pgf_handle h, int32_t on) {
     if (b.model != PGF_MODEL_HYPER_ELASTICITY) return PGF_ENOTSUP;
   if (h->tf) return PGF_OK;
   const int64_t NE = h->n_elem;
-  int rc = pgf::dalloc(&h->d_tf, size_t(10 * NE));
+  int rc = pgf::dalloc(&h->d_tf, size_t(pgf::kTfSlots * NE));
   if (!rc) rc = pgf::dalloc(&h->d_tfbuf, size_t(NE * pgf::kTfBuf));
   if (rc) {
     hipFree(h->d_tf);
@@ -1194,7 +1242,7 @@ int pgf_set_three_field(pgf_handle h, int32_t on) {
   }
   // ThreeFieldVariables::construct: V = 1, P = 0, increments 0
   pgf::launch_fill(h->d_tf, 3 * NE, 1.0, h->stream);
-  pgf::launch_fill(h->d_tf + 3 * NE, 7 * NE, 0.0, h->stream);
+  pgf::launch_fill(h->d_tf + 3 * NE, (pgf::kTfSlots - 3) * NE, 0.0, h->stream);
   pgf::launch_fill(h->d_tfbuf, NE * pgf::kTfBuf, 0.0, h->stream);
   h->tf = true;
   return hipStreamSynchronize(h->stream) == hipSuccess ? PGF_OK : PGF_EHIP;
@@ -1217,6 +1265,7 @@ pgf::TfArgs tf_args(pgf_handle h, const Block &b) {
 int pgf_tf_initial(pgf_handle h, const pgf_nodal_input *in_n, const pgf_nodal_input *in_nm1) {
   if (!h || !h->tf || !in_n || !in_n->r) return PGF_EINVAL;
   const int64_t NE = h->n_elem;
+  pgf::launch_fill_i32(h->d_err, 1, 0, h->stream);
   for (int s = 0; s < 2; ++s) {
     const pgf_nodal_input *in = (s == 1 && in_nm1) ? in_nm1 : in_n;
     if (!in->r) return PGF_EINVAL;
@@ -1229,12 +1278,21 @@ int pgf_tf_initial(pgf_handle h, const pgf_nodal_input *in_n, const pgf_nodal_in
       t.V = h->d_tf + (s == 0 ? 1 : 2) * NE + b.e_base;      // V(n) / V(n-1)
       t.P = h->d_tf + (s == 0 ? 4 : 5) * NE + b.e_base;      // P(n) / P(n-1)
       pgf::launch_tf_initial(b.nne, t, h->stream);
+      // the model's F(n) / F(n-1) from the same displacements (set_F in
+      // compute_cm_initial_conditions, src/constitutive_model.cc:4097-4105)
+      pgf::CPArgs c = cp_args(h, b, in);
+      c.F_np1 = h->d_F[s == 0 ? 1 : 2];
+      pgf::launch_cp_kinematics(b.nne, c, h->stream);
     }
   }
   pgf::launch_copy(h->d_tf, h->d_tf + NE, NE, h->stream);           // V(n+1) = V(n)
   pgf::launch_copy(h->d_tf + 3 * NE, h->d_tf + 4 * NE, NE, h->stream);  // P(n+1) = P(n)
-  pgf::launch_fill(h->d_tf + 6 * NE, 4 * NE, 0.0, h->stream);
-  return hipStreamSynchronize(h->stream) == hipSuccess ? PGF_OK : PGF_EHIP;
+  pgf::launch_fill(h->d_tf + 6 * NE, (pgf::kTfSlots - 6) * NE, 0.0, h->stream);
+  if (hipGetLastError() != hipSuccess) return PGF_EHIP;
+  int32_t err = 0;
+  if (hipMemcpyAsync(&err, h->d_err, sizeof(int32_t), hipMemcpyDeviceToHost, h->stream) != hipSuccess)
+    return PGF_EHIP;
+  return hipStreamSynchronize(h->stream) == hipSuccess ? err : PGF_EHIP;
 }
 
 int pgf_tf_update(pgf_handle h, const double *du) {
@@ -1243,18 +1301,29 @@ int pgf_tf_update(pgf_handle h, const double *du) {
   for (auto &b : h->blocks) {
     pgf::TfArgs t = tf_args(h, b);
     t.du = du;
-    t.ddV = h->d_tf + 7 * NE + b.e_base;
-    t.ddP = h->d_tf + 9 * NE + b.e_base;
+    t.ddV = h->d_tf + 10 * NE + b.e_base;  // the full increments, kept for every trial
+    t.ddP = h->d_tf + 11 * NE + b.e_base;
     pgf::launch_tf_update(b.nne, t, h->stream);
   }
+  double *tf = h->d_tf;
+  pgf::launch_copy(tf + 7 * NE, tf + 10 * NE, NE, h->stream);   // ddV, ddP as computed
+  pgf::launch_copy(tf + 9 * NE, tf + 11 * NE, NE, h->stream);
+  pgf::launch_copy(tf + 12 * NE, tf + 6 * NE, NE, h->stream);   // dV, dP the trials start from
+  pgf::launch_copy(tf + 13 * NE, tf + 8 * NE, NE, h->stream);
   return hipStreamSynchronize(h->stream) == hipSuccess ? PGF_OK : PGF_EHIP;
 }
 
 int pgf_tf_accept(pgf_handle h, double gamma) {
   if (!h || !h->tf) return PGF_EINVAL;
   const int64_t NE = h->n_elem;
-  pgf::launch_axpy(h->d_tf + 6 * NE, h->d_tf + 7 * NE, gamma, NE, h->stream);  // dV += gamma ddV
-  pgf::launch_axpy(h->d_tf + 8 * NE, h->d_tf + 9 * NE, gamma, NE, h->stream);  // dP += gamma ddP
+  double *tf = h->d_tf;
+  for (int v = 0; v < 2; ++v) {  // V, then P: ddX = gamma ddX_full; dX = dX(start) + ddX
+    double *dX = tf + (6 + 2 * v) * NE, *ddX = dX + NE, *full = tf + (10 + v) * NE, *start = tf + (12 + v) * NE;
+    pgf::launch_fill(ddX, NE, 0.0, h->stream);
+    pgf::launch_axpy(ddX, full, gamma, NE, h->stream);  // 0 + gamma x: the rounded product
+    pgf::launch_copy(dX, start, NE, h->stream);
+    pgf::launch_axpy(dX, ddX, 1.0, NE, h->stream);
+  }
   return hipStreamSynchronize(h->stream) == hipSuccess ? PGF_OK : PGF_EHIP;
 }
 
@@ -1387,30 +1456,40 @@ int pgf_state_device_ptr(pgf_handle h, int32_t field, double **ptr) {
 int pgf_set_orientations(pgf_handle h, int32_t n_grains, const double *R, const int32_t *elem_grain) {
   if (!h || n_grains < 1 || !R) return PGF_EINVAL;
   if (!h->has_cp) return PGF_ENOTSUP;
+  // every host array is built and checked before device memory changes
   std::vector<pgf::DevGrain> g(n_grains);
   std::vector<float> q(size_t(n_grains) * 4);
   for (int32_t k = 0; k < n_grains; ++k) {
     pgf::make_grain(R + 9 * k, h->cp_mat_host[0], h->cp_mat_host[1], h->cp_mat_host[2], g[k]);
     pgf::rot_to_quat(R + 9 * k, q.data() + 4 * k);
   }
+  std::vector<std::vector<int32_t>> egs(h->blocks.size()), qgs(h->blocks.size());
+  if (elem_grain)
+    for (size_t bi = 0; bi < h->blocks.size(); ++bi) {
+      const Block &b = h->blocks[bi];
+      if (b.model != PGF_MODEL_CRYSTAL_PLASTICITY) continue;
+      egs[bi].resize(b.ne);
+      qgs[bi].resize(b.ne * b.nqp);
+      for (int64_t k = 0; k < b.ne; ++k) {
+        const int32_t gk = elem_grain[b.elems[k]];
+        if (gk < 0 || gk >= n_grains) return PGF_EINVAL;
+        egs[bi][k] = gk;
+        for (int ip = 0; ip < b.nqp; ++ip) qgs[bi][k * b.nqp + ip] = gk;
+      }
+    }
+  int rc = drop_graph(h);  // a captured pass holds the grain table's address
+  if (rc) return rc;
   hipFree(h->d_grains);
   hipFree(h->d_gquat);
+  h->d_grains = nullptr;
   h->d_gquat = nullptr;
-  int rc = pgf::upload(&h->d_grains, g.data(), g.size());
-  if (rc) return rc;
+  if ((rc = pgf::upload(&h->d_grains, g.data(), g.size()))) return rc;
   if ((rc = pgf::upload(&h->d_gquat, q.data(), q.size()))) return rc;
-  if (!elem_grain) return PGF_OK;
-  for (auto &b : h->blocks) {
-    if (b.model != PGF_MODEL_CRYSTAL_PLASTICITY) continue;
-    std::vector<int32_t> eg(b.ne), qg(b.ne * b.nqp);
-    for (int64_t k = 0; k < b.ne; ++k) {
-      const int32_t gk = elem_grain[b.elems[k]];
-      if (gk < 0 || gk >= n_grains) return PGF_EINVAL;
-      eg[k] = gk;
-      for (int ip = 0; ip < b.nqp; ++ip) qg[k * b.nqp + ip] = gk;
-    }
-    if (hipMemcpy(b.d_egrain, eg.data(), eg.size() * sizeof(int32_t), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(b.d_qgrain, qg.data(), qg.size() * sizeof(int32_t), hipMemcpyHostToDevice) != hipSuccess)
+  for (size_t bi = 0; bi < h->blocks.size(); ++bi) {
+    if (egs[bi].empty()) continue;
+    const Block &b = h->blocks[bi];
+    if (hipMemcpy(b.d_egrain, egs[bi].data(), egs[bi].size() * sizeof(int32_t), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(b.d_qgrain, qgs[bi].data(), qgs[bi].size() * sizeof(int32_t), hipMemcpyHostToDevice) != hipSuccess)
       return PGF_EHIP;
   }
   return PGF_OK;
@@ -1424,16 +1503,22 @@ int pgf_set_cohesive_props(pgf_handle h, int32_t n_props, const pgf_cohesive *pr
     if (!(props[i].chi_c > 0.0) || !(props[i].alpha != 0.0)) return PGF_EINVAL;
     dp[i] = {props[i].sig_c, props[i].chi_c, props[i].beta, props[i].alpha};
   }
-  for (auto &cb : h->cblocks) {
-    std::vector<int32_t> mat(cb.nc);
+  std::vector<std::vector<int32_t>> mats(h->cblocks.size());
+  for (size_t bi = 0; bi < h->cblocks.size(); ++bi) {
+    const pgf::CohBlock &cb = h->cblocks[bi];
+    mats[bi].resize(cb.nc);
     for (int64_t k = 0; k < cb.nc; ++k) {
       const int32_t mi = coel_mat ? coel_mat[cb.elems[k]] : 0;
       if (mi < 0 || mi >= n_props) return PGF_EINVAL;
-      mat[k] = mi;
+      mats[bi][k] = mi;
     }
-    if (hipMemcpy(cb.d_mat, mat.data(), mat.size() * sizeof(int32_t), hipMemcpyHostToDevice) != hipSuccess)
-      return PGF_EHIP;
   }
+  int rc = drop_graph(h);  // a captured pass holds the property table's address
+  if (rc) return rc;
+  for (size_t bi = 0; bi < h->cblocks.size(); ++bi)
+    if (hipMemcpy(h->cblocks[bi].d_mat, mats[bi].data(), mats[bi].size() * sizeof(int32_t), hipMemcpyHostToDevice) !=
+        hipSuccess)
+      return PGF_EHIP;
   hipFree(h->d_cprops);
   h->d_cprops = nullptr;
   return pgf::upload(&h->d_cprops, dp.data(), dp.size());

@@ -32,6 +32,8 @@ struct DevMaterial {
   double G;       // shear modulus of the deviatoric part (neo-Hookean: 2*m10)
   double kappa;   // bulk modulus E/(3(1-2nu))
   double k0, hp;  // J2
+  // J2 continuum damage (viscosity mu, cap wmax, Weibull p1, p2, threshold Yin); mu <= 0: off
+  double mu, wmax, p1, p2, Yin;
 };
 
 // Everything one element launch reads (device pointers).
@@ -54,9 +56,9 @@ struct ElemArgs {
   double *F_np1;            // 9 comps
   const double *P_n;        // plastic n   (J2: Cp^-1, 6 comps)
   double *P_np1;            // plastic n+1
-  const double *Sc_n;       // scalars n   (J2: ep, gam)
+  const double *Sc_n;       // scalars n   (J2: ep, gam; with damage also w, X, H, damaged)
   double *Sc_np1;           // scalars n+1
-  double *cauchy;           // 6 comps (written by integration)
+  double dt;                // time step (J2 damage viscosity)
   // outputs: coloured mode adds into f / vals through off; gather mode (stage != null)
   // writes row (a, b) of element e to stage[soff[e*nne + a] + b*(3nne+1) ...], residual last
   double *f;                // [ndof]
@@ -71,6 +73,9 @@ struct ElemArgs {
   double *tf_buf;
 };
 constexpr int kTfBuf = 34;
+// per-element three-field slots of the handle: V(n+1), V(n), V(n-1), P(n+1), P(n), P(n-1), dV,
+// ddV, dP, ddP, the full ddV and ddP of the last update, the dV and dP its trials start from
+constexpr int kTfSlots = 14;
 
 // Element update and initial conditions of the three-field formulation, one thread per
 // element of a block (block-local arrays).
@@ -195,8 +200,9 @@ struct CohArgs {
 };
 void launch_coh_elem(int nn, int mode, const CohArgs &a, int64_t e_begin, int64_t e_end, hipStream_t s);
 
-void launch_elem(int nne, int mode, const ElemArgs &a, int64_t e_begin, int64_t e_end,
-                 hipStream_t s);
+// damage: the block's J2 points carry continuum damage (state scalars w, X, H, damaged)
+void launch_elem(int nne, int mode, const ElemArgs &a, int64_t e_begin, int64_t e_end, hipStream_t s,
+                 bool damage = false);
 
 // Gather assembly: node n owns the staging blocks [inc_ptr[n], inc_ptr[n+1]) (one per
 // incident element, in a fixed order); block k starts at inc_soff[k], comes from an element
@@ -228,6 +234,8 @@ void launch_tf_initial(int nne, const TfArgs &t, hipStream_t s);
 void launch_axpy(double *y, const double *x, double g, int64_t n, hipStream_t s);  // y += g x
 void launch_fill(double *p, int64_t n, double v, hipStream_t s);
 void launch_fill_i32(int32_t *p, int64_t n, int32_t v, hipStream_t s);
+void launch_rate_max(const double *a, int ncomp, int64_t stride, int64_t n, double scale, const double *d1,
+                     const double *d0, double scale_d, unsigned long long *out, hipStream_t s);
 void launch_copy(double *dst, const double *src, int64_t n, hipStream_t s);
 
 }  // namespace pgf

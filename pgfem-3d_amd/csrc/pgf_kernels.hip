@@ -67,18 +67,84 @@ __device__ __forceinline__ double pick6(double v0, double v1, double v2, double 
   return I == 0 ? v0 : I == 1 ? v1 : I == 2 ? v2 : I == 3 ? v3 : I == 4 ? v4 : v5;
 }
 
+// Continuum damage of the J2 point in the spatial form: tau = om taub and
+// c = om cbar - hv (taub (x) tau0 + tau0 (x) taub), om = 1 - w, hv = evo / 2 (the push-forward
+// of S = (1 - w) Sbar and L = (1 - w) Lbar - evo/2 (Sbar (x) S0 + S0 (x) Sbar),
+// src/cm_j2_plasticity.cc:364-382); a dyad A (x) B is A_I B_K in this Voigt form.
+struct J2Dam {
+  double om, hv, tb[6], t0[6];
+};
+
 // Row I of c (row = stress component, column = engineering strain component).
-__device__ __forceinline__ void j2_c_row(const J2Tan &T, int I, double w, double *row) {
+template <bool DMG = false>
+__device__ __forceinline__ void j2_c_row(const J2Tan &T, int I, double w, double *row, const J2Dam *D = nullptr) {
   const double nI = pick6(T.n[0], T.n[1], T.n[2], T.n[3], T.n[4], T.n[5], I);
   const double sI = pick6(T.s_tr[0], T.s_tr[1], T.s_tr[2], T.s_tr[3], T.s_tr[4], T.s_tr[5], I);
   const double nrm_I = I < 3 ? 1.0 : 0.0;
+  double tbI = 0.0, t0I = 0.0;
+  if constexpr (DMG) {
+    tbI = pick6(D->tb[0], D->tb[1], D->tb[2], D->tb[3], D->tb[4], D->tb[5], I);
+    t0I = pick6(D->t0[0], D->t0[1], D->t0[2], D->t0[3], D->t0[4], D->t0[5], I);
+  }
 #pragma unroll
   for (int K = 0; K < 6; ++K) {
     const double oo = (I < 3 && K < 3) ? 1.0 : 0.0;
     const double Is = (I == K) ? (I < 3 ? 1.0 : 0.5) : 0.0;
     const double cbar = T.tm * (Is - oo * (1.0 / 3.0)) - (2.0 / 3.0) * (sI * (K < 3 ? 1.0 : 0.0) + nrm_I * T.s_tr[K]);
-    row[K] = w * (T.cv1 * oo + T.cv2 * Is + T.a1 * cbar - T.tm * T.b3 * nI * T.n[K] - T.tm * T.b4 * nI * T.dn2[K]);
+    const double cv = T.cv1 * oo + T.cv2 * Is + T.a1 * cbar - T.tm * T.b3 * nI * T.n[K] - T.tm * T.b4 * nI * T.dn2[K];
+    if constexpr (DMG)
+      row[K] = w * (D->om * cv - D->hv * (tbI * D->t0[K] + t0I * D->tb[K]));
+    else
+      row[K] = w * cv;
   }
+}
+
+// The damage update of a J2 point from the total F (Fs, J), restating the reference's
+// CM_J2P_PARAM::integration_algorithm damage step (src/cm_j2_plasticity.cc:184-212):
+// Y0 = G/2 (tr bbar - 3) + kappa U(J), bbar = J^-2/3 Fs Fs^T (j2d_compute_Y0, :162-182), and
+// the viscous Weibull update of damage_int_alg (src/volumetric_damage.cc:196-262): with
+// dmu = dt mu and g = G(Y0) - X_n, a damaging point (g > 0) takes w = w_n + dmu/(1+dmu) g,
+// X = max(X_n, (X_n + dmu G)/(1+dmu)), H = G'(Y0); w is capped at wmax.  Sets D from the
+// undamaged tau (taub) and tau0 = G dev(bbar) + kappa J U'(J) 1, the Kirchhoff stress of Y0.
+__device__ __forceinline__ void j2_damage(const DevMaterial &m, const double *Fs, double J, double dt, double w_n,
+                                          double X_n, const double *taub, double &w, double &X, double &H,
+                                          double &damaged, J2Dam &D) {
+  const double cj = cbrt(J);
+  const double Jm23 = 1.0 / (cj * cj);
+  double b[6];
+#pragma unroll
+  for (int I = 0; I < 6; ++I) {
+    const int i = voigt_i(I), j = voigt_j(I);
+    b[I] = Jm23 * (Fs[i * 3 + 0] * Fs[j * 3 + 0] + Fs[i * 3 + 1] * Fs[j * 3 + 1] + Fs[i * 3 + 2] * Fs[j * 3 + 2]);
+  }
+  const double tr = b[0] + b[1] + b[2];
+  double dU, d2U;
+  vol_pot(m.volPotFlag, J, dU, d2U);
+  const double Y = 0.5 * m.G * (tr - 3.0) + m.kappa * vol_U(m.volPotFlag, J);
+  const double p = m.kappa * J * dU;
+#pragma unroll
+  for (int I = 0; I < 6; ++I) {
+    D.t0[I] = m.G * (b[I] - (I < 3 ? tr * (1.0 / 3.0) : 0.0)) + (I < 3 ? p : 0.0);
+    D.tb[I] = taub[I];
+  }
+  constexpr double kThresh = 0.9999;  // DAMAGE_THRESH, src/volumetric_damage.cc:16
+  double G = 0.0, Hy = 0.0;
+  if (Y > m.Yin) {  // weibull_function / weibull_evolution, src/volumetric_damage.cc:236-262
+    const double z = (Y - m.Yin) / (m.p1 * m.Yin);
+    const double e = exp(-pow(z, m.p2));
+    G = kThresh - kThresh * e;
+    Hy = kThresh * m.p2 / (m.p1 * m.Yin) * e * pow(z, m.p2 - 1.0);
+  }
+  const double dmu = dt * m.mu;
+  const double g = G - X_n;
+  const bool dmg = g > 0.0;
+  w = dmg ? w_n + dmu / (1.0 + dmu) * g : w_n;
+  X = dmg ? fmax(X_n, (X_n + dmu * G) / (1.0 + dmu)) : X_n;
+  H = dmg ? Hy : 0.0;
+  w = fmin(w, m.wmax >= 1.0 ? kThresh : m.wmax);
+  damaged = dmg ? 1.0 : 0.0;
+  D.om = 1.0 - w;
+  D.hv = dmg ? 0.5 * dmu * H / (1.0 + dmu) : 0.0;
 }
 
 // Neo-Hookean / J2 material point in the current configuration.
@@ -219,8 +285,13 @@ constexpr int stage_round(int epb, int nde, int room) {
 // variable per element, condensed): F(n+1) is always Fr (the model stores (theta/J)^1/3 Fr),
 // tau and c take the element pressure, and the rows get the condensation terms.  Its staged
 // variant runs at three waves per SIMD: at four (128 registers) the extra element terms spill.
-template <int NNE, int MODE, bool STG, bool TF>
-__global__ __launch_bounds__(256, STG ? (TF ? 3 : 4) : 1) void elem_kernel(ElemArgs a, int64_t e_begin, int64_t e_end) {
+// VAR: 0 the displacement element, 1 the three-field element (TF), 2 the displacement
+// element with J2 continuum damage (DMG: the undamaged kernels keep their register budget).
+template <int NNE, int MODE, bool STG, int VAR>
+__global__ __launch_bounds__(256, STG ? (VAR != 0 ? 3 : 4) : 1) void elem_kernel(ElemArgs a, int64_t e_begin,
+                                                                                int64_t e_end) {
+  constexpr bool TF = VAR == 1;
+  constexpr bool DMG = VAR == 2;
   constexpr int NQP = nqp_of(NNE);
   constexpr int NDE = NNE * 3;
   constexpr int TPE = (NNE == 4) ? 16 : 32;
@@ -272,7 +343,7 @@ __global__ __launch_bounds__(256, STG ? (TF ? 3 : 4) : 1) void elem_kernel(ElemA
   const int p1slot = p1live ? p1 / (NQP * LPQ) : 0, p1ip = p1live ? (p1 / LPQ) % NQP : 0;
   int64_t p1e = 0;
   DevMaterial pm{};
-  double pCn[6] = {1, 1, 1, 0, 0, 0}, pep = 0.0, pth = 1.0, ppr = 0.0;
+  double pCn[6] = {1, 1, 1, 0, 0, 0}, pep = 0.0, pth = 1.0, ppr = 0.0, pw = 0.0, pX = 0.0;
   if (p1live) {
     const int64_t ie1 = e_begin + blk * EPB + p1slot;
     p1e = a.elist ? int64_t(a.elist[ie1]) : ie1;
@@ -286,6 +357,10 @@ __global__ __launch_bounds__(256, STG ? (TF ? 3 : 4) : 1) void elem_kernel(ElemA
 #pragma unroll
       for (int I = 0; I < 6; ++I) pCn[I] = a.P_n[I * a.n_qp + q];
       pep = a.Sc_n[q];
+      if constexpr (DMG) {
+        pw = a.Sc_n[2 * a.n_qp + q];
+        pX = a.Sc_n[3 * a.n_qp + q];
+      }
     }
   }
 
@@ -416,6 +491,22 @@ __global__ __launch_bounds__(256, STG ? (TF ? 3 : 4) : 1) void elem_kernel(ElemA
     double Cnp1[6], ep1, gm, tau[6];
     J2Tan T;
     j2_point<TAN, TF>(m, is_j2, Fs, Fi, ok ? J : 1.0, Cn, ep_n, INTEG && is_j2, Cnp1, ep1, gm, tau, T, ppr);
+    J2Dam dam;
+    double dw = pw, dXs = pX, dH = 0.0, dflag = 0.0;
+    if constexpr (DMG) {
+      // the damage step of every J2 point with mu > 0, from the state at n (an integrating and
+      // a non-integrating pass at one F give the same values)
+      if (is_j2 && m.mu > 0.0) {
+        j2_damage(m, Fs, ok ? J : 1.0, a.dt, pw, pX, tau, dw, dXs, dH, dflag, dam);
+      } else {
+        dam.om = 1.0;
+        dam.hv = 0.0;
+#pragma unroll
+        for (int I = 0; I < 6; ++I) dam.tb[I] = dam.t0[I] = 0.0;
+      }
+#pragma unroll
+      for (int I = 0; I < 6; ++I) tau[I] *= dam.om;
+    }
     // tau and c go to LDS weighted by the QP's detJ w (zero for a failed QP)
     const double wq = ok ? J0 * wt : 0.0;
     if constexpr (TF) {
@@ -456,12 +547,18 @@ __global__ __launch_bounds__(256, STG ? (TF ? 3 : 4) : 1) void elem_kernel(ElemA
     if (sub == 0 && INTEG && is_j2) {
       a.Sc_np1[q] = ep1;
       a.Sc_np1[NQ + q] = gm;
+      if constexpr (DMG) {
+        a.Sc_np1[2 * NQ + q] = dw;
+        a.Sc_np1[3 * NQ + q] = dXs;
+        a.Sc_np1[4 * NQ + q] = dH;
+        a.Sc_np1[5 * NQ + q] = dflag;
+      }
     }
     if constexpr (TAN) {
 #pragma unroll
       for (int t = 0; t < (6 + LPQ - 1) / LPQ; ++t) {
         const int I = sub + t * LPQ;
-        if (I < 6) j2_c_row(T, I, wq, &s_c[slot][ip][I * 6]);
+        if (I < 6) j2_c_row<DMG>(T, I, wq, &s_c[slot][ip][I * 6], &dam);
       }
     }
     if constexpr (GEN) {  // Gr = Fr Fs^-1
@@ -781,6 +878,32 @@ __global__ void fill_i32_kernel(int32_t *p, int64_t n, int32_t v) {
   for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
     p[i] = v;
 }
+// Largest per-QP subdivision measure over [0, n): scale * sum_c |a[c*stride + q]| over ncomp
+// components (crystal plasticity: a = gamma_dot), plus scale_d * (d1[q] - d0[q]) when d1 is
+// given (a damage increment).  The values are >= 0 (or NaN), so their bit patterns order as
+// unsigned integers and one atomicMax per wave merges them; the result does not depend on
+// the order.
+__global__ __launch_bounds__(256) void rate_max_kernel(const double *a, int ncomp, int64_t stride, int64_t n,
+                                                       double scale, const double *d1, const double *d0,
+                                                       double scale_d, unsigned long long *out) {
+  double m = 0.0;
+  for (int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; q < n; q += int64_t(gridDim.x) * blockDim.x) {
+    double v = 0.0;
+    if (a) {
+      double s = 0.0;
+      for (int c = 0; c < ncomp; ++c) s += fabs(a[c * stride + q]);
+      v = scale * s;
+    }
+    if (d1) v = fmax(v, scale_d * (d1[q] - d0[q]));
+    m = (v > m || v != v) ? v : m;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const double t = __shfl_xor(m, o);
+    m = (t > m || t != t) ? t : m;
+  }
+  if ((threadIdx.x & 63) == 0) atomicMax(out, static_cast<unsigned long long>(__double_as_longlong(m)));
+}
+
 __global__ void copy_kernel(double *dst, const double *src, int64_t n) {
   for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
     dst[i] = src[i];
@@ -881,7 +1004,7 @@ __global__ void axpy_kernel(double *y, const double *x, double g, int64_t n) {
     y[i] += g * x[i];
 }
 
-template <int NNE, bool STG, bool TF>
+template <int NNE, bool STG, int TF>
 void launch_nne(int mode, const ElemArgs &a, int64_t e_begin, int64_t e_end, hipStream_t s) {
   constexpr int TPE = (NNE == 4) ? 16 : 32;
   constexpr int EPB = 256 / TPE;
@@ -911,18 +1034,24 @@ void launch_nne(int mode, const ElemArgs &a, int64_t e_begin, int64_t e_end, hip
 
 }  // namespace
 
-void launch_elem(int nne, int mode, const ElemArgs &a, int64_t e_begin, int64_t e_end,
-                 hipStream_t s) {
+void launch_elem(int nne, int mode, const ElemArgs &a, int64_t e_begin, int64_t e_end, hipStream_t s,
+                 bool damage) {
   const bool stg = a.stage != nullptr;
   if (a.tf_V) {
-    if (nne == 4) (stg ? launch_nne<4, true, true> : launch_nne<4, false, true>)(mode, a, e_begin, e_end, s);
-    else if (nne == 10) (stg ? launch_nne<10, true, true> : launch_nne<10, false, true>)(mode, a, e_begin, e_end, s);
-    else if (nne == 8) (stg ? launch_nne<8, true, true> : launch_nne<8, false, true>)(mode, a, e_begin, e_end, s);
+    if (nne == 4) (stg ? launch_nne<4, true, 1> : launch_nne<4, false, 1>)(mode, a, e_begin, e_end, s);
+    else if (nne == 10) (stg ? launch_nne<10, true, 1> : launch_nne<10, false, 1>)(mode, a, e_begin, e_end, s);
+    else if (nne == 8) (stg ? launch_nne<8, true, 1> : launch_nne<8, false, 1>)(mode, a, e_begin, e_end, s);
     return;
   }
-  if (nne == 4) (stg ? launch_nne<4, true, false> : launch_nne<4, false, false>)(mode, a, e_begin, e_end, s);
-  else if (nne == 10) (stg ? launch_nne<10, true, false> : launch_nne<10, false, false>)(mode, a, e_begin, e_end, s);
-  else if (nne == 8) (stg ? launch_nne<8, true, false> : launch_nne<8, false, false>)(mode, a, e_begin, e_end, s);
+  if (damage) {
+    if (nne == 4) (stg ? launch_nne<4, true, 2> : launch_nne<4, false, 2>)(mode, a, e_begin, e_end, s);
+    else if (nne == 10) (stg ? launch_nne<10, true, 2> : launch_nne<10, false, 2>)(mode, a, e_begin, e_end, s);
+    else if (nne == 8) (stg ? launch_nne<8, true, 2> : launch_nne<8, false, 2>)(mode, a, e_begin, e_end, s);
+    return;
+  }
+  if (nne == 4) (stg ? launch_nne<4, true, 0> : launch_nne<4, false, 0>)(mode, a, e_begin, e_end, s);
+  else if (nne == 10) (stg ? launch_nne<10, true, 0> : launch_nne<10, false, 0>)(mode, a, e_begin, e_end, s);
+  else if (nne == 8) (stg ? launch_nne<8, true, 0> : launch_nne<8, false, 0>)(mode, a, e_begin, e_end, s);
 }
 
 int gather_waves(int32_t maxlen) {
@@ -976,6 +1105,13 @@ void launch_fill_i32(int32_t *p, int64_t n, int32_t v, hipStream_t s) {
   if (n <= 0) return;
   const int64_t blocks = std::min<int64_t>((n + 255) / 256, 8192);
   fill_i32_kernel<<<unsigned(blocks), 256, 0, s>>>(p, n, v);
+}
+
+void launch_rate_max(const double *a, int ncomp, int64_t stride, int64_t n, double scale, const double *d1,
+                     const double *d0, double scale_d, unsigned long long *out, hipStream_t s) {
+  if (n <= 0) return;
+  const int64_t blocks = std::min<int64_t>((n + 255) / 256, 2048);
+  rate_max_kernel<<<unsigned(blocks), 256, 0, s>>>(a, ncomp, stride, n, scale, d1, d0, scale_d, out);
 }
 
 void launch_copy(double *dst, const double *src, int64_t n, hipStream_t s) {

@@ -23,6 +23,13 @@ __device__ __forceinline__ void vol_pot(int flag, double J, double &dU, double &
   }
 }
 
+// U(J) of the same potentials (src/new_potentials.cc:553-573)
+__device__ __forceinline__ double vol_U(int flag, double J) {
+  if (flag == 3) return 0.5 * (J - 1.0) * log(J);
+  if (flag == 99) return 0.5 * (J - 1.0) * (J - 1.0);
+  return 0.5 * (exp(J - 1.0) - log(J) - 1.0);
+}
+
 __host__ __device__ constexpr int nqp_of(int nne) { return nne == 4 ? 1 : (nne == 10 ? 4 : 8); }
 
 // FEMLIB quadrature (FEMLIB/src/femlib.cc:544-588), truncated constants kept.

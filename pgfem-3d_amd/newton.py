@@ -34,6 +34,20 @@ class StepResult:
     solver_iters: list = field(default_factory=list)
     cuts: int = 0                              # subdivisions used (increment())
     gammas: list = field(default_factory=list)  # step length taken per iteration
+    alpha: float = 0.0                          # physics-based subdivision measure at convergence
+
+
+ALPHA_RESTART = 1.25  # Newton_Raphson.cc:91
+
+
+def substeps_after_failure(alpha: float) -> int:
+    """Pieces a failed step is cut into at its first subdivision
+    (SubdivisionScheme::do_subdivision, src/subdivision.cc:139-170): the step shrinks by
+    1/alpha when the physics asks for more than 4/3 of it, else by 0.75; the count is the
+    rounded ratio (round1, src/matice.cc:43-47), and at least one more than before."""
+    gama = 1.0 / alpha if alpha > 4.0 / 3.0 else 0.75
+    i = int(1.0 / gama + 0.5)
+    return i if i > 1 else 2
 
 
 def backtrack_gamma(gam, phi, prev=None):
@@ -67,15 +81,20 @@ def backtrack_gamma(gam, phi, prev=None):
 
 class NewtonDriver:
     def __init__(self, h: P.Handle, Ap, Ai, tol=1e-10, max_iter=25, solver="device", lin_tol=1e-13,
-                 lin_maxit=20000, computer_zero=1e-15, line_search=True, iter_max=None):
+                 lin_maxit=20000, computer_zero=1e-15, line_search=True, iter_max=None,
+                 alpha_restart=ALPHA_RESTART):
         """tol is the reference's nor_min; iter_max its iteration cap (None: max_iter only):
         past iter_max iterations a residual above 20 tol, or any past iter_max + 3, ends the
-        step -- converged if below 5 tol, else failed (Newton_Raphson.cc:1333-1352)."""
+        step -- converged if below 5 tol, else failed (Newton_Raphson.cc:1333-1352).
+        alpha_restart: a converged step whose physics-based subdivision measure
+        (pgf_subdivision_param) exceeds it fails for subdivision (Newton_Raphson.cc:1364-1398);
+        None skips the check."""
         import torch
 
         self.h, self.tol, self.max_iter = h, tol, max_iter
         self.solver, self.lin_tol, self.lin_maxit = solver, lin_tol, lin_maxit
         self.computer_zero, self.line_search, self.iter_max = computer_zero, line_search, iter_max
+        self.alpha_restart = alpha_restart
         self.Ap, self.Ai = Ap, Ai
         dev = torch.device("cuda:0")
         self.f = torch.empty(h.ndof, dtype=torch.float64, device=dev)
@@ -87,25 +106,47 @@ class NewtonDriver:
         self.last_relres = 0.0
 
     def _solve(self, R):
+        """du = K^-1 R; ok is False when the solve failed the reference's check
+        (Newton_Raphson.cc:996-1011): a non-finite residual norm, or one above 500 times the
+        solver tolerance (BiCGStab breakdown or its iteration limit).  The step then fails
+        with INFO = 1 and is subdivided."""
         import torch
 
         if self.solver == "device":
             self.du.zero_()
             it, rr = self.h.solve(self.vals, R, self.du, tol=self.lin_tol, maxit=self.lin_maxit)
             self.last_relres = rr
-            return self.du, it
+            return self.du, it, bool(np.isfinite(rr) and rr <= 500.0 * self.lin_tol)
         import scipy.sparse as sp
         import scipy.sparse.linalg as spla
 
         K = sp.csr_matrix((self.vals.cpu().numpy(), self.Ai, self.Ap), shape=(self.h.ndof, self.h.ndof))
-        x = spla.spsolve(K.tocsc(), R.cpu().numpy())
-        return torch.from_numpy(np.ascontiguousarray(x)).to(R.device), 0
+        Rh = R.cpu().numpy()
+        x = spla.spsolve(K.tocsc(), Rh)
+        nb = np.linalg.norm(Rh)
+        rr = float(np.linalg.norm(K @ x - Rh) / nb) if nb > 0 else 0.0
+        self.last_relres = rr
+        ok = bool(np.all(np.isfinite(x)) and np.isfinite(rr) and rr <= 500.0 * self.lin_tol)
+        return torch.from_numpy(np.ascontiguousarray(x)).to(R.device), 0, ok
+
+    def _accept(self, res, dt):
+        """A converged step: commit it unless its physics-based subdivision measure asks for
+        a smaller step (INFO = 1, Newton_Raphson.cc:1364-1398)."""
+        if self.alpha_restart is not None:
+            res.alpha = self.h.subdivision_param(dt)
+            if not res.alpha <= self.alpha_restart:
+                res.info = 1
+                return
+        res.converged = True
+        self.h.commit_step()
 
     def increment(self, r, defl_from, defl_to, dt=1.0, max_cuts=4, predictor=True) -> StepResult:
         """Take the prescribed values from defl_from (converged) to defl_to, subdividing on
         failure as the reference does (a failing step is reset with pgf_reset_step and the
         remaining increment is applied in halves, up to max_cuts levels; the time step is
-        split with it).  predictor: each piece starts with the reference's predictor."""
+        split with it).  A step rejected by the physics-based measure alpha > 4/3 is cut into
+        round(alpha) pieces instead of halves (substeps_after_failure).  predictor: each piece
+        starts with the reference's predictor."""
         out = StepResult(False, 0)
         pieces = [(0.0, 1.0, 0)]
         r_conv = r.clone()
@@ -127,8 +168,8 @@ class NewtonDriver:
                 out.info = res.info
                 return out
             out.cuts += 1
-            m = 0.5 * (a + b)
-            pieces = [(a, m, lvl + 1), (m, b, lvl + 1)] + pieces
+            k = substeps_after_failure(res.alpha)
+            pieces = [(a + (b - a) * j / k, a + (b - a) * (j + 1) / k, lvl + 1) for j in range(k)] + pieces
         out.converged = True
         return out
 
@@ -150,8 +191,11 @@ class NewtonDriver:
             if info:
                 res.info = info
                 return res
-            du, lit = self._solve(-self.f)
+            du, lit, ok = self._solve(-self.f)
             res.solver_iters.append((lit, self.last_relres))
+            if not ok:
+                res.info = 1
+                return res
             r.add_(du)
             res.gammas.append(1.0)
             it0 = 1
@@ -179,15 +223,16 @@ class NewtonDriver:
             if (self.iter_max is not None and it > self.iter_max and nor > self.tol
                     and (nor > 20.0 * self.tol or it > self.iter_max + 3)):
                 if nor < 5.0 * self.tol:
-                    res.converged = True
-                    self.h.commit_step()
+                    self._accept(res, dt)
                 return res
             if nor <= self.tol or nor2 <= self.computer_zero:
-                res.converged = True
-                self.h.commit_step()
+                self._accept(res, dt)
                 return res
-            du, lit = self._solve(R)
+            du, lit, ok = self._solve(R)
             res.solver_iters.append((lit, self.last_relres))
+            if not ok:
+                res.info = 1
+                return res
             if tf:  # the elements' volume and pressure increments that go with du
                 self.h.tf_update(du)
             gam, prev = 1.0, None
@@ -204,13 +249,12 @@ class NewtonDriver:
                 if phi <= 1.0 - 2e-4 * gam:  # sufficient decrease (LINE.cc:96)
                     break
                 if tf:
-                    self.h.tf_accept(-gam)
+                    self.h.tf_accept(0.0)  # back to the start of the trials
                 gam, prev = backtrack_gamma(gam, phi, prev), (gam, phi)
                 if gam < 0.1 * self.tol:  # the line search failed (LINE.cc:126-127)
                     if nor < 5.0 * self.tol:  # soft convergence: keep u (Newton_Raphson.cc:1246)
                         if run(r) == 0:
-                            res.converged = True
-                            self.h.commit_step()
+                            self._accept(res, dt)
                     return res
             res.gammas.append(gam)
             r.copy_(self.rt)

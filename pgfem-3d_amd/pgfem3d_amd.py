@@ -51,6 +51,7 @@ class Material(C.Structure):
         ("C11", C.c_double), ("C12", C.c_double), ("C44", C.c_double),
         ("tol_M", C.c_double), ("tol_hardening", C.c_double),
         ("max_itr_M", C.c_int32), ("max_itr_stag", C.c_int32),
+        ("mu", C.c_double), ("w_max", C.c_double), ("p1", C.c_double), ("p2", C.c_double), ("Yin", C.c_double),
     ]
 
 
@@ -97,6 +98,7 @@ def lib() -> C.CDLL:
             "pgf_tangent": [P, P, D, P],
             "pgf_prescribed_load": [P, P, D, P],
             "pgf_residual_tangent": [P, P, D, P, P],
+            "pgf_subdivision_param": [P, D, P],
             "pgf_commit_step": [P],
             "pgf_reset_step": [P],
             "pgf_state_ncomp": [P, I32, P],
@@ -151,7 +153,7 @@ def exported_symbols():
         "pgf_mlp_get_params", "pgf_mlp_set_params", "pgf_mlp_get_grads", "pgf_mlp_forward", "pgf_mlp_backward",
         "pgf_mlp_adam_step", "pgf_cp_log_samples", "pgf_set_surrogate", "pgf_solve", "pgf_set_graph", "pgf_set_assembly",
         "pgf_set_three_field", "pgf_tf_initial", "pgf_tf_update", "pgf_tf_accept", "pgf_tf_get", "pgf_tf_set",
-        "pgf_autotune",
+        "pgf_autotune", "pgf_subdivision_param",
     ]
 
 
@@ -266,6 +268,15 @@ def j2_linear(E: float, nu: float, sigma_y: float, H: float, vol_pot: int = 2) -
                     m10=0.5 * G, m01=0.0, k0=sigma_y, hp=H)
 
 
+def j2_damage(E: float, nu: float, sigma_y: float, H: float, mu: float, p1: float, p2: float, Yin: float,
+              w_max: float = 1.0, vol_pot: int = 2) -> Material:
+    """J2 with the viscous Weibull continuum damage (CM_J2P_PARAM damage properties mu, w_max,
+    P1, P2, Yin; src/cm_j2_plasticity.cc:865-932)."""
+    m = j2_linear(E, nu, sigma_y, H, vol_pot)
+    m.mu, m.w_max, m.p1, m.p2, m.Yin = mu, w_max, p1, p2, Yin
+    return m
+
+
 # ---------------------------------------------------------------------------- handle
 class Handle:
     """Device-side element hot path for one mesh (pgf_create ... pgf_destroy)."""
@@ -374,6 +385,13 @@ class Handle:
         self._check_vec(vals, self.nnz, "vals")
         return _check(self._lib.pgf_residual_tangent(self.h, C.byref(inp), dt, _ptr(f), _ptr(vals)),
                       "pgf_residual_tangent")
+
+    def subdivision_param(self, dt=0.0) -> float:
+        """Largest per-QP physics-based subdivision measure of the last integration
+        (crystal plasticity: dt sum |gamma_dot| / 0.005)."""
+        a = C.c_double()
+        _check(self._lib.pgf_subdivision_param(self.h, float(dt), C.byref(a)), "pgf_subdivision_param")
+        return a.value
 
     def commit_step(self):
         _check(self._lib.pgf_commit_step(self.h), "pgf_commit_step")

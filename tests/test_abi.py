@@ -28,3 +28,21 @@ def test_library_exports_every_declared_symbol():
     lib = ctypes.CDLL(P.LIB_PATH)
     for name in declared_functions():
         assert hasattr(lib, name), name
+
+
+CALLER = os.path.join(ROOT, "pgfem-3d_amd", "_build", "abi_caller")
+
+
+def test_c_caller_builds_against_the_header():
+    """tests/abi_caller.c, compiled as C99 against include/pgfem3d_amd.h and linked with the
+    library (pgfem-3d_amd/Makefile), runs up to pgf_create; without a GPU that call returns
+    PGF_ENODEV and the program says so (exit code 3)."""
+    import subprocess
+
+    import torch
+
+    assert os.path.exists(CALLER), "abi_caller is not built"
+    if torch.cuda.is_available():
+        return  # the GPU run is test_gpu_abi_caller.py
+    out = subprocess.run([CALLER, os.devnull], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 3 and "PGF_ENODEV" in out.stdout, (out.returncode, out.stdout, out.stderr)

@@ -84,3 +84,37 @@ def test_graph_replay_and_autotune_are_bitwise_eager(device):
     assert h.residual_tangent(f, vals, rd, defl=dd) == 0
     assert np.array_equal(f.cpu().numpy(), f0) and np.array_equal(vals.cpu().numpy(), v0)
     h.close()
+
+
+def test_setters_drop_the_captured_pass(device):
+    """New orientations and interface properties replace device tables that a captured pass
+    reads: with the graph on, the pass after each setter equals an eager pass of a fresh
+    handle holding the new data (the setters drop the graph; the next pass captures again)."""
+    prob = MixedProblem(3, 2, 3, P.fcc_crystal(), COH, n_grains=8, seed=7)
+    h = _handle(prob)
+    h.set_graph(True)
+    f = torch.empty(prob.ndof, dtype=torch.float64, device=device)
+    vals = torch.empty(int(prob.Ap[-1]), dtype=torch.float64, device=device)
+    r, defl = prob.displacement(3e-4, 2e-5, seed=1)
+    rd, dd = _dev(r, device), _dev(defl, device)
+    assert h.residual_tangent(f, vals, rd, defl=dd) == 0
+    assert h.residual_tangent(f, vals, rd, defl=dd) == 0  # a replay
+    rng = np.random.default_rng(9)
+    R2 = np.stack([np.linalg.qr(rng.standard_normal((3, 3)))[0] for _ in range(len(prob.R))])
+    R2 *= np.sign(np.linalg.det(R2))[:, None, None]
+    coh2 = P.Cohesive(sig_c=300.0, chi_c=2e-5, beta=0.5, alpha=1.0)
+    h.set_orientations(R2, prob.grain)
+    h.set_cohesive_props([coh2])
+    f.fill_(np.nan)
+    vals.fill_(np.nan)
+    assert h.residual_tangent(f, vals, rd, defl=dd) == 0
+    fresh = P.Handle(prob.mesh, prob.ids, prob.ndof, prob.Ap, prob.Ai, prob.mats)
+    fresh.set_orientations(R2, prob.grain)
+    fresh.set_cohesive_props([coh2])
+    f2 = torch.empty_like(f)
+    v2 = torch.empty_like(vals)
+    assert fresh.residual_tangent(f2, v2, rd, defl=dd) == 0
+    assert torch.equal(f, f2) and torch.equal(vals, v2)
+    h.set_graph(False)
+    h.close()
+    fresh.close()

@@ -80,6 +80,17 @@ def test_condensed_pass_and_update(device, etype, n, assembly):
     assert err == 0
     assert np.abs(h.tf_get(P.TF_DDV) - ddV).max() <= 1e-10 * np.abs(ddV).max()
     assert np.abs(h.tf_get(P.TF_DDP) - ddP).max() <= 1e-10 * np.abs(ddP).max()
+    # update_increments_from_NR(gamma): ddV *= gamma, dV += ddV; a trial taken back with
+    # gamma = 0 restores dV exactly
+    full_V, full_P = h.tf_get(P.TF_DDV), h.tf_get(P.TF_DDP)
+    h.tf_accept(0.5)
+    assert np.array_equal(h.tf_get(P.TF_DDV), 0.5 * full_V) and np.array_equal(h.tf_get(P.TF_DDP), 0.5 * full_P)
+    assert np.array_equal(h.tf_get(P.TF_DV), dV + 0.5 * full_V)
+    assert np.array_equal(h.tf_get(P.TF_DP), dP + 0.5 * full_P)
+    h.tf_accept(0.0)
+    assert np.array_equal(h.tf_get(P.TF_DV), dV) and np.array_equal(h.tf_get(P.TF_DP), dP)
+    h.tf_accept(1.0)
+    assert np.array_equal(h.tf_get(P.TF_DDV), full_V) and np.array_equal(h.tf_get(P.TF_DV), dV + full_V)
     h.close()
 
 
@@ -127,6 +138,12 @@ def test_initial_conditions(device, etype):
     for fld, ref in ((P.TF_P_NP1, Pr), (P.TF_P_N, Pr), (P.TF_P_NM1, P0)):
         assert np.abs(h.tf_get(fld) - ref).max() < 1e-12 * np.abs(Pr).max()
     assert not h.tf_get(P.TF_DV).any() and not h.tf_get(P.TF_DP).any()
+    # the model's F(n), F(n-1) from the same displacements (compute_cm_initial_conditions)
+    for fld, rr, dd in ((P.STATE_F_N, r, defl), (P.STATE_F_NM1, r0, defl0)):
+        st = O.OracleState(prob.nqp, False)
+        O.assemble(prob.mesh, prob.mats, prob.ids, prob.ndof, prob.Ap, prob.Ai, st, rr, defl=dd)
+        assert np.abs(h.get_state(fld) - st.F_np1).max() < 1e-12
+        assert np.abs(st.F_np1 - np.eye(3).reshape(9, 1)).max() > 1e-3  # a nonzero displacement
     h.close()
 
 

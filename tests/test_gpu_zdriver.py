@@ -128,3 +128,23 @@ def test_subdivision_recovers_a_large_step(device):
     assert res.converged, res
     assert res.cuts >= 1
     h.close()
+
+
+def test_failed_linear_solve_fails_the_step(device):
+    """A linear solve that stops short (here: a 2-iteration BiCGStab limit) leaves a relative
+    residual far above 500 x its tolerance; the step must fail with INFO = 1 for subdivision
+    (Newton_Raphson.cc:996-1011) instead of line-searching a partial update."""
+    prob = BoxProblem(4, P.PGF_TET4, P.neo_hookean(200e3, 0.3))
+    h = P.Handle(prob.mesh, prob.ids, prob.ndof, prob.Ap, prob.Ai, prob.mats)
+    drv = NW.NewtonDriver(h, prob.Ap, prob.Ai, tol=1e-10, lin_maxit=2)
+    r = torch.zeros(prob.ndof, dtype=torch.float64, device=device)
+    _, defl = prob.displacement(0.01, 0.0, seed=0)
+    res = drv.load_step(r, _dev(defl, device))
+    assert not res.converged and res.info == 1, res
+    assert res.solver_iters and res.solver_iters[-1][1] > 500.0 * drv.lin_tol
+    # with the default limit the same step converges
+    h.reset_step()
+    r.zero_()
+    drv2 = NW.NewtonDriver(h, prob.Ap, prob.Ai, tol=1e-10)
+    assert drv2.load_step(r, _dev(defl, device)).converged
+    h.close()

@@ -159,3 +159,108 @@ def test_j2_tangent_symmetrised_version_fails_fd():
     stress = lambda G: O.j2_update(J2, G, cn, 0.0, False)["S"]  # noqa: E731
     assert _fd_tangent_err(stress, L, F) < 1e-6
     assert _fd_tangent_err(stress, Lsym, F) > 1e-5
+
+
+# ------------------------------------------------------------------ J2 + continuum damage
+J2D = P.j2_damage(70e3, 0.33, 250.0, 1000.0, mu=2.0, p1=2.0, p2=1.5, Yin=5.0)
+
+
+def _stretch(a):
+    lat = -0.35 * a
+    return np.diag([1.0 + a, 1.0 + lat, 1.0 + lat])
+
+
+def test_j2_damage_off_is_plain_j2():
+    off = P.j2_damage(70e3, 0.33, 250.0, 1000.0, mu=0.0, p1=2.0, p2=1.5, Yin=5.0)
+    F = _stretch(0.03) + 0.003 * np.random.default_rng(1).standard_normal((3, 3))
+    cn = np.array([1, 1, 1, 0, 0, 0.0])
+    a = O.j2_update(J2, F, cn, 0.0)
+    b = O.j2d_update(off, F, cn, 0.0, 0.0, 0.0, 1.0)
+    assert b["w"] == 0.0 and np.array_equal(a["S"], b["S"]) and np.array_equal(a["L"], b["L"])
+
+
+def test_weibull_closed_forms():
+    """G(Yin) = 0, G(Yin (1 + P1)) = 0.9999 (1 - 1/e), and H = dG/dY (central differences)."""
+    g, h = O.weibull(J2D, J2D.Yin)
+    assert g == 0.0 and h == 0.0
+    g, _ = O.weibull(J2D, J2D.Yin * (1.0 + J2D.p1))
+    assert abs(g - 0.9999 * (1.0 - np.exp(-1.0))) < 1e-15
+    for Y in (6.0, 9.0, 20.0, 40.0):
+        d = 1e-5 * Y
+        fd = (O.weibull(J2D, Y + d)[0] - O.weibull(J2D, Y - d)[0]) / (2 * d)
+        assert abs(O.weibull(J2D, Y)[1] - fd) < 1e-8 * max(abs(fd), 1e-3)
+
+
+def test_damage_driving_energy_closed_form():
+    """Y0 = G/2 (J^-2/3 (l1^2 + l2^2 + l3^2) - 3) + kappa U(J) for principal stretches, with the
+    Doll-Schweizerhof 7 potential U = (e^(J-1) - ln J - 1)/2."""
+    lam = np.array([1.04, 0.985, 0.99])
+    J = lam.prod()
+    G, kappa = J2D.G, J2D.E / (3 * (1 - 2 * J2D.nu))
+    U = 0.5 * (np.exp(J - 1) - np.log(J) - 1)
+    ref = 0.5 * G * (J ** (-2 / 3) * (lam ** 2).sum() - 3) + kappa * U
+    assert abs(O.j2d_Y0(J2D, np.diag(lam)) - ref) < 1e-12 * ref
+    # the damage threshold of this material is crossed by a 3% stretch, not by 0.5%
+    assert O.j2d_Y0(J2D, _stretch(0.005)) < J2D.Yin < O.j2d_Y0(J2D, _stretch(0.03))
+
+
+def test_viscous_damage_under_constant_load_closed_form():
+    """Held at one F from rest, backward-Euler viscous damage relaxes towards G(Y):
+    w_k = X_k = G (1 - (1 + dt mu)^-k)."""
+    F = _stretch(0.03)
+    Y = O.j2d_Y0(J2D, F)
+    G, _ = O.weibull(J2D, Y)
+    dt = 0.25
+    w = X = 0.0
+    cn = np.array([1, 1, 1, 0, 0, 0.0])
+    ep = 0.0
+    for k in range(1, 9):
+        r = O.j2d_update(J2D, F, cn, ep, w, X, dt, want_L=False)
+        cn, ep, w, X = r["Cpinv"], r["ep"], r["w"], r["X"]
+        exact = G * (1.0 - (1.0 + dt * J2D.mu) ** (-k))
+        assert r["damaged"] == 1 and abs(w - exact) < 1e-14 and abs(X - exact) < 1e-14, k
+    # S = (1 - w) Sbar with Sbar the undamaged J2 stress at the same plastic state
+    r = O.j2d_update(J2D, F, cn, ep, w, X, dt)
+    assert r["damaged"] == 1  # X_k < G: still relaxing
+    bare = O.j2_update(J2, F, cn, ep)
+    np.testing.assert_allclose(r["S"], (1.0 - r["w"]) * bare["S"], rtol=1e-14, atol=1e-12)
+
+
+def test_damage_capped_at_w_max():
+    capped = P.j2_damage(70e3, 0.33, 250.0, 1000.0, mu=50.0, p1=0.5, p2=1.5, Yin=5.0, w_max=0.2)
+    r = O.j2d_update(capped, _stretch(0.04), np.array([1, 1, 1, 0, 0, 0.0]), 0.0, 0.0, 0.0, 1.0, want_L=False)
+    assert r["w"] == 0.2 and r["X"] > 0.2
+
+
+def test_j2_damage_tangent_matches_fd_elastic_range():
+    """Damaging but elastic (no yield): Sbar = S0, so the reference's symmetrised damage term
+    is the exact derivative and L = FD of S (w depends on F through Y0)."""
+    el = P.j2_damage(70e3, 0.33, 1e9, 1000.0, mu=2.0, p1=2.0, p2=1.5, Yin=5.0)
+    rng = np.random.default_rng(8)
+    cn = np.array([1, 1, 1, 0, 0, 0.0])
+    for k in range(3):
+        F = _stretch(0.025 + 0.01 * k) + 0.003 * rng.standard_normal((3, 3))
+        w_n, X_n = 0.01 * k, 0.02 * k
+        r = O.j2d_update(el, F, cn, 0.0, w_n, X_n, 0.5)
+        assert r["damaged"] == 1 and r["gam"] == 0.0 and r["w"] > w_n
+        stress = lambda G: O.j2d_update(el, G, cn, 0.0, w_n, X_n, 0.5, False)["S"]  # noqa: E731
+        assert _fd_tangent_err(stress, r["L"], F) < 1e-6
+
+
+def test_j2_damage_tangent_plastic_differs_only_by_the_symmetrisation():
+    """Plastic and damaging: the exact derivative is (1-w) Lbar - evo Sbar (x) S0; the
+    reference's tangent takes the symmetric half, so L - evo/2 (Sbar (x) S0 - S0 (x) Sbar)
+    matches FD and L itself does not."""
+    F = _stretch(0.03) + np.array([[0, 0.004, 0], [0, 0, 0.002], [0.001, 0, 0]])
+    cn = np.array([1, 1, 1, 0, 0, 0.0])
+    dt = 0.5
+    r = O.j2d_update(J2D, F, cn, 0.0, 0.0, 0.0, dt)
+    assert r["damaged"] == 1 and r["gam"] > 0
+    dmu = dt * J2D.mu
+    evo = dmu * r["H"] / (1 + dmu)
+    Sbar = r["S"] / (1.0 - r["w"])
+    S0 = r["S0"]
+    skew = 0.5 * evo * (np.einsum("ij,kl->ijkl", Sbar, S0) - np.einsum("ij,kl->ijkl", S0, Sbar))
+    stress = lambda G: O.j2d_update(J2D, G, cn, 0.0, 0.0, 0.0, dt, False)["S"]  # noqa: E731
+    assert _fd_tangent_err(stress, r["L"] - skew, F) < 1e-6
+    assert _fd_tangent_err(stress, r["L"], F) > 1e-5
