@@ -236,6 +236,23 @@ int pgf_prescribed_load(pgf_handle h, const pgf_nodal_input *in, double dt, doub
 int pgf_residual_tangent(pgf_handle h, const pgf_nodal_input *in, double dt, double *f,
                          double *vals);
 
+/* Output variables at a time level (0 = n+1, 1 = n, 2 = n-1) of the stored state, as
+ * constitutive_model_update_output_variables (src/constitutive_model.cc:1677-1881) forms
+ * them: per QP (mesh order) S[n_qp*6], the PK2 stress (sig[eid].il[ip].o), and E[n_qp*6], the
+ * Green-Lagrange strain (eps[eid].il[ip].o); per element sigma[n_elem*6], the volume average
+ * of the Cauchy stress (sig[eid].el.o), and eps[n_elem*6], that of the Eulerian-Almansi
+ * strain with doubled shears (eps[eid].el.o).  Components in the reference's order XX, YY,
+ * ZZ, YZ, XZ, XY.  Host arrays; any may be NULL.  The reference calls it after a converged
+ * step, when the state at n is the step's result. */
+int pgf_output_variables(pgf_handle h, int32_t level, double *S, double *E, double *sigma,
+                         double *eps);
+
+/* Starts of the last crystal-plasticity local Newton: *n_learned QPs converged from the
+ * attached network's start (chosen per QP when its residual merit is below the plain
+ * start's), *n_fallback QPs took it, failed and converged from the plain start instead; the
+ * others started plainly.  (No reference counterpart: the learned start's report.) */
+int pgf_cp_start_stats(pgf_handle h, int64_t *n_learned, int64_t *n_fallback);
+
 /* Physics-based subdivision measure of the last integration (cm_get_subdivision_parameter,
  * src/constitutive_model.cc:1980-2001): the largest per-QP value of the models'
  * get_subdiv_param over the mesh.  Crystal plasticity (CP_PARAM::get_subdiv_param,

@@ -22,6 +22,9 @@
 // Computational Inelasticity (1998), boxes 9.1-9.2, and pinned only by the checks above.
 #include "oracle.h"
 
+#include <omp.h>
+
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -269,9 +272,23 @@ void mat_stiff_mr(const double *C, double m10, double m01, double *L) {
         }
 }
 
+// devStress_Linear / matStiff_Linear, src/new_potentials.cc:360-373, :434-453:
+// S = G (C - I), L = 2 G delta_ik delta_jl
+void dev_stress_linear(const double *C, double G, double *S) {
+  for (int i = 0; i < 9; ++i) S[i] = G * (C[i] - ((i % 4 == 0) ? 1.0 : 0.0));
+}
+void mat_stiff_linear(double G, double *L) {
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      for (int k = 0; k < 3; ++k)
+        for (int l = 0; l < 3; ++l) L[idx4(i, j, k, l)] = (i == k && j == l) ? 2.0 * G : 0.0;
+}
+
 // Total S and L of the hyperelastic model (HE_PARAM::update_elasticity,
 // src/plasticity_model_none.cc:329-362, through the in-tree mirror get_material_stress /
-// get_material_stiffness, src/displacement_based_element.cc:1368-1445, damage off).
+// get_material_stiffness, src/displacement_based_element.cc:1368-1445, damage off).  The
+// deviatoric law follows devPotFlag (getDevStressFunc, src/new_potentials.cc:104-129):
+// 1 Mooney-Rivlin (m10, m01), 2 the linear law in C (G).
 int he_update(const orc_material &m, const double *F, double *S, double *L) {
   double C[9], CI[9];
   matmul_tn(F, F, C);
@@ -281,10 +298,13 @@ int he_update(const orc_material &m, const double *F, double *S, double *L) {
   const double kappa = kappa_of(m);
   double dU, d2U, U;
   vol_pot(m.volPotFlag, J, &dU, &d2U, &U);
-  dev_stress_mr(C, m.m10, m.m01, S);
+  const bool lin = m.devPotFlag == 2;
+  if (lin) dev_stress_linear(C, m.G, S);
+  else dev_stress_mr(C, m.m10, m.m01, S);
   for (int i = 0; i < 9; ++i) S[i] += kappa * J * dU * CI[i];
   if (L) {
-    mat_stiff_mr(C, m.m10, m.m01, L);
+    if (lin) mat_stiff_linear(m.G, L);
+    else mat_stiff_mr(C, m.m10, m.m01, L);
     for (int i = 0; i < 3; ++i)
       for (int j = 0; j < 3; ++j)
         for (int k = 0; k < 3; ++k)
@@ -772,6 +792,38 @@ int orc_psparse_apai(int64_t n_nodes, int64_t n_elem, const int64_t *elem_ptr,
 // residuals_el_constitutive_model_1f (src/constitutive_model.cc:3322-3539) assembled by
 // fd_res_assemble (src/fd_residuals.cc:58-73) and stiffness_el_constitutive_model_1f
 // (:2474-2692) assembled by PLoc_Sparse (src/PLoc_Sparse.cc:15-131) into the CSR values.
+int orc_set_threads(int n) {
+  if (n > 0) omp_set_num_threads(n);
+  return omp_get_max_threads();
+}
+
+int orc_colour_elements(int64_t n_nodes, int64_t n_elem, const int64_t *elem_ptr, const int64_t *elem_conn,
+                        int64_t *order, int64_t *cptr) {
+  std::vector<std::vector<uint64_t>> used(1, std::vector<uint64_t>(n_nodes, 0));  // colour bits per node
+  std::vector<int> colour(n_elem);
+  int nc = 0;
+  for (int64_t e = 0; e < n_elem; ++e) {
+    int c = 0;
+    for (;; ++c) {
+      const size_t w = size_t(c / 64);
+      const uint64_t bit = uint64_t(1) << (c % 64);
+      if (w >= used.size()) used.emplace_back(n_nodes, 0);
+      bool free = true;
+      for (int64_t k = elem_ptr[e]; k < elem_ptr[e + 1] && free; ++k) free = !(used[w][elem_conn[k]] & bit);
+      if (free) break;
+    }
+    colour[e] = c;
+    for (int64_t k = elem_ptr[e]; k < elem_ptr[e + 1]; ++k) used[size_t(c / 64)][elem_conn[k]] |= uint64_t(1) << (c % 64);
+    nc = std::max(nc, c + 1);
+  }
+  std::fill(cptr, cptr + nc + 1, 0);
+  for (int64_t e = 0; e < n_elem; ++e) cptr[colour[e] + 1]++;
+  for (int c = 0; c < nc; ++c) cptr[c + 1] += cptr[c];
+  std::vector<int64_t> cur(cptr, cptr + nc);
+  for (int64_t e = 0; e < n_elem; ++e) order[cur[colour[e]]++] = e;
+  return nc;
+}
+
 int orc_assemble(int64_t n_nodes, const double *X, int64_t n_elem, const int64_t *elem_ptr,
                  const int64_t *elem_conn, const int32_t *elem_mat, const orc_material *mats,
                  const int64_t *id, int64_t ndof, const int32_t *Ap, const int32_t *Ai,
@@ -779,15 +831,20 @@ int orc_assemble(int64_t n_nodes, const double *X, int64_t n_elem, const int64_t
                  double dt, int do_residual, int run_integration, int do_tangent,
                  double *F_np1, const double *plastic_n, double *plastic_np1,
                  const double *scalar_n, double *scalar_np1, double *f, double *vals) {
-  (void)n_nodes;
-  int64_t nqp_total = 0;
-  for (int64_t e = 0; e < n_elem; ++e) nqp_total += nint_of(int(elem_ptr[e + 1] - elem_ptr[e]));
-  const int64_t NQ = nqp_total;
+  // QP offsets of the elements, then the elements colour by colour: within a colour no two
+  // elements share a node, so their scatters into f and vals touch distinct entries and run in
+  // parallel; every entry's sum follows the colour order whatever the thread count.
+  std::vector<int64_t> qoff(n_elem + 1, 0);
+  for (int64_t e = 0; e < n_elem; ++e) qoff[e + 1] = qoff[e] + nint_of(int(elem_ptr[e + 1] - elem_ptr[e]));
+  const int64_t NQ = qoff[n_elem];
+  std::vector<int64_t> order(n_elem), cptr(n_elem + 1);
+  const int nc = orc_colour_elements(n_nodes, n_elem, elem_ptr, elem_conn, order.data(), cptr.data());
   if (do_residual && f) std::memset(f, 0, sizeof(double) * ndof);
   if (do_tangent && vals) std::memset(vals, 0, sizeof(double) * Ap[ndof]);
-  int err = 0;
-  int64_t qp0 = 0;
-  for (int64_t e = 0; e < n_elem; ++e) {
+  int err = 0, missing = 0;
+  auto element = [&](int64_t e) -> int {
+    int err = 0;
+    const int64_t qp0 = qoff[e];
     const int nne = int(elem_ptr[e + 1] - elem_ptr[e]);
     const int nint = nint_of(nne);
     const int ndofe = nne * 3;
@@ -886,9 +943,17 @@ int orc_assemble(int64_t n_nodes, const double *X, int64_t n_elem, const int64_t
           vals[p - Ai] += lk[i * ndofe + j];
         }
       }
-    qp0 += nint;
+    return err;
+  };
+  for (int c = 0; c < nc; ++c) {
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : err, missing)
+    for (int64_t k = cptr[c]; k < cptr[c + 1]; ++k) {
+      const int r = element(order[k]);
+      if (r < 0) ++missing;
+      else err += r;
+    }
   }
-  return err;
+  return missing ? -1 : err;
 }
 
 }  // extern "C"

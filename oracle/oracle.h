@@ -66,6 +66,14 @@ int orc_cp_update(const orc_material *mat, const double *P_grain, const double *
                   double *pFnp1, double *g_np1, double *lam_np1, double *S, double *tau, double *gdot,
                   double *dMdF, int *iters);
 
+// --- host threads of the whole-mesh assemblies (OpenMP, colour by colour; 0 = the runtime's
+// default); returns the count the next assembly uses
+int orc_set_threads(int n);
+// greedy element colouring in element order (no two elements of a colour share a node):
+// order[n_elem] the elements by (colour, element), cptr[n_colours + 1]; returns n_colours
+int orc_colour_elements(int64_t n_nodes, int64_t n_elem, const int64_t *elem_ptr, const int64_t *elem_conn,
+                        int64_t *order, int64_t *cptr);
+
 // --- whole mesh: fd_residuals_MP / stiffmat_fd_MP restated for one rank (TL, CM path)
 // state arrays are [comp][n_qp]; plastic = Cpinv (6 comps) for J2, scalar = (ep, gam, w, X, H,
 // damaged): 6 comps.
@@ -111,3 +119,8 @@ int orc_tf_initial(int64_t n_elem, const double *X, const int64_t *elem_ptr, con
                    const int32_t *elem_mat, const orc_material *mats, const double *u_nodal, double *V,
                    double *Pr);
 }
+
+// --- output variables of one QP (oracle_cp.cpp): PK2 S and Cauchy sigma (3x3) from the stored
+// state (plastic: J2 Cp^-1 [6], CP pF [9], unused for the hyperelastic model; w: J2 damage)
+extern "C" int orc_output_qp(const orc_material *mat, const double *L_grain, const double *F, const double *plastic,
+                             double w, double *S_out, double *sig_out);

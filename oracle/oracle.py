@@ -488,3 +488,61 @@ def tf_initial(mesh, mats, u_nodal):
     err = _tf_lib().orc_tf_initial(mesh.n_elem, _p(mesh.X), _p(mesh.elem_ptr), _p(mesh.elem_conn), _p(em), om,
                                    _p(np.ascontiguousarray(u_nodal, dtype=np.float64)), _p(V), _p(Pr))
     return V, Pr, err
+
+
+# ----------------------------------------------------------------------------- output variables
+OUT_ORDER = [(0, 0), (1, 1), (2, 2), (1, 2), (0, 2), (0, 1)]  # XX YY ZZ YZ XZ XY
+
+
+def output_qp(mat, F, plastic=None, w=0.0, L_grain=None):
+    """PK2 S and Cauchy sigma (3x3) of one QP from its stored state (orc_output_qp)."""
+    L = lib()
+    if not getattr(L, "_out_ready", False):
+        L.orc_output_qp.restype = C.c_int
+        L.orc_output_qp.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_void_p,
+                                    C.c_void_p]
+        L._out_ready = True
+    S = np.zeros(9)
+    sig = np.zeros(9)
+    F = np.ascontiguousarray(F, dtype=np.float64).reshape(9)
+    pl = None if plastic is None else np.ascontiguousarray(plastic, dtype=np.float64)
+    Lg = None if L_grain is None else np.ascontiguousarray(L_grain, dtype=np.float64)
+    rc = L.orc_output_qp(C.byref(material(mat)), _p(Lg), _p(F), _p(pl), float(w), _p(S), _p(sig))
+    if rc:
+        raise RuntimeError("orc_output_qp failed")
+    return S.reshape(3, 3), sig.reshape(3, 3)
+
+
+def output_variables(mesh, mats, F, plastic=None, w=None, elem_grain=None, grain_L=None):
+    """constitutive_model_update_output_variables over a mesh from state arrays [comp][n_qp]
+    (mesh QP order): per QP S and E [n_qp, 6], per element the detJ w averages of the Cauchy
+    stress and of the Almansi strain with doubled shears [n_elem, 6], order XX YY ZZ YZ XZ XY."""
+    nqp = F.shape[1]
+    S_out = np.zeros((nqp, 6))
+    E_out = np.zeros((nqp, 6))
+    sig_el = np.zeros((mesh.n_elem, 6))
+    eps_el = np.zeros((mesh.n_elem, 6))
+    q = 0
+    for e in range(mesh.n_elem):
+        conn = mesh.elem_conn[mesh.elem_ptr[e]:mesh.elem_ptr[e + 1]]
+        x = mesh.X[conn]
+        mat = mats[mesh.elem_mat[e] if mesh.elem_mat is not None else 0]
+        vol = 0.0
+        for ip in range(nint(len(conn))):
+            _, _, wq, _ = basis(len(conn), x, ip)
+            Fq = F[:, q].reshape(3, 3)
+            pl = None if plastic is None else plastic[:, q]
+            Lg = None if grain_L is None else grain_L[elem_grain[e]]
+            S, sig = output_qp(mat, Fq, pl, 0.0 if w is None else w[q], Lg)
+            E = 0.5 * (Fq.T @ Fq - np.eye(3))
+            ea = 0.5 * (np.eye(3) - np.linalg.inv(Fq @ Fq.T))
+            for c, (i, j) in enumerate(OUT_ORDER):
+                S_out[q, c] = S[i, j]
+                E_out[q, c] = E[i, j]
+                sig_el[e, c] += wq * sig[i, j]
+                eps_el[e, c] += wq * (1.0 if c < 3 else 2.0) * ea[i, j]
+            vol += wq
+            q += 1
+        sig_el[e] /= vol
+        eps_el[e] /= vol
+    return S_out, E_out, sig_el, eps_el

@@ -527,15 +527,19 @@ extern "C" int orc_assemble_cp(int64_t n_nodes, const double *X, int64_t n_elem,
                                const double *pF_n, double *pF_np1, const double *g_n, double *g_np1,
                                const double *lam_n, double *lam_np1, double *tau_out, double *gdot_out,
                                int32_t *iters_out, const double *F_n, const double *warm, double *f, double *vals) {
-  (void)n_nodes;
-  int64_t NQ = 0;
-  for (int64_t e = 0; e < n_elem; ++e) NQ += orc_nint(int(elem_ptr[e + 1] - elem_ptr[e]));
+  // elements colour by colour, in parallel within a colour (as orc_assemble)
+  std::vector<int64_t> qoff(n_elem + 1, 0);
+  for (int64_t e = 0; e < n_elem; ++e) qoff[e + 1] = qoff[e] + orc_nint(int(elem_ptr[e + 1] - elem_ptr[e]));
+  const int64_t NQ = qoff[n_elem];
+  std::vector<int64_t> order(n_elem), cptr(n_elem + 1);
+  const int nc = orc_colour_elements(n_nodes, n_elem, elem_ptr, elem_conn, order.data(), cptr.data());
   if (do_residual && f) std::memset(f, 0, sizeof(double) * ndof);
   if (do_tangent && vals) std::memset(vals, 0, sizeof(double) * Ap[ndof]);
-  int err = 0;
-  int64_t qp0 = 0;
-  std::vector<double> lk;
-  for (int64_t e = 0; e < n_elem; ++e) {
+  int err = 0, missing = 0;
+  auto element = [&](int64_t e) -> int {
+    int err = 0;
+    const int64_t qp0 = qoff[e];
+    std::vector<double> lk;
     const int nne = int(elem_ptr[e + 1] - elem_ptr[e]);
     const int nint = orc_nint(nne);
     const int ndofe = nne * 3;
@@ -688,7 +692,81 @@ extern "C" int orc_assemble_cp(int64_t n_nodes, const double *X, int64_t n_elem,
           vals[p - Ai] += lk[i * ndofe + j];
         }
       }
-    qp0 += nint;
+    return err;
+  };
+  for (int c = 0; c < nc; ++c) {
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : err, missing)
+    for (int64_t k = cptr[c]; k < cptr[c + 1]; ++k) {
+      const int r = element(order[k]);
+      if (r < 0) ++missing;
+      else err += r;
+    }
   }
-  return err;
+  return missing ? -1 : err;
+}
+
+// Output stress of one QP from its stored state (constitutive_model_update_output_variables,
+// src/constitutive_model.cc:1738-1875): the model's elastic stress on eF, the Cauchy stress
+// sigma = eF eS eF^T / theta_e (theta_e = det eF) and the PK2 S = J F^-1 sigma F^-T.
+//   model 0 (hyperelastic): eF = F, eS from the potentials (orc_he_update);
+//   model 4 (J2, state Cp^-1): eS pulled back to the reference configuration directly,
+//     S = (1 - w) [G J^-2/3 (Cp^-1 - tr(C Cp^-1)/3 C^-1) + kappa J U'(J) C^-1]
+//     (the stress of the elastic left stretch b_e = F Cp^-1 F^T; w the damage, 0 without);
+//   model 1 (crystal plasticity, state pF): eF = F pF^-1, eS the grain's elastic law.
+// L_grain is the grain's rotated cubic stiffness [81] (law 0 of crystal plasticity).
+extern "C" int orc_output_qp(const orc_material *mat, const double *L_grain, const double *F, const double *plastic,
+                             double w, double *S_out, double *sig_out) {
+  const orc_material &m = *mat;
+  const double J = det3(F);
+  double Fi[9];
+  if (!(J > 0.0) || !inv3(F, Fi)) return 1;
+  double sig[9];
+  if (m.model == 1) {
+    Grain gr{};
+    std::memcpy(gr.L, L_grain, sizeof(gr.L));
+    double pFi[9], eF[9], Se[9], Lb[81], T[9];
+    if (!inv3(plastic, pFi)) return 1;
+    mm(F, pFi, eF);
+    bool ok = true;
+    elastic_SL(m, gr, eF, Se, Lb, &ok);
+    if (!ok) return 1;
+    const double te = det3(eF);
+    mm(eF, Se, T);
+    mmt(T, eF, sig);
+    for (int i = 0; i < 9; ++i) sig[i] /= te;
+  } else {
+    double S[9], T[9];
+    if (m.model == 4) {
+      double C[9], CI[9], Cp[9], CCp[9];
+      mtm(F, F, C);
+      if (!inv3(C, CI)) return 1;
+      Cp[0] = plastic[0]; Cp[4] = plastic[1]; Cp[8] = plastic[2];
+      Cp[1] = Cp[3] = plastic[3];
+      Cp[5] = Cp[7] = plastic[4];
+      Cp[2] = Cp[6] = plastic[5];
+      mm(C, Cp, CCp);
+      const double tr3 = (CCp[0] + CCp[4] + CCp[8]) / 3.0;
+      const double kappa = m.E / (3.0 * (1.0 - 2.0 * m.nu));
+      double dU;
+      const int flag = m.volPotFlag;
+      if (flag == 3) dU = (-1 + J) / (2. * J) + std::log(J) * .5;  // src/new_potentials.cc:575-616
+      else if (flag == 99) dU = J - 1;
+      else dU = (std::exp(-1 + J) - 1. / J) * .5;
+      const double Jm23 = std::pow(J, -2.0 / 3.0);
+      for (int i = 0; i < 9; ++i) S[i] = (1.0 - w) * (m.G * Jm23 * (Cp[i] - tr3 * CI[i]) + kappa * J * dU * CI[i]);
+    } else {
+      if (orc_he_update(&m, F, S, nullptr)) return 1;
+    }
+    mm(F, S, T);
+    mmt(T, F, sig);
+    for (int i = 0; i < 9; ++i) sig[i] /= J;
+  }
+  double T2[9], S2[9];
+  mm(Fi, sig, T2);
+  mmt(T2, Fi, S2);
+  for (int i = 0; i < 9; ++i) {
+    S_out[i] = J * S2[i];
+    sig_out[i] = sig[i];
+  }
+  return 0;
 }

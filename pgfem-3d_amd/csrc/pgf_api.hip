@@ -176,6 +176,7 @@ struct pgf_handle_s {
   double *d_Sc[3] = {nullptr, nullptr, nullptr};  // J2: ep, gam (+ w, X, H, damaged); CP: g, lambda
   int nsc = 2;       // scalar components per QP of d_Sc
   bool j2_damage = false;  // a J2 material with continuum damage (mu > 0)
+  bool he_linear = false;  // a hyperelastic material with the linear deviatoric law (devPotFlag 2)
   int32_t *d_err = nullptr;
   double *d_red = nullptr;  // one scalar of a device reduction (pgf_subdivision_param)
   // crystal plasticity
@@ -491,7 +492,10 @@ int32_t enqueue_pass(pgf_handle h, const pgf_nodal_input *in, int mode, double *
           a.tf_dP = h->d_tf + 8 * NE + b.e_base;
           a.tf_buf = h->d_tfbuf + b.e_base * pgf::kTfBuf;
         }
-        pgf::launch_elem(b.nne, mode, a, e0, e1, h->stream, h->j2_damage && b.model == PGF_MODEL_J2_PLASTICITY);
+        const int var = (h->j2_damage && b.model == PGF_MODEL_J2_PLASTICITY)       ? 2
+                        : (h->he_linear && b.model == PGF_MODEL_HYPER_ELASTICITY) ? 3
+                                                                                  : 0;
+        pgf::launch_elem(b.nne, mode, a, e0, e1, h->stream, var);
       }
       ++launches;
     }
@@ -665,11 +669,16 @@ int pgf_create(const pgf_mesh *mesh, const int64_t *id, int64_t ndof, const int3
     for (int64_t k = m.elem_ptr[e]; k < m.elem_ptr[e + 1]; ++k)
       if (m.elem_conn[k] < 0 || m.elem_conn[k] >= m.n_nodes) return PGF_EINVAL;
   }
-  int n_cp = 0, n_j2 = 0, n_dmg = 0, cp_index = -1;
+  int n_cp = 0, n_j2 = 0, n_dmg = 0, n_lin = 0, cp_index = -1;
   for (int32_t i = 0; i < n_mat; ++i) {
     const int md = mats[i].model;
     if (md == PGF_MODEL_HYPER_ELASTICITY) {
-      if (mats[i].m01 != 0.0 || mats[i].devPotFlag != 1) return PGF_ENOTSUP;  // neo-Hookean only
+      if (mats[i].devPotFlag == 2) {  // the linear law in C (devStress_Linear)
+        if (!(mats[i].G > 0.0)) return PGF_EINVAL;
+        ++n_lin;
+      } else if (mats[i].m01 != 0.0 || mats[i].devPotFlag != 1) {
+        return PGF_ENOTSUP;  // Mooney-Rivlin with m01 = 0 (neo-Hookean) or the linear law
+      }
     } else if (md == PGF_MODEL_J2_PLASTICITY) {
       ++n_j2;
       if (mats[i].mu > 0.0) {  // Weibull law: positive threshold and scales
@@ -750,6 +759,7 @@ int pgf_create(const pgf_mesh *mesh, const int64_t *id, int64_t ndof, const int3
     if (md == PGF_MODEL_CRYSTAL_PLASTICITY) h->has_cp = true;
   }
   h->j2_damage = h->has_j2 && n_dmg > 0;
+  h->he_linear = n_lin > 0;
   h->nsc = h->j2_damage ? 6 : 2;
   // crystal-plasticity blocks first: their QPs then form one contiguous range that the
   // element-type-agnostic QP kernels (local Newton, tangent) cover in single launches
@@ -940,7 +950,9 @@ int pgf_create(const pgf_mesh *mesh, const int64_t *id, int64_t ndof, const int3
   for (int32_t i = 0; i < n_mat; ++i) {
     dm[i].model = mats[i].model;
     dm[i].volPotFlag = mats[i].volPotFlag;
-    dm[i].G = (mats[i].model == PGF_MODEL_HYPER_ELASTICITY) ? 2.0 * mats[i].m10 : mats[i].G;
+    dm[i].devPotFlag = mats[i].devPotFlag;
+    dm[i].G = (mats[i].model == PGF_MODEL_HYPER_ELASTICITY && mats[i].devPotFlag != 2) ? 2.0 * mats[i].m10
+                                                                                        : mats[i].G;
     dm[i].kappa = mats[i].E / (3.0 * (1.0 - 2.0 * mats[i].nu));
     dm[i].k0 = mats[i].k0;
     dm[i].hp = mats[i].hp;
@@ -1149,6 +1161,87 @@ int pgf_residual_tangent(pgf_handle h, const pgf_nodal_input *in, double dt, dou
   return run_pass(h, in, pgf::MODE_INTEGRATE | pgf::MODE_RESIDUAL | pgf::MODE_TANGENT, f, vals);
 }
 
+int pgf_output_variables(pgf_handle h, int32_t level, double *S, double *E, double *sigma, double *eps) {
+  if (!h || level < 0 || level > 2) return PGF_EINVAL;
+  const int64_t NQ = h->n_qp, NE = h->n_elem;
+  double *d_S = nullptr, *d_E = nullptr, *d_sig = nullptr, *d_eps = nullptr;
+  int rc = pgf::dalloc(&d_S, size_t(6 * NQ));
+  if (!rc) rc = pgf::dalloc(&d_E, size_t(6 * NQ));
+  if (!rc) rc = pgf::dalloc(&d_sig, size_t(6 * NE));
+  if (!rc) rc = pgf::dalloc(&d_eps, size_t(6 * NE));
+  std::vector<double> tq(6 * NQ), te(6 * NE);
+  if (!rc) {
+    for (auto &b : h->blocks) {
+      pgf::OutArgs o{};
+      o.X = h->d_X;
+      o.conn = b.d_conn;
+      o.mat = b.d_mat;
+      o.mats = h->d_mats;
+      o.model = b.model;
+      o.qgrain = b.d_qgrain;
+      o.grains = h->d_grains;
+      o.cpmats = h->d_cpmat;
+      o.qp_base = b.qp_base;
+      o.n_qp = NQ;
+      o.ne = b.ne;
+      o.F = h->d_F[level];
+      o.P = (h->has_j2 || h->has_cp) ? h->d_P[level] : nullptr;
+      o.Sc = (h->has_j2 || h->has_cp) ? h->d_Sc[level] : nullptr;
+      o.S = d_S;
+      o.E = d_E;
+      o.sig = d_sig + 6 * b.e_base;  // the block's [6][ne] rows
+      o.eps = d_eps + 6 * b.e_base;
+      pgf::launch_output(b.nne, o, h->stream);
+    }
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(h->stream) != hipSuccess) rc = PGF_EHIP;
+  }
+  for (int pass = 0; pass < 2 && !rc; ++pass) {
+    double *out = pass == 0 ? S : E;
+    if (!out) continue;
+    if (hipMemcpy(tq.data(), pass == 0 ? d_S : d_E, sizeof(double) * tq.size(), hipMemcpyDeviceToHost) != hipSuccess) {
+      rc = PGF_EHIP;
+      break;
+    }
+    for (int c = 0; c < 6; ++c)
+      for (int64_t q = 0; q < NQ; ++q) out[h->qp_mesh[q] * 6 + c] = tq[c * NQ + q];
+  }
+  for (int pass = 0; pass < 2 && !rc; ++pass) {
+    double *out = pass == 0 ? sigma : eps;
+    if (!out) continue;
+    if (hipMemcpy(te.data(), pass == 0 ? d_sig : d_eps, sizeof(double) * te.size(), hipMemcpyDeviceToHost) !=
+        hipSuccess) {
+      rc = PGF_EHIP;
+      break;
+    }
+    for (auto &b : h->blocks)  // block b's rows are [6][b.ne] at e_base * 6
+      for (int c = 0; c < 6; ++c)
+        for (int64_t k = 0; k < b.ne; ++k) out[b.elems[k] * 6 + c] = te[b.e_base * 6 + c * b.ne + k];
+  }
+  hipFree(d_S);
+  hipFree(d_E);
+  hipFree(d_sig);
+  hipFree(d_eps);
+  return rc;
+}
+
+int pgf_cp_start_stats(pgf_handle h, int64_t *n_learned, int64_t *n_fallback) {
+  if (!h || !n_learned || !n_fallback) return PGF_EINVAL;
+  if (!h->has_cp) return PGF_ENOTSUP;
+  std::vector<int32_t> it(h->cp_nq);
+  if (hipStreamSynchronize(h->stream) != hipSuccess) return PGF_EHIP;
+  if (h->cp_nq && hipMemcpy(it.data(), h->d_iters, sizeof(int32_t) * h->cp_nq, hipMemcpyDeviceToHost) != hipSuccess)
+    return PGF_EHIP;
+  int64_t nl = 0, nf = 0;
+  for (const int32_t v : it) {
+    const int kind = v >> 24;
+    nl += kind == 1;
+    nf += kind == 2;
+  }
+  *n_learned = nl;
+  *n_fallback = nf;
+  return PGF_OK;
+}
+
 int pgf_subdivision_param(pgf_handle h, double dt, double *alpha) {
   if (!h || !alpha) return PGF_EINVAL;
   if (!(dt > 0.0)) dt = h->dt;
@@ -1231,6 +1324,7 @@ int pgf_set_three_field(pgf_handle h, int32_t on) {
   }
   for (auto &b : h->blocks)
     if (b.model != PGF_MODEL_HYPER_ELASTICITY) return PGF_ENOTSUP;
+  if (h->he_linear) return PGF_ENOTSUP;  // the three-field element takes the neo-Hookean law
   if (h->tf) return PGF_OK;
   const int64_t NE = h->n_elem;
   int rc = pgf::dalloc(&h->d_tf, size_t(pgf::kTfSlots * NE));
@@ -1385,7 +1479,8 @@ int pgf_get_state(pgf_handle h, int32_t field, double *host_out) {
     if (hipStreamSynchronize(h->stream) != hipSuccess) return PGF_EHIP;
     if (hipMemcpy(it.data(), h->d_iters, sizeof(int32_t) * h->n_qp, hipMemcpyDeviceToHost) != hipSuccess)
       return PGF_EHIP;
-    for (int64_t q = 0; q < h->n_qp; ++q) host_out[h->qp_mesh[q]] = double(it[q]);
+    // the local Newton keeps the start it took in the high byte (pgf_cp_start_stats)
+    for (int64_t q = 0; q < h->n_qp; ++q) host_out[h->qp_mesh[q]] = double(it[q] & 0xffffff);
     return PGF_OK;
   }
   double *buf;

@@ -620,7 +620,8 @@ __global__ __launch_bounds__(64) void cp_newton(CPArgs a, int64_t chunk) {
   int64_t q = -1;
   double x[11], A[9], det_pFn = 1.0, g_n = 1.0, gscale = 1.0, dt = a.dt;
   int it = 0, it_att = 0, gid = 0;
-  bool bad_start = false, from_warm = false;
+  bool bad_start = false;
+  int kind = 0;  // the QP's start: 0 plain, 1 the caller's (learned), 2 the caller's, then the plain fallback
   CPMaterial cm{};
   for (;;) {
     // refill the lanes that need a QP, in lane order, from the wave's range
@@ -652,7 +653,7 @@ __global__ __launch_bounds__(64) void cp_newton(CPArgs a, int64_t chunk) {
         it_att = 0;
         // start: the caller's warm start, else the plain start
         bad_start = false;
-        from_warm = false;
+        kind = 0;
         {
           double Fn[9];
 #pragma unroll
@@ -670,7 +671,7 @@ __global__ __launch_bounds__(64) void cp_newton(CPArgs a, int64_t chunk) {
           if (okw && (!okp || mw < mp)) {
 #pragma unroll
             for (int i = 0; i < 11; ++i) x[i] = xw[i];
-            from_warm = true;
+            kind = 1;
             bad_start = false;
           }
         }
@@ -746,10 +747,10 @@ __global__ __launch_bounds__(64) void cp_newton(CPArgs a, int64_t chunk) {
       }
     }
     if (!conv && !fail && it_att >= cm.maxit) fail = true;
-    if (fail && from_warm) {
+    if (fail && kind == 1) {
       // the warm start did not converge: fall back to the plain start with a fresh budget
       // (iterations keep counting)
-      from_warm = false;
+      kind = 2;
       it_att = 0;
       double F[9], pFn[9], pFi[9], Fn[9];
 #pragma unroll
@@ -763,7 +764,7 @@ __global__ __launch_bounds__(64) void cp_newton(CPArgs a, int64_t chunk) {
       fail = false;
     }
     if (conv || fail) {
-      a.iters[qg] = it;
+      a.iters[qg] = it | (kind << 24);  // iterations, and the start in the high byte
       if (fail) {
         atomicAdd(a.err, 1);
       } else {
@@ -1316,6 +1317,135 @@ void launch_ce_mode(int mode, const CPElemArgs &a, int64_t e0, int64_t e1, hipSt
   }
 }
 
+// ------------------------------------------------------------------- output variables
+// One thread per element of a block: the stress from the stored state at the level, as
+// constitutive_model_update_output_variables forms it (src/constitutive_model.cc:1738-1875):
+// eF = F pF^-1 (J2 and the hyperelastic model: the elastic left stretch from Cp^-1), the elastic
+// stress of the model on eF, sigma = eF eS eF^T / theta_e, S = theta F^-1 sigma F^-T,
+// E = (F^T F - I)/2 per QP; sigma and e = (I - (F F^T)^-1)/2 (compute_Eulerian_Almansi_strain,
+// src/utils.cc:46-56) averaged over the element with detJ w, shear strains doubled.
+template <int NNE>
+__global__ __launch_bounds__(256) void output_kernel(OutArgs o) {
+  constexpr int NQP = nqp_of(NNE);
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= o.ne) return;
+  double x[NNE * 3];
+#pragma unroll
+  for (int n = 0; n < NNE; ++n) {
+    const int32_t node = o.conn[e * NNE + n];
+#pragma unroll
+    for (int b = 0; b < 3; ++b) x[n * 3 + b] = o.X[int64_t(node) * 3 + b];
+  }
+  // the reference's output order XX, YY, ZZ, YZ, XZ, XY as (i, j)
+  const int oi[6] = {0, 1, 2, 1, 0, 0}, oj[6] = {0, 1, 2, 2, 2, 1};
+  double vol = 0.0, sa[6] = {0, 0, 0, 0, 0, 0}, ea[6] = {0, 0, 0, 0, 0, 0};
+  const DevMaterial m = o.mats[o.mat ? o.mat[e] : 0];
+  for (int ip = 0; ip < NQP; ++ip) {
+    double k, ee, z, wt;
+    int_point(NNE, ip, k, ee, z, wt);
+    double dN[NNE * 3];
+    shape_dN<NNE>(k, ee, z, dN);
+    double dX[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int n = 0; n < NNE; ++n)
+#pragma unroll
+      for (int ib = 0; ib < 3; ++ib)
+#pragma unroll
+        for (int ic = 0; ic < 3; ++ic) dX[ib * 3 + ic] += dN[n * 3 + ic] * x[n * 3 + ib];
+    const double w = det3(dX) * wt;
+    const int64_t q = o.qp_base + e * NQP + ip;
+    const int64_t NQ = o.n_qp;
+    double F[9], Fi[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) F[i] = o.F[i * NQ + q];
+    const double J = det3(F);
+    inv3(F, Fi);
+    double sig[9];  // Cauchy
+    if (o.model == PGF_MODEL_CRYSTAL_PLASTICITY) {
+      const CPMaterial cm = o.cpmats[0];
+      double pF[9], pFi[9], eF[9], Ce[9], S6[6], S9[9], T[9];
+#pragma unroll
+      for (int i = 0; i < 9; ++i) pF[i] = o.P[i * NQ + q];
+      inv3(pF, pFi);
+      mm3(F, pFi, eF);
+      mtm3(eF, eF, Ce);
+      HeTan t;
+      if (cm.law == 1) el_stress<true>(cm, o.grains[o.qgrain[q - o.qp_base]], eF, Ce, S6, t);
+      else el_stress<false>(cm, o.grains[o.qgrain[q - o.qp_base]], eF, Ce, S6, t);
+      v6_to_9(S6, S9);
+      const double rte = 1.0 / det3(eF);
+      mm3(eF, S9, T);
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+          sig[i * 3 + j] = rte * (T[i * 3] * eF[j * 3] + T[i * 3 + 1] * eF[j * 3 + 1] + T[i * 3 + 2] * eF[j * 3 + 2]);
+    } else {
+      // bbar_e = J^-2/3 F Cp^-1 F^T (Cp^-1 = I for the hyperelastic model); tau = G dev bbar_e +
+      // kappa J U'(J) 1, scaled by (1 - w) with J2 damage; sigma = tau / J
+      double Cn[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+      if (o.model == PGF_MODEL_J2_PLASTICITY) {
+        const double c6[6] = {o.P[0 * NQ + q], o.P[1 * NQ + q], o.P[2 * NQ + q],
+                              o.P[3 * NQ + q], o.P[4 * NQ + q], o.P[5 * NQ + q]};
+        Cn[0] = c6[0]; Cn[4] = c6[1]; Cn[8] = c6[2];
+        Cn[1] = Cn[3] = c6[3];
+        Cn[5] = Cn[7] = c6[4];
+        Cn[2] = Cn[6] = c6[5];
+      }
+      double T[9], b[9];
+      mm3(F, Cn, T);
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+          b[i * 3 + j] = T[i * 3] * F[j * 3] + T[i * 3 + 1] * F[j * 3 + 1] + T[i * 3 + 2] * F[j * 3 + 2];
+      const double cj = cbrt(J);
+      const double Jm23 = 1.0 / (cj * cj);
+      const double tr3 = (b[0] + b[4] + b[8]) * (1.0 / 3.0);
+      double dU, d2U;
+      vol_pot(m.volPotFlag, J, dU, d2U);
+      const double om = (o.model == PGF_MODEL_J2_PLASTICITY && m.mu > 0.0) ? 1.0 - o.Sc[2 * NQ + q] : 1.0;
+      const double rJ = 1.0 / J;
+      if (o.model == PGF_MODEL_HYPER_ELASTICITY && m.devPotFlag == 2) {
+        // the linear law in C: tau = G (b b - b) + kappa J U'(J) 1
+        double bb[9];
+        mm3(b, b, bb);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) sig[i] = rJ * (m.G * (bb[i] - b[i]) + ((i % 4 == 0) ? m.kappa * J * dU : 0.0));
+      } else {
+#pragma unroll
+        for (int i = 0; i < 9; ++i)
+          sig[i] = om * rJ * (m.G * Jm23 * (b[i] - ((i % 4 == 0) ? tr3 : 0.0)) + ((i % 4 == 0) ? m.kappa * J * dU : 0.0));
+      }
+    }
+    // S = J F^-1 sigma F^-T, E = (F^T F - I)/2, e = (I - (F F^T)^-1)/2 = (I - Fi^T Fi)/2
+    double T[9], S9[9], C[9], bi[9];
+    mm3(Fi, sig, T);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+        S9[i * 3 + j] = J * (T[i * 3] * Fi[j * 3] + T[i * 3 + 1] * Fi[j * 3 + 1] + T[i * 3 + 2] * Fi[j * 3 + 2]);
+    mtm3(F, F, C);
+    mtm3(Fi, Fi, bi);
+#pragma unroll
+    for (int I = 0; I < 6; ++I) {
+      const int ij = oi[I] * 3 + oj[I];
+      o.S[I * NQ + q] = S9[ij];
+      o.E[I * NQ + q] = 0.5 * (C[ij] - (I < 3 ? 1.0 : 0.0));
+      sa[I] += w * sig[ij];
+      ea[I] += w * (I < 3 ? 1.0 : 2.0) * 0.5 * ((I < 3 ? 1.0 : 0.0) - bi[ij]);
+    }
+    vol += w;
+  }
+  const double rv = 1.0 / vol;
+#pragma unroll
+  for (int I = 0; I < 6; ++I) {
+    o.sig[I * o.ne + e] = sa[I] * rv;
+    o.eps[I * o.ne + e] = ea[I] * rv;
+  }
+}
+
 // ------------------------------------------------------------------- learned warm start
 // Features of one QP (device QP order rows of 34): F(n) pF(n)^-1, F(n+1) - F(n), 12 x g(n)/g0,
 // grain quaternion; with shift/scale the row is normalised as (x - shift) * scale.
@@ -1404,6 +1534,14 @@ void launch_cp_targets(const CPArgs &a, float *T, hipStream_t s) {
 void launch_cp_warm_from_slip(const CPArgs &a, const float *Y, int ldy, const float *out_scale, double *warm,
                               hipStream_t s) {
   if (a.nq > 0) cp_warm_from_slip<<<unsigned((a.nq + 255) / 256), 256, 0, s>>>(a, Y, ldy, out_scale, warm);
+}
+
+void launch_output(int nne, const OutArgs &o, hipStream_t s) {
+  if (o.ne <= 0) return;
+  const unsigned grid = unsigned((o.ne + 255) / 256);
+  if (nne == 4) output_kernel<4><<<grid, 256, 0, s>>>(o);
+  else if (nne == 10) output_kernel<10><<<grid, 256, 0, s>>>(o);
+  else if (nne == 8) output_kernel<8><<<grid, 256, 0, s>>>(o);
 }
 
 void launch_cp_kinematics(int nne, const CPArgs &a, hipStream_t s) {

@@ -29,7 +29,9 @@ __host__ __device__ constexpr int stage_block(int nne) { return (3 * (3 * nne + 
 struct DevMaterial {
   int32_t model;
   int32_t volPotFlag;
-  double G;       // shear modulus of the deviatoric part (neo-Hookean: 2*m10)
+  int32_t devPotFlag;  // hyperelastic deviatoric law: 1 neo-Hookean, 2 linear in C
+  int32_t pad;
+  double G;       // shear modulus of the deviatoric part (neo-Hookean: 2*m10; linear law: G)
   double kappa;   // bulk modulus E/(3(1-2nu))
   double k0, hp;  // J2
   // J2 continuum damage (viscosity mu, cap wmax, Weibull p1, p2, threshold Yin); mu <= 0: off
@@ -143,6 +145,25 @@ struct CPArgs {
 };
 constexpr int kCPBuf = 97;  // doubles per QP in qpbuf
 
+// Output variables of one block at one time level (constitutive_model_update_output_variables,
+// src/constitutive_model.cc:1677-1881): per QP the PK2 stress S and the Green-Lagrange strain E,
+// per element the volume averages of the Cauchy stress and of the Almansi strain (shears
+// doubled), all in the reference's component order XX, YY, ZZ, YZ, XZ, XY.
+struct OutArgs {
+  const double *X;
+  const int32_t *conn, *mat;        // block connectivity, material per block element
+  const DevMaterial *mats;
+  int model;                        // the block's model
+  const int32_t *qgrain;            // crystal plasticity: grain per block QP
+  const DevGrain *grains;
+  const CPMaterial *cpmats;
+  int64_t qp_base, n_qp, ne;
+  const double *F, *P, *Sc;         // state at the level: F 9, plastic (Cp^-1 6 / pF 9), scalars
+  double *S, *E;                    // [6][n_qp] device QP order
+  double *sig, *eps;                // [6][ne] block elements
+};
+void launch_output(int nne, const OutArgs &o, hipStream_t s);
+
 void launch_cp_kinematics(int nne, const CPArgs &a, hipStream_t s);
 void launch_cp_newton(const CPArgs &a, int64_t chunk, hipStream_t s);
 void launch_cp_tangent(const CPArgs &a, bool with_D, hipStream_t s);
@@ -200,9 +221,11 @@ struct CohArgs {
 };
 void launch_coh_elem(int nn, int mode, const CohArgs &a, int64_t e_begin, int64_t e_end, hipStream_t s);
 
-// damage: the block's J2 points carry continuum damage (state scalars w, X, H, damaged)
+// variant: 0 plain; 2 the block's J2 points carry continuum damage (state scalars w, X, H,
+// damaged); 3 hyperelastic materials that may take the linear deviatoric law (devPotFlag 2).
+// (The three-field variant is selected by a.tf_V.)
 void launch_elem(int nne, int mode, const ElemArgs &a, int64_t e_begin, int64_t e_end, hipStream_t s,
-                 bool damage = false);
+                 int variant = 0);
 
 // Gather assembly: node n owns the staging blocks [inc_ptr[n], inc_ptr[n+1]) (one per
 // incident element, in a fixed order); block k starts at inc_soff[k], comes from an element

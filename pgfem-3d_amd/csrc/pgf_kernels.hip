@@ -99,6 +99,57 @@ __device__ __forceinline__ void j2_c_row(const J2Tan &T, int I, double w, double
   }
 }
 
+// The linear deviatoric law in C of the hyperelastic model (devPotFlag 2: devStress_Linear /
+// matStiff_Linear, src/new_potentials.cc:360-373, :434-453, S = G (C - I), L = 2G I) with the
+// volumetric potential, in the spatial form of the element kernel:
+//   tau = G (b b - b) + kappa J U'(J) 1,  c = G (b_ik b_jl + b_il b_jk) + the volumetric terms
+// (b = F F^T; the minor-symmetric form of the push-forward of 2G delta_ik delta_jl, which
+// the element rows contract only with symmetric tensors).
+struct LinTan {
+  double b[6], G, cv1, cv2;
+};
+
+__device__ __forceinline__ void he_lin_point(const DevMaterial &m, const double *Fs, double J, double *tau,
+                                             LinTan &t) {
+#pragma unroll
+  for (int I = 0; I < 6; ++I) {
+    const int i = voigt_i(I), j = voigt_j(I);
+    t.b[I] = Fs[i * 3 + 0] * Fs[j * 3 + 0] + Fs[i * 3 + 1] * Fs[j * 3 + 1] + Fs[i * 3 + 2] * Fs[j * 3 + 2];
+  }
+  double B[9];
+  B[0] = t.b[0]; B[4] = t.b[1]; B[8] = t.b[2];
+  B[1] = B[3] = t.b[3];
+  B[5] = B[7] = t.b[4];
+  B[2] = B[6] = t.b[5];
+  double dU, d2U;
+  vol_pot(m.volPotFlag, J, dU, d2U);
+  const double p = m.kappa * J * dU;
+#pragma unroll
+  for (int I = 0; I < 6; ++I) {
+    const int i = voigt_i(I), j = voigt_j(I);
+    const double bb = B[i * 3 + 0] * B[0 * 3 + j] + B[i * 3 + 1] * B[1 * 3 + j] + B[i * 3 + 2] * B[2 * 3 + j];
+    tau[I] = m.G * (bb - t.b[I]) + (I < 3 ? p : 0.0);
+  }
+  t.G = m.G;
+  t.cv1 = m.kappa * (J * J * d2U + J * dU);
+  t.cv2 = -2.0 * p;
+}
+
+__device__ __forceinline__ void lin_c_row(const LinTan &t, int I, double w, double *row) {
+  const int i = voigt_i(I), j = voigt_j(I);
+  auto bv = [&](int a, int c) {  // b_ac from the Voigt entries
+    const int K = (a == c) ? a : (a + c == 1 ? 3 : (a + c == 3 ? 4 : 5));
+    return pick6(t.b[0], t.b[1], t.b[2], t.b[3], t.b[4], t.b[5], K);
+  };
+#pragma unroll
+  for (int K = 0; K < 6; ++K) {
+    const int k = voigt_i(K), l = voigt_j(K);
+    const double oo = (I < 3 && K < 3) ? 1.0 : 0.0;
+    const double Is = (I == K) ? (I < 3 ? 1.0 : 0.5) : 0.0;
+    row[K] = w * (t.G * (bv(i, k) * bv(j, l) + bv(i, l) * bv(j, k)) + t.cv1 * oo + t.cv2 * Is);
+  }
+}
+
 // The damage update of a J2 point from the total F (Fs, J), restating the reference's
 // CM_J2P_PARAM::integration_algorithm damage step (src/cm_j2_plasticity.cc:184-212):
 // Y0 = G/2 (tr bbar - 3) + kappa U(J), bbar = J^-2/3 Fs Fs^T (j2d_compute_Y0, :162-182), and
@@ -286,12 +337,15 @@ constexpr int stage_round(int epb, int nde, int room) {
 // tau and c take the element pressure, and the rows get the condensation terms.  Its staged
 // variant runs at three waves per SIMD: at four (128 registers) the extra element terms spill.
 // VAR: 0 the displacement element, 1 the three-field element (TF), 2 the displacement
-// element with J2 continuum damage (DMG: the undamaged kernels keep their register budget).
+// element with J2 continuum damage (DMG: the undamaged kernels keep their register budget),
+// 3 the hyperelastic displacement element whose materials may take the linear deviatoric law
+// (LIN, chosen per element from devPotFlag).
 template <int NNE, int MODE, bool STG, int VAR>
 __global__ __launch_bounds__(256, STG ? (VAR != 0 ? 3 : 4) : 1) void elem_kernel(ElemArgs a, int64_t e_begin,
                                                                                 int64_t e_end) {
   constexpr bool TF = VAR == 1;
   constexpr bool DMG = VAR == 2;
+  constexpr bool LIN = VAR == 3;
   constexpr int NQP = nqp_of(NNE);
   constexpr int NDE = NNE * 3;
   constexpr int TPE = (NNE == 4) ? 16 : 32;
@@ -490,7 +544,12 @@ __global__ __launch_bounds__(256, STG ? (VAR != 0 ? 3 : 4) : 1) void elem_kernel
     for (int I = 0; I < 6; ++I) Cn[I] = pCn[I];
     double Cnp1[6], ep1, gm, tau[6];
     J2Tan T;
-    j2_point<TAN, TF>(m, is_j2, Fs, Fi, ok ? J : 1.0, Cn, ep_n, INTEG && is_j2, Cnp1, ep1, gm, tau, T, ppr);
+    LinTan lt;
+    const bool lin = LIN && m.devPotFlag == 2;
+    if (lin)
+      he_lin_point(m, Fs, ok ? J : 1.0, tau, lt);
+    else
+      j2_point<TAN, TF>(m, is_j2, Fs, Fi, ok ? J : 1.0, Cn, ep_n, INTEG && is_j2, Cnp1, ep1, gm, tau, T, ppr);
     J2Dam dam;
     double dw = pw, dXs = pX, dH = 0.0, dflag = 0.0;
     if constexpr (DMG) {
@@ -558,7 +617,12 @@ __global__ __launch_bounds__(256, STG ? (VAR != 0 ? 3 : 4) : 1) void elem_kernel
 #pragma unroll
       for (int t = 0; t < (6 + LPQ - 1) / LPQ; ++t) {
         const int I = sub + t * LPQ;
-        if (I < 6) j2_c_row<DMG>(T, I, wq, &s_c[slot][ip][I * 6], &dam);
+        if (I < 6) {
+          if (lin)
+            lin_c_row(lt, I, wq, &s_c[slot][ip][I * 6]);
+          else
+            j2_c_row<DMG>(T, I, wq, &s_c[slot][ip][I * 6], &dam);
+        }
       }
     }
     if constexpr (GEN) {  // Gr = Fr Fs^-1
@@ -1035,7 +1099,7 @@ void launch_nne(int mode, const ElemArgs &a, int64_t e_begin, int64_t e_end, hip
 }  // namespace
 
 void launch_elem(int nne, int mode, const ElemArgs &a, int64_t e_begin, int64_t e_end, hipStream_t s,
-                 bool damage) {
+                 int variant) {
   const bool stg = a.stage != nullptr;
   if (a.tf_V) {
     if (nne == 4) (stg ? launch_nne<4, true, 1> : launch_nne<4, false, 1>)(mode, a, e_begin, e_end, s);
@@ -1043,10 +1107,16 @@ void launch_elem(int nne, int mode, const ElemArgs &a, int64_t e_begin, int64_t 
     else if (nne == 8) (stg ? launch_nne<8, true, 1> : launch_nne<8, false, 1>)(mode, a, e_begin, e_end, s);
     return;
   }
-  if (damage) {
+  if (variant == 2) {
     if (nne == 4) (stg ? launch_nne<4, true, 2> : launch_nne<4, false, 2>)(mode, a, e_begin, e_end, s);
     else if (nne == 10) (stg ? launch_nne<10, true, 2> : launch_nne<10, false, 2>)(mode, a, e_begin, e_end, s);
     else if (nne == 8) (stg ? launch_nne<8, true, 2> : launch_nne<8, false, 2>)(mode, a, e_begin, e_end, s);
+    return;
+  }
+  if (variant == 3) {
+    if (nne == 4) (stg ? launch_nne<4, true, 3> : launch_nne<4, false, 3>)(mode, a, e_begin, e_end, s);
+    else if (nne == 10) (stg ? launch_nne<10, true, 3> : launch_nne<10, false, 3>)(mode, a, e_begin, e_end, s);
+    else if (nne == 8) (stg ? launch_nne<8, true, 3> : launch_nne<8, false, 3>)(mode, a, e_begin, e_end, s);
     return;
   }
   if (nne == 4) (stg ? launch_nne<4, true, 0> : launch_nne<4, false, 0>)(mode, a, e_begin, e_end, s);

@@ -99,6 +99,8 @@ def lib() -> C.CDLL:
             "pgf_prescribed_load": [P, P, D, P],
             "pgf_residual_tangent": [P, P, D, P, P],
             "pgf_subdivision_param": [P, D, P],
+            "pgf_cp_start_stats": [P, P, P],
+            "pgf_output_variables": [P, I32, P, P, P, P],
             "pgf_commit_step": [P],
             "pgf_reset_step": [P],
             "pgf_state_ncomp": [P, I32, P],
@@ -153,7 +155,7 @@ def exported_symbols():
         "pgf_mlp_get_params", "pgf_mlp_set_params", "pgf_mlp_get_grads", "pgf_mlp_forward", "pgf_mlp_backward",
         "pgf_mlp_adam_step", "pgf_cp_log_samples", "pgf_set_surrogate", "pgf_solve", "pgf_set_graph", "pgf_set_assembly",
         "pgf_set_three_field", "pgf_tf_initial", "pgf_tf_update", "pgf_tf_accept", "pgf_tf_get", "pgf_tf_set",
-        "pgf_autotune", "pgf_subdivision_param",
+        "pgf_autotune", "pgf_subdivision_param", "pgf_cp_start_stats", "pgf_output_variables",
     ]
 
 
@@ -385,6 +387,24 @@ class Handle:
         self._check_vec(vals, self.nnz, "vals")
         return _check(self._lib.pgf_residual_tangent(self.h, C.byref(inp), dt, _ptr(f), _ptr(vals)),
                       "pgf_residual_tangent")
+
+    def output_variables(self, level=1):
+        """(S, E per QP [n_qp, 6]; Cauchy stress and Almansi strain element averages
+        [n_elem, 6]) at the time level (0 = n+1, 1 = n, 2 = n-1), components XX YY ZZ YZ XZ XY."""
+        S = np.zeros((self.n_qp, 6))
+        E = np.zeros((self.n_qp, 6))
+        sig = np.zeros((self.n_elem, 6))
+        eps = np.zeros((self.n_elem, 6))
+        _check(self._lib.pgf_output_variables(self.h, int(level), _ptr(S), _ptr(E), _ptr(sig), _ptr(eps)),
+               "pgf_output_variables")
+        return S, E, sig, eps
+
+    def cp_start_stats(self):
+        """(QPs converged from the learned start, QPs that fell back to the plain start) of the
+        last crystal-plasticity integration."""
+        a, b = C.c_int64(), C.c_int64()
+        _check(self._lib.pgf_cp_start_stats(self.h, C.byref(a), C.byref(b)), "pgf_cp_start_stats")
+        return a.value, b.value
 
     def subdivision_param(self, dt=0.0) -> float:
         """Largest per-QP physics-based subdivision measure of the last integration

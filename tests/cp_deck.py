@@ -74,10 +74,12 @@ def compare_with_fixture(prob, u, F_n, pF_n, g_n):
     return du, dF, dpF, dg
 
 
-def run_oracle(prob, rot, nsteps, tol=1e-10, predictor=False, iter_max=None, computer_zero=1e-15):
+def run_oracle(prob, rot, nsteps, tol=1e-10, predictor=False, iter_max=None, computer_zero=1e-15, alphas=None):
     """The deck's load steps over the oracle (Newton to tol on the normalised residual, sparse
     direct solve; with predictor each step opens with K du = -f_defl at (u_n, defl_n), as the
-    reference's Newton does): (nodal u, CPState, prescribed values, iterations per step)."""
+    reference's Newton does): (nodal u, CPState, prescribed values, iterations per step).
+    alphas (a list) receives each step's physics-based subdivision measure,
+    max over QPs of dt sum |gamma_dot| / 0.005 (CP_PARAM::get_subdiv_param)."""
     tabs = [O.cp_grain(prob.mat, x) for x in rot]
     gP = np.stack([t[0] for t in tabs])
     gL = np.stack([t[1] for t in tabs])
@@ -99,6 +101,8 @@ def run_oracle(prob, rot, nsteps, tol=1e-10, predictor=False, iter_max=None, com
             r, it, _ = oracle_newton_step(prob, st, r, defl, state, tol=tol, max_iter=40, cp_tables=(gP, gL),
                                           dt=dt, iter_max=iter_max, computer_zero=computer_zero)
         its.append(it)
+        if alphas is not None:
+            alphas.append(float((np.abs(st.gdot).sum(axis=0) * dt / 0.005).max()))
     return nodal_u(prob, r, defl), st, defl, its
 
 

@@ -264,3 +264,27 @@ def test_j2_damage_tangent_plastic_differs_only_by_the_symmetrisation():
     stress = lambda G: O.j2d_update(J2D, G, cn, 0.0, 0.0, 0.0, dt, False)["S"]  # noqa: E731
     assert _fd_tangent_err(stress, r["L"] - skew, F) < 1e-6
     assert _fd_tangent_err(stress, r["L"], F) > 1e-5
+
+
+# ------------------------------------------------------------------ linear deviatoric law
+HE_LIN = P.Material(model=P.MODEL_HYPER_ELASTICITY, devPotFlag=2, volPotFlag=99, E=3.0, nu=0.25, G=1.2)
+
+
+def test_he_linear_law_patch_closed_form():
+    """devPotFlag 2 (devStress_Linear, S_dev = G (C - I)) with the quadratic volumetric potential
+    (volPotFlag 99): F = diag(2, 1, 1) gives S_xx = G (l^2 - 1) + kappa J (J - 1) / l^2 = 3.6 + 1.0
+    = 4.6, the reference's patch-test value (verification_tests/output/stress_tension_disp.out)
+    for its material line (Ex 3, Gyz 1.2, nyz 0.25 -> kappa = 2)."""
+    S, _, rc = O.he_update(HE_LIN, np.diag([2.0, 1.0, 1.0]))
+    # lateral: S_yy = G (1 - 1) + kappa J (J - 1) = 4
+    assert rc == 0 and abs(S[0, 0] - 4.6) < 1e-14 and abs(S[1, 1] - 4.0) < 1e-14 and abs(S[0, 1]) == 0
+
+
+def test_he_linear_law_tangent_matches_fd():
+    rng = np.random.default_rng(12)
+    for _ in range(3):
+        F = _rand_F(rng, 0.05)
+        _, L, rc = O.he_update(HE_LIN, F)
+        assert rc == 0
+        stress = lambda G: O.he_update(HE_LIN, G, False)[0]  # noqa: E731
+        assert _fd_tangent_err(stress, L, F) < 1e-7

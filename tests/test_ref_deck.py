@@ -29,11 +29,16 @@ def test_deck_shape(prob):
 
 
 def test_oracle_matches_reference_restart(prob):
-    u, st, _, _ = cp_deck.run_oracle(prob, prob.rot, cp_deck.FIXTURE_STEP + 1)
+    alphas = []
+    u, st, _, _ = cp_deck.run_oracle(prob, prob.rot, cp_deck.FIXTURE_STEP + 1, alphas=alphas)
     du, dF, dpF, dg = cp_deck.compare_with_fixture(prob, u, st.F_n, st.pF_n, st.g_n)
     assert du < TOL_U and dF < TOL_F and dpF < TOL_PF and dg < TOL_G, (du, dF, dpF, dg)
     # the stretch is large enough for the fixture to tell the state apart from elastic response
     assert st.g_n.max() > 1.05 * prob.mat.g0
+    # the physics-based measure stays far below the 1.25 that would subdivide a step
+    # (Newton_Raphson.cc:1390): the reference's split of its last steps (time_step_info.res)
+    # came from its iteration cap, not from this rule
+    assert len(alphas) == cp_deck.FIXTURE_STEP + 1 and 0.1 < max(alphas) < 0.25, max(alphas)
 
 
 def test_transposed_orientation_misses(prob):
