@@ -87,6 +87,16 @@ def measured_traffic(key):
         return None
 
 
+def measured_flops(key):
+    """FP64 FLOPs per pass from the committed counter summary (64 x (ADD + MUL + TRANS +
+    2 FMA) F64 wave instructions per dispatch, summed over the pass's kernels), or None."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "fp64_counters.json")) as fh:
+            return json.load(fh)[key]["flops_per_pass"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def _timed_passes(h, f, vals, rd, dd, steps, warmup, dt=1.0):
     import torch
 
@@ -132,6 +142,21 @@ def _oracle_threads(n):
     L.orc_set_threads.restype = C.c_int
     L.orc_set_threads.argtypes = [C.c_int]
     return L.orc_set_threads(int(n))
+
+
+def fp64_counter_fields(flops, kern_ms):
+    """The counter-derived FP64 rate of a pass (executed FP64 instructions, profiles/
+    fp64_counters.md) at the live pass time, beside the source-count rate."""
+    if flops is None:
+        return {"counter_flops_per_pass": None}
+    t = flops / (kern_ms * 1e-3) / 1e12
+    return {"counter_flops_per_pass": flops, "counter_tflops": t, "counter_frac": t / FP64_PEAK_TFLOPS,
+            "counter_source": "profiles/fp64_counters.md"}
+
+
+def fp64_entry(tflops, flops, kern_ms):
+    return {"achieved_tflops": tflops, "peak_tflops": FP64_PEAK_TFLOPS, "frac": tflops / FP64_PEAK_TFLOPS,
+            "basis": "source count (flops_per_qp)", **fp64_counter_fields(flops, kern_ms)}
 
 
 def cpu_baseline(n_cells):
@@ -286,20 +311,33 @@ def run_cp(args, dev):
           f"{h.n_colors}; set-up {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
     f = torch.empty(prob.ndof, dtype=torch.float64, device=dev)
     vals = torch.empty(h.nnz, dtype=torch.float64, device=dev)
-    n_hist = 5
     log = None
     if not args.no_surrogate:
+        # training data: the plain configuration-3 run logged over its load history
+        # (args.sur_steps steps of 1e-3, 20 by default: 20M QP states), on a handle of its own
         import surrogate as S
 
-        log = (torch.empty((n_hist * h.n_qp, S.N_FEAT), dtype=torch.float32, device=dev),
-               torch.empty((n_hist * h.n_qp, S.N_OUT), dtype=torch.float32, device=dev))
-    row = 0
-    for s in range(1, n_hist + 1):  # load history to 0.5%, logged for the learned start
+        n_log = args.sur_steps
+        log = (torch.empty((n_log * h.n_qp, S.N_FEAT), dtype=torch.float32, device=dev),
+               torch.empty((n_log * h.n_qp, S.N_OUT), dtype=torch.float32, device=dev))
+        hl = P.Handle(prob.mesh, prob.ids, prob.ndof, prob.Ap, prob.Ai, prob.mats)
+        hl.set_orientations(prob.R, prob.grain)
+        row = 0
+        t0 = time.perf_counter()
+        for s in range(1, n_log + 1):
+            r, defl = prob.displacement(CP_STEP * s, 0.0, seed=s)
+            err = hl.residual(f, torch.from_numpy(r).to(dev), defl=torch.from_numpy(defl).to(dev), dt=1.0)
+            assert err == 0, (s, err)
+            row = S.log_step(hl, log[0], log[1], row)
+            hl.commit_step()
+        hl.close()
+        print(f"[bench] CP: logged {row} QP states over {n_log} plain steps in {time.perf_counter() - t0:.1f}s",
+              file=sys.stderr, flush=True)
+    n_hist = 5
+    for s in range(1, n_hist + 1):  # load history to 0.5%
         r, defl = prob.displacement(CP_STEP * s, 0.0, seed=s)
         err = h.residual(f, torch.from_numpy(r).to(dev), defl=torch.from_numpy(defl).to(dev), dt=1.0)
         assert err == 0, (s, err)
-        if log is not None:
-            row = S.log_step(h, log[0], log[1], row)
         h.commit_step()
     r, defl = prob.displacement(CP_STEP * 6, 0.0, seed=6)
     rd = torch.from_numpy(r).to(dev)
@@ -315,10 +353,11 @@ def run_cp(args, dev):
           f"local Newton iterations mean {mean_it:.2f} max {int(its.max())}, {tflops:.2f} TFLOP/s",
           file=sys.stderr, flush=True)
     if log is not None:
-        # learned warm start: trained on the logged plain history by the library's kernels,
-        # then the same step timed with the network in the loop
+        # learned warm start: trained on the logged plain run by the library's kernels (seeded
+        # 90/10 split), then the same step timed with the network in the loop
         tau_plain = h.get_state(P.STATE_TAU)
         pF_plain = h.get_state(P.STATE_PLASTIC_NP1)
+        n_samples = int(log[0].shape[0])
         t0 = time.perf_counter()
         mlp, nrm, hist = S.train(log[0], log[1], epochs=args.sur_epochs, batch=8192, lr=2e-3, seed=0)
         t_train = time.perf_counter() - t0
@@ -326,20 +365,41 @@ def run_cp(args, dev):
         S.attach(h, mlp, nrm)
         s_sur, kern_sur, _ = _timed_passes(h, f, vals, rd, dd, args.cp_steps, 1)
         its_s = h.get_state(P.STATE_ITERS)[0]
+        n_learned, n_fallback = h.cp_start_stats()
         tau_s = h.get_state(P.STATE_TAU)
         dtau = float(np.abs(tau_s - tau_plain).max() / np.abs(tau_plain).max())
         dpf = float(np.abs(h.get_state(P.STATE_PLASTIC_NP1) - pF_plain).max())
+        h.set_surrogate(None)
+        mlp.close()
+        # the bound any start can reach: the converged point itself as the start (M = pF(n)
+        # pF(n+1)^-1, lambda, g in device QP order)
+        order = h.qp_order()
+        pFn = h.get_state(P.STATE_PLASTIC_N).T.reshape(-1, 3, 3)
+        pF1 = pF_plain.T.reshape(-1, 3, 3)
+        sc = h.get_state(P.STATE_SCALAR_NP1)
+        M = pFn @ np.linalg.inv(pF1)
+        warm_mesh = np.concatenate([M.reshape(-1, 9).T, sc[1][None], sc[0][None]], axis=0)
+        warm = torch.from_numpy(np.ascontiguousarray(warm_mesh[:, order])).to(dev)
+        h.set_warm_start(warm)
+        s_ub, kern_ub, _ = _timed_passes(h, f, vals, rd, dd, args.cp_steps, 1)
+        its_ub = h.get_state(P.STATE_ITERS)[0]
+        h.set_warm_start(None)
         res["surrogate"] = dict(
             value=h.n_qp / s_sur, ms_per_step=1e3 * s_sur, kernel_ms_per_pass=kern_sur, speedup=s_step / s_sur,
             mean_local_iters=float(its_s.mean()), iterations_saved=mean_it - float(its_s.mean()),
+            qps_from_learned_start=n_learned, qps_fallen_back=n_fallback,
             max_rel_tau_diff_vs_plain=dtau, max_abs_pF_diff_vs_plain=dpf, train_s=t_train,
-            train_samples=int(n_hist * h.n_qp), epochs=args.sur_epochs, val_loss=[float(x[2]) for x in hist])
+            train_samples=n_samples, logged_steps=args.sur_steps, split="seeded 90/10", epochs=args.sur_epochs,
+            val_loss=[float(x[2]) for x in hist],
+            upper_bound=dict(start="the converged point", ms_per_step=1e3 * s_ub, kernel_ms_per_pass=kern_ub,
+                             mean_local_iters=float(its_ub.mean())),
+            upper_bound_speedup=s_step / s_ub)
         print(f"[bench] CP learned start: {1e3 * s_sur:.3f} ms/step, speedup {s_step / s_sur:.3f}x, iterations "
-              f"mean {its_s.mean():.2f} (plain {mean_it:.2f}), max rel tau diff {dtau:.1e}, trained on "
-              f"{n_hist * h.n_qp} states in {t_train:.1f}s, val loss {hist[0][2]:.3e}->{hist[-1][2]:.3e}",
+              f"mean {its_s.mean():.2f} (plain {mean_it:.2f}), {n_learned} QPs from the learned start, "
+              f"{n_fallback} fell back, max rel tau diff {dtau:.1e}, trained on {n_samples} states in "
+              f"{t_train:.1f}s, val loss {hist[0][2]:.3e}->{hist[-1][2]:.3e}; converged-point start "
+              f"{1e3 * s_ub:.3f} ms/step ({its_ub.mean():.2f} iterations, bound {s_step / s_ub:.3f}x)",
               file=sys.stderr, flush=True)
-        h.set_surrogate(None)
-        mlp.close()
     h.close()
     return res
 
@@ -410,6 +470,8 @@ def main():
     ap.add_argument("--mix", type=int, nargs=3, default=[41, 32, 46],
                     help="cells per edge of the linear-tet, quadratic-tet and hex boxes of the mixed batch")
     ap.add_argument("--sur-epochs", type=int, default=3)
+    ap.add_argument("--sur-steps", type=int, default=20,
+                    help="plain load steps logged as training data for the learned start (20: 20M QP states)")
     ap.add_argument("--assembly", choices=["gather", "coloured"], default="gather",
                     help="assembly mode of the timed passes (the other mode is timed for comparison)")
     args = ap.parse_args()
@@ -464,8 +526,7 @@ def main():
             "frac": j2["achieved"] / HBM_PEAK_GBS,
             "traffic": measured_traffic(f"j2_tet10_{args.n}cube"),
         },
-        "fp64": {"achieved_tflops": j2["tflops"], "peak_tflops": FP64_PEAK_TFLOPS,
-                 "frac": j2["tflops"] / FP64_PEAK_TFLOPS},
+        "fp64": fp64_entry(j2["tflops"], measured_flops(f"j2_tet10_{args.n}cube"), j2["kern_ms"]),
         "kernel_ms_per_pass": j2["kern_ms"],
         "launches_per_pass": j2["n_launch"],
         "assembly": args.assembly,
@@ -483,7 +544,8 @@ def main():
             "mean_local_iters": cp["mean_iters"],
             "max_local_iters": cp["max_iters"],
             "roofline": {"bound": "fp64", "achieved": cp["tflops"], "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                         "frac": cp["tflops"] / FP64_PEAK_TFLOPS},
+                         "frac": cp["tflops"] / FP64_PEAK_TFLOPS,
+                         **fp64_counter_fields(measured_flops(f"cp_tet4_{args.cp_n}cube"), cp["kern_ms"])},
             "surrogate_speedup": None if cp["surrogate"] is None else cp["surrogate"]["speedup"],
             "surrogate": cp["surrogate"],
             "cpu_baseline": cpu_cp,

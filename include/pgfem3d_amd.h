@@ -403,13 +403,17 @@ int pgf_mlp_set_params(pgf_mlp m, const float *host);
 int pgf_mlp_get_grads(pgf_mlp m, float *host);
 /* Device pointers: X [B][widths[0]], Y [B][widths[n_layers]] (FP32). */
 int pgf_mlp_forward(pgf_mlp m, const float *X, int64_t B, float *Y);
+/* The same forward pass as one launch with the activations in registers and one layer's
+ * weights at a time in LDS, no pre-activations kept: the kernel the learned warm start runs.
+ * Device pointers as pgf_mlp_forward; at most 8 layers. */
+int pgf_mlp_infer(pgf_mlp m, const float *X, int64_t B, float *Y);
 /* Mean-squared-error forward + backward over a batch (B a multiple of 16); gradients are
  * added to the gradient buffer, which pgf_mlp_adam_step consumes and clears. */
 int pgf_mlp_backward(pgf_mlp m, const float *X, const float *T, int64_t B, float *loss);
 int pgf_mlp_adam_step(pgf_mlp m, float lr, float beta1, float beta2, float eps);
 
 /* Crystal-plasticity features and targets of the last integration, device QP order, FP32:
- * X [n_qp][34] = (F(n) pF(n)^-1 (9), F(n+1) - F(n) (9), 12 x g(n)/g0, grain orientation
+ * X [n_qp][34] = (F(n) pF(n)^-1 (9), F(n+1) - F(n) (9), the 12 tau_a(n)/g(n), grain orientation
  * quaternion (w, x, y, z) with w >= 0), T [n_qp][12] = dt * gamma_dot(n+1). */
 int pgf_cp_log_samples(pgf_handle h, float *X, float *T);
 /* Attach a trained MLP (NULL detaches): integrating passes then build the features, run the

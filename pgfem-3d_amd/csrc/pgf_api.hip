@@ -285,6 +285,7 @@ pgf::CPArgs cp_args(pgf_handle h, const Block &b, const pgf_nodal_input *in) {
   c.Sc_np1 = h->d_Sc[0];
   c.F_n = h->d_F[1];
   c.tau = h->d_tau[0];
+  c.tau_n = h->d_tau[1];
   c.gdot = h->d_gdot[0];
   c.iters = h->d_iters;
   c.warm = h->warm;

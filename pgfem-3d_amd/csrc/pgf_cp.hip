@@ -1447,7 +1447,7 @@ __global__ __launch_bounds__(256) void output_kernel(OutArgs o) {
 }
 
 // ------------------------------------------------------------------- learned warm start
-// Features of one QP (device QP order rows of 34): F(n) pF(n)^-1, F(n+1) - F(n), 12 x g(n)/g0,
+// Features of one QP (device QP order rows of 34): F(n) pF(n)^-1, F(n+1) - F(n), tau_a(n)/g(n),
 // grain quaternion; with shift/scale the row is normalised as (x - shift) * scale.
 __global__ void cp_features(CPArgs a, const float *gquat, const float *shift, const float *scale, float *X) {
   const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -1470,9 +1470,10 @@ __global__ void cp_features(CPArgs a, const float *gquat, const float *shift, co
     f[i] = float(Fe[i]);
     f[9 + i] = float(a.F_np1[i * NQ + qg] - Fn[i]);
   }
-  const float gr = float(a.g_n[qg] / a.cpmats[0].g0);
+  // the resolved shear stresses at n over the slip resistance at n (the flow rule's ratio)
+  const double rg = 1.0 / a.g_n[qg];
 #pragma unroll
-  for (int i = 0; i < 12; ++i) f[18 + i] = gr;
+  for (int i = 0; i < 12; ++i) f[18 + i] = float(a.tau_n[i * NQ + qg] * rg);
   const int gid = a.qgrain[t];
 #pragma unroll
   for (int i = 0; i < 4; ++i) f[30 + i] = gquat[gid * 4 + i];

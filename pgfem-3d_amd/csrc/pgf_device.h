@@ -135,6 +135,7 @@ struct CPArgs {
   double *Sc_np1;             // g, lambda at n+1
   const double *F_n;          // 9 comps (plain start: plastic predictor)
   double *tau, *gdot;         // 12 comps at n+1
+  const double *tau_n;        // 12 comps at n (learned-start features)
   int32_t *iters;             // [n_qp]
   const double *warm;         // optional start (M 9, lambda, g) [11][n_qp]
   // tangent-pass output for the element kernel: Me 9, S 6, pJ 1, D = dMe/dF 81 (block-local)

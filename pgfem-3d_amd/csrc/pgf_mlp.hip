@@ -190,6 +190,99 @@ __global__ void zero_f32(float *p, int64_t n) {
   if (i < n) p[i] = 0.f;
 }
 
+// Inference of the whole network in one launch (the learned start's forward pass).
+//
+// Each wave carries two 16-row strips of the batch through every layer with the
+// activations in registers: the layer is computed transposed, D^T = W H^T, so the MFMA
+// output fragment of a 16-feature tile (d[i] = D[4 (l / 16) + i][l % 16]: feature rows,
+// batch columns) is exactly the B fragment of the next layer's 16-wide k block
+// (b[i] = B[4 (l / 16) + i][l % 16]).  Only the weights go through LDS: the workgroup stages
+// one layer's BF16 weights and FP32 biases at a time (<= 128 x 128 x 2 B + 512 B) and every
+// wave reads its A fragments from there, so a layer's weights are read from L2 once per
+// 128 rows instead of once per 16.  No pre-activations are written (inference keeps none).
+constexpr int kInferWaves = 4, kInferStrips = 2, kMaxW = 128;
+
+struct InferArgs {
+  const float *X;      // [B][n_in] fp32 (normalised features)
+  int64_t B;
+  int n_in;
+  const uint16_t *W;   // bf16 parameters (the d_wb copy)
+  const float *P;      // fp32 parameters (biases)
+  int L;
+  int Kp[8], Np[8], n_out;
+  int64_t woff[8], boff[8];
+  float *Y;            // [B][ldy]
+  int ldy;
+};
+
+__global__ __launch_bounds__(256) void mlp_infer(InferArgs g) {
+  __shared__ uint16_t sW[kMaxW * kMaxW];
+  __shared__ float sb[kMaxW];
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int r16 = lane % 16, q4 = (lane / 16) * 4;
+  const int64_t rows_per_chunk = int64_t(kInferWaves) * kInferStrips * 16;
+  for (int64_t c0 = int64_t(blockIdx.x) * rows_per_chunk; c0 < g.B; c0 += int64_t(gridDim.x) * rows_per_chunk) {
+    // layer-0 B fragments straight from the features: b[i] = X[m][16 kb + q4 + i], m = row r16
+    bf16x4 act[kInferStrips][kMaxW / 16];
+#pragma unroll
+    for (int s = 0; s < kInferStrips; ++s) {
+      const int64_t m = c0 + (int64_t(wave) * kInferStrips + s) * 16 + r16;
+#pragma unroll
+      for (int kb = 0; kb < kMaxW / 16; ++kb) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int k = kb * 16 + q4 + i;
+          const float v = (m < g.B && k < g.n_in) ? g.X[m * g.n_in + k] : 0.0f;
+          act[s][kb][i] = short(f2bf(v));
+        }
+      }
+    }
+    for (int l = 0; l < g.L; ++l) {
+      const int Kp = g.Kp[l], Np = g.Np[l];
+      __syncthreads();  // the previous layer's weights are no longer read
+      {
+        const uint64_t *src = reinterpret_cast<const uint64_t *>(g.W + g.woff[l]);
+        uint64_t *dst = reinterpret_cast<uint64_t *>(sW);
+        for (int t = threadIdx.x; t < Np * Kp / 4; t += blockDim.x) dst[t] = src[t];
+        for (int t = threadIdx.x; t < Np; t += blockDim.x) sb[t] = g.P[g.boff[l] + t];
+      }
+      __syncthreads();
+      const bool last = l + 1 == g.L;
+#pragma unroll
+      for (int s = 0; s < kInferStrips; ++s) {
+        bf16x4 nxt[kMaxW / 16];
+        const int64_t m = c0 + (int64_t(wave) * kInferStrips + s) * 16 + r16;
+#pragma unroll
+        for (int j = 0; j < kMaxW / 16; ++j) {
+          if (j * 16 >= Np) break;
+          f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+          for (int kb = 0; kb < kMaxW / 16; ++kb) {
+            if (kb * 16 >= Kp) break;
+            // a[i] = W[16 j + r16][16 kb + q4 + i]: four consecutive bf16 of one weight row
+            const bf16x4 a = *reinterpret_cast<const bf16x4 *>(sW + (j * 16 + r16) * Kp + kb * 16 + q4);
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, act[s][kb], acc, 0, 0, 0);
+          }
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int n = j * 16 + q4 + i;
+            const float z = acc[i] + sb[n];
+            if (last) {
+              if (m < g.B && n < g.n_out) g.Y[m * g.ldy + n] = z;
+            } else {
+              nxt[j][i] = short(f2bf(gelu(z)));
+            }
+          }
+        }
+        if (!last) {
+#pragma unroll
+          for (int j = 0; j < kMaxW / 16; ++j) act[s][j] = nxt[j];
+        }
+      }
+    }
+  }
+}
+
 inline unsigned blocks_of(int64_t n, int t) { return unsigned((n + t - 1) / t); }
 
 }  // namespace
@@ -233,6 +326,31 @@ int ensure_cap(pgf_mlp m, int64_t B) {
   if (hipMalloc(&m->dB, size_t(cap) * 128 * 2) != hipSuccess) return PGF_ENOMEM;
   m->cap = cap;
   return PGF_OK;
+}
+
+// Whole-network inference (mlp_infer) of B rows of fp32 features X [B][K_0] into Y [B][ldy].
+int infer(pgf_mlp m, const float *X, int64_t B, float *Y, int ldy, hipStream_t s) {
+  if (m->L > 8) return PGF_ENOTSUP;
+  InferArgs g{};
+  g.X = X;
+  g.B = B;
+  g.n_in = m->K[0];
+  g.W = m->d_wb;
+  g.P = m->d_p;
+  g.L = m->L;
+  for (int l = 0; l < m->L; ++l) {
+    g.Kp[l] = m->Kp[l];
+    g.Np[l] = m->Np[l];
+    g.woff[l] = m->woff[l];
+    g.boff[l] = m->boff[l];
+  }
+  g.n_out = m->N[m->L - 1];
+  g.Y = Y;
+  g.ldy = ldy;
+  const int64_t chunks = (B + kInferWaves * kInferStrips * 16 - 1) / (kInferWaves * kInferStrips * 16);
+  const unsigned grid = unsigned(chunks < 2048 ? chunks : 2048);
+  mlp_infer<<<grid, 64 * kInferWaves, 0, s>>>(g);
+  return hipGetLastError() == hipSuccess ? PGF_OK : PGF_EHIP;
 }
 
 void refresh_bf16(pgf_mlp m) { to_bf16<<<blocks_of(m->n_param, 256), 256, 0, m->stream>>>(m->d_p, m->d_wb, m->n_param); }
@@ -377,6 +495,13 @@ int pgf_mlp_forward(pgf_mlp m, const float *X, int64_t B, float *Y) {
   return hipStreamSynchronize(m->stream) == hipSuccess ? PGF_OK : PGF_EHIP;
 }
 
+int pgf_mlp_infer(pgf_mlp m, const float *X, int64_t B, float *Y) {
+  if (!m || !X || !Y || B < 1) return PGF_EINVAL;
+  int rc = infer(m, X, B, Y, m->N[m->L - 1], m->stream);
+  if (rc) return rc;
+  return hipStreamSynchronize(m->stream) == hipSuccess ? PGF_OK : PGF_EHIP;
+}
+
 // Forward + backward of the mean squared error over a batch; gradients are ADDED to the
 // gradient buffer (zeroed by each Adam step).  *loss = the batch loss.
 int pgf_mlp_backward(pgf_mlp m, const float *X, const float *T, int64_t B, float *loss) {
@@ -450,18 +575,15 @@ int pgf_mlp_adam_step(pgf_mlp m, float lr, float beta1, float beta2, float eps) 
 }  // extern "C"
 
 // Device-side forward for other translation units (the crystal-plasticity warm start):
-// X fp32 [B][n_in] on the MLP's own stream, outputs in the MLP's Y buffer [B][Np_last].
+// X fp32 [B][n_in] on stream s, outputs in the MLP's Y buffer [B][Np_last], by the fused
+// inference kernel.
 namespace pgf {
 int mlp_forward_device(pgf_mlp m, const float *X, int64_t B, hipStream_t s, const float **Y, int *ldy) {
   int rc = ensure_cap(m, B);
   if (rc) return rc;
-  hipStream_t keep = m->stream;
-  m->stream = s;
-  pack_input<<<blocks_of(B * m->Kp[0], 256), 256, 0, s>>>(X, B, m->K[0], m->Kp[0], m->H[0]);
-  forward_packed(m, B);
-  m->stream = keep;
+  if ((rc = infer(m, X, B, m->Y, m->Np[m->L - 1], s))) return rc;
   *Y = m->Y;
   *ldy = m->Np[m->L - 1];
-  return hipGetLastError() == hipSuccess ? PGF_OK : PGF_EHIP;
+  return PGF_OK;
 }
 }  // namespace pgf

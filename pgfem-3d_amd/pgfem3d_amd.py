@@ -120,6 +120,7 @@ def lib() -> C.CDLL:
             "pgf_mlp_set_params": [P, P],
             "pgf_mlp_get_grads": [P, P],
             "pgf_mlp_forward": [P, P, I64, P],
+            "pgf_mlp_infer": [P, P, I64, P],
             "pgf_mlp_backward": [P, P, P, I64, P],
             "pgf_mlp_adam_step": [P, C.c_float, C.c_float, C.c_float, C.c_float],
             "pgf_cp_log_samples": [P, P, P],
@@ -155,7 +156,7 @@ def exported_symbols():
         "pgf_mlp_get_params", "pgf_mlp_set_params", "pgf_mlp_get_grads", "pgf_mlp_forward", "pgf_mlp_backward",
         "pgf_mlp_adam_step", "pgf_cp_log_samples", "pgf_set_surrogate", "pgf_solve", "pgf_set_graph", "pgf_set_assembly",
         "pgf_set_three_field", "pgf_tf_initial", "pgf_tf_update", "pgf_tf_accept", "pgf_tf_get", "pgf_tf_set",
-        "pgf_autotune", "pgf_subdivision_param", "pgf_cp_start_stats", "pgf_output_variables",
+        "pgf_autotune", "pgf_subdivision_param", "pgf_cp_start_stats", "pgf_output_variables", "pgf_mlp_infer",
     ]
 
 
@@ -679,6 +680,10 @@ class MLP:
     def forward(self, X, Y):
         """X [B, in], Y [B, out]: float32 GPU tensors."""
         _check(self._lib.pgf_mlp_forward(self.m, _ptr(X), X.shape[0], _ptr(Y)), "pgf_mlp_forward")
+
+    def infer(self, X, Y):
+        """The fused inference pass (pgf_mlp_infer): X [B, in], Y [B, out] float32 GPU tensors."""
+        _check(self._lib.pgf_mlp_infer(self.m, _ptr(X), X.shape[0], _ptr(Y)), "pgf_mlp_infer")
 
     def backward(self, X, T):
         """Adds the gradient of the batch MSE to the gradient buffer; returns the loss."""

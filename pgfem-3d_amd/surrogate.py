@@ -1,8 +1,10 @@
 """Learned warm start of the crystal-plasticity local Newton: data logging, normalisation and
 training with the library's own MLP kernels (pgf_mlp_*), and attachment to a handle.
 
-The network maps the 34 features of a QP -- F(n) pF(n)^-1, F(n+1) - F(n), the 12 slip
-resistances g(n)/g0 and the grain's orientation quaternion -- to its 12 slip increments
+The network maps the 34 features of a QP -- F(n) pF(n)^-1, F(n+1) - F(n), the 12 resolved
+shear stresses over the slip resistance tau_a(n)/g(n) (the model has one slip resistance per
+QP, whose ratio to each system's stress is what the flow rule reads) and the grain's
+orientation quaternion -- to its 12 slip increments
 dt * gamma_dot(n+1).  The library turns the prediction into a start (M, lambda, g) of the FP64
 local Newton, which still converges to tol_M and falls back to the plain start when the
 predicted start does not converge.  torch is used only to hold the data on the GPU and to

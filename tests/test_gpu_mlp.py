@@ -89,3 +89,41 @@ def test_training_reduces_loss(device):
             net.adam_step(1e-3)
     assert loss < 0.05 * first
     net.close()
+
+
+def test_fused_inference_layout_exact(device):
+    """The fused inference kernel (activations in registers, the transposed product) on the
+    permutation-and-scale layer: exact, as the layered forward is; and on a two-layer net with
+    an odd batch size its outputs equal the layered forward's."""
+    widths = (40, 30)
+    net = P.MLP(widths, seed=0)
+    K, N = pad16(40), pad16(30)
+    W = np.zeros((N, K), dtype=np.float32)
+    perm = np.random.default_rng(0).permutation(40)[:30]
+    for n in range(30):
+        W[n, perm[n]] = 2.0 ** (n % 5 - 2)
+    b = np.zeros(N, dtype=np.float32)
+    b[:30] = np.arange(30)
+    net.set_params(np.concatenate([W.ravel(), b]))
+    X = np.random.default_rng(1).integers(-64, 64, size=(300, 40)).astype(np.float32)
+    Y = torch.empty((300, 30), dtype=torch.float32, device=device)
+    net.infer(_cuda(X, device), Y)
+    expect = X[:, perm] * (2.0 ** (np.arange(30) % 5 - 2)) + np.arange(30)
+    np.testing.assert_array_equal(Y.cpu().numpy(), expect.astype(np.float32))
+    net.close()
+
+
+def test_fused_inference_matches_layered_forward(device):
+    rng = np.random.default_rng(5)
+    net = P.MLP(WIDTHS, seed=7)
+    ref = MLPRef(WIDTHS, net.get_params())
+    for B in (1, 37, 1000):
+        X = rng.standard_normal((B, 34)).astype(np.float32)
+        Y1 = torch.empty((B, 12), dtype=torch.float32, device=device)
+        Y2 = torch.empty((B, 12), dtype=torch.float32, device=device)
+        net.forward(_cuda(X, device), Y1)
+        net.infer(_cuda(X, device), Y2)
+        assert _rel(Y2.cpu().numpy(), ref.forward(X)) < 5e-3
+        # same products in the same k order: equal up to the MFMA's own summation
+        assert _rel(Y2.cpu().numpy(), Y1.cpu().numpy().astype(np.float64)) < 1e-6, B
+    net.close()
