@@ -42,6 +42,7 @@ extern "C" {
 #define PGF_EHIP (-3)     /* a HIP runtime call failed */
 #define PGF_ENOTSUP (-4)  /* element type / model combination not supported */
 #define PGF_ENODEV (-5)   /* no usable gfx950 device */
+#define PGF_ESCRATCH (-6) /* the call needs more than the library's fixed device buffer */
 
 /* ------------------------------------------------------------- element types */
 /* Identified by nodes per element, as FEMLIB does (FEMLIB/src/femlib.cc:33-37,75-83). */
@@ -264,6 +265,25 @@ int pgf_cp_start_stats(pgf_handle h, int64_t *n_learned, int64_t *n_fallback);
  * rejects a converged step whose value exceeds 1.25 and subdivides it, as Newton_Raphson does
  * (src/Newton_Raphson.cc:1364-1398). */
 int pgf_subdivision_param(pgf_handle h, double dt, double *alpha);
+
+/* pgf_residual_tangent without allocating: the gather assembly's staging blocks go to a
+ * fixed device array of PGF_STATIC_STAGE_DOUBLES doubles (64 MiB) that the library holds on
+ * each GPU it runs on until the process ends, instead of the buffer pgf_residual_tangent
+ * allocates at its first gather pass.  The call makes no hipMalloc, hipFree or other
+ * allocating call; it launches eagerly (even with pgf_set_graph on) and synchronises the
+ * handle's stream to read the failure count, like pgf_residual_tangent.
+ * No counterpart in the reference: it is for callers that cannot allocate device memory
+ * (serving loops, harnesses that forbid allocation).
+ * Returns PGF_ESCRATCH when the mesh's staging blocks exceed the array (pgf_stage_doubles
+ * gives their size), PGF_ENOTSUP when the handle has no gather tables or has a learned start
+ * attached.  Every call on one GPU shares the array: calls that overlap in time on one GPU
+ * (two host threads, or handles on streams that run concurrently) race. */
+#define PGF_STATIC_STAGE_DOUBLES 8388608
+int pgf_residual_tangent_static(pgf_handle h, const pgf_nodal_input *in, double dt, double *f,
+                                double *vals);
+
+/* Size of the gather assembly's staging buffer of the handle's mesh, in doubles. */
+int pgf_stage_doubles(pgf_handle h, int64_t *n);
 
 /* Capture each pass (all of its launches: every element type, the crystal-plasticity QP
  * kernels, the interface elements) into one hipGraph and replay it while the pass mode, the

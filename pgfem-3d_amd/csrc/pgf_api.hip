@@ -7,6 +7,7 @@
 #include <cstdlib>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <vector>
 
@@ -139,6 +140,37 @@ void make_grain(const double *R, double C11, double C12, double C44, DevGrain &g
 }
 
 }  // namespace pgf
+
+// Staging blocks of pgf_residual_tangent_static: one array per GPU, allocated with the code
+// object, never by a call.  Host code reaches it only through the address
+// hipGetSymbolAddress gives on each device (static_stage_ptr); the array's name in host
+// code is not a device address.
+static __device__ __align__(256) double scratch[PGF_STATIC_STAGE_DOUBLES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on `device` (the current device of the calling handle),
+// looked up once per device.
+int static_stage_ptr(int device, double **ptr) {
+  if (device < 0 || device >= kMaxDevices) return PGF_ENODEV;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    int cur = 0;
+    if (hipGetDevice(&cur) != hipSuccess) return PGF_EHIP;
+    if (cur != device && hipSetDevice(device) != hipSuccess) return PGF_EHIP;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != device) hipSetDevice(cur);
+    if (e != hipSuccess || !p) return PGF_EHIP;
+    g_scratch_ptr[device] = static_cast<double *>(p);
+  }
+  *ptr = g_scratch_ptr[device];
+  return PGF_OK;
+}
+}  // namespace
 
 // What a captured pass depends on: a new key re-captures.
 struct GraphKey {
@@ -1162,6 +1194,35 @@ int pgf_residual_tangent(pgf_handle h, const pgf_nodal_input *in, double dt, dou
   return run_pass(h, in, pgf::MODE_INTEGRATE | pgf::MODE_RESIDUAL | pgf::MODE_TANGENT, f, vals);
 }
 
+int pgf_residual_tangent_static(pgf_handle h, const pgf_nodal_input *in, double dt, double *f, double *vals) {
+  if (!h || !f || !vals || !in || !in->r) return PGF_EINVAL;
+  if (h->n_coel && !h->d_cprops) return PGF_EINVAL;  // interface properties not set
+  // the gather tables exist and fit; the learned start sizes its MLP activations on first use
+  if (pgf::gather_waves(h->maxlen) < 1 || !h->gather_fits || h->sur) return PGF_ENOTSUP;
+  if (h->stage_size > PGF_STATIC_STAGE_DOUBLES) return PGF_ESCRATCH;
+  double *stage = nullptr;
+  int rc = static_stage_ptr(h->device, &stage);
+  if (rc) return rc;
+  if (dt > 0.0) h->dt = dt;
+  // eager launches only: instantiating a graph may allocate.  Every staging block the pass
+  // reads is written earlier in the same pass, so what an earlier call left is never read.
+  if (h->profiling) hipEventRecord(h->ev0, h->stream);
+  h->last_launches = enqueue_pass(h, in, pgf::MODE_INTEGRATE | pgf::MODE_RESIDUAL | pgf::MODE_TANGENT, f, vals,
+                                  h->stage_size ? stage : nullptr);
+  if (h->profiling) hipEventRecord(h->ev1, h->stream);
+  if (hipGetLastError() != hipSuccess) return PGF_EHIP;
+  int32_t err = 0;
+  if (hipMemcpyAsync(&err, h->d_err, sizeof(int32_t), hipMemcpyDeviceToHost, h->stream) != hipSuccess)
+    return PGF_EHIP;
+  if (hipStreamSynchronize(h->stream) != hipSuccess) return PGF_EHIP;
+  if (h->profiling) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, h->ev0, h->ev1) != hipSuccess) return PGF_EHIP;
+    h->last_ms = ms;
+  }
+  return err;
+}
+
 int pgf_output_variables(pgf_handle h, int32_t level, double *S, double *E, double *sigma, double *eps) {
   if (!h || level < 0 || level > 2) return PGF_EINVAL;
   const int64_t NQ = h->n_qp, NE = h->n_elem;
@@ -1259,6 +1320,12 @@ int pgf_subdivision_param(pgf_handle h, double dt, double *alpha) {
   if (hipMemcpyAsync(alpha, h->d_red, sizeof(double), hipMemcpyDeviceToHost, h->stream) != hipSuccess)
     return PGF_EHIP;
   return hipStreamSynchronize(h->stream) == hipSuccess ? PGF_OK : PGF_EHIP;
+}
+
+int pgf_stage_doubles(pgf_handle h, int64_t *n) {
+  if (!h || !n) return PGF_EINVAL;
+  *n = h->stage_size;
+  return PGF_OK;
 }
 
 int pgf_commit_step(pgf_handle h) {

@@ -32,7 +32,9 @@ STATE_TAU, STATE_TAU_N, STATE_GDOT, STATE_GDOT_N, STATE_ITERS = 9, 10, 11, 12, 1
 STATE_COHESIVE_NP1, STATE_COHESIVE_N = 14, 15
 TF_V_NP1, TF_V_N, TF_V_NM1, TF_P_NP1, TF_P_N, TF_P_NM1, TF_DV, TF_DDV, TF_DP, TF_DDP = range(10)
 
-ERRORS = {-1: "PGF_EINVAL", -2: "PGF_ENOMEM", -3: "PGF_EHIP", -4: "PGF_ENOTSUP", -5: "PGF_ENODEV"}
+ERRORS = {-1: "PGF_EINVAL", -2: "PGF_ENOMEM", -3: "PGF_EHIP", -4: "PGF_ENOTSUP", -5: "PGF_ENODEV",
+          -6: "PGF_ESCRATCH"}
+STATIC_STAGE_DOUBLES = 8388608  # PGF_STATIC_STAGE_DOUBLES
 
 
 class PGFError(RuntimeError):
@@ -98,6 +100,8 @@ def lib() -> C.CDLL:
             "pgf_tangent": [P, P, D, P],
             "pgf_prescribed_load": [P, P, D, P],
             "pgf_residual_tangent": [P, P, D, P, P],
+            "pgf_residual_tangent_static": [P, P, D, P, P],
+            "pgf_stage_doubles": [P, P],
             "pgf_subdivision_param": [P, D, P],
             "pgf_cp_start_stats": [P, P, P],
             "pgf_output_variables": [P, I32, P, P, P, P],
@@ -156,7 +160,8 @@ def exported_symbols():
         "pgf_mlp_get_params", "pgf_mlp_set_params", "pgf_mlp_get_grads", "pgf_mlp_forward", "pgf_mlp_backward",
         "pgf_mlp_adam_step", "pgf_cp_log_samples", "pgf_set_surrogate", "pgf_solve", "pgf_set_graph", "pgf_set_assembly",
         "pgf_set_three_field", "pgf_tf_initial", "pgf_tf_update", "pgf_tf_accept", "pgf_tf_get", "pgf_tf_set",
-        "pgf_autotune", "pgf_subdivision_param", "pgf_cp_start_stats", "pgf_output_variables", "pgf_mlp_infer",
+        "pgf_autotune", "pgf_residual_tangent_static", "pgf_stage_doubles",
+        "pgf_subdivision_param", "pgf_cp_start_stats", "pgf_output_variables", "pgf_mlp_infer",
     ]
 
 
@@ -389,6 +394,15 @@ class Handle:
         return _check(self._lib.pgf_residual_tangent(self.h, C.byref(inp), dt, _ptr(f), _ptr(vals)),
                       "pgf_residual_tangent")
 
+    def residual_tangent_static(self, f, vals, r, d_r=None, defl=None, defl_d=None, dt=1.0) -> int:
+        """residual_tangent with the staging blocks in the library's fixed device array
+        (no device allocation); PGF_ESCRATCH when the mesh's staging does not fit it."""
+        inp = self._input(r, d_r, defl, defl_d)
+        self._check_vec(f, self.ndof, "f")
+        self._check_vec(vals, self.nnz, "vals")
+        return _check(self._lib.pgf_residual_tangent_static(self.h, C.byref(inp), dt, _ptr(f), _ptr(vals)),
+                      "pgf_residual_tangent_static")
+
     def output_variables(self, level=1):
         """(S, E per QP [n_qp, 6]; Cauchy stress and Almansi strain element averages
         [n_elem, 6]) at the time level (0 = n+1, 1 = n, 2 = n-1), components XX YY ZZ YZ XZ XY."""
@@ -413,6 +427,11 @@ class Handle:
         a = C.c_double()
         _check(self._lib.pgf_subdivision_param(self.h, float(dt), C.byref(a)), "pgf_subdivision_param")
         return a.value
+
+    def stage_doubles(self) -> int:
+        n = C.c_int64()
+        _check(self._lib.pgf_stage_doubles(self.h, C.byref(n)), "pgf_stage_doubles")
+        return n.value
 
     def commit_step(self):
         _check(self._lib.pgf_commit_step(self.h), "pgf_commit_step")
