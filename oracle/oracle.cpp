@@ -1240,3 +1240,381 @@ int orc_tf_initial(int64_t n_elem, const double *X, const int64_t *elem_ptr, con
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------ CM3F, crystal plasticity
+// The CM3F analysis of a CP_PARAM material (ConstitutiveModelIntregrate::integrate_ss,
+// src/constitutive_model.cc:720-880, total Lagrangian, one theta and one P per element):
+//   the model integrates at Fb = (theta/J)^(1/3) Fr (run_integration_algorithm with
+//   tf_factor, :810-818, :1097-1115) and stores Fb as its F(n+1);
+//   M = pF(n+1)^-1, eF = Fr M (compute_deformation_gradient_for_ss, :412-438), eFn = I,
+//   theta_n = tJn = 1, Jn = 1/pJ (pJ = det pF(n+1)), JM = det M, theta_e = theta JM;
+//   Sd, Ld: the deviatoric PK2 and tangent of the material's hyperelastic law at eF
+//   (update_elasticity_dev, include/constitutive_model.h:413-434; Mooney-Rivlin, as the
+//   hyperelastic CM3F above), dU, ddU = kappa U', kappa U'' at theta_e.
+// The blocks are those of CM_ThreeField (src/constitutive_model_3f.cc:189-470) with M and its
+// derivatives in place (Psi = M^T Phi M, DPsi, Lambda = tr(M^-1 dM)).  Three departures, all
+// in the tangent (the residuals are the reference's):
+//   - dM/du and dM/dtheta are the implicit derivatives of the converged return map at Fb,
+//     chained through dFb/du = f (G - tr(Fr^-1 G)/3 Fr) and dFb/dtheta = Fb/(3 theta)
+//     (the reference's CP_PARAM uses its linearised plasticity_compute_dMdu and dM/dtheta = 0,
+//     src/plasticity_model.cc:578-699, include/constitutive_model.h:687-697);
+//   - Kuu and Kut also carry Psi : Ld : sym(M^T Fr^T Fr dM), the change of the elastic strain
+//     through M, which compute_Kuu/compute_Kut leave out;
+//   - Kpu = Kup^T and Kpt = Ktp as the reference sets them (out_loop, :531-555): their terms
+//     in Lambda vanish, since det pF(n+1) = 1 holds along the converged return map.
+// Checks: every block against central differences of the re-integrated residuals; below
+// yield (no slip) the element equals the hyperelastic CM3F element of the same law
+// -> test_oracle_cp_three_field.py.
+namespace {
+
+int cp_tf_element(const orc_material &mat, const double *gP, const double *gL, int nne, const double *x,
+                  const double *u, double theta, double P, double dt, bool integrate, const double *pFn_q,
+                  const double *Fn_q, const double *gn_q, const double *lamn_q, double *Fb_q, double *pF1_q,
+                  double *g1_q, double *lam1_q, double *tau_q, double *gdot_q, int32_t *it_q, bool want_K,
+                  TfElem &E) {
+  const int nd = nne * 3;
+  std::memset(&E, 0, sizeof(E));
+  const double kappa = kappa_of(mat);
+  int err = 0;
+  for (int ip = 0; ip < nint_of(nne); ++ip) {
+    double N[10], dNdX[30], w;
+    if (orc_basis(nne, x, ip, N, dNdX, &w)) { ++err; continue; }
+    double Fr[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    for (int a = 0; a < nne; ++a)
+      for (int b = 0; b < 3; ++b)
+        for (int c = 0; c < 3; ++c) Fr[b * 3 + c] += u[a * 3 + b] * dNdX[a * 3 + c];
+    const double J = det3x3(Fr);
+    double Fi[9];
+    if (!(J > 0.0) || !(theta > 0.0) || inv3x3(Fr, Fi)) { ++err; continue; }
+    const double fac = std::cbrt(theta / J);
+    double *Fb = Fb_q + ip * 9, *pF1 = pF1_q + ip * 9;
+    const double *pFn = pFn_q + ip * 9;
+    if (integrate) {
+      for (int i = 0; i < 9; ++i) Fb[i] = fac * Fr[i];
+      double S0[9], t12[12], gd12[12];
+      int its = 0;
+      const int bad = orc_cp_update(&mat, gP, gL, Fb, Fn_q + ip * 9, pFn, gn_q[ip], lamn_q[ip], dt, nullptr, 1, pF1,
+                                    g1_q + ip, lam1_q + ip, S0, t12, gd12, nullptr, &its);
+      if (it_q) it_q[ip] = its;
+      if (bad) { ++err; continue; }
+      for (int a = 0; a < 12; ++a) {
+        if (tau_q) tau_q[ip * 12 + a] = t12[a];
+        if (gdot_q) gdot_q[ip * 12 + a] = gd12[a];
+      }
+    }
+    // the stored converged state: M = pF(n+1)^-1 and dM/dFb = pF(n)^-1 dMloc/dFb
+    double Me[9], pFni[9], Mloc[9], x0[11], dMdF[81], S1[9], t1[12], gd1[12], pF2[9], g2, l2;
+    if (inv3x3(pF1, Me) || inv3x3(pFn, pFni)) { ++err; continue; }
+    const double pJ = det3x3(pF1), JM = det3x3(Me);
+    matmul(pFn, Me, Mloc);
+    for (int i = 0; i < 9; ++i) x0[i] = Mloc[i];
+    x0[9] = lam1_q[ip];
+    x0[10] = g1_q[ip];
+    if (orc_cp_update(&mat, gP, gL, Fb, nullptr, pFn, gn_q[ip], lamn_q[ip], dt, x0, 0, pF2, &g2, &l2, S1, t1, gd1,
+                      dMdF, nullptr)) { ++err; continue; }
+    double dMedF[81];  // [Me comp][Fb comp]
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j)
+        for (int c = 0; c < 9; ++c) {
+          double s = 0.0;
+          for (int m = 0; m < 3; ++m) s += pFni[i * 3 + m] * dMdF[(m * 3 + j) * 9 + c];
+          dMedF[(i * 3 + j) * 9 + c] = s;
+        }
+    // deviatoric elasticity at eF = Fr M, volumetric terms at theta_e = theta JM
+    double eF[9], eC[9], Sd[9], Ld[81], Cr[9];
+    matmul(Fr, Me, eF);
+    matmul_tn(eF, eF, eC);
+    matmul_tn(Fr, Fr, Cr);
+    dev_stress_mr(eC, mat.m10, mat.m01, Sd);
+    mat_stiff_mr(eC, mat.m10, mat.m01, Ld);
+    double dU, ddU, U;
+    vol_pot(mat.volPotFlag, theta * JM, &dU, &ddU, &U);
+    dU *= kappa;
+    ddU *= kappa;
+    // per DOF k: G_k, Phi_k = sym(G^T Fr), Psi_k = M^T Phi M, tr_k = tr(G Fr^-1), dM_k,
+    // Lambda_k, X_k = Psi_k + sym(M^T Cr dM_k) (the change of the elastic strain), LX = Ld : X
+    double G[30][9], Phi[30][9], Psi[30][9], trk[30], dM[30][9], Lam[30], LX[30][9];
+    auto sym_MtCdM = [&](const double *dMx, double *out) {  // sym(M^T Cr dM)
+      double T[9], V[9];
+      matmul_tn(Me, Cr, T);
+      matmul(T, dMx, V);
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) out[i * 3 + j] = 0.5 * (V[i * 3 + j] + V[j * 3 + i]);
+    };
+    auto L_on = [&](const double *X, double *out) {
+      for (int i = 0; i < 9; ++i) {
+        double s = 0.0;
+        for (int k = 0; k < 9; ++k) s += Ld[i * 9 + k] * X[k];
+        out[i] = s;
+      }
+    };
+    auto tr_MidM = [&](const double *dMx) {  // tr(M^-1 dM) = tr(pF(n+1) dM)
+      double s = 0.0;
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) s += pF1[i * 3 + j] * dMx[j * 3 + i];
+      return s;
+    };
+    for (int k = 0; k < nd; ++k) {
+      tf_dof_grad(dNdX, k, G[k]);
+      double A[9], T[9];
+      matmul_tn(G[k], Fr, A);
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) Phi[k][i * 3 + j] = 0.5 * (A[i * 3 + j] + A[j * 3 + i]);
+      matmul_tn(Me, Phi[k], T);
+      matmul(T, Me, Psi[k]);
+      double tr = 0.0;
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) tr += G[k][j * 3 + i] * Fi[i * 3 + j];
+      trk[k] = tr;
+      double dFb[9];
+      for (int i = 0; i < 9; ++i) dFb[i] = fac * (G[k][i] - tr / 3.0 * Fr[i]);
+      for (int r = 0; r < 9; ++r) {
+        double s = 0.0;
+        for (int c = 0; c < 9; ++c) s += dMedF[r * 9 + c] * dFb[c];
+        dM[k][r] = s;
+      }
+      Lam[k] = tr_MidM(dM[k]);
+      double X[9], Y[9];
+      sym_MtCdM(dM[k], Y);
+      for (int i = 0; i < 9; ++i) X[i] = Psi[k][i] + Y[i];
+      L_on(X, LX[k]);
+    }
+    // theta: dFb/dtheta = Fb / (3 theta) at the current u
+    double dMt[9], Xt[9], LXt[9];
+    for (int r = 0; r < 9; ++r) {
+      double s = 0.0;
+      for (int c = 0; c < 9; ++c) s += dMedF[r * 9 + c] * fac * Fr[c] / (3.0 * theta);
+      dMt[r] = s;
+    }
+    const double Lamt = tr_MidM(dMt);
+    sym_MtCdM(dMt, Xt);
+    L_on(Xt, LXt);
+    const double cw = pJ * w;  // detJxW / Jn
+    const double vt = dU * JM - P;
+    for (int k = 0; k < nd; ++k) {
+      double PsiS = 0.0, PsiLXt = 0.0, DtS = 0.0;
+      for (int i = 0; i < 9; ++i) {
+        PsiS += Psi[k][i] * Sd[i];
+        PsiLXt += Psi[k][i] * LXt[i];
+      }
+      {  // 2 sym(M^T Phi_k dM_t) : Sd
+        double T[9], V[9];
+        matmul_tn(Me, Phi[k], T);
+        matmul(T, dMt, V);
+        for (int i = 0; i < 3; ++i)
+          for (int j = 0; j < 3; ++j) DtS += (V[i * 3 + j] + V[j * 3 + i]) * Sd[i * 3 + j];
+      }
+      const double rk = PsiS + P * J * trk[k];
+      E.Ru[k] += cw * rk;                                          // compute_Ru
+      E.Kup[k] += cw * J * trk[k];                                 // compute_Kup
+      E.Kut[k] += cw * (DtS + PsiLXt - Lamt * rk);                 // compute_Kut (+ Ld term)
+      E.Ktu[k] += cw * (JM * (ddU * theta * JM + dU) * Lam[k] - Lam[k] * vt);  // compute_Ktu
+    }
+    E.Rp += cw * (J - theta);                                      // compute_Rp
+    E.Rt += cw * vt;                                               // compute_Rt
+    E.Ktt += cw * (JM * dU * Lamt + JM * JM * ddU - Lamt * vt);    // compute_Ktt
+    E.Ktp -= cw;                                                   // compute_Ktp
+    if (!want_K) continue;
+    double Sh[9];  // M Sd M^T: sym(G_k^T G_l) : Sh = M^T sym(G_k^T G_l) M : Sd
+    {
+      double T[9];
+      matmul(Me, Sd, T);
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+          double s = 0.0;
+          for (int m = 0; m < 3; ++m) s += T[i * 3 + m] * Me[j * 3 + m];
+          Sh[i * 3 + j] = s;
+        }
+    }
+    for (int k = 0; k < nd; ++k) {
+      double PhiMS[9];  // Phi_k M Sd: 2 sym(dM^T Phi_k M) : Sd = 2 (Phi_k M Sd) : dM
+      {
+        double T[9];
+        matmul(Phi[k], Me, T);
+        matmul(T, Sd, PhiMS);
+      }
+      double PsiS = 0.0;
+      for (int i = 0; i < 9; ++i) PsiS += Psi[k][i] * Sd[i];
+      const double rk = PsiS + P * J * trk[k];
+      for (int l = 0; l < nd; ++l) {
+        double PLX = 0.0, GGS = 0.0, dMS = 0.0, tr2 = 0.0;
+        for (int i = 0; i < 9; ++i) {
+          PLX += Psi[k][i] * LX[l][i];
+          dMS += 2.0 * PhiMS[i] * dM[l][i];
+        }
+        double GG[9];
+        matmul_tn(G[k], G[l], GG);
+        for (int i = 0; i < 3; ++i)
+          for (int j = 0; j < 3; ++j) GGS += 0.5 * (GG[i * 3 + j] + GG[j * 3 + i]) * Sh[i * 3 + j];
+        for (int i = 0; i < 3; ++i)
+          for (int j = 0; j < 3; ++j)
+            for (int kk = 0; kk < 3; ++kk)
+              for (int ll = 0; ll < 3; ++ll) tr2 += G[k][j * 3 + i] * Fi[i * 3 + kk] * G[l][kk * 3 + ll] * Fi[ll * 3 + j];
+        const double pJrJn = P * J * (trk[k] * trk[l] - tr2);
+        E.Kuu[k * nd + l] += cw * (PLX + pJrJn + GGS + dMS - Lam[l] * rk);  // compute_Kuu (+ Ld term)
+      }
+    }
+  }
+  return err;
+}
+
+}  // namespace
+
+extern "C" {
+
+// One CM3F crystal-plasticity element for tests: state at n per QP ([nint][9] pF(n), F(n);
+// [nint] g(n), lambda(n)), integration at (u, theta, P); out the n+1 state ([nint][9] Fb,
+// pF(n+1); g, lambda; [nint][12] tau, gamma_dot), the blocks (Kuu [3nne][3nne], Ru, Kup, Kut,
+// Ktu; sc = Rp, Rt, Ktt, Ktp) and the condensed K, f (any output may be null).
+int orc_cp_tf_element(const orc_material *mat, const double *gP, const double *gL, int nne, const double *x,
+                      const double *u, double theta, double P, double dt, const double *pFn, const double *Fn,
+                      const double *g_n, const double *lam_n, double *Fb, double *pF1, double *g1, double *lam1,
+                      double *tau, double *gdot, double *Kuu, double *Ru, double *Kup, double *Kut, double *Ktu,
+                      double *sc, double *Kc, double *fc) {
+  TfElem E;
+  const int err = cp_tf_element(*mat, gP, gL, nne, x, u, theta, P, dt, true, pFn, Fn, g_n, lam_n, Fb, pF1, g1, lam1,
+                                tau, gdot, nullptr, Kuu || Kc, E);
+  const int nd = 3 * nne;
+  if (Kuu) std::memcpy(Kuu, E.Kuu, sizeof(double) * nd * nd);
+  if (Ru) std::memcpy(Ru, E.Ru, sizeof(double) * nd);
+  if (Kup) std::memcpy(Kup, E.Kup, sizeof(double) * nd);
+  if (Kut) std::memcpy(Kut, E.Kut, sizeof(double) * nd);
+  if (Ktu) std::memcpy(Ktu, E.Ktu, sizeof(double) * nd);
+  if (sc) { sc[0] = E.Rp; sc[1] = E.Rt; sc[2] = E.Ktt; sc[3] = E.Ktp; }
+  if (Kc || fc) tf_condense(nd, E, Kc, fc);
+  return err;
+}
+
+// fd_residuals_MP + stiffmat_fd_MP of the CM3F analysis for crystal plasticity: theta[e],
+// P[e] per element; state arrays [comp][n_qp] in mesh QP order as orc_assemble_cp (with
+// run_integration the return map runs at Fb and F_np1 = Fb is stored, otherwise the stored
+// state is used); grain tables as orc_assemble_cp.
+int orc_assemble_cp_tf(const double *X, int64_t n_elem, const int64_t *elem_ptr, const int64_t *elem_conn,
+                       const int32_t *elem_mat, const orc_material *mats, const int32_t *elem_grain,
+                       const double *grain_P, const double *grain_L, const int64_t *id, int64_t ndof,
+                       const int32_t *Ap, const int32_t *Ai, const double *r, const double *d_r, const double *defl,
+                       const double *defl_d, double dt, const double *theta, const double *P, int run_integration,
+                       int64_t n_qp, double *F_np1, const double *F_n, const double *pF_n, double *pF_np1,
+                       const double *g_n, double *g_np1, const double *lam_n, double *lam_np1, double *tau_out,
+                       double *gdot_out, int32_t *iters_out, double *f, double *vals) {
+  if (f) std::memset(f, 0, sizeof(double) * ndof);
+  if (vals) std::memset(vals, 0, sizeof(double) * Ap[ndof]);
+  int err = 0;
+  int64_t qp0 = 0;
+  for (int64_t e = 0; e < n_elem; ++e) {
+    const int nne = int(elem_ptr[e + 1] - elem_ptr[e]), nd = 3 * nne, nint = nint_of(nne);
+    double x[30], u[30];
+    int64_t cn[30];
+    tf_gather(X, elem_conn + elem_ptr[e], nne, id, r, d_r, defl, defl_d, x, u, cn);
+    // the element's QP state, gathered from the [comp][n_qp] arrays
+    double pFn[72], Fn[72], Fb[72], pF1[72], gn[8], ln[8], g1[8], l1[8], tq[96], gq[96];
+    int32_t its[8] = {0};
+    for (int ip = 0; ip < nint; ++ip) {
+      const int64_t q = qp0 + ip;
+      for (int i = 0; i < 9; ++i) {
+        pFn[ip * 9 + i] = pF_n[i * n_qp + q];
+        Fn[ip * 9 + i] = F_n ? F_n[i * n_qp + q] : ((i % 4 == 0) ? 1.0 : 0.0);
+        Fb[ip * 9 + i] = F_np1[i * n_qp + q];
+        pF1[ip * 9 + i] = pF_np1[i * n_qp + q];
+      }
+      gn[ip] = g_n[q];
+      ln[ip] = lam_n[q];
+      g1[ip] = g_np1[q];
+      l1[ip] = lam_np1[q];
+    }
+    const orc_material &mat = mats[elem_mat ? elem_mat[e] : 0];
+    const double *gP = grain_P + int64_t(elem_grain[e]) * 108, *gL = grain_L + int64_t(elem_grain[e]) * 81;
+    TfElem E;
+    err += cp_tf_element(mat, gP, gL, nne, x, u, theta[e], P[e], dt, run_integration != 0, pFn, Fn, gn, ln, Fb, pF1,
+                         g1, l1, tq, gq, its, vals != nullptr, E);
+    if (run_integration)
+      for (int ip = 0; ip < nint; ++ip) {
+        const int64_t q = qp0 + ip;
+        for (int i = 0; i < 9; ++i) {
+          F_np1[i * n_qp + q] = Fb[ip * 9 + i];
+          pF_np1[i * n_qp + q] = pF1[ip * 9 + i];
+        }
+        g_np1[q] = g1[ip];
+        lam_np1[q] = l1[ip];
+        for (int a = 0; a < 12; ++a) {
+          if (tau_out) tau_out[a * n_qp + q] = tq[ip * 12 + a];
+          if (gdot_out) gdot_out[a * n_qp + q] = gq[ip * 12 + a];
+        }
+        if (iters_out) iters_out[q] = its[ip];
+      }
+    std::vector<double> K(size_t(nd) * nd);
+    double fe[30];
+    tf_condense(nd, E, K.data(), fe);
+    if (vals)
+      for (int a = 0; a < nd; ++a)
+        if (!std::isnormal(K[a * nd + a])) ++err;
+    for (int i = 0; i < nd; ++i) {
+      const int64_t row = cn[i] - 1;
+      if (row < 0) continue;
+      if (f) f[row] += fe[i];
+      if (vals)
+        for (int j = 0; j < nd; ++j) {
+          const int64_t col = cn[j] - 1;
+          if (col < 0) continue;
+          const int32_t *p = Ai + Ap[row], *en = Ai + Ap[row + 1];
+          while (p < en && *p != col) ++p;
+          if (p == en) return -1;
+          vals[p - Ai] += K[i * nd + j];
+        }
+    }
+    qp0 += nint;
+  }
+  return err;
+}
+
+// compute_d_theta_dP for the CM3F crystal-plasticity element at the stored state: the
+// increments of theta and P that go with du (prescribed DOFs zero).
+int orc_cp_tf_update(const double *X, int64_t n_elem, const int64_t *elem_ptr, const int64_t *elem_conn,
+                     const int32_t *elem_mat, const orc_material *mats, const int32_t *elem_grain,
+                     const double *grain_P, const double *grain_L, const int64_t *id, const double *r,
+                     const double *d_r, const double *defl, const double *defl_d, double dt, const double *theta,
+                     const double *P, int64_t n_qp, double *F_np1, const double *pF_n, double *pF_np1,
+                     const double *g_n, double *g_np1, const double *lam_n, double *lam_np1, const double *du,
+                     double *ddV, double *ddP) {
+  int err = 0;
+  int64_t qp0 = 0;
+  for (int64_t e = 0; e < n_elem; ++e) {
+    const int nne = int(elem_ptr[e + 1] - elem_ptr[e]), nd = 3 * nne, nint = nint_of(nne);
+    double x[30], u[30];
+    int64_t cn[30];
+    tf_gather(X, elem_conn + elem_ptr[e], nne, id, r, d_r, defl, defl_d, x, u, cn);
+    double pFn[72], Fn[72], Fb[72], pF1[72], gn[8], ln[8], g1[8], l1[8];
+    for (int ip = 0; ip < nint; ++ip) {
+      const int64_t q = qp0 + ip;
+      for (int i = 0; i < 9; ++i) {
+        pFn[ip * 9 + i] = pF_n[i * n_qp + q];
+        Fn[ip * 9 + i] = (i % 4 == 0) ? 1.0 : 0.0;
+        Fb[ip * 9 + i] = F_np1[i * n_qp + q];
+        pF1[ip * 9 + i] = pF_np1[i * n_qp + q];
+      }
+      gn[ip] = g_n[q];
+      ln[ip] = lam_n[q];
+      g1[ip] = g_np1[q];
+      l1[ip] = lam_np1[q];
+    }
+    const orc_material &mat = mats[elem_mat ? elem_mat[e] : 0];
+    const double *gP = grain_P + int64_t(elem_grain[e]) * 108, *gL = grain_L + int64_t(elem_grain[e]) * 81;
+    TfElem E;
+    err += cp_tf_element(mat, gP, gL, nne, x, u, theta[e], P[e], dt, false, pFn, Fn, gn, ln, Fb, pF1, g1, l1, nullptr,
+                         nullptr, nullptr, false, E);
+    const double Kpt = E.Ktp;
+    double Kpu_du = 0.0, Ktu_du = 0.0;
+    for (int k = 0; k < nd; ++k) {
+      const double d = cn[k] > 0 ? du[cn[k] - 1] : 0.0;
+      Kpu_du += E.Kup[k] * d;
+      Ktu_du += E.Ktu[k] * d;
+    }
+    const double dth = -(Kpu_du + E.Rp) / Kpt;
+    ddV[e] = dth;
+    ddP[e] = -(E.Rt + Ktu_du + E.Ktt * dth) / E.Ktp;
+    qp0 += nint;
+  }
+  return err;
+}
+
+}  // extern "C"

@@ -124,3 +124,27 @@ int orc_tf_initial(int64_t n_elem, const double *X, const int64_t *elem_ptr, con
 // state (plastic: J2 Cp^-1 [6], CP pF [9], unused for the hyperelastic model; w: J2 damage)
 extern "C" int orc_output_qp(const orc_material *mat, const double *L_grain, const double *F, const double *plastic,
                              double w, double *S_out, double *sig_out);
+
+// --- CM3F for crystal plasticity (oracle.cpp, end): one element, the whole mesh, the update
+extern "C" {
+int orc_cp_tf_element(const orc_material *mat, const double *gP, const double *gL, int nne, const double *x,
+                      const double *u, double theta, double P, double dt, const double *pFn, const double *Fn,
+                      const double *g_n, const double *lam_n, double *Fb, double *pF1, double *g1, double *lam1,
+                      double *tau, double *gdot, double *Kuu, double *Ru, double *Kup, double *Kut, double *Ktu,
+                      double *sc, double *Kc, double *fc);
+int orc_assemble_cp_tf(const double *X, int64_t n_elem, const int64_t *elem_ptr, const int64_t *elem_conn,
+                       const int32_t *elem_mat, const orc_material *mats, const int32_t *elem_grain,
+                       const double *grain_P, const double *grain_L, const int64_t *id, int64_t ndof,
+                       const int32_t *Ap, const int32_t *Ai, const double *r, const double *d_r, const double *defl,
+                       const double *defl_d, double dt, const double *theta, const double *P, int run_integration,
+                       int64_t n_qp, double *F_np1, const double *F_n, const double *pF_n, double *pF_np1,
+                       const double *g_n, double *g_np1, const double *lam_n, double *lam_np1, double *tau_out,
+                       double *gdot_out, int32_t *iters_out, double *f, double *vals);
+int orc_cp_tf_update(const double *X, int64_t n_elem, const int64_t *elem_ptr, const int64_t *elem_conn,
+                     const int32_t *elem_mat, const orc_material *mats, const int32_t *elem_grain,
+                     const double *grain_P, const double *grain_L, const int64_t *id, const double *r,
+                     const double *d_r, const double *defl, const double *defl_d, double dt, const double *theta,
+                     const double *P, int64_t n_qp, double *F_np1, const double *pF_n, double *pF_np1,
+                     const double *g_n, double *g_np1, const double *lam_n, double *lam_np1, const double *du,
+                     double *ddV, double *ddP);
+}

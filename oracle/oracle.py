@@ -490,6 +490,78 @@ def tf_initial(mesh, mats, u_nodal):
     return V, Pr, err
 
 
+# ----------------------------------------------------------------------------- CM3F, crystal plasticity
+def _cptf_lib():
+    L = lib()
+    if not getattr(L, "_cptf_ready", False):
+        vp, i64, d = C.c_void_p, C.c_int64, C.c_double
+        L.orc_cp_tf_element.argtypes = [vp, vp, vp, C.c_int, vp, vp, d, d, d] + [vp] * 18
+        L.orc_assemble_cp_tf.argtypes = ([vp, i64, vp, vp, vp, vp, vp, vp, vp, vp, i64, vp, vp, vp, vp, vp, vp, d, vp,
+                                          vp, C.c_int, i64] + [vp] * 13)
+        L.orc_cp_tf_update.argtypes = [vp, i64] + [vp] * 12 + [d, vp, vp, i64] + [vp] * 10
+        for fn in ("orc_cp_tf_element", "orc_assemble_cp_tf", "orc_cp_tf_update"):
+            getattr(L, fn).restype = C.c_int
+        L._cptf_ready = True
+    return L
+
+
+def cp_tf_element(mat, gP, gL, nne, x, u, theta, P, dt, pFn, g_n, lam_n, Fn=None):
+    """One CM3F crystal-plasticity element from the state at n ([nint, 3, 3] pF(n), F(n);
+    [nint] g(n), lambda(n)), integrated at (u, theta, P): its n+1 state, blocks and condensed
+    K and f."""
+    nd = 3 * nne
+    nint = len(g_n)
+    eye = np.repeat(np.eye(3)[None], nint, axis=0)
+    st = dict(Fb=np.zeros((nint, 9)), pF1=np.zeros((nint, 9)), g1=np.zeros(nint), lam1=np.zeros(nint),
+              tau=np.zeros((nint, 12)), gdot=np.zeros((nint, 12)))
+    out = dict(Kuu=np.zeros((nd, nd)), Ru=np.zeros(nd), Kup=np.zeros(nd), Kut=np.zeros(nd), Ktu=np.zeros(nd),
+               sc=np.zeros(4), Kc=np.zeros((nd, nd)), fc=np.zeros(nd))
+    om = material(mat)
+    f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+    rc = _cptf_lib().orc_cp_tf_element(
+        C.byref(om), _p(f64(gP)), _p(f64(gL)), nne, _p(f64(x)), _p(f64(u)), float(theta), float(P), float(dt),
+        _p(f64(pFn).reshape(-1)), _p(f64(eye if Fn is None else Fn).reshape(-1)), _p(f64(g_n)), _p(f64(lam_n)),
+        *[_p(st[k]) for k in ("Fb", "pF1", "g1", "lam1", "tau", "gdot")],
+        *[_p(out[k]) for k in ("Kuu", "Ru", "Kup", "Kut", "Ktu", "sc", "Kc", "fc")])
+    out["Rp"], out["Rt"], out["Ktt"], out["Ktp"] = out.pop("sc")
+    out.update(st)
+    out["rc"] = rc
+    return out
+
+
+def assemble_cp_tf(mesh, mats, elem_grain, grain_P, grain_L, ids, ndof, Ap, Ai, st: "CPState", theta, P, r,
+                   d_r=None, defl=None, defl_d=None, dt=1.0, integrate=True, tangent=True):
+    """Condensed CM3F residual and tangent of a crystal-plasticity mesh (state in st, [comp][n_qp]
+    mesh QP order; with integrate the return map runs at (theta/J)^(1/3) F and st.F_np1 holds
+    that F). Returns (f, vals, err)."""
+    om, em = _mesh_args(mesh, mats)
+    f = np.zeros(ndof)
+    vals = np.zeros(Ap[-1]) if tangent else None
+    f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+    err = _cptf_lib().orc_assemble_cp_tf(
+        _p(mesh.X), mesh.n_elem, _p(mesh.elem_ptr), _p(mesh.elem_conn), _p(em), om,
+        _p(np.ascontiguousarray(elem_grain, dtype=np.int32)), _p(f64(grain_P)), _p(f64(grain_L)), _p(ids), ndof,
+        _p(Ap), _p(Ai), _p(r), _p(d_r), _p(defl), _p(defl_d), float(dt), _p(f64(theta)), _p(f64(P)), int(integrate),
+        st.n_qp, _p(st.F_np1), _p(st.F_n), _p(st.pF_n), _p(st.pF_np1), _p(st.g_n), _p(st.g_np1), _p(st.lam_n),
+        _p(st.lam_np1), _p(st.tau), _p(st.gdot), _p(st.iters), _p(f), _p(vals))
+    return f, vals, err
+
+
+def cp_tf_update(mesh, mats, elem_grain, grain_P, grain_L, ids, st: "CPState", theta, P, du, r, d_r=None,
+                 defl=None, defl_d=None, dt=1.0):
+    """Increments (ddV, ddP) of the CM3F crystal-plasticity elements for the displacement
+    increment du, at the stored state."""
+    om, em = _mesh_args(mesh, mats)
+    ddV, ddP = np.zeros(mesh.n_elem), np.zeros(mesh.n_elem)
+    f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+    err = _cptf_lib().orc_cp_tf_update(
+        _p(mesh.X), mesh.n_elem, _p(mesh.elem_ptr), _p(mesh.elem_conn), _p(em), om,
+        _p(np.ascontiguousarray(elem_grain, dtype=np.int32)), _p(f64(grain_P)), _p(f64(grain_L)), _p(ids), _p(r),
+        _p(d_r), _p(defl), _p(defl_d), float(dt), _p(f64(theta)), _p(f64(P)), st.n_qp, _p(st.F_np1), _p(st.pF_n),
+        _p(st.pF_np1), _p(st.g_n), _p(st.g_np1), _p(st.lam_n), _p(st.lam_np1), _p(f64(du)), _p(ddV), _p(ddP))
+    return ddV, ddP, err
+
+
 # ----------------------------------------------------------------------------- output variables
 OUT_ORDER = [(0, 0), (1, 1), (2, 2), (1, 2), (0, 2), (0, 1)]  # XX YY ZZ YZ XZ XY
 
