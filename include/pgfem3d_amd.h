@@ -326,7 +326,8 @@ int pgf_reset_step(pgf_handle h);
 
 /* ============================================================ three-field (CM3F) */
 /* The CM3F analysis (-cm3f: analysis_type CM3F, include/enumerations.h:16) for meshes whose
- * materials are all hyperelastic: one volume variable theta and one pressure P per element
+ * materials are hyperelastic (neo-Hookean deviatoric law) or crystal plasticity with the
+ * hyperelastic elastic law (cp_elastic = 1): one volume variable theta and one pressure P per element
  * (get_3f_pressure_volume_number, src/three_field_element.cc:1399-1416), condensed out of
  * the element residual and tangent (condense_K_3F_to_1F / condense_F_3F_to_1F,
  * src/condense.cc:304-385).  With the mode on, pgf_residual / pgf_tangent /
@@ -336,8 +337,13 @@ int pgf_reset_step(pgf_handle h);
  * pgf_commit_step folds the increments in and rolls the variables over
  * (update_np1_from_increments + update_for_next_time_step, Newton_Raphson.cc:683-689);
  * pgf_reset_step zeroes the increments (ThreeFieldVariables::reset).
- * on = 0 turns the mode off and frees its memory.  PGF_ENOTSUP if a material is not
- * hyperelastic. */
+ * Crystal plasticity integrates at (theta/J)^(1/3) F, which becomes the model's F(n+1)
+ * (run_integration_algorithm with tf_factor, src/constitutive_model.cc:810-818), and its
+ * element takes M = pF(n+1)^-1 and the implicit dM/du into the CM_ThreeField blocks
+ * (src/constitutive_model_3f.cc:189-470; residual passes also run the tangent kernel, since
+ * the condensed residual needs Kut).
+ * on = 0 turns the mode off and frees its memory.  PGF_ENOTSUP for J2 plasticity, the linear
+ * deviatoric law, and crystal plasticity with the cubic law. */
 int pgf_set_three_field(pgf_handle h, int32_t on);
 /* compute_cm_initial_conditions (src/constitutive_model.cc:4035-4160) for the three-field
  * analysis, with compute_3f_initial_conditions (src/three_field_element.cc:1328-1386): theta

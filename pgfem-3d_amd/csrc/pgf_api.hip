@@ -324,6 +324,10 @@ pgf::CPArgs cp_args(pgf_handle h, const Block &b, const pgf_nodal_input *in) {
   c.qpbuf = h->d_qpbuf + b.qp_base * pgf::kCPBuf;
   c.err = h->d_err;
   c.law = h->cp_law;
+  if (h->tf && b.ne > 0 && b.model == PGF_MODEL_CRYSTAL_PLASTICITY) {  // CM3F: the kinematics scales F
+    c.tf_V = h->d_tf + b.e_base;
+    c.tf_dV = h->d_tf + 6 * h->n_elem + b.e_base;
+  }
   return c;
 }
 
@@ -337,6 +341,7 @@ pgf::CPArgs cp_args_all(pgf_handle h, const pgf_nodal_input *in) {
   all.d_qgrain = h->d_qgrain_all;
   pgf::CPArgs c = cp_args(h, all, in);
   c.conn = nullptr;
+  c.tf_V = c.tf_dV = nullptr;
   return c;
 }
 
@@ -359,7 +364,8 @@ void cp_prepass(pgf_handle h, const pgf_nodal_input *in, int mode) {
     }
     pgf::launch_cp_newton(all, h->cp_chunk, h->stream);
   }
-  pgf::launch_cp_tangent(all, (mode & pgf::MODE_TANGENT) != 0, h->stream);
+  // the CM3F residual's condensation needs dM/dF too
+  pgf::launch_cp_tangent(all, (mode & pgf::MODE_TANGENT) != 0 || h->tf, h->stream);
 }
 
 pgf::ElemArgs base_args(pgf_handle h, const pgf_nodal_input *in) {
@@ -508,7 +514,17 @@ int32_t enqueue_pass(pgf_handle h, const pgf_nodal_input *in, int mode, double *
         ce.stage = gather ? stage : nullptr;
         ce.elist = gather ? nullptr : b.d_clist;
         ce.err = h->d_err;
-        pgf::launch_cp_elem(b.nne, mode, ce, e0, e1, h->stream);
+        if (h->tf) {
+          const int64_t NE = h->n_elem;
+          ce.tf_V = h->d_tf + b.e_base;
+          ce.tf_P = h->d_tf + 3 * NE + b.e_base;
+          ce.tf_dV = h->d_tf + 6 * NE + b.e_base;
+          ce.tf_dP = h->d_tf + 8 * NE + b.e_base;
+          ce.tf_buf = h->d_tfbuf + b.e_base * pgf::kTfBuf;
+          pgf::launch_cp_tf_elem(b.nne, mode, ce, e0, e1, h->stream);
+        } else {
+          pgf::launch_cp_elem(b.nne, mode, ce, e0, e1, h->stream);
+        }
       } else {
         a.conn = b.d_conn;
         a.edof = b.d_edof;
@@ -1007,7 +1023,7 @@ int pgf_create(const pgf_mesh *mesh, const int64_t *id, int64_t ndof, const int3
     std::vector<int32_t> mat(b.ne);
     for (int64_t k = 0; k < b.ne; ++k) mat[k] = m.elem_mat ? m.elem_mat[b.elems[k]] : 0;
     if ((rc = pgf::upload(&b.d_conn, bconn[bi].data(), bconn[bi].size()))) return fail(rc);
-    if (b.model != PGF_MODEL_CRYSTAL_PLASTICITY) {  // the element kernel's DOF ids, element-major
+    {  // the element DOF ids, element-major (element kernel; CM3F update and initial conditions)
       std::vector<int32_t> edof(bconn[bi].size() * 3);
       for (size_t i = 0; i < bconn[bi].size(); ++i)
         for (int c = 0; c < 3; ++c) edof[i * 3 + c] = dofid[size_t(bconn[bi][i]) * 3 + c];
@@ -1390,8 +1406,11 @@ int pgf_set_three_field(pgf_handle h, int32_t on) {
     h->tf = false;
     return PGF_OK;
   }
+  // the hyperelastic model, and crystal plasticity with its hyperelastic law (the CM3F
+  // element splits the law into deviatoric and volumetric parts; the cubic law has no split)
   for (auto &b : h->blocks)
-    if (b.model != PGF_MODEL_HYPER_ELASTICITY) return PGF_ENOTSUP;
+    if (b.model != PGF_MODEL_HYPER_ELASTICITY && !(b.model == PGF_MODEL_CRYSTAL_PLASTICITY && h->cp_law == 1))
+      return PGF_ENOTSUP;
   if (h->he_linear) return PGF_ENOTSUP;  // the three-field element takes the neo-Hookean law
   if (h->tf) return PGF_OK;
   const int64_t NE = h->n_elem;
@@ -1444,6 +1463,7 @@ int pgf_tf_initial(pgf_handle h, const pgf_nodal_input *in_n, const pgf_nodal_in
       // compute_cm_initial_conditions, src/constitutive_model.cc:4097-4105)
       pgf::CPArgs c = cp_args(h, b, in);
       c.F_np1 = h->d_F[s == 0 ? 1 : 2];
+      c.tf_V = c.tf_dV = nullptr;
       pgf::launch_cp_kinematics(b.nne, c, h->stream);
     }
   }

@@ -602,9 +602,20 @@ __global__ __launch_bounds__(256) void cp_kinematics(CPArgs a) {
       for (int ic = 0; ic < 3; ++ic) F[ib * 3 + ic] += u[n * 3 + ib] * g[ic];
   }
   const int64_t q = a.qp_base + tq;
+  const double J = det3(F);
+  bool okF = ok && J > 0.0;  // getJacobian, src/utils.cc:513-527
+  if (a.tf_V) {
+    // CM3F: the model integrates at (theta/J)^(1/3) F and keeps it as its F(n+1)
+    // (run_integration_algorithm with tf_factor, src/constitutive_model.cc:810-818, :1109-1115)
+    const double th = a.tf_V[e] + a.tf_dV[e];
+    okF = okF && th > 0.0;
+    const double fac = okF ? cbrt(th / J) : 1.0;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) F[i] *= fac;
+  }
 #pragma unroll
   for (int i = 0; i < 9; ++i) a.F_np1[i * a.n_qp + q] = F[i];
-  if (!ok || !(det3(F) > 0.0)) atomicAdd(a.err, 1);  // getJacobian, src/utils.cc:513-527
+  if (!okF) atomicAdd(a.err, 1);
 }
 
 // ------------------------------------------------------------------- Newton kernel
@@ -1283,6 +1294,436 @@ __global__ __launch_bounds__(256) void cp_elem(CPElemArgs a, int64_t e_begin, in
   }
 }
 
+// ------------------------------------------------------------------- CM3F element kernel
+// The three-field (CM3F) element of crystal plasticity with the hyperelastic law, condensed to
+// the displacement DOFs (ConstitutiveModelIntregrate::integrate_ss, src/constitutive_model.cc
+// :720-880, total Lagrangian, one theta and one P per element; the blocks of CM_ThreeField,
+// src/constitutive_model_3f.cc:189-470; condense_K_3F_to_1F / condense_F_3F_to_1F,
+// src/condense.cc:304-385).  Per QP, from the tangent pass's qpbuf: M = pF(n+1)^-1, pJ and
+// dM/dFb (Fb = (theta/J)^(1/3) F, the model's F); eF = F M with the unscaled F, Sd and Ld the
+// deviatoric PK2 and tangent of the neo-Hookean law at eF, dU, ddU = kappa U', kappa U'' at
+// theta_e = theta det M.  For a DOF k (node a, component b) with G_k = e_b (x) dN_a:
+//   Phi_k = sym(G_k^T F), Psi_k = M^T Phi_k M, tr_k = tr(G_k F^-1),
+//   dM_k = dM/dFb : dFb/du_k, dFb/du_k = f (G_k - tr_k/3 F), Lambda_k = tr(M^-1 dM_k),
+//   X_k = Psi_k + sym(M^T F^T F dM_k) (the change of the elastic strain),
+// and with cw = detJ w pJ (1/Jn), r_k = Psi_k : Sd + P J tr_k:
+//   Ru_k  += cw r_k,                       Kup_k += cw J tr_k,
+//   Kuu_kl += cw (Psi_k : Ld : X_l + P J (tr_k tr_l - tr(G_k F^-1 G_l F^-1))
+//                 + sym(G_k^T G_l) : M Sd M^T + 2 (Phi_k M Sd) : dM_l - Lambda_l r_k),
+//   Kut_k += cw (2 sym(M^T Phi_k dM_t) : Sd + Psi_k : Ld : X_t - Lambda_t r_k),
+//   Ktu_k += cw (JM (ddU theta JM + dU) - (dU JM - P)) Lambda_k,
+//   Rp += cw (J - theta), Rt += cw (dU JM - P), Ktt += cw (JM dU Lambda_t + JM^2 ddU
+//   - Lambda_t (dU JM - P)), Ktp -= cw, with the theta direction dM_t = dM/dFb : Fb/(3 theta).
+// The element's rows: one lane per DOF row, the QPs one at a time through LDS (the QP's
+// tensors, then every column's LX = Ld : X, dM, tr, Lambda, then the row sums).
+template <int NNE, int MODE, bool STG>
+__global__ __launch_bounds__(256) void cp_tf_elem(CPElemArgs a, int64_t e_begin, int64_t e_end) {
+  constexpr int NQP = nqp_of(NNE);
+  constexpr int NDE = NNE * 3;
+  constexpr int TPE = (NNE == 4) ? 16 : 32;
+  constexpr int EPB = 256 / TPE;
+  constexpr bool RES = (MODE & MODE_RESIDUAL) != 0;
+  constexpr bool TAN = (MODE & MODE_TANGENT) != 0;
+  constexpr bool DFL = (MODE & MODE_DEFL_LOAD) != 0;
+  static_assert(!DFL || (TAN && !RES), "MODE_DEFL_LOAD goes with MODE_TANGENT alone");
+  constexpr bool WK = TAN && !DFL;
+  constexpr bool WR = RES || DFL;
+  // per QP tensors in s_q: Fr 0, M 9, M^-1 18, M^T Cr 27, Sd 36, M Sd M^T 45, Ci (Voigt) 54,
+  // c1 c2 c3 60, dM/dFb : Fr 63, dM_t 72, Ld : X_t (Voigt) 81, cw 87, J 88, JM 89, dU 90,
+  // ddU 91, f 92, Lambda_t 93, dU JM - P 94, M Sd 95, M Sd dM_t^T 104, Fr^-1 113
+  constexpr int QD = 122;
+  constexpr int CD = 17;  // per DOF column: LX (Voigt) 0, dM 6, tr 15, Lambda 16
+  __shared__ double s_x[EPB][NDE], s_u[EPB][NDE];
+  __shared__ double s_dl[EPB][DFL ? NDE : 1];
+  __shared__ int32_t s_dof[EPB][NDE], s_node[EPB][NNE];
+  __shared__ double s_dN[EPB][NQP][NDE], s_w[EPB][NQP];
+  __shared__ double s_q[EPB][QD], s_D[EPB][81], s_col[EPB][NDE][CD], s_gn[EPB][NDE];
+  __shared__ double s_vec[EPB][2][NDE];
+
+  const int slot = threadIdx.x / TPE, lane = threadIdx.x % TPE;
+  const int64_t ie = e_begin + int64_t(blockIdx.x) * EPB + slot;
+  const bool active = ie < e_end;
+  const int64_t e = (a.elist && active) ? int64_t(a.elist[ie]) : ie;
+  if (active && lane < NNE) {
+    const int32_t node = a.conn[e * NNE + lane];
+    s_node[slot][lane] = node;
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      s_x[slot][lane * 3 + b] = a.X[int64_t(node) * 3 + b];
+      const int32_t d = a.dofid[int64_t(node) * 3 + b];
+      s_dof[slot][lane * 3 + b] = d;
+      double u = 0.0, dl = 0.0;
+      if (d > 0) {
+        u = a.r[d - 1] + (a.d_r ? a.d_r[d - 1] : 0.0);
+      } else if (d < 0) {
+        dl = a.defl_d ? a.defl_d[-d - 1] : 0.0;
+        u = (a.defl ? a.defl[-d - 1] : 0.0) + (DFL ? 0.0 : dl);
+      }
+      s_u[slot][lane * 3 + b] = u;
+      if constexpr (DFL) s_dl[slot][lane * 3 + b] = dl;
+    }
+  }
+  __syncthreads();
+  if (active && lane < NQP) {  // physical shape-function gradients and detJ w of every QP
+    const int ip = lane;
+    double k, ee, z, wt;
+    int_point(NNE, ip, k, ee, z, wt);
+    double dN[NDE];
+    shape_dN<NNE>(k, ee, z, dN);
+    double dX[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int n = 0; n < NNE; ++n)
+#pragma unroll
+      for (int ib = 0; ib < 3; ++ib)
+#pragma unroll
+        for (int ic = 0; ic < 3; ++ic) dX[ib * 3 + ic] += dN[n * 3 + ic] * s_x[slot][n * 3 + ib];
+    const double J0 = det3(dX);
+    double Xi[9];
+    const bool ok = J0 > 0.0 && inv3(dX, Xi);
+    if (!ok) atomicAdd(a.err, 1);
+#pragma unroll
+    for (int n = 0; n < NNE; ++n)
+#pragma unroll
+      for (int ib = 0; ib < 3; ++ib)
+        s_dN[slot][ip][n * 3 + ib] =
+            dN[n * 3 + 0] * Xi[0 * 3 + ib] + dN[n * 3 + 1] * Xi[1 * 3 + ib] + dN[n * 3 + 2] * Xi[2 * 3 + ib];
+    s_w[slot][ip] = ok ? J0 * wt : 0.0;
+  }
+  const double theta = active ? a.tf_V[e] + a.tf_dV[e] : 1.0;
+  const double Pp = active ? a.tf_P[e] + a.tf_dP[e] : 0.0;
+  const CPMaterial cm = a.cpmats[0];
+  const bool rowlane = active && lane < NDE;
+  const int ra = lane / 3, rb = lane % 3;
+  double K[NDE];
+#pragma unroll
+  for (int i = 0; i < NDE; ++i) K[i] = 0.0;
+  double Ru = 0.0, Kup = 0.0, Kut = 0.0, Ktu = 0.0, Rp = 0.0, Rt = 0.0, Ktt = 0.0, Ktp = 0.0;
+  __syncthreads();
+#pragma unroll 1
+  for (int ip = 0; ip < NQP; ++ip) {
+    if (active) {
+      const double *qb = a.qpbuf + (e * NQP + ip) * kCPBuf;
+      for (int i = lane; i < 81; i += TPE) s_D[slot][i] = qb[16 + i];
+    }
+    lds_barrier();
+    if (active && lane == 0) {  // the QP's tensors
+      double *q = s_q[slot];
+      const double *dN = s_dN[slot][ip];
+      double Fr[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+#pragma unroll
+      for (int n = 0; n < NNE; ++n)
+#pragma unroll
+        for (int ib = 0; ib < 3; ++ib)
+#pragma unroll
+          for (int ic = 0; ic < 3; ++ic) Fr[ib * 3 + ic] += s_u[slot][n * 3 + ib] * dN[n * 3 + ic];
+      const double *qb = a.qpbuf + (e * NQP + ip) * kCPBuf;
+      double Me[9], Mi[9], Fi[9];
+#pragma unroll
+      for (int i = 0; i < 9; ++i) Me[i] = qb[i];
+      const double pJ = qb[15];
+      const double J = det3(Fr);
+      const bool okq = J > 0.0 && theta > 0.0 && inv3(Fr, Fi) && inv3(Me, Mi);
+      if (!okq) atomicAdd(a.err, 1);
+      const double JM = det3(Me);
+      double eF[9], Ce[9], Cr[9], MtC[9];
+      mm3(Fr, Me, eF);
+      mtm3(eF, eF, Ce);
+      mtm3(Fr, Fr, Cr);
+      mtm3(Me, Cr, MtC);
+      // neo-Hookean deviatoric PK2 and tangent coefficients at Ce (he_law without kappa)
+      double Ci9[9];
+      inv3(Ce, Ci9);
+      const double d = pow(det3(Ce), -1.0 / 3.0);
+      const double trC = Ce[0] + Ce[4] + Ce[8];
+      const double aa = 2.0 * cm.m10 * d;
+      double Sd[9];
+#pragma unroll
+      for (int i = 0; i < 9; ++i) Sd[i] = aa * ((i % 4 == 0) ? 1.0 : 0.0) - aa * trC * (1.0 / 3.0) * Ci9[i];
+      HeTan td;
+#pragma unroll
+      for (int I = 0; I < 6; ++I) td.Ci[I] = Ci9[vi(I) * 3 + vj(I)];
+      td.c1 = 2.0 * aa * trC * (1.0 / 9.0);
+      td.c2 = -2.0 * aa * (1.0 / 3.0);
+      td.c3 = 2.0 * aa * trC * (1.0 / 3.0);
+      double dU, ddU;
+      vol_pot(cm.vol_flag, theta * JM, dU, ddU);
+      dU *= cm.kappa;
+      ddU *= cm.kappa;
+      const double fac = okq ? cbrt(theta / J) : 1.0;
+      // dM/dFb : Fr and the theta direction dM_t = f / (3 theta) dM/dFb : Fr
+      double DFr[9], dMt[9];
+#pragma unroll
+      for (int r = 0; r < 9; ++r) {
+        double s = 0.0;
+#pragma unroll
+        for (int c = 0; c < 9; ++c) s += s_D[slot][r * 9 + c] * Fr[c];
+        DFr[r] = s;
+        dMt[r] = fac / (3.0 * theta) * s;
+      }
+      double Lt = 0.0;  // tr(M^-1 dM_t)
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) Lt += Mi[i * 3 + j] * dMt[j * 3 + i];
+      double Xt[9], T[9], LXt[6];
+      mm3(MtC, dMt, T);
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) Xt[i * 3 + j] = 0.5 * (T[i * 3 + j] + T[j * 3 + i]);
+      he_dot(td, Xt, LXt);
+      double MeSd[9], Sh[9], W[9];
+      mm3(Me, Sd, MeSd);
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          Sh[i * 3 + j] = MeSd[i * 3 + 0] * Me[j * 3 + 0] + MeSd[i * 3 + 1] * Me[j * 3 + 1] + MeSd[i * 3 + 2] * Me[j * 3 + 2];
+          W[i * 3 + j] = MeSd[i * 3 + 0] * dMt[j * 3 + 0] + MeSd[i * 3 + 1] * dMt[j * 3 + 1] + MeSd[i * 3 + 2] * dMt[j * 3 + 2];
+        }
+#pragma unroll
+      for (int i = 0; i < 9; ++i) {
+        q[i] = Fr[i];
+        q[9 + i] = Me[i];
+        q[18 + i] = Mi[i];
+        q[27 + i] = MtC[i];
+        q[36 + i] = Sd[i];
+        q[45 + i] = Sh[i];
+        q[63 + i] = DFr[i];
+        q[72 + i] = dMt[i];
+        q[95 + i] = MeSd[i];
+        q[104 + i] = W[i];
+        q[113 + i] = Fi[i];
+      }
+#pragma unroll
+      for (int I = 0; I < 6; ++I) {
+        q[54 + I] = td.Ci[I];
+        q[81 + I] = LXt[I];
+      }
+      q[60] = td.c1;
+      q[61] = td.c2;
+      q[62] = td.c3;
+      q[87] = okq ? s_w[slot][ip] * pJ : 0.0;
+      q[88] = J;
+      q[89] = JM;
+      q[90] = dU;
+      q[91] = ddU;
+      q[92] = fac;
+      q[93] = Lt;
+      q[94] = dU * JM - Pp;
+    }
+    lds_barrier();
+    const double *q = s_q[slot];
+    const double *dN = s_dN[slot][ip];
+    double Psi[9], Phi[9];
+    if (rowlane) {  // the DOF's column data, and the row's own Psi
+      const double *Fr = q, *Me = q + 9, *Fi = q + 113;
+      const double ga[3] = {dN[ra * 3], dN[ra * 3 + 1], dN[ra * 3 + 2]};
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) Phi[i * 3 + j] = 0.5 * (ga[i] * Fr[rb * 3 + j] + ga[j] * Fr[rb * 3 + i]);
+      double T[9];
+      mm3(Phi, Me, T);
+      mtm3(Me, T, Psi);
+      // g_a = F^-T dN_a: tr_k = g_a,b
+      double g[3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) g[i] = ga[0] * Fi[0 * 3 + i] + ga[1] * Fi[1 * 3 + i] + ga[2] * Fi[2 * 3 + i];
+      s_gn[slot][lane] = g[rb];
+      const double trk = g[rb], fac = q[92];
+      double dM[9];
+#pragma unroll
+      for (int r = 0; r < 9; ++r) {
+        const double *Dr = s_D[slot] + r * 9 + rb * 3;
+        dM[r] = fac * (Dr[0] * ga[0] + Dr[1] * ga[1] + Dr[2] * ga[2] - trk * (1.0 / 3.0) * q[63 + r]);
+      }
+      double Lk = 0.0;
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) Lk += q[18 + i * 3 + j] * dM[j * 3 + i];
+      double X[9], V[9];
+      mm3(q + 27, dM, V);
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) X[i * 3 + j] = Psi[i * 3 + j] + 0.5 * (V[i * 3 + j] + V[j * 3 + i]);
+      HeTan td;
+#pragma unroll
+      for (int I = 0; I < 6; ++I) td.Ci[I] = q[54 + I];
+      td.c1 = q[60];
+      td.c2 = q[61];
+      td.c3 = q[62];
+      double LX[6];
+      he_dot(td, X, LX);
+      double *col = s_col[slot][lane];
+#pragma unroll
+      for (int I = 0; I < 6; ++I) col[I] = LX[I];
+#pragma unroll
+      for (int r = 0; r < 9; ++r) col[6 + r] = dM[r];
+      col[15] = trk;
+      col[16] = Lk;
+    }
+    lds_barrier();
+    if (active) {
+      const double cw = q[87], J = q[88], JM = q[89], dU = q[90], ddU = q[91], Lt = q[93], vt = q[94];
+      Rp += cw * (J - theta);  // compute_Rp
+      Rt += cw * vt;           // compute_Rt
+      Ktt += cw * (JM * dU * Lt + JM * JM * ddU - Lt * vt);  // compute_Ktt
+      Ktp -= cw;               // compute_Ktp
+    }
+    if (rowlane) {
+      const double cw = q[87], J = q[88], JM = q[89], dU = q[90], ddU = q[91], Lt = q[93], vt = q[94];
+      const double *Sd = q + 36;
+      double PsiS = 0.0;
+#pragma unroll
+      for (int i = 0; i < 9; ++i) PsiS += Psi[i] * Sd[i];
+      const double trk = s_col[slot][lane][15], Lk = s_col[slot][lane][16];
+      const double rk = PsiS + Pp * J * trk;
+      // Psi : Ld : X_t and 2 sym(M^T Phi dM_t) : Sd = 2 Phi : (M Sd dM_t^T)
+      const double *LXt = q + 81, *W = q + 104;
+      const double PLt = Psi[0] * LXt[0] + Psi[4] * LXt[1] + Psi[8] * LXt[2] +
+                         2.0 * (Psi[1] * LXt[3] + Psi[5] * LXt[4] + Psi[2] * LXt[5]);
+      double DtS = 0.0;
+#pragma unroll
+      for (int i = 0; i < 9; ++i) DtS += 2.0 * Phi[i] * W[i];
+      Ru += cw * rk;                                                // compute_Ru
+      Kup += cw * J * trk;                                          // compute_Kup
+      Kut += cw * (DtS + PLt - Lt * rk);                            // compute_Kut
+      Ktu += cw * (JM * (ddU * theta * JM + dU) * Lk - Lk * vt);    // compute_Ktu
+      if constexpr (TAN) {
+        // Z = 2 Phi_k M Sd, t = M Sd M^T dN_a
+        double Z[9];
+        mm3(Phi, q + 95, Z);
+        const double *Sh = q + 45;
+        const double ga[3] = {dN[ra * 3], dN[ra * 3 + 1], dN[ra * 3 + 2]};
+        double ta[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) ta[i] = Sh[i * 3] * ga[0] + Sh[i * 3 + 1] * ga[1] + Sh[i * 3 + 2] * ga[2];
+        double gk[3];  // g_a (F^-T dN_a), all three components
+#pragma unroll
+        for (int i = 0; i < 3; ++i) gk[i] = s_gn[slot][ra * 3 + i];
+#pragma unroll
+        for (int l = 0; l < NDE; ++l) {
+          const int w = l / 3, bp = l % 3;
+          const double *col = s_col[slot][l];
+          const double PLX = Psi[0] * col[0] + Psi[4] * col[1] + Psi[8] * col[2] +
+                             2.0 * (Psi[1] * col[3] + Psi[5] * col[4] + Psi[2] * col[5]);
+          double dMS = 0.0;
+#pragma unroll
+          for (int i = 0; i < 9; ++i) dMS += 2.0 * Z[i] * col[6 + i];
+          const double GGS = (bp == rb) ? ta[0] * dN[w * 3] + ta[1] * dN[w * 3 + 1] + ta[2] * dN[w * 3 + 2] : 0.0;
+          const double tr2 = gk[bp] * s_gn[slot][w * 3 + rb];
+          K[l] += cw * (PLX + Pp * J * (trk * col[15] - tr2) + GGS + dMS - col[16] * rk);  // compute_Kuu
+        }
+      }
+    }
+    lds_barrier();  // the next QP overwrites s_q, s_D, s_col, s_gn
+  }
+  // condensation: K = Kuu - Kup Ktu / Ktp + Kup Ktt Kpu / (Ktp Kpt) - Kut Kpu / Kpt, with
+  // Kpu = Kup^T, Kpt = Ktp (IntegrateThreeFieldStiffness::out_loop, constitutive_model.cc:531-555)
+  if (rowlane) {
+    s_vec[slot][0][lane] = Kup;
+    s_vec[slot][1][lane] = Ktu;
+  }
+  lds_barrier();
+  double fe = 0.0;
+  if (rowlane) {
+    const double Kpt = Ktp;
+    if constexpr (TAN) {
+#pragma unroll
+      for (int l = 0; l < NDE; ++l)
+        K[l] = K[l] - Kup / Ktp * s_vec[slot][1][l] + Kup / Ktp * Ktt / Kpt * s_vec[slot][0][l] -
+               Kut / Kpt * s_vec[slot][0][l];
+      double diag = 0.0;
+#pragma unroll
+      for (int i = 0; i < NDE; ++i) diag = (i == lane) ? K[i] : diag;
+      if (!isnormal(diag)) atomicAdd(a.err, 1);
+    }
+    if constexpr (RES) fe = Ru + Kup / Ktp * Ktt / Kpt * Rp - Kup / Ktp * Rt - Kut / Kpt * Rp;
+    if constexpr (DFL) {
+      double s = 0.0;
+#pragma unroll
+      for (int i = 0; i < NDE; ++i) s += K[i] * s_dl[slot][i];
+      fe = s;
+    }
+    if constexpr (RES) {  // the element update's data (compute_d_theta_dP)
+      double *tb = a.tf_buf + e * kTfBuf;
+      tb[lane] = Kup;
+      tb[kTfKtu + lane] = Ktu;
+      if (lane == 0) {
+        tb[NDE] = Rp;
+        tb[NDE + 1] = Rt;
+        tb[NDE + 2] = Ktt;
+        tb[NDE + 3] = -Ktp;
+      }
+    }
+  }
+  if (!rowlane) return;
+  const int32_t dof = s_dof[slot][lane];
+  if (dof <= 0) return;
+  constexpr int LD = NDE + 1;
+  if constexpr (STG) {  // gather mode: the row straight into its staging block
+    double *srow = a.stage + a.soff[e * NNE + ra] + rb * LD;
+    if constexpr (WK) {
+#pragma unroll
+      for (int i = 0; i < NDE; ++i) srow[i] = K[i];
+    }
+    if constexpr (WR) srow[NDE] = fe;
+    return;
+  }
+  const int16_t *off = a.off + (e * NNE + ra) * NNE;
+  if constexpr (WR) {
+    if (off[ra] <= -2) a.f[dof - 1] = fe;
+    else a.f[dof - 1] += fe;
+  }
+  if constexpr (WK) {
+    const int32_t node = s_node[slot][ra];
+    int bp = 0;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) bp += (j < rb && s_dof[slot][ra * 3 + j] > 0) ? 1 : 0;
+    double *row = a.vals + (int64_t(a.row_start[node]) + int64_t(bp) * a.rowlen[node]);
+#pragma unroll
+    for (int w = 0; w < NNE; ++w) {
+      const int o = off[w];
+      if (o == -1) continue;
+      const bool first = o <= -2;
+      const int base = first ? (-o - 2) : o;
+      int gp = 0;
+#pragma unroll
+      for (int gg = 0; gg < 3; ++gg) {
+        if (s_dof[slot][w * 3 + gg] <= 0) continue;
+        double *p = row + base + gp;
+        *p = first ? K[w * 3 + gg] : *p + K[w * 3 + gg];
+        ++gp;
+      }
+    }
+  }
+}
+
+template <int NNE, int MODE>
+void launch_tfe(const CPElemArgs &a, int64_t e0, int64_t e1, hipStream_t s) {
+  constexpr int TPE = (NNE == 4) ? 16 : 32;
+  constexpr int EPB = 256 / TPE;
+  const int64_t n = e1 - e0;
+  if (n <= 0) return;
+  const unsigned grid = unsigned((n + EPB - 1) / EPB);
+  if (a.stage) cp_tf_elem<NNE, MODE, true><<<grid, 256, 0, s>>>(a, e0, e1);
+  else cp_tf_elem<NNE, MODE, false><<<grid, 256, 0, s>>>(a, e0, e1);
+}
+
+template <int NNE>
+void launch_tfe_mode(int mode, const CPElemArgs &a, int64_t e0, int64_t e1, hipStream_t s) {
+  switch (mode & (MODE_RESIDUAL | MODE_TANGENT | MODE_DEFL_LOAD)) {
+    case MODE_RESIDUAL: launch_tfe<NNE, MODE_RESIDUAL>(a, e0, e1, s); break;
+    case MODE_TANGENT: launch_tfe<NNE, MODE_TANGENT>(a, e0, e1, s); break;
+    case MODE_RESIDUAL | MODE_TANGENT: launch_tfe<NNE, MODE_RESIDUAL | MODE_TANGENT>(a, e0, e1, s); break;
+    case MODE_TANGENT | MODE_DEFL_LOAD: launch_tfe<NNE, MODE_TANGENT | MODE_DEFL_LOAD>(a, e0, e1, s); break;
+    default: break;
+  }
+}
+
 template <int NNE>
 void launch_kin(const CPArgs &a, hipStream_t s) {
   const int64_t n = a.ne * nqp_of(NNE);
@@ -1577,6 +2018,11 @@ void launch_cp_elem(int nne, int mode, const CPElemArgs &a, int64_t e0, int64_t 
   if (nne == 4) launch_ce_mode<4>(mode, a, e0, e1, s);
   else if (nne == 10) launch_ce_mode<10>(mode, a, e0, e1, s);
   else if (nne == 8) launch_ce_mode<8>(mode, a, e0, e1, s);
+}
+void launch_cp_tf_elem(int nne, int mode, const CPElemArgs &a, int64_t e0, int64_t e1, hipStream_t s) {
+  if (nne == 4) launch_tfe_mode<4>(mode, a, e0, e1, s);
+  else if (nne == 10) launch_tfe_mode<10>(mode, a, e0, e1, s);
+  else if (nne == 8) launch_tfe_mode<8>(mode, a, e0, e1, s);
 }
 
 }  // namespace pgf

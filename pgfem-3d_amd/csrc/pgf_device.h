@@ -70,11 +70,13 @@ struct ElemArgs {
   int32_t *err;             // failure counter
   // three-field (CM3F) blocks, per block element (block-local index): theta = V(n+1) + dV and
   // P = P(n+1) + dP, and the element data of the last residual pass for the update (kTfBuf
-  // doubles: kup[3nne], fp, ft, Ktt, V).  Null on a displacement-only handle.
+  // doubles: kup[3nne], fp, ft, Ktt, V = -Ktp, then from kTfKtu on ktu[3nne], which only the
+  // crystal-plasticity element fills).  Null on a displacement-only handle.
   const double *tf_V, *tf_dV, *tf_P, *tf_dP;
   double *tf_buf;
 };
-constexpr int kTfBuf = 34;
+constexpr int kTfKtu = 34;
+constexpr int kTfBuf = 64;
 // per-element three-field slots of the handle: V(n+1), V(n), V(n-1), P(n+1), P(n), P(n-1), dV,
 // ddV, dP, ddP, the full ddV and ddP of the last update, the dV and dP its trials start from
 constexpr int kTfSlots = 14;
@@ -143,6 +145,9 @@ struct CPArgs {
   int32_t *err;
   int32_t *queue;             // work-queue counter
   int32_t law;                // cpmats[0].law, read on the host to pick the kernels
+  // three-field (CM3F) blocks: per block element theta = tf_V + tf_dV; the kinematics then
+  // stores the model's F(n+1) = (theta/J)^(1/3) F.  Null otherwise.
+  const double *tf_V, *tf_dV;
 };
 constexpr int kCPBuf = 97;  // doubles per QP in qpbuf
 
@@ -185,8 +190,14 @@ struct CPElemArgs {
   const int64_t *soff;  // gather mode: as ElemArgs
   double *stage;
   int32_t *err;
+  // three-field (CM3F) blocks: as ElemArgs (block-local per element)
+  const double *tf_V, *tf_dV, *tf_P, *tf_dP;
+  double *tf_buf;
 };
 void launch_cp_elem(int nne, int mode, const CPElemArgs &a, int64_t e_begin, int64_t e_end, hipStream_t s);
+// The CM3F element of crystal plasticity (hyperelastic elastic law), condensed to the
+// displacement DOFs; gather (a.stage) or coloured mode as launch_cp_elem.
+void launch_cp_tf_elem(int nne, int mode, const CPElemArgs &a, int64_t e_begin, int64_t e_end, hipStream_t s);
 
 // Learned warm start: features / targets of a block's QPs (rows indexed by device QP) and the
 // Newton start from predicted slip increments (block-local rows of Y).

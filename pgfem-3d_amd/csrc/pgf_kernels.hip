@@ -977,24 +977,29 @@ __global__ void copy_kernel(double *dst, const double *src, int64_t n) {
 // src/constitutive_model.cc:3916-3999; compute_d_theta_dP, src/constitutive_model_3f.cc:494-549),
 // one thread per element, from the element data of the last residual pass and the
 // displacement increment du (prescribed DOFs zero: def_elem TYPE 2, src/utils.cc:1365-1389):
-//   d_theta = -(Kpu du + fp) / Kpt = (Kup . du + fp) / V,   dP = -(ft + Ktt d_theta) / Ktp.
+//   d_theta = -(Kpu du + fp) / Kpt = (Kup . du + fp) / V,
+//   dP = -(ft + Ktu du + Ktt d_theta) / Ktp = (ft + Ktu . du + Ktt d_theta) / V
+// (Ktu = 0 for the hyperelastic model, whose buffer keeps zeros there).
 template <int NNE>
 __global__ __launch_bounds__(256) void tf_update_kernel(TfArgs t) {
   constexpr int NDE = 3 * NNE;
   const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (e >= t.ne) return;
   const double *b = t.buf + e * kTfBuf;
-  double s = 0.0;
+  double s = 0.0, s2 = 0.0;
 #pragma unroll
   for (int k = 0; k < NDE; ++k) {
     const int32_t d = t.edof[e * NDE + k];
-    if (d > 0) s += b[k] * t.du[d - 1];
+    if (d > 0) {
+      s += b[k] * t.du[d - 1];
+      s2 += b[kTfKtu + k] * t.du[d - 1];
+    }
   }
   const double V = b[NDE + 3];
   const double rV = V > 0.0 ? 1.0 / V : 0.0;
   const double dth = (s + b[NDE]) * rV;
   t.ddV[e] = dth;
-  t.ddP[e] = (b[NDE + 1] + b[NDE + 2] * dth) * rV;
+  t.ddP[e] = (b[NDE + 1] + s2 + b[NDE + 2] * dth) * rV;
 }
 
 // Initial volume and pressure of each element (compute_3f_initial_conditions,
