@@ -296,6 +296,48 @@ def run_tf(args, dev):
     return res
 
 
+def run_cp_tf(args, dev):
+    """CM3F crystal plasticity: the CP workload's mesh (55^3 P1 tets, 64 grains) with the
+    hyperelastic elastic law, five load steps of 1e-3 then the sixth timed, in the three-field
+    mode (theta = 1, P = 0 at the start of each step) and in the displacement formulation on a
+    handle of its own with the same history."""
+    import torch
+
+    import pgfem3d_amd as P
+    from problems import CPBoxProblem, fcc_crystal_he
+
+    mat = fcc_crystal_he()
+    prob = CPBoxProblem(args.cp_n, P.PGF_TET4, mat, n_grains=64, seed=2)
+    f = torch.empty(prob.ndof, dtype=torch.float64, device=dev)
+    out = {}
+    for tf in (False, True):
+        h = P.Handle(prob.mesh, prob.ids, prob.ndof, prob.Ap, prob.Ai, prob.mats)
+        h.set_orientations(prob.R, prob.grain)
+        vals = torch.empty(h.nnz, dtype=torch.float64, device=dev)
+        if tf:
+            h.set_three_field(True)
+            h.tf_initial(torch.zeros(prob.ndof, dtype=torch.float64, device=dev),
+                         torch.zeros(1, dtype=torch.float64, device=dev))
+        for s in range(1, 6):
+            r, defl = prob.displacement(CP_STEP * s, 0.0, seed=s)
+            err = h.residual(f, torch.from_numpy(r).to(dev), defl=torch.from_numpy(defl).to(dev), dt=1.0)
+            assert err == 0, (tf, s, err)
+            h.commit_step()
+        r, defl = prob.displacement(CP_STEP * 6, 0.0, seed=6)
+        h.set_profiling(True)
+        s_step, kern_ms, n_launch = _timed_passes(h, f, vals, torch.from_numpy(r).to(dev),
+                                                  torch.from_numpy(defl).to(dev), args.cp_steps, 1)
+        out["tf" if tf else "1f"] = (s_step, kern_ms, n_launch, float(h.get_state(P.STATE_ITERS)[0].mean()))
+        h.close()
+    s_tf, k_tf, n_l, its = out["tf"]
+    res = dict(n_qp=prob.nqp, s_step=s_tf, kern_ms=k_tf, n_launch=n_l, mean_iters=its, kern_ms_1f=out["1f"][1],
+               s_step_1f=out["1f"][0])
+    print(f"[bench] CM3F CP (hyperelastic law) {prob.mesh.n_elem} P1 tets: {1e3 * s_tf:.3f} ms/step wall, pass "
+          f"{k_tf:.3f} ms over {n_l} launches, local iterations {its:.2f}; displacement formulation "
+          f"{out['1f'][1]:.3f} ms", file=sys.stderr, flush=True)
+    return res
+
+
 def run_cp(args, dev):
     import torch
 
@@ -487,6 +529,7 @@ def main():
     cp = None if args.no_cp else run_cp(args, dev)
     mixed = None if args.no_mixed else run_mixed(args, dev)
     tf = None if args.no_tf else run_tf(args, dev)
+    cptf = None if (args.no_tf or args.no_cp) else run_cp_tf(args, dev)
 
     cpu = cpu_cp = None
     if not args.no_cpu:
@@ -568,6 +611,21 @@ def main():
             "displacement_formulation_kernel_ms_per_pass": tf["kern_ms_1f"],
             "roofline": {"bound": "hbm", "achieved": tf["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": tf["achieved"] / HBM_PEAK_GBS},
+        }
+    if cptf is not None:
+        flops = measured_flops(f"cp_tf_tet4_{args.cp_n}cube")
+        ach = None if flops is None else flops / (cptf["kern_ms"] * 1e-3) / 1e12
+        out["three_field_cp"] = {
+            "workload": f"CM3F FCC crystal plasticity with the hyperelastic elastic law, {args.cp_n}^3 cells x 6 "
+                        f"linear tets ({cptf['n_qp']} QPs), 64 Voronoi grains, step 6 of 1e-3/s at dt = 1 s "
+                        "(theta = 1, P = 0 at the step's start); one fused residual + tangent + CSR assembly pass",
+            "value": cptf["n_qp"] / cptf["s_step"], "unit": "QP updates/s", "ms_per_step": 1e3 * cptf["s_step"],
+            "kernel_ms_per_pass": cptf["kern_ms"], "launches_per_pass": cptf["n_launch"],
+            "mean_local_iters": cptf["mean_iters"],
+            "displacement_formulation_kernel_ms_per_pass": cptf["kern_ms_1f"],
+            "roofline": {"bound": "fp64", "achieved": ach, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": None if ach is None else ach / FP64_PEAK_TFLOPS,
+                         "basis": "executed FP64 instructions (profiles/fp64_counters.md) over the live pass time"},
         }
     print(json.dumps(out))
 

@@ -1406,8 +1406,9 @@ __global__ __launch_bounds__(256) void cp_tf_elem(CPElemArgs a, int64_t e_begin,
       for (int i = lane; i < 81; i += TPE) s_D[slot][i] = qb[16 + i];
     }
     lds_barrier();
-    if (active && lane == 0) {  // the QP's tensors
-      double *q = s_q[slot];
+    // the QP's tensors in three rounds, lanes of an element working side by side
+    double *q = s_q[slot];
+    if (active && lane == 0) {  // Fr, M, M^-1, Fr^-1, J, det M
       const double *dN = s_dN[slot][ip];
       double Fr[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
 #pragma unroll
@@ -1420,100 +1421,98 @@ __global__ __launch_bounds__(256) void cp_tf_elem(CPElemArgs a, int64_t e_begin,
       double Me[9], Mi[9], Fi[9];
 #pragma unroll
       for (int i = 0; i < 9; ++i) Me[i] = qb[i];
-      const double pJ = qb[15];
       const double J = det3(Fr);
       const bool okq = J > 0.0 && theta > 0.0 && inv3(Fr, Fi) && inv3(Me, Mi);
       if (!okq) atomicAdd(a.err, 1);
-      const double JM = det3(Me);
-      double eF[9], Ce[9], Cr[9], MtC[9];
-      mm3(Fr, Me, eF);
-      mtm3(eF, eF, Ce);
-      mtm3(Fr, Fr, Cr);
-      mtm3(Me, Cr, MtC);
-      // neo-Hookean deviatoric PK2 and tangent coefficients at Ce (he_law without kappa)
-      double Ci9[9];
-      inv3(Ce, Ci9);
-      const double d = pow(det3(Ce), -1.0 / 3.0);
-      const double trC = Ce[0] + Ce[4] + Ce[8];
-      const double aa = 2.0 * cm.m10 * d;
-      double Sd[9];
-#pragma unroll
-      for (int i = 0; i < 9; ++i) Sd[i] = aa * ((i % 4 == 0) ? 1.0 : 0.0) - aa * trC * (1.0 / 3.0) * Ci9[i];
-      HeTan td;
-#pragma unroll
-      for (int I = 0; I < 6; ++I) td.Ci[I] = Ci9[vi(I) * 3 + vj(I)];
-      td.c1 = 2.0 * aa * trC * (1.0 / 9.0);
-      td.c2 = -2.0 * aa * (1.0 / 3.0);
-      td.c3 = 2.0 * aa * trC * (1.0 / 3.0);
-      double dU, ddU;
-      vol_pot(cm.vol_flag, theta * JM, dU, ddU);
-      dU *= cm.kappa;
-      ddU *= cm.kappa;
-      const double fac = okq ? cbrt(theta / J) : 1.0;
-      // dM/dFb : Fr and the theta direction dM_t = f / (3 theta) dM/dFb : Fr
-      double DFr[9], dMt[9];
-#pragma unroll
-      for (int r = 0; r < 9; ++r) {
-        double s = 0.0;
-#pragma unroll
-        for (int c = 0; c < 9; ++c) s += s_D[slot][r * 9 + c] * Fr[c];
-        DFr[r] = s;
-        dMt[r] = fac / (3.0 * theta) * s;
-      }
-      double Lt = 0.0;  // tr(M^-1 dM_t)
-#pragma unroll
-      for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j) Lt += Mi[i * 3 + j] * dMt[j * 3 + i];
-      double Xt[9], T[9], LXt[6];
-      mm3(MtC, dMt, T);
-#pragma unroll
-      for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j) Xt[i * 3 + j] = 0.5 * (T[i * 3 + j] + T[j * 3 + i]);
-      he_dot(td, Xt, LXt);
-      double MeSd[9], Sh[9], W[9];
-      mm3(Me, Sd, MeSd);
-#pragma unroll
-      for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j) {
-          Sh[i * 3 + j] = MeSd[i * 3 + 0] * Me[j * 3 + 0] + MeSd[i * 3 + 1] * Me[j * 3 + 1] + MeSd[i * 3 + 2] * Me[j * 3 + 2];
-          W[i * 3 + j] = MeSd[i * 3 + 0] * dMt[j * 3 + 0] + MeSd[i * 3 + 1] * dMt[j * 3 + 1] + MeSd[i * 3 + 2] * dMt[j * 3 + 2];
-        }
 #pragma unroll
       for (int i = 0; i < 9; ++i) {
         q[i] = Fr[i];
         q[9 + i] = Me[i];
         q[18 + i] = Mi[i];
-        q[27 + i] = MtC[i];
-        q[36 + i] = Sd[i];
-        q[45 + i] = Sh[i];
-        q[63 + i] = DFr[i];
-        q[72 + i] = dMt[i];
-        q[95 + i] = MeSd[i];
-        q[104 + i] = W[i];
         q[113 + i] = Fi[i];
       }
-#pragma unroll
-      for (int I = 0; I < 6; ++I) {
-        q[54 + I] = td.Ci[I];
-        q[81 + I] = LXt[I];
-      }
-      q[60] = td.c1;
-      q[61] = td.c2;
-      q[62] = td.c3;
-      q[87] = okq ? s_w[slot][ip] * pJ : 0.0;
+      q[87] = okq ? s_w[slot][ip] * qb[15] : 0.0;  // detJ w / Jn, 1/Jn = pJ
       q[88] = J;
-      q[89] = JM;
-      q[90] = dU;
-      q[91] = ddU;
-      q[92] = fac;
-      q[93] = Lt;
-      q[94] = dU * JM - Pp;
+      q[89] = det3(Me);
+      q[92] = okq ? cbrt(theta / J) : 1.0;
     }
     lds_barrier();
-    const double *q = s_q[slot];
+    if (active && lane == 0) {  // M^T Cr
+      double Cr[9], MtC[9];
+      mtm3(q, q, Cr);
+      mtm3(q + 9, Cr, MtC);
+#pragma unroll
+      for (int i = 0; i < 9; ++i) q[27 + i] = MtC[i];
+    } else if (active && lane == 1) {
+      // neo-Hookean deviatoric PK2 and tangent coefficients at Ce = (F M)^T F M (he_law
+      // without kappa), and the volumetric derivatives at theta_e = theta det M
+      double eF[9], Ce[9], Ci9[9];
+      mm3(q, q + 9, eF);
+      mtm3(eF, eF, Ce);
+      inv3(Ce, Ci9);
+      const double d = pow(det3(Ce), -1.0 / 3.0);
+      const double trC = Ce[0] + Ce[4] + Ce[8];
+      const double aa = 2.0 * cm.m10 * d;
+#pragma unroll
+      for (int i = 0; i < 9; ++i) q[36 + i] = aa * ((i % 4 == 0) ? 1.0 : 0.0) - aa * trC * (1.0 / 3.0) * Ci9[i];
+#pragma unroll
+      for (int I = 0; I < 6; ++I) q[54 + I] = Ci9[vi(I) * 3 + vj(I)];
+      q[60] = 2.0 * aa * trC * (1.0 / 9.0);
+      q[61] = -2.0 * aa * (1.0 / 3.0);
+      q[62] = 2.0 * aa * trC * (1.0 / 3.0);
+      const double JM = q[89];
+      double dU, ddU;
+      vol_pot(cm.vol_flag, theta * JM, dU, ddU);
+      q[90] = dU * cm.kappa;
+      q[91] = ddU * cm.kappa;
+      q[94] = dU * cm.kappa * JM - Pp;
+    } else if (active && lane >= 2 && lane < 11) {  // row r of dM/dFb : Fr and of dM_t
+      const int r = lane - 2;
+      double sd = 0.0;
+#pragma unroll
+      for (int c = 0; c < 9; ++c) sd += s_D[slot][r * 9 + c] * q[c];
+      q[63 + r] = sd;
+      q[72 + r] = q[92] / (3.0 * theta) * sd;  // dM_t = f / (3 theta) dM/dFb : Fr
+    }
+    lds_barrier();
+    if (active && lane == 0) {  // tr(M^-1 dM_t) and Ld : sym(M^T Cr dM_t)
+      double Lt = 0.0;
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) Lt += q[18 + i * 3 + j] * q[72 + j * 3 + i];
+      q[93] = Lt;
+      double T[9], Xt[9], LXt[6];
+      mm3(q + 27, q + 72, T);
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) Xt[i * 3 + j] = 0.5 * (T[i * 3 + j] + T[j * 3 + i]);
+      HeTan td;
+#pragma unroll
+      for (int I = 0; I < 6; ++I) td.Ci[I] = q[54 + I];
+      td.c1 = q[60];
+      td.c2 = q[61];
+      td.c3 = q[62];
+      he_dot(td, Xt, LXt);
+#pragma unroll
+      for (int I = 0; I < 6; ++I) q[81 + I] = LXt[I];
+    } else if (active && lane == 1) {  // M Sd, M Sd M^T, M Sd dM_t^T
+      double MeSd[9];
+      mm3(q + 9, q + 36, MeSd);
+#pragma unroll
+      for (int i = 0; i < 9; ++i) q[95 + i] = MeSd[i];
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          q[45 + i * 3 + j] = MeSd[i * 3 + 0] * q[9 + j * 3 + 0] + MeSd[i * 3 + 1] * q[9 + j * 3 + 1] +
+                              MeSd[i * 3 + 2] * q[9 + j * 3 + 2];
+          q[104 + i * 3 + j] = MeSd[i * 3 + 0] * q[72 + j * 3 + 0] + MeSd[i * 3 + 1] * q[72 + j * 3 + 1] +
+                               MeSd[i * 3 + 2] * q[72 + j * 3 + 2];
+        }
+    }
+    lds_barrier();
     const double *dN = s_dN[slot][ip];
     double Psi[9], Phi[9];
     if (rowlane) {  // the DOF's column data, and the row's own Psi
@@ -1604,7 +1603,9 @@ __global__ __launch_bounds__(256) void cp_tf_elem(CPElemArgs a, int64_t e_begin,
         double gk[3];  // g_a (F^-T dN_a), all three components
 #pragma unroll
         for (int i = 0; i < 3; ++i) gk[i] = s_gn[slot][ra * 3 + i];
-#pragma unroll
+        // not unrolled: the column data are read from LDS one column at a time (unrolled, the
+        // compiler hoists all of them and spills)
+#pragma unroll 1
         for (int l = 0; l < NDE; ++l) {
           const int w = l / 3, bp = l % 3;
           const double *col = s_col[slot][l];
