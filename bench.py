@@ -242,11 +242,19 @@ def run_j2(args, dev):
     h.set_profiling(True)
     modes = {"gather": P.ASSEMBLY_GATHER, "coloured": P.ASSEMBLY_COLOURED}
     h.set_assembly(modes[args.assembly])
+    tuning = None
+    if args.assembly == "gather" and not args.no_autotune:
+        t0 = time.perf_counter()
+        h.autotune(rd, defl=dd)  # the library's own tuner picks the gather tile
+        tuning = h.tuning()
+        print(f"[bench] J2: autotuned gather tile {tuning['gather_waves']} nodes/workgroup x "
+              f"{tuning['gather_batch']} blocks in flight ({time.perf_counter() - t0:.1f}s)", file=sys.stderr,
+              flush=True)
     s_step, kern_ms, n_launch = _timed_passes(h, f, vals, rd, dd, args.steps, args.warmup)
     nbytes = algorithmic_bytes(h.n_qp, h.nnz, prob.ndof, prob.mesh.n_nodes, prob.mesh.n_elem, 10)
     res = dict(n_qp=h.n_qp, nnz=h.nnz, ndof=prob.ndof, n_elem=prob.mesh.n_elem, s_step=s_step, kern_ms=kern_ms,
                n_launch=n_launch, achieved=nbytes / (kern_ms * 1e-3) / 1e9,
-               tflops=flops_per_qp(10) * h.n_qp / (kern_ms * 1e-3) / 1e12)
+               tflops=flops_per_qp(10) * h.n_qp / (kern_ms * 1e-3) / 1e12, tuning=tuning)
     print(f"[bench] J2 ({args.assembly} assembly): {1e3 * s_step:.3f} ms/step wall, element pass {kern_ms:.3f} ms "
           f"over {n_launch} launches, {res['achieved']:.0f} GB/s algorithmic, {res['tflops']:.2f} TFLOP/s",
           file=sys.stderr, flush=True)
@@ -385,12 +393,18 @@ def run_cp(args, dev):
     rd = torch.from_numpy(r).to(dev)
     dd = torch.from_numpy(defl).to(dev)
     h.set_profiling(True)
+    tuning = None
+    if not args.no_autotune:  # points per wavefront of the work queue and the gather tile
+        h.autotune(rd, defl=dd)
+        tuning = h.tuning()
+        print(f"[bench] CP: autotuned {tuning['cp_chunk']} QPs per wave range, gather tile "
+              f"{tuning['gather_waves']} x {tuning['gather_batch']}", file=sys.stderr, flush=True)
     s_step, kern_ms, n_launch = _timed_passes(h, f, vals, rd, dd, args.cp_steps, 1)
     its = h.get_state(P.STATE_ITERS)[0]
     mean_it = float(its.mean())
     tflops = cp_flops_per_qp(mean_it) * h.n_qp / (kern_ms * 1e-3) / 1e12
     res = dict(n_qp=h.n_qp, s_step=s_step, kern_ms=kern_ms, n_launch=n_launch, mean_iters=mean_it,
-               max_iters=int(its.max()), tflops=tflops, surrogate=None)
+               max_iters=int(its.max()), tflops=tflops, surrogate=None, tuning=tuning)
     print(f"[bench] CP: {1e3 * s_step:.3f} ms/step wall, kernels {kern_ms:.3f} ms/pass over {n_launch} launches, "
           f"local Newton iterations mean {mean_it:.2f} max {int(its.max())}, {tflops:.2f} TFLOP/s",
           file=sys.stderr, flush=True)
@@ -511,6 +525,8 @@ def main():
     ap.add_argument("--no-tf", action="store_true", help="skip the three-field (CM3F) passes")
     ap.add_argument("--mix", type=int, nargs=3, default=[41, 32, 46],
                     help="cells per edge of the linear-tet, quadratic-tet and hex boxes of the mixed batch")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="time the J2 and CP passes at the default launch configuration")
     ap.add_argument("--sur-epochs", type=int, default=3)
     ap.add_argument("--sur-steps", type=int, default=20,
                     help="plain load steps logged as training data for the learned start (20: 20M QP states)")
@@ -570,6 +586,7 @@ def main():
             "traffic": measured_traffic(f"j2_tet10_{args.n}cube"),
         },
         "fp64": fp64_entry(j2["tflops"], measured_flops(f"j2_tet10_{args.n}cube"), j2["kern_ms"]),
+        "tuning": j2["tuning"],
         "kernel_ms_per_pass": j2["kern_ms"],
         "launches_per_pass": j2["n_launch"],
         "assembly": args.assembly,
@@ -583,6 +600,7 @@ def main():
             "value": cp["n_qp"] / cp["s_step"],
             "unit": "QP updates/s",
             "ms_per_step": 1e3 * cp["s_step"],
+            "tuning": cp["tuning"],
             "kernel_ms_per_pass": cp["kern_ms"],
             "mean_local_iters": cp["mean_iters"],
             "max_local_iters": cp["max_iters"],

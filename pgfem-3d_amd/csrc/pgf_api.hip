@@ -178,9 +178,11 @@ struct GraphKey {
   const void *r, *d_r, *defl, *defl_d, *f, *vals, *warm;
   double dt;
   int64_t chunk;
+  int32_t gwaves, gbatch;
   bool operator==(const GraphKey &o) const {
     return mode == o.mode && r == o.r && d_r == o.d_r && defl == o.defl && defl_d == o.defl_d && f == o.f &&
-           vals == o.vals && warm == o.warm && dt == o.dt && chunk == o.chunk;
+           vals == o.vals && warm == o.warm && dt == o.dt && chunk == o.chunk && gwaves == o.gwaves &&
+           gbatch == o.gbatch;
   }
 };
 
@@ -223,6 +225,8 @@ struct pgf_handle_s {
   int32_t cp_law = 0;  // elastic law of the crystal-plasticity material (pgf_material.cp_elastic)
   int64_t cp_nq = 0;         // crystal-plasticity QPs: device QPs [0, cp_nq)
   int64_t cp_chunk = 256;    // QPs per wave range of the local-Newton work queue (autotuned)
+  int32_t gather_waves = 4;  // nodes per gather workgroup (autotuned; capped by LDS)
+  int32_t gather_batch = 4;  // staging blocks each gather half-wave keeps in flight (autotuned)
   int32_t *d_qgrain_all = nullptr;  // [n_qp] grain of every device QP
   float *d_gquat = nullptr;  // [n_grains][4] orientation quaternions (learned warm start features)
   // learned warm start
@@ -594,7 +598,7 @@ int32_t enqueue_pass(pgf_handle h, const pgf_nodal_input *in, int mode, double *
     g.vals = vout;
     g.n_nodes = h->n_nodes;
     g.maxlen = h->maxlen;
-    pgf::launch_gather(mode, g, h->stream);
+    pgf::launch_gather(mode, g, h->gather_waves, h->gather_batch, h->stream);
     ++launches;
   }
   return launches;
@@ -621,7 +625,8 @@ int run_pass(pgf_handle h, const pgf_nodal_input *in, int mode, double *f, doubl
     if (rc) return rc;
   }
   const bool graph = h->use_graph && !h->sur;  // the learned start may size buffers: eager
-  GraphKey key{mode, in->r, in->d_r, in->defl, in->defl_d, f, vals, h->warm, h->dt, h->cp_chunk};
+  GraphKey key{mode, in->r, in->d_r, in->defl, in->defl_d, f, vals, h->warm, h->dt, h->cp_chunk, h->gather_waves,
+               h->gather_batch};
   if (graph && !(h->gexec && key == h->gkey)) {
     if (h->gexec) {
       hipGraphExecDestroy(h->gexec);
@@ -1795,42 +1800,99 @@ int pgf_set_assembly(pgf_handle h, int32_t mode) {
 int pgf_autotune(pgf_handle h, const pgf_nodal_input *in, double dt, int64_t *chunk_out) {
   if (!h || !in || !in->r) return PGF_EINVAL;
   if (dt > 0.0) h->dt = dt;
-  if (!h->cp_nq) {
+  const bool tune_cp = h->cp_nq > 0;
+  const bool tune_gather = h->assembly == PGF_ASSEMBLY_GATHER && h->stage_size > 0;
+  if (!tune_cp && !tune_gather) {
     if (chunk_out) *chunk_out = h->cp_chunk;
     return PGF_OK;
   }
-  double *f = nullptr;
+  double *f = nullptr, *vals = nullptr;
   int rc = pgf::dalloc(&f, size_t(h->ndof));
-  if (rc) return rc;
-  const bool prof = h->profiling, graph = h->use_graph;
-  if ((rc = pgf_set_profiling(h, 1))) {  // creates the timing events on first use
+  if (!rc && tune_gather) rc = pgf::dalloc(&vals, size_t(h->nnz));
+  if (rc) {
     hipFree(f);
     return rc;
   }
+  const bool prof = h->profiling, graph = h->use_graph;
+  if ((rc = pgf_set_profiling(h, 1))) {  // creates the timing events on first use
+    hipFree(f);
+    hipFree(vals);
+    return rc;
+  }
   h->use_graph = false;
-  const int64_t cand[] = {64, 128, 256, 512, 1024, 2048};
-  int64_t best = h->cp_chunk;
-  double best_ms = 1e300;
-  for (int64_t c : cand) {
-    h->cp_chunk = c;
-    double ms = 1e300;
+  // the best of three timed passes (pgf_last_pass_ms) of one configuration
+  auto time_pass = [&](int mode, double *vv, double &ms) -> int {
+    ms = 1e300;
     for (int rep = 0; rep < 3; ++rep) {
-      rc = run_pass(h, in, pgf::MODE_INTEGRATE | pgf::MODE_RESIDUAL, f, nullptr);
-      if (rc < 0) break;
-      ms = std::min(ms, h->last_ms);
+      const int r = run_pass(h, in, mode, f, vv);
+      if (r < 0) return r;
+      ms = std::min(ms, double(h->last_ms));
     }
-    if (rc < 0) break;
-    if (ms < best_ms) {
-      best_ms = ms;
-      best = c;
+    return PGF_OK;
+  };
+  // 1. points per wavefront of the crystal-plasticity work queue, on integrating passes
+  if (tune_cp) {
+    const int64_t cand[] = {64, 128, 256, 512, 1024, 2048};
+    int64_t best = h->cp_chunk;
+    double best_ms = 1e300;
+    for (int64_t c : cand) {
+      h->cp_chunk = c;
+      double ms;
+      if ((rc = time_pass(pgf::MODE_INTEGRATE | pgf::MODE_RESIDUAL, nullptr, ms)) < 0) break;
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = c;
+      }
     }
+    h->cp_chunk = best;
+  }
+  // 2. the gather's tile: nodes per workgroup x staging blocks in flight, on tangent passes
+  //    (every configuration gives the same sums in the same order)
+  if (rc >= 0 && tune_gather) {
+    const int32_t waves[] = {1, 2, 4}, batches[] = {2, 4, 8};
+    int32_t bw = h->gather_waves, bb = h->gather_batch;
+    double best_ms = 1e300;
+    for (int32_t w : waves)
+      for (int32_t b : batches) {
+        if (w > pgf::gather_waves(h->maxlen)) continue;
+        h->gather_waves = w;
+        h->gather_batch = b;
+        double ms;
+        if ((rc = time_pass(pgf::MODE_RESIDUAL | pgf::MODE_TANGENT, vals, ms)) < 0) break;
+        if (ms < best_ms) {
+          best_ms = ms;
+          bw = w;
+          bb = b;
+        }
+      }
+    h->gather_waves = bw;
+    h->gather_batch = bb;
   }
   hipFree(f);
-  h->cp_chunk = best;
+  hipFree(vals);
   h->profiling = prof;
   h->use_graph = graph;
-  if (chunk_out) *chunk_out = best;
+  if (chunk_out) *chunk_out = h->cp_chunk;
   return rc < 0 ? rc : PGF_OK;
+}
+
+int pgf_get_tuning(pgf_handle h, int64_t *cp_chunk, int32_t *gather_waves, int32_t *gather_batch) {
+  if (!h) return PGF_EINVAL;
+  if (cp_chunk) *cp_chunk = h->cp_chunk;
+  if (gather_waves) *gather_waves = std::min(h->gather_waves, std::max(pgf::gather_waves(h->maxlen), 1));
+  if (gather_batch) *gather_batch = h->gather_batch;
+  return PGF_OK;
+}
+
+int pgf_set_tuning(pgf_handle h, int64_t cp_chunk, int32_t gather_waves, int32_t gather_batch) {
+  if (!h) return PGF_EINVAL;
+  if (cp_chunk > 0 && cp_chunk < 64) return PGF_EINVAL;
+  if (gather_waves > 0 && !(gather_waves == 1 || gather_waves == 2 || gather_waves == 4)) return PGF_EINVAL;
+  if (gather_batch > 0 && !(gather_batch == 2 || gather_batch == 4 || gather_batch == 8)) return PGF_EINVAL;
+  if (cp_chunk > 0) h->cp_chunk = cp_chunk;
+  if (gather_waves > 0) h->gather_waves = gather_waves;
+  if (gather_batch > 0) h->gather_batch = gather_batch;
+  return PGF_OK;
 }
 
 }  // extern "C"

@@ -259,7 +259,9 @@ struct GatherArgs {
 };
 // Wavefronts per workgroup for a given maxlen (0: rows too long for LDS).
 int gather_waves(int32_t maxlen);
-void launch_gather(int mode, const GatherArgs &g, hipStream_t s);
+// waves: nodes per workgroup (capped by the LDS each node needs); batch: staging blocks each
+// half-wave keeps in flight (2, 4 or 8)
+void launch_gather(int mode, const GatherArgs &g, int waves, int batch, hipStream_t s);
 // Jacobi-preconditioned BiCGStab on a CSR matrix; work holds 9 n + 1025 doubles.
 int bicgstab(hipStream_t s, const int32_t *Ap, const int32_t *Ai, const double *vals, const double *b, double *x,
              int64_t n, double tol, int maxit, int *iters, double *relres, double *work);

@@ -854,13 +854,12 @@ __global__ __launch_bounds__(256, STG ? (VAR != 0 ? 3 : 4) : 1) void elem_kernel
 // the node's staging blocks, then stored once.  Each half-wave takes one block at a time
 // (blocks k, k+1 side by side); lane c < 3nne of a half owns element column c -- it decodes
 // the column's position once and loads the column's three rows -- and lane 3nne the three
-// residual entries.  Blocks are loaded kGatherBatch pairs ahead; the adds then follow in
+// residual entries.  Blocks are loaded NB pairs ahead; the adds then follow in
 // block order (the lower half's block before the upper half's, with a wave barrier between),
 // and within one block every target is distinct: the sums run in the same order on every
-// call.  Elements have at most 10 nodes (3nne + 1 <= 32 lanes).
-constexpr int kGatherBatch = 4;
-
-template <bool RES, bool TAN>
+// call.  Elements have at most 10 nodes (3nne + 1 <= 32 lanes).  NB and the nodes per
+// workgroup are launch choices (autotuned, pgf_autotune).
+template <bool RES, bool TAN, int NB>
 __global__ __launch_bounds__(256) void gather_kernel(GatherArgs g) {
   extern __shared__ double s_acc[];
   const int wpb = blockDim.x / 64;
@@ -885,11 +884,11 @@ __global__ __launch_bounds__(256) void gather_kernel(GatherArgs g) {
   for (int b = 0; b < 3; ++b) rowoff[b] = __popc(fm & ((1 << b) - 1)) * len;
   const int half = lane >> 5, c = lane & 31;
   const int64_t kb = g.inc_ptr[node], ke = g.inc_ptr[node + 1];
-  for (int64_t k0 = kb; k0 < ke; k0 += 2 * kGatherBatch) {
-    double v[kGatherBatch][3];
-    int dst[kGatherBatch];  // column in the rows, -2 = residual entries, -1 = nothing
+  for (int64_t k0 = kb; k0 < ke; k0 += 2 * NB) {
+    double v[NB][3];
+    int dst[NB];  // column in the rows, -2 = residual entries, -1 = nothing
 #pragma unroll
-    for (int j = 0; j < kGatherBatch; ++j) {
+    for (int j = 0; j < NB; ++j) {
       dst[j] = -1;
       const int64_t k = k0 + 2 * j + half;
       if (k >= ke) continue;
@@ -912,7 +911,7 @@ __global__ __launch_bounds__(256) void gather_kernel(GatherArgs g) {
     }
     // the three rows of one block are distinct targets: read the three sums, then write them
 #pragma unroll
-    for (int j = 0; j < kGatherBatch; ++j)
+    for (int j = 0; j < NB; ++j)
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
         if (half == h && dst[j] != -1) {
@@ -1135,17 +1134,25 @@ int gather_waves(int32_t maxlen) {
   return int(std::min<int64_t>(4, w));
 }
 
-void launch_gather(int mode, const GatherArgs &g, hipStream_t s) {
-  const int wpb = gather_waves(g.maxlen);
+template <int NB>
+void launch_gather_nb(bool res, bool tan, const GatherArgs &g, unsigned grid, int threads, size_t lds,
+                      hipStream_t s) {
+  if (res && tan) gather_kernel<true, true, NB><<<grid, threads, lds, s>>>(g);
+  else if (res) gather_kernel<true, false, NB><<<grid, threads, lds, s>>>(g);
+  else if (tan) gather_kernel<false, true, NB><<<grid, threads, lds, s>>>(g);
+}
+
+void launch_gather(int mode, const GatherArgs &g, int waves, int batch, hipStream_t s) {
+  const int wpb = std::min(gather_waves(g.maxlen), std::max(waves, 1));
   if (g.n_nodes <= 0 || wpb < 1) return;
   const unsigned grid = unsigned((g.n_nodes + wpb - 1) / wpb);
   const size_t lds = size_t(wpb) * (3 * size_t(std::max<int32_t>(g.maxlen, 1)) + 3) * sizeof(double);
   // the prescribed-load pass stages its vector in the residual entries and no K rows
   const bool dfl = (mode & MODE_DEFL_LOAD) != 0;
   const bool res = (mode & MODE_RESIDUAL) != 0 || dfl, tan = (mode & MODE_TANGENT) != 0 && !dfl;
-  if (res && tan) gather_kernel<true, true><<<grid, 64 * wpb, lds, s>>>(g);
-  else if (res) gather_kernel<true, false><<<grid, 64 * wpb, lds, s>>>(g);
-  else if (tan) gather_kernel<false, true><<<grid, 64 * wpb, lds, s>>>(g);
+  if (batch <= 2) launch_gather_nb<2>(res, tan, g, grid, 64 * wpb, lds, s);
+  else if (batch <= 4) launch_gather_nb<4>(res, tan, g, grid, 64 * wpb, lds, s);
+  else launch_gather_nb<8>(res, tan, g, grid, 64 * wpb, lds, s);
 }
 
 void launch_tf_update(int nne, const TfArgs &t, hipStream_t s) {

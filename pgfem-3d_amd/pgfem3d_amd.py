@@ -132,6 +132,8 @@ def lib() -> C.CDLL:
             "pgf_set_graph": [P, C.c_int],
         "pgf_set_assembly": [P, C.c_int32],
             "pgf_autotune": [P, P, D, P],
+            "pgf_get_tuning": [P, P, P, P],
+            "pgf_set_tuning": [P, I64, I32, I32],
             "pgf_set_surrogate": [P, P, P, P, P],
             "pgf_set_three_field": [P, I32],
             "pgf_tf_initial": [P, P, P],
@@ -162,6 +164,7 @@ def exported_symbols():
         "pgf_set_three_field", "pgf_tf_initial", "pgf_tf_update", "pgf_tf_accept", "pgf_tf_get", "pgf_tf_set",
         "pgf_autotune", "pgf_residual_tangent_static", "pgf_stage_doubles",
         "pgf_subdivision_param", "pgf_cp_start_stats", "pgf_output_variables", "pgf_mlp_infer",
+        "pgf_get_tuning", "pgf_set_tuning",
     ]
 
 
@@ -364,6 +367,17 @@ class Handle:
         c = C.c_int64()
         _check(self._lib.pgf_autotune(self.h, C.byref(inp), dt, C.byref(c)), "pgf_autotune")
         return c.value
+
+    def tuning(self) -> dict:
+        """The launch configuration in use: CP points per wavefront, gather nodes per workgroup
+        and gather blocks in flight."""
+        c, w, b = C.c_int64(), C.c_int32(), C.c_int32()
+        _check(self._lib.pgf_get_tuning(self.h, C.byref(c), C.byref(w), C.byref(b)), "pgf_get_tuning")
+        return {"cp_chunk": c.value, "gather_waves": w.value, "gather_batch": b.value}
+
+    def set_tuning(self, cp_chunk=0, gather_waves=0, gather_batch=0):
+        _check(self._lib.pgf_set_tuning(self.h, int(cp_chunk), int(gather_waves), int(gather_batch)),
+               "pgf_set_tuning")
 
     def set_stream(self, stream_ptr: int | None):
         _check(self._lib.pgf_set_stream(self.h, stream_ptr), "pgf_set_stream")
