@@ -338,8 +338,9 @@ int pgf_reset_step(pgf_handle h);
 
 /* ============================================================ three-field (CM3F) */
 /* The CM3F analysis (-cm3f: analysis_type CM3F, include/enumerations.h:16) for meshes whose
- * materials are hyperelastic (neo-Hookean deviatoric law) or crystal plasticity with the
- * hyperelastic elastic law (cp_elastic = 1): one volume variable theta and one pressure P per element
+ * materials are hyperelastic (neo-Hookean deviatoric law), J2 plasticity without damage, or
+ * crystal plasticity with the hyperelastic elastic law (cp_elastic = 1): one volume variable
+ * theta and one pressure P per element
  * (get_3f_pressure_volume_number, src/three_field_element.cc:1399-1416), condensed out of
  * the element residual and tangent (condense_K_3F_to_1F / condense_F_3F_to_1F,
  * src/condense.cc:304-385).  With the mode on, pgf_residual / pgf_tangent /
@@ -354,8 +355,11 @@ int pgf_reset_step(pgf_handle h);
  * element takes M = pF(n+1)^-1 and the implicit dM/du into the CM_ThreeField blocks
  * (src/constitutive_model_3f.cc:189-470; residual passes also run the tangent kernel, since
  * the condensed residual needs Kut).
- * on = 0 turns the mode off and frees its memory.  PGF_ENOTSUP for J2 plasticity, the linear
- * deviatoric law, and crystal plasticity with the cubic law. */
+ * J2 (CM_J2P_PARAM: pF = I, so M = I and dM/du = 0) takes the deviatoric part of its return
+ * map's stress and tangent (update_elasticity_dev, src/cm_j2_plasticity.cc:384-461) with the
+ * element pressure.
+ * on = 0 turns the mode off and frees its memory.  PGF_ENOTSUP for J2 with continuum damage,
+ * the linear deviatoric law, and crystal plasticity with the cubic law. */
 int pgf_set_three_field(pgf_handle h, int32_t on);
 /* compute_cm_initial_conditions (src/constitutive_model.cc:4035-4160) for the three-field
  * analysis, with compute_3f_initial_conditions (src/three_field_element.cc:1328-1386): theta

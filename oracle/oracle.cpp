@@ -987,8 +987,16 @@ void tf_dof_grad(const double *dNdX, int k, double *G) {  // Grad_du = ST_(a,b):
   for (int J = 0; J < 3; ++J) G[b * 3 + J] = dNdX[a * 3 + J];
 }
 
+// J2: the stored state enters through jst (per QP Cp^-1 at n [nint][6] and ep at n [nint]; the
+// n+1 values are written to Cp1 / ep1 / gam1 when given).
+struct TfJ2State {
+  const double *Cpn, *epn;
+  double *Cp1, *ep1, *gam1;
+};
+
 int tf_element(const orc_material &mat, int nne, const double *x, const double *u, double theta, double P,
-               bool want_K, TfElem &E, double *Fbar_out /* [nint][9] or null */) {
+               bool want_K, TfElem &E, double *Fbar_out /* [nint][9] or null */,
+               const TfJ2State *jst = nullptr) {
   const int nd = nne * 3;
   std::memset(&E, 0, sizeof(E));
   const double kappa = kappa_of(mat);
@@ -1014,8 +1022,38 @@ int tf_element(const orc_material &mat, int nne, const double *x, const double *
     }
     double C[9], S[9], L[81];
     matmul_tn(Fr, Fr, C);
-    dev_stress_mr(C, mat.m10, mat.m01, S);
-    if (want_K) mat_stiff_mr(C, mat.m10, mat.m01, L);
+    if (mat.model == 4) {
+      // J2 (CM_J2P_PARAM: pF = I, so M = I and dM/du = 0 as for the hyperelastic model): the
+      // return map at Fr (its stored state does not change with the volumetric scaling of Fb,
+      // so Fr and Fb integrate alike), and S, L less their volumetric parts kappa U(J) --
+      // the deviatoric stress and tangent of update_elasticity_dev
+      // (src/cm_j2_plasticity.cc:384-461)
+      const double c6[6] = {1, 1, 1, 0, 0, 0};
+      const double *Cn = (jst && jst->Cpn) ? jst->Cpn + ip * 6 : c6;
+      const double epn = (jst && jst->epn) ? jst->epn[ip] : 0.0;
+      double Cp1[6], ep1, gam, Lf[81], CI[9];
+      if (orc_j2_update(&mat, Fr, Cn, epn, Cp1, &ep1, &gam, S, Lf)) { ++err; continue; }
+      if (jst && jst->Cp1)
+        for (int i = 0; i < 6; ++i) jst->Cp1[ip * 6 + i] = Cp1[i];
+      if (jst && jst->ep1) jst->ep1[ip] = ep1;
+      if (jst && jst->gam1) jst->gam1[ip] = gam;
+      inv3x3(C, CI);
+      double vdU, vd2U, vU;
+      vol_pot(mat.volPotFlag, tJr, &vdU, &vd2U, &vU);
+      const double kap = kappa_of(mat);
+      for (int i = 0; i < 9; ++i) S[i] -= kap * tJr * vdU * CI[i];
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+          for (int k = 0; k < 3; ++k)
+            for (int l = 0; l < 3; ++l) {
+              const int q = idx4(i, j, k, l);
+              L[q] = Lf[q] - (kap * (tJr * vdU + tJr * tJr * vd2U) * CI[i * 3 + j] * CI[k * 3 + l] -
+                              2.0 * kap * tJr * vdU * CI[i * 3 + k] * CI[l * 3 + j]);
+            }
+    } else {
+      dev_stress_mr(C, mat.m10, mat.m01, S);
+      if (want_K) mat_stiff_mr(C, mat.m10, mat.m01, L);
+    }
     const double *M = I9, *eFn = I9;
     double Psi[30][9], trGF[30];
     for (int k = 0; k < nd; ++k) {
@@ -1111,9 +1149,11 @@ extern "C" {
 // One CM3F element (material form) for tests: the blocks of CM_ThreeField (Kuu row-major
 // [3nne][3nne], Ru, Kup, Kut, Ktu; sc = Rp, Rt, Ktt, Ktp) and the condensed K and f.
 int orc_tf_element(const orc_material *mat, int nne, const double *x, const double *u, double theta, double P,
-                   double *Kuu, double *Ru, double *Kup, double *sc, double *Kc, double *fc) {
+                   double *Kuu, double *Ru, double *Kup, double *sc, double *Kc, double *fc, const double *Cpn,
+                   const double *epn) {
   TfElem E;
-  const int err = tf_element(*mat, nne, x, u, theta, P, Kuu || Kc, E, nullptr);
+  const TfJ2State jst{Cpn, epn, nullptr, nullptr, nullptr};
+  const int err = tf_element(*mat, nne, x, u, theta, P, Kuu || Kc, E, nullptr, &jst);
   const int nd = 3 * nne;
   if (Kuu) std::memcpy(Kuu, E.Kuu, sizeof(double) * nd * nd);
   if (Ru) std::memcpy(Ru, E.Ru, sizeof(double) * nd);
@@ -1131,22 +1171,38 @@ int orc_assemble_tf(int64_t n_elem, const double *X, const int64_t *elem_ptr, co
                     const int32_t *elem_mat, const orc_material *mats, const int64_t *id, int64_t ndof,
                     const int32_t *Ap, const int32_t *Ai, const double *r, const double *d_r, const double *defl,
                     const double *defl_d, const double *theta, const double *P, int run_integration,
-                    double *F_np1, int64_t n_qp, double *f, double *vals) {
+                    double *F_np1, int64_t n_qp, double *f, double *vals, const double *plastic_n,
+                    double *plastic_np1, const double *scalar_n, double *scalar_np1) {
   if (f) std::memset(f, 0, sizeof(double) * ndof);
   if (vals) std::memset(vals, 0, sizeof(double) * Ap[ndof]);
   int err = 0;
   int64_t qp0 = 0;
   for (int64_t e = 0; e < n_elem; ++e) {
-    const int nne = int(elem_ptr[e + 1] - elem_ptr[e]), nd = 3 * nne;
+    const int nne = int(elem_ptr[e + 1] - elem_ptr[e]), nd = 3 * nne, nint = nint_of(nne);
     double x[30], u[30], Fb[72];
     int64_t cn[30];
     tf_gather(X, elem_conn + elem_ptr[e], nne, id, r, d_r, defl, defl_d, x, u, cn);
     TfElem E;
     const orc_material &mat = mats[elem_mat ? elem_mat[e] : 0];
-    err += tf_element(mat, nne, x, u, theta[e], P[e], vals != nullptr, E, Fb);
+    // J2: the element's QP state from the [comp][n_qp] arrays (Cp^-1 6 comps, scalars ep, gam, ...)
+    double Cpn[48], epn[8], Cp1[48], ep1[8], gam1[8];
+    for (int ip = 0; ip < nint; ++ip) {
+      const int64_t q = qp0 + ip;
+      for (int i = 0; i < 6; ++i) Cpn[ip * 6 + i] = plastic_n ? plastic_n[i * n_qp + q] : (i < 3 ? 1.0 : 0.0);
+      epn[ip] = scalar_n ? scalar_n[q] : 0.0;
+    }
+    const TfJ2State jst{Cpn, epn, Cp1, ep1, gam1};
+    err += tf_element(mat, nne, x, u, theta[e], P[e], vals != nullptr, E, Fb, &jst);
     if (run_integration && F_np1)
-      for (int ip = 0; ip < nint_of(nne); ++ip)
+      for (int ip = 0; ip < nint; ++ip)
         for (int i = 0; i < 9; ++i) F_np1[i * n_qp + qp0 + ip] = Fb[ip * 9 + i];
+    if (run_integration && mat.model == 4 && plastic_np1 && scalar_np1)
+      for (int ip = 0; ip < nint; ++ip) {
+        const int64_t q = qp0 + ip;
+        for (int i = 0; i < 6; ++i) plastic_np1[i * n_qp + q] = Cp1[ip * 6 + i];
+        scalar_np1[q] = ep1[ip];
+        scalar_np1[n_qp + q] = gam1[ip];
+      }
     std::vector<double> K(size_t(nd) * nd);
     double fe[30];
     tf_condense(nd, E, K.data(), fe);
@@ -1179,15 +1235,25 @@ int orc_assemble_tf(int64_t n_elem, const double *X, const int64_t *elem_ptr, co
 int orc_tf_update(int64_t n_elem, const double *X, const int64_t *elem_ptr, const int64_t *elem_conn,
                   const int32_t *elem_mat, const orc_material *mats, const int64_t *id, const double *r,
                   const double *d_r, const double *defl, const double *defl_d, const double *theta, const double *P,
-                  const double *du, double *ddV, double *ddP) {
+                  const double *du, double *ddV, double *ddP, int64_t n_qp, const double *plastic_n,
+                  const double *scalar_n) {
   int err = 0;
+  int64_t qp0 = 0;
   for (int64_t e = 0; e < n_elem; ++e) {
-    const int nne = int(elem_ptr[e + 1] - elem_ptr[e]), nd = 3 * nne;
+    const int nne = int(elem_ptr[e + 1] - elem_ptr[e]), nd = 3 * nne, nint = nint_of(nne);
     double x[30], u[30];
     int64_t cn[30];
     tf_gather(X, elem_conn + elem_ptr[e], nne, id, r, d_r, defl, defl_d, x, u, cn);
+    double Cpn[48], epn[8];
+    for (int ip = 0; ip < nint; ++ip) {
+      const int64_t q = qp0 + ip;
+      for (int i = 0; i < 6; ++i) Cpn[ip * 6 + i] = plastic_n ? plastic_n[i * n_qp + q] : (i < 3 ? 1.0 : 0.0);
+      epn[ip] = scalar_n ? scalar_n[q] : 0.0;
+    }
+    qp0 += nint;
+    const TfJ2State jst{Cpn, epn, nullptr, nullptr, nullptr};
     TfElem E;
-    err += tf_element(mats[elem_mat ? elem_mat[e] : 0], nne, x, u, theta[e], P[e], false, E, nullptr);
+    err += tf_element(mats[elem_mat ? elem_mat[e] : 0], nne, x, u, theta[e], P[e], false, E, nullptr, &jst);
     const double Kpt = E.Ktp;
     double Kpu_du = 0.0, Ktu_du = 0.0;
     for (int k = 0; k < nd; ++k) {

@@ -104,17 +104,23 @@ int orc_assemble_coh(const double *X, int64_t n_coel, const int64_t *coel_ptr, c
 
 // --- three-field CM3F formulation of the hyperelastic model (oracle.cpp, end)
 extern "C" {
+// J2 materials (model 4) take their state: Cpn [nint][6] / epn [nint] for one element; for the
+// mesh the [comp][n_qp] arrays of orc_assemble (Cp^-1 6 comps; scalars ep, gam, ...).  Null
+// for the hyperelastic model.
 int orc_tf_element(const orc_material *mat, int nne, const double *x, const double *u, double theta, double P,
-                   double *Kuu, double *Ru, double *Kup, double *sc, double *Kc, double *fc);
+                   double *Kuu, double *Ru, double *Kup, double *sc, double *Kc, double *fc, const double *Cpn,
+                   const double *epn);
 int orc_assemble_tf(int64_t n_elem, const double *X, const int64_t *elem_ptr, const int64_t *elem_conn,
                     const int32_t *elem_mat, const orc_material *mats, const int64_t *id, int64_t ndof,
                     const int32_t *Ap, const int32_t *Ai, const double *r, const double *d_r, const double *defl,
                     const double *defl_d, const double *theta, const double *P, int run_integration,
-                    double *F_np1, int64_t n_qp, double *f, double *vals);
+                    double *F_np1, int64_t n_qp, double *f, double *vals, const double *plastic_n,
+                    double *plastic_np1, const double *scalar_n, double *scalar_np1);
 int orc_tf_update(int64_t n_elem, const double *X, const int64_t *elem_ptr, const int64_t *elem_conn,
                   const int32_t *elem_mat, const orc_material *mats, const int64_t *id, const double *r,
                   const double *d_r, const double *defl, const double *defl_d, const double *theta, const double *P,
-                  const double *du, double *ddV, double *ddP);
+                  const double *du, double *ddV, double *ddP, int64_t n_qp, const double *plastic_n,
+                  const double *scalar_n);
 int orc_tf_initial(int64_t n_elem, const double *X, const int64_t *elem_ptr, const int64_t *elem_conn,
                    const int32_t *elem_mat, const orc_material *mats, const double *u_nodal, double *V,
                    double *Pr);

@@ -423,10 +423,10 @@ def _tf_lib():
     L = lib()
     if not getattr(L, "_tf_ready", False):
         vp, i64 = C.c_void_p, C.c_int64
-        L.orc_tf_element.argtypes = [vp, C.c_int, vp, vp, C.c_double, C.c_double, vp, vp, vp, vp, vp, vp]
+        L.orc_tf_element.argtypes = [vp, C.c_int, vp, vp, C.c_double, C.c_double] + [vp] * 8
         L.orc_assemble_tf.argtypes = [i64, vp, vp, vp, vp, vp, vp, i64, vp, vp, vp, vp, vp, vp, vp, vp, C.c_int, vp, i64,
-                                      vp, vp]
-        L.orc_tf_update.argtypes = [i64] + [vp] * 15
+                                      vp, vp, vp, vp, vp, vp]
+        L.orc_tf_update.argtypes = [i64] + [vp] * 15 + [i64, vp, vp]
         L.orc_tf_initial.argtypes = [i64] + [vp] * 8
         for fn in ("orc_tf_element", "orc_assemble_tf", "orc_tf_update", "orc_tf_initial"):
             getattr(L, fn).restype = C.c_int
@@ -434,15 +434,18 @@ def _tf_lib():
     return L
 
 
-def tf_element(mat, nne, x, u, theta, P):
-    """One CM3F element (material form): its blocks and the condensed K and f."""
+def tf_element(mat, nne, x, u, theta, P, Cpn=None, epn=None):
+    """One CM3F element (material form): its blocks and the condensed K and f (J2: the state at
+    n per QP, Cpn [nint, 6] and epn [nint]; None = the virgin state)."""
     nd = 3 * nne
     out = dict(Kuu=np.zeros((nd, nd)), Ru=np.zeros(nd), Kup=np.zeros(nd), sc=np.zeros(4), Kc=np.zeros((nd, nd)),
                fc=np.zeros(nd))
     om = material(mat)
+    f64 = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float64)
     rc = _tf_lib().orc_tf_element(C.byref(om), nne, _p(np.ascontiguousarray(x, dtype=np.float64)),
                                   _p(np.ascontiguousarray(u, dtype=np.float64)), float(theta), float(P),
-                                  *[_p(out[k]) for k in ("Kuu", "Ru", "Kup", "sc", "Kc", "fc")])
+                                  *[_p(out[k]) for k in ("Kuu", "Ru", "Kup", "sc", "Kc", "fc")], _p(f64(Cpn)),
+                                  _p(f64(epn)))
     out["Rp"], out["Rt"], out["Ktt"], out["Ktp"] = out.pop("sc")
     out["rc"] = rc
     return out
@@ -455,29 +458,36 @@ def _mesh_args(mesh, mats):
 
 
 def assemble_tf(mesh, mats, ids, ndof, Ap, Ai, theta, P, r, d_r=None, defl=None, defl_d=None, integrate=True,
-                F_np1=None, tangent=True):
-    """Condensed CM3F residual and tangent over the mesh. Returns (f, vals, err)."""
+                F_np1=None, tangent=True, state: "OracleState" = None):
+    """Condensed CM3F residual and tangent over the mesh (J2 materials take their state from
+    state, and with integrate its n+1 arrays receive the new state). Returns (f, vals, err)."""
     om, em = _mesh_args(mesh, mats)
     f = np.zeros(ndof)
     vals = np.zeros(Ap[-1]) if tangent else None
-    nq = F_np1.shape[1] if F_np1 is not None else 0
+    nq = F_np1.shape[1] if F_np1 is not None else (state.n_qp if state is not None else 0)
+    st = state
     err = _tf_lib().orc_assemble_tf(mesh.n_elem, _p(mesh.X), _p(mesh.elem_ptr), _p(mesh.elem_conn), _p(em), om,
                                     _p(ids), ndof, _p(Ap), _p(Ai), _p(r), _p(d_r), _p(defl), _p(defl_d),
                                     _p(np.ascontiguousarray(theta, dtype=np.float64)),
                                     _p(np.ascontiguousarray(P, dtype=np.float64)), int(integrate), _p(F_np1), nq,
-                                    _p(f), _p(vals))
+                                    _p(f), _p(vals), _p(None if st is None else st.P_n),
+                                    _p(None if st is None else st.P_np1), _p(None if st is None else st.S_n),
+                                    _p(None if st is None else st.S_np1))
     return f, vals, err
 
 
-def tf_update(mesh, mats, ids, theta, P, du, r, d_r=None, defl=None, defl_d=None):
+def tf_update(mesh, mats, ids, theta, P, du, r, d_r=None, defl=None, defl_d=None, state: "OracleState" = None):
     """Increments (ddV, ddP) of the elements' volume and pressure for the displacement increment du."""
     om, em = _mesh_args(mesh, mats)
     ddV, ddP = np.zeros(mesh.n_elem), np.zeros(mesh.n_elem)
+    st = state
     err = _tf_lib().orc_tf_update(mesh.n_elem, _p(mesh.X), _p(mesh.elem_ptr), _p(mesh.elem_conn), _p(em), om,
                                   _p(ids), _p(r), _p(d_r), _p(defl), _p(defl_d),
                                   _p(np.ascontiguousarray(theta, dtype=np.float64)),
                                   _p(np.ascontiguousarray(P, dtype=np.float64)),
-                                  _p(np.ascontiguousarray(du, dtype=np.float64)), _p(ddV), _p(ddP))
+                                  _p(np.ascontiguousarray(du, dtype=np.float64)), _p(ddV), _p(ddP),
+                                  0 if st is None else st.n_qp, _p(None if st is None else st.P_n),
+                                  _p(None if st is None else st.S_n))
     return ddV, ddP, err
 
 

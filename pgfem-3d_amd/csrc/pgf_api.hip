@@ -1411,12 +1411,16 @@ int pgf_set_three_field(pgf_handle h, int32_t on) {
     h->tf = false;
     return PGF_OK;
   }
-  // the hyperelastic model, and crystal plasticity with its hyperelastic law (the CM3F
-  // element splits the law into deviatoric and volumetric parts; the cubic law has no split)
+  // the hyperelastic model, J2 plasticity (pF = I: its return map's deviatoric stress and
+  // tangent with the element pressure, in the hyperelastic model's kernel), and crystal
+  // plasticity with its hyperelastic law (the CM3F element splits the law into deviatoric and
+  // volumetric parts; the cubic law has no split)
   for (auto &b : h->blocks)
-    if (b.model != PGF_MODEL_HYPER_ELASTICITY && !(b.model == PGF_MODEL_CRYSTAL_PLASTICITY && h->cp_law == 1))
+    if (b.model != PGF_MODEL_HYPER_ELASTICITY && b.model != PGF_MODEL_J2_PLASTICITY &&
+        !(b.model == PGF_MODEL_CRYSTAL_PLASTICITY && h->cp_law == 1))
       return PGF_ENOTSUP;
   if (h->he_linear) return PGF_ENOTSUP;  // the three-field element takes the neo-Hookean law
+  if (h->j2_damage) return PGF_ENOTSUP;  // the three-field kernel carries no damage step
   if (h->tf) return PGF_OK;
   const int64_t NE = h->n_elem;
   int rc = pgf::dalloc(&h->d_tf, size_t(pgf::kTfSlots * NE));

@@ -167,8 +167,12 @@ def test_commit_and_reset(device):
     h.close()
 
 
-def test_three_field_needs_hyperelastic_materials(device):
-    prob = BoxProblem(2, 4, P.j2_linear(200e3, 0.3, 300.0, 1000.0), seed=1)
+@pytest.mark.parametrize("mat", [P.j2_damage(200e3, 0.3, 300.0, 1000.0, mu=2.0, p1=2.0, p2=1.5, Yin=2.0),
+                                 P.Material(model=P.MODEL_HYPER_ELASTICITY, devPotFlag=2, volPotFlag=2, E=200e3,
+                                            nu=0.3, G=80e3)],
+                         ids=["j2-damage", "linear-law"])
+def test_three_field_refuses_laws_without_its_split(device, mat):
+    prob = BoxProblem(2, 4, mat, seed=1)
     h = P.Handle(prob.mesh, prob.ids, prob.ndof, prob.Ap, prob.Ai, prob.mats)
     with pytest.raises(P.PGFError):
         h.set_three_field(True)
