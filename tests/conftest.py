@@ -31,3 +31,37 @@ def rel_inf(a, b):
     den = np.abs(b).max() if b.size else 0.0
     num = np.abs(a - b).max() if b.size else 0.0
     return 0.0 if num == 0.0 else num / max(den, 1e-300)
+
+
+def rel_cols(a, b, floor=1e-6):
+    """Largest error of a [comp][n] array relative to each column's own size: max over columns q
+    of max_c |a[c,q] - b[c,q]| / max(max_c |b[c,q]|, floor * max |b|).  Holds each QP (or
+    element) to its own magnitude rather than the array's largest entry; the floor only keeps
+    columns that are zero to rounding from dividing by zero."""
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    if b.ndim == 1:
+        a, b = a[None], b[None]
+    big = np.abs(b).max() if b.size else 0.0
+    if big == 0.0:
+        return float(np.abs(a).max()) if a.size else 0.0
+    scale = np.maximum(np.abs(b).max(axis=0), floor * big)
+    return float((np.abs(a - b).max(axis=0) / scale).max())
+
+
+def rel_csr_rows(a, b, Ap, floor=1e-6):
+    """rel_cols for CSR values: each row's error relative to that row's largest entry."""
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    big = np.abs(b).max()
+    worst = 0.0
+    err = np.abs(a - b)
+    rows = np.repeat(np.arange(len(Ap) - 1), np.diff(Ap))
+    row_max = np.zeros(len(Ap) - 1)
+    np.maximum.at(row_max, rows, np.abs(b))
+    row_err = np.zeros(len(Ap) - 1)
+    np.maximum.at(row_err, rows, err)
+    scale = np.maximum(row_max, floor * big)
+    if len(scale):
+        worst = float((row_err / scale).max())
+    return worst

@@ -2,7 +2,8 @@
 residual and tangent, coloured assembly) against the CPU oracle on the same seeded inputs.
 
 Tolerance (FP64): 1e-10 relative in the infinity norm of every output array per step
-(residual, CSR values, F, pF, g, lambda, tau, gamma_dot).  Both sides stop the local Newton
+(residual, CSR values, F, pF, g, lambda, tau, gamma_dot), and 1e-10 relative to each QP's own
+tau and pF and to each CSR row's own largest entry (conftest.rel_cols / rel_csr_rows).  Both sides stop the local Newton
 on a full-step size below tol_M = 1e-12, after which a quadratically converging iteration
 sits at round-off, so the two solutions agree to far below the tolerance.
 """
@@ -12,7 +13,7 @@ import torch
 
 import oracle as O
 import pgfem3d_amd as P
-from conftest import rel_inf
+from conftest import rel_cols, rel_csr_rows, rel_inf
 from problems import CPBoxProblem, fcc_crystal_he
 
 TOL = 1e-10
@@ -40,6 +41,8 @@ def _compare_state(h, st, step):
     if np.abs(st.lam_np1).max() > 0:
         assert np.abs(sc[1] - st.lam_np1).max() <= TOL * max(1.0, np.abs(st.lam_np1).max()), step
     assert rel_inf(h.get_state(P.STATE_TAU), st.tau) < TOL, step
+    assert rel_cols(h.get_state(P.STATE_TAU), st.tau) < TOL, step
+    assert rel_cols(h.get_state(P.STATE_PLASTIC_NP1), st.pF_np1) < TOL, step
     gd = h.get_state(P.STATE_GDOT)
     assert np.abs(gd - st.gdot).max() <= TOL * max(np.abs(st.gdot).max(), 1e-300), step
 
@@ -58,6 +61,7 @@ def _run(prob, device, stretches, dt=1.0, noise=2e-5, check_modes=True):
         assert err == erro == 0, (step, err, erro)
         assert rel_inf(f.cpu().numpy(), fo) < TOL, (step, rel_inf(f.cpu().numpy(), fo))
         assert rel_inf(vals.cpu().numpy(), vo) < TOL, (step, rel_inf(vals.cpu().numpy(), vo))
+        assert rel_csr_rows(vals.cpu().numpy(), vo, prob.Ap) < TOL, step
         _compare_state(h, st, step)
         its = h.get_state(P.STATE_ITERS)[0]
         # same algorithm on both sides: iteration counts agree except where a convergence

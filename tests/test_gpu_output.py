@@ -2,16 +2,16 @@
 constitutive_model_update_output_variables (orc_output_qp / output_variables, pinned in
 test_oracle_output.py): per QP the PK2 stress and the Green-Lagrange strain, per element the
 volume averages of the Cauchy stress and of the Almansi strain, within 1e-10 relative of each
-array's largest entry -- for the hyperelastic model, J2 in the plastic range, J2 with damage
-and crystal plasticity with both elastic laws, at n+1 after an integration and at n after the
-commit."""
+array's largest entry and of each QP's (element's) own largest entry -- for the hyperelastic
+model, J2 in the plastic range, J2 with damage and crystal plasticity with both elastic laws,
+at n+1 after an integration and at n after the commit."""
 import numpy as np
 import pytest
 import torch
 
 import oracle as O
 import pgfem3d_amd as P
-from conftest import rel_inf
+from conftest import rel_cols, rel_inf
 from problems import BoxProblem, CPBoxProblem, fcc_crystal_he
 
 pytestmark = pytest.mark.gpu
@@ -27,6 +27,7 @@ def _check(h, ref, level):
     for name, a, b in zip(("S", "E", "sigma", "eps"), got, ref):
         assert np.abs(b).max() > 0, name
         assert rel_inf(a, b) < TOL, (name, level, rel_inf(a, b))
+        assert rel_cols(np.asarray(a).T, np.asarray(b).T) < TOL, (name, level)  # each QP / element
 
 
 @pytest.mark.parametrize("mat,etype,n", [
