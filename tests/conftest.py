@@ -53,15 +53,20 @@ def rel_csr_rows(a, b, Ap, floor=1e-6):
     """rel_cols for CSR values: each row's error relative to that row's largest entry."""
     a = np.asarray(a, dtype=np.float64)
     b = np.asarray(b, dtype=np.float64)
+    Ap = np.asarray(Ap, dtype=np.int64)
+    full = np.diff(Ap) > 0  # empty rows hold nothing to compare
+    if not full.any():
+        return 0.0
+    rows = np.flatnonzero(full)
     big = np.abs(b).max()
     worst = 0.0
-    err = np.abs(a - b)
-    rows = np.repeat(np.arange(len(Ap) - 1), np.diff(Ap))
-    row_max = np.zeros(len(Ap) - 1)
-    np.maximum.at(row_max, rows, np.abs(b))
-    row_err = np.zeros(len(Ap) - 1)
-    np.maximum.at(row_err, rows, err)
-    scale = np.maximum(row_max, floor * big)
-    if len(scale):
-        worst = float((row_err / scale).max())
+    step = max(1, len(rows) * (1 << 24) // max(len(b), 1))  # about 16M entries per chunk
+    for k in range(0, len(rows), step):
+        rs = rows[k:k + step]
+        lo, hi = Ap[rs[0]], Ap[rs[-1] + 1]
+        starts = Ap[rs] - lo
+        row_max = np.maximum.reduceat(np.abs(b[lo:hi]), starts)
+        row_err = np.maximum.reduceat(np.abs(a[lo:hi] - b[lo:hi]), starts)
+        scale = np.maximum(np.maximum(row_max, floor * big), 1e-300)
+        worst = max(worst, float((row_err / scale).max()))
     return worst
