@@ -62,7 +62,8 @@ def flops_per_qp(nne):
 
 def cp_flops_per_qp(mean_iters, nne=4):
     """FP64 operations per QP of the crystal-plasticity pass, counted from the kernels (FMA = 2,
-    divide, pow and cbrt as one each):
+    divide, pow and cbrt as one each; the flow rule's |tau/g|^19 is eight multiplies of binary
+    powering, counted as one, which the ~1e4 totals absorb):
       material-point evaluation (cp_point) ~1.4e3: eF, C_e, S = L:E, Mandel stress, 12 tau_a,
         12 power-law rates, R_M;
       Jacobian ~6.2e3: 12 x (d tau_a/dM 260 + rank-1 update of the 9x9 block 162 + borders

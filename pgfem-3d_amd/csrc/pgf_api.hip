@@ -1062,6 +1062,11 @@ int pgf_create(const pgf_mesh *mesh, const int64_t *id, int64_t ndof, const int3
     c.kappa = cpm.E / (3.0 * (1.0 - 2.0 * cpm.nu));
     c.law = cpm.cp_elastic;
     c.vol_flag = cpm.volPotFlag;
+    {
+      // the flow rule's exponent, evaluated as the kernels do
+      const double e = 1.0 / c.m - 1.0;
+      c.ipow = (e >= 0.0 && e <= 64.0 && e == std::floor(e)) ? int32_t(e) : -1;
+    }
     h->cp_g0 = cpm.g0;
     h->cp_law = cpm.cp_elastic;
     h->cp_mat_host = {cpm.C11, cpm.C12, cpm.C44};

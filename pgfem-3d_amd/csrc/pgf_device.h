@@ -114,7 +114,8 @@ struct DevGrain {
 struct CPMaterial {
   double gd0, m, h0, g0, gs0, gds, w, tol;
   double m10, kappa;
-  int32_t maxit, law, vol_flag, pad;
+  int32_t maxit, law, vol_flag;
+  int32_t ipow;  // 1/m - 1 when it is a whole number in [0, 64] (binary powering), else -1
 };
 
 // Arguments shared by the crystal-plasticity QP kernels (device pointers, SoA [comp][n_qp]).

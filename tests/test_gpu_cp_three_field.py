@@ -116,10 +116,14 @@ def test_prescribed_load(device):
 def test_newton_on_linear_tets_lands_on_the_displacement_solution(device):
     prob = CPBoxProblem(3, P.PGF_TET4, MAT, n_grains=4, seed=9)
     h = _handle(prob)
-    drv = NW.NewtonDriver(h, prob.Ap, prob.Ai, tol=1e-12, max_iter=30, alpha_restart=None)
+    # 1e-11: the assembled residual of these steps rounds at about 1e-12 of the step's first
+    # residual (forces of the accumulated strain against the increment's), where a tighter
+    # tolerance is met or missed by chance in the line search
+    tol = 1e-11
+    drv = NW.NewtonDriver(h, prob.Ap, prob.Ai, tol=tol, max_iter=30, alpha_restart=None)
     h1 = P.Handle(prob.mesh, prob.ids, prob.ndof, prob.Ap, prob.Ai, prob.mats)
     h1.set_orientations(prob.R, prob.grain)
-    d1 = NW.NewtonDriver(h1, prob.Ap, prob.Ai, tol=1e-12, max_iter=30, alpha_restart=None)
+    d1 = NW.NewtonDriver(h1, prob.Ap, prob.Ai, tol=tol, max_iter=30, alpha_restart=None)
     r = torch.zeros(prob.ndof, dtype=torch.float64, device=device)
     r1 = torch.zeros_like(r)
     for s in range(4):
