@@ -263,6 +263,8 @@ __device__ __forceinline__ bool cp_point(const CPMaterial &cm, const DevGrain &g
   v6_to_9(p.S, S9);
   mm3(p.Ce, S9, p.Sig);
   const double expo = 1.0 / cm.m - 1.0;
+  // one reciprocal of g for the twelve systems (the divisions are software sequences)
+  const double rg = 1.0 / g, cdg = cm.gd0 * rg / cm.m;
   p.gsum = 0.0;
 #pragma unroll
   for (int a = 0; a < 12; ++a) {
@@ -270,11 +272,11 @@ __device__ __forceinline__ bool cp_point(const CPMaterial &cm, const DevGrain &g
 #pragma unroll
     for (int i = 0; i < 9; ++i) t += p.Sig[i] * gr.P[a][i];
     p.tau[a] = t;
-    const double r = t / g;
+    const double r = t * rg;
     const double pw = flow_pow(fabs(r), expo, cm.ipow);
     p.gdot[a] = cm.gd0 * r * pw;
-    p.dgdt[a] = cm.gd0 / (cm.m * g) * pw;
-    p.dgdg[a] = -p.dgdt[a] * t / g;
+    p.dgdt[a] = cdg * pw;
+    p.dgdg[a] = -p.dgdt[a] * r;
     p.gsum += fabs(p.gdot[a]);
   }
   if (cm.w == 0.0) {
@@ -363,16 +365,17 @@ __device__ __forceinline__ void cp_jacobian_re(const CPMaterial &cm, const DevGr
   }
   double sg = 0.0;
   const double expo = 1.0 / cm.m - 1.0;
+  const double rg = 1.0 / g, cdg = cm.gd0 * rg / cm.m;
   for (int a = 0; a < 12; ++a) {
     // slip-system quantities recomputed from the Mandel stress (as cp_point evaluates them)
     double t = 0.0;
 #pragma unroll
     for (int i = 0; i < 9; ++i) t += p.Sig[i] * gr.P[a][i];
-    const double r = t / g;
+    const double r = t * rg;
     const double pw = flow_pow(fabs(r), expo, cm.ipow);
     const double gd = cm.gd0 * r * pw;
-    const double dgdt = cm.gd0 / (cm.m * g) * pw;
-    const double dgdg = -dgdt * t / g;
+    const double dgdt = cdg * pw;
+    const double dgdg = -dgdt * r;
     double D[9];
     dtau_dM<HE>(gr, OPQ ? opaque(gr.Lv) : gr.Lv, a, AtE, p, D);
     const double co = dt * dgdt;
@@ -472,12 +475,13 @@ __device__ __forceinline__ bool cp_merit_at(const CPMaterial &cm, const DevGrain
   v6_to_9(S, S9);
   mm3(Ce, S9, Sig);
   const double expo = 1.0 / cm.m - 1.0;
+  const double rg = 1.0 / g;
   double gsum = 0.0, sl[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   for (int a = 0; a < 12; ++a) {
     double t = 0.0;
 #pragma unroll
     for (int i = 0; i < 9; ++i) t += Sig[i] * gr.P[a][i];
-    const double r = t / g;
+    const double r = t * rg;
     const double gd = cm.gd0 * r * flow_pow(fabs(r), expo, cm.ipow);
     gsum += fabs(gd);
 #pragma unroll
@@ -845,11 +849,12 @@ __global__ __launch_bounds__(64) void cp_tangent(CPArgs a, int64_t nqb) {
     return;
   }
   const double expo = 1.0 / cm.m - 1.0;
+  const double rg = 1.0 / x[10], cdg = cm.gd0 * rg / cm.m;
   for (int al = 0; al < 12; ++al) {  // tau and gamma_dot as cp_point evaluates them
     double tt = 0.0;
 #pragma unroll
     for (int i = 0; i < 9; ++i) tt += p.Sig[i] * gr.P[al][i];
-    const double r = tt / x[10];
+    const double r = tt * rg;
     a.tau[al * NQ + qg] = tt;
     a.gdot[al * NQ + qg] = cm.gd0 * r * flow_pow(fabs(r), expo, cm.ipow);
   }
@@ -885,9 +890,9 @@ __global__ __launch_bounds__(64) void cp_tangent(CPArgs a, int64_t nqb) {
       double tt = 0.0;
 #pragma unroll
       for (int i = 0; i < 9; ++i) tt += p.Sig[i] * gr.P[al][i];
-      const double rr = tt / x[10];
+      const double rr = tt * rg;
       const double pw = flow_pow(fabs(rr), expo, cm.ipow);
-      const double dgdt = cm.gd0 / (cm.m * x[10]) * pw;
+      const double dgdt = cdg * pw;
       const double co = -a.dt * dgdt;
       const double sg = ((cm.gd0 * rr * pw < 0) ? -1.0 : 1.0) * dgdt;
 #pragma unroll
