@@ -23,6 +23,7 @@
 //
 // Kernels:
 //   cp_kinematics  F(n+1) at every QP of a block (gather of the element nodes)
+//   cp_start_kernel  the starting points (and the choice of a caller's start), one per lane
 //   cp_newton      persistent: each wave owns a QP range; a lane that converges takes the
 //                  next QP of its wave at once, so uneven iteration counts do not idle lanes
 //   cp_tangent     at the converged state: S, tau, gdot and dMe/dF (Me = pF(n+1)^-1), for the
@@ -638,8 +639,56 @@ __global__ __launch_bounds__(256) void cp_kinematics(CPArgs a) {
   if (!okF) atomicAdd(a.err, 1);
 }
 
+// ------------------------------------------------------------------- start kernel
+// The local Newton's starting points, one thread per QP, ahead of the Newton kernel: the plain
+// start (the plastic predictor pulled back to yield, about ten stress evaluations) and, with a
+// caller's start attached, which of the two has the lower residual merit.  In the Newton
+// kernel a lane takes a new QP whenever its last one converges, so work done at a refill runs
+// with the other lanes of the wave masked off; here it runs on every lane.  qpbuf row q holds
+// the plain start x(0..10) and the choice kStartFlag (bit 0: the plain start failed, bit 1:
+// take the caller's start) until the tangent kernel overwrites the row.
+constexpr int kStartFlag = 11;
+template <bool HE>
+__global__ __launch_bounds__(64) void cp_start_kernel(CPArgs a) {
+  const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (q >= a.nq) return;
+  const int64_t NQ = a.n_qp;
+  const int64_t qg = a.qp_base + q;
+  const CPMaterial cm = a.cpmats[0];
+  const DevGrain &gr = a.grains[a.qgrain[q]];
+  double F[9], pFn[9], pFi[9], Fn[9], A[9], x[11];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    F[i] = a.F_np1[i * NQ + qg];
+    pFn[i] = a.pF_n[i * NQ + qg];
+    Fn[i] = a.F_n[i * NQ + qg];
+  }
+  inv3(pFn, pFi);
+  mm3(F, pFi, A);
+  const double g_n = a.g_n[qg];
+  const bool bad = !cp_start<HE>(cm, gr, A, F, Fn, pFn, pFi, g_n, a.lam_n[qg], x);
+  int flag = bad ? 1 : 0;
+  if (a.warm) {
+    // a caller's start is taken when its residual merit is below the plain start's
+    const double det_pFn = det3(pFn);
+    const double gscale = 1.0 / (fabs(cm.gs0 - cm.g0) * g_n + 1e-300);
+    double xw[11];
+#pragma unroll
+    for (int i = 0; i < 11; ++i) xw[i] = a.warm[i * NQ + qg];
+    double mw = 0.0, mp = 0.0;
+    const bool okw = cp_merit_at<HE>(cm, gr, A, det_pFn, g_n, a.dt, xw, gscale, mw);
+    const bool okp = !bad && cp_merit_at<HE>(cm, gr, A, det_pFn, g_n, a.dt, x, gscale, mp);
+    if (okw && (!okp || mw < mp)) flag |= 2;
+  }
+  double *row = a.qpbuf + q * kCPBuf;
+#pragma unroll
+  for (int i = 0; i < 11; ++i) row[i] = x[i];
+  row[kStartFlag] = double(flag);
+}
+
 // ------------------------------------------------------------------- Newton kernel
-// One wave per workgroup; wave w owns QPs [w*chunk, (w+1)*chunk) of the block.
+// One wave per workgroup; wave w owns QPs [w*chunk, (w+1)*chunk) of the block.  The starting
+// points come from cp_start_kernel (qpbuf).
 template <bool HE>
 __global__ __launch_bounds__(64) void cp_newton(CPArgs a, int64_t chunk) {
   const int lane = threadIdx.x;
@@ -682,29 +731,19 @@ __global__ __launch_bounds__(64) void cp_newton(CPArgs a, int64_t chunk) {
         gscale = 1.0 / (fabs(cm.gs0 - cm.g0) * g_n + 1e-300);
         it = 0;
         it_att = 0;
-        // start: the caller's warm start, else the plain start
-        bad_start = false;
+        // start: the caller's warm start where cp_start_kernel chose it, else the plain start
+        const double *row = a.qpbuf + q * kCPBuf;
+        const int flag = int(row[kStartFlag]);
+        bad_start = (flag & 1) != 0;
         kind = 0;
-        {
-          double Fn[9];
+        if (flag & 2) {
 #pragma unroll
-          for (int i = 0; i < 9; ++i) Fn[i] = a.F_n[i * NQ + qg];
-          bad_start = !cp_start<HE>(cm, a.grains[gid], A, F, Fn, pFn, pFi, g_n, a.lam_n[qg], x);
-        }
-        if (a.warm) {
-          // a caller's start is taken when its residual merit is below the plain start's
-          double xw[11];
+          for (int i = 0; i < 11; ++i) x[i] = a.warm[i * NQ + qg];
+          kind = 1;
+          bad_start = false;
+        } else {
 #pragma unroll
-          for (int i = 0; i < 11; ++i) xw[i] = a.warm[i * NQ + qg];
-          double mw = 0.0, mp = 0.0;
-          const bool okw = cp_merit_at<HE>(cm, a.grains[gid], A, det_pFn, g_n, dt, xw, gscale, mw);
-          const bool okp = !bad_start && cp_merit_at<HE>(cm, a.grains[gid], A, det_pFn, g_n, dt, x, gscale, mp);
-          if (okw && (!okp || mw < mp)) {
-#pragma unroll
-            for (int i = 0; i < 11; ++i) x[i] = xw[i];
-            kind = 1;
-            bad_start = false;
-          }
+          for (int i = 0; i < 11; ++i) x[i] = row[i];
         }
       }
     }
@@ -783,15 +822,10 @@ __global__ __launch_bounds__(64) void cp_newton(CPArgs a, int64_t chunk) {
       // (iterations keep counting)
       kind = 2;
       it_att = 0;
-      double F[9], pFn[9], pFi[9], Fn[9];
+      const double *row = a.qpbuf + q * kCPBuf;
 #pragma unroll
-      for (int i = 0; i < 9; ++i) {
-        F[i] = a.F_np1[i * NQ + qg];
-        pFn[i] = a.pF_n[i * NQ + qg];
-        Fn[i] = a.F_n[i * NQ + qg];
-      }
-      inv3(pFn, pFi);
-      bad_start = !cp_start<HE>(cm, gr, A, F, Fn, pFn, pFi, g_n, a.lam_n[qg], x);
+      for (int i = 0; i < 11; ++i) x[i] = row[i];
+      bad_start = (int(row[kStartFlag]) & 1) != 0;
       fail = false;
     }
     if (conv || fail) {
@@ -2019,8 +2053,14 @@ void launch_cp_newton(const CPArgs &a, int64_t chunk, hipStream_t s) {
   if (nq <= 0) return;
   if (chunk < 64) chunk = 64;
   const unsigned grid = unsigned((nq + chunk - 1) / chunk);
-  if (a.law == 1) cp_newton<true><<<grid, 64, 0, s>>>(a, chunk);
-  else cp_newton<false><<<grid, 64, 0, s>>>(a, chunk);
+  const unsigned sgrid = unsigned((nq + 63) / 64);
+  if (a.law == 1) {
+    cp_start_kernel<true><<<sgrid, 64, 0, s>>>(a);
+    cp_newton<true><<<grid, 64, 0, s>>>(a, chunk);
+  } else {
+    cp_start_kernel<false><<<sgrid, 64, 0, s>>>(a);
+    cp_newton<false><<<grid, 64, 0, s>>>(a, chunk);
+  }
 }
 
 void launch_cp_tangent(const CPArgs &a, bool with_D, hipStream_t s) {
