@@ -342,6 +342,15 @@ __device__ __forceinline__ void dtau_dM(const DevGrain &gr, const double *Lv, in
   for (int i = 0; i < 9; ++i) D[i] = 2.0 * T[i];
 }
 
+// d tau_a / d M from a Q_a formed earlier: 2 A^T eF sym(Q_a)
+__device__ __forceinline__ void dtau_dM_q(const double *AtE, const double *Q6, double *D) {
+  double sQ[9], T[9];
+  v6_to_9(Q6, sQ);
+  mm3(AtE, sQ, T);
+#pragma unroll
+  for (int i = 0; i < 9; ++i) D[i] = 2.0 * T[i];
+}
+
 // Bordered Jacobian [Ah b c; d^T 0 0; e^T 0 f] at p.
 struct Border {
   double b[9], c[9], d[9], e[9], f, dRg;
@@ -349,11 +358,12 @@ struct Border {
 
 // Bordered Jacobian of the local system at p; the per-system slip quantities are recomputed
 // from the Mandel stress with cp_point's arithmetic.  OPQ: read the grain stiffness per slip
-// system rather than hoisting it (the tangent kernel, where the hoisted copy spills).
-template <bool HE, bool OPQ>
+// system rather than hoisting it (the tangent kernel, where the hoisted copy spills).  QPRE:
+// the Q_a of qsym were formed earlier and sit in LDS at qpre[(a * 6 + I) * 64].
+template <bool HE, bool OPQ, bool QPRE = false>
 __device__ __forceinline__ void cp_jacobian_re(const CPMaterial &cm, const DevGrain &gr, double dt, const double *x,
                                             const double *A, const CPPoint &p, double g_n,
-                                            double (&Ah)[81], Border &B) {
+                                            double (&Ah)[81], Border &B, const double *qpre = nullptr) {
   const double lam = x[9], g = x[10];
   double AtE[9];
   mtm3(A, p.eF, AtE);
@@ -378,7 +388,14 @@ __device__ __forceinline__ void cp_jacobian_re(const CPMaterial &cm, const DevGr
     const double dgdt = cdg * pw;
     const double dgdg = -dgdt * r;
     double D[9];
-    dtau_dM<HE>(gr, OPQ ? opaque(gr.Lv) : gr.Lv, a, AtE, p, D);
+    if constexpr (QPRE) {
+      double Q6[6];
+#pragma unroll
+      for (int I = 0; I < 6; ++I) Q6[I] = qpre[(a * 6 + I) * 64];
+      dtau_dM_q(AtE, Q6, D);
+    } else {
+      dtau_dM<HE>(gr, OPQ ? opaque(gr.Lv) : gr.Lv, a, AtE, p, D);
+    }
     const double co = dt * dgdt;
     const double sgn = (gd < 0) ? -1.0 : 1.0;
 #pragma unroll
@@ -854,6 +871,9 @@ __global__ __launch_bounds__(64) void cp_newton(CPArgs a, int64_t chunk) {
 // with WITH_D, dMe/dF = pF(n)^-1 dM/dF from the implicit derivative of the local system.
 template <bool HE, bool WITH_D>
 __global__ __launch_bounds__(64) void cp_tangent(CPArgs a, int64_t nqb) {
+  // the Q_a of the twelve systems, formed once for the right-hand sides and read back by the
+  // Jacobian (lane-fastest: conflict-free)
+  __shared__ double sQ[WITH_D ? 72 * 64 : 1];
   const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (t >= nqb) return;
   const int64_t NQ = a.n_qp;
@@ -921,6 +941,8 @@ __global__ __launch_bounds__(64) void cp_tangent(CPArgs a, int64_t nqb) {
     for (int al = 0; al < 12; ++al) {
       double Q[6];
       qsym<HE>(gr, opaque(gr.Lv), al, p, Q);
+#pragma unroll
+      for (int I = 0; I < 6; ++I) sQ[(al * 6 + I) * 64 + threadIdx.x] = Q[I];
       double tt = 0.0;
 #pragma unroll
       for (int i = 0; i < 9; ++i) tt += p.Sig[i] * gr.P[al][i];
@@ -942,7 +964,7 @@ __global__ __launch_bounds__(64) void cp_tangent(CPArgs a, int64_t nqb) {
     // slip loop, so the factors and the accumulators are not live together)
     double Ah[81];
     Border B;
-    cp_jacobian_re<HE, true>(cm, gr, a.dt, x, A, p, g_n, Ah, B);
+    cp_jacobian_re<HE, true, true>(cm, gr, a.dt, x, A, p, g_n, Ah, B, sQ + threadIdx.x);
     int piv[9];
     if (!lu9(Ah, piv)) {
       atomicAdd(a.err, 1);
