@@ -68,14 +68,15 @@ def cp_flops_per_qp(mean_iters, nne=4):
         12 power-law rates, R_M;
       Jacobian ~6.2e3: 12 x (d tau_a/dM 260 + rank-1 update of the 9x9 block 162 + borders
         36), lambda terms 400, Schur complement 162;
-      9x9 LU 500 and two solves 324; one line-search evaluation 1.4e3;
-      => Newton iteration ~9.8e3 (the evaluation at the top of the iteration included);
+      9x9 LU 500 and two solves 324;
+      => Newton iteration ~8.4e3 (one evaluation: a line-search trial's evaluation is the next
+      iteration's point);
       start ~3.5e3 (up to 10 tau-ratio evaluations of ~350);
       tangent pass ~1.2e4 (evaluation, Jacobian, LU, nine bordered solves, d tau/dF);
       element rows: nine row-basis vectors per QP x ~650 (B, Y = L:B, Z, z = Z:D, W, Fr W),
         then 3*nne rows x 2 (50 + 9 nne) (S g_a, residual, the three-basis combination, the
         geometric term and the row of K)."""
-    return 9.8e3 * mean_iters + 3.5e3 + 1.2e4 + 9 * 650 + 3 * nne * 2 * (50 + 9 * nne)
+    return 8.4e3 * mean_iters + 3.5e3 + 1.2e4 + 9 * 650 + 3 * nne * 2 * (50 + 9 * nne)
 
 
 def measured_traffic(key):

@@ -719,6 +719,11 @@ __global__ __launch_bounds__(64) void cp_newton(CPArgs a, int64_t chunk) {
   int it = 0, it_att = 0, gid = 0;
   bool bad_start = false;
   int kind = 0;  // the QP's start: 0 plain, 1 the caller's (learned), 2 the caller's, then the plain fallback
+  // the lane's phase: evaluate the start, a line-search trial x + alpha dx, or the full step
+  // after 40 halvings without a decrease
+  constexpr int kPhStart = 0, kPhTrial = 1, kPhFull = 2;
+  int phase = kPhStart, ls = 0;
+  double dx[11], f0 = 0.0, alpha = 1.0;
   CPMaterial cm{};
   for (;;) {
     // refill the lanes that need a QP, in lane order, from the wave's range
@@ -748,6 +753,7 @@ __global__ __launch_bounds__(64) void cp_newton(CPArgs a, int64_t chunk) {
         gscale = 1.0 / (fabs(cm.gs0 - cm.g0) * g_n + 1e-300);
         it = 0;
         it_att = 0;
+        phase = kPhStart;
         // start: the caller's warm start where cp_start_kernel chose it, else the plain start
         const double *row = a.qpbuf + q * kCPBuf;
         const int flag = int(row[kStartFlag]);
@@ -768,16 +774,49 @@ __global__ __launch_bounds__(64) void cp_newton(CPArgs a, int64_t chunk) {
     if (__all(done)) break;
     if (done || need) continue;
 
-    // ---- one Newton iteration of this lane's QP
+    // ---- one turn of this lane's QP: every live lane evaluates the material point once, at its
+    // start or at its line-search trial x + alpha dx; a lane whose start or accepted trial this
+    // is then takes a Newton step from it.  A trial's evaluation is thus also the point of the
+    // next iteration (one evaluation per iteration, and none of it at a refill's masked width).
     const DevGrain &gr = a.grains[gid];
     const int64_t qg = a.qp_base + q;
-    ++it;
-    ++it_att;
-    bool fail = false, conv = false;
+    bool fail = false, conv = false, step = false;
+    double xc[11];
+#pragma unroll
+    for (int i = 0; i < 11; ++i)
+      xc[i] = (phase == kPhStart) ? x[i] : x[i] + (phase == kPhTrial ? alpha : 1.0) * dx[i];
     CPPoint p;
-    if (bad_start || !cp_point<HE>(cm, gr, A, det_pFn, g_n, dt, x, p)) {
-      fail = true;
+    const bool valid = !(phase == kPhStart && bad_start) && cp_point<HE>(cm, gr, A, det_pFn, g_n, dt, xc, p);
+    if (phase == kPhStart) {
+      ++it;
+      ++it_att;
+      if (valid) step = true;
+      else fail = true;
     } else {
+      bool accept = false;
+      if (phase == kPhTrial) {
+        accept = valid && cp_merit(p, gscale) <= (1.0 - 1e-4 * alpha) * f0;
+        if (!accept) {
+          alpha *= 0.5;
+          if (++ls == 40) phase = kPhFull;  // no decrease found: the full step, if it is valid
+        }
+      } else {
+        accept = valid;
+        if (!valid) fail = true;
+      }
+      if (accept) {
+#pragma unroll
+        for (int i = 0; i < 11; ++i) x[i] = xc[i];
+        if (it_att >= cm.maxit) {  // the end of the iteration that took this step
+          fail = true;
+        } else {
+          ++it;
+          ++it_att;
+          step = true;
+        }
+      }
+    }
+    if (step) {
       double Ah[81];
       Border B;
       cp_jacobian_re<HE, false>(cm, gr, dt, x, A, p, g_n, Ah, B);
@@ -795,7 +834,6 @@ __global__ __launch_bounds__(64) void cp_newton(CPArgs a, int64_t chunk) {
         double r9[9];
 #pragma unroll
         for (int i = 0; i < 9; ++i) r9[i] = -p.R[i];
-        double dx[11];
         bordered_solve(Ah, piv, B, v, dTv, r9, -p.R[9], -p.R[10], dx);
         double du = 0.0;
 #pragma unroll
@@ -806,34 +844,14 @@ __global__ __launch_bounds__(64) void cp_newton(CPArgs a, int64_t chunk) {
 #pragma unroll
           for (int i = 0; i < 11; ++i) x[i] += dx[i];
           conv = true;
-        } else {
-          const double f0 = cp_merit(p, gscale);
-          double alpha = 1.0, xn[11];
-          bool ok = false;
-          for (int ls = 0; ls < 40; ++ls) {
-#pragma unroll
-            for (int i = 0; i < 11; ++i) xn[i] = x[i] + alpha * dx[i];
-            double mn = 0.0;
-            if (cp_merit_at<HE>(cm, gr, A, det_pFn, g_n, dt, xn, gscale, mn) && mn <= (1.0 - 1e-4 * alpha) * f0) {
-              ok = true;
-              break;
-            }
-            alpha *= 0.5;
-          }
-          if (!ok) {
-#pragma unroll
-            for (int i = 0; i < 11; ++i) xn[i] = x[i] + dx[i];
-            double mn = 0.0;
-            if (!cp_merit_at<HE>(cm, gr, A, det_pFn, g_n, dt, xn, gscale, mn)) fail = true;
-          }
-          if (!fail) {
-#pragma unroll
-            for (int i = 0; i < 11; ++i) x[i] = xn[i];
-          }
+        } else {  // backtracking on half the squared residual norm, from the full step
+          f0 = cp_merit(p, gscale);
+          alpha = 1.0;
+          ls = 0;
+          phase = kPhTrial;
         }
       }
     }
-    if (!conv && !fail && it_att >= cm.maxit) fail = true;
     if (fail && kind == 1) {
       // the warm start did not converge: fall back to the plain start with a fresh budget
       // (iterations keep counting)
@@ -844,6 +862,7 @@ __global__ __launch_bounds__(64) void cp_newton(CPArgs a, int64_t chunk) {
       for (int i = 0; i < 11; ++i) x[i] = row[i];
       bad_start = (int(row[kStartFlag]) & 1) != 0;
       fail = false;
+      phase = kPhStart;
     }
     if (conv || fail) {
       a.iters[qg] = it | (kind << 24);  // iterations, and the start in the high byte
