@@ -1149,10 +1149,16 @@ __global__ __launch_bounds__(256) void cp_elem(CPElemArgs a, int64_t e_begin, in
       s_S[slot][ip][I] = S6[I];
       s_Sh[slot][ip][I] = Sh9[vi(I) * 3 + vj(I)];
     }
-    if constexpr (TAN) {
+    if constexpr (TAN && NQP > 1) {
 #pragma unroll
       for (int i = 0; i < 81; ++i) s_D[slot][ip][i] = qb[16 + i];
     }
+  }
+  if constexpr (TAN && NQP == 1) {
+    // one QP per element (linear tets): its dMe/dF is spread over the element's lanes
+    // (consecutive lanes on consecutive doubles) rather than 81 loads on the QP's one lane
+    if (active)
+      for (int i = lane; i < 81; i += TPE) s_D[slot][0][i] = a.qpbuf[e * kCPBuf + 16 + i];
   }
   __syncthreads();
   if constexpr (TAN) {
