@@ -297,7 +297,8 @@ int pgf_set_graph(pgf_handle h, int on);
 /* Autotune the launch configuration by timing passes at the given input (no reference
  * counterpart: the reference's element loop has no launch shape):
  *   - points per wavefront of the crystal-plasticity local-Newton work queue (QPs per wave
- *     range, 64..2048), on integrating residual passes;
+ *     range, 64..2048, or 1: every wave draws its QPs from one global queue as its lanes free
+ *     up), on integrating residual passes;
  *   - the gather assembly's tile (gather mode): nodes per workgroup (1, 2, 4; capped by the
  *     LDS a node's rows take) x staging blocks each half-wave keeps in flight (2, 4, 8), on
  *     residual + tangent passes.
@@ -306,8 +307,8 @@ int pgf_set_graph(pgf_handle h, int on);
 int pgf_autotune(pgf_handle h, const pgf_nodal_input *in, double dt, int64_t *chunk);
 /* The launch configuration in use (pgf_autotune's choice or pgf_set_tuning's). */
 int pgf_get_tuning(pgf_handle h, int64_t *cp_chunk, int32_t *gather_waves, int32_t *gather_batch);
-/* Set the launch configuration: cp_chunk >= 64, gather_waves in {1, 2, 4}, gather_batch in
- * {2, 4, 8}; a value <= 0 keeps the current one. */
+/* Set the launch configuration: cp_chunk = 1 (one global queue) or >= 64, gather_waves in
+ * {1, 2, 4}, gather_batch in {2, 4, 8}; a value <= 0 keeps the current one. */
 int pgf_set_tuning(pgf_handle h, int64_t cp_chunk, int32_t gather_waves, int32_t gather_batch);
 
 /* Assembly of element rows into f and the CSR values (the role of fd_res_assemble and

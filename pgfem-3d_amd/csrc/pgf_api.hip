@@ -224,7 +224,9 @@ struct pgf_handle_s {
   double cp_g0 = 0.0, dt = 1.0;
   int32_t cp_law = 0;  // elastic law of the crystal-plasticity material (pgf_material.cp_elastic)
   int64_t cp_nq = 0;         // crystal-plasticity QPs: device QPs [0, cp_nq)
-  int64_t cp_chunk = 256;    // QPs per wave range of the local-Newton work queue (autotuned)
+  int64_t cp_chunk = 256;    // QPs per wave range of the local-Newton work queue (autotuned; 1:
+                             // every wave draws from one global queue)
+  unsigned long long *d_qnext = nullptr;  // the global queue's counter
   int32_t gather_waves = 4;  // nodes per gather workgroup (autotuned; capped by LDS)
   int32_t gather_batch = 4;  // staging blocks each gather half-wave keeps in flight (autotuned)
   int32_t *d_qgrain_all = nullptr;  // [n_qp] grain of every device QP
@@ -324,6 +326,7 @@ pgf::CPArgs cp_args(pgf_handle h, const Block &b, const pgf_nodal_input *in) {
   c.tau_n = h->d_tau[1];
   c.gdot = h->d_gdot[0];
   c.iters = h->d_iters;
+  c.qnext = h->cp_chunk == 1 ? h->d_qnext : nullptr;
   c.warm = h->warm;
   c.qpbuf = h->d_qpbuf + b.qp_base * pgf::kCPBuf;
   c.err = h->d_err;
@@ -1076,6 +1079,7 @@ int pgf_create(const pgf_mesh *mesh, const int64_t *id, int64_t ndof, const int3
       if ((rc = pgf::dalloc(&h->d_gdot[t], size_t(h->n_qp) * 12))) return fail(rc);
     }
     if ((rc = pgf::dalloc(&h->d_iters, size_t(h->n_qp)))) return fail(rc);
+    if ((rc = pgf::dalloc(&h->d_qnext, 1))) return fail(rc);
     if ((rc = pgf::dalloc(&h->d_qpbuf, size_t(h->n_qp) * pgf::kCPBuf))) return fail(rc);
     if (hipMemset(h->d_iters, 0, sizeof(int32_t) * h->n_qp) != hipSuccess) return fail(PGF_EHIP);
     if ((rc = pgf::dalloc(&h->d_qgrain_all, size_t(h->n_qp)))) return fail(rc);
@@ -1144,6 +1148,7 @@ int pgf_destroy(pgf_handle h) {
     hipFree(h->d_gdot[t]);
   }
   hipFree(h->d_iters);
+  hipFree(h->d_qnext);
   hipFree(h->d_qpbuf);
   for (int t = 0; t < 3; ++t) {
     hipFree(h->d_F[t]);
@@ -1841,7 +1846,7 @@ int pgf_autotune(pgf_handle h, const pgf_nodal_input *in, double dt, int64_t *ch
   };
   // 1. points per wavefront of the crystal-plasticity work queue, on integrating passes
   if (tune_cp) {
-    const int64_t cand[] = {64, 128, 256, 512, 1024, 2048};
+    const int64_t cand[] = {1, 64, 128, 256, 512, 1024, 2048};  // 1: one global queue
     int64_t best = h->cp_chunk;
     double best_ms = 1e300;
     for (int64_t c : cand) {
@@ -1895,7 +1900,7 @@ int pgf_get_tuning(pgf_handle h, int64_t *cp_chunk, int32_t *gather_waves, int32
 
 int pgf_set_tuning(pgf_handle h, int64_t cp_chunk, int32_t gather_waves, int32_t gather_batch) {
   if (!h) return PGF_EINVAL;
-  if (cp_chunk > 0 && cp_chunk < 64) return PGF_EINVAL;
+  if (cp_chunk > 1 && cp_chunk < 64) return PGF_EINVAL;
   if (gather_waves > 0 && !(gather_waves == 1 || gather_waves == 2 || gather_waves == 4)) return PGF_EINVAL;
   if (gather_batch > 0 && !(gather_batch == 2 || gather_batch == 4 || gather_batch == 8)) return PGF_EINVAL;
   if (cp_chunk > 0) h->cp_chunk = cp_chunk;

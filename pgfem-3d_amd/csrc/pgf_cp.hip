@@ -35,6 +35,7 @@
 //                  B = Me^T sAA Me, Y = L:B, W = Me Y Me^T, Sh = Me S Me^T, Cr = Fr^T Fr.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 
 #include "pgf_device.h"
@@ -704,13 +705,17 @@ __global__ __launch_bounds__(64) void cp_start_kernel(CPArgs a) {
 }
 
 // ------------------------------------------------------------------- Newton kernel
-// One wave per workgroup; wave w owns QPs [w*chunk, (w+1)*chunk) of the block.  The starting
-// points come from cp_start_kernel (qpbuf).
+// One wave per workgroup.  With a.qnext set, every wave takes its QPs from one global queue:
+// at each refill the wave's first lane that needs a QP claims as many consecutive QPs as lanes
+// need one, so lanes stay busy until the queue runs dry.  Otherwise wave w owns QPs
+// [w*chunk, (w+1)*chunk) of the block.  Which lane takes a QP does not change its result.  The
+// starting points come from cp_start_kernel (qpbuf).
 template <bool HE>
 __global__ __launch_bounds__(64) void cp_newton(CPArgs a, int64_t chunk) {
   const int lane = threadIdx.x;
-  const int64_t q_lo = int64_t(blockIdx.x) * chunk;
-  const int64_t q_hi = (q_lo + chunk < a.nq) ? q_lo + chunk : a.nq;
+  const bool gq = a.qnext != nullptr;
+  const int64_t q_lo = gq ? 0 : int64_t(blockIdx.x) * chunk;
+  const int64_t q_hi = (gq || q_lo + chunk >= a.nq) ? a.nq : q_lo + chunk;
   int64_t cursor = q_lo;  // wave-uniform
   const int64_t NQ = a.n_qp;
   bool need = true, done = false;
@@ -729,6 +734,12 @@ __global__ __launch_bounds__(64) void cp_newton(CPArgs a, int64_t chunk) {
     // refill the lanes that need a QP, in lane order, from the wave's range
     const uint64_t mask = __ballot(need && !done);
     const int nneed = __popcll(mask);
+    if (gq && nneed > 0) {  // claim nneed QPs from the global queue (wave-uniform branch)
+      const int leader = __ffsll((unsigned long long)mask) - 1;
+      long long base = 0;
+      if (lane == leader) base = (long long)atomicAdd(a.qnext, (unsigned long long)nneed);
+      cursor = __shfl(base, leader);
+    }
     if (need && !done) {
       const int rank = __popcll(mask & ((uint64_t(1) << lane) - 1));
       const int64_t cand = cursor + rank;
@@ -770,7 +781,7 @@ __global__ __launch_bounds__(64) void cp_newton(CPArgs a, int64_t chunk) {
         }
       }
     }
-    cursor += nneed;
+    if (!gq) cursor += nneed;
     if (__all(done)) break;
     if (done || need) continue;
 
@@ -2095,11 +2106,23 @@ void launch_cp_kinematics(int nne, const CPArgs &a, hipStream_t s) {
   else if (nne == 8) launch_kin<8>(a, s);
 }
 
+__global__ void cp_queue_reset(unsigned long long *q) { *q = 0; }
+// resident waves of the persistent Newton kernel on an MI355X: 256 CUs x 4 SIMDs, one wave each
+constexpr int64_t kQueueWaves = 1024;
+
 void launch_cp_newton(const CPArgs &a, int64_t chunk, hipStream_t s) {
   const int64_t nq = a.nq;
   if (nq <= 0) return;
-  if (chunk < 64) chunk = 64;
-  const unsigned grid = unsigned((nq + chunk - 1) / chunk);
+  if (a.qnext) {
+    // one persistent wave per SIMD of the chip (the kernel holds all 512 registers of a lane),
+    // draining the global queue; the counter is reset by a kernel, so a captured pass replays it
+    cp_queue_reset<<<1, 1, 0, s>>>(a.qnext);
+    chunk = 64;
+  } else if (chunk < 64) {
+    chunk = 64;
+  }
+  const unsigned grid = a.qnext ? unsigned(std::min<int64_t>((nq + 63) / 64, kQueueWaves))
+                                : unsigned((nq + chunk - 1) / chunk);
   const unsigned sgrid = unsigned((nq + 63) / 64);
   if (a.law == 1) {
     cp_start_kernel<true><<<sgrid, 64, 0, s>>>(a);

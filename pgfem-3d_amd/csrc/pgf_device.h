@@ -144,7 +144,7 @@ struct CPArgs {
   // tangent-pass output for the element kernel: Me 9, S 6, pJ 1, D = dMe/dF 81 (block-local)
   double *qpbuf;
   int32_t *err;
-  int32_t *queue;             // work-queue counter
+  unsigned long long *qnext;  // global work queue of the local Newton (cp_chunk = 1), else null
   int32_t law;                // cpmats[0].law, read on the host to pick the kernels
   // three-field (CM3F) blocks: per block element theta = tf_V + tf_dV; the kinematics then
   // stores the model's F(n+1) = (theta/J)^(1/3) F.  Null otherwise.

@@ -80,7 +80,7 @@ def test_graph_replay_and_autotune_are_bitwise_eager(device):
         h.set_graph(True)
     h.set_graph(False)
     chunk = h.autotune(rd, defl=dd)
-    assert chunk in (64, 128, 256, 512, 1024, 2048)
+    assert chunk in (1, 64, 128, 256, 512, 1024, 2048)  # 1: one global queue
     assert h.residual_tangent(f, vals, rd, defl=dd) == 0
     assert np.array_equal(f.cpu().numpy(), f0) and np.array_equal(vals.cpu().numpy(), v0)
     h.close()

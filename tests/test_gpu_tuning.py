@@ -64,7 +64,7 @@ def test_points_per_wavefront_and_gather_tile_on_a_mixed_batch(device):
     r, defl = prob.displacement(4e-4, 2e-5, seed=2)
     rd, dd = _dev(r, device), _dev(defl, device)
     f0, v0 = _pass(h, f, vals, rd, dd)
-    for c, w, b in ((64, 1, 2), (2048, 2, 8), (512, 4, 4)):
+    for c, w, b in ((64, 1, 2), (2048, 2, 8), (1, 2, 4), (512, 4, 4)):  # 1: one global queue
         h.set_tuning(cp_chunk=c, gather_waves=w, gather_batch=b)
         f1, v1 = _pass(h, f, vals, rd, dd)
         assert np.array_equal(f1, f0) and np.array_equal(v1, v0), (c, w, b)
