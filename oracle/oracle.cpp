@@ -1167,6 +1167,28 @@ int orc_tf_element(const orc_material *mat, int nne, const double *x, const doub
 // stiffness_el_constitutive_model_3f, src/constitutive_model.cc:2714-2735, :3565-3590):
 // theta[e] = V_np1 + dV and P[e] = P_np1 + dP of each element; with run_integration the
 // model's F(n+1) (per QP) is stored.
+// The element loop of an assembly, colour by colour (orc_colour_elements): within a colour no two
+// elements share a node, so their scatters touch distinct entries of f and vals and run on the
+// host's threads; every entry's sum follows the colour order whatever the thread count.
+// element(e) returns its error count, or -1 when a CSR entry is missing.
+extern "C++" template <class Fn>
+int assemble_by_colour(int64_t n_elem, const int64_t *elem_ptr, const int64_t *elem_conn, Fn element) {
+  int64_t n_nodes = 0;
+  for (int64_t k = 0; k < elem_ptr[n_elem]; ++k) n_nodes = std::max<int64_t>(n_nodes, elem_conn[k] + 1);
+  std::vector<int64_t> order(n_elem), cptr(n_elem + 1);
+  const int nc = orc_colour_elements(n_nodes, n_elem, elem_ptr, elem_conn, order.data(), cptr.data());
+  int err = 0, missing = 0;
+  for (int c = 0; c < nc; ++c) {
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : err, missing)
+    for (int64_t k = cptr[c]; k < cptr[c + 1]; ++k) {
+      const int r = element(order[k]);
+      if (r < 0) ++missing;
+      else err += r;
+    }
+  }
+  return missing ? -1 : err;
+}
+
 int orc_assemble_tf(int64_t n_elem, const double *X, const int64_t *elem_ptr, const int64_t *elem_conn,
                     const int32_t *elem_mat, const orc_material *mats, const int64_t *id, int64_t ndof,
                     const int32_t *Ap, const int32_t *Ai, const double *r, const double *d_r, const double *defl,
@@ -1175,9 +1197,11 @@ int orc_assemble_tf(int64_t n_elem, const double *X, const int64_t *elem_ptr, co
                     double *plastic_np1, const double *scalar_n, double *scalar_np1) {
   if (f) std::memset(f, 0, sizeof(double) * ndof);
   if (vals) std::memset(vals, 0, sizeof(double) * Ap[ndof]);
-  int err = 0;
-  int64_t qp0 = 0;
-  for (int64_t e = 0; e < n_elem; ++e) {
+  std::vector<int64_t> qoff(n_elem + 1, 0);
+  for (int64_t e = 0; e < n_elem; ++e) qoff[e + 1] = qoff[e] + nint_of(int(elem_ptr[e + 1] - elem_ptr[e]));
+  return assemble_by_colour(n_elem, elem_ptr, elem_conn, [&](int64_t e) -> int {
+    int err = 0;
+    const int64_t qp0 = qoff[e];
     const int nne = int(elem_ptr[e + 1] - elem_ptr[e]), nd = 3 * nne, nint = nint_of(nne);
     double x[30], u[30], Fb[72];
     int64_t cn[30];
@@ -1223,9 +1247,8 @@ int orc_assemble_tf(int64_t n_elem, const double *X, const int64_t *elem_ptr, co
           vals[p - Ai] += K[i * nd + j];
         }
     }
-    qp0 += nint_of(nne);
-  }
-  return err;
+    return err;
+  });
 }
 
 // constitutive_model_update_NR for CM3F without inertia (src/constitutive_model.cc:3916-3999,
@@ -1565,9 +1588,11 @@ int orc_assemble_cp_tf(const double *X, int64_t n_elem, const int64_t *elem_ptr,
                        double *gdot_out, int32_t *iters_out, double *f, double *vals) {
   if (f) std::memset(f, 0, sizeof(double) * ndof);
   if (vals) std::memset(vals, 0, sizeof(double) * Ap[ndof]);
-  int err = 0;
-  int64_t qp0 = 0;
-  for (int64_t e = 0; e < n_elem; ++e) {
+  std::vector<int64_t> qoff(n_elem + 1, 0);
+  for (int64_t e = 0; e < n_elem; ++e) qoff[e + 1] = qoff[e] + nint_of(int(elem_ptr[e + 1] - elem_ptr[e]));
+  return assemble_by_colour(n_elem, elem_ptr, elem_conn, [&](int64_t e) -> int {
+    int err = 0;
+    const int64_t qp0 = qoff[e];
     const int nne = int(elem_ptr[e + 1] - elem_ptr[e]), nd = 3 * nne, nint = nint_of(nne);
     double x[30], u[30];
     int64_t cn[30];
@@ -1628,9 +1653,8 @@ int orc_assemble_cp_tf(const double *X, int64_t n_elem, const int64_t *elem_ptr,
           vals[p - Ai] += K[i * nd + j];
         }
     }
-    qp0 += nint;
-  }
-  return err;
+    return err;
+  });
 }
 
 // compute_d_theta_dP for the CM3F crystal-plasticity element at the stored state: the

@@ -7,7 +7,10 @@ meshes of up to 10^3 cells):
   * crystal plasticity on 55^3 cells x 6 linear tets (998,250 QPs) in 64 Voronoi grains, three
     load steps of 1e-3 with the state committed between them;
   * the mixed batch: 41^3 x 6 linear tets, 32^3 x 6 quadratic tets and 46^3 hexes in 64 grains
-    with interface elements on the grain boundaries, two load steps.
+    with interface elements on the grain boundaries, two load steps;
+  * the CM3F three-field element (one volume and one pressure per element, condensed): a
+    neo-Hookean material on the J2 mesh (48^3 x 6 quadratic tets) and crystal plasticity with
+    the hyperelastic law on the CP mesh, one pass each at seeded element volumes and pressures.
 
 Each compares f to its largest entry, K to each CSR row's own largest entry and the state per
 QP to its own size, within 1e-10 of the oracle (orc_assemble / orc_assemble_cp /
@@ -23,7 +26,7 @@ import torch
 import oracle as O
 import pgfem3d_amd as P
 from conftest import rel_cols, rel_csr_rows, rel_inf
-from problems import BoxProblem, CPBoxProblem, MixedProblem
+from problems import BoxProblem, CPBoxProblem, MixedProblem, fcc_crystal_he
 
 pytestmark = pytest.mark.gpu
 TOL = 1e-10
@@ -134,4 +137,59 @@ def test_mixed_batch_with_interfaces(device):
         h.commit_step()
         st.commit()
         cst.commit()
+    h.close()
+
+
+def _tf_fields(h, theta, Pe):
+    for fld, v in ((P.TF_V_NP1, theta), (P.TF_P_NP1, Pe), (P.TF_DV, 0 * theta), (P.TF_DP, 0 * Pe)):
+        h.tf_set(fld, v)
+
+
+def test_cm3f_neo_hookean_p2_48cube(device):
+    print("full size: CM3F neo-Hookean 48^3 P2 tets", flush=True)
+    prob = BoxProblem(48, P.PGF_TET10, P.neo_hookean(70e3, 0.33))
+    ne = prob.mesh.n_elem
+    h = P.Handle(prob.mesh, prob.ids, prob.ndof, prob.Ap, prob.Ai, prob.mats)
+    h.set_three_field(True)
+    rng = np.random.default_rng(21)
+    theta = 1.0 + 1e-3 * rng.standard_normal(ne)
+    Pe = 100.0 * rng.standard_normal(ne)
+    _tf_fields(h, theta, Pe)
+    r, defl = prob.displacement(0.015, 2e-4, seed=11)
+    f = torch.empty(prob.ndof, dtype=torch.float64, device=device)
+    vals = torch.empty(h.nnz, dtype=torch.float64, device=device)
+    assert h.residual_tangent(f, vals, _dev(r, device), defl=_dev(defl, device)) == 0
+    Fo = np.zeros((9, prob.nqp))
+    fo, vo, err = O.assemble_tf(prob.mesh, prob.mats, prob.ids, prob.ndof, prob.Ap, prob.Ai, theta, Pe, r,
+                                defl=defl, F_np1=Fo)
+    assert err == 0
+    _check("cm3f he", f, vals, fo, vo, prob.Ap)
+    assert np.abs(h.get_state(P.STATE_F_NP1) - Fo).max() < 1e-12
+    h.close()
+
+
+def test_cm3f_crystal_plasticity_p1_55cube(device):
+    print("full size: CM3F crystal plasticity 55^3 P1 tets, 64 grains", flush=True)
+    mat = fcc_crystal_he()
+    prob = CPBoxProblem(55, P.PGF_TET4, mat, n_grains=64, seed=2)
+    ne = prob.mesh.n_elem
+    h = P.Handle(prob.mesh, prob.ids, prob.ndof, prob.Ap, prob.Ai, prob.mats)
+    h.set_orientations(prob.R, prob.grain)
+    h.set_three_field(True)
+    st = O.CPState(prob.nqp, mat.g0)
+    rng = np.random.default_rng(22)
+    theta = 1.0 + 2e-4 * rng.standard_normal(ne)
+    Pe = 60.0 * rng.standard_normal(ne)
+    _tf_fields(h, theta, Pe)
+    r, defl = prob.displacement(1e-3, 0.0, seed=1)
+    f = torch.empty(prob.ndof, dtype=torch.float64, device=device)
+    vals = torch.empty(h.nnz, dtype=torch.float64, device=device)
+    assert h.residual_tangent(f, vals, _dev(r, device), defl=_dev(defl, device)) == 0
+    fo, vo, err = O.assemble_cp_tf(prob.mesh, prob.mats, prob.grain, prob.gP, prob.gL, prob.ids, prob.ndof, prob.Ap,
+                                   prob.Ai, st, theta, Pe, r, defl=defl)
+    assert err == 0
+    _check("cm3f cp", f, vals, fo, vo, prob.Ap)
+    assert np.abs(h.get_state(P.STATE_F_NP1) - st.F_np1).max() < 1e-12
+    assert rel_cols(h.get_state(P.STATE_PLASTIC_NP1), st.pF_np1) < TOL
+    assert np.mean(np.abs(st.gdot).max(axis=0) > 1e-5) > 0.5  # most points slip
     h.close()
