@@ -118,6 +118,10 @@ void make_grain(const double *R, double C11, double C12, double C44, DevGrain &g
     }
     for (int i = 0; i < 3; ++i)
       for (int j = 0; j < 3; ++j) g.P[a][i * 3 + j] = s[i] * m[j];
+    for (int i = 0; i < 3; ++i) {
+      g.s[a][i] = s[i];
+      g.m[a][i] = m[i];
+    }
   }
   auto L0 = [&](int i, int j, int k, int l) {
     double v = 0.0;

@@ -180,6 +180,17 @@ __device__ __forceinline__ double flow_pow(double x, double e, int ie) {
   return pow(x, e);
 }
 
+// The resolved shear stress X : P_a with the Schmid tensor in its factored form P_a = s_a (x) m_a:
+// s_a . (X m_a).  The kernels of the local solve keep the 6 numbers of each system's s and m
+// rather than the 9 of P_a: 72 doubles of the grain instead of 108 in the registers.
+__device__ __forceinline__ double resolved(const DevGrain &gr, int a, const double *X) {
+  const double *s = gr.s[a], *m = gr.m[a];
+  double t = 0.0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) t += s[i] * (X[i * 3 + 0] * m[0] + X[i * 3 + 1] * m[1] + X[i * 3 + 2] * m[2]);
+  return t;
+}
+
 // 9x9 LU with partial pivoting; row swaps are predicated so every index is static.
 __device__ __forceinline__ bool lu9(double (&A)[81], int (&piv)[9]) {
 #pragma unroll
@@ -270,9 +281,7 @@ __device__ __forceinline__ bool cp_point(const CPMaterial &cm, const DevGrain &g
   p.gsum = 0.0;
 #pragma unroll
   for (int a = 0; a < 12; ++a) {
-    double t = 0.0;
-#pragma unroll
-    for (int i = 0; i < 9; ++i) t += p.Sig[i] * gr.P[a][i];
+    const double t = resolved(gr, a, p.Sig);
     p.tau[a] = t;
     const double r = t * rg;
     const double pw = flow_pow(fabs(r), expo, cm.ipow);
@@ -291,15 +300,20 @@ __device__ __forceinline__ bool cp_point(const CPMaterial &cm, const DevGrain &g
     p.gs = 0.0;
     p.dgs = 0.0;
   }
+  double sl[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // sum_a gdot_a s_a (x) m_a, in system order
+#pragma unroll
+  for (int a = 0; a < 12; ++a)
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const double c = p.gdot[a] * gr.s[a][i];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) sl[i * 3 + j] += c * gr.m[a][j];
+    }
 #pragma unroll
   for (int i = 0; i < 3; ++i)
 #pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      double s = 0.0;
-#pragma unroll
-      for (int a = 0; a < 12; ++a) s += p.gdot[a] * gr.P[a][i * 3 + j];
-      p.R[i * 3 + j] = M[i * 3 + j] - ((i == j) ? 1.0 : 0.0) + dt * s - dt * lam * p.pJ * p.Mi[j * 3 + i];
-    }
+    for (int j = 0; j < 3; ++j)
+      p.R[i * 3 + j] = M[i * 3 + j] - ((i == j) ? 1.0 : 0.0) + dt * sl[i * 3 + j] - dt * lam * p.pJ * p.Mi[j * 3 + i];
   p.R[9] = p.pJ - 1.0;
   p.R[10] = g * (p.gs - cm.g0 + dt * cm.h0 * p.gsum) - (p.gs - cm.g0) * g_n - dt * cm.h0 * p.gs * p.gsum;
   return true;
@@ -308,11 +322,22 @@ __device__ __forceinline__ bool cp_point(const CPMaterial &cm, const DevGrain &g
 // Q_a = sym(P_a S + 1/2 L:(C_e P_a)) in Voigt order, so that d tau_a / d M = 2 A^T eF Q_a
 template <bool HE>
 __device__ __forceinline__ void qsym(const DevGrain &gr, const double *Lv, int a, const CPPoint &p, double *Q6) {
-  const double *P = gr.P[a];
-  double S9[9], PS[9], CP[9];
+  // with P = s (x) m: P S = s (x) (S m) (S symmetric) and C_e P = (C_e s) (x) m
+  const double *s = gr.s[a], *m = gr.m[a];
+  double S9[9], PS[9], CP[9], v[3], w[3];
   v6_to_9(p.S, S9);
-  mm3(P, S9, PS);
-  mm3(p.Ce, P, CP);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    v[i] = S9[i * 3 + 0] * m[0] + S9[i * 3 + 1] * m[1] + S9[i * 3 + 2] * m[2];
+    w[i] = p.Ce[i * 3 + 0] * s[0] + p.Ce[i * 3 + 1] * s[1] + p.Ce[i * 3 + 2] * s[2];
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      PS[i * 3 + j] = s[i] * v[j];
+      CP[i * 3 + j] = w[i] * m[j];
+    }
   double LCP[6];
   if constexpr (HE) he_dot(p.he, CP, LCP);
   else lv_dot(Lv, CP, LCP);
@@ -380,9 +405,7 @@ __device__ __forceinline__ void cp_jacobian_re(const CPMaterial &cm, const DevGr
   const double rg = 1.0 / g, cdg = cm.gd0 * rg / cm.m;
   for (int a = 0; a < 12; ++a) {
     // slip-system quantities recomputed from the Mandel stress (as cp_point evaluates them)
-    double t = 0.0;
-#pragma unroll
-    for (int i = 0; i < 9; ++i) t += p.Sig[i] * gr.P[a][i];
+    const double t = resolved(gr, a, p.Sig);
     const double r = t * rg;
     const double pw = flow_pow(fabs(r), expo, cm.ipow);
     const double gd = cm.gd0 * r * pw;
@@ -401,10 +424,11 @@ __device__ __forceinline__ void cp_jacobian_re(const CPMaterial &cm, const DevGr
     const double sgn = (gd < 0) ? -1.0 : 1.0;
 #pragma unroll
     for (int r = 0; r < 9; ++r) {
-      const double pr = gr.P[a][r] * co;
+      const double Pr = gr.s[a][r / 3] * gr.m[a][r % 3];
+      const double pr = Pr * co;
 #pragma unroll
       for (int c = 0; c < 9; ++c) Ah[r * 9 + c] += pr * D[c];
-      B.c[r] += dt * gr.P[a][r] * dgdg;
+      B.c[r] += dt * Pr * dgdg;
     }
 #pragma unroll
     for (int c = 0; c < 9; ++c) B.e[c] += sgn * dgdt * D[c];
@@ -497,14 +521,16 @@ __device__ __forceinline__ bool cp_merit_at(const CPMaterial &cm, const DevGrain
   const double rg = 1.0 / g;
   double gsum = 0.0, sl[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   for (int a = 0; a < 12; ++a) {
-    double t = 0.0;
-#pragma unroll
-    for (int i = 0; i < 9; ++i) t += Sig[i] * gr.P[a][i];
+    const double t = resolved(gr, a, Sig);
     const double r = t * rg;
     const double gd = cm.gd0 * r * flow_pow(fabs(r), expo, cm.ipow);
     gsum += fabs(gd);
 #pragma unroll
-    for (int i = 0; i < 9; ++i) sl[i] += gd * gr.P[a][i];
+    for (int i = 0; i < 3; ++i) {
+      const double c = gd * gr.s[a][i];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) sl[i * 3 + j] += c * gr.m[a][j];
+    }
   }
   double gs;
   if (cm.w == 0.0) gs = cm.gs0;
@@ -538,12 +564,7 @@ __device__ __forceinline__ double tau_ratio_max(const CPMaterial &cm, const DevG
   mm3(Ce, S9, Sig);
   double mx = 0.0;
 #pragma unroll
-  for (int a = 0; a < 12; ++a) {
-    double t = 0.0;
-#pragma unroll
-    for (int i = 0; i < 9; ++i) t += Sig[i] * gr.P[a][i];
-    mx = fmax(mx, fabs(t));
-  }
+  for (int a = 0; a < 12; ++a) mx = fmax(mx, fabs(resolved(gr, a, Sig)));
   return mx * rg;
 }
 
@@ -935,9 +956,7 @@ __global__ __launch_bounds__(64) void cp_tangent(CPArgs a, int64_t nqb) {
   const double expo = 1.0 / cm.m - 1.0;
   const double rg = 1.0 / x[10], cdg = cm.gd0 * rg / cm.m;
   for (int al = 0; al < 12; ++al) {  // tau and gamma_dot as cp_point evaluates them
-    double tt = 0.0;
-#pragma unroll
-    for (int i = 0; i < 9; ++i) tt += p.Sig[i] * gr.P[al][i];
+    const double tt = resolved(gr, al, p.Sig);
     const double r = tt * rg;
     a.tau[al * NQ + qg] = tt;
     a.gdot[al * NQ + qg] = cm.gd0 * r * flow_pow(fabs(r), expo, cm.ipow);
@@ -973,9 +992,7 @@ __global__ __launch_bounds__(64) void cp_tangent(CPArgs a, int64_t nqb) {
       qsym<HE>(gr, opaque(gr.Lv), al, p, Q);
 #pragma unroll
       for (int I = 0; I < 6; ++I) sQ[(al * 6 + I) * 64 + threadIdx.x] = Q[I];
-      double tt = 0.0;
-#pragma unroll
-      for (int i = 0; i < 9; ++i) tt += p.Sig[i] * gr.P[al][i];
+      const double tt = resolved(gr, al, p.Sig);
       const double rr = tt * rg;
       const double pw = flow_pow(fabs(rr), expo, cm.ipow);
       const double dgdt = cdg * pw;
@@ -983,7 +1000,7 @@ __global__ __launch_bounds__(64) void cp_tangent(CPArgs a, int64_t nqb) {
       const double sg = ((cm.gd0 * rr * pw < 0) ? -1.0 : 1.0) * dgdt;
 #pragma unroll
       for (int r = 0; r < 9; ++r) {
-        const double cr = co * gr.P[al][r];
+        const double cr = co * (gr.s[al][r / 3] * gr.m[al][r % 3]);
 #pragma unroll
         for (int I = 0; I < 6; ++I) R[r][I] += cr * Q[I];
       }

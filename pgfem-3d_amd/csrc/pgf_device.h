@@ -107,6 +107,7 @@ enum : int { MODE_INTEGRATE = 1, MODE_RESIDUAL = 2, MODE_TANGENT = 4, MODE_DEFL_
 struct DevGrain {
   double P[12][9];
   double Lv[36];
+  double s[12][3], m[12][3];  // slip directions and plane normals in the sample frame: P = s (x) m
 };
 
 // law 0: cubic St Venant-Kirchhoff through the grain's Lv; law 1: neo-Hookean deviatoric part

@@ -39,8 +39,12 @@ def _run(prob, device, stretches, solver="device", cp=False, predictor=False, co
     else:
         st = O.OracleState(prob.nqp, prob.mats[0].model == P.MODEL_J2_PLASTICITY)
     # the step sequence is fixed on both sides: no physics-based rejection (at 1e-3 per step
-    # with dt = 1 these boxes reach dt sum |gamma_dot| / 0.005 = 1.4, which would subdivide)
-    drv = NW.NewtonDriver(h, prob.Ap, prob.Ai, tol=1e-12, solver=solver, computer_zero=computer_zero,
+    # with dt = 1 these boxes reach dt sum |gamma_dot| / 0.005 = 1.4, which would subdivide).
+    # Crystal plasticity: 1e-11, above the rounding of its assembled residual (about 1e-12 of a
+    # step's first residual, the stiff flow rule amplifying the rounding of the local solves),
+    # where a tighter tolerance is met or missed by chance in the line search on either side
+    tol = 1e-11 if cp else 1e-12
+    drv = NW.NewtonDriver(h, prob.Ap, prob.Ai, tol=tol, solver=solver, computer_zero=computer_zero,
                           alpha_restart=None)
     r_gpu = torch.zeros(prob.ndof, dtype=torch.float64, device=device)
     r_cpu = np.zeros(prob.ndof)
@@ -53,13 +57,13 @@ def _run(prob, device, stretches, solver="device", cp=False, predictor=False, co
         if predictor:  # from u(n) at defl(n), the predictor takes the increment
             dn = np.zeros_like(defl) if defl_n is None else defl_n
             res = drv.load_step(r_gpu, _dev(dn, device), defl_d=_dev(defl - dn, device))
-            r_cpu, it_cpu, gam_cpu = oracle_newton_step(prob, st, r_cpu, dn, ost, tol=1e-12, cp_tables=tables,
+            r_cpu, it_cpu, gam_cpu = oracle_newton_step(prob, st, r_cpu, dn, ost, tol=tol, cp_tables=tables,
                                                          defl_d=defl - dn, computer_zero=computer_zero)
             defl_n = defl
         else:
             r_gpu += _dev(guess - g_prev, device)  # converged u(n) + homogeneous increment
             res = drv.load_step(r_gpu, _dev(defl, device))
-            r_cpu, it_cpu, gam_cpu = oracle_newton_step(prob, st, r_cpu + guess - g_prev, defl, ost, tol=1e-12,
+            r_cpu, it_cpu, gam_cpu = oracle_newton_step(prob, st, r_cpu + guess - g_prev, defl, ost, tol=tol,
                                                          cp_tables=tables, computer_zero=computer_zero)
             g_prev = guess
         assert res.converged, (s, res.norms, res.info, res.solver_iters, res.gammas)
