@@ -129,6 +129,10 @@ void make_grain(const double *R, double C11, double C12, double C44, DevGrain &g
     if (i != j && ((i == k && j == l) || (i == l && j == k))) v += C44;
     return v;
   };
+  for (int i = 0; i < 9; ++i) g.R[i] = R[i];
+  g.cub[0] = C11 - C12;
+  g.cub[1] = C12;
+  g.cub[2] = 2.0 * C44;
   static const int vi[6] = {0, 1, 2, 0, 1, 0}, vj[6] = {0, 1, 2, 1, 2, 2};
   for (int I = 0; I < 6; ++I)
     for (int J = 0; J < 6; ++J) {

@@ -76,6 +76,56 @@ __device__ __forceinline__ void lv_dot(const double *Lv, const double *E9, doubl
     S6[I] = s;
   }
 }
+// S (Voigt tensor comps) = L : sym(X) for the grain's cubic stiffness, through the crystal
+// frame: E' = R^T sym(X) R, S'_ii = (C11 - C12) E'_ii + C12 tr E', S'_ij = 2 C44 E'_ij,
+// S = R S' R^T.  12 numbers of the grain (R and the constants) instead of the 36 of its rotated
+// stiffness; the same tensor up to rounding.  Every multiply-add is an explicit fma and nothing
+// else is contracted, so the result does not depend on the code around a call (the residual
+// and the fused residual-tangent passes give the same bits).
+__device__ __forceinline__ double fma3(double a0, double b0, double a1, double b1, double a2, double b2) {
+#pragma clang fp contract(off)
+  return __builtin_fma(a0, b0, __builtin_fma(a1, b1, a2 * b2));
+}
+__device__ __forceinline__ void cubic_dot(const double *R, const double *cub, const double *X9, double *S6) {
+#pragma clang fp contract(off)
+  double E[9], T[9], Ep[9];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) E[i * 3 + j] = 0.5 * (X9[i * 3 + j] + X9[j * 3 + i]);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)  // T = E R
+#pragma unroll
+    for (int j = 0; j < 3; ++j) T[i * 3 + j] = fma3(E[i * 3 + 0], R[0 * 3 + j], E[i * 3 + 1], R[1 * 3 + j], E[i * 3 + 2], R[2 * 3 + j]);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)  // E' = R^T T (symmetric: the upper triangle)
+#pragma unroll
+    for (int j = i; j < 3; ++j) Ep[i * 3 + j] = fma3(R[0 * 3 + i], T[0 * 3 + j], R[1 * 3 + i], T[1 * 3 + j], R[2 * 3 + i], T[2 * 3 + j]);
+  const double tr = (Ep[0] + Ep[4]) + Ep[8];
+  const double ct = cub[1] * tr;
+  double Sp[9];
+  Sp[0] = __builtin_fma(cub[0], Ep[0], ct);
+  Sp[4] = __builtin_fma(cub[0], Ep[4], ct);
+  Sp[8] = __builtin_fma(cub[0], Ep[8], ct);
+  Sp[1] = Sp[3] = cub[2] * Ep[1];
+  Sp[2] = Sp[6] = cub[2] * Ep[2];
+  Sp[5] = Sp[7] = cub[2] * Ep[5];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)  // T = S' R^T
+#pragma unroll
+    for (int j = 0; j < 3; ++j) T[i * 3 + j] = fma3(Sp[i * 3 + 0], R[j * 3 + 0], Sp[i * 3 + 1], R[j * 3 + 1], Sp[i * 3 + 2], R[j * 3 + 2]);
+  double S[9];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)  // S = R T (symmetric: the upper triangle)
+#pragma unroll
+    for (int j = i; j < 3; ++j) S[i * 3 + j] = fma3(R[i * 3 + 0], T[0 * 3 + j], R[i * 3 + 1], T[1 * 3 + j], R[i * 3 + 2], T[2 * 3 + j]);
+  S6[0] = S[0];
+  S6[1] = S[4];
+  S6[2] = S[8];
+  S6[3] = S[1];
+  S6[4] = S[5];
+  S6[5] = S[2];
+}
 __device__ __forceinline__ void v6_to_9(const double *s, double *S) {
   S[0] = s[0]; S[4] = s[1]; S[8] = s[2];
   S[1] = S[3] = s[3];
@@ -154,7 +204,7 @@ __device__ __forceinline__ void el_stress(const CPMaterial &cm, const DevGrain &
     double E[9];
 #pragma unroll
     for (int i = 0; i < 9; ++i) E[i] = 0.5 * (Ce[i] - ((i % 4 == 0) ? 1.0 : 0.0));
-    lv_dot(gr.Lv, E, S6);
+    cubic_dot(gr.R, gr.cub, E, S6);
   }
 }
 // L : X at the state el_stress was evaluated at
@@ -340,7 +390,7 @@ __device__ __forceinline__ void qsym(const DevGrain &gr, const double *Lv, int a
     }
   double LCP[6];
   if constexpr (HE) he_dot(p.he, CP, LCP);
-  else lv_dot(Lv, CP, LCP);
+  else cubic_dot(gr.R, gr.cub, CP, LCP);
 #pragma unroll
   for (int I = 0; I < 6; ++I) {
     const int i = vi(I), j = vj(I);

@@ -108,6 +108,8 @@ struct DevGrain {
   double P[12][9];
   double Lv[36];
   double s[12][3], m[12][3];  // slip directions and plane normals in the sample frame: P = s (x) m
+  double R[9];                // crystal -> sample rotation
+  double cub[3];              // cubic constants C11 - C12, C12, 2 C44 (crystal frame)
 };
 
 // law 0: cubic St Venant-Kirchhoff through the grain's Lv; law 1: neo-Hookean deviatoric part
