@@ -120,7 +120,7 @@ void make_grain(const double *R, double C11, double C12, double C44, DevGrain &g
       for (int j = 0; j < 3; ++j) g.P[a][i * 3 + j] = s[i] * m[j];
     for (int i = 0; i < 3; ++i) {
       g.s[a][i] = s[i];
-      g.m[a][i] = m[i];
+      g.m[a / 3][i] = m[i];  // the same for the plane's three systems
     }
   }
   auto L0 = [&](int i, int j, int k, int l) {

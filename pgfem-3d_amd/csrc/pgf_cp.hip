@@ -234,7 +234,7 @@ __device__ __forceinline__ double flow_pow(double x, double e, int ie) {
 // s_a . (X m_a).  The kernels of the local solve keep the 6 numbers of each system's s and m
 // rather than the 9 of P_a: 72 doubles of the grain instead of 108 in the registers.
 __device__ __forceinline__ double resolved(const DevGrain &gr, int a, const double *X) {
-  const double *s = gr.s[a], *m = gr.m[a];
+  const double *s = gr.s[a], *m = gr.m[a / 3];
   double t = 0.0;
 #pragma unroll
   for (int i = 0; i < 3; ++i) t += s[i] * (X[i * 3 + 0] * m[0] + X[i * 3 + 1] * m[1] + X[i * 3 + 2] * m[2]);
@@ -357,7 +357,7 @@ __device__ __forceinline__ bool cp_point(const CPMaterial &cm, const DevGrain &g
     for (int i = 0; i < 3; ++i) {
       const double c = p.gdot[a] * gr.s[a][i];
 #pragma unroll
-      for (int j = 0; j < 3; ++j) sl[i * 3 + j] += c * gr.m[a][j];
+      for (int j = 0; j < 3; ++j) sl[i * 3 + j] += c * gr.m[a / 3][j];
     }
 #pragma unroll
   for (int i = 0; i < 3; ++i)
@@ -373,7 +373,7 @@ __device__ __forceinline__ bool cp_point(const CPMaterial &cm, const DevGrain &g
 template <bool HE>
 __device__ __forceinline__ void qsym(const DevGrain &gr, const double *Lv, int a, const CPPoint &p, double *Q6) {
   // with P = s (x) m: P S = s (x) (S m) (S symmetric) and C_e P = (C_e s) (x) m
-  const double *s = gr.s[a], *m = gr.m[a];
+  const double *s = gr.s[a], *m = gr.m[a / 3];
   double S9[9], PS[9], CP[9], v[3], w[3];
   v6_to_9(p.S, S9);
 #pragma unroll
@@ -474,7 +474,7 @@ __device__ __forceinline__ void cp_jacobian_re(const CPMaterial &cm, const DevGr
     const double sgn = (gd < 0) ? -1.0 : 1.0;
 #pragma unroll
     for (int r = 0; r < 9; ++r) {
-      const double Pr = gr.s[a][r / 3] * gr.m[a][r % 3];
+      const double Pr = gr.s[a][r / 3] * gr.m[a / 3][r % 3];
       const double pr = Pr * co;
 #pragma unroll
       for (int c = 0; c < 9; ++c) Ah[r * 9 + c] += pr * D[c];
@@ -579,7 +579,7 @@ __device__ __forceinline__ bool cp_merit_at(const CPMaterial &cm, const DevGrain
     for (int i = 0; i < 3; ++i) {
       const double c = gd * gr.s[a][i];
 #pragma unroll
-      for (int j = 0; j < 3; ++j) sl[i * 3 + j] += c * gr.m[a][j];
+      for (int j = 0; j < 3; ++j) sl[i * 3 + j] += c * gr.m[a / 3][j];
     }
   }
   double gs;
@@ -1050,7 +1050,7 @@ __global__ __launch_bounds__(64) void cp_tangent(CPArgs a, int64_t nqb) {
       const double sg = ((cm.gd0 * rr * pw < 0) ? -1.0 : 1.0) * dgdt;
 #pragma unroll
       for (int r = 0; r < 9; ++r) {
-        const double cr = co * (gr.s[al][r / 3] * gr.m[al][r % 3]);
+        const double cr = co * (gr.s[al][r / 3] * gr.m[al / 3][r % 3]);
 #pragma unroll
         for (int I = 0; I < 6; ++I) R[r][I] += cr * Q[I];
       }

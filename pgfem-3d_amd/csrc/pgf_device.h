@@ -107,7 +107,8 @@ enum : int { MODE_INTEGRATE = 1, MODE_RESIDUAL = 2, MODE_TANGENT = 4, MODE_DEFL_
 struct DevGrain {
   double P[12][9];
   double Lv[36];
-  double s[12][3], m[12][3];  // slip directions and plane normals in the sample frame: P = s (x) m
+  double s[12][3];  // slip directions in the sample frame; P_a = s_a (x) m[a / 3]
+  double m[4][3];   // the four slip-plane normals (systems 3k..3k+2 share plane k)
   double R[9];                // crystal -> sample rotation
   double cub[3];              // cubic constants C11 - C12, C12, 2 C44 (crystal frame)
 };
