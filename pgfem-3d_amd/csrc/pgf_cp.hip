@@ -800,7 +800,7 @@ __global__ __launch_bounds__(64) void cp_newton(CPArgs a, int64_t chunk) {
   constexpr int kPhStart = 0, kPhTrial = 1, kPhFull = 2;
   int phase = kPhStart, ls = 0;
   double dx[11], f0 = 0.0, alpha = 1.0;
-  CPMaterial cm{};
+  const CPMaterial cm = a.cpmats[0];  // one material for every CP point: wave-uniform, loaded once
   for (;;) {
     // refill the lanes that need a QP, in lane order, from the wave's range
     const uint64_t mask = __ballot(need && !done);
@@ -831,7 +831,6 @@ __global__ __launch_bounds__(64) void cp_newton(CPArgs a, int64_t chunk) {
         det_pFn = det3(pFn);
         g_n = a.g_n[qg];
         gid = a.qgrain[q];
-        cm = a.cpmats[0];
         gscale = 1.0 / (fabs(cm.gs0 - cm.g0) * g_n + 1e-300);
         it = 0;
         it_att = 0;
