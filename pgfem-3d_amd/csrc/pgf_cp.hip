@@ -184,10 +184,9 @@ __device__ __forceinline__ void he_dot(const HeTan &t, const double *X9, double 
   }
 }
 
-// The grain's 6x6 stiffness is loop-invariant in the tangent kernel's slip-system loop; hiding
-// its address there keeps the compiler from hoisting its 36 loads (72 registers live across
-// the loop, which spill) and has it read them from cache where they are used.  (In the local
-// Newton the hoisted copy serves many evaluations and pays for itself.)
+// A pointer whose value the compiler cannot see through: loads from it are not hoisted out of
+// loops or merged with earlier loads from the same address, so the data is read from cache
+// where it is used instead of being held in registers across a long stretch of code.
 __device__ __forceinline__ const double *opaque(const double *p) {
   asm volatile("" : "+v"(p));
   return p;
@@ -371,7 +370,7 @@ __device__ __forceinline__ bool cp_point(const CPMaterial &cm, const DevGrain &g
 
 // Q_a = sym(P_a S + 1/2 L:(C_e P_a)) in Voigt order, so that d tau_a / d M = 2 A^T eF Q_a
 template <bool HE>
-__device__ __forceinline__ void qsym(const DevGrain &gr, const double *Lv, int a, const CPPoint &p, double *Q6) {
+__device__ __forceinline__ void qsym(const DevGrain &gr, int a, const CPPoint &p, double *Q6) {
   // with P = s (x) m: P S = s (x) (S m) (S symmetric) and C_e P = (C_e s) (x) m
   const double *s = gr.s[a], *m = gr.m[a / 3];
   double S9[9], PS[9], CP[9], v[3], w[3];
@@ -398,19 +397,11 @@ __device__ __forceinline__ void qsym(const DevGrain &gr, const double *Lv, int a
   }
 }
 
-// u^T Q w for Q symmetric in Voigt order
-__device__ __forceinline__ double quad_form(const double *Q, const double *u, const double *w) {
-  return u[0] * (Q[0] * w[0] + Q[3] * w[1] + Q[5] * w[2]) + u[1] * (Q[3] * w[0] + Q[1] * w[1] + Q[4] * w[2]) +
-         u[2] * (Q[5] * w[0] + Q[4] * w[1] + Q[2] * w[2]);
-}
-
-// d tau_a / d M = 2 A^T eF sym(P_a S + 1/2 L:(C_e P_a)); Lv: the grain's stiffness (its address
-// made opaque by callers that must not hoist its loads)
+// d tau_a / d M = 2 A^T eF sym(P_a S + 1/2 L:(C_e P_a))
 template <bool HE>
-__device__ __forceinline__ void dtau_dM(const DevGrain &gr, const double *Lv, int a, const double *AtE,
-                                        const CPPoint &p, double *D) {
+__device__ __forceinline__ void dtau_dM(const DevGrain &gr, int a, const double *AtE, const CPPoint &p, double *D) {
   double Q6[6], sQ[9];
-  qsym<HE>(gr, Lv, a, p, Q6);
+  qsym<HE>(gr, a, p, Q6);
   v6_to_9(Q6, sQ);
   double T[9];
   mm3(AtE, sQ, T);
@@ -433,10 +424,9 @@ struct Border {
 };
 
 // Bordered Jacobian of the local system at p; the per-system slip quantities are recomputed
-// from the Mandel stress with cp_point's arithmetic.  OPQ: read the grain stiffness per slip
-// system rather than hoisting it (the tangent kernel, where the hoisted copy spills).  QPRE:
-// the Q_a of qsym were formed earlier and sit in LDS at qpre[(a * 6 + I) * 64].
-template <bool HE, bool OPQ, bool QPRE = false>
+// from the Mandel stress with cp_point's arithmetic.  QPRE: the Q_a of qsym were formed
+// earlier and sit in LDS at qpre[(a * 6 + I) * 64].
+template <bool HE, bool QPRE = false>
 __device__ __forceinline__ void cp_jacobian_re(const CPMaterial &cm, const DevGrain &gr, double dt, const double *x,
                                             const double *A, const CPPoint &p, double g_n,
                                             double (&Ah)[81], Border &B, const double *qpre = nullptr) {
@@ -468,7 +458,7 @@ __device__ __forceinline__ void cp_jacobian_re(const CPMaterial &cm, const DevGr
       for (int I = 0; I < 6; ++I) Q6[I] = qpre[(a * 6 + I) * 64];
       dtau_dM_q(AtE, Q6, D);
     } else {
-      dtau_dM<HE>(gr, OPQ ? opaque(gr.Lv) : gr.Lv, a, AtE, p, D);
+      dtau_dM<HE>(gr, a, AtE, p, D);
     }
     const double co = dt * dgdt;
     const double sgn = (gd < 0) ? -1.0 : 1.0;
@@ -900,7 +890,7 @@ __global__ __launch_bounds__(64) void cp_newton(CPArgs a, int64_t chunk) {
     if (step) {
       double Ah[81];
       Border B;
-      cp_jacobian_re<HE, false>(cm, gr, dt, x, A, p, g_n, Ah, B);
+      cp_jacobian_re<HE>(cm, gr, dt, x, A, p, g_n, Ah, B);
       int piv[9];
       if (!lu9(Ah, piv)) {
         fail = true;
@@ -1025,42 +1015,31 @@ __global__ __launch_bounds__(64) void cp_tangent(CPArgs a, int64_t nqb) {
   out[15] = det3(pF1);
   if constexpr (WITH_D) {
     // dtau_a/dF = A^-T D_a (pF(n)^-1 M)^T with D_a = 2 A^T eF Q_a, so dtau_a/dF = 2 eF Q_a BM^T
-    // (BM = pF(n)^-1 M).  The right-hand sides are linear in Q_a: with R_r = sum_a co_a P_a[r] Q_a
-    // and Rg = -sum_a sg_a Q_a, column c = (i, j) of F has rhs[r] = 2 (eF R_r BM^T)_ij.
+    // (BM = pF(n)^-1 M).  Column c = (i, j) of F has the right-hand side
+    // rhs[r] = sum_a co_a P_a[r] q_a and the hardening row -sum_a sg_a q_a, with
+    // q_a = 2 (eF Q_a BM^T)_ij = 2 (Q_a u).w (u = row i of eF, w = row j of BM).  They are formed
+    // after the factorization from the Q_a in LDS, three columns (one row of eF) at a time, so
+    // no accumulator over the slip systems is live across the Jacobian and its LU.
     double BM[9];
     mm3(pFi, x, BM);
-    double R[9][6], Rg[6];
+    // d gamma_dot_a / d tau_a, with the sign of gamma_dot_a (co_a = -dt |dg_a|, sg_a = dg_a)
+    double dg[12];
 #pragma unroll
-    for (int I = 0; I < 6; ++I) {
-      Rg[I] = 0.0;
-#pragma unroll
-      for (int r = 0; r < 9; ++r) R[r][I] = 0.0;
-    }
     for (int al = 0; al < 12; ++al) {
       double Q[6];
-      qsym<HE>(gr, opaque(gr.Lv), al, p, Q);
+      qsym<HE>(gr, al, p, Q);
 #pragma unroll
       for (int I = 0; I < 6; ++I) sQ[(al * 6 + I) * 64 + threadIdx.x] = Q[I];
       const double tt = resolved(gr, al, p.Sig);
       const double rr = tt * rg;
       const double pw = flow_pow(fabs(rr), expo, cm.ipow);
       const double dgdt = cdg * pw;
-      const double co = -a.dt * dgdt;
-      const double sg = ((cm.gd0 * rr * pw < 0) ? -1.0 : 1.0) * dgdt;
-#pragma unroll
-      for (int r = 0; r < 9; ++r) {
-        const double cr = co * (gr.s[al][r / 3] * gr.m[al / 3][r % 3]);
-#pragma unroll
-        for (int I = 0; I < 6; ++I) R[r][I] += cr * Q[I];
-      }
-#pragma unroll
-      for (int I = 0; I < 6; ++I) Rg[I] -= sg * Q[I];
+      dg[al] = (cm.gd0 * rr * pw < 0) ? -dgdt : dgdt;
     }
-    // the local Jacobian at the converged point, factored once for the ten solves (after the
-    // slip loop, so the factors and the accumulators are not live together)
+    // the local Jacobian at the converged point, factored once for the ten solves
     double Ah[81];
     Border B;
-    cp_jacobian_re<HE, true, true>(cm, gr, a.dt, x, A, p, g_n, Ah, B, sQ + threadIdx.x);
+    cp_jacobian_re<HE, true>(cm, gr, a.dt, x, A, p, g_n, Ah, B, sQ + threadIdx.x);
     int piv[9];
     if (!lu9(Ah, piv)) {
       atomicAdd(a.err, 1);
@@ -1074,21 +1053,63 @@ __global__ __launch_bounds__(64) void cp_tangent(CPArgs a, int64_t nqb) {
 #pragma unroll
     for (int i = 0; i < 9; ++i) dTv += B.d[i] * v[i];
 #pragma unroll
-    for (int c = 0; c < 9; ++c) {
-      const double *u = p.eF + (c / 3) * 3, *w = BM + (c % 3) * 3;
-      double rhs[9];
+    for (int i = 0; i < 3; ++i) {
+      const double *u = p.eF + i * 3;
+      double rhs[3][9], rgc[3];
 #pragma unroll
-      for (int r = 0; r < 9; ++r) rhs[r] = 2.0 * quad_form(R[r], u, w);
-      const double rgc = 2.0 * quad_form(Rg, u, w);
-      double sol[11];
-      bordered_solve(Ah, piv, B, v, dTv, rhs, 0.0, B.dRg * rgc, sol);
-      // dMe/dF[:, c] = pF(n)^-1 dM/dF[:, c]
+      for (int j = 0; j < 3; ++j) {
+        rgc[j] = 0.0;
 #pragma unroll
-      for (int i = 0; i < 3; ++i)
+        for (int r = 0; r < 9; ++r) rhs[j][r] = 0.0;
+      }
+      // the slip directions and plane normals read where used: with the grain's address visible
+      // the compiler keeps the Jacobian's loads of them live through the LU, and spills
+      const DevGrain &go = *reinterpret_cast<const DevGrain *>(opaque(reinterpret_cast<const double *>(&gr)));
+      // P_a = s_a (x) m_a: the three systems of plane k summed (vs = sum co_a q_a s_a) before
+      // the outer product with m_k
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        double vs[3][3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) vs[j][0] = vs[j][1] = vs[j][2] = 0.0;
+#pragma unroll
+        for (int al = 3 * k; al < 3 * k + 3; ++al) {
+          double Q[6];
+#pragma unroll
+          for (int I = 0; I < 6; ++I) Q[I] = sQ[(al * 6 + I) * 64 + threadIdx.x];
+          const double Qu[3] = {Q[0] * u[0] + Q[3] * u[1] + Q[5] * u[2], Q[3] * u[0] + Q[1] * u[1] + Q[4] * u[2],
+                                Q[5] * u[0] + Q[4] * u[1] + Q[2] * u[2]};
+          const double cad = -a.dt * fabs(dg[al]);
+#pragma unroll
+          for (int j = 0; j < 3; ++j) {
+            const double *w = BM + j * 3;
+            const double q = 2.0 * (Qu[0] * w[0] + Qu[1] * w[1] + Qu[2] * w[2]);
+            const double ta = cad * q;
+            rgc[j] -= dg[al] * q;
+#pragma unroll
+            for (int l = 0; l < 3; ++l) vs[j][l] += ta * go.s[al][l];
+          }
+        }
 #pragma unroll
         for (int j = 0; j < 3; ++j)
-          out[16 + (i * 3 + j) * 9 + c] = pFi[i * 3 + 0] * sol[0 * 3 + j] + pFi[i * 3 + 1] * sol[1 * 3 + j] +
-                                          pFi[i * 3 + 2] * sol[2 * 3 + j];
+#pragma unroll
+          for (int l = 0; l < 3; ++l)
+#pragma unroll
+            for (int n = 0; n < 3; ++n) rhs[j][l * 3 + n] += vs[j][l] * go.m[k][n];
+      }
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const int c = i * 3 + j;
+        double sol[11];
+        bordered_solve(Ah, piv, B, v, dTv, rhs[j], 0.0, B.dRg * rgc[j], sol);
+        // dMe/dF[:, c] = pF(n)^-1 dM/dF[:, c]
+#pragma unroll
+        for (int ii = 0; ii < 3; ++ii)
+#pragma unroll
+          for (int jj = 0; jj < 3; ++jj)
+            out[16 + (ii * 3 + jj) * 9 + c] = pFi[ii * 3 + 0] * sol[0 * 3 + jj] + pFi[ii * 3 + 1] * sol[1 * 3 + jj] +
+                                               pFi[ii * 3 + 2] * sol[2 * 3 + jj];
+      }
     }
   }
 }

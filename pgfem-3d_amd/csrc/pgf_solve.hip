@@ -96,21 +96,27 @@ int bicgstab(hipStream_t s, const int32_t *Ap, const int32_t *Ai, const double *
   double *r = work, *rh = r + n, *p = rh + n, *v = p + n, *ph = v + n, *sv = ph + n, *sh = sv + n, *t = sh + n,
          *dinv = t + n, *part = dinv + n, *sc = part + 1024;
   const unsigned gs = unsigned(std::min<int64_t>(1024, (n + 255) / 256));
+  // the first failed runtime call; a failed dot product reads NaN, which ends the iteration
+  hipError_t err = hipSuccess;
+  auto chk = [&](hipError_t e) {
+    if (e != hipSuccess && err == hipSuccess) err = e;
+    return e == hipSuccess;
+  };
   auto dot = [&](const double *a, const double *c) {
     dot_partial<<<gs, 256, 0, s>>>(a, c, n, part);
     sum_partials<<<1, 256, 0, s>>>(part, int(gs), sc);
     double h = 0.0;
-    hipMemcpyAsync(&h, sc, sizeof(double), hipMemcpyDeviceToHost, s);
-    hipStreamSynchronize(s);
+    if (!chk(hipMemcpyAsync(&h, sc, sizeof(double), hipMemcpyDeviceToHost, s)) || !chk(hipStreamSynchronize(s)))
+      return std::nan("");
     return h;
   };
   const unsigned sg = unsigned((n * kLanesPerRow + 255) / 256);
   inv_diag<<<nb(n), 256, 0, s>>>(Ap, Ai, vals, dinv, n);
   spmv<<<sg, 256, 0, s>>>(Ap, Ai, vals, x, r, n);
   residual_from<<<nb(n), 256, 0, s>>>(b, r, n);
-  hipMemcpyAsync(rh, r, n * sizeof(double), hipMemcpyDeviceToDevice, s);
-  hipMemsetAsync(p, 0, n * sizeof(double), s);
-  hipMemsetAsync(v, 0, n * sizeof(double), s);
+  chk(hipMemcpyAsync(rh, r, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  chk(hipMemsetAsync(p, 0, n * sizeof(double), s));
+  chk(hipMemsetAsync(v, 0, n * sizeof(double), s));
   const double bn = std::sqrt(dot(b, b));
   const double bnorm = bn > 0.0 ? bn : 1.0;
   double rho = 1.0, alpha = 1.0, omega = 1.0;
@@ -127,7 +133,7 @@ int bicgstab(hipStream_t s, const int32_t *Ap, const int32_t *Ai, const double *
     const double rv = dot(rh, v);
     if (rv == 0.0 || !std::isfinite(rv)) break;
     alpha = rho1 / rv;
-    hipMemcpyAsync(sv, r, n * sizeof(double), hipMemcpyDeviceToDevice, s);
+    chk(hipMemcpyAsync(sv, r, n * sizeof(double), hipMemcpyDeviceToDevice, s));
     axpby<<<nb(n), 256, 0, s>>>(-alpha, v, 1.0, sv, n);  // s = r - alpha v
     jacobi<<<nb(n), 256, 0, s>>>(dinv, sv, sh, n);
     spmv<<<sg, 256, 0, s>>>(Ap, Ai, vals, sh, t, n);
@@ -135,7 +141,7 @@ int bicgstab(hipStream_t s, const int32_t *Ap, const int32_t *Ai, const double *
     omega = (tt > 0.0) ? dot(t, sv) / tt : 0.0;
     axpby<<<nb(n), 256, 0, s>>>(alpha, ph, 1.0, x, n);  // x += alpha ph + omega sh
     axpby<<<nb(n), 256, 0, s>>>(omega, sh, 1.0, x, n);
-    hipMemcpyAsync(r, sv, n * sizeof(double), hipMemcpyDeviceToDevice, s);
+    chk(hipMemcpyAsync(r, sv, n * sizeof(double), hipMemcpyDeviceToDevice, s));
     axpby<<<nb(n), 256, 0, s>>>(-omega, t, 1.0, r, n);  // r = s - omega t
     rho = rho1;
     res = std::sqrt(dot(r, r)) / bnorm;
@@ -143,7 +149,7 @@ int bicgstab(hipStream_t s, const int32_t *Ap, const int32_t *Ai, const double *
   }
   *iters = it;
   *relres = res;
-  return (hipGetLastError() == hipSuccess) ? 0 : -1;
+  return (err == hipSuccess && hipGetLastError() == hipSuccess) ? 0 : -1;
 }
 
 }  // namespace pgf
