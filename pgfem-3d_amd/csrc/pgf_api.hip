@@ -336,7 +336,7 @@ pgf::CPArgs cp_args(pgf_handle h, const Block &b, const pgf_nodal_input *in) {
   c.iters = h->d_iters;
   c.qnext = h->cp_chunk == 1 ? h->d_qnext : nullptr;
   c.warm = h->warm;
-  c.qpbuf = h->d_qpbuf + b.qp_base * pgf::kCPBuf;
+  c.qpbuf = h->d_qpbuf;  // indexed by device QP (pgf::qrec)
   c.err = h->d_err;
   c.law = h->cp_law;
   if (h->tf && b.ne > 0 && b.model == PGF_MODEL_CRYSTAL_PLASTICITY) {  // CM3F: the kinematics scales F
@@ -517,7 +517,7 @@ int32_t enqueue_pass(pgf_handle h, const pgf_nodal_input *in, int mode, double *
         ce.grains = h->d_grains;
         ce.cpmats = h->d_cpmat;
         ce.law = h->cp_law;
-        ce.qpbuf = h->d_qpbuf + b.qp_base * pgf::kCPBuf;
+        ce.qpbuf = h->d_qpbuf;  // indexed by device QP (pgf::qrec)
         ce.qp_base = b.qp_base;
         ce.r = in->r;
         ce.d_r = in->d_r;
@@ -1088,7 +1088,9 @@ int pgf_create(const pgf_mesh *mesh, const int64_t *id, int64_t ndof, const int3
     }
     if ((rc = pgf::dalloc(&h->d_iters, size_t(h->n_qp)))) return fail(rc);
     if ((rc = pgf::dalloc(&h->d_qnext, 1))) return fail(rc);
-    if ((rc = pgf::dalloc(&h->d_qpbuf, size_t(h->n_qp) * pgf::kCPBuf))) return fail(rc);
+    if ((rc = pgf::dalloc(&h->d_qpbuf, size_t((h->n_qp + pgf::kRecStride - 1) / pgf::kRecStride) *
+                                             pgf::kCPBuf * pgf::kRecStride)))
+      return fail(rc);
     if (hipMemset(h->d_iters, 0, sizeof(int32_t) * h->n_qp) != hipSuccess) return fail(PGF_EHIP);
     if ((rc = pgf::dalloc(&h->d_qgrain_all, size_t(h->n_qp)))) return fail(rc);
     if (hipMemset(h->d_qgrain_all, 0, sizeof(int32_t) * h->n_qp) != hipSuccess) return fail(PGF_EHIP);

@@ -723,9 +723,9 @@ __global__ __launch_bounds__(256) void cp_kinematics(CPArgs a) {
 // start (the plastic predictor pulled back to yield, about ten stress evaluations) and, with a
 // caller's start attached, which of the two has the lower residual merit.  In the Newton
 // kernel a lane takes a new QP whenever its last one converges, so work done at a refill runs
-// with the other lanes of the wave masked off; here it runs on every lane.  qpbuf row q holds
-// the plain start x(0..10) and the choice kStartFlag (bit 0: the plain start failed, bit 1:
-// take the caller's start) until the tangent kernel overwrites the row.
+// with the other lanes of the wave masked off; here it runs on every lane.  The QP's record in
+// qpbuf holds the plain start x(0..10) and the choice kStartFlag (bit 0: the plain start
+// failed, bit 1: take the caller's start) until the tangent kernel overwrites it.
 constexpr int kStartFlag = 11;
 template <bool HE>
 __global__ __launch_bounds__(64) void cp_start_kernel(CPArgs a) {
@@ -759,10 +759,10 @@ __global__ __launch_bounds__(64) void cp_start_kernel(CPArgs a) {
     const bool okp = !bad && cp_merit_at<HE>(cm, gr, A, det_pFn, g_n, a.dt, x, gscale, mp);
     if (okw && (!okp || mw < mp)) flag |= 2;
   }
-  double *row = a.qpbuf + q * kCPBuf;
+  double *row = a.qpbuf + qrec(qg);
 #pragma unroll
-  for (int i = 0; i < 11; ++i) row[i] = x[i];
-  row[kStartFlag] = double(flag);
+  for (int i = 0; i < 11; ++i) row[i * kRecStride] = x[i];
+  row[kStartFlag * kRecStride] = double(flag);
 }
 
 // ------------------------------------------------------------------- Newton kernel
@@ -826,8 +826,8 @@ __global__ __launch_bounds__(64) void cp_newton(CPArgs a, int64_t chunk) {
         it_att = 0;
         phase = kPhStart;
         // start: the caller's warm start where cp_start_kernel chose it, else the plain start
-        const double *row = a.qpbuf + q * kCPBuf;
-        const int flag = int(row[kStartFlag]);
+        const double *row = a.qpbuf + qrec(qg);
+        const int flag = int(row[kStartFlag * kRecStride]);
         bad_start = (flag & 1) != 0;
         kind = 0;
         if (flag & 2) {
@@ -837,7 +837,7 @@ __global__ __launch_bounds__(64) void cp_newton(CPArgs a, int64_t chunk) {
           bad_start = false;
         } else {
 #pragma unroll
-          for (int i = 0; i < 11; ++i) x[i] = row[i];
+          for (int i = 0; i < 11; ++i) x[i] = row[i * kRecStride];
         }
       }
     }
@@ -928,10 +928,10 @@ __global__ __launch_bounds__(64) void cp_newton(CPArgs a, int64_t chunk) {
       // (iterations keep counting)
       kind = 2;
       it_att = 0;
-      const double *row = a.qpbuf + q * kCPBuf;
+      const double *row = a.qpbuf + qrec(qg);
 #pragma unroll
-      for (int i = 0; i < 11; ++i) x[i] = row[i];
-      bad_start = (int(row[kStartFlag]) & 1) != 0;
+      for (int i = 0; i < 11; ++i) x[i] = row[i * kRecStride];
+      bad_start = (int(row[kStartFlag * kRecStride]) & 1) != 0;
       fail = false;
       phase = kPhStart;
     }
@@ -985,11 +985,11 @@ __global__ __launch_bounds__(64) void cp_tangent(CPArgs a, int64_t nqb) {
   const double g_n = a.g_n[qg];
   CPPoint p;
   ok = ok && cp_point<HE>(cm, gr, A, det3(pFn), g_n, a.dt, x, p);
-  double *out = a.qpbuf + t * kCPBuf;
+  double *out = a.qpbuf + qrec(qg);  // field i at out[i * kRecStride]
   if (!ok) {
     atomicAdd(a.err, 1);
 #pragma unroll
-    for (int i = 0; i < kCPBuf; ++i) out[i] = 0.0;
+    for (int i = 0; i < kCPBuf; ++i) out[i * kRecStride] = 0.0;
     return;
   }
   const double expo = 1.0 / cm.m - 1.0;
@@ -1009,10 +1009,10 @@ __global__ __launch_bounds__(64) void cp_tangent(CPArgs a, int64_t nqb) {
     el_stress<HE>(cm, gr, eF, Ce, S, ht);
   }
 #pragma unroll
-  for (int i = 0; i < 9; ++i) out[i] = Me[i];
+  for (int i = 0; i < 9; ++i) out[i * kRecStride] = Me[i];
 #pragma unroll
-  for (int i = 0; i < 6; ++i) out[9 + i] = S[i];
-  out[15] = det3(pF1);
+  for (int i = 0; i < 6; ++i) out[(9 + i) * kRecStride] = S[i];
+  out[15 * kRecStride] = det3(pF1);
   if constexpr (WITH_D) {
     // dtau_a/dF = A^-T D_a (pF(n)^-1 M)^T with D_a = 2 A^T eF Q_a, so dtau_a/dF = 2 eF Q_a BM^T
     // (BM = pF(n)^-1 M).  Column c = (i, j) of F has the right-hand side
@@ -1107,8 +1107,8 @@ __global__ __launch_bounds__(64) void cp_tangent(CPArgs a, int64_t nqb) {
         for (int ii = 0; ii < 3; ++ii)
 #pragma unroll
           for (int jj = 0; jj < 3; ++jj)
-            out[16 + (ii * 3 + jj) * 9 + c] = pFi[ii * 3 + 0] * sol[0 * 3 + jj] + pFi[ii * 3 + 1] * sol[1 * 3 + jj] +
-                                               pFi[ii * 3 + 2] * sol[2 * 3 + jj];
+            out[(16 + c * 9 + ii * 3 + jj) * kRecStride] =
+                pFi[ii * 3 + 0] * sol[0 * 3 + jj] + pFi[ii * 3 + 1] * sol[1 * 3 + jj] + pFi[ii * 3 + 2] * sol[2 * 3 + jj];
       }
     }
   }
@@ -1199,15 +1199,15 @@ __global__ __launch_bounds__(256) void cp_elem(CPElemArgs a, int64_t e_begin, in
         for (int ic = 0; ic < 3; ++ic) Fr[ib * 3 + ic] += s_u[slot][n * 3 + ib] * g[ic];
       }
     }
-    const double *qb = a.qpbuf + (e * NQP + ip) * kCPBuf;
+    const double *qb = a.qpbuf + qrec(a.qp_base + e * NQP + ip);  // field i at qb[i * kRecStride]
     double Me[9], S9[9], Mi[9];
 #pragma unroll
-    for (int i = 0; i < 9; ++i) Me[i] = qb[i];
+    for (int i = 0; i < 9; ++i) Me[i] = qb[i * kRecStride];
     double S6[6];
 #pragma unroll
-    for (int i = 0; i < 6; ++i) S6[i] = qb[9 + i];
+    for (int i = 0; i < 6; ++i) S6[i] = qb[(9 + i) * kRecStride];
     v6_to_9(S6, S9);
-    const double pJ = qb[15];
+    const double pJ = qb[15 * kRecStride];
     inv3(Me, Mi);
     double MeS[9], Sh9[9], Cr[9], CrMe[9];
     mm3(Me, S9, MeS);
@@ -1249,14 +1249,15 @@ __global__ __launch_bounds__(256) void cp_elem(CPElemArgs a, int64_t e_begin, in
     }
     if constexpr (TAN && NQP > 1) {
 #pragma unroll
-      for (int i = 0; i < 81; ++i) s_D[slot][ip][i] = qb[16 + i];
+      for (int i = 0; i < 81; ++i) s_D[slot][ip][i] = qb[(16 + (i % 9) * 9 + i / 9) * kRecStride];
     }
   }
   if constexpr (TAN && NQP == 1) {
     // one QP per element (linear tets): its dMe/dF is spread over the element's lanes
     // (consecutive lanes on consecutive doubles) rather than 81 loads on the QP's one lane
     if (active)
-      for (int i = lane; i < 81; i += TPE) s_D[slot][0][i] = a.qpbuf[e * kCPBuf + 16 + i];
+      for (int i = lane; i < 81; i += TPE)
+        s_D[slot][0][i] = a.qpbuf[qrec(a.qp_base + e) + (16 + (i % 9) * 9 + i / 9) * kRecStride];
   }
   __syncthreads();
   if constexpr (TAN) {
@@ -1602,8 +1603,8 @@ __global__ __launch_bounds__(256) void cp_tf_elem(CPElemArgs a, int64_t e_begin,
 #pragma unroll 1
   for (int ip = 0; ip < NQP; ++ip) {
     if (active) {
-      const double *qb = a.qpbuf + (e * NQP + ip) * kCPBuf;
-      for (int i = lane; i < 81; i += TPE) s_D[slot][i] = qb[16 + i];
+      const double *qb = a.qpbuf + qrec(a.qp_base + e * NQP + ip);
+      for (int i = lane; i < 81; i += TPE) s_D[slot][i] = qb[(16 + (i % 9) * 9 + i / 9) * kRecStride];
     }
     lds_barrier();
     // the QP's tensors in three rounds, lanes of an element working side by side
@@ -1617,10 +1618,10 @@ __global__ __launch_bounds__(256) void cp_tf_elem(CPElemArgs a, int64_t e_begin,
         for (int ib = 0; ib < 3; ++ib)
 #pragma unroll
           for (int ic = 0; ic < 3; ++ic) Fr[ib * 3 + ic] += s_u[slot][n * 3 + ib] * dN[n * 3 + ic];
-      const double *qb = a.qpbuf + (e * NQP + ip) * kCPBuf;
+      const double *qb = a.qpbuf + qrec(a.qp_base + e * NQP + ip);
       double Me[9], Mi[9], Fi[9];
 #pragma unroll
-      for (int i = 0; i < 9; ++i) Me[i] = qb[i];
+      for (int i = 0; i < 9; ++i) Me[i] = qb[i * kRecStride];
       const double J = det3(Fr);
       const bool okq = J > 0.0 && theta > 0.0 && inv3(Fr, Fi) && inv3(Me, Mi);
       if (!okq) atomicAdd(a.err, 1);
@@ -1631,7 +1632,7 @@ __global__ __launch_bounds__(256) void cp_tf_elem(CPElemArgs a, int64_t e_begin,
         q[18 + i] = Mi[i];
         q[113 + i] = Fi[i];
       }
-      q[87] = okq ? s_w[slot][ip] * qb[15] : 0.0;  // detJ w / Jn, 1/Jn = pJ
+      q[87] = okq ? s_w[slot][ip] * qb[15 * kRecStride] : 0.0;  // detJ w / Jn, 1/Jn = pJ
       q[88] = J;
       q[89] = det3(Me);
       q[92] = okq ? cbrt(theta / J) : 1.0;

@@ -145,7 +145,9 @@ struct CPArgs {
   const double *tau_n;        // 12 comps at n (learned-start features)
   int32_t *iters;             // [n_qp]
   const double *warm;         // optional start (M 9, lambda, g) [11][n_qp]
-  // tangent-pass output for the element kernel: Me 9, S 6, pJ 1, D = dMe/dF 81 (block-local)
+  // per-QP records (qrec): the plain start, then the tangent pass's output for the element
+  // kernel, Me 9, S 6, pJ 1, D = dMe/dF 81 by columns (field 16 + 9 c + r holds D[r][c]);
+  // indexed by device QP
   double *qpbuf;
   int32_t *err;
   unsigned long long *qnext;  // global work queue of the local Newton (cp_chunk = 1), else null
@@ -155,6 +157,17 @@ struct CPArgs {
   const double *tf_V, *tf_dV;
 };
 constexpr int kCPBuf = 97;  // doubles per QP in qpbuf
+// qpbuf holds the records of 8 QPs at a time with the field outermost: field f of device QP q
+// is at qpbuf[qrec(q) + f * kRecStride].  A wave's store of one field over its 64 QPs then
+// writes eight 64-byte runs, and the next field's store the other halves of the same 128-byte
+// lines, where a 776-byte row per QP has each store touch 64 rows, and lines leave the L2
+// partly written.  (Groups of 64 QPs make each store one 512-byte run, but the field offsets
+// then exceed the stores' immediate offsets and the address registers spill.)  The buffer
+// holds ceil(n_qp / 8) * 8 records.
+constexpr int kRecStride = 8;
+__host__ __device__ constexpr int64_t qrec(int64_t q) {  // q >= 0
+  return int64_t((uint64_t(q) / kRecStride) * (uint64_t(kCPBuf) * kRecStride) + uint64_t(q) % kRecStride);
+}
 
 // Output variables of one block at one time level (constitutive_model_update_output_variables,
 // src/constitutive_model.cc:1677-1881): per QP the PK2 stress S and the Green-Lagrange strain E,
