@@ -1135,6 +1135,9 @@ __global__ __launch_bounds__(256) void cp_elem(CPElemArgs a, int64_t e_begin, in
   static_assert(!DFL || (TAN && !RES), "MODE_DEFL_LOAD goes with MODE_TANGENT alone");
   constexpr bool WK = TAN && !DFL;  // K rows are written
   constexpr bool WR = RES || DFL;   // the residual entry is written
+  // dMe/dF to LDS by all of an element's lanes (tets), or by each QP's lane (hex8, where the
+  // 648 values of eight QPs measured slower spread over 32 lanes)
+  constexpr bool SPREAD_D = NNE != 8;
   __shared__ double s_x[EPB][NDE], s_u[EPB][NDE];
   __shared__ double s_dl[EPB][DFL ? NDE : 1];
   __shared__ int32_t s_dof[EPB][NDE], s_node[EPB][NNE];
@@ -1247,17 +1250,20 @@ __global__ __launch_bounds__(256) void cp_elem(CPElemArgs a, int64_t e_begin, in
       s_S[slot][ip][I] = S6[I];
       s_Sh[slot][ip][I] = Sh9[vi(I) * 3 + vj(I)];
     }
-    if constexpr (TAN && NQP > 1) {
+    if constexpr (TAN && !SPREAD_D) {
 #pragma unroll
-      for (int i = 0; i < 81; ++i) s_D[slot][ip][i] = qb[(16 + (i % 9) * 9 + i / 9) * kRecStride];
+      for (int f = 0; f < 81; ++f) s_D[slot][ip][(f % 9) * 9 + f / 9] = qb[(16 + f) * kRecStride];
     }
   }
-  if constexpr (TAN && NQP == 1) {
-    // one QP per element (linear tets): its dMe/dF is spread over the element's lanes
-    // (consecutive lanes on consecutive doubles) rather than 81 loads on the QP's one lane
+  if constexpr (TAN && SPREAD_D) {
+    // the element's dMe/dF (NQP x 81) copied by all of its lanes, consecutive lanes on
+    // consecutive record fields (field 16 + 9 c + r holds D[r][c]), rather than 81 loads on each
+    // QP's lane
     if (active)
-      for (int i = lane; i < 81; i += TPE)
-        s_D[slot][0][i] = a.qpbuf[qrec(a.qp_base + e) + (16 + (i % 9) * 9 + i / 9) * kRecStride];
+      for (int j = lane; j < NQP * 81; j += TPE) {
+        const int ip = j / 81, f = j % 81;
+        s_D[slot][ip][(f % 9) * 9 + f / 9] = a.qpbuf[qrec(a.qp_base + e * NQP + ip) + (16 + f) * kRecStride];
+      }
   }
   __syncthreads();
   if constexpr (TAN) {
@@ -1604,7 +1610,7 @@ __global__ __launch_bounds__(256) void cp_tf_elem(CPElemArgs a, int64_t e_begin,
   for (int ip = 0; ip < NQP; ++ip) {
     if (active) {
       const double *qb = a.qpbuf + qrec(a.qp_base + e * NQP + ip);
-      for (int i = lane; i < 81; i += TPE) s_D[slot][i] = qb[(16 + (i % 9) * 9 + i / 9) * kRecStride];
+      for (int f = lane; f < 81; f += TPE) s_D[slot][(f % 9) * 9 + f / 9] = qb[(16 + f) * kRecStride];
     }
     lds_barrier();
     // the QP's tensors in three rounds, lanes of an element working side by side
