@@ -781,7 +781,14 @@ __global__ __launch_bounds__(64) void cp_newton(CPArgs a, int64_t chunk) {
   const int64_t NQ = a.n_qp;
   bool need = true, done = false;
   int64_t q = -1;
-  double x[11], A[9], det_pFn = 1.0, g_n = 1.0, gscale = 1.0, dt = a.dt;
+  // The lane's QP data that lives across turns sits in LDS (lane-fastest; each lane touches
+  // only its own slots), read where used, rather than in registers held through the material
+  // point and the Jacobian's LU, where it spilled: A (9), x (11), dx (11), det pF(n), g(n),
+  // gscale.  17 KB per wave.
+  __shared__ double sPk[34 * 64];
+  double *pk = sPk + lane;  // field i at pk[i * 64]
+  constexpr int kA = 0, kX = 9, kDX = 20, kDet = 31, kGn = 32, kGs = 33;
+  const double dt = a.dt;
   int it = 0, it_att = 0, gid = 0;
   bool bad_start = false;
   int kind = 0;  // the QP's start: 0 plain, 1 the caller's (learned), 2 the caller's, then the plain fallback
@@ -789,7 +796,7 @@ __global__ __launch_bounds__(64) void cp_newton(CPArgs a, int64_t chunk) {
   // after 40 halvings without a decrease
   constexpr int kPhStart = 0, kPhTrial = 1, kPhFull = 2;
   int phase = kPhStart, ls = 0;
-  double dx[11], f0 = 0.0, alpha = 1.0;
+  double f0 = 0.0, alpha = 1.0;
   const CPMaterial cm = a.cpmats[0];  // one material for every CP point: wave-uniform, loaded once
   for (;;) {
     // refill the lanes that need a QP, in lane order, from the wave's range
@@ -810,7 +817,7 @@ __global__ __launch_bounds__(64) void cp_newton(CPArgs a, int64_t chunk) {
         q = cand;
         need = false;
         const int64_t qg = a.qp_base + q;
-        double F[9], pFn[9], pFi[9];
+        double F[9], pFn[9], pFi[9], A[9];
 #pragma unroll
         for (int i = 0; i < 9; ++i) {
           F[i] = a.F_np1[i * NQ + qg];
@@ -818,10 +825,13 @@ __global__ __launch_bounds__(64) void cp_newton(CPArgs a, int64_t chunk) {
         }
         inv3(pFn, pFi);
         mm3(F, pFi, A);
-        det_pFn = det3(pFn);
-        g_n = a.g_n[qg];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) pk[(kA + i) * 64] = A[i];
+        pk[kDet * 64] = det3(pFn);
+        const double g_n = a.g_n[qg];
+        pk[kGn * 64] = g_n;
         gid = a.qgrain[q];
-        gscale = 1.0 / (fabs(cm.gs0 - cm.g0) * g_n + 1e-300);
+        pk[kGs * 64] = 1.0 / (fabs(cm.gs0 - cm.g0) * g_n + 1e-300);
         it = 0;
         it_att = 0;
         phase = kPhStart;
@@ -832,12 +842,12 @@ __global__ __launch_bounds__(64) void cp_newton(CPArgs a, int64_t chunk) {
         kind = 0;
         if (flag & 2) {
 #pragma unroll
-          for (int i = 0; i < 11; ++i) x[i] = a.warm[i * NQ + qg];
+          for (int i = 0; i < 11; ++i) pk[(kX + i) * 64] = a.warm[i * NQ + qg];
           kind = 1;
           bad_start = false;
         } else {
 #pragma unroll
-          for (int i = 0; i < 11; ++i) x[i] = row[i * kRecStride];
+          for (int i = 0; i < 11; ++i) pk[(kX + i) * 64] = row[i * kRecStride];
         }
       }
     }
@@ -852,12 +862,16 @@ __global__ __launch_bounds__(64) void cp_newton(CPArgs a, int64_t chunk) {
     const DevGrain &gr = a.grains[gid];
     const int64_t qg = a.qp_base + q;
     bool fail = false, conv = false, step = false;
-    double xc[11];
+    double xc[11], A[9];
 #pragma unroll
     for (int i = 0; i < 11; ++i)
-      xc[i] = (phase == kPhStart) ? x[i] : x[i] + (phase == kPhTrial ? alpha : 1.0) * dx[i];
+      xc[i] = (phase == kPhStart) ? pk[(kX + i) * 64]
+                                  : pk[(kX + i) * 64] + (phase == kPhTrial ? alpha : 1.0) * pk[(kDX + i) * 64];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) A[i] = pk[(kA + i) * 64];
     CPPoint p;
-    const bool valid = !(phase == kPhStart && bad_start) && cp_point<HE>(cm, gr, A, det_pFn, g_n, dt, xc, p);
+    const bool valid =
+        !(phase == kPhStart && bad_start) && cp_point<HE>(cm, gr, A, pk[kDet * 64], pk[kGn * 64], dt, xc, p);
     if (phase == kPhStart) {
       ++it;
       ++it_att;
@@ -866,7 +880,7 @@ __global__ __launch_bounds__(64) void cp_newton(CPArgs a, int64_t chunk) {
     } else {
       bool accept = false;
       if (phase == kPhTrial) {
-        accept = valid && cp_merit(p, gscale) <= (1.0 - 1e-4 * alpha) * f0;
+        accept = valid && cp_merit(p, pk[kGs * 64]) <= (1.0 - 1e-4 * alpha) * f0;
         if (!accept) {
           alpha *= 0.5;
           if (++ls == 40) phase = kPhFull;  // no decrease found: the full step, if it is valid
@@ -877,7 +891,7 @@ __global__ __launch_bounds__(64) void cp_newton(CPArgs a, int64_t chunk) {
       }
       if (accept) {
 #pragma unroll
-        for (int i = 0; i < 11; ++i) x[i] = xc[i];
+        for (int i = 0; i < 11; ++i) pk[(kX + i) * 64] = xc[i];
         if (it_att >= cm.maxit) {  // the end of the iteration that took this step
           fail = true;
         } else {
@@ -890,7 +904,11 @@ __global__ __launch_bounds__(64) void cp_newton(CPArgs a, int64_t chunk) {
     if (step) {
       double Ah[81];
       Border B;
-      cp_jacobian_re<HE>(cm, gr, dt, x, A, p, g_n, Ah, B);
+      // the point of this step (the start, or the accepted trial); the Jacobian reads lambda and g
+      double xj[11] = {};
+      xj[9] = pk[(kX + 9) * 64];
+      xj[10] = pk[(kX + 10) * 64];
+      cp_jacobian_re<HE>(cm, gr, dt, xj, A, p, pk[kGn * 64], Ah, B);
       int piv[9];
       if (!lu9(Ah, piv)) {
         fail = true;
@@ -902,7 +920,7 @@ __global__ __launch_bounds__(64) void cp_newton(CPArgs a, int64_t chunk) {
         double dTv = 0.0;
 #pragma unroll
         for (int i = 0; i < 9; ++i) dTv += B.d[i] * v[i];
-        double r9[9];
+        double r9[9], dx[11];
 #pragma unroll
         for (int i = 0; i < 9; ++i) r9[i] = -p.R[i];
         bordered_solve(Ah, piv, B, v, dTv, r9, -p.R[9], -p.R[10], dx);
@@ -910,13 +928,15 @@ __global__ __launch_bounds__(64) void cp_newton(CPArgs a, int64_t chunk) {
 #pragma unroll
         for (int i = 0; i < 9; ++i) du = fmax(du, fabs(dx[i]));
         du = fmax(du, fabs(dx[9]) * dt);
-        du = fmax(du, fabs(dx[10]) / g_n);
+        du = fmax(du, fabs(dx[10]) / pk[kGn * 64]);
         if (du < cm.tol) {
 #pragma unroll
-          for (int i = 0; i < 11; ++i) x[i] += dx[i];
+          for (int i = 0; i < 11; ++i) pk[(kX + i) * 64] = pk[(kX + i) * 64] + dx[i];
           conv = true;
         } else {  // backtracking on half the squared residual norm, from the full step
-          f0 = cp_merit(p, gscale);
+#pragma unroll
+          for (int i = 0; i < 11; ++i) pk[(kDX + i) * 64] = dx[i];
+          f0 = cp_merit(p, pk[kGs * 64]);
           alpha = 1.0;
           ls = 0;
           phase = kPhTrial;
@@ -930,7 +950,7 @@ __global__ __launch_bounds__(64) void cp_newton(CPArgs a, int64_t chunk) {
       it_att = 0;
       const double *row = a.qpbuf + qrec(qg);
 #pragma unroll
-      for (int i = 0; i < 11; ++i) x[i] = row[i * kRecStride];
+      for (int i = 0; i < 11; ++i) pk[(kX + i) * 64] = row[i * kRecStride];
       bad_start = (int(row[kStartFlag * kRecStride]) & 1) != 0;
       fail = false;
       phase = kPhStart;
@@ -941,7 +961,9 @@ __global__ __launch_bounds__(64) void cp_newton(CPArgs a, int64_t chunk) {
         atomicAdd(a.err, 1);
       } else {
         // pF(n+1) = M^-1 pF(n)
-        double Mi[9], pFn[9], pF1[9];
+        double x[11], Mi[9], pFn[9], pF1[9];
+#pragma unroll
+        for (int i = 0; i < 11; ++i) x[i] = pk[(kX + i) * 64];
         inv3(x, Mi);
 #pragma unroll
         for (int i = 0; i < 9; ++i) pFn[i] = a.pF_n[i * NQ + qg];
